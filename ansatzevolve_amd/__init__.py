@@ -1,0 +1,228 @@
+"""MI355X statevector engine for AnsatzEvolve-style TUPS ansatze.
+
+The product is ``libansatzsynth.so`` (C ABI in include/ansatzsynth.h, a
+drop-in for the reference's AnsatzSynthInterface).  This module is the
+Python host mirror of that interface: ``AnsatzSynth`` wraps one ctx and
+calls the C entry points by their reference names.  There is no CPU
+fallback: if the library is missing, importing the bindings fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libansatzsynth.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes) of every C entry point in include/ansatzsynth.h
+C_API = [
+    ("cleanup", ctypes.c_int, [_vp]),
+    ("getEnergy", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("getFinalState", ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _vp]),
+    ("getFinalStateComplex", ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _vp, _vp]),
+    ("getGradient_COMP", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("getHessian_COMP", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("init", _vp, []),
+    ("setExcitation", ctypes.c_int, [ctypes.c_int, _ip, _ip, _vp]),
+    ("setExcitationScale", ctypes.c_int, [ctypes.c_int, _ip, _ip, _dp, _vp]),
+    ("setHamiltonian", ctypes.c_int, [ctypes.c_int, _ip, _ip, _dp, _vp]),
+    ("setInitialState", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _dp, _vp]),
+    ("setInitialStateComplex", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _vp, _vp]),
+    ("setTraceInterfaceCalls", None, [ctypes.c_int]),
+    # extensions
+    ("setHamiltonianIntegrals", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("setNuclearEnergy", ctypes.c_int, [ctypes.c_double, _vp]),
+    ("setHamiltonianStorage", ctypes.c_int, [ctypes.c_int, _vp]),
+    ("getEnergyGradient", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _vp]),
+    ("getEnergies", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _vp]),
+    ("getEnergyGradients", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _vp]),
+    ("getDimensions", ctypes.c_int, [ctypes.POINTER(ctypes.c_longlong), _ip, _ip, _vp]),
+]
+
+REFERENCE_SYMBOLS = [n for n, _, _ in C_API[:13]]
+
+_LIB = None
+
+
+def load_library(path=LIBPATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError("libansatzsynth.so is not built (python -m ansatzevolve_amd.build); "
+                          "there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, res, args in C_API:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+class AnsatzError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__("%s returned %d" % (fn, code))
+        self.fn = fn
+        self.code = code
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+class AnsatzSynth:
+    """One ctx of the C interface.  Index conventions are the reference's:
+    one-based basis indices (bit string + 1) and one-based operator
+    components (0 = absent).  Methods raise AnsatzError on a non-zero code;
+    ``call`` returns raw codes."""
+
+    def __init__(self):
+        self.lib = load_library()
+        self.ctx = ctypes.c_void_p(self.lib.init())
+        self.nparams = None
+
+    def close(self):
+        if self.ctx is not None and self.ctx.value:
+            self.lib.cleanup(ctypes.byref(self.ctx))
+        self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, fn, code):
+        if code != 0:
+            raise AnsatzError(fn, code)
+
+    # ---- set-up (reference C interface)
+    def set_initial_state(self, num_qubits, indexes_1based, coeffs):
+        i, ip = _i(indexes_1based)
+        c, cp = _d(coeffs)
+        self._chk("setInitialState", self.lib.setInitialState(num_qubits, len(i), ip, cp, self.ctx))
+
+    def set_hamiltonian(self, i_1based, j_1based, coeffs):
+        a, ap = _i(i_1based)
+        b, bp = _i(j_1based)
+        c, cp = _d(coeffs)
+        self._chk("setHamiltonian", self.lib.setHamiltonian(len(a), ap, bp, cp, self.ctx))
+
+    def set_excitation(self, operators_1based, order_1based, scale=None):
+        """operators_1based: (M, 4) one-based components, 0 = absent."""
+        ops = np.asarray(operators_1based, dtype=np.int32).reshape(-1, 4)
+        M = len(ops)
+        flat, fp = _i(ops.T.reshape(-1))  # operators[M*j + i] = component j of operator i
+        o, op = _i(order_1based)
+        if scale is None:
+            code = self.lib.setExcitation(M, fp, op, self.ctx)
+        else:
+            s, sp = _d(scale)
+            code = self.lib.setExcitationScale(M, fp, op, sp, self.ctx)
+        self._chk("setExcitation", code)
+        self.nparams = int(np.max(order_1based))
+
+    # ---- extensions
+    def set_integrals(self, h, g):
+        h = np.asarray(h, dtype=np.float64)
+        n = h.shape[0]
+        a, ap = _d(h.reshape(-1))
+        b, bp = _d(np.asarray(g, dtype=np.float64).reshape(-1))
+        self._chk("setHamiltonianIntegrals", self.lib.setHamiltonianIntegrals(n, ap, bp, self.ctx))
+
+    def set_nuclear_energy(self, e):
+        self._chk("setNuclearEnergy", self.lib.setNuclearEnergy(float(e), self.ctx))
+
+    def set_storage(self, mode):
+        self._chk("setHamiltonianStorage", self.lib.setHamiltonianStorage(int(mode), self.ctx))
+
+    def dimensions(self):
+        d = ctypes.c_longlong()
+        no = ctypes.c_int()
+        npar = ctypes.c_int()
+        self._chk("getDimensions", self.lib.getDimensions(ctypes.byref(d), ctypes.byref(no), ctypes.byref(npar),
+                                                          self.ctx))
+        return d.value, no.value, npar.value
+
+    # ---- evaluations
+    def energy(self, angles):
+        a, ap = _d(angles)
+        E = ctypes.c_double()
+        self._chk("getEnergy", self.lib.getEnergy(len(a), ap, ctypes.byref(E), self.ctx))
+        return E.value
+
+    def gradient(self, angles):
+        a, ap = _d(angles)
+        g = np.zeros(len(a))
+        self._chk("getGradient_COMP", self.lib.getGradient_COMP(len(a), ap, g.ctypes.data_as(_dp), self.ctx))
+        return g
+
+    def hessian(self, angles):
+        a, ap = _d(angles)
+        P = len(a)
+        H = np.zeros(P * P)
+        self._chk("getHessian_COMP", self.lib.getHessian_COMP(P, ap, H.ctypes.data_as(_dp), self.ctx))
+        return H.reshape(P, P).T.copy()  # column-major -> H[i, j]
+
+    def final_state(self, angles, num_qubits):
+        a, ap = _d(angles)
+        n = 1 << num_qubits
+        out = np.zeros(n)
+        self._chk("getFinalState", self.lib.getFinalState(len(a), ap, n, out.ctypes.data_as(_dp), self.ctx))
+        return out
+
+    def energy_gradient(self, angles):
+        a, ap = _d(angles)
+        g = np.zeros(len(a))
+        E = ctypes.c_double()
+        self._chk("getEnergyGradient", self.lib.getEnergyGradient(len(a), ap, ctypes.byref(E),
+                                                                  g.ctypes.data_as(_dp), self.ctx))
+        return E.value, g
+
+    def energies(self, angles2d):
+        A = np.ascontiguousarray(angles2d, dtype=np.float64)
+        nvec, na = A.shape
+        E = np.zeros(nvec)
+        self._chk("getEnergies", self.lib.getEnergies(nvec, na, A.ctypes.data_as(_dp), E.ctypes.data_as(_dp),
+                                                      self.ctx))
+        return E
+
+    def energy_gradients(self, angles2d):
+        A = np.ascontiguousarray(angles2d, dtype=np.float64)
+        nvec, na = A.shape
+        E = np.zeros(nvec)
+        G = np.zeros((nvec, na))
+        self._chk("getEnergyGradients", self.lib.getEnergyGradients(nvec, na, A.ctypes.data_as(_dp),
+                                                                    E.ctypes.data_as(_dp),
+                                                                    G.ctypes.data_as(_dp), self.ctx))
+        return E, G
+
+
+def from_workload(w, storage=0):
+    """An AnsatzSynth set up from ansatzevolve_amd.workloads.make_workload()."""
+    a = AnsatzSynth()
+    a.set_initial_state(w["nq"], [w["start_idx"] + 1], [1.0])
+    a.set_integrals(w["h"], w["g"])
+    if storage:
+        a.set_storage(storage)
+    ops = np.asarray(w["ops"]) + 1  # -1 (absent) -> 0
+    a.set_excitation(ops, np.asarray(w["order"]) + 1, w.get("scale"))
+    return a

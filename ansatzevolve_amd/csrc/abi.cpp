@@ -1,0 +1,428 @@
+// The C ABI (include/ansatzsynth.h).  Argument checks, their order and the
+// return codes follow the reference's src/AnsatzSynthInterface.cpp; every
+// call on a ctx takes that ctx's lock.  No exception crosses the boundary.
+#include "../../include/ansatzsynth.h"
+
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "manager.hpp"
+
+using ae::log_msg;
+using ae::Manager;
+
+namespace {
+bool traceInterfaceCalls = false;
+
+template <class T>
+std::string join(const T *p, int n)
+{
+    std::string s = "[";
+    for (int i = 0; i < n; i++) {
+        if (i) s += ", ";
+        s += std::to_string(p[i]);
+    }
+    return s + "]";
+}
+} // namespace
+
+extern "C" {
+
+void setTraceInterfaceCalls(int val) { traceInterfaceCalls = val != 0; }
+
+void *init(void)
+{
+    Manager *ptr = new (std::nothrow) Manager;
+    if (traceInterfaceCalls) log_msg("Constructed object %p", (void *)ptr);
+    return ptr;
+}
+
+int cleanup(void *ctx)
+{
+    if (traceInterfaceCalls) log_msg("cleanup called with void** %p", ctx);
+    if (ctx == nullptr) {
+        log_msg("(void**) ctx is nullptr, something has gone wrong. Potential Leak");
+        return 1;
+    }
+    void **myCtx = (void **)ctx;
+    if (*myCtx == nullptr) return 0;
+    delete static_cast<Manager *>(*myCtx);
+    *myCtx = nullptr;
+    return 0;
+}
+
+int setExcitation(int nparams, const int *operators, const int *orderfile, void *ctx)
+{
+    return setExcitationScale(nparams, operators, orderfile, nullptr, ctx);
+}
+
+int setExcitationScale(int nparams, const int *operators, const int *orderfile, const double *scale, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (nparams < 1) {
+        log_msg("nparams < 1");
+        return 2;
+    }
+    if (traceInterfaceCalls) {
+        log_msg("setExcitationScale called with nParams %d operators %s orderfile %s", nparams,
+                join(operators, 4 * nparams).c_str(), join(orderfile, nparams).c_str());
+        if (scale) log_msg("scale %s", join(scale, nparams).c_str());
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<std::array<int8_t, 4>> excs(nparams);
+    for (int i = 0; i < nparams; i++)
+        for (int j = 0; j < 4; j++) {
+            if (operators[nparams * j + i] - 1 > std::numeric_limits<int8_t>::max()) {
+                log_msg("Excitation out of bounds, max is 128: %d", operators[nparams * j + i]);
+                return 3;
+            }
+            excs[i][j] = (int8_t)(operators[nparams * j + i] - 1);  // Fortran order
+        }
+    if (!m->storeOperators(excs)) return 4;
+    std::vector<std::pair<int, double>> dep;
+    for (int i = 0; i < nparams; i++) {
+        if (orderfile[i] > nparams) {
+            log_msg("orderfile out of bounds: %d", orderfile[i]);
+            return 5;
+        }
+        dep.push_back({orderfile[i] - 1, scale ? scale[i] : 1.0});
+    }
+    if (!m->storeParameterDependencies(dep)) return 6;
+    return 0;
+}
+
+int setHamiltonian(int n, const int *iindexes, const int *jindexes, const double *coeffs, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (traceInterfaceCalls)
+        log_msg("setHamiltonian called with N %d iIndexes %s jIndexes %s", n, join(iindexes, n).c_str(),
+                join(jindexes, n).c_str());
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (n < 1) {
+        log_msg("N < 1");
+        return 2;
+    }
+    std::vector<int> iv(iindexes, iindexes + n), jv(jindexes, jindexes + n);
+    for (int &x : iv) x--;
+    for (int &x : jv) x--;
+    std::vector<double> cv(coeffs, coeffs + n);
+    if (!m->storeHamiltonian(std::move(iv), std::move(jv), std::move(cv))) return 3;
+    return 0;
+}
+
+int setInitialState(int numqubits, int n, const int *iindexes, const double *coeffs, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (traceInterfaceCalls)
+        log_msg("setInitialState called with numQubits %d N %d iIndexes %s coeffs %s", numqubits, n,
+                join(iindexes, n).c_str(), join(coeffs, n).c_str());
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (n < 1) {
+        log_msg("N < 1");
+        return 2;
+    }
+    if (numqubits < 1) {
+        log_msg("numQubits < 1");
+        return 2;
+    }
+    std::vector<int> iv(iindexes, iindexes + n);
+    for (int &x : iv) x--;
+    std::vector<double> cv(coeffs, coeffs + n);
+    if (!m->storeInitial(numqubits, iv, cv)) return 3;
+    return 0;
+}
+
+int setInitialStateComplex(int, int, const int *, const ANSATZ_DOUBLE_COMPLEX *, void *)
+{
+    log_msg("Cannot set a complex initial state without building in complex mode");
+    return 4;
+}
+
+int getEnergy(int nangles, const double *angles, double *energy, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 3;
+    }
+    if (traceInterfaceCalls) log_msg("getEnergy called with NAngles %d", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    double E = 0;
+    if (!m->getExpectationValue(std::vector<double>(angles, angles + nangles), E)) return 3;
+    *energy = E;
+    return 0;
+}
+
+int getFinalState(int nangles, const double *angles, int nbasisvectors, double *finalstate, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (finalstate == nullptr) {
+        log_msg("nullptr finalState passed");
+        return 3;
+    }
+    if (nbasisvectors < 1) {
+        log_msg("NBasisVectors < 1");
+        return 4;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 5;
+    }
+    if (traceInterfaceCalls) log_msg("getFinalState called with NAngles %d NBasisVectors %d", nangles, nbasisvectors);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> state;
+    if (!m->getFinalState(std::vector<double>(angles, angles + nangles), state)) return 6;
+    if (state.size() != (size_t)nbasisvectors) {
+        log_msg("Final state does not have the right number allocated, NBasisVectors should be %zu", state.size());
+        return 7;
+    }
+    std::memcpy(finalstate, state.data(), state.size() * sizeof(double));
+    return 0;
+}
+
+int getFinalStateComplex(int, const double *, int, ANSATZ_DOUBLE_COMPLEX *, void *)
+{
+    log_msg("Cannot get a complex final state without building in complex mode");
+    return 4;
+}
+
+int getGradient_COMP(int nangles, const double *angles, double *gradient, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (gradient == nullptr) {
+        log_msg("nullptr gradient passed");
+        return 3;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 4;
+    }
+    if (traceInterfaceCalls) log_msg("getGradient_COMP called with NAngles %d", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> g;
+    if (!m->getGradientComp(std::vector<double>(angles, angles + nangles), g)) return 5;
+    if ((size_t)nangles != g.size()) {
+        log_msg("Gradient is the wrong size, This is a Bug on the implementation side");
+        return -6;
+    }
+    std::memcpy(gradient, g.data(), g.size() * sizeof(double));
+    return 0;
+}
+
+int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (hessian == nullptr) {
+        log_msg("nullptr hessian passed");
+        return 3;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 4;
+    }
+    if (traceInterfaceCalls) log_msg("getHessian_COMP called with NAngles %d", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> h;
+    if (!m->getHessianComp(std::vector<double>(angles, angles + nangles), h)) return 5;
+    const int P = m->numberOfParameters();
+    if (nangles != P) {
+        log_msg("Hessian has wrong dimensions, This is a bug on the implementation side");
+        return -6;
+    }
+    for (int i = 0; i < P; i++)
+        for (int j = 0; j < P; j++) hessian[(size_t)P * j + i] = h[(size_t)i * P + j];
+    return 0;
+}
+
+// ------------------------------------------------------------ extensions
+int setHamiltonianIntegrals(int norb, const double *h, const double *g, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (norb < 1 || h == nullptr || g == nullptr) {
+        log_msg("norb < 1 or integrals missing");
+        return 2;
+    }
+    if (traceInterfaceCalls) log_msg("setHamiltonianIntegrals called with norb %d", norb);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->storeIntegrals(norb, h, g)) return 3;
+    return 0;
+}
+
+int setNuclearEnergy(double energy, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->storeNuclearEnergy(energy)) return 3;
+    return 0;
+}
+
+int setHamiltonianStorage(int mode, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (mode < 0 || mode > 2) {
+        log_msg("storage mode must be 0, 1 or 2");
+        return 2;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->storeHamiltonianStorage(mode)) return 3;
+    return 0;
+}
+
+int getEnergyGradient(int nangles, const double *angles, double *energy, double *gradient, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (gradient == nullptr) {
+        log_msg("nullptr gradient passed");
+        return 3;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 4;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> g;
+    double E = 0;
+    if (!m->getEnergyGradient(std::vector<double>(angles, angles + nangles), E, g)) return 5;
+    if ((size_t)nangles != g.size()) {
+        log_msg("Gradient is the wrong size: nangles must equal the number of parameters");
+        return -6;
+    }
+    std::memcpy(gradient, g.data(), g.size() * sizeof(double));
+    if (energy) *energy = E;
+    return 0;
+}
+
+int getEnergies(int nvec, int nangles, const double *angles, double *energies, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr || energies == nullptr) {
+        log_msg("nullptr angles or energies passed");
+        return 2;
+    }
+    if (nvec < 1 || nangles < 1) {
+        log_msg("nvec < 1 or NAngles < 1");
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> E;
+    if (!m->getExpectationValues(nvec, nangles, angles, E)) return 5;
+    std::memcpy(energies, E.data(), sizeof(double) * nvec);
+    return 0;
+}
+
+int getEnergyGradients(int nvec, int nangles, const double *angles, double *energies, double *gradients, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr || energies == nullptr || gradients == nullptr) {
+        log_msg("nullptr angles, energies or gradients passed");
+        return 2;
+    }
+    if (nvec < 1 || nangles < 1) {
+        log_msg("nvec < 1 or NAngles < 1");
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> E, g;
+    if (!m->getEnergyGradients(nvec, nangles, angles, E, g)) return 5;
+    if (g.size() != (size_t)nvec * nangles) {
+        log_msg("Gradient is the wrong size: nangles must equal the number of parameters");
+        return -6;
+    }
+    std::memcpy(energies, E.data(), sizeof(double) * nvec);
+    std::memcpy(gradients, g.data(), sizeof(double) * g.size());
+    return 0;
+}
+
+int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    long long d;
+    int no, np;
+    if (!m->getDimensions(d, no, np)) return 5;
+    if (sectordim) *sectordim = d;
+    if (noperators) *noperators = no;
+    if (nparams) *nparams = np;
+    return 0;
+}
+
+} // extern "C"

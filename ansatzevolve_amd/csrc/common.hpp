@@ -1,0 +1,60 @@
+// Shared definitions of the MI355X statevector engine.
+#pragma once
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace ae {
+
+// Sector kinds, chosen exactly as the reference's manager chooses its
+// compressor (src/AnsatzManager.cpp:88-100): particle number and S_z fixed,
+// particle number only, or the full Fock space.
+enum SectorMode : int { kFull = 0, kNumber = 1, kSz = 2 };
+
+// One excitation operator on spin-orbital bit strings, reduced to the three
+// masks the pair rule needs (src/fusedevolve.cpp:27-145):
+//   b is paired with b' = b ^ cre ^ ann when (b & (cre|ann)) == ann (the
+//   "annihilate" side, phase +1) or == cre (phase -1); the phase is further
+//   multiplied by (-1)^popc(b & mask) and by `flip` (-1 per out-of-order
+//   index pair, a0 > a1 and a2 > a3).
+struct DevOp {
+    uint32_t cre;
+    uint32_t ann;
+    uint32_t mask;
+    int32_t flip;
+};
+
+// Builds the DevOp of a validated operator (a: zero-based, -1 = absent).
+// Returns false for operators the reference's real build cannot evolve:
+// anything that is not a single a^+_a a_b (a != b) or a double with four
+// distinct indices, or that addresses a spin-orbital >= nq.
+inline bool make_devop(const int8_t a[4], int nq, DevOp &o, std::string &why)
+{
+    auto inr = [nq](int x) { return x >= 0 && x < nq; };
+    if (a[2] < 0 && a[3] < 0) {
+        if (!inr(a[0]) || !inr(a[1])) { why = "single excitation index out of range"; return false; }
+        if (a[0] == a[1]) { why = "number operators need the complex build"; return false; }
+        o.cre = 1u << a[0];
+        o.ann = 1u << a[1];
+        uint32_t m = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1);
+        o.mask = m & ~(o.cre | o.ann);
+        o.flip = 1;
+        return true;
+    }
+    if (a[2] < 0 || a[3] < 0) { why = "operator is neither a single nor a double excitation"; return false; }
+    for (int i = 0; i < 4; i++)
+        if (!inr(a[i])) { why = "double excitation index out of range"; return false; }
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++)
+            if (a[i] == a[j]) { why = "double excitation with repeated index (diagonal part) is unsupported in the real build"; return false; }
+    o.cre = (1u << a[0]) | (1u << a[1]);
+    o.ann = (1u << a[2]) | (1u << a[3]);
+    uint32_t m = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1) ^ ((1u << a[2]) - 1) ^ ((1u << a[3]) - 1);
+    o.mask = m & ~(o.cre | o.ann);
+    o.flip = ((a[0] > a[1]) ? -1 : 1) * ((a[2] > a[3]) ? -1 : 1);
+    return true;
+}
+
+void log_msg(const char *fmt, ...);
+
+} // namespace ae

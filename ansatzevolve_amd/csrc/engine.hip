@@ -1,0 +1,799 @@
+// Device engine of the statevector path (host side of the kernels).
+//
+// One HIP stream per engine (one engine per ctx).  Everything an evaluation
+// touches -- psi_0, psi, H psi, the Hamiltonian, the sector maps, the
+// operator table and the sin/cos table -- is resident in HBM; a call moves
+// only the per-operator sin/cos (16 B per operator) in and the results out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace ae {
+
+#define HCHK(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            err = std::string(#expr) + " failed: " + hipGetErrorString(e_);           \
+            return false;                                                              \
+        }                                                                              \
+    } while (0)
+
+#define LCHK()                                                                         \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess) {                                                        \
+            err = std::string("kernel launch failed: ") + hipGetErrorString(e_);      \
+            return false;                                                              \
+        }                                                                              \
+    } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    bool ensure(size_t cnt, std::string &err)
+    {
+        if (cnt <= n && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (cnt == 0) cnt = 1;
+        hipError_t e = hipMalloc(&p, cnt * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            err = std::string("hipMalloc of ") + std::to_string(cnt * sizeof(T)) + " bytes failed: " +
+                  hipGetErrorString(e);
+            return false;
+        }
+        n = cnt;
+        return true;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+struct TileSet {
+    uint32_t U = 0;
+    int ntiles = 0, maxRows = 0;
+    DBuf<int32_t> tileStart, rows;
+    DBuf<int8_t> localRow;
+};
+
+struct Seg {
+    int k0, k1;
+    int tileset;  // -1: a single operator applied by the generic kernel
+};
+
+static long long binom_ll(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    long long r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+struct EngineImpl {
+    hipStream_t stream = nullptr;
+    int nq = 0, mode = 0, nel = 0, nup = 0, ndown = 0, n = 0;
+    long long N = 0;
+    int na = 0, nb = 0;
+    size_t ldsMax = 65536;
+    std::vector<uint32_t> states, unA, unB;
+    std::vector<int32_t> rkA, rkB;
+    DBuf<uint32_t> d_states, d_unA, d_unB, d_binom;
+    DBuf<int32_t> d_rkA, d_rkB;
+
+    int M = 0, P = 0;
+    std::vector<DevOp> hops;
+    std::vector<int> order;
+    std::vector<double> scale;
+    DBuf<DevOp> d_ops;
+    std::vector<Seg> segs;
+    std::vector<std::unique_ptr<TileSet>> tilesets;
+    int slots = 1, Ggen = 1;
+    int ngeneric = 0;
+
+    bool haveStart = false;
+    DBuf<double> d_start, d_psi, d_phi, d_lam;
+    DBuf<double2> d_cs, d_csneg;
+    DBuf<double> d_partial, d_epart, d_out;
+    std::vector<double2> h_cs, h_csneg;
+
+    int hkind = 0;
+    long long nnz = 0;
+    int W = 0;
+    DBuf<int64_t> d_rowptr;
+    DBuf<int32_t> d_col;
+    DBuf<double> d_val;
+
+    // batch and Hessian workspaces
+    DBuf<double> b_psi, b_lam, b_partial, b_epart, b_out;
+    DBuf<double2> b_cs, b_csneg;
+    DBuf<double> hz_B, hz_HB, hz_psi2, hz_lam2, hz_xpart, hz_dpart, hz_gram, hz_out, hz_epart;
+
+    int eslots() const { return (int)((N + kBlock - 1) / kBlock); }
+
+    DevSector sec() const
+    {
+        DevSector s;
+        s.mode = mode;
+        s.n = n;
+        s.nel = nel;
+        s.nb = nb;
+        s.maskB = (n > 0) ? ((1u << n) - 1) : 0;
+        s.states = d_states.p;
+        s.rkA = d_rkA.p;
+        s.rkB = d_rkB.p;
+        s.binom = d_binom.p;
+        return s;
+    }
+
+    SegArgs segargs(const Seg &sg, bool reverse) const
+    {
+        const TileSet &t = *tilesets[sg.tileset];
+        SegArgs a;
+        a.n = n;
+        a.nb = nb;
+        a.maskB = (1u << n) - 1;
+        a.unA = d_unA.p;
+        a.unB = d_unB.p;
+        a.rkA = d_rkA.p;
+        a.rkB = d_rkB.p;
+        a.tileStart = t.tileStart.p;
+        a.rows = t.rows.p;
+        a.localRow = t.localRow.p;
+        a.ops = d_ops.p;
+        a.k0 = sg.k0;
+        a.k1 = sg.k1;
+        a.maxRows = t.maxRows;
+        a.reverse = reverse ? 1 : 0;
+        return a;
+    }
+
+    size_t seg_lds(const Seg &sg, int nv) const
+    {
+        return (size_t)nv * tilesets[sg.tileset]->maxRows * nb * sizeof(double);
+    }
+
+    // Tiles for the alpha-bit set U (S_z sector).
+    int tileset_for(uint32_t U, std::string &err)
+    {
+        for (size_t i = 0; i < tilesets.size(); i++)
+            if (tilesets[i]->U == U) return (int)i;
+        auto t = std::make_unique<TileSet>();
+        t->U = U;
+        std::map<uint32_t, std::vector<int>> groups;  // spectator pattern -> alpha ranks
+        for (int ra = 0; ra < na; ra++) groups[unA[ra] & ~U].push_back(ra);
+        std::vector<int32_t> start, rows;
+        std::vector<int8_t> local(na, 0);
+        int maxRows = 0;
+        for (auto &kv : groups) {
+            start.push_back((int32_t)rows.size());
+            int li = 0;
+            for (int ra : kv.second) {  // ascending alpha rank within a tile
+                rows.push_back(ra);
+                local[ra] = (int8_t)li++;
+            }
+            maxRows = std::max(maxRows, li);
+        }
+        start.push_back((int32_t)rows.size());
+        t->ntiles = (int)groups.size();
+        t->maxRows = maxRows;
+        if (maxRows > kMaxTileRows) {
+            err = "tile too large";
+            return -1;
+        }
+        if (!t->tileStart.ensure(start.size(), err) || !t->rows.ensure(rows.size(), err) ||
+            !t->localRow.ensure(local.size(), err))
+            return -1;
+        if (hipMemcpy(t->tileStart.p, start.data(), start.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t->rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t->localRow.p, local.data(), local.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "tile upload failed";
+            return -1;
+        }
+        tilesets.push_back(std::move(t));
+        return (int)tilesets.size() - 1;
+    }
+
+    void fill_cs(const double *angles, int nvec)
+    {
+        h_cs.resize((size_t)nvec * M);
+        for (size_t i = 0; i < (size_t)nvec * M; i++) {
+            double s, c;
+            // the reference's mysincos (src/linalg.h:31-35)
+            __builtin_sincos(angles[i], &s, &c);
+            h_cs[i] = make_double2(c, s);
+        }
+    }
+
+    bool upload_cs(const double *angles, int nvec, DBuf<double2> &fwd, DBuf<double2> &rev, std::string &err)
+    {
+        fill_cs(angles, nvec);
+        const size_t cnt = (size_t)nvec * M;
+        if (!fwd.ensure(cnt, err) || !rev.ensure(cnt, err)) return false;
+        // sincos(-x) = (-sin x, cos x): the reverse sweep's negated angles
+        h_csneg.resize(cnt);
+        for (size_t i = 0; i < cnt; i++) h_csneg[i] = make_double2(h_cs[i].x, -h_cs[i].y);
+        HCHK(hipMemcpyAsync(fwd.p, h_cs.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(rev.p, h_csneg.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, stream));
+        HCHK(hipStreamSynchronize(stream));  // the host tables are reused by the next call
+        return true;
+    }
+
+    // Forward sweep of nvec vectors with per-vector angle tables.
+    bool sweep_forward(double *v, int nvec, const double2 *cs, std::string &err)
+    {
+        for (const Seg &sg : segs) {
+            if (sg.tileset >= 0) {
+                const TileSet &t = *tilesets[sg.tileset];
+                hipLaunchKernelGGL((k_seg<1, false>), dim3(t.ntiles, nvec), dim3(kBlock), seg_lds(sg, 1), stream,
+                                   segargs(sg, false), v, nullptr, N, cs, (long long)M, nullptr, 0, 0);
+            } else {
+                hipLaunchKernelGGL((k_op_generic<false>), dim3(Ggen, nvec), dim3(kBlock), 0, stream, v, nullptr, N,
+                                   N, sec(), d_ops.p, sg.k0, cs, (long long)M, nullptr, 0, 0);
+            }
+            LCHK();
+        }
+        return true;
+    }
+
+    // Adjoint sweep of (phi, lambda) pairs with per-operator brackets.
+    bool sweep_reverse(double *phi, double *lam, int nvec, const double2 *csneg, double *partial, std::string &err)
+    {
+        const long long pstride = (long long)M * slots;
+        const long long total = pstride * nvec;
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 4096)),
+                           dim3(kBlock), 0, stream, partial, total, 0.0);
+        LCHK();
+        for (int si = (int)segs.size() - 1; si >= 0; si--) {
+            const Seg &sg = segs[si];
+            if (sg.tileset >= 0) {
+                const TileSet &t = *tilesets[sg.tileset];
+                hipLaunchKernelGGL((k_seg<2, true>), dim3(t.ntiles, nvec), dim3(kBlock), seg_lds(sg, 2), stream,
+                                   segargs(sg, true), phi, lam, N, csneg, (long long)M, partial, slots, pstride);
+            } else {
+                hipLaunchKernelGGL((k_op_generic<true>), dim3(Ggen, nvec), dim3(kBlock), 0, stream, phi, lam, N, N,
+                                   sec(), d_ops.p, sg.k0, csneg, (long long)M, partial, slots, pstride);
+            }
+            LCHK();
+        }
+        return true;
+    }
+
+    // y = M^T x for nvec vectors (stride N); energy partials epart[v][eslots].
+    bool ham_apply(const double *x, double *y, int nvec, double *epart, std::string &err)
+    {
+        const int es = eslots();
+        if (hkind == 2) {
+            int v = 0;
+            while (v < nvec) {
+                const int left = nvec - v;
+                const double *xv = x + (size_t)v * N;
+                double *yv = y + (size_t)v * N;
+                double *ev = epart + (size_t)v * es;
+                if (left >= 8) {
+                    hipLaunchKernelGGL(k_ell<8>, dim3(es, left / 8), dim3(kBlock), 0, stream, d_col.p, d_val.p, W, N,
+                                       xv, yv, N, ev, es);
+                    v += (left / 8) * 8;
+                } else if (left >= 4) {
+                    hipLaunchKernelGGL(k_ell<4>, dim3(es, 1), dim3(kBlock), 0, stream, d_col.p, d_val.p, W, N, xv,
+                                       yv, N, ev, es);
+                    v += 4;
+                } else if (left >= 2) {
+                    hipLaunchKernelGGL(k_ell<2>, dim3(es, 1), dim3(kBlock), 0, stream, d_col.p, d_val.p, W, N, xv,
+                                       yv, N, ev, es);
+                    v += 2;
+                } else {
+                    hipLaunchKernelGGL(k_ell<1>, dim3(es, 1), dim3(kBlock), 0, stream, d_col.p, d_val.p, W, N, xv,
+                                       yv, N, ev, es);
+                    v += 1;
+                }
+                LCHK();
+            }
+            return true;
+        }
+        if (hkind == 1) {
+            hipLaunchKernelGGL(k_rows, dim3(es, nvec), dim3(kBlock), 0, stream, d_rowptr.p, d_col.p, d_val.p, N, x, y,
+                               N, epart, es);
+            LCHK();
+            return true;
+        }
+        err = "no Hamiltonian on the device";
+        return false;
+    }
+};
+
+Engine::Engine() : d_(new EngineImpl) {}
+
+Engine::~Engine()
+{
+    if (d_ && d_->stream) (void)hipStreamDestroy(d_->stream);
+}
+
+long long Engine::dim() const { return d_->N; }
+int Engine::mode() const { return d_->mode; }
+const std::vector<uint32_t> &Engine::states() const { return d_->states; }
+long long Engine::ham_nnz() const { return d_->nnz; }
+int Engine::ham_kind() const { return d_->hkind; }
+int Engine::num_segments() const { return (int)d_->segs.size(); }
+int Engine::num_generic_ops() const { return d_->ngeneric; }
+
+long long Engine::rank(uint32_t b) const
+{
+    const auto &st = d_->states;
+    auto it = std::lower_bound(st.begin(), st.end(), b);
+    if (it == st.end() || *it != b) return -1;
+    return (long long)(it - st.begin());
+}
+
+bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (nq < 1 || nq > 30) {
+        err = "number of qubits must be 1..30";
+        return false;
+    }
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HCHK(hipGetDeviceProperties(&prop, dev));
+    e.ldsMax = prop.sharedMemPerBlock;
+    if (const char *b = getenv("AE_LDS_BUDGET")) e.ldsMax = std::min<size_t>(e.ldsMax, (size_t)atol(b));
+    HCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    e.nq = nq;
+    e.mode = mode;
+    e.nel = nel;
+    e.nup = nup;
+    e.ndown = ndown;
+    e.n = nq / 2;
+    if (mode == kSz && (nq % 2 != 0)) {
+        err = "S_z sector needs an even qubit count";
+        return false;
+    }
+    // sector bit strings, ascending (src/operatorpool.cpp:207-242,
+    // src/operatorpool.h:49-77)
+    e.states.clear();
+    if (mode == kSz) {
+        const int n = e.n;
+        e.rkA.assign(1u << n, -1);
+        e.rkB.assign(1u << n, -1);
+        e.unA.clear();
+        e.unB.clear();
+        for (uint32_t s = 0; s < (1u << n); s++) {
+            if (__builtin_popcount(s) == nup) {
+                e.rkA[s] = (int32_t)e.unA.size();
+                e.unA.push_back(s);
+            }
+            if (__builtin_popcount(s) == ndown) {
+                e.rkB[s] = (int32_t)e.unB.size();
+                e.unB.push_back(s);
+            }
+        }
+        e.na = (int)e.unA.size();
+        e.nb = (int)e.unB.size();
+        e.N = (long long)e.na * e.nb;
+        e.states.resize(e.N);
+        for (long long i = 0; i < e.N; i++)
+            e.states[i] = (e.unA[i / e.nb] << n) | e.unB[i % e.nb];
+        if (!e.d_unA.ensure(e.na, err) || !e.d_unB.ensure(e.nb, err) || !e.d_rkA.ensure(1u << n, err) ||
+            !e.d_rkB.ensure(1u << n, err))
+            return false;
+        HCHK(hipMemcpy(e.d_unA.p, e.unA.data(), 4 * e.na, hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(e.d_unB.p, e.unB.data(), 4 * e.nb, hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(e.d_rkA.p, e.rkA.data(), 4 * (1u << n), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(e.d_rkB.p, e.rkB.data(), 4 * (1u << n), hipMemcpyHostToDevice));
+    } else {
+        const uint32_t full = 1u << nq;
+        if (mode == kNumber) {
+            e.states.reserve((size_t)binom_ll(nq, nel));
+            for (uint32_t s = 0; s < full; s++)
+                if (__builtin_popcount(s) == nel) e.states.push_back(s);
+        } else {
+            e.states.resize(full);
+            std::iota(e.states.begin(), e.states.end(), 0u);
+        }
+        e.N = (long long)e.states.size();
+        std::vector<uint32_t> bn(33 * 33, 0);
+        for (int p = 0; p < 33; p++)
+            for (int t = 0; t < 33; t++) bn[p * 33 + t] = (uint32_t)std::min<long long>(binom_ll(p, t), 0xffffffffll);
+        if (!e.d_binom.ensure(bn.size(), err)) return false;
+        HCHK(hipMemcpy(e.d_binom.p, bn.data(), 4 * bn.size(), hipMemcpyHostToDevice));
+    }
+    if (!e.d_states.ensure(e.N, err)) return false;
+    HCHK(hipMemcpy(e.d_states.p, e.states.data(), 4 * e.N, hipMemcpyHostToDevice));
+    e.Ggen = (int)std::min<long long>((e.N + kBlock - 1) / kBlock, 1024);
+    if (!e.d_psi.ensure(e.N, err) || !e.d_phi.ensure(e.N, err) || !e.d_lam.ensure(e.N, err) ||
+        !e.d_start.ensure(e.N, err))
+        return false;
+    return true;
+}
+
+bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<int> &order,
+                           const std::vector<double> &scale, int nparams, std::string &err)
+{
+    EngineImpl &e = *d_;
+    const int M = (int)(ops4.size() / 4);
+    e.M = M;
+    e.P = nparams;
+    e.order = order;
+    e.scale = scale;
+    e.hops.resize(M);
+    for (int k = 0; k < M; k++)
+        if (!make_devop(&ops4[4 * k], e.nq, e.hops[k], err)) {
+            err = "operator " + std::to_string(k + 1) + ": " + err;
+            return false;
+        }
+    if (!e.d_ops.ensure(M, err)) return false;
+    HCHK(hipMemcpy(e.d_ops.p, e.hops.data(), sizeof(DevOp) * M, hipMemcpyHostToDevice));
+
+    // Fused segments (S_z sector): maximal runs of consecutive operators whose
+    // union U of alpha bits keeps a tile of two vectors within the LDS budget.
+    // Operator order is preserved, so the unitary is unchanged.
+    e.segs.clear();
+    e.ngeneric = 0;
+    int maxTiles = 0;
+    if (e.mode == kSz) {
+        const int n = e.n;
+        const uint32_t amask = (1u << n) - 1;
+        auto rows_for = [&](int u) {
+            long long r = 0;
+            for (int j = 0; j <= u; j++) r = std::max(r, binom_ll(u, j));
+            return r;
+        };
+        auto fits = [&](uint32_t U) {
+            const int u = __builtin_popcount(U);
+            const long long rows = rows_for(u);
+            return rows <= kMaxTileRows && (size_t)(2 * rows * e.nb * sizeof(double)) <= e.ldsMax;
+        };
+        int k = 0;
+        const int maxOps = 64;
+        while (k < M) {
+            const uint32_t Uk = ((e.hops[k].cre | e.hops[k].ann) >> n) & amask;
+            if (!fits(Uk)) {
+                e.segs.push_back({k, k + 1, -1});
+                e.ngeneric++;
+                k++;
+                continue;
+            }
+            uint32_t U = Uk;
+            int k1 = k + 1;
+            while (k1 < M && k1 - k < maxOps) {
+                const uint32_t Un = U | (((e.hops[k1].cre | e.hops[k1].ann) >> n) & amask);
+                if (!fits(Un)) break;
+                U = Un;
+                k1++;
+            }
+            const int ts = e.tileset_for(U, err);
+            if (ts < 0) return false;
+            maxTiles = std::max(maxTiles, e.tilesets[ts]->ntiles);
+            e.segs.push_back({k, k1, ts});
+            k = k1;
+        }
+        for (auto &t : e.tilesets) {
+            const size_t lds = (size_t)2 * t->maxRows * e.nb * sizeof(double);
+            if (lds > 65536) {
+                HCHK(hipFuncSetAttribute((const void *)k_seg<2, true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                HCHK(hipFuncSetAttribute((const void *)k_seg<2, false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                HCHK(hipFuncSetAttribute((const void *)k_seg<1, false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
+        }
+    } else {
+        for (int k = 0; k < M; k++) e.segs.push_back({k, k + 1, -1});
+        e.ngeneric = M;
+    }
+    e.slots = std::max(maxTiles, e.Ggen);
+    if (!e.d_cs.ensure(M, err) || !e.d_csneg.ensure(M, err) || !e.d_partial.ensure((size_t)M * e.slots, err) ||
+        !e.d_epart.ensure(e.eslots(), err) || !e.d_out.ensure(M + 1, err))
+        return false;
+    return true;
+}
+
+bool Engine::set_start(const std::vector<double> &start, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if ((long long)start.size() != e.N) {
+        err = "start vector has the wrong size";
+        return false;
+    }
+    HCHK(hipMemcpy(e.d_start.p, start.data(), 8 * e.N, hipMemcpyHostToDevice));
+    e.haveStart = true;
+    return true;
+}
+
+bool Engine::set_ham_rows(const std::vector<int64_t> &rowptr, const std::vector<int32_t> &col,
+                          const std::vector<double> &val, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if ((long long)rowptr.size() != e.N + 1) {
+        err = "row pointer has the wrong size";
+        return false;
+    }
+    if (!e.d_rowptr.ensure(rowptr.size(), err) || !e.d_col.ensure(col.size(), err) ||
+        !e.d_val.ensure(val.size(), err))
+        return false;
+    HCHK(hipMemcpy(e.d_rowptr.p, rowptr.data(), 8 * rowptr.size(), hipMemcpyHostToDevice));
+    if (!col.empty()) {
+        HCHK(hipMemcpy(e.d_col.p, col.data(), 4 * col.size(), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(e.d_val.p, val.data(), 8 * val.size(), hipMemcpyHostToDevice));
+    }
+    e.nnz = (long long)val.size();
+    e.hkind = 1;
+    return true;
+}
+
+bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std::vector<double> &g, int storage,
+                               std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (e.mode != kSz) {
+        err = "integral Hamiltonians need the S_z sector (initial state and operators conserving S_z)";
+        return false;
+    }
+    if (2 * norb != e.nq) {
+        err = "number of qubits must be twice the number of orbitals";
+        return false;
+    }
+    const int n = e.n, ku = e.nup, kd = e.ndown;
+    const long long W = 1 + (long long)kd * (n - kd) + (long long)ku * (n - ku) +
+                        binom_ll(kd, 2) * binom_ll(n - kd, 2) + binom_ll(ku, 2) * binom_ll(n - ku, 2) +
+                        (long long)ku * (n - ku) * kd * (n - kd);
+    const size_t bytes = (size_t)W * e.N * 12;
+    size_t freeB = 0, totalB = 0;
+    HCHK(hipMemGetInfo(&freeB, &totalB));
+    const bool fitsExplicit = bytes + (size_t)(4ull << 30) < freeB;
+    if (storage == kHamMatrixFree || (storage == kHamAuto && !fitsExplicit)) {
+        err = "matrix-free Hamiltonian application is not available in this build (explicit rows need " +
+              std::to_string(bytes >> 20) + " MiB)";
+        return false;
+    }
+    if (!fitsExplicit) {
+        err = "explicit Hamiltonian rows do not fit in device memory";
+        return false;
+    }
+    DBuf<double> dh, dg;
+    if (!dh.ensure(h.size(), err) || !dg.ensure(g.size(), err)) return false;
+    HCHK(hipMemcpy(dh.p, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(dg.p, g.data(), 8 * g.size(), hipMemcpyHostToDevice));
+    e.d_rowptr.release();
+    if (!e.d_col.ensure((size_t)W * e.N, err) || !e.d_val.ensure((size_t)W * e.N, err)) return false;
+    EllBuildArgs a;
+    a.n = n;
+    a.ku = ku;
+    a.kd = kd;
+    a.nb = e.nb;
+    a.N = e.N;
+    a.W = (int)W;
+    a.unA = e.d_unA.p;
+    a.unB = e.d_unB.p;
+    a.rkA = e.d_rkA.p;
+    a.rkB = e.d_rkB.p;
+    a.h = dh.p;
+    a.g = dg.p;
+    a.col = e.d_col.p;
+    a.val = e.d_val.p;
+    hipLaunchKernelGGL(k_ell_build, dim3((unsigned)((e.N + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream, a);
+    LCHK();
+    HCHK(hipStreamSynchronize(e.stream));
+    e.W = (int)W;
+    e.nnz = W * e.N;
+    e.hkind = 2;
+    return true;
+}
+
+bool Engine::evolve(const double *angles, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
+    HCHK(hipMemcpyAsync(e.d_psi.p, e.d_start.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.sweep_forward(e.d_psi.p, 1, e.d_cs.p, err)) return false;
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::energy(double *E, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.ham_apply(e.d_psi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
+                       e.d_out.p, 1.0);
+    LCHK();
+    HCHK(hipMemcpyAsync(E, e.d_out.p, 8, hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::gradient(const double *angles, double *E, double *deriv, std::string &err)
+{
+    EngineImpl &e = *d_;
+    // sin/cos of the current angles (the forward sweep may have used them
+    // already; recomputing keeps this call self-contained)
+    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
+    HCHK(hipMemcpyAsync(e.d_phi.p, e.d_psi.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
+                       e.d_out.p + e.M, 1.0);
+    LCHK();
+    if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, e.d_csneg.p, e.d_partial.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, e.M, e.slots,
+                       (long long)e.slots, 1LL, e.d_out.p, 2.0);
+    LCHK();
+    std::vector<double> out(e.M + 1);
+    HCHK(hipMemcpyAsync(out.data(), e.d_out.p, 8 * (e.M + 1), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    std::memcpy(deriv, out.data(), 8 * e.M);
+    if (E) *E = out[e.M];
+    return true;
+}
+
+bool Engine::download_state(double *out, std::string &err)
+{
+    EngineImpl &e = *d_;
+    HCHK(hipMemcpyAsync(out, e.d_psi.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::batch(int nvec, const double *angles, double *E, double *deriv, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    const size_t NV = (size_t)nvec * e.N;
+    const int es = e.eslots();
+    if (!e.b_psi.ensure(NV, err) || !e.b_lam.ensure(NV, err) || !e.b_epart.ensure((size_t)nvec * es, err) ||
+        !e.b_out.ensure((size_t)nvec * (e.M + 1), err))
+        return false;
+    if (deriv && !e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
+    if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
+    for (int v = 0; v < nvec; v++)
+        HCHK(hipMemcpyAsync(e.b_psi.p + (size_t)v * e.N, e.d_start.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
+    if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
+                       e.b_out.p, 1.0);
+    LCHK();
+    if (deriv) {
+        if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
+                           e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
+        LCHK();
+    }
+    HCHK(hipMemcpyAsync(E, e.b_out.p, 8 * nvec, hipMemcpyDeviceToHost, e.stream));
+    if (deriv)
+        HCHK(hipMemcpyAsync(deriv, e.b_out.p + nvec, 8 * (size_t)nvec * e.M, hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+// Per-parameter Hessian, d2E = 2<psi|H|d2 psi> + 2<d psi|H|d psi>.
+//   1. psi = U psi_0, lambda = H psi, adjoint sweep: brackets give the
+//      gradient and the sweep ends at lambda_0 = U^T H psi.
+//   2. One forward sweep carries psi (from psi_0), lambda (from lambda_0) and
+//      the per-parameter tangent accumulators B_mu; at operator k, after the
+//      rotation, B_{order_k} += scale_k G_k psi, and
+//      X[mu][k] = <lambda|G_k|B_mu>, D[k] = <lambda|G_k G_k|psi>.
+//      At the end B_mu = d psi / d theta_mu.
+//   3. THT = 2 B (H B)^T;  Y[mu][nu] = sum_{k in nu} scale_k X[mu][k];
+//      Hessian = THT + 2 (Y + Y^T) - 2 diag(sum_{k in mu} scale_k^2 D[k]).
+// This is the reference's quantity (src/fusedevolve.cpp:2437-2780) obtained
+// with forward-propagated tangents instead of one adjoint sweep per angle.
+bool Engine::hessian(const double *angles, double *hess, double *grad, double *E, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    const int M = e.M, P = e.P;
+    const long long N = e.N;
+    // 1.
+    if (!evolve(angles, err)) return false;
+    std::vector<double> deriv(M);
+    double E0 = 0;
+    if (!gradient(angles, &E0, deriv.data(), err)) return false;
+    // d_phi ~ psi_0 now, d_lam = lambda_0.
+    // parameters ordered by their first operator, so the active set at
+    // operator k is a prefix
+    std::vector<int> first(P, M);
+    for (int k = 0; k < M; k++) first[e.order[k]] = std::min(first[e.order[k]], k);
+    std::vector<int> perm(P);  // perm position -> parameter
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return first[a] < first[b]; });
+    std::vector<int> pos(P);
+    for (int i = 0; i < P; i++) pos[perm[i]] = i;
+    const int G = (int)std::min<long long>((N + kBlock - 1) / kBlock, 512);
+    const size_t PN = (size_t)P * N;
+    if (!e.hz_B.ensure(PN, err) || !e.hz_HB.ensure(PN, err) || !e.hz_psi2.ensure(2 * N, err) ||
+        !e.hz_lam2.ensure(2 * N, err) || !e.hz_xpart.ensure((size_t)M * P * G, err) ||
+        !e.hz_dpart.ensure((size_t)M * G, err) || !e.hz_out.ensure((size_t)M * P + M + (size_t)P * P, err) ||
+        !e.hz_epart.ensure((size_t)P * e.eslots(), err))
+        return false;
+    auto fill = [&](double *p, size_t cnt) {
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<size_t>((cnt + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0,
+                           e.stream, p, (long long)cnt, 0.0);
+    };
+    fill(e.hz_B.p, PN);
+    fill(e.hz_xpart.p, (size_t)M * P * G);
+    LCHK();
+    double *psiA = e.hz_psi2.p, *psiB = e.hz_psi2.p + N;
+    double *lamA = e.hz_lam2.p, *lamB = e.hz_lam2.p + N;
+    HCHK(hipMemcpyAsync(psiA, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+    HCHK(hipMemcpyAsync(lamA, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+    const DevSector sec = e.sec();
+    int nactive = 0;
+    for (int k = 0; k < M; k++) {
+        while (nactive < P && first[perm[nactive]] <= k) nactive++;
+        const int slices = (nactive + 3) / 4;
+        hipLaunchKernelGGL((k_tangent<4>), dim3(G, std::max(slices, 1)), dim3(kBlock), 0, e.stream, psiA, psiB, lamA,
+                           lamB, e.hz_B.p, N, nactive, N, sec, e.d_ops.p, k, e.d_cs.p, pos[e.order[k]],
+                           e.scale[k], e.hz_xpart.p + (size_t)k * P * G, G, e.hz_dpart.p + (size_t)k * G);
+        LCHK();
+        std::swap(psiA, psiB);
+        std::swap(lamA, lamB);
+    }
+    // 3. H B, Gram, reductions
+    if (!e.ham_apply(e.hz_B.p, e.hz_HB.p, P, e.hz_epart.p, err)) return false;
+    const int chunks = 64;
+    const int tiles = (P + 15) / 16;
+    if (!e.hz_gram.ensure((size_t)chunks * P * P, err)) return false;
+    hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_HB.p, N, P, N,
+                       chunks, e.hz_gram.p);
+    LCHK();
+    double *outX = e.hz_out.p, *outD = outX + (size_t)M * P, *outT = outD + M;
+    hipLaunchKernelGGL(k_reduce, dim3(M * P), dim3(kBlock), 0, e.stream, e.hz_xpart.p, M * P, G, (long long)G, 1LL,
+                       outX, 1.0);
+    hipLaunchKernelGGL(k_reduce, dim3(M), dim3(kBlock), 0, e.stream, e.hz_dpart.p, M, G, (long long)G, 1LL, outD, 1.0);
+    hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
+                       (long long)P * P, outT, 2.0);
+    LCHK();
+    std::vector<double> hout((size_t)M * P + M + (size_t)P * P);
+    HCHK(hipMemcpyAsync(hout.data(), e.hz_out.p, 8 * hout.size(), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    const double *X = hout.data(), *D = X + (size_t)M * P, *T = D + M;
+    // assemble in parameter order
+    std::vector<double> Y((size_t)P * P, 0.0), Dg(P, 0.0);
+    for (int k = 0; k < M; k++) {
+        const int nu = e.order[k];
+        for (int mu = 0; mu < P; mu++) Y[(size_t)mu * P + nu] += e.scale[k] * X[(size_t)k * P + pos[mu]];
+        Dg[nu] += e.scale[k] * e.scale[k] * D[k];
+    }
+    for (int i = 0; i < P; i++)
+        for (int j = 0; j < P; j++) {
+            const int a = std::min(pos[i], pos[j]), b = std::max(pos[i], pos[j]);
+            double v = T[(size_t)a * P + b] + 2.0 * (Y[(size_t)i * P + j] + Y[(size_t)j * P + i]);
+            if (i == j) v -= 2.0 * Dg[i];
+            hess[(size_t)i * P + j] = v;
+        }
+    if (grad) {
+        for (int p = 0; p < P; p++) grad[p] = 0;
+        for (int k = 0; k < M; k++) grad[e.order[k]] += e.scale[k] * deriv[k];
+    }
+    if (E) *E = E0;
+    return true;
+}
+
+} // namespace ae

@@ -1,0 +1,73 @@
+// Device engine: the statevector, the sector index maps, the operator
+// sweeps and the Hamiltonian, all resident in HBM, driven on one HIP stream.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ae {
+
+struct EngineImpl;
+
+// Which form the Hamiltonian takes on the device.
+enum HamStorage : int { kHamAuto = 0, kHamExplicit = 1, kHamMatrixFree = 2 };
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Sector: nq qubits; mode kSz uses (nup, ndown), kNumber uses nel.
+    bool init(int nq, int mode, int nel, int nup, int ndown, std::string &err);
+    long long dim() const;
+    int mode() const;
+    // Host copy of the sector's bit strings, ascending.
+    const std::vector<uint32_t> &states() const;
+    // Sector index of a full-space bit string, -1 if outside.
+    long long rank(uint32_t bits) const;
+
+    // Operators in circuit order (M x 4, zero-based, -1 absent) and the
+    // parameter map (order[k] = parameter of operator k, scale[k]).
+    bool set_operators(const std::vector<int8_t> &ops4, const std::vector<int> &order,
+                       const std::vector<double> &scale, int nparams, std::string &err);
+    // psi_0 over the sector.
+    bool set_start(const std::vector<double> &start, std::string &err);
+
+    // Hamiltonian as rows of y = M^T x over the sector (output row j holds
+    // (input index, value) pairs), built on the host from triplets.
+    bool set_ham_rows(const std::vector<int64_t> &rowptr, const std::vector<int32_t> &col,
+                      const std::vector<double> &val, std::string &err);
+    // Hamiltonian from integrals (S_z sector only), built on the device.
+    bool set_ham_integrals(int norb, const std::vector<double> &h, const std::vector<double> &g, int storage,
+                           std::string &err);
+
+    // psi = U(angles) psi_0; angles per operator.
+    bool evolve(const double *angles, std::string &err);
+    // E = <psi|H|psi> for the current psi.
+    bool energy(double *E, std::string &err);
+    // Adjoint sweep from the current psi (which must be U(angles) psi_0):
+    // deriv[k] = dE/d(angle_k) per operator; E optional.
+    bool gradient(const double *angles, double *E, double *deriv, std::string &err);
+    // Per-parameter Hessian (row-major P x P), gradient and energy.
+    bool hessian(const double *angles, double *hess, double *grad, double *E, std::string &err);
+    // The current psi over the sector.
+    bool download_state(double *out, std::string &err);
+    // Batched: nvec angle vectors (row-major nvec x M).  deriv may be null.
+    bool batch(int nvec, const double *angles, double *E, double *deriv, std::string &err);
+
+    // Diagnostics for the benchmark and tests.
+    long long ham_nnz() const;
+    int ham_kind() const; // 0 none, 1 rows, 2 uniform-width rows, 3 matrix-free
+    int num_segments() const;
+    int num_generic_ops() const;
+
+private:
+    std::unique_ptr<EngineImpl> d_;
+};
+
+} // namespace ae

@@ -1,0 +1,472 @@
+// HIP kernels of the statevector engine (gfx950).  See kernels.hpp.
+#include "kernels.hpp"
+
+namespace ae {
+
+__global__ void __launch_bounds__(kBlock) k_fill(double *__restrict__ p, long long n, double v)
+{
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        p[i] = v;
+}
+
+// ------------------------------------------------------------- rotations
+// Rotation on a pair (b, b') with phase s (src/fusedevolve.cpp:213-234):
+//   x_b <- C x_b + s S x_b',   x_b' <- -s S x_b + C x_b'.
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_op_generic(double *__restrict__ v0, double *__restrict__ v1, long long vstride, long long N, DevSector sec,
+                 const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, long long csStride,
+                 double *__restrict__ partial, int slots, long long pstride)
+{
+    __shared__ double red[kBlock / 64];
+    const int vb = blockIdx.y;
+    v0 += vb * vstride;
+    if (BR) v1 += vb * vstride;
+    const DevOp o = ops[k];
+    const double2 t = cs[vb * csStride + k];
+    const double C = t.x, S = t.y;
+    double acc = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        uint32_t b2;
+        const uint32_t b = sec.states[i];
+        const int s = pair_sign(o, b, b2);
+        if (s == 0 || b2 < b) continue;
+        const long long j = sec.rank(b2);
+        if (j < 0) continue;
+        const double x = v0[i], y = v0[j];
+        if (BR) {
+            const double lx = v1[i], ly = v1[j];
+            acc += s * (lx * y - ly * x);
+            v1[i] = s * ly * S + lx * C;
+            v1[j] = -s * lx * S + ly * C;
+        }
+        v0[i] = s * y * S + x * C;
+        v0[j] = -s * x * S + y * C;
+    }
+    if (BR) {
+        const double tot = block_sum(acc, red);
+        if (threadIdx.x == 0) partial[vb * pstride + (long long)k * slots + blockIdx.x] = tot;
+    }
+}
+
+template __global__ void k_op_generic<false>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                             const double2 *, long long, double *, int, long long);
+template __global__ void k_op_generic<true>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                            const double2 *, long long, double *, int, long long);
+
+// One workgroup per tile: the tile's rows of NV vectors are staged in LDS,
+// every operator of the segment is applied there (barrier between
+// operators), brackets are reduced per operator, and the rows are written
+// back once.  HBM/L2 traffic per segment: one read and one write per vector.
+template <int NV, bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_seg(SegArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+          const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+          long long pstride)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[kBlock / 64];
+    __shared__ int rowRank[kMaxTileRows];
+    __shared__ uint32_t rowAlpha[kMaxTileRows];
+
+    const int tile = blockIdx.x, vb = blockIdx.y;
+    v0 += vb * vstride;
+    if (NV > 1) v1 += vb * vstride;
+    cs += vb * csStride;
+    const int r0 = a.tileStart[tile];
+    const int R = a.tileStart[tile + 1] - r0;
+    const int nb = a.nb;
+    if (threadIdx.x < R) {
+        const int ra = a.rows[r0 + threadIdx.x];
+        rowRank[threadIdx.x] = ra;
+        rowAlpha[threadIdx.x] = a.unA[ra];
+    }
+    __syncthreads();
+    double *L0 = lds;
+    double *L1 = lds + (size_t)a.maxRows * nb;
+    for (int r = 0; r < R; r++) {
+        const double *s0 = v0 + (long long)rowRank[r] * nb;
+        for (int c = threadIdx.x; c < nb; c += kBlock) L0[r * nb + c] = s0[c];
+        if (NV > 1) {
+            const double *s1 = v1 + (long long)rowRank[r] * nb;
+            for (int c = threadIdx.x; c < nb; c += kBlock) L1[r * nb + c] = s1[c];
+        }
+    }
+    __syncthreads();
+
+    for (int kk = 0; kk < a.k1 - a.k0; kk++) {
+        const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+        const DevOp o = a.ops[k];
+        const double2 t = cs[k];
+        const double C = t.x, S = t.y;
+        double acc = 0;
+        for (int r = 0; r < R; r++) {
+            const uint32_t alpha = rowAlpha[r];
+            const uint32_t ab = alpha << a.n;
+            for (int c = threadIdx.x; c < nb; c += kBlock) {
+                uint32_t b2;
+                const uint32_t b = ab | a.unB[c];
+                const int s = pair_sign(o, b, b2);
+                if (s == 0 || b2 < b) continue;
+                const uint32_t alpha2 = b2 >> a.n;
+                const int r2 = (alpha2 == alpha) ? r : a.localRow[a.rkA[alpha2]];
+                const int c2 = a.rkB[b2 & a.maskB];
+                const int i1 = r * nb + c, i2 = r2 * nb + c2;
+                const double x = L0[i1], y = L0[i2];
+                if (BR) {
+                    const double lx = L1[i1], ly = L1[i2];
+                    acc += s * (lx * y - ly * x);
+                    L1[i1] = s * ly * S + lx * C;
+                    L1[i2] = -s * lx * S + ly * C;
+                } else if (NV > 1) {
+                    const double lx = L1[i1], ly = L1[i2];
+                    L1[i1] = s * ly * S + lx * C;
+                    L1[i2] = -s * lx * S + ly * C;
+                }
+                L0[i1] = s * y * S + x * C;
+                L0[i2] = -s * x * S + y * C;
+            }
+        }
+        if (BR) {
+            const double tot = block_sum(acc, red);  // contains the barrier
+            if (threadIdx.x == 0) partial[vb * pstride + (long long)k * slots + tile] = tot;
+        } else {
+            __syncthreads();
+        }
+    }
+
+    for (int r = 0; r < R; r++) {
+        double *d0 = v0 + (long long)rowRank[r] * nb;
+        for (int c = threadIdx.x; c < nb; c += kBlock) d0[c] = L0[r * nb + c];
+        if (NV > 1) {
+            double *d1 = v1 + (long long)rowRank[r] * nb;
+            for (int c = threadIdx.x; c < nb; c += kBlock) d1[c] = L1[r * nb + c];
+        }
+    }
+}
+
+#define AE_SEG(NV, BR)                                                                                          \
+    template __global__ void k_seg<NV, BR>(SegArgs, double *, double *, long long, const double2 *, long long, \
+                                           double *, int, long long);
+AE_SEG(1, false)
+AE_SEG(2, false)
+AE_SEG(2, true)
+#undef AE_SEG
+
+// ----------------------------------------------------------- Hamiltonian
+template <int V>
+__global__ void __launch_bounds__(kBlock)
+    k_ell(const int32_t *__restrict__ col, const double *__restrict__ val, int W, long long N,
+          const double *__restrict__ x, double *__restrict__ y, long long vstride, double *__restrict__ epart,
+          int eslots)
+{
+    __shared__ double red[kBlock / 64];
+    const long long j = blockIdx.x * (long long)kBlock + threadIdx.x;
+    const int vb = blockIdx.y * V;
+    x += vb * vstride;
+    y += vb * vstride;
+    double acc[V];
+#pragma unroll
+    for (int t = 0; t < V; t++) acc[t] = 0;
+    if (j < N) {
+        const int32_t *cp = col + j;
+        const double *vp = val + j;
+#pragma unroll 4
+        for (int e = 0; e < W; e++) {
+            const int c = __builtin_nontemporal_load(cp + (size_t)e * N);
+            const double v = __builtin_nontemporal_load(vp + (size_t)e * N);
+#pragma unroll
+            for (int t = 0; t < V; t++) acc[t] += v * x[t * vstride + c];
+        }
+#pragma unroll
+        for (int t = 0; t < V; t++) y[t * vstride + j] = acc[t];
+    }
+#pragma unroll
+    for (int t = 0; t < V; t++) {
+        const double p = (j < N) ? x[t * vstride + j] * acc[t] : 0.0;
+        const double tot = block_sum(p, red);
+        if (threadIdx.x == 0) epart[(long long)(vb + t) * eslots + blockIdx.x] = tot;
+    }
+}
+
+template __global__ void k_ell<1>(const int32_t *, const double *, int, long long, const double *, double *, long long,
+                                  double *, int);
+template __global__ void k_ell<2>(const int32_t *, const double *, int, long long, const double *, double *, long long,
+                                  double *, int);
+template __global__ void k_ell<4>(const int32_t *, const double *, int, long long, const double *, double *, long long,
+                                  double *, int);
+template __global__ void k_ell<8>(const int32_t *, const double *, int, long long, const double *, double *, long long,
+                                  double *, int);
+
+__global__ void __launch_bounds__(kBlock)
+    k_rows(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+           long long N, const double *__restrict__ x, double *__restrict__ y, long long vstride,
+           double *__restrict__ epart, int eslots)
+{
+    __shared__ double red[kBlock / 64];
+    const long long j = blockIdx.x * (long long)kBlock + threadIdx.x;
+    const int vb = blockIdx.y;
+    x += vb * vstride;
+    y += vb * vstride;
+    double acc = 0;
+    if (j < N) {
+        for (int64_t p = rowptr[j]; p < rowptr[j + 1]; p++) acc += x[col[p]] * val[p];
+        y[j] = acc;
+    }
+    const double pe = (j < N) ? x[j] * acc : 0.0;
+    const double tot = block_sum(pe, red);
+    if (threadIdx.x == 0) epart[(long long)vb * eslots + blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_reduce(const double *__restrict__ part, int nrows, int ncols, long long rstride, long long cstride,
+             double *__restrict__ out, double scale)
+{
+    __shared__ double red[kBlock / 64];
+    const int r = blockIdx.x;
+    if (r >= nrows) return;
+    double acc = 0;
+    for (int c = threadIdx.x; c < ncols; c += kBlock) acc += part[r * rstride + c * cstride];
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) out[r] = scale * tot;
+}
+
+// Value of the reference's two-body operator a^+_a a^+_b a_c a_d (a < b,
+// c < d) from chemist integrals in the _twoEInts.bin layout, and its
+// sign mask (src/hamiltonianmatrix.cpp:445-529).  Values at or below
+// 1e-13 in magnitude are dropped as the reference drops them (:483, :509).
+__device__ __forceinline__ double two_body(int a, int b, int c, int d, int n, const double *__restrict__ g,
+                                           uint32_t &mask)
+{
+    const int sa = a >= n, sb = b >= n, sc = c >= n, sd = d >= n;
+    const int pa = a - sa * n, pb = b - sb * n, pc = c - sc * n, pd = d - sd * n;
+    double E = 0;
+    if (sa == sc && sb == sd) E += g[(((size_t)pb * n + pd) * n + pa) * n + pc];
+    if (sa == sd && sb == sc) E -= g[(((size_t)pb * n + pc) * n + pa) * n + pd];
+    const uint32_t m = ((1u << a) - 1) ^ ((1u << b) - 1) ^ ((1u << c) - 1) ^ ((1u << d) - 1);
+    mask = m & ~((1u << a) | (1u << b) | (1u << c) | (1u << d));
+    return fabs(E) > 1e-13 ? E : 0.0;
+}
+
+__device__ __forceinline__ double one_body(int a, int c, int n, const double *__restrict__ h, uint32_t &mask)
+{
+    const double E = h[(size_t)(a % n) * n + (c % n)];
+    mask = (((1u << a) - 1) ^ ((1u << c) - 1)) & ~((1u << a) | (1u << c));
+    return fabs(E) > 1e-13 ? E : 0.0;
+}
+
+__device__ __forceinline__ double signed_by(uint32_t jb, uint32_t mask, double v)
+{
+    return (__popc(jb & mask) & 1) ? -v : v;
+}
+
+// Row j = (alpha, beta) of y = M^T x.  Entry order (identical for every row):
+// diagonal, down singles, up singles, down-down doubles, up-up doubles,
+// up-down doubles.  An entry (i, M(i,j)) sums every operator of the
+// reference's list that maps basis state i to j (create bits C subset of j,
+// destroy bits D subset of i), each with sign (-1)^popc(j & mask)
+// (the matrix-free rule, src/hamiltonianmatrix.cpp:915-1009).
+__global__ void __launch_bounds__(kBlock) k_ell_build(EllBuildArgs a)
+{
+    const long long j = blockIdx.x * (long long)kBlock + threadIdx.x;
+    if (j >= a.N) return;
+    const int n = a.n;
+    const long long ra = j / a.nb;
+    const int rb = (int)(j - ra * a.nb);
+    const uint32_t alpha = a.unA[ra], beta = a.unB[rb];
+    const uint32_t jb = (alpha << n) | beta;
+    int occ[32], occD[16], virD[16], occU[16], virU[16];
+    int no = 0, nod = 0, nvd = 0, nou = 0, nvu = 0;
+    for (int p = 0; p < n; p++) {
+        if (beta >> p & 1) { occD[nod++] = p; occ[no++] = p; }
+        else virD[nvd++] = p;
+    }
+    for (int p = 0; p < n; p++) {
+        if (alpha >> p & 1) { occU[nou++] = p + n; occ[no++] = p + n; }
+        else virU[nvu++] = p + n;
+    }
+    int32_t *col = a.col + j;
+    double *val = a.val + j;
+    const size_t N = (size_t)a.N;
+    int e = 0;
+    uint32_t m;
+    // diagonal
+    {
+        double v = 0;
+        for (int x = 0; x < no; x++) v += signed_by(jb, 0, one_body(occ[x], occ[x], n, a.h, m));
+        for (int x = 0; x < no; x++)
+            for (int y = x + 1; y < no; y++) {
+                const double t = two_body(occ[x], occ[y], occ[x], occ[y], n, a.g, m);
+                v += signed_by(jb, m, t);
+            }
+        col[e * N] = (int32_t)j;
+        val[e * N] = v;
+        e++;
+    }
+    // singles: i = j - {r} + {q}; one-body op (C={r}, D={q}) plus two-body
+    // ops (C={r,x}, D={q,x}) for every spectator x occupied in both.
+    for (int spin = 0; spin < 2; spin++) {
+        const int *oc = spin ? occU : occD;
+        const int *vi = spin ? virU : virD;
+        const int nocc = spin ? nou : nod, nvir = spin ? nvu : nvd;
+        for (int ir = 0; ir < nocc; ir++)
+            for (int iq = 0; iq < nvir; iq++) {
+                const int r = oc[ir], q = vi[iq];
+                double v = 0.0;
+                {
+                    const double t = one_body(r, q, n, a.h, m);
+                    v += signed_by(jb, m, t);
+                }
+                for (int xx = 0; xx < no; xx++) {
+                    const int x = occ[xx];
+                    if (x == r) continue;
+                    const int c1 = r < x ? r : x, c2 = r < x ? x : r;
+                    const int d1 = q < x ? q : x, d2 = q < x ? x : q;
+                    const double t = two_body(c1, c2, d1, d2, n, a.g, m);
+                    v += signed_by(jb, m, t);
+                }
+                const uint32_t ib = jb ^ (1u << r) ^ (1u << q);
+                const long long ci = (long long)a.rkA[ib >> n] * a.nb + a.rkB[ib & ((1u << n) - 1)];
+                col[e * N] = (int32_t)ci;
+                val[e * N] = v;
+                e++;
+            }
+    }
+    // same-spin doubles: C = {r1 < r2}, D = {q1 < q2}
+    for (int spin = 0; spin < 2; spin++) {
+        const int *oc = spin ? occU : occD;
+        const int *vi = spin ? virU : virD;
+        const int nocc = spin ? nou : nod, nvir = spin ? nvu : nvd;
+        for (int i1 = 0; i1 < nocc; i1++)
+            for (int i2 = i1 + 1; i2 < nocc; i2++)
+                for (int j1 = 0; j1 < nvir; j1++)
+                    for (int j2 = j1 + 1; j2 < nvir; j2++) {
+                        const int r1 = oc[i1], r2 = oc[i2], q1 = vi[j1], q2 = vi[j2];
+                        const double t = two_body(r1, r2, q1, q2, n, a.g, m);
+                        const uint32_t ib = jb ^ (1u << r1) ^ (1u << r2) ^ (1u << q1) ^ (1u << q2);
+                        const long long ci = (long long)a.rkA[ib >> n] * a.nb + a.rkB[ib & ((1u << n) - 1)];
+                        col[e * N] = (int32_t)ci;
+                        val[e * N] = signed_by(jb, m, t);
+                        e++;
+                    }
+    }
+    // opposite-spin doubles: C = {r_down < r_up}, D = {q_down < q_up}
+    for (int iu = 0; iu < nou; iu++)
+        for (int ju = 0; ju < nvu; ju++)
+            for (int id = 0; id < nod; id++)
+                for (int jd = 0; jd < nvd; jd++) {
+                    const int ru = occU[iu], qu = virU[ju], rd = occD[id], qd = virD[jd];
+                    const double t = two_body(rd, ru, qd, qu, n, a.g, m);
+                    const uint32_t ib = jb ^ (1u << ru) ^ (1u << qu) ^ (1u << rd) ^ (1u << qd);
+                    const long long ci = (long long)a.rkA[ib >> n] * a.nb + a.rkB[ib & ((1u << n) - 1)];
+                    col[e * N] = (int32_t)ci;
+                    val[e * N] = signed_by(jb, m, t);
+                    e++;
+                }
+}
+
+// ------------------------------------------------------------- Hessian
+// Forward tangent step for operator k.  psi, lambda are rotated out of place
+// (every grid.y slice recomputes them in registers; slice 0 writes them),
+// each tangent accumulator B_mu (mu < nactive) in place; B_{injectMu} gets
+// scale_k * G_k psi'; per block: X[mu] = <lambda'|G_k|B'_mu> and
+// D = <lambda'|G_k G_k|psi'> = -sum_pairs (lambda'_b psi'_b + lambda'_b' psi'_b').
+template <int MU>
+__global__ void __launch_bounds__(kBlock)
+    k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut, const double *__restrict__ lamIn,
+              double *__restrict__ lamOut, double *__restrict__ B, long long bstride, int nactive, long long N,
+              DevSector sec, const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, int injectMu,
+              double scalek, double *__restrict__ xpart, int G, double *__restrict__ dpart)
+{
+    __shared__ double red[kBlock / 64];
+    const DevOp o = ops[k];
+    const double C = cs[k].x, S = cs[k].y;
+    const int mu0 = blockIdx.y * MU;
+    const bool lead = blockIdx.y == 0;
+    double acc[MU];
+#pragma unroll
+    for (int t = 0; t < MU; t++) acc[t] = 0;
+    double dacc = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < N; i += (long long)G * kBlock) {
+        uint32_t b2;
+        const uint32_t b = sec.states[i];
+        const int s = pair_sign(o, b, b2);
+        long long j = -1;
+        if (s != 0) {
+            if (b2 < b) continue;
+            j = sec.rank(b2);
+        }
+        if (j < 0) {
+            if (lead) {
+                psiOut[i] = psiIn[i];
+                lamOut[i] = lamIn[i];
+            }
+            continue;
+        }
+        const double pi = psiIn[i], pj = psiIn[j], li = lamIn[i], lj = lamIn[j];
+        const double pi2 = s * pj * S + pi * C, pj2 = -s * pi * S + pj * C;
+        const double li2 = s * lj * S + li * C, lj2 = -s * li * S + lj * C;
+        if (lead) {
+            psiOut[i] = pi2;
+            psiOut[j] = pj2;
+            lamOut[i] = li2;
+            lamOut[j] = lj2;
+            dacc -= li2 * pi2 + lj2 * pj2;
+        }
+#pragma unroll
+        for (int t = 0; t < MU; t++) {
+            const int mu = mu0 + t;
+            if (mu >= nactive) break;
+            double *Bm = B + mu * bstride;
+            const double bi = Bm[i], bj = Bm[j];
+            double bi2 = s * bj * S + bi * C, bj2 = -s * bi * S + bj * C;
+            if (mu == injectMu) {
+                bi2 += scalek * s * pj2;
+                bj2 -= scalek * s * pi2;
+            }
+            Bm[i] = bi2;
+            Bm[j] = bj2;
+            acc[t] += s * (li2 * bj2 - lj2 * bi2);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < MU; t++) {
+        const double tot = block_sum(acc[t], red);
+        if (threadIdx.x == 0 && mu0 + t < nactive) xpart[(long long)(mu0 + t) * G + blockIdx.x] = tot;
+    }
+    if (lead) {
+        const double tot = block_sum(dacc, red);
+        if (threadIdx.x == 0) dpart[blockIdx.x] = tot;
+    }
+}
+
+template __global__ void k_tangent<4>(const double *, double *, const double *, double *, double *, long long, int,
+                                      long long, DevSector, const DevOp *, int, const double2 *, int, double,
+                                      double *, int, double *);
+
+__global__ void __launch_bounds__(kBlock)
+    k_gram(const double *__restrict__ A, const double *__restrict__ B, long long stride, int P, long long N,
+           int chunks, double *__restrict__ part)
+{
+    __shared__ double sa[16][65];
+    __shared__ double sb[16][65];
+    const int tm = blockIdx.x * 16, tn = blockIdx.y * 16, chunk = blockIdx.z;
+    const long long lo = N * chunk / chunks, hi = N * (chunk + 1) / chunks;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc = 0;
+    for (long long base = lo; base < hi; base += 64) {
+        for (int t = threadIdx.x; t < 16 * 64; t += kBlock) {
+            const int r = t >> 6, c = t & 63;
+            const long long i = base + c;
+            sa[r][c] = (tm + r < P && i < hi) ? A[(tm + r) * stride + i] : 0.0;
+            sb[r][c] = (tn + r < P && i < hi) ? B[(tn + r) * stride + i] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int c = 0; c < 64; c++) acc += sa[ty][c] * sb[tx][c];
+        __syncthreads();
+    }
+    if (tm + ty < P && tn + tx < P) part[((long long)chunk * P + tm + ty) * P + tn + tx] = acc;
+}
+
+} // namespace ae

@@ -1,0 +1,169 @@
+// HIP kernels of the statevector engine (gfx950, wave64, FP64 on VALU).
+//
+// Data layout in HBM (S_z sector, n spatial orbitals, k_up / k_down electrons):
+//   sector index i = rank_up(alpha) * N_down + rank_down(beta), i.e. the
+//   reference's ascending full-index order (src/operatorpool.cpp:207-242),
+//   alpha-major; a state vector is N = N_up * N_down doubles, each alpha
+//   string owns one contiguous row of N_down amplitudes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace ae {
+
+constexpr int kBlock = 256;
+constexpr int kMaxTileRows = 64;
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum (fixed tree); result valid in thread 0.  `red`
+// holds kBlock/64 doubles; ends with a barrier so `red` can be reused.
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; i++) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+// Pair rule of one operator on bit string b (src/fusedevolve.cpp:27-145).
+// Returns 0 if the operator does not act on b, else the phase s = +-1 and
+// the partner in b2.
+__device__ __forceinline__ int pair_sign(const DevOp &o, uint32_t b, uint32_t &b2)
+{
+    const uint32_t m = b & (o.cre | o.ann);
+    int s;
+    if (m == o.ann) s = 1;
+    else if (m == o.cre) s = -1;
+    else return 0;
+    b2 = b ^ o.cre ^ o.ann;
+    if (__popc(b & o.mask) & 1) s = -s;
+    return s * o.flip;
+}
+
+// Sector rank in any of the three sector kinds.
+struct DevSector {
+    int mode;  // SectorMode
+    int n;     // nq / 2
+    int nel;   // particle number (kNumber)
+    int nb;    // N_down (kSz)
+    uint32_t maskB;
+    const uint32_t *states;  // N bit strings, ascending
+    const int32_t *rkA;      // 2^n -> rank among alpha strings (-1 if wrong count)
+    const int32_t *rkB;      // 2^n -> rank among beta strings
+    const uint32_t *binom;   // 33 x 33, binom[p*33+t] = C(p, t)
+
+    __device__ __forceinline__ long long rank(uint32_t b) const
+    {
+        if (mode == kSz) {
+            const int a = rkA[b >> n], c = rkB[b & maskB];
+            return (a < 0 || c < 0) ? -1 : (long long)a * nb + c;
+        }
+        if (mode == kNumber) {
+            if (__popc(b) != nel) return -1;
+            // colex rank = ascending-integer rank among strings of equal popcount
+            long long r = 0;
+            int t = 1;
+            while (b) {
+                const int p = __ffs(b) - 1;
+                r += binom[p * 33 + t];
+                t++;
+                b &= b - 1;
+            }
+            return r;
+        }
+        return (long long)b;
+    }
+};
+
+// Tiles of one fused segment: alpha strings that agree outside the alpha
+// bits U touched by the segment form one tile; the segment's operators map
+// every (alpha, beta) of a tile to the same tile.
+struct SegArgs {
+    int n, nb;
+    uint32_t maskB;
+    const uint32_t *unA, *unB;
+    const int32_t *rkA, *rkB;
+    const int32_t *tileStart;  // ntiles + 1
+    const int32_t *rows;       // alpha ranks, grouped by tile
+    const int8_t *localRow;    // alpha rank -> row within its tile
+    const DevOp *ops;
+    int k0, k1;
+    int maxRows;
+    int reverse;  // 1: apply k1-1 .. k0 (adjoint sweep), 0: k0 .. k1-1
+};
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_fill(double *__restrict__ p, long long n, double v);
+
+// One operator on whole vectors (any sector); BR = also rotate v1 and
+// accumulate the adjoint bracket s*(v1_b v0_b' - v1_b' v0_b) before the
+// rotation, one partial per block.  grid.y = independent vector batch.
+template <bool BR>
+__global__ void k_op_generic(double *__restrict__ v0, double *__restrict__ v1, long long vstride, long long N,
+                             DevSector sec, const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs,
+                             long long csStride, double *__restrict__ partial, int slots, long long pstride);
+
+// A fused segment of operators on one tile held in LDS (S_z sector).
+template <int NV, bool BR>
+__global__ void k_seg(SegArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+                      const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+                      long long pstride);
+
+// y = M^T x with uniform-width rows stored entry-major (col/val[e*N + j]);
+// V vectors per thread share each (col, val) load; per-block partials of
+// sum_j x_j y_j for the energy.
+template <int V>
+__global__ void k_ell(const int32_t *__restrict__ col, const double *__restrict__ val, int W, long long N,
+                      const double *__restrict__ x, double *__restrict__ y, long long vstride,
+                      double *__restrict__ epart, int eslots);
+
+// y = M^T x with variable rows (rowptr/col/val), same energy partials.
+__global__ void k_rows(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                       const double *__restrict__ val, long long N, const double *__restrict__ x,
+                       double *__restrict__ y, long long vstride, double *__restrict__ epart, int eslots);
+
+// out[r] = scale * sum_c part[r*rstride + c*cstride], c < ncols; block per row.
+__global__ void k_reduce(const double *__restrict__ part, int nrows, int ncols, long long rstride,
+                         long long cstride, double *__restrict__ out, double scale);
+
+// Uniform-width rows of the S_z-sector Hamiltonian from integrals.
+struct EllBuildArgs {
+    int n, ku, kd, nb;
+    long long N;
+    int W;
+    const uint32_t *unA, *unB;
+    const int32_t *rkA, *rkB;
+    const double *h, *g;
+    int32_t *col;
+    double *val;
+};
+__global__ void k_ell_build(EllBuildArgs a);
+
+// Hessian tangent step for operator k (see Engine::hessian).
+template <int MU>
+__global__ void k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut,
+                          const double *__restrict__ lamIn, double *__restrict__ lamOut, double *__restrict__ B,
+                          long long bstride, int nactive, long long N, DevSector sec,
+                          const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, int injectMu,
+                          double scalek, double *__restrict__ xpart, int G, double *__restrict__ dpart);
+
+// part[chunk][i][j] = sum over the chunk of A[i][.] * B[j][.]
+__global__ void k_gram(const double *__restrict__ A, const double *__restrict__ B, long long stride, int P,
+                       long long N, int chunks, double *__restrict__ part);
+
+} // namespace ae

@@ -1,0 +1,138 @@
+/* AnsatzEvolve statevector engine for MI355X (gfx950) -- C ABI.
+ *
+ * Part 1 is a drop-in for the reference's C interface
+ * (apu727/AnsatzEvolve src/Generated/AnsatzSynthInterface.h:14-26, generated
+ * from src/AnsatzSynthInterface.f90 by gfortran -fc-prototypes): same names,
+ * same signatures, same argument meaning, same return codes.  Existing C and
+ * Fortran callers (the bind(C) module in src/AnsatzSynthInterface.f90) link
+ * libansatzsynth.so instead of libAnsatzSynthInterface.a +
+ * libcppAnsatzSynthLib.a and need no source change.
+ *
+ * Part 2 adds entry points the reference's C interface lacks (integral
+ * Hamiltonians, nuclear energy, fused and batched evaluations).  They follow
+ * the same conventions: opaque ctx from init(), caller-owned host arrays that
+ * are copied, int status codes, no exceptions, one lock per ctx.
+ *
+ * Conventions (identical to the reference):
+ *  - basis index = bit string + 1 (one-based); bit q = spin-orbital q; bits
+ *    0..n-1 spin-down, n..2n-1 spin-up;
+ *  - operators[nparams*j + i] = one-based component j of operator i (0 = absent),
+ *    (a,b,c,d) = a^+_a a^+_b a_c a_d, single (a,b,0,0) = a^+_a a_b;
+ *  - angles may be given per operator (length = number of operators) or per
+ *    parameter (length = max(orderfile)) wherever the reference accepts both;
+ *  - gradients and Hessians are per parameter; Hessians are column-major,
+ *    hessian[nangles*j + i] = d2E / dtheta_i dtheta_j;
+ *  - energies exclude nuclear repulsion unless setNuclearEnergy was called.
+ */
+#ifndef ANSATZSYNTH_H
+#define ANSATZSYNTH_H
+
+#include <stddef.h>
+#ifdef __cplusplus
+#include <complex>
+#define ANSATZ_DOUBLE_COMPLEX std::complex<double>
+extern "C" {
+#else
+#define ANSATZ_DOUBLE_COMPLEX double _Complex
+#endif
+
+/* ---------------- Part 1: the reference C interface ---------------- */
+
+/* ctx is the address of the handle (void**), as Fortran passes c_ptr by
+ * reference; sets *ctx = NULL.  0 ok (also when *ctx is already NULL),
+ * 1 ctx == NULL.  Replaces src/AnsatzSynthInterface.cpp:42-58. */
+int cleanup(void *ctx);
+
+/* 1 ctx NULL, 2 angles NULL, 3 nangles < 1 or evaluation failed.
+ * Replaces src/AnsatzSynthInterface.cpp:313-344. */
+int getEnergy(int nangles, const double *angles, double *energy, void *ctx);
+
+/* Dense full-Fock final state (index = bit string).  1 ctx NULL, 2 angles
+ * NULL, 3 finalstate NULL, 4 nbasisvectors < 1, 5 nangles < 1, 6 failure,
+ * 7 nbasisvectors != 2^numqubits.  Replaces src/AnsatzSynthInterface.cpp:362-420. */
+int getFinalState(int nangles, const double *angles, int nbasisvectors, double *finalstate, void *ctx);
+
+/* Real build: logs and returns 4 (src/AnsatzSynthInterface.cpp:487-497). */
+int getFinalStateComplex(int nangles, const double *angles, int nbasisvectors,
+                         ANSATZ_DOUBLE_COMPLEX *finalstate, void *ctx);
+
+/* Per-parameter gradient.  1 ctx NULL, 2 angles NULL, 3 gradient NULL,
+ * 4 nangles < 1, 5 failure, -6 nangles != number of parameters.
+ * Replaces src/AnsatzSynthInterface.cpp:519-562. */
+int getGradient_COMP(int nangles, const double *angles, double *gradient, void *ctx);
+
+/* Per-parameter Hessian, column-major.  Codes as getGradient_COMP.
+ * Replaces src/AnsatzSynthInterface.cpp:578-629. */
+int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx);
+
+/* New context; free with cleanup(&ctx).  Replaces src/AnsatzSynthInterface.cpp:28-33. */
+void *init(void);
+
+/* = setExcitationScale(..., NULL, ctx).  Replaces src/AnsatzSynthInterface.cpp:64-67. */
+int setExcitation(int nparams, const int *operators, const int *orderfile, void *ctx);
+
+/* 1 ctx NULL, 2 nparams < 1, 3 operator index - 1 > 127, 4 operators not
+ * stored (e.g. after the first evaluation), 5 orderfile entry > nparams,
+ * 6 parameter map not stored.  Replaces src/AnsatzSynthInterface.cpp:84-140. */
+int setExcitationScale(int nparams, const int *operators, const int *orderfile, const double *scalefactor,
+                       void *ctx);
+
+/* One-based full-Fock triplets (i, j, v), duplicates summed; the operator is
+ * applied as y = M^T x.  1 ctx NULL, 2 n < 1, 3 not stored.
+ * Replaces src/AnsatzSynthInterface.cpp:154-187. */
+int setHamiltonian(int n, const int *iindexes, const int *jindexes, const double *coeffs, void *ctx);
+
+/* One-based basis indices.  1 ctx NULL, 2 n < 1 or numqubits < 1,
+ * 3 not stored.  Replaces src/AnsatzSynthInterface.cpp:201-236. */
+int setInitialState(int numqubits, int n, const int *iindexes, const double *coeffs, void *ctx);
+
+/* Real build: returns 4 (src/AnsatzSynthInterface.cpp:286-298). */
+int setInitialStateComplex(int numqubits, int n, const int *iindexes, const ANSATZ_DOUBLE_COMPLEX *coeffs,
+                           void *ctx);
+
+/* Global: non-zero logs every later call's arguments to stderr.
+ * Replaces src/AnsatzSynthInterface.cpp:18-21. */
+void setTraceInterfaceCalls(int val);
+
+/* ---------------- Part 2: extensions ---------------- */
+
+/* Molecular-orbital integrals instead of triplets: h[norb*norb] row-major,
+ * g[norb^4] in the layout of the reference's _twoEInts.bin
+ * (src/hamiltonianmatrix.cpp:402-554; the reference reads them from files
+ * via storeRunPath, src/AnsatzManager.h:177).  numqubits must be 2*norb.
+ * 1 ctx NULL, 2 norb < 1 or h/g NULL, 3 not stored (after construction). */
+int setHamiltonianIntegrals(int norb, const double *h, const double *g, void *ctx);
+
+/* Nuclear repulsion added to every energy (stateAnsatzManager::storeNuclearEnergy,
+ * src/AnsatzManager.cpp:400-413).  1 ctx NULL, 3 not stored. */
+int setNuclearEnergy(double energy, void *ctx);
+
+/* How an integral Hamiltonian is applied: 0 automatic (explicit sparse matrix
+ * when it fits in device memory, else matrix-free), 1 explicit, 2 matrix-free.
+ * 1 ctx NULL, 2 bad mode, 3 after construction. */
+int setHamiltonianStorage(int mode, void *ctx);
+
+/* One evaluation of energy and per-parameter gradient at the same angles
+ * (FusedEvolve::evolve + evolveDerivative(&E), src/fusedevolve.cpp:2105-2159,
+ * 2263-2346).  Codes as getGradient_COMP; energy may be NULL. */
+int getEnergyGradient(int nangles, const double *angles, double *energy, double *gradient, void *ctx);
+
+/* Batched energies for nvec angle vectors (row-major nvec x nangles), as the
+ * reference's stateAnsatzManager::getExpectationValues
+ * (src/AnsatzManager.cpp:709-754).  1 ctx NULL, 2 angles/energies NULL,
+ * 3 nvec < 1 or nangles < 1, 5 failure. */
+int getEnergies(int nvec, int nangles, const double *angles, double *energies, void *ctx);
+
+/* Batched energy + gradient, nvec vectors; gradients row-major nvec x nangles.
+ * Codes as getEnergies, -6 nangles != number of parameters. */
+int getEnergyGradients(int nvec, int nangles, const double *angles, double *energies, double *gradients,
+                       void *ctx);
+
+/* Sector dimension, number of operators and parameters after construction
+ * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
+int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANSATZSYNTH_H */

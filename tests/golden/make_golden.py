@@ -1,0 +1,128 @@
+"""Writes reference_vectors.npz: energies, gradients, Hessians and states
+computed by the reference's own CPU build (oracle/_ref/libref_ansatz.so,
+built from the reference sources by `make -C oracle ref`).
+
+Inputs are regenerated from ansatzevolve_amd.workloads (seeded); each
+integral set is stored with a checksum so a changed generator is caught.
+Run in the build container:  python tests/golden/make_golden.py [--big]
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from ansatzevolve_amd.workloads import (hf_index, make_workload, seeded_angles, synthetic_integrals,  # noqa: E402
+                                        tups_operators)
+from oracle.oracle import RefEngine, RefFusedEvolve  # noqa: E402
+
+OUT = os.path.join(HERE, "reference_vectors.npz")
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()[:16]
+
+
+def integral_case(n, layers, seed, nsets, hessian=True):
+    h, g = synthetic_integrals(n, seed)
+    ops, order, scale = tups_operators(n, layers)
+    P = int(order.max()) + 1
+    thetas = seeded_angles(P, seed + 7, count=nsets)
+    r = RefEngine(2 * n, [hf_index(n)], [1.0], ops, order, scale, h=h, g=g)
+    E, G, H = [], [], []
+    for th in thetas:
+        E.append(r.energy(th))
+        G.append(r.gradient(th))
+        if hessian:
+            H.append(r.hessian(th))
+    r.close()
+    out = {"theta": thetas, "E": np.array(E), "grad": np.array(G), "digest": digest(h, g)}
+    if hessian:
+        out["hess"] = np.array(H)
+    return out
+
+
+def main():
+    big = "--big" in sys.argv
+    data = {}
+    meta = {}
+    # test.F90 set-up (triplet Hamiltonian), two KAT angle sets + two random
+    with open(os.path.join(HERE, "testf90_kat.json")) as f:
+        k = json.load(f)
+    ops = np.array(k["operators_1based"]) - 1
+    order = np.array(k["order_1based"]) - 1
+    trip = (np.array(k["ham_i_1based"]) - 1, np.array(k["ham_j_1based"]) - 1, np.array(k["ham_coeffs"]))
+    r = RefEngine(8, [k["initial_index_1based"] - 1], [1.0], ops, order, triplets=trip)
+    thetas = [np.array(k["angles_hf"]), np.array(k["angles_min"])] + list(seeded_angles(9, 4242, count=2))
+    data["kat_theta"] = np.array(thetas)
+    data["kat_E"] = np.array([r.energy(t) for t in thetas])
+    data["kat_grad"] = np.array([r.gradient(t) for t in thetas])
+    data["kat_hess"] = np.array([r.hessian(t) for t in thetas])
+    data["kat_state"] = np.array([r.state(t, 256) for t in thetas])
+    r.close()
+    # integral Hamiltonians, TUPS, HF start
+    cases = [("n4L2", 4, 2, 1001, 3, True), ("n6L2", 6, 2, 1006, 2, True), ("n8L2", 8, 2, 1008, 2, True),
+             ("n6L3", 6, 3, 1016, 2, True)]
+    for name, n, layers, seed, nsets, hess in cases:
+        t0 = time.time()
+        c = integral_case(n, layers, seed, nsets, hess)
+        for key, v in c.items():
+            if key == "digest":
+                meta[name] = {"n": n, "layers": layers, "seed": seed, "digest": v}
+            else:
+                data[name + "_" + key] = v
+        print(name, "%.1fs" % (time.time() - t0), file=sys.stderr)
+    # the 10-orbital workload (63,504 determinants)
+    t0 = time.time()
+    w = make_workload(2)
+    r = RefEngine(w["nq"], [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"], h=w["h"], g=w["g"])
+    th2 = np.stack([w["theta"], seeded_angles(w["P"], w["seed"] + 2)])
+    data["cfg2_theta"] = th2
+    data["cfg2_E"] = np.array([r.energy(t) for t in th2])
+    data["cfg2_grad"] = np.array([r.gradient(t) for t in th2])
+    data["cfg2_hess"] = r.hessian(th2[0])[None]
+    r.close()
+    meta["cfg2"] = {"config": 2, "seed": w["seed"], "digest": digest(w["h"], w["g"])}
+    print("cfg2 %.1fs" % (time.time() - t0), file=sys.stderr)
+    if big:
+        # the 12-orbital workload (853,776 determinants): the reference's
+        # matrix-free path, FusedEvolve built as its benchmark mode does
+        t0 = time.time()
+        w = make_workload(3)
+        fe = RefFusedEvolve(w["nq"], w["nup"], w["ndown"], [w["start_idx"]], [1.0], w["ops"], w["order"],
+                            w["scale"], w["h"], w["g"])
+        x = w["scale"] * w["theta"][w["order"]]
+        E, g, t = fe.egrad(x, reps=1)
+        data["cfg3_theta"] = w["theta"][None]
+        data["cfg3_E"] = np.array([E])
+        data["cfg3_grad"] = g[None]
+        meta["cfg3"] = {"config": 3, "seed": w["seed"], "digest": digest(w["h"], w["g"]),
+                        "ref_egrad_seconds": t, "ref_threads": fe.threads()}
+        fe.close()
+        print("cfg3 %.1fs (egrad %.2fs)" % (time.time() - t0, t), file=sys.stderr)
+    data["meta_json"] = np.array(json.dumps(meta))
+    if os.path.exists(OUT) and not big:
+        # keep previously generated large cases
+        old = np.load(OUT, allow_pickle=False)
+        for key in old.files:
+            if key.startswith("cfg3") and key not in data:
+                data[key] = old[key]
+        om = json.loads(str(old["meta_json"]))
+        if "cfg3" in om:
+            meta["cfg3"] = om["cfg3"]
+            data["meta_json"] = np.array(json.dumps(meta))
+    np.savez_compressed(OUT, **data)
+    print("wrote", OUT, file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the
+header declares, and the argument checks return the reference's codes
+(src/AnsatzSynthInterface.cpp).  Nothing here launches a kernel."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ansatzevolve_amd as ae
+from conftest import gpu_present
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ansatzsynth.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_]\w*)\s*\(", text)) - {"extern", "defined"})
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return ae.load_library()
+
+
+def test_header_declares_reference_interface():
+    syms = header_symbols()
+    for s in ae.REFERENCE_SYMBOLS:
+        assert s in syms
+    assert set(syms) == {n for n, _, _ in ae.C_API}
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", ae.LIBPATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for s in header_symbols():
+        assert s in exported, s
+        assert getattr(lib, s) is not None
+
+
+def _ctx(lib):
+    return ctypes.c_void_p(lib.init())
+
+
+def test_cleanup_codes(lib):
+    assert lib.cleanup(None) == 1
+    null = ctypes.c_void_p(None)
+    assert lib.cleanup(ctypes.byref(null)) == 0
+    c = _ctx(lib)
+    assert c.value
+    assert lib.cleanup(ctypes.byref(c)) == 0
+    assert c.value is None
+    assert lib.cleanup(ctypes.byref(c)) == 0
+
+
+def _ip(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dp(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def test_set_excitation_codes(lib):
+    c = _ctx(lib)
+    ops, op = _ip([1, 2, 0, 0])
+    order, orp = _ip([1])
+    assert lib.setExcitation(1, op, orp, None) == 1
+    assert lib.setExcitation(0, op, orp, c) == 2
+    big, bp = _ip([200, 2, 0, 0])
+    assert lib.setExcitation(1, bp, orp, c) == 3
+    o2, o2p = _ip([2])
+    assert lib.setExcitation(1, op, o2p, c) == 5
+    assert lib.setExcitation(1, op, orp, c) == 0
+    sc, scp = _dp([0.5])
+    assert lib.setExcitationScale(1, op, orp, scp, c) == 0
+    lib.cleanup(ctypes.byref(c))
+
+
+def test_set_hamiltonian_and_initial_codes(lib):
+    c = _ctx(lib)
+    i, ip = _ip([1])
+    v, vp = _dp([1.0])
+    assert lib.setHamiltonian(1, ip, ip, vp, None) == 1
+    assert lib.setHamiltonian(0, ip, ip, vp, c) == 2
+    assert lib.setHamiltonian(1, ip, ip, vp, c) == 0
+    assert lib.setInitialState(4, 1, ip, vp, None) == 1
+    assert lib.setInitialState(4, 0, ip, vp, c) == 2
+    assert lib.setInitialState(0, 1, ip, vp, c) == 2
+    assert lib.setInitialState(4, 1, ip, vp, c) == 0
+    assert lib.setInitialStateComplex(4, 1, ip, None, c) == 4
+    assert lib.setNuclearEnergy(1.0, None) == 1
+    assert lib.setNuclearEnergy(1.0, c) == 0
+    assert lib.setHamiltonianStorage(7, c) == 2
+    assert lib.setHamiltonianIntegrals(0, vp, vp, c) == 2
+    lib.cleanup(ctypes.byref(c))
+
+
+def test_getter_argument_codes(lib):
+    c = _ctx(lib)
+    a, ap = _dp([0.0, 0.0])
+    out, outp = _dp(np.zeros(16))
+    E = ctypes.c_double()
+    assert lib.getEnergy(2, ap, ctypes.byref(E), None) == 1
+    assert lib.getEnergy(2, None, ctypes.byref(E), c) == 2
+    assert lib.getEnergy(0, ap, ctypes.byref(E), c) == 3
+    assert lib.getFinalState(2, ap, 16, outp, None) == 1
+    assert lib.getFinalState(2, None, 16, outp, c) == 2
+    assert lib.getFinalState(2, ap, 16, None, c) == 3
+    assert lib.getFinalState(2, ap, 0, outp, c) == 4
+    assert lib.getFinalState(0, ap, 16, outp, c) == 5
+    assert lib.getFinalStateComplex(2, ap, 16, None, c) == 4
+    for fn in (lib.getGradient_COMP, lib.getHessian_COMP):
+        assert fn(2, ap, outp, None) == 1
+        assert fn(2, None, outp, c) == 2
+        assert fn(2, ap, None, c) == 3
+        assert fn(0, ap, outp, c) == 4
+    # nothing set up yet: construction fails before any device work
+    assert lib.getEnergy(2, ap, ctypes.byref(E), c) == 3
+    assert lib.getGradient_COMP(2, ap, outp, c) == 5
+    assert lib.getHessian_COMP(2, ap, outp, c) == 5
+    assert lib.getFinalState(2, ap, 16, outp, c) == 6
+    lib.cleanup(ctypes.byref(c))
+
+
+@pytest.mark.skipif(gpu_present(), reason="checks behaviour on a host without a GPU")
+def test_no_cpu_fallback_without_gpu(lib):
+    """A complete set-up still fails (code 3) on a host without a GPU:
+    the library never computes on the CPU."""
+    with ae.AnsatzSynth() as a:
+        a.set_initial_state(4, [0b0101 + 1], [1.0])
+        a.set_hamiltonian([6], [6], [1.0])
+        a.set_excitation([[1, 2, 0, 0]], [1])
+        with pytest.raises(ae.AnsatzError) as ei:
+            a.energy([0.1])
+        assert ei.value.code == 3

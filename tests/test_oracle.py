@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check is chosen so that a plausible mistake fails it: the reference's
+own known answers (test.F90), golden vectors written by the reference's CPU
+build, an independent Jordan-Wigner construction of every generator and of
+the Hamiltonian, exact diagonalisation, finite differences, unitarity and
+closed forms.
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from ansatzevolve_amd.workloads import (hf_index, make_workload, sector_dim, seeded_angles, synthetic_integrals,
+                                        tups_operators)
+from oracle.oracle import OracleEngine, apply_exc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def kat():
+    with open(os.path.join(GOLD, "testf90_kat.json")) as f:
+        return json.load(f)
+
+
+def golden():
+    return np.load(os.path.join(GOLD, "reference_vectors.npz"), allow_pickle=False)
+
+
+def kat_oracle():
+    k = kat()
+    o = OracleEngine(8, [k["initial_index_1based"] - 1], [1.0], np.array(k["operators_1based"]) - 1,
+                     np.array(k["order_1based"]) - 1)
+    o.set_triplets(np.array(k["ham_i_1based"]) - 1, np.array(k["ham_j_1based"]) - 1, k["ham_coeffs"])
+    return k, o
+
+
+def integral_oracle(n, layers, seed, explicit=True):
+    h, g = synthetic_integrals(n, seed)
+    ops, order, scale = tups_operators(n, layers)
+    o = OracleEngine(2 * n, [hf_index(n)], [1.0], ops, order, scale)
+    o.set_integrals(h, g, explicit=explicit)
+    return o, h, g
+
+
+# ------------------------------------------------------------ known answers
+def test_testf90_known_answers():
+    """src/test.F90:67-78, 98-109."""
+    k, o = kat_oracle()
+    assert o.mode == 2 and o.N == 36 and o.P == 9
+    E, g = o.energy_gradient(k["angles_hf"])
+    assert abs(E - k["energy_hf"]) < 1e-12
+    assert abs(np.linalg.norm(g) - k["grad_norm_hf"]) < 1e-12
+    E, g = o.energy_gradient(k["angles_min"])
+    assert abs(E - k["energy_min"]) < 1e-12
+    assert np.linalg.norm(g) < 1e-10  # stationary at the reference's minimum
+
+
+def test_against_reference_build_kat():
+    d = golden()
+    k, o = kat_oracle()
+    for t, E, g, H, st in zip(d["kat_theta"], d["kat_E"], d["kat_grad"], d["kat_hess"], d["kat_state"]):
+        Hh, gh, Eh = o.hessian(t)
+        assert abs(o.energy(t) - E) < 1e-12
+        assert np.max(np.abs(o.energy_gradient(t)[1] - g)) < 1e-12
+        assert np.max(np.abs(Hh - H)) < 1e-11
+        assert abs(Eh - E) < 1e-12 and np.max(np.abs(gh - g)) < 1e-12
+        full = np.zeros(256)
+        full[o.states()] = o.evolve(t)
+        assert np.max(np.abs(full - st)) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["n4L2", "n6L2", "n8L2", "n6L3"])
+def test_against_reference_build_integrals(name):
+    d = golden()
+    meta = json.loads(str(d["meta_json"]))[name]
+    o, h, g = integral_oracle(meta["n"], meta["layers"], meta["seed"])
+    for i, t in enumerate(d[name + "_theta"]):
+        E, gr = o.energy_gradient(t)
+        assert abs(E - d[name + "_E"][i]) < 1e-11
+        assert np.max(np.abs(gr - d[name + "_grad"][i])) < 1e-11
+        H, _, _ = o.hessian(t)
+        assert np.max(np.abs(H - d[name + "_hess"][i])) < 1e-10
+
+
+def test_against_reference_build_config2_matrix_free():
+    """63,504 determinants, the oracle's matrix-free H against the reference."""
+    d = golden()
+    w = make_workload(2)
+    o = OracleEngine(w["nq"], [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
+    o.set_integrals(w["h"], w["g"], explicit=False)
+    t = d["cfg2_theta"][0]
+    E, g = o.energy_gradient(t)
+    assert abs(E - d["cfg2_E"][0]) < 1e-10
+    assert np.max(np.abs(g - d["cfg2_grad"][0])) < 1e-10
+
+
+# ------------------------------------------------------------ independent constructions
+def _jw(seq, b):
+    """Apply ladder operators (rightmost first) with Jordan-Wigner signs."""
+    sign = 1
+    for kind, p in seq:
+        occ = (b >> p) & 1
+        if (bin(b & ((1 << p) - 1)).count("1") & 1):
+            sign = -sign
+        if kind == "a":
+            if not occ:
+                return None, 0
+            b ^= 1 << p
+        else:
+            if occ:
+                return None, 0
+            b |= 1 << p
+    return b, sign
+
+
+def test_generators_match_jordan_wigner():
+    """Every pair rule equals -(T - T^+) for singles T = a+_a a_b and
+    +(T - T^+) for doubles T = a+_a a+_b a_c a_d, T built independently."""
+    rng = np.random.default_rng(3)
+    nq = 8
+    seen = {True: set(), False: set()}
+    for _ in range(120):
+        single = rng.random() < 0.5
+        if single:
+            a, b = rng.choice(nq, 2, replace=False)
+            op, seq = [a, b, -1, -1], [("a", b), ("c", a)]
+        else:
+            a, b, c, dd = rng.choice(nq, 4, replace=False)
+            op, seq = [a, b, c, dd], [("a", dd), ("a", c), ("c", b), ("c", a)]
+        for s0 in range(1 << nq):
+            s1, sg = apply_exc(s0, op)
+            if s1 == s0:
+                continue
+            val = 0
+            t1, g1 = _jw(seq, s1)
+            if t1 == s0:
+                val += g1
+            t2, g2 = _jw(seq, s0)
+            if t2 == s1:
+                val -= g2
+            seen[single].add(sg * val)
+    assert seen[True] == {-1} and seen[False] == {1}
+
+
+def test_manual_rotation_sign():
+    """Manual.md:180-190: with T = 3 1 0 0 on |0011>, the coefficient of
+    |0011> becomes cos(t) a - sin(t) c."""
+    partner, s = apply_exc(0b0011, [2, 0, -1, -1])
+    assert partner != 0b0011 and s == -1
+
+
+def test_hamiltonian_matches_second_quantisation():
+    """Sector H element-wise against H = sum h a+a + 1/2 sum (pq|rs) a+a+aa."""
+    n, nq = 4, 8
+    o, h, g = integral_oracle(n, 2, 123)
+    H = np.zeros((1 << nq, 1 << nq))
+    for st in range(1 << nq):
+        for P in range(nq):
+            for Q in range(nq):
+                if P // n == Q // n:
+                    b, s = _jw([("a", Q), ("c", P)], st)
+                    if b is not None:
+                        H[b, st] += h[P % n, Q % n] * s
+        for P, Q, R, S in itertools.product(range(nq), repeat=4):
+            if P // n != Q // n or R // n != S // n:
+                continue
+            b, s = _jw([("a", Q), ("a", S), ("c", R), ("c", P)], st)
+            if b is not None:
+                H[b, st] += 0.5 * g[P % n, Q % n, R % n, S % n] * s
+    st = o.states()
+    Ho = np.array([o.ham_apply(np.eye(o.N)[i]) for i in range(o.N)]).T
+    assert np.max(np.abs(H[np.ix_(st, st)] - Ho)) < 1e-13
+
+
+def test_variational_bound_and_ed():
+    o, h, g = integral_oracle(4, 2, 55)
+    Ho = np.array([o.ham_apply(np.eye(o.N)[i]) for i in range(o.N)]).T
+    e0 = np.linalg.eigvalsh(Ho)[0]
+    for t in seeded_angles(o.P, 9, count=20):
+        assert o.energy(t) >= e0 - 1e-12
+
+
+def test_hf_energy_closed_form():
+    n = 6
+    o, h, g = integral_oracle(n, 2, 77)
+    occ = list(range(0, n, 2))
+    E = 2 * sum(h[p, p] for p in occ)
+    for p in occ:
+        for q in occ:
+            E += 2 * g[p, p, q, q] - g[p, q, q, p]
+    assert abs(o.energy(np.zeros(o.P)) - E) < 1e-12
+
+
+# ------------------------------------------------------------ derivatives
+def test_gradient_finite_difference():
+    o, h, g = integral_oracle(4, 2, 11)
+    t = seeded_angles(o.P, 12)
+    E, gr = o.energy_gradient(t)
+    eps = 1e-5
+    fd = np.array([(o.energy(t + eps * e) - o.energy(t - eps * e)) / (2 * eps) for e in np.eye(o.P)])
+    assert np.max(np.abs(fd - gr)) < 1e-8
+
+
+def test_hessian_finite_difference_and_symmetry():
+    o, h, g = integral_oracle(4, 2, 13)
+    t = seeded_angles(o.P, 14)
+    H, gr, E = o.hessian(t)
+    assert np.max(np.abs(H - H.T)) < 1e-12
+    eps = 1e-5
+    fd = np.array([(o.energy_gradient(t + eps * e)[1] - o.energy_gradient(t - eps * e)[1]) / (2 * eps)
+                   for e in np.eye(o.P)])
+    assert np.max(np.abs(fd - H)) < 1e-7
+    E2, g2 = o.energy_gradient(t)
+    assert abs(E - E2) < 1e-13 and np.max(np.abs(g2 - gr)) < 1e-12
+
+
+def test_unitarity():
+    o, h, g = integral_oracle(6, 2, 21)
+    for t in seeded_angles(o.P, 22, count=3):
+        assert abs(np.linalg.norm(o.evolve(t)) - 1) < 1e-13
+
+
+# ------------------------------------------------------------ sector
+@pytest.mark.parametrize("n", [2, 4, 6])
+def test_sector_enumeration(n):
+    o, h, g = integral_oracle(n, 1, 1)
+    st = o.states()
+    assert len(st) == sector_dim(n, n // 2, n // 2) == math.comb(n, n // 2) ** 2
+    assert np.all(np.diff(st.astype(np.int64)) > 0)
+    lo = (1 << n) - 1
+    assert all(bin(s & lo).count("1") == n // 2 and bin(s >> n).count("1") == n // 2 for s in st)
+
+
+def test_sector_choice_follows_symmetry():
+    """src/AnsatzManager.cpp:88-100: S_z breaking operators or mixed start
+    states fall back to particle-number or full space."""
+    ops = np.array([[0, 4, -1, -1]])  # moves a down electron to an up orbital
+    o = OracleEngine(8, [0b00010001], [1.0], ops, [0])
+    assert o.mode == 1 and o.N == math.comb(8, 2)
+    o = OracleEngine(8, [0b1, 0b11], [0.6, 0.8], np.array([[0, 1, -1, -1]]), [0])
+    assert o.mode == 0 and o.N == 256
