@@ -41,6 +41,11 @@ C_API = [
     ("getEnergies", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _vp]),
     ("getEnergyGradients", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _vp]),
     ("getDimensions", ctypes.c_int, [ctypes.POINTER(ctypes.c_longlong), _ip, _ip, _vp]),
+    ("uploadAngleBatch", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _vp]),
+    ("runAngleBatch", ctypes.c_int, [ctypes.c_int, _vp]),
+    ("downloadAngleBatch", ctypes.c_int, [_dp, _dp, _vp]),
+    ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
+    ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
 ]
 
 REFERENCE_SYMBOLS = [n for n, _, _ in C_API[:13]]
@@ -214,6 +219,34 @@ class AnsatzSynth:
                                                                     E.ctypes.data_as(_dp),
                                                                     G.ctypes.data_as(_dp), self.ctx))
         return E, G
+
+
+    # ---- device-resident batch
+    def upload_batch(self, angles2d):
+        A = np.ascontiguousarray(angles2d, dtype=np.float64)
+        self._batch_shape = A.shape
+        self._chk("uploadAngleBatch", self.lib.uploadAngleBatch(A.shape[0], A.shape[1], A.ctypes.data_as(_dp),
+                                                                self.ctx))
+
+    def run_batch(self, with_gradient=True):
+        self._chk("runAngleBatch", self.lib.runAngleBatch(1 if with_gradient else 0, self.ctx))
+
+    def download_batch(self, with_gradient=True):
+        nvec, na = self._batch_shape
+        E = np.zeros(nvec)
+        G = np.zeros((nvec, na)) if with_gradient else None
+        self._chk("downloadAngleBatch", self.lib.downloadAngleBatch(
+            E.ctypes.data_as(_dp), G.ctypes.data_as(_dp) if with_gradient else None, self.ctx))
+        return E, G
+
+    def set_kernel_timing(self, on):
+        self._chk("setKernelTiming", self.lib.setKernelTiming(1 if on else 0, self.ctx))
+
+    def kernel_timing(self, cls):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._chk("getKernelTiming", self.lib.getKernelTiming(cls, ctypes.byref(ms), ctypes.byref(n), self.ctx))
+        return ms.value, n.value
 
 
 def from_workload(w, storage=0):

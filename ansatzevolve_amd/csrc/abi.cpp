@@ -426,3 +426,87 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
 }
 
 } // extern "C"
+
+extern "C" {
+
+int uploadAngleBatch(int nvec, int nangles, const double *angles, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (nvec < 1 || nangles < 1) {
+        log_msg("nvec < 1 or NAngles < 1");
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    return m->uploadAngleBatch(nvec, nangles, angles) ? 0 : 5;
+}
+
+int runAngleBatch(int withgradient, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    return m->runAngleBatch(withgradient != 0) ? 0 : 5;
+}
+
+int downloadAngleBatch(double *energies, double *gradients, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (energies == nullptr) {
+        log_msg("nullptr energies passed");
+        return 2;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (gradients && m->batchAngleCount() != m->numberOfParameters()) {
+        log_msg("Gradients need per-parameter angles in the uploaded batch");
+        return -6;
+    }
+    return m->downloadAngleBatch(energies, gradients) ? 0 : 5;
+}
+
+int setKernelTiming(int on, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    return m->setKernelTiming(on != 0) ? 0 : 5;
+}
+
+int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (kernelclass < 0 || kernelclass > 2 || milliseconds == nullptr || launches == nullptr) {
+        log_msg("bad kernel class or NULL output");
+        return 2;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    double ms = 0;
+    long long n = 0;
+    if (!m->getKernelTiming(kernelclass, ms, n)) return 5;
+    *milliseconds = ms;
+    *launches = n;
+    return 0;
+}
+
+} // extern "C"

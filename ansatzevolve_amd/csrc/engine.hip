@@ -117,6 +117,7 @@ struct EngineImpl {
     long long nnz = 0;
     int W = 0;
     DBuf<int64_t> d_rowptr;
+    DBuf<double> d_h, d_g, d_diag, d_FA, d_FB;  // matrix-free tables
     DBuf<int32_t> d_col;
     DBuf<double> d_val;
 
@@ -125,7 +126,83 @@ struct EngineImpl {
     DBuf<double2> b_cs, b_csneg;
     DBuf<double> hz_B, hz_HB, hz_psi2, hz_lam2, hz_xpart, hz_dpart, hz_gram, hz_out, hz_epart;
 
-    int eslots() const { return (int)((N + kBlock - 1) / kBlock); }
+    int bnvec = 0;  // size of the uploaded angle batch
+
+    // HIP-event timing of kernel classes (see Engine::set_timing)
+    bool timingOn = false;
+    struct Mark {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> evpool;
+    double tms[3] = {0, 0, 0};
+    long long tcnt[3] = {0, 0, 0};
+
+    hipEvent_t take_event()
+    {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    int mark_begin(int cls)
+    {
+        if (!timingOn) return -1;
+        Mark m{cls, take_event(), take_event()};
+        (void)hipEventRecord(m.a, stream);
+        marks.push_back(m);
+        return (int)marks.size() - 1;
+    }
+    void mark_end(int id)
+    {
+        if (id >= 0) (void)hipEventRecord(marks[id].b, stream);
+    }
+    // after a stream synchronisation
+    void collect_marks()
+    {
+        for (auto &m : marks) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+                tms[m.cls] += ms;
+                tcnt[m.cls] += 1;
+            }
+            evpool.push_back(m.a);
+            evpool.push_back(m.b);
+        }
+        marks.clear();
+    }
+
+    // energy partials per vector: one per row block (rows) or per alpha
+    // string (matrix-free); buffers are sized for the larger of the two
+    int eslots() const { return hkind == 3 ? na : (int)((N + kBlock - 1) / kBlock); }
+    int eslots_alloc() const { return std::max(na, (int)((N + kBlock - 1) / kBlock)); }
+
+    MfArgs mfargs() const
+    {
+        MfArgs a;
+        a.n = n;
+        a.ku = nup;
+        a.kd = ndown;
+        a.na = na;
+        a.nb = nb;
+        a.N = N;
+        a.unA = d_unA.p;
+        a.unB = d_unB.p;
+        a.rkA = d_rkA.p;
+        a.rkB = d_rkB.p;
+        a.h = d_h.p;
+        a.g = d_g.p;
+        a.diag = d_diag.p;
+        a.FA = d_FA.p;
+        a.FB = d_FB.p;
+        return a;
+    }
+    size_t mf_lds() const { return (size_t)(nb + 2 * n * n) * sizeof(double) + ((size_t)2 << n); }
 
     DevSector sec() const
     {
@@ -238,6 +315,7 @@ struct EngineImpl {
     // Forward sweep of nvec vectors with per-vector angle tables.
     bool sweep_forward(double *v, int nvec, const double2 *cs, std::string &err)
     {
+        const int mk = mark_begin(1);
         for (const Seg &sg : segs) {
             if (sg.tileset >= 0) {
                 const TileSet &t = *tilesets[sg.tileset];
@@ -249,6 +327,7 @@ struct EngineImpl {
             }
             LCHK();
         }
+        mark_end(mk);
         return true;
     }
 
@@ -260,6 +339,7 @@ struct EngineImpl {
         hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 4096)),
                            dim3(kBlock), 0, stream, partial, total, 0.0);
         LCHK();
+        const int mk = mark_begin(2);
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
             if (sg.tileset >= 0) {
@@ -272,6 +352,7 @@ struct EngineImpl {
             }
             LCHK();
         }
+        mark_end(mk);
         return true;
     }
 
@@ -286,6 +367,7 @@ struct EngineImpl {
                 const double *xv = x + (size_t)v * N;
                 double *yv = y + (size_t)v * N;
                 double *ev = epart + (size_t)v * es;
+                const int mk = mark_begin(0);
                 if (left >= 8) {
                     hipLaunchKernelGGL(k_ell<8>, dim3(es, left / 8), dim3(kBlock), 0, stream, d_col.p, d_val.p, W, N,
                                        xv, yv, N, ev, es);
@@ -303,13 +385,23 @@ struct EngineImpl {
                                        yv, N, ev, es);
                     v += 1;
                 }
+                mark_end(mk);
                 LCHK();
             }
             return true;
         }
+        if (hkind == 3) {
+            const int mk = mark_begin(0);
+            hipLaunchKernelGGL(k_mf_sigma, dim3(na, nvec), dim3(kBlock), mf_lds(), stream, mfargs(), x, y, N, epart);
+            mark_end(mk);
+            LCHK();
+            return true;
+        }
         if (hkind == 1) {
+            const int mk = mark_begin(0);
             hipLaunchKernelGGL(k_rows, dim3(es, nvec), dim3(kBlock), 0, stream, d_rowptr.p, d_col.p, d_val.p, N, x, y,
                                N, epart, es);
+            mark_end(mk);
             LCHK();
             return true;
         }
@@ -501,7 +593,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     }
     e.slots = std::max(maxTiles, e.Ggen);
     if (!e.d_cs.ensure(M, err) || !e.d_csneg.ensure(M, err) || !e.d_partial.ensure((size_t)M * e.slots, err) ||
-        !e.d_epart.ensure(e.eslots(), err) || !e.d_out.ensure(M + 1, err))
+        !e.d_epart.ensure(e.eslots_alloc(), err) || !e.d_out.ensure(M + 1, err))
         return false;
     return true;
 }
@@ -559,19 +651,40 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
     size_t freeB = 0, totalB = 0;
     HCHK(hipMemGetInfo(&freeB, &totalB));
     const bool fitsExplicit = bytes + (size_t)(4ull << 30) < freeB;
+    if (!e.d_h.ensure(h.size(), err) || !e.d_g.ensure(g.size(), err)) return false;
+    HCHK(hipMemcpy(e.d_h.p, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(e.d_g.p, g.data(), 8 * g.size(), hipMemcpyHostToDevice));
     if (storage == kHamMatrixFree || (storage == kHamAuto && !fitsExplicit)) {
-        err = "matrix-free Hamiltonian application is not available in this build (explicit rows need " +
-              std::to_string(bytes >> 20) + " MiB)";
-        return false;
+        // matrix-free: diagonal and same-spin single tables, then sigma on the fly
+        const long long nsA = (long long)ku * (n - ku), nsB = (long long)kd * (n - kd);
+        if (!e.d_diag.ensure(e.N, err) || !e.d_FA.ensure(std::max(1ll, nsA * e.na), err) ||
+            !e.d_FB.ensure(std::max(1ll, nsB * e.nb), err))
+            return false;
+        if (e.nb > 16 * kBlock || n > 15) {
+            err = "matrix-free sigma supports at most 15 orbitals";
+            return false;
+        }
+        if (e.mf_lds() > 65536)
+            HCHK(hipFuncSetAttribute((const void *)k_mf_sigma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)e.mf_lds()));
+        const MfArgs a = e.mfargs();
+        const long long cnt = std::max<long long>(e.N, std::max(e.na, e.nb));
+        hipLaunchKernelGGL(k_mf_tables, dim3((unsigned)((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream, a);
+        LCHK();
+        HCHK(hipStreamSynchronize(e.stream));
+        e.d_col.release();
+        e.d_val.release();
+        e.d_rowptr.release();
+        e.W = 0;
+        e.nnz = W * e.N;  // implicit entries
+        e.hkind = 3;
+        return true;
     }
     if (!fitsExplicit) {
         err = "explicit Hamiltonian rows do not fit in device memory";
         return false;
     }
-    DBuf<double> dh, dg;
-    if (!dh.ensure(h.size(), err) || !dg.ensure(g.size(), err)) return false;
-    HCHK(hipMemcpy(dh.p, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(dg.p, g.data(), 8 * g.size(), hipMemcpyHostToDevice));
+    const DBuf<double> &dh = e.d_h, &dg = e.d_g;
     e.d_rowptr.release();
     if (!e.d_col.ensure((size_t)W * e.N, err) || !e.d_val.ensure((size_t)W * e.N, err)) return false;
     EllBuildArgs a;
@@ -655,38 +768,102 @@ bool Engine::download_state(double *out, std::string &err)
     return true;
 }
 
-bool Engine::batch(int nvec, const double *angles, double *E, double *deriv, std::string &err)
+bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
 {
     EngineImpl &e = *d_;
     if (!e.haveStart || e.M == 0 || e.hkind == 0) {
         err = "engine not set up";
         return false;
     }
-    const size_t NV = (size_t)nvec * e.N;
-    const int es = e.eslots();
-    if (!e.b_psi.ensure(NV, err) || !e.b_lam.ensure(NV, err) || !e.b_epart.ensure((size_t)nvec * es, err) ||
-        !e.b_out.ensure((size_t)nvec * (e.M + 1), err))
+    if (nvec < 1) {
+        err = "empty batch";
         return false;
-    if (deriv && !e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
+    }
+    const size_t NV = (size_t)nvec * e.N;
+    if (!e.b_psi.ensure(NV, err) || !e.b_lam.ensure(NV, err) ||
+        !e.b_epart.ensure((size_t)nvec * e.eslots_alloc(), err) || !e.b_out.ensure((size_t)nvec * (e.M + 1), err) ||
+        !e.b_partial.ensure((size_t)nvec * e.M * e.slots, err))
+        return false;
     if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
-    for (int v = 0; v < nvec; v++)
-        HCHK(hipMemcpyAsync(e.b_psi.p + (size_t)v * e.N, e.d_start.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    e.bnvec = nvec;
+    return true;
+}
+
+// Enqueues psi_v = U(x_v) psi_0, E_v, and (optionally) the adjoint sweep for
+// every vector of the uploaded batch; results stay in b_out.
+bool Engine::batch_run(bool with_gradient, std::string &err)
+{
+    EngineImpl &e = *d_;
+    const int nvec = e.bnvec;
+    if (nvec < 1) {
+        err = "no angle batch uploaded";
+        return false;
+    }
+    const int es = e.eslots();
+    const long long total = (long long)nvec * e.N;
+    hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                       dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
+    LCHK();
     if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
     if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
     hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
                        e.b_out.p, 1.0);
     LCHK();
-    if (deriv) {
+    if (with_gradient) {
         if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
         hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
                            e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
         LCHK();
     }
+    return true;
+}
+
+bool Engine::batch_download(double *E, double *deriv, std::string &err)
+{
+    EngineImpl &e = *d_;
+    const int nvec = e.bnvec;
     HCHK(hipMemcpyAsync(E, e.b_out.p, 8 * nvec, hipMemcpyDeviceToHost, e.stream));
     if (deriv)
         HCHK(hipMemcpyAsync(deriv, e.b_out.p + nvec, 8 * (size_t)nvec * e.M, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
+    e.collect_marks();
     return true;
+}
+
+int Engine::batch_size() const { return d_->bnvec; }
+
+bool Engine::synchronize(std::string &err)
+{
+    HCHK(hipStreamSynchronize(d_->stream));
+    d_->collect_marks();
+    return true;
+}
+
+void Engine::set_timing(bool on)
+{
+    EngineImpl &e = *d_;
+    e.timingOn = on;
+    for (int c = 0; c < 3; c++) {
+        e.tms[c] = 0;
+        e.tcnt[c] = 0;
+    }
+}
+
+bool Engine::timing(int cls, double *ms, long long *launches, std::string &err)
+{
+    if (cls < 0 || cls > 2) {
+        err = "unknown kernel class";
+        return false;
+    }
+    if (!synchronize(err)) return false;
+    *ms = d_->tms[cls];
+    *launches = d_->tcnt[cls];
+    return true;
+}
+
+bool Engine::batch(int nvec, const double *angles, double *E, double *deriv, std::string &err)
+{
+    return batch_upload(nvec, angles, err) && batch_run(deriv != nullptr, err) && batch_download(E, deriv, err);
 }
 
 // Per-parameter Hessian, d2E = 2<psi|H|d2 psi> + 2<d psi|H|d psi>.
@@ -730,7 +907,7 @@ bool Engine::hessian(const double *angles, double *hess, double *grad, double *E
     if (!e.hz_B.ensure(PN, err) || !e.hz_HB.ensure(PN, err) || !e.hz_psi2.ensure(2 * N, err) ||
         !e.hz_lam2.ensure(2 * N, err) || !e.hz_xpart.ensure((size_t)M * P * G, err) ||
         !e.hz_dpart.ensure((size_t)M * G, err) || !e.hz_out.ensure((size_t)M * P + M + (size_t)P * P, err) ||
-        !e.hz_epart.ensure((size_t)P * e.eslots(), err))
+        !e.hz_epart.ensure((size_t)P * e.eslots_alloc(), err))
         return false;
     auto fill = [&](double *p, size_t cnt) {
         hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<size_t>((cnt + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0,

@@ -59,6 +59,20 @@ public:
     bool download_state(double *out, std::string &err);
     // Batched: nvec angle vectors (row-major nvec x M).  deriv may be null.
     bool batch(int nvec, const double *angles, double *E, double *deriv, std::string &err);
+    // The same in three steps, so that repeated evaluations run with their
+    // inputs resident on the device: upload (sin/cos tables of the angles),
+    // run (enqueue only; no host synchronisation), download (synchronise,
+    // copy energies and per-operator derivatives).
+    bool batch_upload(int nvec, const double *angles, std::string &err);
+    bool batch_run(bool with_gradient, std::string &err);
+    bool batch_download(double *E, double *deriv, std::string &err);
+    int batch_size() const;
+    bool synchronize(std::string &err);
+
+    // HIP-event timing of kernel classes on the engine's stream:
+    // 0 Hamiltonian application, 1 forward sweep, 2 adjoint sweep.
+    void set_timing(bool on);
+    bool timing(int cls, double *ms, long long *launches, std::string &err);
 
     // Diagnostics for the benchmark and tests.
     long long ham_nnz() const;

@@ -9,6 +9,14 @@ __global__ void __launch_bounds__(kBlock) k_fill(double *__restrict__ p, long lo
         p[i] = v;
 }
 
+__global__ void __launch_bounds__(kBlock)
+    k_bcast(const double *__restrict__ src, double *__restrict__ dst, long long n, int nvec)
+{
+    const long long total = n * nvec;
+    for (long long t = blockIdx.x * (long long)kBlock + threadIdx.x; t < total; t += (long long)gridDim.x * kBlock)
+        dst[t] = src[t % n];
+}
+
 // ------------------------------------------------------------- rotations
 // Rotation on a pair (b, b') with phase s (src/fusedevolve.cpp:213-234):
 //   x_b <- C x_b + s S x_b',   x_b' <- -s S x_b + C x_b'.
@@ -363,6 +371,229 @@ __global__ void __launch_bounds__(kBlock) k_ell_build(EllBuildArgs a)
                     val[e * N] = signed_by(jb, m, t);
                     e++;
                 }
+}
+
+// ------------------------------------------------------- matrix-free sigma
+__device__ __forceinline__ double thr13(double v) { return fabs(v) > 1e-13 ? v : 0.0; }
+
+// (-1)^(number of occupied orbitals strictly between r and q)
+__device__ __forceinline__ int between_parity(uint32_t s, int r, int q)
+{
+    const int lo = r < q ? r : q, hi = r < q ? q : r;
+    const uint32_t m = ((1u << hi) - 1) & ~((1u << (lo + 1)) - 1);
+    return __popc(s & m) & 1;
+}
+
+// Signed single-excitation value of one spin string (orbitals 0..n-1,
+// spin-orbital offset `off`): one-body term plus the same-spin spectator
+// two-body terms, each with its own sign (src/hamiltonianmatrix.cpp:512-541).
+__device__ double same_spin_single(uint32_t s, int off, int r, int q, int n, const double *__restrict__ h,
+                                   const double *__restrict__ g)
+{
+    const uint32_t jb = s << off;
+    uint32_t m;
+    double v = 0.0;
+    {
+        const double t = one_body(r + off, q + off, n, h, m);
+        v += signed_by(jb, m, t);
+    }
+    for (uint32_t o = s; o; o &= o - 1) {
+        const int x = __ffs(o) - 1;
+        if (x == r) continue;
+        const int R = r + off, Q = q + off, X = x + off;
+        const double t = two_body(R < X ? R : X, R < X ? X : R, Q < X ? Q : X, Q < X ? X : Q, n, g, m);
+        v += signed_by(jb, m, t);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mf_tables(MfArgs a)
+{
+    const long long idx = blockIdx.x * (long long)kBlock + threadIdx.x;
+    const int n = a.n;
+    const uint32_t full = (1u << n) - 1;
+    if (idx < a.N) {
+        const long long ra = idx / a.nb;
+        const int rb = (int)(idx - ra * a.nb);
+        const uint32_t jb = (a.unA[ra] << n) | a.unB[rb];
+        uint32_t m;
+        double v = 0;
+        for (uint32_t o = jb; o; o &= o - 1) {
+            const int x = __ffs(o) - 1;
+            v += one_body(x, x, n, a.h, m);
+            for (uint32_t o2 = o & (o - 1); o2; o2 &= o2 - 1) {
+                const int y = __ffs(o2) - 1;
+                v += two_body(x, y, x, y, n, a.g, m);  // mask 0: no sign
+            }
+        }
+        a.diag[idx] = v;
+    }
+    if (idx < a.nb) {
+        const uint32_t s = a.unB[idx];
+        int t = 0;
+        for (uint32_t o = s; o; o &= o - 1)
+            for (uint32_t w = ~s & full; w; w &= w - 1, t++)
+                a.FB[(long long)t * a.nb + idx] = same_spin_single(s, 0, __ffs(o) - 1, __ffs(w) - 1, n, a.h, a.g);
+    }
+    if (idx < a.na) {
+        const uint32_t s = a.unA[idx];
+        int t = 0;
+        for (uint32_t o = s; o; o &= o - 1)
+            for (uint32_t w = ~s & full; w; w &= w - 1, t++)
+                a.FA[(long long)t * a.na + idx] = same_spin_single(s, n, __ffs(o) - 1, __ffs(w) - 1, n, a.h, a.g);
+    }
+}
+
+// sigma(alpha, .) = (M^T x)(alpha, .): the row's own beta singles and
+// doubles (x row staged in LDS), then for every alpha single the source row
+// in LDS with the up-down doubles through that row, then the up-up doubles.
+// Up-down double (r_a q_a | r_b q_b): value g[r_a][q_a][r_b][q_b] in the
+// reference's lookup, sign = product of the two strings' between-parities.
+__global__ void __launch_bounds__(kBlock)
+    k_mf_sigma(MfArgs a, const double *__restrict__ x, double *__restrict__ y, long long vstride,
+               double *__restrict__ epart)
+{
+    extern __shared__ double sm[];
+    __shared__ double red[kBlock / 64];
+    const int n = a.n, nb = a.nb, nn = n * n;
+    const uint32_t full = (1u << n) - 1;
+    double *xrow = sm;                       // nb
+    double *gsl = xrow + nb;                 // n*n
+    double *Ja = gsl + nn;                   // n*n
+    short *rkb = (short *)(Ja + nn);         // 2^n
+    const int ra = blockIdx.x, vb = blockIdx.y;
+    x += vb * vstride;
+    y += vb * vstride;
+    const uint32_t alpha = a.unA[ra];
+    for (int s = threadIdx.x; s < (1 << n); s += kBlock) rkb[s] = (short)a.rkB[s];
+    for (int e = threadIdx.x; e < nn; e += kBlock) {
+        double v = 0;
+        for (uint32_t o = alpha; o; o &= o - 1) {
+            const int xo = __ffs(o) - 1;
+            v += thr13(a.g[(size_t)(xo * n + xo) * nn + e]);
+        }
+        Ja[e] = v;
+    }
+    const double *xa = x + (long long)ra * nb;
+    for (int c = threadIdx.x; c < nb; c += kBlock) xrow[c] = xa[c];
+    __syncthreads();
+
+    constexpr int JM = 16;  // up to 16 * kBlock beta strings (n <= 14)
+    double acc[JM];
+#pragma unroll
+    for (int j = 0; j < JM; j++) acc[j] = 0;
+
+    // same alpha: diagonal, beta singles, beta-beta doubles
+#pragma unroll
+    for (int j = 0; j < JM; j++) {
+        const int c = threadIdx.x + j * kBlock;
+        if (c >= nb) break;
+        const uint32_t b = a.unB[c];
+        double v = a.diag[(long long)ra * nb + c] * xrow[c];
+        int t = 0;
+        for (uint32_t o = b; o; o &= o - 1) {
+            const int r = __ffs(o) - 1;
+            for (uint32_t w = ~b & full; w; w &= w - 1, t++) {
+                const int q = __ffs(w) - 1;
+                const int c2 = rkb[b ^ (1u << r) ^ (1u << q)];
+                const double jv = Ja[r * n + q];
+                const double val = a.FB[(long long)t * nb + c] + (between_parity(b, r, q) ? -jv : jv);
+                v += val * xrow[c2];
+            }
+        }
+        for (uint32_t o1 = b; o1; o1 &= o1 - 1) {
+            const int r1 = __ffs(o1) - 1;
+            for (uint32_t o2 = o1 & (o1 - 1); o2; o2 &= o2 - 1) {
+                const int r2 = __ffs(o2) - 1;
+                for (uint32_t w1 = ~b & full; w1; w1 &= w1 - 1) {
+                    const int q1 = __ffs(w1) - 1;
+                    for (uint32_t w2 = w1 & (w1 - 1); w2; w2 &= w2 - 1) {
+                        const int q2 = __ffs(w2) - 1;
+                        uint32_t m;
+                        const double tv = two_body(r1, r2, q1, q2, n, a.g, m);
+                        const int c2 = rkb[b ^ (1u << r1) ^ (1u << r2) ^ (1u << q1) ^ (1u << q2)];
+                        v += signed_by(b, m, tv) * xrow[c2];
+                    }
+                }
+            }
+        }
+        acc[j] = v;
+    }
+
+    // alpha singles and up-down doubles, one source row at a time
+    int tA = 0;
+    for (uint32_t oa = alpha; oa; oa &= oa - 1) {
+        const int rA = __ffs(oa) - 1;
+        for (uint32_t wa = ~alpha & full; wa; wa &= wa - 1, tA++) {
+            const int qA = __ffs(wa) - 1;
+            const uint32_t alpha2 = alpha ^ (1u << rA) ^ (1u << qA);
+            const int ra2 = a.rkA[alpha2];
+            const int sA = between_parity(alpha, rA, qA);
+            __syncthreads();
+            const double *xs = x + (long long)ra2 * nb;
+            for (int c = threadIdx.x; c < nb; c += kBlock) xrow[c] = xs[c];
+            const double *gb = a.g + (size_t)(rA * n + qA) * nn;
+            for (int e = threadIdx.x; e < nn; e += kBlock) gsl[e] = thr13(gb[e]);
+            __syncthreads();
+            const double FAv = a.FA[(long long)tA * a.na + ra];
+#pragma unroll
+            for (int j = 0; j < JM; j++) {
+                const int c = threadIdx.x + j * kBlock;
+                if (c >= nb) break;
+                const uint32_t b = a.unB[c];
+                double jb = 0, dsum = 0;
+                for (uint32_t o = b; o; o &= o - 1) {
+                    const int r = __ffs(o) - 1;
+                    jb += gsl[r * n + r];
+                    for (uint32_t w = ~b & full; w; w &= w - 1) {
+                        const int q = __ffs(w) - 1;
+                        const double t = gsl[r * n + q] * xrow[rkb[b ^ (1u << r) ^ (1u << q)]];
+                        dsum += between_parity(b, r, q) ? -t : t;
+                    }
+                }
+                const double single = FAv + (sA ? -jb : jb);
+                acc[j] += single * xrow[c] + (sA ? -dsum : dsum);
+            }
+        }
+    }
+
+    // up-up doubles: value independent of beta
+    const uint32_t jbA = alpha << n;
+    for (uint32_t o1 = alpha; o1; o1 &= o1 - 1) {
+        const int r1 = __ffs(o1) - 1;
+        for (uint32_t o2 = o1 & (o1 - 1); o2; o2 &= o2 - 1) {
+            const int r2 = __ffs(o2) - 1;
+            for (uint32_t w1 = ~alpha & full; w1; w1 &= w1 - 1) {
+                const int q1 = __ffs(w1) - 1;
+                for (uint32_t w2 = w1 & (w1 - 1); w2; w2 &= w2 - 1) {
+                    const int q2 = __ffs(w2) - 1;
+                    uint32_t m;
+                    const double tv = two_body(r1 + n, r2 + n, q1 + n, q2 + n, n, a.g, m);
+                    const double val = signed_by(jbA, m, tv);
+                    const int ra2 = a.rkA[alpha ^ (1u << r1) ^ (1u << r2) ^ (1u << q1) ^ (1u << q2)];
+                    const double *xs = x + (long long)ra2 * nb;
+#pragma unroll
+                    for (int j = 0; j < JM; j++) {
+                        const int c = threadIdx.x + j * kBlock;
+                        if (c >= nb) break;
+                        acc[j] += val * xs[c];
+                    }
+                }
+            }
+        }
+    }
+
+    double ep = 0;
+    double *ya = y + (long long)ra * nb;
+#pragma unroll
+    for (int j = 0; j < JM; j++) {
+        const int c = threadIdx.x + j * kBlock;
+        if (c >= nb) break;
+        ya[c] = acc[j];
+        ep += xa[c] * acc[j];
+    }
+    const double tot = block_sum(ep, red);
+    if (threadIdx.x == 0) epart[(long long)vb * a.na + ra] = tot;
 }
 
 // ------------------------------------------------------------- Hessian

@@ -109,6 +109,8 @@ struct SegArgs {
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_fill(double *__restrict__ p, long long n, double v);
+// dst[v*n + i] = src[i] for v < nvec
+__global__ void k_bcast(const double *__restrict__ src, double *__restrict__ dst, long long n, int nvec);
 
 // One operator on whole vectors (any sector); BR = also rotate v1 and
 // accumulate the adjoint bracket s*(v1_b v0_b' - v1_b' v0_b) before the
@@ -153,6 +155,27 @@ struct EllBuildArgs {
     double *val;
 };
 __global__ void k_ell_build(EllBuildArgs a);
+
+// Matrix-free sigma build (S_z sector) from integrals, string driven: one
+// workgroup per (alpha string, vector), threads over beta strings.  Tables
+// built once per Hamiltonian by k_mf_tables:
+//   diag[N]            diagonal elements,
+//   FB[t*nb + b]       beta single t of string b: its sign times the one-body
+//                      term plus its same-spin two-body spectator terms,
+//   FA[t*na + a]       the same for alpha strings.
+// Single t of a string enumerates r over occupied orbitals (ascending) and q
+// over empty ones (ascending): t = ir * (n - k) + iq.
+struct MfArgs {
+    int n, ku, kd, na, nb;
+    long long N;
+    const uint32_t *unA, *unB;
+    const int32_t *rkA, *rkB;
+    const double *h, *g;
+    double *diag, *FA, *FB;
+};
+__global__ void k_mf_tables(MfArgs a);
+__global__ void k_mf_sigma(MfArgs a, const double *__restrict__ x, double *__restrict__ y, long long vstride,
+                           double *__restrict__ epart);
 
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>

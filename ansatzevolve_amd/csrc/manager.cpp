@@ -206,6 +206,10 @@ void Manager::setOperatorSymmetry()
     const uint32_t up = ((nq >= 32) ? 0xffffffffu : ((1u << nq) - 1)) ^ down;
     for (const auto &e : m_excitations) {
         uint32_t cr = 0, de = 0;
+        if (e[0] >= nq || e[1] >= nq || e[2] >= nq || e[3] >= nq) {
+            m_OperatorSZSym = false;  // out of range: rejected later by the engine
+            continue;
+        }
         if (e[0] >= 0 && e[1] >= 0 && e[2] < 0 && e[3] < 0) {
             cr = 1u << e[0];
             de = 1u << e[1];
@@ -509,6 +513,74 @@ bool Manager::getEnergyGradients(int nvec, int nangles, const double *angles, st
     for (int v = 0; v < nvec; v++) {
         E[v] += m_nuclearEnergy;
         compressGradient(deriv.data() + (size_t)v * M, grads.data() + (size_t)v * m_numberOfUniqueParameters);
+    }
+    return true;
+}
+
+bool Manager::uploadAngleBatch(int nvec, int nangles, const double *angles)
+{
+    if (!validateToRun()) return false;
+    std::vector<double> dec;
+    if (!decompressBatch(nvec, nangles, angles, dec)) return false;
+    std::string err;
+    if (!m_engine->batch_upload(nvec, dec.data(), err)) {
+        log_msg("Batch upload failed: %s", err.c_str());
+        return false;
+    }
+    m_batchNangles = nangles;
+    return true;
+}
+
+bool Manager::runAngleBatch(bool withGradient)
+{
+    if (!m_isConstructed || m_engine->batch_size() < 1) {
+        log_msg("No angle batch uploaded");
+        return false;
+    }
+    std::string err;
+    if (!m_engine->batch_run(withGradient, err)) {
+        log_msg("Batch run failed: %s", err.c_str());
+        return false;
+    }
+    return true;
+}
+
+bool Manager::downloadAngleBatch(double *energies, double *gradients)
+{
+    if (!m_isConstructed || m_engine->batch_size() < 1) {
+        log_msg("No angle batch uploaded");
+        return false;
+    }
+    const int nvec = m_engine->batch_size();
+    const size_t M = m_excitations.size();
+    std::vector<double> deriv(gradients ? (size_t)nvec * M : 0);
+    std::string err;
+    if (!m_engine->batch_download(energies, gradients ? deriv.data() : nullptr, err)) {
+        log_msg("Batch download failed: %s", err.c_str());
+        return false;
+    }
+    for (int v = 0; v < nvec; v++) {
+        energies[v] += m_nuclearEnergy;
+        if (gradients)
+            compressGradient(deriv.data() + (size_t)v * M, gradients + (size_t)v * m_numberOfUniqueParameters);
+    }
+    return true;
+}
+
+bool Manager::setKernelTiming(bool on)
+{
+    if (!validateToRun()) return false;
+    m_engine->set_timing(on);
+    return true;
+}
+
+bool Manager::getKernelTiming(int cls, double &ms, long long &launches)
+{
+    if (!validateToRun()) return false;
+    std::string err;
+    if (!m_engine->timing(cls, &ms, &launches, err)) {
+        log_msg("Timing query failed: %s", err.c_str());
+        return false;
     }
     return true;
 }

@@ -37,7 +37,14 @@ public:
                             std::vector<double> &grads);
     bool getDimensions(long long &dim, int &nops, int &nparams);
 
+    bool uploadAngleBatch(int nvec, int nangles, const double *angles);
+    bool runAngleBatch(bool withGradient);
+    bool downloadAngleBatch(double *energies, double *gradients);
+    bool setKernelTiming(bool on);
+    bool getKernelTiming(int cls, double &ms, long long &launches);
+
     int numberOfParameters() const { return m_numberOfUniqueParameters; }
+    int batchAngleCount() const { return m_batchNangles; }
 
 private:
     bool validateToRun();
@@ -65,6 +72,7 @@ private:
 
     std::unique_ptr<Engine> m_engine;
     std::vector<double> m_angles;  // per-operator angles of the current psi
+    int m_batchNangles = 0;        // angle count of the uploaded batch
 };
 
 } // namespace ae

@@ -1,0 +1,215 @@
+"""Benchmark: TUPS energy + adjoint-gradient evaluations per second.
+
+Workload: 12 spatial orbitals / 12 electrons, S_z = 0 sector of 853,776
+determinants, seeded synthetic integrals, TUPS with 6 layers (198
+parameters, 330 operators), explicit Hamiltonian rows on the device.  One
+step = one batch of 64 angle vectors (theta_b = theta_0 + 1e-2 U(-pi, pi))
+evaluated to energy + gradient.  Inputs (Hamiltonian, psi_0, the angle
+batch) are resident in HBM before the timed region; a step enqueues device
+work only.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nvec 64]
+                    [--config 3] [--no-cpu-baseline]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--nvec", type=int, default=64)
+    p.add_argument("--config", type=int, default=3)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    return p.parse_args()
+
+
+def cpu_baseline(config, threads):
+    """The reference's own CPU build (oracle/_ref, compiled from its sources):
+    FusedEvolve::evolve + evolveDerivative(&E) on one angle vector, its
+    matrix-free Hamiltonian path, in a child process with `threads` threads."""
+    code = r"""
+import json, sys, time, os
+sys.path.insert(0, %r)
+import numpy as np
+from ansatzevolve_amd.workloads import make_workload
+from oracle.oracle import RefFusedEvolve
+w = make_workload(%d)
+t0 = time.time()
+fe = RefFusedEvolve(w["nq"], w["nup"], w["ndown"], [w["start_idx"]], [1.0], w["ops"], w["order"],
+                    w["scale"], w["h"], w["g"])
+x = w["scale"] * w["theta"][w["order"]]
+E, g, t = fe.egrad(x, reps=1)
+print("RESULT " + json.dumps({"t": t, "setup": time.time() - t0 - t, "threads": fe.threads(), "E": E}))
+""" % (ROOT, config)
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    for line in r.stdout.splitlines():
+        if line.startswith("RESULT "):
+            return json.loads(line[7:])
+    raise RuntimeError("reference CPU run failed: " + r.stderr[-2000:])
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the design for "
+              "sharding is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from ansatzevolve_amd import from_workload, load_library
+    from ansatzevolve_amd.workloads import make_workload
+
+    load_library()
+    w = make_workload(args.config)
+    N, M, P = w["N"], w["M"], w["P"]
+    rng = np.random.default_rng(w["seed"] + 3)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-math.pi, math.pi, size=(args.nvec, P))
+    thetas[0] = w["theta"]
+
+    print("[bench] set-up: %d determinants, %d operators, %d parameters" % (N, M, P), file=sys.stderr, flush=True)
+    t0 = time.time()
+    a = from_workload(w)
+    d, M2, P2 = a.dimensions()  # constructs: device sector maps, operators, Hamiltonian rows
+    setup_s = time.time() - t0
+    assert (d, M2, P2) == (N, M, P)
+
+    # parity of the benchmarked path against the reference CPU build's golden
+    # vector for theta_0 (tests/golden/reference_vectors.npz, when present)
+    parity = None
+    gold = os.path.join(ROOT, "tests", "golden", "reference_vectors.npz")
+    if os.path.exists(gold):
+        gz = np.load(gold, allow_pickle=False)
+        key = "cfg%d_" % args.config
+        if key + "E" in gz.files and np.array_equal(gz[key + "theta"][0], w["theta"]):
+            E1, g1 = a.energy_gradient(w["theta"])
+            parity = {"dE": abs(E1 - float(gz[key + "E"][0])),
+                      "max_dgrad": float(np.max(np.abs(g1 - gz[key + "grad"][0]))), "against": "reference CPU build"}
+
+    # single-vector E+grad through the reference-style entry point
+    t1 = time.time()
+    nsingle = 3
+    for i in range(nsingle):
+        a.energy_gradient(thetas[i + 1])
+    single_rate = nsingle / (time.time() - t1)
+
+    a.upload_batch(thetas)
+    for _ in range(args.warmup):
+        a.run_batch(True)
+    a.download_batch(True)
+    a.set_kernel_timing(True)
+    a.download_batch(True)  # drains the stream
+    a.set_kernel_timing(True)
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        a.run_batch(True)
+    E, G = a.download_batch(True)  # waits for the stream
+    t_end = time.perf_counter()
+    elapsed = t_end - t_start
+    ham_ms, ham_n = a.kernel_timing(0)
+    fwd_ms, fwd_n = a.kernel_timing(1)
+    rev_ms, rev_n = a.kernel_timing(2)
+    a.set_kernel_timing(False)
+
+    # batched results must equal single-vector results
+    Es, gs = a.energy_gradient(thetas[-1])
+    batch_consistency = max(abs(Es - E[-1]), float(np.max(np.abs(gs - G[-1]))))
+
+    evals = args.steps * args.nvec
+    value = evals / elapsed
+    # dominant kernel: Hamiltonian application, one launch = all vectors of a
+    # batch against one stream of the explicit rows.  Algorithmic bytes per
+    # launch: 12 B per stored entry (8 B value + 4 B column) + 16 B per
+    # amplitude per vector (x read, y written).
+    nnz = N * w_row_width(w)
+    nv_launch = args.nvec if args.nvec % 8 == 0 else 1
+    bytes_launch = 12.0 * nnz + 16.0 * N * nv_launch
+    avg_ms = ham_ms / max(ham_n, 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    result = {
+        "metric": "TUPS E+grad evals/s at 0.85M dets",
+        "value": value,
+        "unit": "E+grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded integrals and angles)",
+        "config": {
+            "workload": "12 orbitals / 12 electrons, Sz=0, %d determinants, TUPS L=%d (%d parameters, %d operators), "
+                        "%d angle vectors per step, explicit Hamiltonian rows (%d entries)"
+                        % (N, w["layers"], P, M, args.nvec, nnz),
+            "batch": args.nvec,
+            "determinants": N,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_ell (Hamiltonian rows x %d vectors)" % nv_launch,
+            "avg_launch_ms": avg_ms,
+            "launches": ham_n,
+            "algorithmic_bytes_per_launch": bytes_launch,
+        },
+        "breakdown_ms_per_step": {"hamiltonian": ham_ms / args.steps, "forward_sweep": fwd_ms / args.steps,
+                                  "adjoint_sweep": rev_ms / args.steps},
+        "single_vector_egrad_per_s": single_rate,
+        "setup_s": setup_s,
+        "parity": parity,
+        "batch_vs_single_max_diff": batch_consistency,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref_ansatz.so")):
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print("[bench] reference CPU baseline on %d threads" % threads, file=sys.stderr, flush=True)
+        try:
+            cb = cpu_baseline(args.config, threads)
+            result["cpu_baseline"] = {
+                "value": 1.0 / cb["t"],
+                "unit": "E+grad evals/s",
+                "cores": cb["threads"],
+                "kind": "reference",
+                "sample": "one E+grad (FusedEvolve::evolve + evolveDerivative) at theta_0 of the same workload, "
+                          "reference CPU build, its matrix-free Hamiltonian path; %.1f s" % cb["t"],
+                "dE_vs_gpu": abs(cb["E"] - a.energy(w["theta"])),
+            }
+        except Exception as ex:  # report, do not hide the GPU number
+            result["cpu_baseline"] = {"error": str(ex)[:300]}
+    a.close()
+    print(json.dumps(result))
+
+
+def w_row_width(w):
+    n, ku, kd = w["n"], w["nup"], w["ndown"]
+    c = math.comb
+    return (1 + kd * (n - kd) + ku * (n - ku) + c(kd, 2) * c(n - kd, 2) + c(ku, 2) * c(n - ku, 2)
+            + ku * (n - ku) * kd * (n - kd))
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,28 @@ int getEnergies(int nvec, int nangles, const double *angles, double *energies, v
 int getEnergyGradients(int nvec, int nangles, const double *angles, double *energies, double *gradients,
                        void *ctx);
 
+/* Device-resident batch, for callers that evaluate the same angle vectors
+ * repeatedly (benchmarks, line searches re-run after a set-up change) or
+ * want to overlap host work with the device:
+ *   uploadAngleBatch  copies nvec angle vectors to the device (sin/cos tables);
+ *   runAngleBatch     enqueues energies (and gradients if withgradient != 0)
+ *                     for the uploaded batch and returns without waiting;
+ *   downloadAngleBatch waits and copies energies / per-parameter gradients
+ *                     (gradients may be NULL).
+ * Codes: 1 ctx NULL, 2 pointer NULL, 3 nvec or nangles < 1, 5 failure,
+ * -6 gradients requested with nangles != number of parameters. */
+int uploadAngleBatch(int nvec, int nangles, const double *angles, void *ctx);
+int runAngleBatch(int withgradient, void *ctx);
+int downloadAngleBatch(double *energies, double *gradients, void *ctx);
+
+/* Kernel timing with HIP events on the ctx's stream.  setKernelTiming(1)
+ * resets and starts recording; getKernelTiming waits for the stream and
+ * returns the summed milliseconds and launch count of one kernel class:
+ * 0 Hamiltonian application, 1 forward sweep, 2 adjoint sweep.
+ * 1 ctx NULL, 2 bad class, 5 failure. */
+int setKernelTiming(int on, void *ctx);
+int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, void *ctx);
+
 /* Sector dimension, number of operators and parameters after construction
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);

@@ -70,10 +70,11 @@ def _integral_case(n, layers, seed, nparam_sets=2):
     return w, o, thetas
 
 
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
 @pytest.mark.parametrize("n,layers", [(4, 2), (6, 2), (8, 2)])
-def test_integrals_energy_gradient_hessian(gpu, n, layers):
+def test_integrals_energy_gradient_hessian(gpu, n, layers, storage):
     w, o, thetas = _integral_case(n, layers, 1000 + n)
-    with from_workload(w) as a:
+    with from_workload(w, storage=storage) as a:
         for th in thetas:
             E, g = a.energy_gradient(th)
             Eo, go = o.energy_gradient(th)
@@ -113,15 +114,36 @@ def test_per_operator_angles_and_cache(gpu):
         assert ei.value.code == 3
 
 
-def test_config2_energy_gradient(gpu):
-    """10 orbitals, 63,504 determinants, TUPS L=5 against the oracle."""
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
+def test_config2_energy_gradient(gpu, storage):
+    """10 orbitals, 63,504 determinants, TUPS L=5 against the reference's
+    own CPU build (golden vectors)."""
+    gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
     w = make_workload(2)
-    o = OracleEngine(w["nq"], [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
-    o.set_integrals(w["h"], w["g"], explicit=False)
-    with from_workload(w) as a:
+    with from_workload(w, storage=storage) as a:
         d, M, P = a.dimensions()
         assert (d, M, P) == (63504, 225, 135)
+        for i, th in enumerate(gz["cfg2_theta"]):
+            E, g = a.energy_gradient(th)
+            assert abs(E - gz["cfg2_E"][i]) <= E_TOL
+            assert np.max(np.abs(g - gz["cfg2_grad"][i])) <= G_TOL
+        H = a.hessian(gz["cfg2_theta"][0])
+        assert np.max(np.abs(H - gz["cfg2_hess"][0])) <= H_TOL
+
+
+@pytest.mark.parametrize("storage", [2, 1], ids=["matrixfree", "rows"])
+def test_config3_energy_gradient(gpu, storage):
+    """12 orbitals, 853,776 determinants, TUPS L=6 against the reference's
+    own CPU build (its matrix-free path), and batched against single."""
+    gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
+    if "cfg3_E" not in gz.files:
+        pytest.skip("no 12-orbital golden vector")
+    w = make_workload(3)
+    assert np.array_equal(gz["cfg3_theta"][0], w["theta"])
+    with from_workload(w, storage=storage) as a:
         E, g = a.energy_gradient(w["theta"])
-        Eo, go = o.energy_gradient(w["theta"])
-        assert abs(E - Eo) <= E_TOL
-        assert np.max(np.abs(g - go)) <= G_TOL
+        assert abs(E - gz["cfg3_E"][0]) <= E_TOL
+        assert np.max(np.abs(g - gz["cfg3_grad"][0])) <= G_TOL
+        th = np.stack([w["theta"], w["theta"] + 1e-2])
+        Eb, Gb = a.energy_gradients(th)
+        assert abs(Eb[0] - E) <= 1e-12 and np.max(np.abs(Gb[0] - g)) <= 1e-12
