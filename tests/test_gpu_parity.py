@@ -147,3 +147,27 @@ def test_config3_energy_gradient(gpu, storage):
         th = np.stack([w["theta"], w["theta"] + 1e-2])
         Eb, Gb = a.energy_gradients(th)
         assert abs(Eb[0] - E) <= 1e-12 and np.max(np.abs(Gb[0] - g)) <= 1e-12
+
+
+HAMS_KEYS = ["H4_L1_", "H4_L5_", "H6_L1_", "H6_L5_", "H8_L1_", "H10_L1_", "H10_L5_", "H14_L5_"]
+
+
+@pytest.mark.parametrize("key", HAMS_KEYS)
+def test_hams_minima(gpu, key):
+    """The reference's H-chain minima (Hams/*/L*/*_Parameters.dat, electronic
+    energy) through the C ABI; H14 L5 has 11,778,624 determinants and runs
+    the matrix-free sigma build."""
+    d = np.load(os.path.join(HERE, "golden", "hams.npz"), allow_pickle=False)
+    nq = int(d[key + "nq"])
+    with AnsatzSynth() as a:
+        a.set_initial_state(nq, d[key + "init_idx0"] + 1, d[key + "init_coeff"])
+        a.set_integrals(d[key + "h"].reshape(nq // 2, nq // 2), d[key + "g"])
+        a.set_excitation(d[key + "ops1"], d[key + "order0"] + 1, d[key + "scale"])
+        for E_ref, p in zip(d[key + "energies"], d[key + "params"]):
+            x = d[key + "scale"] * p[d[key + "order0"]]
+            E = a.energy(x)
+            assert abs(E - E_ref) <= E_TOL, (key, E, E_ref)
+            P = int(d[key + "order0"].max()) + 1
+            E2, g = a.energy_gradient(p[:P])
+            assert abs(E2 - E_ref) <= E_TOL
+            assert np.linalg.norm(g) < 1e-5

@@ -244,3 +244,31 @@ def test_sector_choice_follows_symmetry():
     assert o.mode == 1 and o.N == math.comb(8, 2)
     o = OracleEngine(8, [0b1, 0b11], [0.6, 0.8], np.array([[0, 1, -1, -1]]), [0])
     assert o.mode == 0 and o.N == 256
+
+
+# ------------------------------------------------------------ reference data sets
+def hams():
+    return np.load(os.path.join(GOLD, "hams.npz"), allow_pickle=False)
+
+
+def hams_case(d, key):
+    nq = int(d[key + "nq"])
+    o = OracleEngine(nq, d[key + "init_idx0"], d[key + "init_coeff"], d[key + "ops1"] - 1, d[key + "order0"],
+                     d[key + "scale"])
+    return o
+
+
+@pytest.mark.parametrize("key", ["H4_L1_", "H4_L5_", "H6_L1_", "H6_L5_", "H8_L1_", "H10_L1_"])
+def test_hams_minima_energies(key):
+    """Hams/*/L*/*_Parameters.dat: the reference's minima ("Energy of
+    minimum", the electronic energy); angles per operator =
+    parameter[order] * scale (src/TUPSLoadingUtils.cpp:176-296)."""
+    d = hams()
+    o = hams_case(d, key)
+    nq = int(d[key + "nq"])
+    o.set_integrals(d[key + "h"], d[key + "g"], explicit=nq <= 16)
+    for E_ref, p in zip(d[key + "energies"], d[key + "params"]):
+        x = d[key + "scale"] * p[d[key + "order0"]]
+        E, g = o.energy_gradient(x)
+        assert abs(E - E_ref) < 1e-10
+        assert np.linalg.norm(g) < 1e-5  # stationary point
