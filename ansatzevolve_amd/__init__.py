@@ -46,6 +46,7 @@ C_API = [
     ("downloadAngleBatch", ctypes.c_int, [_dp, _dp, _vp]),
     ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
+    ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
 ]
 
 REFERENCE_SYMBOLS = [n for n, _, _ in C_API[:13]]
@@ -247,6 +248,18 @@ class AnsatzSynth:
         n = ctypes.c_longlong()
         self._chk("getKernelTiming", self.lib.getKernelTiming(cls, ctypes.byref(ms), ctypes.byref(n), self.ctx))
         return ms.value, n.value
+
+
+    def optimise(self, angles, stop_energy=float("nan"), max_trace=100000):
+        """Plain Newton optimisation from `angles`; returns (angles, E, trace)
+        with trace rows (seconds, energy, |g|, Hessian evals, energy evals)."""
+        a = np.array(angles, dtype=np.float64)
+        E = ctypes.c_double()
+        tr = np.zeros((max_trace, 5))
+        nt = ctypes.c_int()
+        self._chk("optimise", self.lib.optimise(len(a), a.ctypes.data_as(_dp), ctypes.byref(E), float(stop_energy),
+                                                max_trace, tr.ctypes.data_as(_dp), ctypes.byref(nt), self.ctx))
+        return a, E.value, tr[:nt.value].copy()
 
 
 def from_workload(w, storage=0):

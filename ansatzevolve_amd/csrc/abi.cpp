@@ -3,6 +3,7 @@
 // call on a ctx takes that ctx's lock.  No exception crosses the boundary.
 #include "../../include/ansatzsynth.h"
 
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -510,3 +511,35 @@ int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, 
 }
 
 } // extern "C"
+
+extern "C" int optimise(int nangles, double *angles, double *energy, double stopenergy, int maxtrace, double *trace,
+                        int *ntrace, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr || energy == nullptr) {
+        log_msg("nullptr angles or energy passed");
+        return 2;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> a(angles, angles + nangles), tr;
+    double E = 0;
+    const bool useStop = !std::isnan(stopenergy);
+    if (!m->optimise(a, E, tr, stopenergy, useStop)) return 5;
+    std::memcpy(angles, a.data(), sizeof(double) * nangles);
+    *energy = E;
+    if (trace && ntrace) {
+        const int rows = (int)(tr.size() / 5);
+        const int w = rows < maxtrace ? rows : maxtrace;
+        std::memcpy(trace, tr.data(), sizeof(double) * 5 * (size_t)(w > 0 ? w : 0));
+        *ntrace = w;
+    }
+    return 0;
+}

@@ -595,3 +595,79 @@ bool Manager::getDimensions(long long &dim, int &nops, int &nparams)
 }
 
 } // namespace ae
+
+namespace ae {
+
+bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
+                       bool useStop)
+{
+    if (!validateToRun()) return false;
+    const size_t M = m_excitations.size();
+    const int P = m_numberOfUniqueParameters;
+    const bool perParam = (int)angles.size() == P && angles.size() != M;
+    std::vector<double> x;
+    if (angles.size() == M) x = angles;
+    else if (perParam) {
+        x.resize(M);
+        for (size_t k = 0; k < M; k++)
+            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
+    } else {
+        log_msg("Incorrect number of angles given: %zu", angles.size());
+        return false;
+    }
+    std::vector<int> order(M);
+    std::vector<double> scale(M);
+    for (size_t k = 0; k < M; k++) {
+        order[k] = m_parameterDependency[k].first;
+        scale[k] = m_parameterDependency[k].second;
+    }
+    Engine *eng = m_engine.get();
+    std::string err;
+    EvalFns f;
+    f.hessian = [&](const std::vector<double> &a, std::vector<double> &H, std::vector<double> &g, double &En) {
+        if (!eng->hessian(a.data(), H.data(), g.data(), &En, err)) return false;
+        En += m_nuclearEnergy;
+        return true;
+    };
+    f.egrad = [&](const std::vector<double> &a, double &En, std::vector<double> &g) {
+        std::vector<double> deriv(M);
+        if (!eng->evolve(a.data(), err) || !eng->gradient(a.data(), &En, deriv.data(), err)) return false;
+        En += m_nuclearEnergy;
+        compressGradient(deriv.data(), g.data());
+        return true;
+    };
+    NewtonDriver nd(order, scale, P, f);
+    if (useStop) nd.set_stop_energy(stopEnergy);
+    const bool ok = nd.optimise(x, E);
+    // psi now belongs to the last evaluated angles; force a re-evolve next time
+    m_angles.assign(M, std::nan(""));
+    if (!ok) {
+        log_msg("Optimisation failed: %s", err.c_str());
+        return false;
+    }
+    trace.clear();
+    for (const auto &tp : nd.trace) {
+        trace.push_back(tp.seconds);
+        trace.push_back(tp.energy);
+        trace.push_back(tp.gradNorm);
+        trace.push_back((double)tp.hessianEvals);
+        trace.push_back((double)tp.energyEvals);
+    }
+    if (perParam) {
+        // normCompress: first operator of each parameter, divided by its scale
+        // (src/tupsquantities.cpp:655-666)
+        std::vector<bool> seen(P, false);
+        for (size_t k = 0; k < M; k++) {
+            const int p = m_parameterDependency[k].first;
+            if (!seen[p]) {
+                angles[p] = x[k] / m_parameterDependency[k].second;
+                seen[p] = true;
+            }
+        }
+    } else {
+        angles = x;
+    }
+    return true;
+}
+
+} // namespace ae

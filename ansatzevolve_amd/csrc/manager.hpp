@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "newton.hpp"
 
 namespace ae {
 
@@ -41,6 +42,11 @@ public:
     bool runAngleBatch(bool withGradient);
     bool downloadAngleBatch(double *energies, double *gradients);
     bool setKernelTiming(bool on);
+    // Plain Newton optimiser from `angles` (per operator or per parameter,
+    // returned in the same form); trace rows: seconds, energy, |g|,
+    // Hessian evaluations, energy evaluations.
+    bool optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
+                  bool useStop);
     bool getKernelTiming(int cls, double &ms, long long &launches);
 
     int numberOfParameters() const { return m_numberOfUniqueParameters; }

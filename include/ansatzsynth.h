@@ -150,6 +150,20 @@ int downloadAngleBatch(double *energies, double *gradients, void *ctx);
 int setKernelTiming(int on, void *ctx);
 int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, void *ctx);
 
+/* The plain optimiser (stateAnsatzManager::optimise, src/AnsatzManager.cpp:756-773,
+ * which is C++/Python-only in the reference): Newton steps with an
+ * |eigenvalue|^-1 inverse Hessian, a cubic-model line search and
+ * negative-curvature escape steps, until |gradient| < 1e-12 or no further
+ * progress (src/tupsquantities.cpp:704-1236, 1467-1598, 1940-2011).
+ * angles (length = operators or parameters) are the start and receive the
+ * result in the same form; energy receives the final energy.  When
+ * stopenergy is not NaN the run also stops once an energy <= stopenergy is
+ * seen.  If trace != NULL, up to maxtrace rows of (seconds, energy,
+ * |gradient|, Hessian evaluations, energy evaluations) are written and
+ * *ntrace is set.  1 ctx NULL, 2 pointer NULL, 3 nangles < 1, 5 failure. */
+int optimise(int nangles, double *angles, double *energy, double stopenergy, int maxtrace, double *trace,
+             int *ntrace, void *ctx);
+
 /* Sector dimension, number of operators and parameters after construction
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);

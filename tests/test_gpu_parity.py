@@ -171,3 +171,35 @@ def test_hams_minima(gpu, key):
             E2, g = a.energy_gradient(p[:P])
             assert abs(E2 - E_ref) <= E_TOL
             assert np.linalg.norm(g) < 1e-5
+
+
+def test_newton_reaches_reference_minimum(gpu):
+    """The plain optimiser from a perturbation of the reference's H4 L1
+    minimum returns to it (Hams/H4/L1/H4_Parameters.dat: -2.838345804245581)
+    with a stationary gradient and a positive semi-definite Hessian."""
+    d = np.load(os.path.join(HERE, "golden", "hams.npz"), allow_pickle=False)
+    key = "H4_L1_"
+    P = int(d[key + "order0"].max()) + 1
+    with AnsatzSynth() as a:
+        a.set_initial_state(8, d[key + "init_idx0"] + 1, d[key + "init_coeff"])
+        a.set_integrals(d[key + "h"].reshape(4, 4), d[key + "g"])
+        a.set_excitation(d[key + "ops1"], d[key + "order0"] + 1, d[key + "scale"])
+        start = d[key + "params"][0][:P] + 0.05 * np.cos(np.arange(P))
+        th, E, trace = a.optimise(start)
+        assert abs(E - d[key + "energies"][0]) <= 1e-10
+        assert np.linalg.norm(a.gradient(th)) < 1e-9
+        assert np.linalg.eigvalsh(a.hessian(th)).min() > -1e-8
+        assert len(trace) >= 2 and trace[-1, 1] <= trace[0, 1] + 1e-12
+
+
+def test_newton_synthetic_converges(gpu):
+    """From the Hartree-Fock angles of a synthetic n=6 TUPS, the optimiser
+    ends at a stationary point below the start, matching the oracle there."""
+    w, o, thetas = _integral_case(6, 2, 4242)
+    with from_workload(w) as a:
+        P = int(w["order"].max()) + 1
+        th, E, trace = a.optimise(0.1 * np.ones(P))
+        Eo, go = o.energy_gradient(th)
+        assert abs(E - Eo) <= E_TOL
+        assert np.linalg.norm(go) < 1e-8
+        assert E < o.energy(0.1 * np.ones(P))
