@@ -75,7 +75,9 @@ struct TileSet {
 
 struct Seg {
     int k0, k1;
-    int tileset;  // -1: a single operator applied by the generic kernel
+    int tileset;  // LDS tiles (>= 0); -1: generic kernel (one operator) or orbit segment
+    int orbA = -1, orbB = -1;  // orbit segment: alpha / beta class sets
+    long long orbits = 0;      // alpha classes x beta classes
 };
 
 static long long binom_ll(int n, int k)
@@ -241,22 +243,54 @@ struct EngineImpl {
         return a;
     }
 
+    int orb_grid(const Seg &sg) const
+    {
+        return (int)std::max<long long>(1, std::min<long long>((sg.orbits + kBlock - 1) / kBlock, 2048));
+    }
+
+    OrbArgs orbargs(const Seg &sg, bool reverse) const
+    {
+        const TileSet &A = *tilesets[sg.orbA], &B = *tilesets[sg.orbB];
+        OrbArgs a;
+        a.n = n;
+        a.nb = nb;
+        a.maskB = (1u << n) - 1;
+        a.unA = d_unA.p;
+        a.unB = d_unB.p;
+        a.aStart = A.tileStart.p;
+        a.aRows = A.rows.p;
+        a.bStart = B.tileStart.p;
+        a.bRows = B.rows.p;
+        a.nAc = A.ntiles;
+        a.nBc = B.ntiles;
+        a.ops = d_ops.p;
+        a.k0 = sg.k0;
+        a.k1 = sg.k1;
+        a.reverse = reverse ? 1 : 0;
+        return a;
+    }
+
     size_t seg_lds(const Seg &sg, int nv) const
     {
         return (size_t)nv * tilesets[sg.tileset]->maxRows * nb * sizeof(double);
     }
 
     // Tiles for the alpha-bit set U (S_z sector).
-    int tileset_for(uint32_t U, std::string &err)
+    // Classes of strings of one spin agreeing outside the bits U (alpha when
+    // beta == false): the tiles of an LDS segment and the orbit classes.
+    int tileset_for(uint32_t U, std::string &err, bool beta = false)
     {
+        const uint32_t key = U | (beta ? 0x80000000u : 0u);
         for (size_t i = 0; i < tilesets.size(); i++)
-            if (tilesets[i]->U == U) return (int)i;
+            if (tilesets[i]->U == key) return (int)i;
         auto t = std::make_unique<TileSet>();
-        t->U = U;
-        std::map<uint32_t, std::vector<int>> groups;  // spectator pattern -> alpha ranks
-        for (int ra = 0; ra < na; ra++) groups[unA[ra] & ~U].push_back(ra);
+        t->U = key;
+        const std::vector<uint32_t> &un = beta ? unB : unA;
+        const int cnt = beta ? nb : na;
+        std::map<uint32_t, std::vector<int>> groups;  // spectator pattern -> ranks
+        for (int ra = 0; ra < cnt; ra++) groups[un[ra] & ~U].push_back(ra);
         std::vector<int32_t> start, rows;
-        std::vector<int8_t> local(na, 0);
+        std::vector<int8_t> local(cnt, 0);
         int maxRows = 0;
         for (auto &kv : groups) {
             start.push_back((int32_t)rows.size());
@@ -317,7 +351,10 @@ struct EngineImpl {
     {
         const int mk = mark_begin(1);
         for (const Seg &sg : segs) {
-            if (sg.tileset >= 0) {
+            if (sg.orbA >= 0) {
+                hipLaunchKernelGGL((k_orb<false>), dim3(orb_grid(sg), nvec), dim3(kBlock), 0, stream,
+                                   orbargs(sg, false), v, nullptr, N, cs, (long long)M, nullptr, 0, 0);
+            } else if (sg.tileset >= 0) {
                 const TileSet &t = *tilesets[sg.tileset];
                 hipLaunchKernelGGL((k_seg<1, false>), dim3(t.ntiles, nvec), dim3(kBlock), seg_lds(sg, 1), stream,
                                    segargs(sg, false), v, nullptr, N, cs, (long long)M, nullptr, 0, 0);
@@ -342,7 +379,10 @@ struct EngineImpl {
         const int mk = mark_begin(2);
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
-            if (sg.tileset >= 0) {
+            if (sg.orbA >= 0) {
+                hipLaunchKernelGGL((k_orb<true>), dim3(orb_grid(sg), nvec), dim3(kBlock), 0, stream,
+                                   orbargs(sg, true), phi, lam, N, csneg, (long long)M, partial, slots, pstride);
+            } else if (sg.tileset >= 0) {
                 const TileSet &t = *tilesets[sg.tileset];
                 hipLaunchKernelGGL((k_seg<2, true>), dim3(t.ntiles, nvec), dim3(kBlock), seg_lds(sg, 2), stream,
                                    segargs(sg, true), phi, lam, N, csneg, (long long)M, partial, slots, pstride);
@@ -554,8 +594,33 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         };
         int k = 0;
         const int maxOps = 64;
+        const char *sw = getenv("AE_SWEEP");
+        const bool useOrbits = !(sw && std::string(sw) == "tile");
+        auto bitsA = [&](int kk) { return ((e.hops[kk].cre | e.hops[kk].ann) >> n) & amask; };
+        auto bitsB = [&](int kk) { return (e.hops[kk].cre | e.hops[kk].ann) & amask; };
         while (k < M) {
-            const uint32_t Uk = ((e.hops[k].cre | e.hops[k].ann) >> n) & amask;
+            const uint32_t Uk = bitsA(k);
+            if (useOrbits && __builtin_popcount(Uk) <= 2 && __builtin_popcount(bitsB(k)) <= 2) {
+                // orbit segment: extend while both spins stay within two orbitals
+                uint32_t Ua = Uk, Ub = bitsB(k);
+                int k1 = k + 1;
+                while (k1 < M && k1 - k < kOrbMaxOps) {
+                    const uint32_t na2 = Ua | bitsA(k1), nb2 = Ub | bitsB(k1);
+                    if (__builtin_popcount(na2) > 2 || __builtin_popcount(nb2) > 2) break;
+                    Ua = na2;
+                    Ub = nb2;
+                    k1++;
+                }
+                Seg sg{k, k1, -1};
+                sg.orbA = e.tileset_for(Ua, err, false);
+                sg.orbB = e.tileset_for(Ub, err, true);
+                if (sg.orbA < 0 || sg.orbB < 0) return false;
+                sg.orbits = (long long)e.tilesets[sg.orbA]->ntiles * e.tilesets[sg.orbB]->ntiles;
+                maxTiles = std::max(maxTiles, e.orb_grid(sg));
+                e.segs.push_back(sg);
+                k = k1;
+                continue;
+            }
             if (!fits(Uk)) {
                 e.segs.push_back({k, k + 1, -1});
                 e.ngeneric++;

@@ -153,7 +153,103 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-#define AE_SEG(NV, BR)                                                                                          \
+// Orbit segment: thread per orbit (alpha class x beta class, <= 4 members),
+// every operator of the segment applied in registers.  Each unordered member
+// pair (p, q) is tested once per operator; the rotation is orientation-free
+// (the phase of the partner is minus the phase of the member), so the
+// member with the lower slot is taken as "first".  Brackets accumulate per
+// operator in registers and are reduced once per block at the end.
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orb(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+          const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+          long long pstride)
+{
+    __shared__ double red[kBlock / 64];
+    const int vb = blockIdx.y;
+    v0 += vb * vstride;
+    if (BR) v1 += vb * vstride;
+    cs += vb * csStride;
+    const int nk = a.k1 - a.k0;
+    double acc[kOrbMaxOps];
+#pragma unroll
+    for (int t = 0; t < kOrbMaxOps; t++) acc[t] = 0;
+    const long long total = (long long)a.nAc * a.nBc;
+    for (long long t = blockIdx.x * (long long)kBlock + threadIdx.x; t < total; t += (long long)gridDim.x * kBlock) {
+        const int ia = (int)(t / a.nBc), ib = (int)(t - (long long)ia * a.nBc);
+        const int a0 = a.aStart[ia], Ra = a.aStart[ia + 1] - a0;
+        const int b0 = a.bStart[ib], Rb = a.bStart[ib + 1] - b0;
+        if (Ra * Rb == 1) continue;  // a lone determinant: no operator of the segment acts on it
+        int ra[2], rb[2];
+        uint32_t st[4];
+        bool ok[4];
+        long long idx[4];
+        double x[4], l[4];
+        ra[0] = a.aRows[a0];
+        ra[1] = Ra > 1 ? a.aRows[a0 + 1] : ra[0];
+        rb[0] = a.bRows[b0];
+        rb[1] = Rb > 1 ? a.bRows[b0 + 1] : rb[0];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int i = p >> 1, j = p & 1;
+            ok[p] = (i < Ra) && (j < Rb);
+            st[p] = (a.unA[ra[i]] << a.n) | a.unB[rb[j]];
+            idx[p] = (long long)ra[i] * a.nb + rb[j];
+            x[p] = ok[p] ? v0[idx[p]] : 0.0;
+            if (BR) l[p] = ok[p] ? v1[idx[p]] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < kOrbMaxOps; kk++) {
+            if (kk >= nk) break;
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const DevOp o = a.ops[k];
+            const double2 tc = cs[k];
+            const double C = tc.x, S = tc.y;
+            double br = 0;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                uint32_t b2;
+                const int s = ok[p] ? pair_sign(o, st[p], b2) : 0;
+#pragma unroll
+                for (int q = p + 1; q < 4; q++) {
+                    if (s == 0 || !ok[q] || b2 != st[q]) continue;
+                    const double xp = x[p], xq = x[q];
+                    if (BR) {
+                        const double lp = l[p], lq = l[q];
+                        br += s * (lp * xq - lq * xp);
+                        l[p] = s * lq * S + lp * C;
+                        l[q] = -s * lp * S + lq * C;
+                    }
+                    x[p] = s * xq * S + xp * C;
+                    x[q] = -s * xp * S + xq * C;
+                }
+            }
+            acc[kk] += br;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            if (ok[p]) {
+                v0[idx[p]] = x[p];
+                if (BR) v1[idx[p]] = l[p];
+            }
+    }
+    if (BR) {
+#pragma unroll
+        for (int kk = 0; kk < kOrbMaxOps; kk++) {
+            if (kk >= nk) break;
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const double tot = block_sum(acc[kk], red);
+            if (threadIdx.x == 0) partial[vb * pstride + (long long)k * slots + blockIdx.x] = tot;
+        }
+    }
+}
+
+template __global__ void k_orb<false>(OrbArgs, double *, double *, long long, const double2 *, long long, double *, int,
+                                      long long);
+template __global__ void k_orb<true>(OrbArgs, double *, double *, long long, const double2 *, long long, double *, int,
+                                     long long);
+
+#define AE_SEG(NV, BR)                                                                                       \
     template __global__ void k_seg<NV, BR>(SegArgs, double *, double *, long long, const double2 *, long long, \
                                            double *, int, long long);
 AE_SEG(1, false)

@@ -107,8 +107,32 @@ struct SegArgs {
     int reverse;  // 1: apply k1-1 .. k0 (adjoint sweep), 0: k0 .. k1-1
 };
 
+// Orbit segments: consecutive operators that together touch at most two
+// alpha and two beta spin-orbitals (Ua, Ub).  Determinants that agree outside
+// Ua and Ub form an orbit of at most 2 x 2 members that the segment maps into
+// itself; one thread owns one orbit in registers for the whole segment.
+// Alpha classes: alpha strings agreeing outside Ua (1 or 2 members), the
+// same for beta.  An orbit is (alpha class, beta class).
+constexpr int kOrbMaxOps = 8;
+struct OrbArgs {
+    int n, nb;
+    uint32_t maskB;
+    const uint32_t *unA, *unB;
+    const int32_t *aStart, *aRows;  // alpha classes: members aRows[aStart[c] ..]
+    const int32_t *bStart, *bRows;  // beta classes
+    int nAc, nBc;
+    const DevOp *ops;
+    int k0, k1;
+    int reverse;
+};
+
 // ---------------------------------------------------------------- kernels
 __global__ void k_fill(double *__restrict__ p, long long n, double v);
+
+template <bool BR>
+__global__ void k_orb(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+                      const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+                      long long pstride);
 // dst[v*n + i] = src[i] for v < nvec
 __global__ void k_bcast(const double *__restrict__ src, double *__restrict__ dst, long long n, int nvec);
 

@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--config", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--storage", type=int, default=0,
+                   help="Hamiltonian form: 0 automatic, 1 explicit rows, 2 matrix-free sigma")
     return p.parse_args()
 
 
@@ -86,7 +88,7 @@ def main():
 
     print("[bench] set-up: %d determinants, %d operators, %d parameters" % (N, M, P), file=sys.stderr, flush=True)
     t0 = time.time()
-    a = from_workload(w)
+    a = from_workload(w, storage=args.storage)
     d, M2, P2 = a.dimensions()  # constructs: device sector maps, operators, Hamiltonian rows
     setup_s = time.time() - t0
     assert (d, M2, P2) == (N, M, P)
