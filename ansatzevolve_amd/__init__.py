@@ -47,6 +47,18 @@ C_API = [
     ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
+    # learned warm start
+    ("mlpCreate", _vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]),
+    ("mlpDestroy", ctypes.c_int, [_vp]),
+    ("mlpParameterCount", ctypes.c_longlong, [_vp]),
+    ("mlpGetParameters", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("mlpSetParameters", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("mlpForward", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    ("mlpGradient", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                   ctypes.POINTER(ctypes.c_float)]),
+    ("mlpTrainStep", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
 ]
 
 REFERENCE_SYMBOLS = [n for n, _, _ in C_API[:13]]

@@ -71,6 +71,7 @@ struct TileSet {
     int ntiles = 0, maxRows = 0;
     DBuf<int32_t> tileStart, rows;
     DBuf<int8_t> localRow;
+    DBuf<int4> desc;  // classes of at most two members: {rank0, rank1 or -1, string0, string1}
 };
 
 struct Seg {
@@ -129,6 +130,11 @@ struct EngineImpl {
     DBuf<double> hz_B, hz_HB, hz_psi2, hz_lam2, hz_xpart, hz_dpart, hz_gram, hz_out, hz_epart;
 
     int bnvec = 0;  // size of the uploaded angle batch
+
+    // hipGraph of the fused single-vector E+grad sequence
+    bool useGraph = true;
+    bool orbLds = false;  // orbit segments stage their alpha rows in LDS (k_orbl)
+    hipGraphExec_t egGraph = nullptr;
 
     // HIP-event timing of kernel classes (see Engine::set_timing)
     bool timingOn = false;
@@ -261,6 +267,8 @@ struct EngineImpl {
         a.aRows = A.rows.p;
         a.bStart = B.tileStart.p;
         a.bRows = B.rows.p;
+        a.aDesc = A.desc.p;
+        a.bDesc = B.desc.p;
         a.nAc = A.ntiles;
         a.nBc = B.ntiles;
         a.ops = d_ops.p;
@@ -311,6 +319,18 @@ struct EngineImpl {
         if (!t->tileStart.ensure(start.size(), err) || !t->rows.ensure(rows.size(), err) ||
             !t->localRow.ensure(local.size(), err))
             return -1;
+        if (maxRows <= 2) {
+            std::vector<int4> d(t->ntiles);
+            for (int c = 0; c < t->ntiles; c++) {
+                const int r0 = rows[start[c]], r1 = (start[c + 1] - start[c] > 1) ? rows[start[c] + 1] : -1;
+                d[c] = make_int4(r0, r1, (int)un[r0], r1 >= 0 ? (int)un[r1] : (int)un[r0]);
+            }
+            if (!t->desc.ensure(d.size(), err) ||
+                hipMemcpy(t->desc.p, d.data(), d.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+                err = "class descriptor upload failed";
+                return -1;
+            }
+        }
         if (hipMemcpy(t->tileStart.p, start.data(), start.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(t->rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(t->localRow.p, local.data(), local.size(), hipMemcpyHostToDevice) != hipSuccess) {
@@ -351,7 +371,11 @@ struct EngineImpl {
     {
         const int mk = mark_begin(1);
         for (const Seg &sg : segs) {
-            if (sg.orbA >= 0) {
+            if (sg.orbA >= 0 && orbLds) {
+                hipLaunchKernelGGL((k_orbl<false>), dim3(tilesets[sg.orbA]->ntiles, nvec), dim3(kBlock),
+                                   (size_t)2 * nb * sizeof(double), stream, orbargs(sg, false), v, nullptr, N, cs,
+                                   (long long)M, nullptr, 0, 0);
+            } else if (sg.orbA >= 0) {
                 hipLaunchKernelGGL((k_orb<false>), dim3(orb_grid(sg), nvec), dim3(kBlock), 0, stream,
                                    orbargs(sg, false), v, nullptr, N, cs, (long long)M, nullptr, 0, 0);
             } else if (sg.tileset >= 0) {
@@ -379,7 +403,11 @@ struct EngineImpl {
         const int mk = mark_begin(2);
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
-            if (sg.orbA >= 0) {
+            if (sg.orbA >= 0 && orbLds) {
+                hipLaunchKernelGGL((k_orbl<true>), dim3(tilesets[sg.orbA]->ntiles, nvec), dim3(kBlock),
+                                   (size_t)4 * nb * sizeof(double), stream, orbargs(sg, true), phi, lam, N, csneg,
+                                   (long long)M, partial, slots, pstride);
+            } else if (sg.orbA >= 0) {
                 hipLaunchKernelGGL((k_orb<true>), dim3(orb_grid(sg), nvec), dim3(kBlock), 0, stream,
                                    orbargs(sg, true), phi, lam, N, csneg, (long long)M, partial, slots, pstride);
             } else if (sg.tileset >= 0) {
@@ -454,6 +482,7 @@ Engine::Engine() : d_(new EngineImpl) {}
 
 Engine::~Engine()
 {
+    if (d_ && d_->egGraph) (void)hipGraphExecDestroy(d_->egGraph);
     if (d_ && d_->stream) (void)hipStreamDestroy(d_->stream);
 }
 
@@ -487,6 +516,7 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
     e.ldsMax = prop.sharedMemPerBlock;
     if (const char *b = getenv("AE_LDS_BUDGET")) e.ldsMax = std::min<size_t>(e.ldsMax, (size_t)atol(b));
     HCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    if (const char *g = getenv("AE_GRAPH")) e.useGraph = std::string(g) != "0";
     e.nq = nq;
     e.mode = mode;
     e.nel = nel;
@@ -596,6 +626,13 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         const int maxOps = 64;
         const char *sw = getenv("AE_SWEEP");
         const bool useOrbits = !(sw && std::string(sw) == "tile");
+        e.orbLds = sw && std::string(sw) == "orblds";
+        if (e.orbLds && (size_t)4 * e.nb * sizeof(double) > 65536) {
+            HCHK(hipFuncSetAttribute((const void *)k_orbl<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(4 * e.nb * sizeof(double))));
+            HCHK(hipFuncSetAttribute((const void *)k_orbl<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(4 * e.nb * sizeof(double))));
+        }
         auto bitsA = [&](int kk) { return ((e.hops[kk].cre | e.hops[kk].ann) >> n) & amask; };
         auto bitsB = [&](int kk) { return (e.hops[kk].cre | e.hops[kk].ann) & amask; };
         while (k < M) {
@@ -616,7 +653,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
                 sg.orbB = e.tileset_for(Ub, err, true);
                 if (sg.orbA < 0 || sg.orbB < 0) return false;
                 sg.orbits = (long long)e.tilesets[sg.orbA]->ntiles * e.tilesets[sg.orbB]->ntiles;
-                maxTiles = std::max(maxTiles, e.orb_grid(sg));
+                maxTiles = std::max(maxTiles, std::max(e.orb_grid(sg), e.tilesets[sg.orbA]->ntiles));
                 e.segs.push_back(sg);
                 k = k1;
                 continue;
@@ -799,6 +836,61 @@ bool Engine::energy(double *E, std::string &err)
     LCHK();
     HCHK(hipMemcpyAsync(E, e.d_out.p, 8, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
+    const unsigned cgrid = (unsigned)std::min<long long>((e.N + kBlock - 1) / kBlock, 4096);
+    auto enqueue = [&]() -> bool {
+        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_start.p, e.d_psi.p, e.N, 1);
+        LCHK();
+        if (!e.sweep_forward(e.d_psi.p, 1, e.d_cs.p, err)) return false;
+        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_psi.p, e.d_phi.p, e.N, 1);
+        LCHK();
+        if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
+                           e.d_out.p + e.M, 1.0);
+        LCHK();
+        if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, e.d_csneg.p, e.d_partial.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, e.M, e.slots,
+                           (long long)e.slots, 1LL, e.d_out.p, 2.0);
+        LCHK();
+        return true;
+    };
+    if (e.useGraph && !e.timingOn) {
+        if (!e.egGraph) {
+            // captured once: the sin/cos tables and every buffer keep their
+            // device addresses, so the graph replays for any angles
+            hipGraph_t g = nullptr;
+            HCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+            const bool ok = enqueue();
+            const hipError_t ec = hipStreamEndCapture(e.stream, &g);
+            if (!ok) {
+                if (g) (void)hipGraphDestroy(g);
+                return false;
+            }
+            HCHK(ec);
+            const hipError_t ei = hipGraphInstantiate(&e.egGraph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCHK(ei);
+        }
+        HCHK(hipGraphLaunch(e.egGraph, e.stream));
+    } else if (!enqueue()) {
+        return false;
+    }
+    std::vector<double> out(e.M + 1);
+    HCHK(hipMemcpyAsync(out.data(), e.d_out.p, 8 * (e.M + 1), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    e.collect_marks();
+    std::memcpy(deriv, out.data(), 8 * e.M);
+    if (E) *E = out[e.M];
     return true;
 }
 

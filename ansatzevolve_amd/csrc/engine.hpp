@@ -53,6 +53,9 @@ public:
     // Adjoint sweep from the current psi (which must be U(angles) psi_0):
     // deriv[k] = dE/d(angle_k) per operator; E optional.
     bool gradient(const double *angles, double *E, double *deriv, std::string &err);
+    // psi = U(angles) psi_0, then E and the adjoint sweep, as one device
+    // sequence (replayed from a hipGraph after the first call).
+    bool energy_gradient(const double *angles, double *E, double *deriv, std::string &err);
     // Per-parameter Hessian (row-major P x P), gradient and energy.
     bool hessian(const double *angles, double *hess, double *grad, double *E, std::string &err);
     // The current psi over the sector.

@@ -177,23 +177,22 @@ __global__ void __launch_bounds__(kBlock)
     const long long total = (long long)a.nAc * a.nBc;
     for (long long t = blockIdx.x * (long long)kBlock + threadIdx.x; t < total; t += (long long)gridDim.x * kBlock) {
         const int ia = (int)(t / a.nBc), ib = (int)(t - (long long)ia * a.nBc);
-        const int a0 = a.aStart[ia], Ra = a.aStart[ia + 1] - a0;
-        const int b0 = a.bStart[ib], Rb = a.bStart[ib + 1] - b0;
+        const int4 da = a.aDesc[ia], db = a.bDesc[ib];
+        const int Ra = da.y >= 0 ? 2 : 1, Rb = db.y >= 0 ? 2 : 1;
         if (Ra * Rb == 1) continue;  // a lone determinant: no operator of the segment acts on it
-        int ra[2], rb[2];
+        const int ra[2] = {da.x, da.y >= 0 ? da.y : da.x};
+        const int rb[2] = {db.x, db.y >= 0 ? db.y : db.x};
+        const uint32_t sa[2] = {(uint32_t)da.z << a.n, (uint32_t)da.w << a.n};
+        const uint32_t sb[2] = {(uint32_t)db.z, (uint32_t)db.w};
         uint32_t st[4];
         bool ok[4];
         long long idx[4];
         double x[4], l[4];
-        ra[0] = a.aRows[a0];
-        ra[1] = Ra > 1 ? a.aRows[a0 + 1] : ra[0];
-        rb[0] = a.bRows[b0];
-        rb[1] = Rb > 1 ? a.bRows[b0 + 1] : rb[0];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int i = p >> 1, j = p & 1;
             ok[p] = (i < Ra) && (j < Rb);
-            st[p] = (a.unA[ra[i]] << a.n) | a.unB[rb[j]];
+            st[p] = sa[i] | sb[j];
             idx[p] = (long long)ra[i] * a.nb + rb[j];
             x[p] = ok[p] ? v0[idx[p]] : 0.0;
             if (BR) l[p] = ok[p] ? v1[idx[p]] : 0.0;
@@ -243,6 +242,118 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
 }
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orbl(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+           const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+           long long pstride)
+{
+    extern __shared__ double lds[];
+    __shared__ double red[kBlock / 64];
+    const int ia = blockIdx.x, vb = blockIdx.y, nb = a.nb;
+    v0 += vb * vstride;
+    if (BR) v1 += vb * vstride;
+    cs += vb * csStride;
+    const int nk = a.k1 - a.k0;
+    const int a0 = a.aStart[ia], Ra = a.aStart[ia + 1] - a0;
+    int ra[2];
+    ra[0] = a.aRows[a0];
+    ra[1] = Ra > 1 ? a.aRows[a0 + 1] : ra[0];
+    const uint32_t al0 = a.unA[ra[0]] << a.n, al1 = a.unA[ra[1]] << a.n;
+    double *X = lds;           // [2][nb]
+    double *L = lds + 2 * nb;  // [2][nb] (adjoint sweep)
+    for (int r = 0; r < Ra; r++) {
+        const double *s0 = v0 + (long long)ra[r] * nb;
+        for (int c = threadIdx.x; c < nb; c += kBlock) X[r * nb + c] = s0[c];
+        if (BR) {
+            const double *s1 = v1 + (long long)ra[r] * nb;
+            for (int c = threadIdx.x; c < nb; c += kBlock) L[r * nb + c] = s1[c];
+        }
+    }
+    __syncthreads();
+    double acc[kOrbMaxOps];
+#pragma unroll
+    for (int t = 0; t < kOrbMaxOps; t++) acc[t] = 0;
+    for (int ib = threadIdx.x; ib < a.nBc; ib += kBlock) {
+        const int b0 = a.bStart[ib], Rb = a.bStart[ib + 1] - b0;
+        if (Ra * Rb == 1) continue;  // a lone determinant: no operator of the segment acts on it
+        int rb[2];
+        rb[0] = a.bRows[b0];
+        rb[1] = Rb > 1 ? a.bRows[b0 + 1] : rb[0];
+        const uint32_t be0 = a.unB[rb[0]], be1 = a.unB[rb[1]];
+        uint32_t st[4];
+        bool ok[4];
+        int li[4];
+        double x[4], l[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int i = p >> 1, j = p & 1;
+            ok[p] = (i < Ra) && (j < Rb);
+            st[p] = (i ? al1 : al0) | (j ? be1 : be0);
+            li[p] = i * nb + rb[j];
+            x[p] = ok[p] ? X[li[p]] : 0.0;
+            if (BR) l[p] = ok[p] ? L[li[p]] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < kOrbMaxOps; kk++) {
+            if (kk >= nk) break;
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const DevOp o = a.ops[k];
+            const double2 tc = cs[k];
+            const double C = tc.x, S = tc.y;
+            double br = 0;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                uint32_t b2;
+                const int s = ok[p] ? pair_sign(o, st[p], b2) : 0;
+#pragma unroll
+                for (int q = p + 1; q < 4; q++) {
+                    if (s == 0 || !ok[q] || b2 != st[q]) continue;
+                    const double xp = x[p], xq = x[q];
+                    if (BR) {
+                        const double lp = l[p], lq = l[q];
+                        br += s * (lp * xq - lq * xp);
+                        l[p] = s * lq * S + lp * C;
+                        l[q] = -s * lp * S + lq * C;
+                    }
+                    x[p] = s * xq * S + xp * C;
+                    x[q] = -s * xp * S + xq * C;
+                }
+            }
+            acc[kk] += br;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            if (ok[p]) {
+                X[li[p]] = x[p];
+                if (BR) L[li[p]] = l[p];
+            }
+    }
+    __syncthreads();
+    for (int r = 0; r < Ra; r++) {
+        double *d0 = v0 + (long long)ra[r] * nb;
+        for (int c = threadIdx.x; c < nb; c += kBlock) d0[c] = X[r * nb + c];
+        if (BR) {
+            double *d1 = v1 + (long long)ra[r] * nb;
+            for (int c = threadIdx.x; c < nb; c += kBlock) d1[c] = L[r * nb + c];
+        }
+    }
+    if (BR) {
+#pragma unroll
+        for (int kk = 0; kk < kOrbMaxOps; kk++) {
+            if (kk >= nk) break;
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const double tot = block_sum(acc[kk], red);
+            if (threadIdx.x == 0) partial[vb * pstride + (long long)k * slots + ia] = tot;
+        }
+    }
+}
+
+template __global__ void k_orbl<false>(OrbArgs, double *, double *, long long, const double2 *, long long, double *,
+                                       int, long long);
+template __global__ void k_orbl<true>(OrbArgs, double *, double *, long long, const double2 *, long long, double *,
+                                      int, long long);
 
 template __global__ void k_orb<false>(OrbArgs, double *, double *, long long, const double2 *, long long, double *, int,
                                       long long);

@@ -120,6 +120,8 @@ struct OrbArgs {
     const uint32_t *unA, *unB;
     const int32_t *aStart, *aRows;  // alpha classes: members aRows[aStart[c] ..]
     const int32_t *bStart, *bRows;  // beta classes
+    // packed class descriptors {rank0, rank1 (or -1), string0, string1}
+    const int4 *aDesc, *bDesc;
     int nAc, nBc;
     const DevOp *ops;
     int k0, k1;
@@ -133,6 +135,15 @@ template <bool BR>
 __global__ void k_orb(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
                       const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
                       long long pstride);
+
+// The same with one workgroup per alpha class: the class's rows (1 or 2)
+// are loaded into LDS with contiguous row reads, threads process beta-class
+// orbits from LDS, and the rows are stored back contiguously.  Partials are
+// indexed by alpha class.
+template <bool BR>
+__global__ void k_orbl(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,
+                       const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+                       long long pstride);
 // dst[v*n + i] = src[i] for v < nvec
 __global__ void k_bcast(const double *__restrict__ src, double *__restrict__ dst, long long n, int nvec);
 

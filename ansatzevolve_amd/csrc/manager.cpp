@@ -416,15 +416,42 @@ bool Manager::getFinalState(const std::vector<double> &angles, std::vector<doubl
     return true;
 }
 
-bool Manager::getGradientComp(const std::vector<double> &angles, std::vector<double> &grad)
+// Decompresses `angles` like setAngles; if they differ from the cached
+// angles, evolves and differentiates in one fused device sequence,
+// otherwise differentiates the cached psi (evolveDerivative,
+// src/fusedevolve.cpp:2263-2346).
+bool Manager::gradientAt(const std::vector<double> &angles, double *E, std::vector<double> &deriv)
 {
-    if (!setAngles(angles)) return false;
-    std::string err;
-    std::vector<double> deriv(m_excitations.size());
-    if (!m_engine->gradient(m_angles.data(), nullptr, deriv.data(), err)) {
-        log_msg("Gradient failed: %s", err.c_str());
+    if (!validateToRun()) return false;
+    const size_t M = m_excitations.size();
+    std::vector<double> x;
+    if (angles.size() == M) {
+        x = angles;
+    } else if ((int)angles.size() == m_numberOfUniqueParameters) {
+        x.resize(M);
+        for (size_t k = 0; k < M; k++)
+            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
+    } else {
+        log_msg("Incorrect number of angles given: %zu", angles.size());
         return false;
     }
+    std::string err;
+    deriv.resize(M);
+    bool ok;
+    if (x != m_angles) {
+        ok = m_engine->energy_gradient(x.data(), E, deriv.data(), err);
+        m_angles = ok ? x : std::vector<double>(M, std::nan(""));
+    } else {
+        ok = m_engine->gradient(x.data(), E, deriv.data(), err);
+    }
+    if (!ok) log_msg("Gradient failed: %s", err.c_str());
+    return ok;
+}
+
+bool Manager::getGradientComp(const std::vector<double> &angles, std::vector<double> &grad)
+{
+    std::vector<double> deriv;
+    if (!gradientAt(angles, nullptr, deriv)) return false;
     grad.resize(m_numberOfUniqueParameters);
     compressGradient(deriv.data(), grad.data());
     return true;
@@ -432,13 +459,8 @@ bool Manager::getGradientComp(const std::vector<double> &angles, std::vector<dou
 
 bool Manager::getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad)
 {
-    if (!setAngles(angles)) return false;
-    std::string err;
-    std::vector<double> deriv(m_excitations.size());
-    if (!m_engine->gradient(m_angles.data(), &E, deriv.data(), err)) {
-        log_msg("Gradient failed: %s", err.c_str());
-        return false;
-    }
+    std::vector<double> deriv;
+    if (!gradientAt(angles, &E, deriv)) return false;
     E += m_nuclearEnergy;
     grad.resize(m_numberOfUniqueParameters);
     compressGradient(deriv.data(), grad.data());

@@ -57,6 +57,7 @@ private:
     bool construct();
     void setOperatorSymmetry();
     bool setAngles(std::vector<double> angles);
+    bool gradientAt(const std::vector<double> &angles, double *E, std::vector<double> &deriv);
     bool decompressBatch(int nvec, int nangles, const double *angles, std::vector<double> &out);
     void compressGradient(const double *deriv, double *grad) const;
 

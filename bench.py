@@ -111,6 +111,13 @@ def main():
     for i in range(nsingle):
         a.energy_gradient(thetas[i + 1])
     single_rate = nsingle / (time.time() - t1)
+    # kernel-class breakdown of the single-vector path (eager launches)
+    a.set_kernel_timing(True)
+    for i in range(nsingle):
+        a.energy_gradient(thetas[i + 4])
+    single_breakdown = {name: a.kernel_timing(c)[0] / nsingle
+                        for c, name in ((0, "hamiltonian"), (1, "forward_sweep"), (2, "adjoint_sweep"))}
+    a.set_kernel_timing(False)
 
     a.upload_batch(thetas)
     for _ in range(args.warmup):
@@ -160,8 +167,10 @@ def main():
         "data": "synthetic (seeded integrals and angles)",
         "config": {
             "workload": "12 orbitals / 12 electrons, Sz=0, %d determinants, TUPS L=%d (%d parameters, %d operators), "
-                        "%d angle vectors per step, explicit Hamiltonian rows (%d entries)"
-                        % (N, w["layers"], P, M, args.nvec, nnz),
+                        "%d angle vectors per step, %s (%d entries)"
+                        % (N, w["layers"], P, M, args.nvec,
+                           "matrix-free sigma build" if args.storage == 2 else "explicit Hamiltonian rows", nnz),
+            "hamiltonian": "matrix-free" if args.storage == 2 else "explicit rows",
             "batch": args.nvec,
             "determinants": N,
             "parallelism": "single GPU",
@@ -181,6 +190,7 @@ def main():
         "breakdown_ms_per_step": {"hamiltonian": ham_ms / args.steps, "forward_sweep": fwd_ms / args.steps,
                                   "adjoint_sweep": rev_ms / args.steps},
         "single_vector_egrad_per_s": single_rate,
+        "single_vector_breakdown_ms": single_breakdown,
         "setup_s": setup_s,
         "parity": parity,
         "batch_vs_single_max_diff": batch_consistency,

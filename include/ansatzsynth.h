@@ -168,6 +168,24 @@ int optimise(int nangles, double *angles, double *energy, double stopenergy, int
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);
 
+/* ---------------- Part 3: learned warm start (no reference counterpart) ----
+ * A 3-layer FP32 network x -> tanh(x W0 + b0) -> tanh(. W1 + b1) -> . W2 + b2
+ * proposing initial angles from integral features, with device kernels for
+ * inference and training.  Parameters are laid out W0[in][hid], b0, W1, b1,
+ * W2[hid][out], b2; arrays are row-major [batch][features].  Loss is the
+ * masked mean squared error sum m (y - t)^2 / sum m.
+ * Codes: 1 NULL handle/pointer, 3 bad size, 5 device failure. */
+void *mlpCreate(int nin, int nhidden, int nout, unsigned long long seed);
+int mlpDestroy(void *mlp);
+long long mlpParameterCount(void *mlp);
+int mlpGetParameters(void *mlp, float *params);
+int mlpSetParameters(void *mlp, const float *params);
+int mlpForward(void *mlp, int batch, const float *x, float *y);
+int mlpGradient(void *mlp, int batch, const float *x, const float *target, const float *mask, float *loss,
+                float *grads);
+int mlpTrainStep(void *mlp, int batch, const float *x, const float *target, const float *mask, float lr,
+                 float *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -203,3 +203,22 @@ def test_newton_synthetic_converges(gpu):
         assert abs(E - Eo) <= E_TOL
         assert np.linalg.norm(go) < 1e-8
         assert E < o.energy(0.1 * np.ones(P))
+
+
+def test_graph_replay_matches_eager(gpu):
+    """getEnergyGradient at new angles runs the fused sequence from a
+    hipGraph; every replay (interleaved with eager calls) must equal the
+    eager adjoint sweep at the same angles (getGradient_COMP on the cached
+    state) and the oracle."""
+    w, o, thetas = _integral_case(6, 3, 99, nparam_sets=4)
+    with from_workload(w) as a:
+        for rep in range(2):
+            for th in thetas:
+                E, g = a.energy_gradient(th)       # fused, graph
+                g_eager = a.gradient(th)           # cached psi, eager sweep
+                E_eager = a.energy(th)
+                assert abs(E - E_eager) <= 1e-13
+                assert np.max(np.abs(g - g_eager)) <= 1e-13
+                Eo, go = o.energy_gradient(th)
+                assert abs(E - Eo) <= E_TOL and np.max(np.abs(g - go)) <= G_TOL
+                a.energies(thetas[:2])            # other device work between replays

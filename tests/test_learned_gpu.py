@@ -1,0 +1,92 @@
+"""The warm-start network's HIP kernels against a float64 numpy restatement:
+forward pass, loss, parameter gradients (and central finite differences of
+the numpy loss), and one Adam step."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def np_forward(p, X):
+    W0, b0, W1, b1, W2, b2 = [q.astype(np.float64) for q in p]
+    a1 = np.tanh(X @ W0 + b0)
+    a2 = np.tanh(a1 @ W1 + b1)
+    return a1, a2, a2 @ W2 + b2
+
+
+def np_loss_grad(p, X, T, M):
+    W0, b0, W1, b1, W2, b2 = [q.astype(np.float64) for q in p]
+    X = X.astype(np.float64)
+    a1, a2, y = np_forward(p, X)
+    ms = M.sum()
+    loss = float((M * (y - T) ** 2).sum() / ms)
+    dy = 2 * M * (y - T) / ms
+    gW2 = a2.T @ dy
+    gb2 = dy.sum(0)
+    d2 = (dy @ W2.T) * (1 - a2 ** 2)
+    gW1 = a1.T @ d2
+    gb1 = d2.sum(0)
+    d1 = (d2 @ W1.T) * (1 - a1 ** 2)
+    gW0 = X.T @ d1
+    gb0 = d1.sum(0)
+    return loss, np.concatenate([q.reshape(-1) for q in (gW0, gb0, gW1, gb1, gW2, gb2)])
+
+
+@pytest.fixture
+def net(gpu):
+    from ansatzevolve_amd.learned import WarmStartMLP
+    m = WarmStartMLP(37, 19, hidden=24, seed=5)
+    yield m
+    m.close()
+
+
+def data(seed, B=11, nin=37, nout=19):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(B, nin)).astype(np.float32)
+    T = rng.normal(size=(B, nout)).astype(np.float32)
+    M = (rng.random((B, nout)) < 0.8).astype(np.float32)
+    return X, T, M
+
+
+def test_forward_matches_numpy(net):
+    X, T, M = data(1)
+    Y = net.forward(X)
+    _, _, Yn = np_forward(net.unpack(), X.astype(np.float64))
+    np.testing.assert_allclose(Y, Yn, rtol=1e-4, atol=1e-5)
+
+
+def test_gradient_matches_numpy_and_finite_differences(net):
+    X, T, M = data(2)
+    loss, g = net.gradient(X, T, M)
+    p = net.unpack()
+    ln, gn = np_loss_grad(p, X, T, M)
+    assert abs(loss - ln) <= 1e-4 * max(1.0, abs(ln))
+    np.testing.assert_allclose(g, gn, rtol=1e-3, atol=1e-5)
+    # finite differences of the float64 loss pin the analytic numpy gradient
+    flat = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(0)
+    for idx in rng.choice(len(flat), 12, replace=False):
+        e = np.zeros_like(flat)
+        e[idx] = 1e-6
+        lp = np_loss_grad(net.unpack(flat + e), X, T, M)[0]
+        lm = np_loss_grad(net.unpack(flat - e), X, T, M)[0]
+        assert abs((lp - lm) / 2e-6 - gn[idx]) < 1e-5 + 1e-4 * abs(gn[idx])
+
+
+def test_adam_step_matches_numpy(net):
+    X, T, M = data(3)
+    p0 = net.get_params().astype(np.float64)
+    _, g = net.gradient(X, T, M)
+    net.train_step(X, T, M, 1e-2)
+    p1 = net.get_params()
+    # first Adam step with zero moments: p -= lr * g / (|g| + eps)
+    exp = p0 - 1e-2 * g / (np.abs(g) + 1e-8)
+    np.testing.assert_allclose(p1, exp, rtol=1e-5, atol=1e-6)
+
+
+def test_training_reduces_loss(net):
+    X, T, M = data(4, B=32)
+    first = net.train_step(X, T, M, 3e-3)
+    for _ in range(200):
+        last = net.train_step(X, T, M, 3e-3)
+    assert last < 0.5 * first
