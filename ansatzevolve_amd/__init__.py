@@ -47,6 +47,10 @@ C_API = [
     ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
+    ("setEngineOption", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp]),
+    ("getEngineOption", ctypes.c_int, [ctypes.c_int, _ip, _vp]),
+    ("autotune", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                ctypes.c_int, _vp]),
     # learned warm start
     ("mlpCreate", _vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]),
     ("mlpDestroy", ctypes.c_int, [_vp]),
@@ -260,6 +264,28 @@ class AnsatzSynth:
         n = ctypes.c_longlong()
         self._chk("getKernelTiming", self.lib.getKernelTiming(cls, ctypes.byref(ms), ctypes.byref(n), self.ctx))
         return ms.value, n.value
+
+    # engine option keys (setEngineOption)
+    OPT_SWEEP, OPT_LANE_MIN, OPT_GRAPH, OPT_FUSE_DEPTH, OPT_LDS_BUDGET = range(5)
+
+    def set_option(self, key, value):
+        self._chk("setEngineOption", self.lib.setEngineOption(int(key), int(value), self.ctx))
+
+    def get_option(self, key):
+        v = ctypes.c_int()
+        self._chk("getEngineOption", self.lib.getEngineOption(int(key), ctypes.byref(v), self.ctx))
+        return v.value
+
+    def autotune(self, thetas, cache=None, reps=3):
+        """Runs the batch autotuner on the angle batch `thetas` (nvec x
+        angles) and returns its report (dict); the choice stays applied."""
+        import json
+        t = np.ascontiguousarray(np.atleast_2d(thetas), dtype=np.float64)
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._chk("autotune", self.lib.autotune(t.shape[0], t.shape[1], t.ctypes.data_as(_dp),
+                                                cache.encode() if cache else None, int(reps), buf, len(buf),
+                                                self.ctx))
+        return json.loads(buf.value.decode())
 
 
     def optimise(self, angles, stop_energy=float("nan"), max_trace=100000):

@@ -12,7 +12,7 @@ LIB = os.path.join(HERE, "libansatzsynth.so")
 ARCH = os.environ.get("AE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "engine.hip", "mlp.hip"]
-CXX_SOURCES = ["manager.cpp", "newton.cpp", "abi.cpp"]
+CXX_SOURCES = ["manager.cpp", "newton.cpp", "autotune.cpp", "abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -426,6 +426,62 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
     return 0;
 }
 
+int setEngineOption(int key, int value, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    return m->setEngineOption(key, value) ? 0 : 5;
+}
+
+int getEngineOption(int key, int *value, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (value == nullptr) {
+        log_msg("nullptr value passed");
+        return 2;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    int v = 0;
+    if (!m->getEngineOption(key, v)) return 5;
+    *value = v;
+    return 0;
+}
+
+int autotune(int nvec, int nangles, const double *angles, const char *cachefile, int reps, char *report,
+             int reportlen, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (nvec < 1 || nangles < 1) {
+        log_msg("nvec or nangles < 1");
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::string rep;
+    if (!m->autotune(nvec, nangles, angles, cachefile, reps, rep)) return 5;
+    if (report && reportlen > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)reportlen - 1);
+        std::memcpy(report, rep.data(), n);
+        report[n] = '\0';
+    }
+    return 0;
+}
+
 } // extern "C"
 
 extern "C" {

@@ -1,0 +1,187 @@
+// Autotuner of the batched E+grad sequence.
+//
+// Candidates are execution choices that leave the mathematics unchanged:
+// the sweep kind (register orbits, LDS-staged orbits, LDS tiles only), the
+// fusion depth (most operators per pass) and the batch layout (lane layout
+// or one vector per grid slice).  Each candidate runs the caller's angle
+// batch once to warm up (and capture its hipGraph), then `reps` timed
+// replays; the median wall time decides.  Every candidate's energies and
+// per-operator derivatives are compared with the first candidate's, so a
+// choice is only accepted when the results agree (the report carries the
+// largest deviation).  A choice can be recorded per problem shape in a
+// text file of JSON lines and re-applied without measuring.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "manager.hpp"
+
+namespace ae {
+
+void log_msg(const char *fmt, ...);
+
+namespace {
+
+struct Candidate {
+    int sweep, depth, laneMin;
+};
+
+std::string shape_key(const Engine &eng, int nvec, int nops)
+{
+    std::ostringstream k;
+    k << "N=" << eng.dim() << " M=" << nops << " ham=" << eng.ham_kind() << " nvec=" << nvec;
+    return k.str();
+}
+
+std::string cand_json(const Candidate &c)
+{
+    std::ostringstream o;
+    o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin << "}";
+    return o.str();
+}
+
+// Last record of `key` in the cache file, if any.
+bool lookup(const std::string &file, const std::string &key, Candidate &c)
+{
+    std::ifstream in(file);
+    if (!in) return false;
+    std::string line;
+    bool found = false;
+    const std::string tag = "\"key\": \"" + key + "\"";
+    while (std::getline(in, line)) {
+        if (line.find(tag) == std::string::npos) continue;
+        Candidate t{};
+        if (std::sscanf(line.c_str() + line.find("\"choice\""),
+                        "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d}", &t.sweep, &t.depth,
+                        &t.laneMin) == 3) {
+            c = t;
+            found = true;
+        }
+    }
+    return found;
+}
+
+bool apply(Engine &eng, const Candidate &c, std::string &err)
+{
+    return eng.set_option(kOptSweep, c.sweep, err) && eng.set_option(kOptFuseDepth, c.depth, err) &&
+           eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptGraph, 1, err);
+}
+
+} // namespace
+
+bool Manager::autotune(int nvec, int nangles, const double *angles, const char *cacheFile, int reps,
+                       std::string &report)
+{
+    if (!validateToRun()) return false;
+    std::vector<double> dec;
+    if (!decompressBatch(nvec, nangles, angles, dec)) return false;
+    Engine &eng = *m_engine;
+    const int M = (int)m_excitations.size();
+    const std::string key = shape_key(eng, nvec, M);
+    std::string err;
+    Candidate best{};
+    if (cacheFile && *cacheFile && lookup(cacheFile, key, best)) {
+        if (!apply(eng, best, err)) {
+            log_msg("Autotune: applying the recorded choice failed: %s", err.c_str());
+            return false;
+        }
+        report = "{\"key\": \"" + key + "\", \"source\": \"cache\", \"choice\": " + cand_json(best) + "}";
+        return true;
+    }
+    reps = std::max(1, reps);
+
+    std::vector<Candidate> cands;
+    const bool sz = eng.mode() == kSz;
+    const int sweeps[3] = {kSweepOrbit, kSweepOrbitLds, kSweepTile};
+    const int depths[3] = {0, 2, 1};
+    for (int lanes = 0; lanes < (nvec > 1 ? 2 : 1); lanes++)
+        for (int s : sweeps) {
+            if (!sz && s != kSweepOrbit) continue;          // other sectors sweep one operator per pass
+            if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
+            for (int d : depths) {
+                if (!sz && d != 0) continue;
+                cands.push_back({s, d, lanes ? 1 : 0});
+            }
+        }
+
+    const Candidate saved{eng.option(kOptSweep), eng.option(kOptFuseDepth), eng.option(kOptLaneMin)};
+    std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
+    std::ostringstream rep;
+    rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";
+    double bestMs = 1e300, worstDev = 0;
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+        const Candidate &c = cands[ci];
+        if (!apply(eng, c, err) || !eng.batch_upload(nvec, dec.data(), err) || !eng.batch_run(true, err) ||
+            !eng.batch_download(E.data(), D.data(), err)) {
+            log_msg("Autotune candidate %s failed: %s", cand_json(c).c_str(), err.c_str());
+            std::string e2;
+            apply(eng, saved, e2);
+            return false;
+        }
+        std::vector<double> ms;
+        for (int r = 0; r < reps; r++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (!eng.batch_run(true, err) || !eng.synchronize(err)) {
+                log_msg("Autotune run failed: %s", err.c_str());
+                return false;
+            }
+            ms.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (!eng.batch_download(E.data(), D.data(), err)) return false;
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        double dev = 0;
+        if (ci == 0) {
+            E0 = E;
+            D0 = D;
+        } else {
+            for (int v = 0; v < nvec; v++) dev = std::max(dev, std::fabs(E[v] - E0[v]));
+            for (size_t i = 0; i < D.size(); i++) dev = std::max(dev, std::fabs(D[i] - D0[i]));
+        }
+        worstDev = std::max(worstDev, dev);
+        rep << (ci ? ", " : "") << "{\"choice\": " << cand_json(c) << ", \"ms\": " << med << ", \"max_dev\": " << dev
+            << "}";
+        if (med < bestMs) {
+            bestMs = med;
+            best = c;
+        }
+    }
+    if (!apply(eng, best, err)) {
+        log_msg("Autotune: applying the choice failed: %s", err.c_str());
+        return false;
+    }
+    rep << "], \"max_dev\": " << worstDev << ", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs << "}";
+    report = rep.str();
+    if (cacheFile && *cacheFile) {
+        std::ofstream out(cacheFile, std::ios::app);
+        if (out)
+            out << "{\"key\": \"" << key << "\", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs
+                << ", \"max_dev\": " << worstDev << "}\n";
+        else
+            log_msg("Autotune: cannot append to %s", cacheFile);
+    }
+    return true;
+}
+
+bool Manager::setEngineOption(int key, int value)
+{
+    if (!validateToRun()) return false;
+    std::string err;
+    if (!m_engine->set_option(key, value, err)) {
+        log_msg("Engine option %d = %d rejected: %s", key, value, err.c_str());
+        return false;
+    }
+    return true;
+}
+
+bool Manager::getEngineOption(int key, int &value)
+{
+    if (!validateToRun()) return false;
+    value = m_engine->option(key);
+    return value >= 0;
+}
+
+} // namespace ae

@@ -55,6 +55,20 @@ inline bool make_devop(const int8_t a[4], int nq, DevOp &o, std::string &why)
     return true;
 }
 
+// Host form of the pair rule: 0 if the operator does not act on b, else the
+// phase and the partner in b2.
+inline int host_pair_sign(const DevOp &o, uint32_t b, uint32_t &b2)
+{
+    const uint32_t m = b & (o.cre | o.ann);
+    int s;
+    if (m == o.ann) s = 1;
+    else if (m == o.cre) s = -1;
+    else return 0;
+    b2 = b ^ o.cre ^ o.ann;
+    if (__builtin_popcount(b & o.mask) & 1) s = -s;
+    return s * o.flip;
+}
+
 void log_msg(const char *fmt, ...);
 
 } // namespace ae

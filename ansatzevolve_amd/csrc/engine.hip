@@ -79,6 +79,8 @@ struct Seg {
     int tileset;  // LDS tiles (>= 0); -1: generic kernel (one operator) or orbit segment
     int orbA = -1, orbB = -1;  // orbit segment: alpha / beta class sets
     long long orbits = 0;      // alpha classes x beta classes
+    uint32_t ua = 0, ub = 0;   // orbit segment: its alpha / beta bits (within one spin string)
+    int progOff = -1;          // orbit segment: offset of its program in d_prog
 };
 
 static long long binom_ll(int n, int k)
@@ -105,6 +107,7 @@ struct EngineImpl {
     std::vector<int> order;
     std::vector<double> scale;
     DBuf<DevOp> d_ops;
+    DBuf<uint32_t> d_prog;  // orbit programs of the orbit segments (OrbArgs::prog)
     std::vector<Seg> segs;
     std::vector<std::unique_ptr<TileSet>> tilesets;
     int slots = 1, Ggen = 1;
@@ -131,10 +134,107 @@ struct EngineImpl {
 
     int bnvec = 0;  // size of the uploaded angle batch
 
-    // hipGraph of the fused single-vector E+grad sequence
+    // lane-layout batch workspaces (see kernels.hpp)
+    DBuf<double> bl_psi, bl_lam, bl_partial;
+    DBuf<double2> bl_cs, bl_csneg;
+    std::vector<double2> h_csl, h_cslneg;
+    int lslots = 1;   // blocks of every lane-layout adjoint launch (= bracket partial rows per operator)
+    int lslotsF = 1;  // blocks of a lane-layout forward launch
+
+    // tunable execution choices (Engine::set_option)
+    int sweepKind = kSweepOrbit;
+    int laneMin = 16;  // batches of at least this many vectors use the lane layout (0: never)
+    int fuseDepth = 0;  // most operators per segment (0: the kernels' own limits)
+    size_t ldsDevice = 65536;
     bool useGraph = true;
     bool orbLds = false;  // orbit segments stage their alpha rows in LDS (k_orbl)
+    std::vector<int8_t> ops4;  // kept to re-segment when the sweep kind changes
+
+    // hipGraphs: the fused single-vector E+grad sequence, the batch sequence
     hipGraphExec_t egGraph = nullptr;
+    hipGraphExec_t bGraph = nullptr;
+    int bGraphNvec = 0;
+    bool bGraphGrad = false;
+    bool bGraphLanes = false;
+
+    bool use_lanes(int nvec) const { return laneMin > 0 && nvec >= laneMin; }
+
+    // Orbit programs (OrbArgs::prog): for each orbit type and operator of an
+    // orbit segment, the member pairs the operator rotates and their phases,
+    // from the pair rule on representative members with the spectator bits
+    // cleared.
+    bool build_orbit_programs(std::string &err)
+    {
+        std::vector<uint32_t> prog;
+        for (Seg &sg : segs) {
+            sg.progOff = -1;
+            if (sg.orbA < 0) continue;
+            sg.progOff = (int)prog.size();
+            uint32_t bitsA[2] = {0, 0}, bitsB[2] = {0, 0};
+            {
+                int i = 0;
+                for (uint32_t u = sg.ua; u && i < 2; u &= u - 1) bitsA[i++] = u & (~u + 1);
+                i = 0;
+                for (uint32_t u = sg.ub; u && i < 2; u &= u - 1) bitsB[i++] = u & (~u + 1);
+            }
+            const bool twoA = __builtin_popcount(sg.ua) == 2, twoB = __builtin_popcount(sg.ub) == 2;
+            for (int ta = 0; ta < 3; ta++)
+                for (int tb = 0; tb < 3; tb++) {
+                    // members in descriptor order: lower string (lower rank) first
+                    uint32_t al[2], bl[2];
+                    int Ra = 1, Rb = 1;
+                    if (ta == 0) {
+                        al[0] = bitsA[0];
+                        al[1] = bitsA[1];
+                        Ra = twoA ? 2 : 1;
+                    } else {
+                        al[0] = al[1] = (ta == 2) ? sg.ua : 0u;
+                    }
+                    if (tb == 0) {
+                        bl[0] = bitsB[0];
+                        bl[1] = bitsB[1];
+                        Rb = twoB ? 2 : 1;
+                    } else {
+                        bl[0] = bl[1] = (tb == 2) ? sg.ub : 0u;
+                    }
+                    uint32_t st[4];
+                    bool ok[4];
+                    for (int p = 0; p < 4; p++) {
+                        const int i = p >> 1, j = p & 1;
+                        ok[p] = i < Ra && j < Rb;
+                        st[p] = (al[i] << n) | bl[j];
+                    }
+                    static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+                    for (int kk = 0; kk < kOrbMaxOps; kk++) {
+                        uint32_t w = 0;
+                        if (sg.k0 + kk < sg.k1) {
+                            const DevOp &o = hops[sg.k0 + kk];
+                            for (int r = 0; r < 6; r++) {
+                                const int p = PP[r], q = QQ[r];
+                                if (!ok[p] || !ok[q]) continue;
+                                uint32_t b2 = 0;
+                                const int sgn = host_pair_sign(o, st[p], b2);
+                                if (sgn == 0 || b2 != st[q]) continue;
+                                w |= 1u << r;
+                                if (sgn < 0) w |= 1u << (8 + r);
+                            }
+                        }
+                        prog.push_back(w);
+                    }
+                }
+        }
+        if (prog.empty()) prog.push_back(0);
+        if (!d_prog.ensure(prog.size(), err)) return false;
+        HCHK(hipMemcpy(d_prog.p, prog.data(), prog.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        return true;
+    }
+
+    void drop_graphs()
+    {
+        if (egGraph) (void)hipGraphExecDestroy(egGraph);
+        if (bGraph) (void)hipGraphExecDestroy(bGraph);
+        egGraph = bGraph = nullptr;
+    }
 
     // HIP-event timing of kernel classes (see Engine::set_timing)
     bool timingOn = false;
@@ -275,6 +375,11 @@ struct EngineImpl {
         a.k0 = sg.k0;
         a.k1 = sg.k1;
         a.reverse = reverse ? 1 : 0;
+        a.prog = d_prog.p + (sg.progOff >= 0 ? sg.progOff : 0);
+        a.uaLocal = sg.ua;
+        a.ubLocal = sg.ub;
+        const uint32_t full = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1);
+        a.spect = full & ~((sg.ua << n) | sg.ub);
         return a;
     }
 
@@ -424,6 +529,76 @@ struct EngineImpl {
         return true;
     }
 
+    // sin/cos tables of a batch in lane layout, padding lanes (1, 0)
+    bool upload_cs_lanes(const double *angles, int nvec, std::string &err)
+    {
+        fill_cs(angles, nvec);
+        const int nch = (nvec + kLanes - 1) / kLanes;
+        const size_t cnt = (size_t)nch * M * kLanes;
+        if (!bl_cs.ensure(cnt, err) || !bl_csneg.ensure(cnt, err)) return false;
+        h_csl.assign(cnt, make_double2(1.0, 0.0));
+        h_cslneg.assign(cnt, make_double2(1.0, 0.0));
+        for (int v = 0; v < nvec; v++) {
+            const int c = v / kLanes, l = v % kLanes;
+            for (int k = 0; k < M; k++) {
+                const double2 t = h_cs[(size_t)v * M + k];
+                const size_t o = ((size_t)c * M + k) * kLanes + l;
+                h_csl[o] = t;
+                h_cslneg[o] = make_double2(t.x, -t.y);
+            }
+        }
+        HCHK(hipMemcpyAsync(bl_cs.p, h_csl.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(bl_csneg.p, h_cslneg.data(), cnt * sizeof(double2), hipMemcpyHostToDevice, stream));
+        HCHK(hipStreamSynchronize(stream));
+        return true;
+    }
+
+    // Forward sweep of lane-layout chunks.
+    bool sweep_forward_lanes(double *v, int nch, std::string &err)
+    {
+        const int mk = mark_begin(1);
+        const long long cstride = N * kLanes;
+        for (const Seg &sg : segs) {
+            if (sg.orbA >= 0) {
+                hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, orbargs(sg, false),
+                                   v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                LCHK();
+            } else {
+                for (int k = sg.k0; k < sg.k1; k++) {
+                    hipLaunchKernelGGL((k_opv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, v, nullptr,
+                                       cstride, N, sec(), d_ops.p, k, bl_cs.p, M, nullptr, lslotsF);
+                    LCHK();
+                }
+            }
+        }
+        mark_end(mk);
+        return true;
+    }
+
+    // Adjoint sweep of lane-layout (phi, lambda) chunks; every operator's
+    // partial rows are written by exactly one launch of lslots blocks.
+    bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
+    {
+        const int mk = mark_begin(2);
+        const long long cstride = N * kLanes;
+        for (int si = (int)segs.size() - 1; si >= 0; si--) {
+            const Seg &sg = segs[si];
+            if (sg.orbA >= 0) {
+                hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, orbargs(sg, true), phi,
+                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                LCHK();
+            } else {
+                for (int k = sg.k1 - 1; k >= sg.k0; k--) {
+                    hipLaunchKernelGGL((k_opv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, phi, lam, cstride,
+                                       N, sec(), d_ops.p, k, bl_csneg.p, M, bl_partial.p, lslots);
+                    LCHK();
+                }
+            }
+        }
+        mark_end(mk);
+        return true;
+    }
+
     // y = M^T x for nvec vectors (stride N); energy partials epart[v][eslots].
     bool ham_apply(const double *x, double *y, int nvec, double *epart, std::string &err)
     {
@@ -482,7 +657,7 @@ Engine::Engine() : d_(new EngineImpl) {}
 
 Engine::~Engine()
 {
-    if (d_ && d_->egGraph) (void)hipGraphExecDestroy(d_->egGraph);
+    if (d_) d_->drop_graphs();
     if (d_ && d_->stream) (void)hipStreamDestroy(d_->stream);
 }
 
@@ -514,9 +689,15 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
     hipDeviceProp_t prop;
     HCHK(hipGetDeviceProperties(&prop, dev));
     e.ldsMax = prop.sharedMemPerBlock;
+    e.ldsDevice = prop.sharedMemPerBlock;
     if (const char *b = getenv("AE_LDS_BUDGET")) e.ldsMax = std::min<size_t>(e.ldsMax, (size_t)atol(b));
     HCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
     if (const char *g = getenv("AE_GRAPH")) e.useGraph = std::string(g) != "0";
+    if (const char *sw = getenv("AE_SWEEP")) {
+        const std::string v(sw);
+        e.sweepKind = v == "tile" ? kSweepTile : v == "orblds" ? kSweepOrbitLds : kSweepOrbit;
+    }
+    if (const char *l = getenv("AE_LANES")) e.laneMin = std::max(0, atoi(l));
     e.nq = nq;
     e.mode = mode;
     e.nel = nel;
@@ -590,6 +771,8 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
 {
     EngineImpl &e = *d_;
     const int M = (int)(ops4.size() / 4);
+    e.drop_graphs();
+    if (&ops4 != &e.ops4) e.ops4 = ops4;
     e.M = M;
     e.P = nparams;
     e.order = order;
@@ -623,10 +806,10 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
             return rows <= kMaxTileRows && (size_t)(2 * rows * e.nb * sizeof(double)) <= e.ldsMax;
         };
         int k = 0;
-        const int maxOps = 64;
-        const char *sw = getenv("AE_SWEEP");
-        const bool useOrbits = !(sw && std::string(sw) == "tile");
-        e.orbLds = sw && std::string(sw) == "orblds";
+        const int maxOps = e.fuseDepth > 0 ? std::min(64, e.fuseDepth) : 64;
+        const int maxOrb = e.fuseDepth > 0 ? std::min(kOrbMaxOps, e.fuseDepth) : kOrbMaxOps;
+        const bool useOrbits = e.sweepKind != kSweepTile;
+        e.orbLds = e.sweepKind == kSweepOrbitLds;
         if (e.orbLds && (size_t)4 * e.nb * sizeof(double) > 65536) {
             HCHK(hipFuncSetAttribute((const void *)k_orbl<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(4 * e.nb * sizeof(double))));
@@ -641,7 +824,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
                 // orbit segment: extend while both spins stay within two orbitals
                 uint32_t Ua = Uk, Ub = bitsB(k);
                 int k1 = k + 1;
-                while (k1 < M && k1 - k < kOrbMaxOps) {
+                while (k1 < M && k1 - k < maxOrb) {
                     const uint32_t na2 = Ua | bitsA(k1), nb2 = Ub | bitsB(k1);
                     if (__builtin_popcount(na2) > 2 || __builtin_popcount(nb2) > 2) break;
                     Ua = na2;
@@ -649,6 +832,8 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
                     k1++;
                 }
                 Seg sg{k, k1, -1};
+                sg.ua = Ua;
+                sg.ub = Ub;
                 sg.orbA = e.tileset_for(Ua, err, false);
                 sg.orbB = e.tileset_for(Ub, err, true);
                 if (sg.orbA < 0 || sg.orbB < 0) return false;
@@ -694,6 +879,17 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         e.ngeneric = M;
     }
     e.slots = std::max(maxTiles, e.Ggen);
+    if (!e.build_orbit_programs(err)) return false;
+    {
+        long long work = 1;
+        for (const Seg &sg : e.segs) work = std::max(work, sg.orbA >= 0 ? sg.orbits : e.N);
+        // enough waves to keep the chip's HBM pipeline full: every resident
+        // wave (4 per SIMD at the adjoint kernel's register use, 6 forward)
+        // holds two orbits' loads in flight
+        const long long waves = (work + 1) / 2;
+        e.lslots = (int)std::max<long long>(1, std::min<long long>(1024, (waves + 3) / 4));
+        e.lslotsF = (int)std::max<long long>(1, std::min<long long>(1536, (waves + 3) / 4));
+    }
     if (!e.d_cs.ensure(M, err) || !e.d_csneg.ensure(M, err) || !e.d_partial.ensure((size_t)M * e.slots, err) ||
         !e.d_epart.ensure(e.eslots_alloc(), err) || !e.d_out.ensure(M + 1, err))
         return false;
@@ -707,6 +903,7 @@ bool Engine::set_start(const std::vector<double> &start, std::string &err)
         err = "start vector has the wrong size";
         return false;
     }
+    e.drop_graphs();
     HCHK(hipMemcpy(e.d_start.p, start.data(), 8 * e.N, hipMemcpyHostToDevice));
     e.haveStart = true;
     return true;
@@ -720,6 +917,7 @@ bool Engine::set_ham_rows(const std::vector<int64_t> &rowptr, const std::vector<
         err = "row pointer has the wrong size";
         return false;
     }
+    e.drop_graphs();
     if (!e.d_rowptr.ensure(rowptr.size(), err) || !e.d_col.ensure(col.size(), err) ||
         !e.d_val.ensure(val.size(), err))
         return false;
@@ -745,6 +943,7 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
         err = "number of qubits must be twice the number of orbitals";
         return false;
     }
+    e.drop_graphs();
     const int n = e.n, ku = e.nup, kd = e.ndown;
     const long long W = 1 + (long long)kd * (n - kd) + (long long)ku * (n - ku) +
                         binom_ll(kd, 2) * binom_ll(n - kd, 2) + binom_ll(ku, 2) * binom_ll(n - ku, 2) +
@@ -938,16 +1137,27 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
     }
     const size_t NV = (size_t)nvec * e.N;
     if (!e.b_psi.ensure(NV, err) || !e.b_lam.ensure(NV, err) ||
-        !e.b_epart.ensure((size_t)nvec * e.eslots_alloc(), err) || !e.b_out.ensure((size_t)nvec * (e.M + 1), err) ||
-        !e.b_partial.ensure((size_t)nvec * e.M * e.slots, err))
+        !e.b_epart.ensure((size_t)nvec * e.eslots_alloc(), err) || !e.b_out.ensure((size_t)nvec * (e.M + 1), err))
         return false;
-    if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
+    if (e.use_lanes(nvec)) {
+        const size_t nch = (size_t)(nvec + kLanes - 1) / kLanes;
+        if (!e.bl_psi.ensure(nch * e.N * kLanes, err) || !e.bl_lam.ensure(nch * e.N * kLanes, err) ||
+            !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err))
+            return false;
+        if (!e.upload_cs_lanes(angles, nvec, err)) return false;
+    } else {
+        if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
+        if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
+    }
     e.bnvec = nvec;
     return true;
 }
 
 // Enqueues psi_v = U(x_v) psi_0, E_v, and (optionally) the adjoint sweep for
-// every vector of the uploaded batch; results stay in b_out.
+// every vector of the uploaded batch; results stay in b_out.  Batches of at
+// least laneMin vectors sweep in lane layout and transpose around the
+// Hamiltonian application.  The sequence is captured once per (batch size,
+// gradient, layout) and replayed as one hipGraph launch.
 bool Engine::batch_run(bool with_gradient, std::string &err)
 {
     EngineImpl &e = *d_;
@@ -956,22 +1166,75 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
         err = "no angle batch uploaded";
         return false;
     }
-    const int es = e.eslots();
-    const long long total = (long long)nvec * e.N;
-    hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
-                       dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
-    LCHK();
-    if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
-    if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
-    hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
-                       e.b_out.p, 1.0);
-    LCHK();
-    if (with_gradient) {
-        if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
-        hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
-                           e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
+    const bool lanes = e.use_lanes(nvec);
+    auto enqueue = [&]() -> bool {
+        const int es = e.eslots();
+        if (lanes) {
+            const int nch = (nvec + kLanes - 1) / kLanes;
+            const long long total = (long long)nch * e.N * kLanes;
+            const dim3 tgrid((unsigned)((e.N + kLanes - 1) / kLanes), nch);
+            hipLaunchKernelGGL(k_bcast_lanes, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                               dim3(kBlock), 0, e.stream, e.d_start.p, e.bl_psi.p, e.N, nch);
+            LCHK();
+            if (!e.sweep_forward_lanes(e.bl_psi.p, nch, err)) return false;
+            hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.bl_psi.p, e.b_psi.p, e.N, nvec);
+            LCHK();
+            if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+            hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es,
+                               1LL, e.b_out.p, 1.0);
+            LCHK();
+            if (with_gradient) {
+                hipLaunchKernelGGL(k_to_lanes, tgrid, dim3(kBlock), 0, e.stream, e.b_lam.p, e.bl_lam.p, e.N, nvec);
+                LCHK();
+                if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
+                hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p, e.M,
+                                   e.lslots, nvec, e.b_out.p + nvec, 2.0);
+                LCHK();
+            }
+            return true;
+        }
+        const long long total = (long long)nvec * e.N;
+        hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                           dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
         LCHK();
+        if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
+        if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
+                           e.b_out.p, 1.0);
+        LCHK();
+        if (with_gradient) {
+            if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
+            hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
+                               e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
+            LCHK();
+        }
+        return true;
+    };
+    if (!e.useGraph || e.timingOn) return enqueue();
+    if (e.bGraph && (e.bGraphNvec != nvec || e.bGraphGrad != with_gradient || e.bGraphLanes != lanes)) {
+        (void)hipGraphExecDestroy(e.bGraph);
+        e.bGraph = nullptr;
     }
+    if (!e.bGraph) {
+        // buffers only grow between uploads of a batch size, so the captured
+        // addresses stay valid; a new size recaptures
+        hipGraph_t g = nullptr;
+        HCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+        const bool ok = enqueue();
+        const hipError_t ec = hipStreamEndCapture(e.stream, &g);
+        if (!ok) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        HCHK(ec);
+        const hipError_t ei = hipGraphInstantiate(&e.bGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HCHK(ei);
+        e.bGraphNvec = nvec;
+        e.bGraphGrad = with_gradient;
+        e.bGraphLanes = lanes;
+    }
+    HCHK(hipGraphLaunch(e.bGraph, e.stream));
     return true;
 }
 
@@ -988,6 +1251,80 @@ bool Engine::batch_download(double *E, double *deriv, std::string &err)
 }
 
 int Engine::batch_size() const { return d_->bnvec; }
+
+bool Engine::set_option(int key, int value, std::string &err)
+{
+    EngineImpl &e = *d_;
+    switch (key) {
+    case kOptSweep:
+        if (value < kSweepOrbit || value > kSweepTile) {
+            err = "unknown sweep kind";
+            return false;
+        }
+        if (value == e.sweepKind) return true;
+        e.sweepKind = value;
+        if (e.M > 0) {
+            const std::vector<int8_t> ops4 = e.ops4;
+            const std::vector<int> order = e.order;
+            const std::vector<double> scale = e.scale;
+            if (!set_operators(ops4, order, scale, e.P, err)) return false;
+        }
+        e.bnvec = 0;  // batch workspaces are re-sized for the new segmentation at the next upload
+        return true;
+    case kOptLaneMin:
+        if (value < 0) {
+            err = "lane threshold must be >= 0";
+            return false;
+        }
+        e.laneMin = value;
+        e.bnvec = 0;  // the uploaded batch's tables are in the old layout
+        e.drop_graphs();
+        return true;
+    case kOptGraph:
+        e.useGraph = value != 0;
+        e.drop_graphs();
+        return true;
+    case kOptFuseDepth:
+    case kOptLdsBudget: {
+        if (value < 0 || (key == kOptLdsBudget && value > 0 && value < 4096)) {
+            err = key == kOptFuseDepth ? "fusion depth must be >= 0" : "LDS budget must be 0 or >= 4096 bytes";
+            return false;
+        }
+        if (key == kOptFuseDepth) {
+            if (value == e.fuseDepth) return true;
+            e.fuseDepth = value;
+        } else {
+            const size_t b = value == 0 ? e.ldsDevice : std::min<size_t>(e.ldsDevice, (size_t)value);
+            if (b == e.ldsMax) return true;
+            e.ldsMax = b;
+        }
+        if (e.M > 0) {
+            const std::vector<int8_t> ops4 = e.ops4;
+            const std::vector<int> order = e.order;
+            const std::vector<double> scale = e.scale;
+            if (!set_operators(ops4, order, scale, e.P, err)) return false;
+        }
+        e.bnvec = 0;
+        return true;
+    }
+    default:
+        err = "unknown engine option";
+        return false;
+    }
+}
+
+int Engine::option(int key) const
+{
+    const EngineImpl &e = *d_;
+    switch (key) {
+    case kOptSweep: return e.sweepKind;
+    case kOptLaneMin: return e.laneMin;
+    case kOptGraph: return e.useGraph ? 1 : 0;
+    case kOptFuseDepth: return e.fuseDepth;
+    case kOptLdsBudget: return (int)e.ldsMax;
+    default: return -1;
+    }
+}
 
 bool Engine::synchronize(std::string &err)
 {

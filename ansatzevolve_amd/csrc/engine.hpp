@@ -15,6 +15,23 @@ struct EngineImpl;
 // Which form the Hamiltonian takes on the device.
 enum HamStorage : int { kHamAuto = 0, kHamExplicit = 1, kHamMatrixFree = 2 };
 
+// How operator sweeps are fused (S_z sector).
+//   kSweepOrbit:    runs of operators touching <= 2 alpha and <= 2 beta
+//                   orbitals as register orbits, the rest as LDS tiles;
+//   kSweepOrbitLds: the same, orbit segments staged through LDS rows;
+//   kSweepTile:     LDS tiles only.
+enum SweepKind : int { kSweepOrbit = 0, kSweepOrbitLds = 1, kSweepTile = 2 };
+
+// Execution choices (Engine::set_option); none changes a result beyond
+// floating-point summation order.
+enum EngineOption : int {
+    kOptSweep = 0,    // SweepKind
+    kOptLaneMin = 1,  // batches of >= value vectors use the lane layout; 0 never
+    kOptGraph = 2,    // 1: replay hipGraphs, 0: eager launches
+    kOptFuseDepth = 3,  // most operators fused into one pass (0: no limit beyond the kernel's)
+    kOptLdsBudget = 4,  // bytes of LDS an LDS-tile segment may use (0: the device's limit)
+};
+
 class Engine {
 public:
     Engine();
@@ -76,6 +93,9 @@ public:
     // 0 Hamiltonian application, 1 forward sweep, 2 adjoint sweep.
     void set_timing(bool on);
     bool timing(int cls, double *ms, long long *launches, std::string &err);
+
+    bool set_option(int key, int value, std::string &err);
+    int option(int key) const;
 
     // Diagnostics for the benchmark and tests.
     long long ham_nnz() const;

@@ -360,6 +360,279 @@ template __global__ void k_orb<false>(OrbArgs, double *, double *, long long, co
 template __global__ void k_orb<true>(OrbArgs, double *, double *, long long, const double2 *, long long, double *, int,
                                      long long);
 
+// ------------------------------------------------ lane layout (batches)
+// Per-wave bracket sums of a segment's operators -> one partial row of 64
+// lanes per (operator, block).
+template <int NK>
+__device__ __forceinline__ void lanes_block_partials(const double (&acc)[NK], int nk, int k0, int k1, int reverse,
+                                                     double *red, double *__restrict__ partial, int c, int M,
+                                                     int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    const int lane = threadIdx.x & (kLanes - 1), wv = threadIdx.x / kLanes;
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++)
+        if (kk < nk) red[(kk * WPB + wv) * kLanes + lane] = acc[kk];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < WPB; w++) s += red[(kk * WPB + w) * kLanes + l];
+        const int k = reverse ? k1 - 1 - kk : k0 + kk;
+        partial[(((long long)c * M + k) * slots + blockIdx.x) * kLanes + l] = s;
+    }
+}
+
+// One orbit of a lane-layout wave: member indices, presence, type and the
+// spectator bits are wave-uniform; amplitudes are per lane (per vector).
+struct LaneOrbit {
+    long long idx[4];
+    bool ok[4];
+    bool live;
+    int type;
+    uint32_t spect;
+};
+
+__device__ __forceinline__ void lane_orbit_open(const OrbArgs &a, int t, LaneOrbit &o)
+{
+    const int ia = t / a.nBc, ib = t - ia * a.nBc;
+    const int4 da = a.aDesc[ia], db = a.bDesc[ib];
+    const int Ra = da.y >= 0 ? 2 : 1, Rb = db.y >= 0 ? 2 : 1;
+    o.live = Ra * Rb > 1;  // a lone determinant: no operator of the segment acts on it
+    const int ta = Ra == 2 ? 0 : (((uint32_t)da.z & a.uaLocal) ? 2 : 1);
+    const int tb = Rb == 2 ? 0 : (((uint32_t)db.z & a.ubLocal) ? 2 : 1);
+    o.type = ta * 3 + tb;
+    o.spect = (((uint32_t)da.z << a.n) | (uint32_t)db.z) & a.spect;
+    const int ra[2] = {da.x, da.y >= 0 ? da.y : da.x};
+    const int rb[2] = {db.x, db.y >= 0 ? db.y : db.x};
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int i = p >> 1, j = p & 1;
+        o.ok[p] = o.live && (i < Ra) && (j < Rb);
+        o.idx[p] = ((long long)ra[i] * a.nb + rb[j]) * kLanes;
+    }
+}
+
+template <bool BR>
+__device__ __forceinline__ void lane_orbit_apply(const OrbArgs &a, int nk, const LaneOrbit &o, double (&x)[4],
+                                                 double (&l)[4], const double (&C)[kOrbMaxOps],
+                                                 const double (&S)[kOrbMaxOps], double (&acc)[kOrbMaxOps])
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+    const uint32_t *prog = a.prog + o.type * kOrbMaxOps;
+#pragma unroll
+    for (int kk = 0; kk < kOrbMaxOps; kk++) {
+        if (kk >= nk) break;
+        const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+        const uint32_t w = prog[a.reverse ? nk - 1 - kk : kk];
+        if (!(w & 0x3f)) continue;
+        const bool neg = __popc(o.spect & a.ops[k].mask) & 1;
+        double br = 0;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            if (!((w >> r) & 1)) continue;
+            const int p = PP[r], q = QQ[r];
+            const double s = (((w >> (8 + r)) & 1) != neg) ? -1.0 : 1.0;
+            const double xp = x[p], xq = x[q];
+            if (BR) {
+                const double lp = l[p], lq = l[q];
+                br += s * (lp * xq - lq * xp);
+                l[p] = s * lq * S[kk] + lp * C[kk];
+                l[q] = -s * lp * S[kk] + lq * C[kk];
+            }
+            x[p] = s * xq * S[kk] + xp * C[kk];
+            x[q] = -s * xp * S[kk] + xq * C[kk];
+        }
+        acc[kk] += br;
+    }
+}
+
+// Orbit segment in lane layout.  Each wave takes two orbits per iteration
+// and issues the loads of both before computing either, so that twice the
+// bytes are in flight per wave (the kernel is bound by HBM latency x bytes
+// in flight, not by arithmetic).
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+           const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    __shared__ double red[BR ? kOrbMaxOps * WPB * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    v0 += c * cstride + lane;
+    if (BR) v1 += c * cstride + lane;
+    cs += (long long)c * M * kLanes + lane;
+    const int nk = a.k1 - a.k0;
+    double C[kOrbMaxOps], S[kOrbMaxOps], acc[kOrbMaxOps];
+#pragma unroll
+    for (int kk = 0; kk < kOrbMaxOps; kk++) {
+        acc[kk] = 0;
+        C[kk] = 1;
+        S[kk] = 0;
+        if (kk < nk) {
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const double2 t = cs[(long long)k * kLanes];
+            C[kk] = t.x;
+            S[kk] = t.y;
+        }
+    }
+    const int total = a.nAc * a.nBc;
+    const int stride = gridDim.x * WPB;
+    for (int t = blockIdx.x * WPB + wv; t < total; t += 2 * stride) {
+        LaneOrbit o0, o1;
+        lane_orbit_open(a, t, o0);
+        const bool has1 = t + stride < total;
+        if (has1) lane_orbit_open(a, t + stride, o1);
+        else o1.live = false;
+        if (!o0.live && !o1.live) continue;
+        if (!o1.live) {
+#pragma unroll
+            for (int p = 0; p < 4; p++) o1.ok[p] = false;
+        }
+        double x0[4], l0[4], x1[4], l1[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            x0[p] = o0.ok[p] ? v0[o0.idx[p]] : 0.0;
+            x1[p] = o1.ok[p] ? v0[o1.idx[p]] : 0.0;
+            if (BR) {
+                l0[p] = o0.ok[p] ? v1[o0.idx[p]] : 0.0;
+                l1[p] = o1.ok[p] ? v1[o1.idx[p]] : 0.0;
+            }
+        }
+        if (o0.live) lane_orbit_apply<BR>(a, nk, o0, x0, l0, C, S, acc);
+        if (o1.live) lane_orbit_apply<BR>(a, nk, o1, x1, l1, C, S, acc);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (o0.ok[p]) {
+                v0[o0.idx[p]] = x0[p];
+                if (BR) v1[o0.idx[p]] = l0[p];
+            }
+            if (o1.ok[p]) {
+                v0[o1.idx[p]] = x1[p];
+                if (BR) v1[o1.idx[p]] = l1[p];
+            }
+        }
+    }
+    if (BR) lanes_block_partials<kOrbMaxOps>(acc, nk, a.k0, a.k1, a.reverse, red, partial, c, M, slots);
+}
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_opv(double *__restrict__ v0, double *__restrict__ v1, long long cstride, long long N, DevSector sec,
+          const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, int M, double *__restrict__ partial,
+          int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    __shared__ double red[BR ? WPB * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    v0 += c * cstride + lane;
+    if (BR) v1 += c * cstride + lane;
+    const DevOp o = ops[k];
+    const double2 t = cs[((long long)c * M + k) * kLanes + lane];
+    const double C = t.x, S = t.y;
+    double acc[1] = {0};
+    for (long long i = blockIdx.x * WPB + wv; i < N; i += (long long)gridDim.x * WPB) {
+        uint32_t b2;
+        const uint32_t b = sec.states[i];
+        const int s = pair_sign(o, b, b2);
+        if (s == 0 || b2 < b) continue;
+        const long long j = sec.rank(b2);
+        if (j < 0) continue;
+        const double x = v0[i * kLanes], y = v0[j * kLanes];
+        if (BR) {
+            const double lx = v1[i * kLanes], ly = v1[j * kLanes];
+            acc[0] += s * (lx * y - ly * x);
+            v1[i * kLanes] = s * ly * S + lx * C;
+            v1[j * kLanes] = -s * lx * S + ly * C;
+        }
+        v0[i * kLanes] = s * y * S + x * C;
+        v0[j * kLanes] = -s * x * S + y * C;
+    }
+    if (BR) lanes_block_partials<1>(acc, 1, k, k + 1, 0, red, partial, c, M, slots);
+}
+
+template __global__ void k_orbv<false>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orbv<true>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_opv<false>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                      const double2 *, int, double *, int);
+template __global__ void k_opv<true>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                     const double2 *, int, double *, int);
+
+// Vector-major [v][N] <-> lane layout, through a 64 x 64 LDS tile.
+__global__ void __launch_bounds__(kBlock)
+    k_to_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nvec)
+{
+    __shared__ double tile[kLanes][kLanes + 1];
+    const int c = blockIdx.y;
+    const long long i0 = blockIdx.x * (long long)kLanes;
+    const int tx = threadIdx.x & (kLanes - 1), ty = threadIdx.x / kLanes;
+    for (int r = ty; r < kLanes; r += kBlock / kLanes) {  // r = lane (vector), tx = amplitude
+        const int v = c * kLanes + r;
+        const long long i = i0 + tx;
+        tile[r][tx] = (v < nvec && i < N) ? src[(long long)v * N + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < kLanes; r += kBlock / kLanes) {  // r = amplitude, tx = lane
+        const long long i = i0 + r;
+        if (i < N) dst[((long long)c * N + i) * kLanes + tx] = tile[tx][r];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_from_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nvec)
+{
+    __shared__ double tile[kLanes][kLanes + 1];
+    const int c = blockIdx.y;
+    const long long i0 = blockIdx.x * (long long)kLanes;
+    const int tx = threadIdx.x & (kLanes - 1), ty = threadIdx.x / kLanes;
+    for (int r = ty; r < kLanes; r += kBlock / kLanes) {  // r = amplitude, tx = lane
+        const long long i = i0 + r;
+        tile[r][tx] = (i < N) ? src[((long long)c * N + i) * kLanes + tx] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < kLanes; r += kBlock / kLanes) {  // r = lane (vector), tx = amplitude
+        const int v = c * kLanes + r;
+        const long long i = i0 + tx;
+        if (v < nvec && i < N) dst[(long long)v * N + i] = tile[tx][r];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_bcast_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nchunks)
+{
+    const long long total = N * kLanes * nchunks;
+    for (long long t = blockIdx.x * (long long)kBlock + threadIdx.x; t < total; t += (long long)gridDim.x * kBlock)
+        dst[t] = src[(t / kLanes) % N];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_reduce_lanes(const double *__restrict__ part, int M, int slots, int nvec, double *__restrict__ out,
+                   double scale)
+{
+    // block per (chunk, operator); thread = lane x slot group
+    const int c = blockIdx.x / M, k = blockIdx.x - c * M;
+    const int lane = threadIdx.x & (kLanes - 1), g = threadIdx.x / kLanes;
+    constexpr int G = kBlock / kLanes;
+    __shared__ double red[G][kLanes];
+    const double *p = part + ((long long)c * M + k) * slots * kLanes;
+    double s = 0;
+    for (int j = g; j < slots; j += G) s += p[(long long)j * kLanes + lane];
+    red[g][lane] = s;
+    __syncthreads();
+    if (g == 0) {
+        double tot = 0;
+#pragma unroll
+        for (int w = 0; w < G; w++) tot += red[w][lane];
+        const int v = c * kLanes + lane;
+        if (v < nvec) out[(long long)v * M + k] = scale * tot;
+    }
+}
+
 #define AE_SEG(NV, BR)                                                                                       \
     template __global__ void k_seg<NV, BR>(SegArgs, double *, double *, long long, const double2 *, long long, \
                                            double *, int, long long);

@@ -126,6 +126,18 @@ struct OrbArgs {
     const DevOp *ops;
     int k0, k1;
     int reverse;
+    // Orbit program of the segment (lane-layout kernel): an orbit's type is
+    // (alpha shape, beta shape), shape 0 = two members, 1 = one member with
+    // the segment's bits empty, 2 = one member with them full.  Word
+    // prog[type * kOrbMaxOps + kk] for operator k0 + kk: bit r (r < 6) marks
+    // member pair r = (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) as a rotation pair,
+    // bit 8 + r its phase -1 seen from the first member, phase computed with
+    // the spectator bits cleared.  The spectator part is
+    // (-1)^popc(spectators & mask) per operator.
+    const uint32_t *prog;
+    uint32_t uaLocal;   // the segment's alpha bits within an alpha string
+    uint32_t ubLocal;   // its beta bits within a beta string
+    uint32_t spect;     // full-string mask of the spectator bits
 };
 
 // ---------------------------------------------------------------- kernels
@@ -146,6 +158,32 @@ __global__ void k_orbl(OrbArgs a, double *__restrict__ v0, double *__restrict__ 
                        long long pstride);
 // dst[v*n + i] = src[i] for v < nvec
 __global__ void k_bcast(const double *__restrict__ src, double *__restrict__ dst, long long n, int nvec);
+
+// Lane layout of a batch: vectors in chunks of 64, amplitude i of vector
+// 64c + l at [(c*N + i)*64 + l].  One wavefront's 64 lanes are the chunk's
+// 64 vectors, so every index, class and sign computation is wave-uniform
+// (scalar unit) and each amplitude access is one contiguous 512-byte load.
+// Angle tables: cs[(c*M + k)*64 + l]; bracket partials:
+// partial[((c*M + k)*slots + block)*64 + l], one row per block and operator
+// of the segment (every block of the fixed `slots`-block grid writes its row).
+constexpr int kLanes = 64;
+
+template <bool BR>
+__global__ void k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+                       const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
+// one operator on whole lane-layout vectors (any sector), wave per amplitude
+template <bool BR>
+__global__ void k_opv(double *__restrict__ v0, double *__restrict__ v1, long long cstride, long long N, DevSector sec,
+                      const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, int M,
+                      double *__restrict__ partial, int slots);
+// [v][N] -> lane layout (lanes >= nvec zero-filled) and back; grid (ceil(N/64), chunks)
+__global__ void k_to_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nvec);
+__global__ void k_from_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nvec);
+// dst (lane layout, nchunks chunks) = src in every lane
+__global__ void k_bcast_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nchunks);
+// out[v*M + k] = scale * sum over blocks of the partial rows; grid chunks*M
+__global__ void k_reduce_lanes(const double *__restrict__ part, int M, int slots, int nvec, double *__restrict__ out,
+                               double scale);
 
 // One operator on whole vectors (any sector); BR = also rotate v1 and
 // accumulate the adjoint bracket s*(v1_b v0_b' - v1_b' v0_b) before the

@@ -48,6 +48,11 @@ public:
     bool optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
                   bool useStop);
     bool getKernelTiming(int cls, double &ms, long long &launches);
+    // Execution choices of the device engine (EngineOption) and the
+    // autotuner that picks them for a batch shape (autotune.cpp).
+    bool setEngineOption(int key, int value);
+    bool getEngineOption(int key, int &value);
+    bool autotune(int nvec, int nangles, const double *angles, const char *cacheFile, int reps, std::string &report);
 
     int numberOfParameters() const { return m_numberOfUniqueParameters; }
     int batchAngleCount() const { return m_batchNangles; }

@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -40,6 +41,10 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--storage", type=int, default=0,
                    help="Hamiltonian form: 0 automatic, 1 explicit rows, 2 matrix-free sigma")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the library's default sweep kind, fusion depth and batch layout")
+    p.add_argument("--calib-copy", action="store_true",
+                   help="run one plain 2 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -80,6 +85,15 @@ def main():
     from ansatzevolve_amd.workloads import make_workload
 
     load_library()
+    if args.calib_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: %d bytes read, %d written" % (src.numel() * 8, dst.numel() * 8),
+              file=sys.stderr, flush=True)
+        del src, dst
     w = make_workload(args.config)
     N, M, P = w["N"], w["M"], w["P"]
     rng = np.random.default_rng(w["seed"] + 3)
@@ -119,19 +133,37 @@ def main():
                         for c, name in ((0, "hamiltonian"), (1, "forward_sweep"), (2, "adjoint_sweep"))}
     a.set_kernel_timing(False)
 
+    tune = None
+    if not args.no_autotune:
+        print("[bench] autotuning the batch (sweep kind x fusion depth x layout)", file=sys.stderr, flush=True)
+        rep = a.autotune(thetas, reps=2)
+        tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
+                "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
+                                round(c["ms"], 3)] for c in rep.get("candidates", [])],
+                "candidate_fields": ["sweep", "depth", "lane_min", "ms"]}
+    opts = {"sweep": a.get_option(a.OPT_SWEEP), "fuse_depth": a.get_option(a.OPT_FUSE_DEPTH),
+            "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH)}
+
+    # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
     for _ in range(args.warmup):
         a.run_batch(True)
-    a.download_batch(True)
-    a.set_kernel_timing(True)
     a.download_batch(True)  # drains the stream
-    a.set_kernel_timing(True)
+    print("[bench] timing %d steps of %d vectors" % (args.steps, args.nvec), file=sys.stderr, flush=True)
     t_start = time.perf_counter()
     for _ in range(args.steps):
         a.run_batch(True)
     E, G = a.download_batch(True)  # waits for the stream
     t_end = time.perf_counter()
     elapsed = t_end - t_start
+
+    # kernel-class breakdown: the same sequence launched eagerly with HIP
+    # events around each class on the engine's stream
+    nbd = max(1, min(args.steps, 3))
+    a.set_kernel_timing(True)
+    for _ in range(nbd):
+        a.run_batch(True)
+    a.download_batch(True)
     ham_ms, ham_n = a.kernel_timing(0)
     fwd_ms, fwd_n = a.kernel_timing(1)
     rev_ms, rev_n = a.kernel_timing(2)
@@ -143,15 +175,28 @@ def main():
 
     evals = args.steps * args.nvec
     value = evals / elapsed
-    # dominant kernel: Hamiltonian application, one launch = all vectors of a
-    # batch against one stream of the explicit rows.  Algorithmic bytes per
-    # launch: 12 B per stored entry (8 B value + 4 B column) + 16 B per
-    # amplitude per vector (x read, y written).
     nnz = N * w_row_width(w)
-    nv_launch = args.nvec if args.nvec % 8 == 0 else 1
-    bytes_launch = 12.0 * nnz + 16.0 * N * nv_launch
     avg_ms = ham_ms / max(ham_n, 1)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    vec_per_launch = args.nvec / max(1.0, ham_n / nbd)
+    if args.storage == 2:
+        # matrix-free sigma: one launch = all vectors; algorithmic work =
+        # 2 flops per Hamiltonian entry per vector (FP64, vector ALU)
+        flops = 2.0 * nnz * vec_per_launch
+        roof = {"bound": "mfma", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "kernel": "k_mf_sigma (matrix-free sigma, %d vectors)" % vec_per_launch,
+                "algorithmic_flops_per_launch": flops,
+                "note": "FP64 vector-ALU work priced against the FP64 dense peak"}
+    else:
+        # explicit rows: 12 B per stored entry (8 B value + 4 B column) per
+        # launch + 16 B per amplitude per vector (x read, y written)
+        bytes_launch = 12.0 * nnz + 16.0 * N * vec_per_launch
+        roof = {"bound": "hbm", "achieved": bytes_launch / (avg_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "kernel": "k_ell (Hamiltonian rows x %d vectors)" % vec_per_launch,
+                "algorithmic_bytes_per_launch": bytes_launch}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches"] = ham_n
     result = {
         "metric": "TUPS E+grad evals/s at 0.85M dets",
         "value": value,
@@ -175,20 +220,11 @@ def main():
             "determinants": N,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_ell (Hamiltonian rows x %d vectors)" % nv_launch,
-            "avg_launch_ms": avg_ms,
-            "launches": ham_n,
-            "algorithmic_bytes_per_launch": bytes_launch,
-        },
-        "breakdown_ms_per_step": {"hamiltonian": ham_ms / args.steps, "forward_sweep": fwd_ms / args.steps,
-                                  "adjoint_sweep": rev_ms / args.steps},
+        "roofline": roof,
+        "breakdown_ms_per_step": {"hamiltonian": ham_ms / nbd, "forward_sweep": fwd_ms / nbd,
+                                  "adjoint_sweep": rev_ms / nbd},
+        "engine_options": opts,
+        "autotune": tune,
         "single_vector_egrad_per_s": single_rate,
         "single_vector_breakdown_ms": single_breakdown,
         "setup_s": setup_s,

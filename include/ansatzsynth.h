@@ -168,6 +168,33 @@ int optimise(int nangles, double *angles, double *energy, double stopenergy, int
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);
 
+/* Execution choices of the device engine; none changes a result beyond the
+ * order of floating-point sums.  Keys:
+ *   0 sweep kind: 0 register orbits (+ LDS tiles), 1 LDS-staged orbits,
+ *     2 LDS tiles only;
+ *   1 lane-layout threshold: batches of >= value vectors sweep with one
+ *     vector per wavefront lane (0: never);
+ *   2 hipGraph replay: 1 on, 0 eager launches;
+ *   3 fusion depth: most operators per pass (0: the kernels' own limit);
+ *   4 LDS budget in bytes of a tile segment (0: the device's limit).
+ * Changing keys 0, 3 or 4 re-segments the operators; an uploaded angle
+ * batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
+ * 5 rejected value or failure. */
+int setEngineOption(int key, int value, void *ctx);
+int getEngineOption(int key, int *value, void *ctx);
+
+/* Autotuner of the batched E+grad sequence for nvec angle vectors
+ * (nangles = operators or parameters per vector): runs every candidate
+ * (sweep kind x fusion depth x batch layout), reps timed replays each,
+ * checks that all candidates give the same energies and derivatives, and
+ * leaves the fastest applied.  If cachefile is not NULL, a recorded choice
+ * for this shape (sector dimension, operators, Hamiltonian form, nvec) is
+ * applied without measuring, and a new choice is appended as one JSON line.
+ * report (if not NULL) receives a JSON summary, truncated to reportlen - 1
+ * characters.  1 ctx NULL, 2 angles NULL, 3 nvec < 1, 5 failure. */
+int autotune(int nvec, int nangles, const double *angles, const char *cachefile, int reps, char *report,
+             int reportlen, void *ctx);
+
 /* ---------------- Part 3: learned warm start (no reference counterpart) ----
  * A 3-layer FP32 network x -> tanh(x W0 + b0) -> tanh(. W1 + b1) -> . W2 + b2
  * proposing initial angles from integral features, with device kernels for

@@ -1,0 +1,167 @@
+"""Batched evaluation through the C ABI: the lane layout (one angle vector
+per wavefront lane) against the vector-major layout, the single-vector path
+and the CPU oracle; hipGraph replays of the batch; the autotuner's
+invariance across its candidates and its recorded choice."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from ansatzevolve_amd import AnsatzSynth, from_workload
+from ansatzevolve_amd.workloads import hf_index, make_workload, seeded_angles, synthetic_integrals, tups_operators
+from oracle.oracle import OracleEngine
+
+pytestmark = pytest.mark.gpu
+
+E_TOL, G_TOL = 1e-10, 1e-9
+SAME = 1e-12  # layouts and fusion choices differ only in summation order
+
+
+def _tups_case(n, layers, seed, nvec):
+    h, g = synthetic_integrals(n, seed)
+    ops, order, scale = tups_operators(n, layers)
+    w = dict(n=n, nq=2 * n, h=h, g=g, ops=ops, order=order, scale=scale, start_idx=hf_index(n))
+    thetas = seeded_angles(int(order.max()) + 1, seed + 7, count=nvec)
+    return w, thetas
+
+
+def _batch(a, thetas, lane_min):
+    a.set_option(AnsatzSynth.OPT_LANE_MIN, lane_min)
+    return a.energy_gradients(thetas)
+
+
+@pytest.mark.parametrize("nvec", [1, 17, 64, 70])
+def test_lane_layout_matches_vector_major_and_single(gpu, nvec):
+    """TUPS on 8 orbitals (orbit segments only); 70 vectors = one full and
+    one partial chunk of 64 lanes."""
+    w, thetas = _tups_case(8, 3, 4242, nvec)
+    with from_workload(w) as a:
+        El, Gl = _batch(a, thetas, 1)
+        Ev, Gv = _batch(a, thetas, 0)
+        assert np.max(np.abs(El - Ev)) <= SAME
+        assert np.max(np.abs(Gl - Gv)) <= SAME
+        for v in sorted({0, nvec // 2, nvec - 1}):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(El[v] - Es) <= SAME
+            assert np.max(np.abs(Gl[v] - gs)) <= SAME
+
+
+def test_lane_layout_generic_operators(gpu):
+    """Same-spin doubles over four alpha orbitals and singles spanning
+    three orbitals: segments that are not orbits run one operator per pass
+    (k_opv) in the lane layout.  Checked against the oracle per vector."""
+    n = 5
+    h, g = synthetic_integrals(n, 515)
+    ops = np.array([[n + 0, n + 1, n + 2, n + 3],   # alpha-alpha double
+                    [0, 2, -1, -1], [n + 0, n + 2, -1, -1],
+                    [0, n + 0, 1, n + 1],           # up-down pair double
+                    [1, 2, 3, 4],                   # beta-beta double
+                    [n + 1, n + 4, -1, -1], [2, 4, -1, -1]], dtype=np.int64)
+    order = np.array([0, 1, 1, 2, 3, 4, 5])
+    scale = np.array([1.0, 1.0, -0.5, 1.0, 0.7, 1.0, 2.0])
+    start = int(sum(1 << i for i in (0, 1, n + 0, n + 1)))  # two beta, two alpha electrons
+    w = dict(n=n, nq=2 * n, h=h, g=g, ops=ops, order=order, scale=scale, start_idx=start)
+    o = OracleEngine(2 * n, [start], [1.0], ops, order, scale)
+    o.set_integrals(h, g, explicit=True)
+    thetas = seeded_angles(6, 99, count=33)
+    with from_workload(w) as a:
+        El, Gl = _batch(a, thetas, 1)
+        Ev, Gv = _batch(a, thetas, 0)
+        assert np.max(np.abs(El - Ev)) <= SAME and np.max(np.abs(Gl - Gv)) <= SAME
+        for v in (0, 16, 32):
+            Eo, go = o.energy_gradient(thetas[v])
+            assert abs(El[v] - Eo) <= E_TOL
+            assert np.max(np.abs(Gl[v] - go)) <= G_TOL
+
+
+def test_lane_layout_number_sector(gpu):
+    """Operators that move an electron between spins: the particle-number
+    sector (no S_z tiles), explicit triplet Hamiltonian."""
+    nq, nel = 8, 4
+    states = [b for b in range(1 << nq) if bin(b).count("1") == nel]
+    rng = np.random.default_rng(31)
+    Hd = rng.normal(size=(len(states), len(states)))
+    Hd = 0.5 * (Hd + Hd.T)
+    ii, jj = np.nonzero(np.abs(Hd) > 0.8)
+    ti = np.array([states[i] for i in ii])
+    tj = np.array([states[j] for j in jj])
+    tv = Hd[ii, jj]
+    ops = np.array([[0, 4, -1, -1], [1, 5, -1, -1], [0, 1, 6, 7], [2, 5, -1, -1], [3, 6, 2, 7]])
+    order = np.array([0, 1, 2, 3, 1])
+    start = 0b00001111
+    o = OracleEngine(nq, [start], [1.0], ops, order)
+    o.set_triplets(ti, tj, tv)
+    thetas = seeded_angles(4, 5, count=20)
+    with AnsatzSynth() as a:
+        a.set_initial_state(nq, [start + 1], [1.0])
+        a.set_hamiltonian(ti + 1, tj + 1, tv)
+        a.set_excitation(ops + 1, order + 1)
+        El, Gl = _batch(a, thetas, 1)
+        Ev, Gv = _batch(a, thetas, 0)
+        assert np.max(np.abs(El - Ev)) <= SAME and np.max(np.abs(Gl - Gv)) <= SAME
+        for v in (0, 7, 19):
+            Eo, go = o.energy_gradient(thetas[v])
+            assert abs(El[v] - Eo) <= E_TOL
+            assert np.max(np.abs(Gl[v] - go)) <= G_TOL
+
+
+@pytest.mark.parametrize("lane_min", [1, 0])
+def test_batch_graph_replays(gpu, lane_min):
+    """The batch sequence is captured once per shape and replayed; new angle
+    uploads of the same size, eager single-vector calls in between, and
+    every replay must equal the eager (graph off) batch."""
+    w, thetas = _tups_case(6, 3, 808, 24)
+    other = seeded_angles(thetas.shape[1], 809, count=24)
+    with from_workload(w) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, lane_min)
+        a.set_option(AnsatzSynth.OPT_GRAPH, 0)
+        ref = {k: a.energy_gradients(t) for k, t in (("a", thetas), ("b", other))}
+        a.set_option(AnsatzSynth.OPT_GRAPH, 1)
+        for rep, k in itertools.product(range(3), ("a", "b")):
+            t = thetas if k == "a" else other
+            a.upload_batch(t)
+            a.run_batch(True)
+            E, G = a.download_batch(True)
+            assert np.max(np.abs(E - ref[k][0])) <= SAME, (rep, k)
+            assert np.max(np.abs(G - ref[k][1])) <= SAME, (rep, k)
+            a.energy_gradient(t[rep])  # other device work between replays
+
+
+def test_options_and_fusion_depth(gpu):
+    """Every sweep kind and fusion depth gives the same batch."""
+    w, thetas = _tups_case(6, 2, 31, 20)
+    with from_workload(w) as a:
+        base = a.energy_gradients(thetas)
+        for sweep, depth, lanes in itertools.product((0, 1, 2), (0, 1, 2, 3), (0, 1)):
+            a.set_option(AnsatzSynth.OPT_SWEEP, sweep)
+            a.set_option(AnsatzSynth.OPT_FUSE_DEPTH, depth)
+            a.set_option(AnsatzSynth.OPT_LANE_MIN, lanes)
+            assert a.get_option(AnsatzSynth.OPT_SWEEP) == sweep
+            E, G = a.energy_gradients(thetas)
+            assert np.max(np.abs(E - base[0])) <= SAME, (sweep, depth, lanes)
+            assert np.max(np.abs(G - base[1])) <= SAME, (sweep, depth, lanes)
+
+
+def test_autotune_invariance_and_cache(gpu, tmp_path):
+    """10 orbitals, 63,504 determinants: every candidate agrees, the choice
+    is applied and recorded, and a second call re-applies it from the file."""
+    w = make_workload(2)
+    rng = np.random.default_rng(5)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-np.pi, np.pi, size=(16, w["P"]))
+    cache = str(tmp_path / "tune.jsonl")
+    with from_workload(w) as a:
+        rep = a.autotune(thetas, cache=cache, reps=2)
+        assert rep["source"] == "measured"
+        assert len(rep["candidates"]) >= 6
+        assert rep["max_dev"] <= SAME
+        ch = rep["choice"]
+        assert a.get_option(AnsatzSynth.OPT_SWEEP) == ch["sweep"]
+        assert a.get_option(AnsatzSynth.OPT_FUSE_DEPTH) == ch["depth"]
+        E, G = a.energy_gradients(thetas)
+        Es, gs = a.energy_gradient(thetas[3])
+        assert abs(E[3] - Es) <= SAME and np.max(np.abs(G[3] - gs)) <= SAME
+        rep2 = a.autotune(thetas, cache=cache)
+        assert rep2["source"] == "cache" and rep2["choice"] == ch
+    with open(cache) as f:
+        assert len(f.readlines()) == 1
