@@ -111,3 +111,221 @@ class WarmStartMLP:
         loss = ctypes.c_float()
         assert self.lib.mlpTrainStep(self.h, X.shape[0], xp, tp, mp, ctypes.c_float(lr), ctypes.byref(loss)) == 0
         return loss.value
+
+
+# ------------------------------------------------------------------ pipeline
+# Instances of the learned warm-start workload: n spatial orbitals at half
+# filling, site spacing s ~ U(1.0, 2.5), the seeded integral generator, TUPS
+# with L = n / 2 layers, the alternating determinant as the start.
+WARM_SEED = 20260500
+
+
+def warm_instance(index, n):
+    from .workloads import hf_index, synthetic_integrals, tups_operators
+    rng = np.random.default_rng(WARM_SEED + 1000 * n + index)
+    spacing = float(rng.uniform(1.0, 2.5))
+    h, g = synthetic_integrals(n, WARM_SEED + 100000 + 1000 * n + index, spacing=spacing)
+    ops, order, scale = tups_operators(n, n // 2)
+    return dict(n=n, nq=2 * n, h=h, g=g, ops=ops, order=order, scale=scale, start_idx=hf_index(n),
+                P=int(order.max()) + 1, spacing=spacing, index=index)
+
+
+def wrap_angles(t):
+    """Angles into (-pi, pi]: every TUPS rotation is 2 pi periodic."""
+    return np.pi - np.mod(np.pi - np.asarray(t, dtype=np.float64), 2 * np.pi)
+
+
+def time_to(trace, target):
+    """(seconds, Newton iterations) until the trace's energy <= target, or
+    (inf, inf) if it never gets there.  Trace rows: seconds, energy, |g|,
+    Hessian evaluations, energy evaluations."""
+    for i, row in enumerate(trace):
+        if row[1] <= target:
+            return float(row[0]), i
+    return float("inf"), float("inf")
+
+
+class WarmStart:
+    """Feature normalisation + the GPU network + angle padding for a set of
+    orbital counts; predicts per-parameter angles for an instance."""
+
+    def __init__(self, nmax, pmax, hidden=256, seed=0):
+        self.nmax, self.pmax = nmax, pmax
+        self.nin = 5 * nmax * nmax
+        self.net = WarmStartMLP(self.nin, pmax, hidden=hidden, seed=seed)
+        self.mu = np.zeros(self.nin)
+        self.sd = np.ones(self.nin)
+
+    def close(self):
+        self.net.close()
+
+    def inputs(self, insts):
+        X = np.array([features(w["h"], w["g"], self.nmax) for w in insts])
+        return ((X - self.mu) / self.sd).astype(np.float32)
+
+    def fit(self, insts, targets, epochs=300, lr=1e-3, batch=64, seed=0):
+        """Masked MSE on wrapped optimised angles, Adam on the GPU."""
+        X = np.array([features(w["h"], w["g"], self.nmax) for w in insts])
+        self.mu, self.sd = X.mean(0), X.std(0) + 1e-8
+        Xn = ((X - self.mu) / self.sd).astype(np.float32)
+        T = np.zeros((len(insts), self.pmax), np.float32)
+        Mk = np.zeros_like(T)
+        for i, t in enumerate(targets):
+            T[i, :len(t)] = wrap_angles(t)
+            Mk[i, :len(t)] = 1
+        rng = np.random.default_rng(seed)
+        losses = []
+        for ep in range(epochs):
+            perm = rng.permutation(len(insts))
+            for b in range(0, len(perm), batch):
+                j = perm[b:b + batch]
+                losses.append(self.net.train_step(Xn[j], T[j], Mk[j], lr))
+        return losses
+
+    def predict(self, insts):
+        Y = self.net.forward(self.inputs(insts)).astype(np.float64)
+        return [Y[i, :w["P"]] for i, w in enumerate(insts)]
+
+
+def solve(w, theta0, stop_energy=float("nan")):
+    """Plain GPU Newton from theta0; returns (theta, E, trace)."""
+    from . import from_workload
+    with from_workload(w) as a:
+        return a.optimise(np.asarray(theta0, dtype=np.float64), stop_energy=stop_energy)
+
+
+def start_energies(w, thetas):
+    """Exact FP64 energies of candidate starts (one batched launch)."""
+    from . import from_workload
+    with from_workload(w) as a:
+        return a.energies(np.atleast_2d(thetas))
+
+
+def finetune_energy(model, insts, steps=300, lr=3e-4, log=None):
+    """Second training stage: minimise the mean exact FP64 energy of the
+    network's proposals over the training instances.  dE/dtheta comes from
+    the library's adjoint gradient on the GPU; it enters the network's
+    backward pass through the masked-MSE kernel with the target
+    t = y - (sum m) / (2 B) * dE/dtheta, whose loss gradient is exactly
+    (1 / B) dE/dtheta."""
+    from . import from_workload
+    engines = [from_workload(w) for w in insts]
+    X = model.inputs(insts)
+    B = len(insts)
+    Mk = np.zeros((B, model.pmax), np.float32)
+    for i, w in enumerate(insts):
+        Mk[i, :w["P"]] = 1
+    msum = float(Mk.sum())
+    hist = []
+    try:
+        for step in range(steps):
+            Y = model.net.forward(X).astype(np.float64)
+            G = np.zeros((B, model.pmax))
+            Es = np.zeros(B)
+            for i, (w, a) in enumerate(zip(insts, engines)):
+                Es[i], G[i, :w["P"]] = a.energy_gradient(Y[i, :w["P"]])
+            T = (Y - msum / (2.0 * B) * G).astype(np.float32)
+            model.net.train_step(X, T, Mk, lr)
+            hist.append(float(Es.mean()))
+            if log and step % 50 == 0:
+                log("[warmstart] energy fine-tuning step %d: mean E %.6f" % (step, hist[-1]))
+    finally:
+        for a in engines:
+            a.close()
+    return hist
+
+
+def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidden=256, log=print):
+    """Generate (plain optimiser from the standard guess theta = 0), train
+    (supervised on the optimised angles, then energy fine-tuning), and
+    evaluate time-to-1e-6 Ha on the held-out instances with the fallback
+    rule: the network's proposal is used only if its exact energy is not
+    above the standard guess's.  The target energy of an instance is the
+    lower of the two runs' final energies plus 1e-6 Ha; a run that ends in a
+    higher minimum never reaches it.  Returns a JSON-able summary."""
+    import time
+    insts = []
+    for n in ns:
+        insts += [warm_instance(i, n) for i in range(count)]
+    rng = np.random.default_rng(WARM_SEED)
+    order = rng.permutation(len(insts))
+    insts = [insts[i] for i in order]
+    ntr = int(train_frac * len(insts))
+    train, held = insts[:ntr], insts[ntr:]
+    t0 = time.time()
+    plain = []
+    for k, w in enumerate(insts):
+        plain.append(solve(w, np.zeros(w["P"])))
+        if k % 20 == 0:
+            log("[warmstart] plain solves %d/%d (%.0f s)" % (k + 1, len(insts), time.time() - t0))
+    gen_s = time.time() - t0
+    model = WarmStart(max(ns), max(w["P"] for w in insts), hidden=hidden)
+    t1 = time.time()
+    losses = model.fit(train, [plain[k][0] for k in range(ntr)], epochs=epochs)
+    log("[warmstart] supervised stage: %d steps, loss %.4f -> %.4f" % (len(losses), losses[0], losses[-1]))
+    ehist = finetune_energy(model, train, steps=energy_steps, log=log) if energy_steps else []
+    train_s = time.time() - t1
+    t2 = time.time()
+    props = model.predict(held)
+    infer_s = (time.time() - t2) / max(1, len(held))
+    rows = []
+    for k, (w, p) in enumerate(zip(held, props)):
+        ts = time.time()
+        Es = start_energies(w, np.stack([np.zeros(w["P"]), p]))
+        use = bool(Es[1] <= Es[0])
+        choose_s = time.time() - ts + infer_s
+        th1, E1, tr1 = solve(w, p if use else np.zeros(w["P"]))
+        th0, E0, tr0 = plain[ntr + k]
+        best = min(E0, E1)
+        t_plain, i_plain = time_to(tr0, best + 1e-6)
+        t_mlp, i_mlp = time_to(tr1, best + 1e-6)
+        rows.append(dict(n=w["n"], index=w["index"], E_std_start=float(Es[0]), E_mlp_start=float(Es[1]),
+                         used_mlp=use, E_plain=float(E0), E_mlp=float(E1), t_plain=t_plain, t_mlp=t_mlp + choose_s,
+                         it_plain=i_plain, it_mlp=i_mlp, t_plain_conv=float(tr0[-1][0]) if len(tr0) else 0.0,
+                         t_mlp_conv=float(tr1[-1][0]) + choose_s if len(tr1) else 0.0,
+                         gap_std=float(Es[0] - best), gap_start=float(min(Es[0], Es[1]) - best)))
+    model.close()
+
+    def med(key, sel=None):
+        v = np.array([r[key] for r in rows if sel is None or sel(r)], dtype=float)
+        return float(np.median(v)) if len(v) else None
+
+    def summ(sel=None):
+        rs = [r for r in rows if sel is None or sel(r)]
+        return dict(count=len(rs), median_t_plain=med("t_plain", sel), median_t_mlp=med("t_mlp", sel),
+                    median_it_plain=med("it_plain", sel), median_it_mlp=med("it_mlp", sel),
+                    plain_misses=int(sum(np.isinf(r["t_plain"]) for r in rs)),
+                    mlp_misses=int(sum(np.isinf(r["t_mlp"]) for r in rs)),
+                    proposals_used=int(sum(r["used_mlp"] for r in rs)),
+                    median_start_gap_std=med("gap_std", sel), median_start_gap_chosen=med("gap_start", sel))
+    summary = dict(ns=list(ns), instances=len(insts), train=ntr, held_out=len(held), generate_s=gen_s,
+                   train_s=train_s, supervised_loss=[float(losses[0]), float(losses[-1])],
+                   energy_finetune_mean_E=[ehist[0], ehist[-1]] if ehist else None, all=summ(), rows=rows)
+    for n in ns:
+        summary["n%d" % n] = summ(lambda r, n=n: r["n"] == n)
+    return summary
+
+
+def _main():
+    import argparse
+    import json
+    import sys
+    p = argparse.ArgumentParser(description="learned warm start: generate, train, evaluate")
+    p.add_argument("--ns", type=int, nargs="+", default=[6, 8])
+    p.add_argument("--count", type=int, default=80, help="instances per orbital count")
+    p.add_argument("--epochs", type=int, default=300, help="supervised epochs")
+    p.add_argument("--energy-steps", type=int, default=300, help="energy fine-tuning steps")
+    p.add_argument("--out", default="")
+    a = p.parse_args()
+    s = run_pipeline(a.ns, a.count, epochs=a.epochs, energy_steps=a.energy_steps,
+                     log=lambda m: print(m, file=sys.stderr, flush=True))
+    txt = json.dumps(s)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt)
+    brief = {k: v for k, v in s.items() if k != "rows"}
+    print(json.dumps(brief))
+
+
+if __name__ == "__main__":
+    _main()
