@@ -5,6 +5,7 @@
 // operator table and the sin/cos table -- is resident in HBM; a call moves
 // only the per-operator sin/cos (16 B per operator) in and the results out.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -124,6 +125,13 @@ struct EngineImpl {
     int W = 0;
     DBuf<int64_t> d_rowptr;
     DBuf<double> d_h, d_g, d_diag, d_FA, d_FB;  // matrix-free tables
+    // lane-layout matrix-free sigma (k_sigv_sparse + two DGEMMs)
+    bool sigv = false;
+    DBuf<double> d_Vaa, d_Vbb;
+    DBuf<uint32_t> d_ts;
+    int ntp = 0;
+    rocblas_handle blas = nullptr;
+    DBuf<double> bl_epart;
     DBuf<int32_t> d_col;
     DBuf<double> d_val;
 
@@ -599,6 +607,45 @@ struct EngineImpl {
         return true;
     }
 
+    // Lane-layout sigma for nch chunks and the per-vector energies
+    // out[v] = <x_v|y_v> (matrix-free storage, n <= 12).
+    bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err)
+    {
+        const int mk = mark_begin(0);
+        SigvArgs s;
+        s.m = mfargs();
+        s.ts = d_ts.p;
+        s.ntp = ntp;
+        s.x = x;
+        s.y = y;
+        s.cstride = N * kLanes;
+        const unsigned gx = (unsigned)((na + 7) / 8 * 64);
+        hipLaunchKernelGGL(k_sigv_sparse, dim3(gx, nch), dim3(kSigThreads), kSigLds, stream, s);
+        LCHK();
+        const double one = 1.0;
+        // beta part: per alpha row, Y_a (nb x 64) += V_bb X_a  (column-major view: Y_a^T += X_a^T V_bb^T)
+        if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, kLanes, nb, nb, &one,
+                                          x, kLanes, (rocblas_stride)nb * kLanes, d_Vbb.p, nb, 0, &one, y, kLanes,
+                                          (rocblas_stride)nb * kLanes, na * nch) != rocblas_status_success) {
+            err = "rocblas_dgemm_strided_batched (beta part) failed";
+            return false;
+        }
+        // alpha part: per chunk, Y (na x nb*64) += V_aa X
+        if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, nb * kLanes, na, na,
+                                          &one, x, nb * kLanes, (rocblas_stride)N * kLanes, d_Vaa.p, na, 0, &one, y,
+                                          nb * kLanes, (rocblas_stride)N * kLanes, nch) != rocblas_status_success) {
+            err = "rocblas_dgemm_strided_batched (alpha part) failed";
+            return false;
+        }
+        mark_end(mk);
+        const int G = 512;
+        hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, bl_epart.p);
+        LCHK();
+        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, stream, bl_epart.p, 1, G, nvec, out, 1.0);
+        LCHK();
+        return true;
+    }
+
     // y = M^T x for nvec vectors (stride N); energy partials epart[v][eslots].
     bool ham_apply(const double *x, double *y, int nvec, double *epart, std::string &err)
     {
@@ -658,6 +705,7 @@ Engine::Engine() : d_(new EngineImpl) {}
 Engine::~Engine()
 {
     if (d_) d_->drop_graphs();
+    if (d_ && d_->blas) (void)rocblas_destroy_handle(d_->blas);
     if (d_ && d_->stream) (void)hipStreamDestroy(d_->stream);
 }
 
@@ -979,6 +1027,41 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
         e.W = 0;
         e.nnz = W * e.N;  // implicit entries
         e.hkind = 3;
+        e.sigv = false;
+        if (n <= 12 && e.nb <= kSigMaxNb) {
+            // tables of the lane-layout sigma: dense same-spin rows, beta entries
+            const size_t na = e.na, nb = e.nb;
+            e.ntp = (kd * (n - kd) + kd + 3) / 4 * 4;
+            if (!e.d_Vaa.ensure(na * na, err) || !e.d_Vbb.ensure(nb * nb, err) ||
+                !e.d_ts.ensure(nb * e.ntp, err))
+                return false;
+            HCHK(hipMemsetAsync(e.d_Vaa.p, 0, na * na * sizeof(double), e.stream));
+            HCHK(hipMemsetAsync(e.d_Vbb.p, 0, nb * nb * sizeof(double), e.stream));
+            hipLaunchKernelGGL(k_dense_spin, dim3((unsigned)((na + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
+                               a, 1, e.d_Vaa.p);
+            LCHK();
+            hipLaunchKernelGGL(k_dense_spin, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
+                               a, 0, e.d_Vbb.p);
+            LCHK();
+            hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
+                               a, e.d_ts.p, e.ntp);
+            LCHK();
+            HCHK(hipStreamSynchronize(e.stream));
+            HCHK(hipFuncSetAttribute((const void *)k_sigv_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kSigLds));
+            if (!e.blas) {
+                if (rocblas_create_handle(&e.blas) != rocblas_status_success) {
+                    e.blas = nullptr;
+                    err = "rocblas_create_handle failed";
+                    return false;
+                }
+            }
+            if (rocblas_set_stream(e.blas, e.stream) != rocblas_status_success) {
+                err = "rocblas_set_stream failed";
+                return false;
+            }
+            e.sigv = true;
+        }
         return true;
     }
     if (!fitsExplicit) {
@@ -1142,7 +1225,7 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
     if (e.use_lanes(nvec)) {
         const size_t nch = (size_t)(nvec + kLanes - 1) / kLanes;
         if (!e.bl_psi.ensure(nch * e.N * kLanes, err) || !e.bl_lam.ensure(nch * e.N * kLanes, err) ||
-            !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err))
+            !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err))
             return false;
         if (!e.upload_cs_lanes(angles, nvec, err)) return false;
     } else {
@@ -1177,6 +1260,17 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
                                dim3(kBlock), 0, e.stream, e.d_start.p, e.bl_psi.p, e.N, nch);
             LCHK();
             if (!e.sweep_forward_lanes(e.bl_psi.p, nch, err)) return false;
+            if (e.hkind == 3 && e.sigv) {
+                // matrix-free sigma in lane layout: no transposes
+                if (!e.ham_apply_lanes(e.bl_psi.p, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
+                if (with_gradient) {
+                    if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
+                    hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
+                                       e.M, e.lslots, nvec, e.b_out.p + nvec, 2.0);
+                    LCHK();
+                }
+                return true;
+            }
             hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.bl_psi.p, e.b_psi.p, e.N, nvec);
             LCHK();
             if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
@@ -1217,7 +1311,10 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
     }
     if (!e.bGraph) {
         // buffers only grow between uploads of a batch size, so the captured
-        // addresses stay valid; a new size recaptures
+        // addresses stay valid; a new size recaptures.  One eager run first:
+        // library GEMM kernels load lazily and must not load under capture.
+        if (!enqueue()) return false;
+        HCHK(hipStreamSynchronize(e.stream));
         hipGraph_t g = nullptr;
         HCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
         const bool ok = enqueue();

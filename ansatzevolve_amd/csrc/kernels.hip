@@ -1076,6 +1076,210 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) epart[(long long)vb * a.na + ra] = tot;
 }
 
+// ------------------------------------------- matrix-free sigma, lane layout
+// Same-spin single/double parts as dense rows (k_dense_spin): row s of
+// V[s][s'] = signed value of every same-spin single (F table) and double
+// taking string s' to s.  Used as DGEMM operands for the alpha-alpha and
+// the beta-beta (alpha-independent) terms.
+__global__ void __launch_bounds__(kBlock) k_dense_spin(MfArgs a, int spin, double *__restrict__ V)
+{
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int cnt = spin ? a.na : a.nb;
+    if (s >= cnt) return;
+    const int n = a.n, off = spin ? n : 0;
+    const uint32_t full = (1u << n) - 1;
+    const uint32_t str = spin ? a.unA[s] : a.unB[s];
+    const int32_t *rk = spin ? a.rkA : a.rkB;
+    const double *F = spin ? a.FA : a.FB;
+    double *row = V + (size_t)s * cnt;
+    int t = 0;
+    for (uint32_t o = str; o; o &= o - 1)
+        for (uint32_t w = ~str & full; w; w &= w - 1, t++) {
+            const int r = __ffs(o) - 1, q = __ffs(w) - 1;
+            row[rk[str ^ (1u << r) ^ (1u << q)]] += F[(size_t)t * cnt + s];
+        }
+    const uint32_t js = str << off;
+    for (uint32_t o1 = str; o1; o1 &= o1 - 1) {
+        const int r1 = __ffs(o1) - 1;
+        for (uint32_t o2 = o1 & (o1 - 1); o2; o2 &= o2 - 1) {
+            const int r2 = __ffs(o2) - 1;
+            for (uint32_t w1 = ~str & full; w1; w1 &= w1 - 1) {
+                const int q1 = __ffs(w1) - 1;
+                for (uint32_t w2 = w1 & (w1 - 1); w2; w2 &= w2 - 1) {
+                    const int q2 = __ffs(w2) - 1;
+                    uint32_t m;
+                    const double tv = two_body(r1 + off, r2 + off, q1 + off, q2 + off, n, a.g, m);
+                    row[rk[str ^ (1u << r1) ^ (1u << r2) ^ (1u << q1) ^ (1u << q2)]] += signed_by(js, m, tv);
+                }
+            }
+        }
+    }
+}
+
+// Beta entry table of the lane-layout sigma: per beta string b, its singles
+// (occupied r ascending x empty q ascending) and then its diagonal entries
+// (r = q over occupied r), padded to a multiple of 4 with an entry whose
+// coefficient slot is zero.  Entry = (byte offset of the source beta J in a
+// staged row of kSigVC doubles) | (byte offset of the coefficient
+// (e + parity * kSigNN), e = r * n + q, in a coefficient row) << 16.
+__global__ void __launch_bounds__(kBlock) k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= a.nb) return;
+    const int n = a.n;
+    const uint32_t full = (1u << n) - 1, s = a.unB[b];
+    uint32_t *row = ts + (size_t)b * ntp;
+    int t = 0;
+    for (uint32_t o = s; o; o &= o - 1)
+        for (uint32_t w = ~s & full; w; w &= w - 1) {
+            const int r = __ffs(o) - 1, q = __ffs(w) - 1;
+            const int J = a.rkB[s ^ (1u << r) ^ (1u << q)];
+            const int e = r * n + q + (between_parity(s, r, q) ? kSigNN : 0);
+            row[t++] = (uint32_t)(J * kSigVC * 8) | ((uint32_t)(e * 8) << 16);
+        }
+    for (uint32_t o = s; o; o &= o - 1) {
+        const int r = __ffs(o) - 1;
+        row[t++] = (uint32_t)(b * kSigVC * 8) | ((uint32_t)((r * n + r) * 8) << 16);
+    }
+    for (; t < ntp; t++) row[t] = (uint32_t)(2 * kSigNN * 8) << 16;
+}
+
+// Lane-layout sigma, sparse part: for output (alpha, beta), the alpha-beta
+// doubles and the beta-diagonal part of the alpha singles through every
+// alpha single's source row, the alpha-dependent spectator part of the beta
+// singles (identity row), and the diagonal.  One workgroup per (alpha,
+// 8-vector sub-chunk); source rows are staged in LDS two at a time with their
+// signed coefficient rows; thread = (vector v, beta slot).  The LDS layout
+// is fixed so every read uses a compile-time offset:
+//   [coef row 0 | coef row 1 | x row 0 | x row 1], coef row = kSigGS doubles
+//   (+coef[e], -coef[e] at kSigNN + e, zero at 2 kSigNN), x row = kSigMaxNb x
+//   kSigVC doubles.
+__global__ void __launch_bounds__(kSigThreads) k_sigv_sparse(SigvArgs a)
+{
+    extern __shared__ double sm[];
+    const MfArgs &m = a.m;
+    const int n = m.n, nb = m.nb, nn = n * n;
+    // XCD-aware order: the 8 sub-chunks of one alpha row run on one XCD
+    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
+    const int ar = xcd + 8 * (k >> 3), sub = k & 7;
+    if (ar >= m.na) return;
+    const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigVC;
+    const double *__restrict__ x = a.x + coff;
+    double *__restrict__ y = a.y + coff;
+    char *lds = (char *)sm;
+    double *gs0 = sm, *gs1 = sm + kSigGS;
+    double *xs0 = sm + 2 * kSigGS, *xs1 = xs0 + kSigMaxNb * kSigVC;
+    const int tid = threadIdx.x, v = tid & (kSigVC - 1), slot = tid / kSigVC;
+    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const int ka = __popc(alpha), ne = n - ka;
+    const int R = 1 + ka * ne;
+    double acc[kSigBI];
+#pragma unroll
+    for (int i = 0; i < kSigBI; i++) acc[i] = 0;
+    for (int r0 = 0; r0 < R; r0 += 2) {
+        const bool two = r0 + 1 < R;
+        __syncthreads();
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && !two) break;
+            const int r = r0 + h;
+            double *gs = h ? gs1 : gs0;
+            double *xs = h ? xs1 : xs0;
+            int J;
+            if (r == 0) {
+                // identity row: J_alpha[e] = sum over occupied alpha o of g[oo][e] (r != q only)
+                J = ar;
+                for (int e = tid; e < kSigGS; e += kSigThreads) gs[e] = 0.0;
+                __syncthreads();
+                for (int e = tid; e < nn; e += kSigThreads) {
+                    double vj = 0;
+                    if (e / n != e % n)
+                        for (uint32_t o = alpha; o; o &= o - 1) {
+                            const int xo = __ffs(o) - 1;
+                            vj += thr13(m.g[(size_t)(xo * n + xo) * nn + e]);
+                        }
+                    gs[e] = vj;
+                    gs[kSigNN + e] = -vj;
+                }
+            } else {
+                const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
+                uint32_t oo = alpha, ww = ~alpha & full;
+                for (int i = 0; i < ir; i++) oo &= oo - 1;
+                for (int i = 0; i < iq; i++) ww &= ww - 1;
+                const int rA = __ffs(oo) - 1, qA = __ffs(ww) - 1;
+                J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
+                const bool neg = between_parity(alpha, rA, qA);
+                const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+                for (int e = tid; e < kSigGS; e += kSigThreads) gs[e] = 0.0;
+                __syncthreads();
+                for (int e = tid; e < nn; e += kSigThreads) {
+                    const double cf = neg ? -thr13(gb[e]) : thr13(gb[e]);
+                    gs[e] = cf;
+                    gs[kSigNN + e] = -cf;
+                }
+            }
+            const double *src = x + (long long)J * nb * kLanes;
+            for (int i = tid; i < nb * kSigVC; i += kSigThreads) {
+                const int bb = i / kSigVC, vv = i - bb * kSigVC;
+                xs[i] = src[(long long)bb * kLanes + vv];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSigBI; i++) {
+            const int beta = slot + kSigSlots * i;
+            if (beta < nb) {
+                const uint4 *te = (const uint4 *)(a.ts + (size_t)beta * a.ntp);
+                double s0 = 0, s1 = 0;
+                const int vo = v * 8;
+                for (int t4 = 0; t4 < a.ntp / 4; t4++) {
+                    const uint4 q = te[t4];
+                    const uint32_t ee[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t xb = (ee[u] & 0xffffu) + vo, cb = ee[u] >> 16;
+                        const double c0 = *(const double *)(lds + cb);
+                        const double x0 = *(const double *)(lds + 2 * kSigGS * 8 + xb);
+                        s0 += c0 * x0;
+                        if (two) {
+                            const double c1 = *(const double *)(lds + kSigGS * 8 + cb);
+                            const double x1 = *(const double *)(lds + (2 * kSigGS + kSigMaxNb * kSigVC) * 8 + xb);
+                            s1 += c1 * x1;
+                        }
+                    }
+                }
+                acc[i] += s0 + s1;
+                if (r0 == 0) acc[i] += m.diag[(long long)ar * nb + beta] * xs0[beta * kSigVC + v];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSigBI; i++) {
+        const int beta = slot + kSigSlots * i;
+        if (beta < nb) y[((long long)ar * nb + beta) * kLanes + v] = acc[i];
+    }
+}
+
+// Per-lane dot products of lane-layout chunks: part[(c * G + block) * 64 + l].
+__global__ void __launch_bounds__(kBlock)
+    k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N, double *__restrict__ part)
+{
+    constexpr int G = kBlock / kLanes;
+    __shared__ double red[G][kLanes];
+    const int l = threadIdx.x & (kLanes - 1), g = threadIdx.x / kLanes, c = blockIdx.y;
+    const long long base = (long long)c * N * kLanes + l;
+    double s = 0;
+    for (long long i = blockIdx.x * (long long)G + g; i < N; i += (long long)gridDim.x * G)
+        s += x[base + i * kLanes] * y[base + i * kLanes];
+    red[g][l] = s;
+    __syncthreads();
+    if (g == 0) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < G; w++) t += red[w][l];
+        part[((long long)c * gridDim.x + blockIdx.x) * kLanes + l] = t;
+    }
+}
+
 // ------------------------------------------------------------- Hessian
 // Forward tangent step for operator k.  psi, lambda are rotated out of place
 // (every grid.y slice recomputes them in registers; slice 0 writes them),

@@ -250,6 +250,36 @@ __global__ void k_mf_tables(MfArgs a);
 __global__ void k_mf_sigma(MfArgs a, const double *__restrict__ x, double *__restrict__ y, long long vstride,
                            double *__restrict__ epart);
 
+// Matrix-free sigma in lane layout (batches, n <= 12, N_down <= 924):
+//   y = sparse part (k_sigv_sparse: alpha-beta doubles, the beta-diagonal
+//       parts of alpha singles, the alpha-dependent parts of beta singles,
+//       the diagonal)
+//     + V_bb applied to every alpha row (DGEMM, beta singles' alpha-free
+//       parts and beta-beta doubles)
+//     + V_aa applied across alpha rows (DGEMM, alpha singles' beta-free parts
+//       and alpha-alpha doubles).
+constexpr int kSigVC = 8;          // vectors per workgroup (a sub-chunk of the 64 lanes)
+constexpr int kSigThreads = 1024;
+constexpr int kSigSlots = kSigThreads / kSigVC;
+constexpr int kSigMaxNb = 924;
+constexpr int kSigBI = (kSigMaxNb + kSigSlots - 1) / kSigSlots;
+constexpr int kSigNN = 144;        // coefficient slots per sign (n <= 12)
+constexpr int kSigGS = 2 * kSigNN + 8;
+constexpr size_t kSigLds = (size_t)(2 * kSigGS + 2 * kSigMaxNb * kSigVC) * sizeof(double);
+struct SigvArgs {
+    MfArgs m;
+    const uint32_t *ts;  // nb x ntp beta entries (k_beta_table)
+    int ntp;
+    const double *x;
+    double *y;
+    long long cstride;   // N * 64
+};
+__global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
+__global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp);
+__global__ void k_sigv_sparse(SigvArgs a);  // grid (ceil(na / 8) * 64, chunks), kSigThreads, kSigLds
+__global__ void k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N,
+                            double *__restrict__ part);  // grid (G, chunks)
+
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>
 __global__ void k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut,

@@ -31,12 +31,14 @@ def _batch(a, thetas, lane_min):
     return a.energy_gradients(thetas)
 
 
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
 @pytest.mark.parametrize("nvec", [1, 17, 64, 70])
-def test_lane_layout_matches_vector_major_and_single(gpu, nvec):
+def test_lane_layout_matches_vector_major_and_single(gpu, nvec, storage):
     """TUPS on 8 orbitals (orbit segments only); 70 vectors = one full and
-    one partial chunk of 64 lanes."""
+    one partial chunk of 64 lanes.  Matrix-free storage runs the lane-layout
+    sigma (sparse kernel + two DGEMMs) against the per-vector sigma."""
     w, thetas = _tups_case(8, 3, 4242, nvec)
-    with from_workload(w) as a:
+    with from_workload(w, storage=storage) as a:
         El, Gl = _batch(a, thetas, 1)
         Ev, Gv = _batch(a, thetas, 0)
         assert np.max(np.abs(El - Ev)) <= SAME
@@ -47,7 +49,8 @@ def test_lane_layout_matches_vector_major_and_single(gpu, nvec):
             assert np.max(np.abs(Gl[v] - gs)) <= SAME
 
 
-def test_lane_layout_generic_operators(gpu):
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
+def test_lane_layout_generic_operators(gpu, storage):
     """Same-spin doubles over four alpha orbitals and singles spanning
     three orbitals: segments that are not orbits run one operator per pass
     (k_opv) in the lane layout.  Checked against the oracle per vector."""
@@ -65,7 +68,7 @@ def test_lane_layout_generic_operators(gpu):
     o = OracleEngine(2 * n, [start], [1.0], ops, order, scale)
     o.set_integrals(h, g, explicit=True)
     thetas = seeded_angles(6, 99, count=33)
-    with from_workload(w) as a:
+    with from_workload(w, storage=storage) as a:
         El, Gl = _batch(a, thetas, 1)
         Ev, Gv = _batch(a, thetas, 0)
         assert np.max(np.abs(El - Ev)) <= SAME and np.max(np.abs(Gl - Gv)) <= SAME
@@ -106,14 +109,15 @@ def test_lane_layout_number_sector(gpu):
             assert np.max(np.abs(Gl[v] - go)) <= G_TOL
 
 
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
 @pytest.mark.parametrize("lane_min", [1, 0])
-def test_batch_graph_replays(gpu, lane_min):
+def test_batch_graph_replays(gpu, lane_min, storage):
     """The batch sequence is captured once per shape and replayed; new angle
     uploads of the same size, eager single-vector calls in between, and
     every replay must equal the eager (graph off) batch."""
     w, thetas = _tups_case(6, 3, 808, 24)
     other = seeded_angles(thetas.shape[1], 809, count=24)
-    with from_workload(w) as a:
+    with from_workload(w, storage=storage) as a:
         a.set_option(AnsatzSynth.OPT_LANE_MIN, lane_min)
         a.set_option(AnsatzSynth.OPT_GRAPH, 0)
         ref = {k: a.energy_gradients(t) for k, t in (("a", thetas), ("b", other))}
@@ -165,3 +169,21 @@ def test_autotune_invariance_and_cache(gpu, tmp_path):
         assert rep2["source"] == "cache" and rep2["choice"] == ch
     with open(cache) as f:
         assert len(f.readlines()) == 1
+
+
+def test_lane_sigma_config2_against_reference(gpu):
+    """10 orbitals, 63,504 determinants, matrix-free storage: a 64-vector
+    batch through the lane-layout sigma against the reference build's
+    golden energies and gradients (the golden angles repeated in the batch)."""
+    gz = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"),
+                 allow_pickle=False)
+    w = make_workload(2)
+    th = gz["cfg2_theta"]
+    reps = np.concatenate([th] * (64 // len(th) + 1))[:64]
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        E, G = a.energy_gradients(reps)
+        for v in range(64):
+            i = v % len(th)
+            assert abs(E[v] - gz["cfg2_E"][i]) <= E_TOL
+            assert np.max(np.abs(G[v] - gz["cfg2_grad"][i])) <= G_TOL
