@@ -552,7 +552,7 @@ int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, 
         log_msg("nullptr ctx passed");
         return 1;
     }
-    if (kernelclass < 0 || kernelclass > 2 || milliseconds == nullptr || launches == nullptr) {
+    if (kernelclass < 0 || kernelclass > 3 || milliseconds == nullptr || launches == nullptr) {
         log_msg("bad kernel class or NULL output");
         return 2;
     }

@@ -252,8 +252,9 @@ struct EngineImpl {
     };
     std::vector<Mark> marks;
     std::vector<hipEvent_t> evpool;
-    double tms[3] = {0, 0, 0};
-    long long tcnt[3] = {0, 0, 0};
+    static constexpr int kClasses = 4;  // 0 Hamiltonian (sparse part), 1 forward, 2 adjoint, 3 dense same-spin GEMMs
+    double tms[kClasses] = {0, 0, 0, 0};
+    long long tcnt[kClasses] = {0, 0, 0, 0};
 
     hipEvent_t take_event()
     {
@@ -562,24 +563,27 @@ struct EngineImpl {
     }
 
     // Forward sweep of lane-layout chunks.
+    // (with timing on, every launch is bracketed by its own events)
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
-        const int mk = mark_begin(1);
         const long long cstride = N * kLanes;
         for (const Seg &sg : segs) {
             if (sg.orbA >= 0) {
+                const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, orbargs(sg, false),
                                    v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                mark_end(mk);
                 LCHK();
             } else {
                 for (int k = sg.k0; k < sg.k1; k++) {
+                    const int mk = mark_begin(1);
                     hipLaunchKernelGGL((k_opv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, v, nullptr,
                                        cstride, N, sec(), d_ops.p, k, bl_cs.p, M, nullptr, lslotsF);
+                    mark_end(mk);
                     LCHK();
                 }
             }
         }
-        mark_end(mk);
         return true;
     }
 
@@ -587,23 +591,25 @@ struct EngineImpl {
     // partial rows are written by exactly one launch of lslots blocks.
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
     {
-        const int mk = mark_begin(2);
         const long long cstride = N * kLanes;
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
             if (sg.orbA >= 0) {
+                const int mk = mark_begin(2);
                 hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, orbargs(sg, true), phi,
                                    lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                mark_end(mk);
                 LCHK();
             } else {
                 for (int k = sg.k1 - 1; k >= sg.k0; k--) {
+                    const int mk = mark_begin(2);
                     hipLaunchKernelGGL((k_opv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, phi, lam, cstride,
                                        N, sec(), d_ops.p, k, bl_csneg.p, M, bl_partial.p, lslots);
+                    mark_end(mk);
                     LCHK();
                 }
             }
         }
-        mark_end(mk);
         return true;
     }
 
@@ -621,7 +627,9 @@ struct EngineImpl {
         s.cstride = N * kLanes;
         const unsigned gx = (unsigned)((na + 7) / 8 * 64);
         hipLaunchKernelGGL(k_sigv_sparse, dim3(gx, nch), dim3(kSigThreads), kSigLds, stream, s);
+        mark_end(mk);
         LCHK();
+        const int mk3 = mark_begin(3);
         const double one = 1.0;
         // beta part: per alpha row, Y_a (nb x 64) += V_bb X_a  (column-major view: Y_a^T += X_a^T V_bb^T)
         if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, kLanes, nb, nb, &one,
@@ -637,7 +645,7 @@ struct EngineImpl {
             err = "rocblas_dgemm_strided_batched (alpha part) failed";
             return false;
         }
-        mark_end(mk);
+        mark_end(mk3);
         const int G = 512;
         hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, bl_epart.p);
         LCHK();
@@ -1003,7 +1011,11 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
     if (!e.d_h.ensure(h.size(), err) || !e.d_g.ensure(g.size(), err)) return false;
     HCHK(hipMemcpy(e.d_h.p, h.data(), 8 * h.size(), hipMemcpyHostToDevice));
     HCHK(hipMemcpy(e.d_g.p, g.data(), 8 * g.size(), hipMemcpyHostToDevice));
-    if (storage == kHamMatrixFree || (storage == kHamAuto && !fitsExplicit)) {
+    // automatic choice: matrix-free from 12 orbitals (its sigma outruns the
+    // explicit rows there, single vectors and batches alike) or when the
+    // rows do not fit; explicit rows below
+    const bool preferMf = n >= 12;
+    if (storage == kHamMatrixFree || (storage == kHamAuto && (!fitsExplicit || preferMf))) {
         // matrix-free: diagonal and same-spin single tables, then sigma on the fly
         const long long nsA = (long long)ku * (n - ku), nsB = (long long)kd * (n - kd);
         if (!e.d_diag.ensure(e.N, err) || !e.d_FA.ensure(std::max(1ll, nsA * e.na), err) ||
@@ -1404,6 +1416,10 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.bnvec = 0;
         return true;
     }
+    case kOptHamKind:
+    case kOptSegments:
+        err = "read-only engine property";
+        return false;
     default:
         err = "unknown engine option";
         return false;
@@ -1419,6 +1435,8 @@ int Engine::option(int key) const
     case kOptGraph: return e.useGraph ? 1 : 0;
     case kOptFuseDepth: return e.fuseDepth;
     case kOptLdsBudget: return (int)e.ldsMax;
+    case kOptHamKind: return e.hkind;
+    case kOptSegments: return (int)e.segs.size();
     default: return -1;
     }
 }
@@ -1434,7 +1452,7 @@ void Engine::set_timing(bool on)
 {
     EngineImpl &e = *d_;
     e.timingOn = on;
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < EngineImpl::kClasses; c++) {
         e.tms[c] = 0;
         e.tcnt[c] = 0;
     }
@@ -1442,7 +1460,7 @@ void Engine::set_timing(bool on)
 
 bool Engine::timing(int cls, double *ms, long long *launches, std::string &err)
 {
-    if (cls < 0 || cls > 2) {
+    if (cls < 0 || cls >= EngineImpl::kClasses) {
         err = "unknown kernel class";
         return false;
     }

@@ -30,6 +30,8 @@ enum EngineOption : int {
     kOptGraph = 2,    // 1: replay hipGraphs, 0: eager launches
     kOptFuseDepth = 3,  // most operators fused into one pass (0: no limit beyond the kernel's)
     kOptLdsBudget = 4,  // bytes of LDS an LDS-tile segment may use (0: the device's limit)
+    kOptHamKind = 5,    // read only: 1 rows, 2 uniform-width rows, 3 matrix-free
+    kOptSegments = 6,   // read only: fused segments per sweep
 };
 
 class Engine {
@@ -90,7 +92,9 @@ public:
     bool synchronize(std::string &err);
 
     // HIP-event timing of kernel classes on the engine's stream:
-    // 0 Hamiltonian application, 1 forward sweep, 2 adjoint sweep.
+    // 0 Hamiltonian application (sparse part), 1 forward sweep, 2 adjoint
+    // sweep, 3 dense same-spin GEMMs of the lane-layout sigma.  Lane-layout
+    // sweeps record one interval per launch.
     void set_timing(bool on);
     bool timing(int cls, double *ms, long long *launches, std::string &err);
 

@@ -2,14 +2,16 @@
 
 Workload: 12 spatial orbitals / 12 electrons, S_z = 0 sector of 853,776
 determinants, seeded synthetic integrals, TUPS with 6 layers (198
-parameters, 330 operators), explicit Hamiltonian rows on the device.  One
+parameters, 330 operators), the automatic Hamiltonian form (matrix-free at
+12 orbitals; --storage 1 forces the explicit rows, 18.6 GB).  One
 step = one batch of 64 angle vectors (theta_b = theta_0 + 1e-2 U(-pi, pi))
 evaluated to energy + gradient.  Inputs (Hamiltonian, psi_0, the angle
 batch) are resident in HBM before the timed region; a step enqueues device
 work only.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nvec 64]
-                    [--config 3] [--no-cpu-baseline]
+                    [--config 3] [--storage 0|1|2] [--no-cpu-baseline]
+                    [--no-autotune] [--calib-copy]
 
 Prints one JSON line.
 """
@@ -167,7 +169,11 @@ def main():
     ham_ms, ham_n = a.kernel_timing(0)
     fwd_ms, fwd_n = a.kernel_timing(1)
     rev_ms, rev_n = a.kernel_timing(2)
+    gemm_ms, gemm_n = a.kernel_timing(3)
     a.set_kernel_timing(False)
+    ham_kind = a.get_option(a.OPT_HAM_KIND)
+    nseg = a.get_option(a.OPT_SEGMENTS)
+    lanes = opts["lane_min"] > 0 and args.nvec >= opts["lane_min"]
 
     # batched results must equal single-vector results
     Es, gs = a.energy_gradient(thetas[-1])
@@ -176,27 +182,62 @@ def main():
     evals = args.steps * args.nvec
     value = evals / elapsed
     nnz = N * w_row_width(w)
-    avg_ms = ham_ms / max(ham_n, 1)
-    vec_per_launch = args.nvec / max(1.0, ham_n / nbd)
-    if args.storage == 2:
-        # matrix-free sigma: one launch = all vectors; algorithmic work =
-        # 2 flops per Hamiltonian entry per vector (FP64, vector ALU)
-        flops = 2.0 * nnz * vec_per_launch
-        roof = {"bound": "mfma", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "kernel": "k_mf_sigma (matrix-free sigma, %d vectors)" % vec_per_launch,
-                "algorithmic_flops_per_launch": flops,
-                "note": "FP64 vector-ALU work priced against the FP64 dense peak"}
+    n_, ku, kd = w["n"], w["nup"], w["ndown"]
+    rooflines = {}
+
+    def per_launch(ms, cnt, per_interval):
+        # lane-layout sweeps time each launch; other paths one interval per sweep
+        return ms / max(1, cnt * per_interval)
+
+    def hbm(kernel, bytes_launch, avg_ms, launches, per_step_ms):
+        ach = bytes_launch / (avg_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "kernel": kernel, "avg_launch_ms": avg_ms, "launches": launches,
+                "algorithmic_bytes_per_launch": bytes_launch, "ms_per_step": per_step_ms}
+
+    def fp64(kernel, flops_launch, avg_ms, launches, per_step_ms, note):
+        ach = flops_launch / (avg_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kernel, "avg_launch_ms": avg_ms,
+                "launches": launches, "algorithmic_flops_per_launch": flops_launch, "ms_per_step": per_step_ms,
+                "note": note}
+
+    # sweeps: every segment reads and writes each vector once (forward: psi;
+    # adjoint: psi and lambda), 8 B per amplitude per access
+    k_fwd = "k_orbv<false> (lane layout)" if lanes else "k_orb<false>"
+    k_rev = "k_orbv<true> (lane layout)" if lanes else "k_orb<true>"
+    rooflines["forward_sweep"] = hbm(k_fwd, 16.0 * N * args.nvec, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
+                                     fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
+    rooflines["adjoint_sweep"] = hbm(k_rev, 32.0 * N * args.nvec, per_launch(rev_ms, rev_n, 1 if lanes else nseg),
+                                     rev_n if lanes else rev_n * nseg, rev_ms / nbd)
+    if ham_kind == 3 and gemm_n > 0:
+        # lane-layout matrix-free sigma: the sparse kernel's algorithmic work
+        # per element = alpha-beta doubles + beta-diagonal parts of alpha
+        # singles + alpha-dependent parts of beta singles + diagonal, 2 flops
+        # each per vector; the DGEMMs cover the same-spin rows densely
+        per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
+        rooflines["hamiltonian"] = fp64("k_sigv_sparse (lane-layout sigma, %d vectors)" % args.nvec,
+                                        2.0 * N * per_el * args.nvec, ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
+                                        "FP64 vector-ALU/LDS work priced against the FP64 peak")
+        na = math.comb(n_, ku)
+        nb = math.comb(n_, kd)
+        gflops = 2.0 * na * na * nb * 64 * math.ceil(args.nvec / 64) + 2.0 * nb * nb * 64 * na * math.ceil(args.nvec / 64)
+        rooflines["same_spin_dgemm"] = fp64("rocBLAS dgemm_strided_batched x2 (dense same-spin rows)", gflops,
+                                            gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd,
+                                            "dense FLOPs of both GEMMs on FP64 MFMA")
+    elif ham_kind == 3:
+        rooflines["hamiltonian"] = fp64("k_mf_sigma (matrix-free sigma)", 2.0 * nnz * args.nvec / max(1.0, ham_n / nbd),
+                                        ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
+                                        "FP64 vector-ALU work priced against the FP64 peak")
     else:
         # explicit rows: 12 B per stored entry (8 B value + 4 B column) per
         # launch + 16 B per amplitude per vector (x read, y written)
-        bytes_launch = 12.0 * nnz + 16.0 * N * vec_per_launch
-        roof = {"bound": "hbm", "achieved": bytes_launch / (avg_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "kernel": "k_ell (Hamiltonian rows x %d vectors)" % vec_per_launch,
-                "algorithmic_bytes_per_launch": bytes_launch}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["avg_launch_ms"] = avg_ms
-    roof["launches"] = ham_n
+        vpl = args.nvec / max(1.0, ham_n / nbd)
+        rooflines["hamiltonian"] = hbm("k_ell (Hamiltonian rows x %d vectors)" % vpl, 12.0 * nnz + 16.0 * N * vpl,
+                                       ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd)
+    dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"])
+    roof = dict(rooflines[dominant])
+    roof["class"] = dominant
     result = {
         "metric": "TUPS E+grad evals/s at 0.85M dets",
         "value": value,
@@ -214,15 +255,16 @@ def main():
             "workload": "12 orbitals / 12 electrons, Sz=0, %d determinants, TUPS L=%d (%d parameters, %d operators), "
                         "%d angle vectors per step, %s (%d entries)"
                         % (N, w["layers"], P, M, args.nvec,
-                           "matrix-free sigma build" if args.storage == 2 else "explicit Hamiltonian rows", nnz),
-            "hamiltonian": "matrix-free" if args.storage == 2 else "explicit rows",
+                           "matrix-free sigma build" if ham_kind == 3 else "explicit Hamiltonian rows", nnz),
+            "hamiltonian": "matrix-free" if ham_kind == 3 else "explicit rows",
             "batch": args.nvec,
             "determinants": N,
             "parallelism": "single GPU",
         },
         "roofline": roof,
-        "breakdown_ms_per_step": {"hamiltonian": ham_ms / nbd, "forward_sweep": fwd_ms / nbd,
-                                  "adjoint_sweep": rev_ms / nbd},
+        "roofline_by_class": rooflines,
+        "breakdown_ms_per_step": {"hamiltonian": ham_ms / nbd, "same_spin_dgemm": gemm_ms / nbd,
+                                  "forward_sweep": fwd_ms / nbd, "adjoint_sweep": rev_ms / nbd},
         "engine_options": opts,
         "autotune": tune,
         "single_vector_egrad_per_s": single_rate,

@@ -107,8 +107,9 @@ int setHamiltonianIntegrals(int norb, const double *h, const double *g, void *ct
  * src/AnsatzManager.cpp:400-413).  1 ctx NULL, 3 not stored. */
 int setNuclearEnergy(double energy, void *ctx);
 
-/* How an integral Hamiltonian is applied: 0 automatic (explicit sparse matrix
- * when it fits in device memory, else matrix-free), 1 explicit, 2 matrix-free.
+/* How an integral Hamiltonian is applied: 0 automatic (matrix-free from 12
+ * orbitals or when the explicit rows do not fit in device memory, explicit
+ * rows otherwise), 1 explicit, 2 matrix-free.
  * 1 ctx NULL, 2 bad mode, 3 after construction. */
 int setHamiltonianStorage(int mode, void *ctx);
 
@@ -145,8 +146,10 @@ int downloadAngleBatch(double *energies, double *gradients, void *ctx);
 /* Kernel timing with HIP events on the ctx's stream.  setKernelTiming(1)
  * resets and starts recording; getKernelTiming waits for the stream and
  * returns the summed milliseconds and launch count of one kernel class:
- * 0 Hamiltonian application, 1 forward sweep, 2 adjoint sweep.
- * 1 ctx NULL, 2 bad class, 5 failure. */
+ * 0 Hamiltonian application (its sparse kernel), 1 forward sweep,
+ * 2 adjoint sweep, 3 dense same-spin GEMMs of the batched matrix-free
+ * sigma.  Batched (lane-layout) sweeps count every kernel launch, the
+ * other paths one interval per sweep.  1 ctx NULL, 2 bad class, 5 failure. */
 int setKernelTiming(int on, void *ctx);
 int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, void *ctx);
 
@@ -176,7 +179,10 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     vector per wavefront lane (0: never);
  *   2 hipGraph replay: 1 on, 0 eager launches;
  *   3 fusion depth: most operators per pass (0: the kernels' own limit);
- *   4 LDS budget in bytes of a tile segment (0: the device's limit).
+ *   4 LDS budget in bytes of a tile segment (0: the device's limit);
+ *   5 (read only) Hamiltonian form: 1 rows, 2 uniform-width rows,
+ *     3 matrix-free;
+ *   6 (read only) fused segments per sweep.
  * Changing keys 0, 3 or 4 re-segments the operators; an uploaded angle
  * batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */
