@@ -26,7 +26,7 @@ void log_msg(const char *fmt, ...);
 namespace {
 
 struct Candidate {
-    int sweep, depth, laneMin;
+    int sweep, depth, laneMin, groups;
 };
 
 std::string shape_key(const Engine &eng, int nvec, int nops)
@@ -39,7 +39,8 @@ std::string shape_key(const Engine &eng, int nvec, int nops)
 std::string cand_json(const Candidate &c)
 {
     std::ostringstream o;
-    o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin << "}";
+    o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
+      << ", \"groups\": " << c.groups << "}";
     return o.str();
 }
 
@@ -55,8 +56,8 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         if (line.find(tag) == std::string::npos) continue;
         Candidate t{};
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
-                        "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d}", &t.sweep, &t.depth,
-                        &t.laneMin) == 3) {
+                        "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d}",
+                        &t.sweep, &t.depth, &t.laneMin, &t.groups) == 4) {
             c = t;
             found = true;
         }
@@ -67,7 +68,8 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
 bool apply(Engine &eng, const Candidate &c, std::string &err)
 {
     return eng.set_option(kOptSweep, c.sweep, err) && eng.set_option(kOptFuseDepth, c.depth, err) &&
-           eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptGraph, 1, err);
+           eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneMin, c.laneMin, err) &&
+           eng.set_option(kOptGraph, 1, err);
 }
 
 } // namespace
@@ -103,11 +105,14 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
             if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
             for (int d : depths) {
                 if (!sz && d != 0) continue;
-                cands.push_back({s, d, lanes ? 1 : 0});
+                cands.push_back({s, d, lanes ? 1 : 0, 1});
+                // the lane layout can also fuse two disjoint groups per pass
+                if (lanes && sz && s == kSweepOrbit && d == 0) cands.push_back({s, d, 1, 2});
             }
         }
 
-    const Candidate saved{eng.option(kOptSweep), eng.option(kOptFuseDepth), eng.option(kOptLaneMin)};
+    const Candidate saved{eng.option(kOptSweep), eng.option(kOptFuseDepth), eng.option(kOptLaneMin),
+                          eng.option(kOptLaneGroups)};
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";

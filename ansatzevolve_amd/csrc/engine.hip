@@ -75,13 +75,27 @@ struct TileSet {
     DBuf<int4> desc;  // classes of at most two members: {rank0, rank1 or -1, string0, string1}
 };
 
+// Classes of one spin for a pair of disjoint bit groups (g1, g2): strings with
+// the same bits outside the groups and the same electron count in each group.
+struct GroupSet {
+    uint32_t key1 = 0, key2 = 0;
+    bool beta = false;
+    int ncls = 0;
+    DBuf<int4> ranks;
+    DBuf<uint32_t> info;
+};
+
 struct Seg {
     int k0, k1;
     int tileset;  // LDS tiles (>= 0); -1: generic kernel (one operator) or orbit segment
     int orbA = -1, orbB = -1;  // orbit segment: alpha / beta class sets
     long long orbits = 0;      // alpha classes x beta classes
     uint32_t ua = 0, ub = 0;   // orbit segment: its alpha / beta bits (within one spin string)
-    int progOff = -1;          // orbit segment: offset of its program in d_prog
+    int progOff = -1;          // orbit segment: offset of its program (two-group: of its (g, h) list)
+    int two = 0;               // lane layout: two-group orbit segment (k_orb2v)
+    int gA = -1, gB = -1;      // two-group segment: alpha / beta group sets
+    uint32_t grA[2] = {0, 0}, grB[2] = {0, 0};
+    int prog2Off = -1;
 };
 
 static long long binom_ll(int n, int k)
@@ -110,6 +124,12 @@ struct EngineImpl {
     DBuf<DevOp> d_ops;
     DBuf<uint32_t> d_prog;  // orbit programs of the orbit segments (OrbArgs::prog)
     std::vector<Seg> segs;
+    std::vector<Seg> lsegs;  // segmentation of the lane-layout sweeps
+    std::vector<std::unique_ptr<GroupSet>> gsets;
+    DBuf<uint32_t> d_lprog;                 // one-group orbit programs of lsegs
+    DBuf<unsigned long long> d_prog2;       // two-group orbit programs of lsegs
+    DBuf<uint32_t> d_opgh;
+    int maxGroups = 1;  // groups per spin of a lane-layout orbit segment (1 or 2; the autotuner measures both)
     std::vector<std::unique_ptr<TileSet>> tilesets;
     int slots = 1, Ggen = 1;
     int ngeneric = 0;
@@ -171,12 +191,13 @@ struct EngineImpl {
     // orbit segment, the member pairs the operator rotates and their phases,
     // from the pair rule on representative members with the spectator bits
     // cleared.
-    bool build_orbit_programs(std::string &err)
+    bool build_orbit_programs(std::string &err) { return build_orbit_programs(segs, d_prog, err); }
+    bool build_orbit_programs(std::vector<Seg> &list, DBuf<uint32_t> &dprog, std::string &err)
     {
         std::vector<uint32_t> prog;
-        for (Seg &sg : segs) {
+        for (Seg &sg : list) {
             sg.progOff = -1;
-            if (sg.orbA < 0) continue;
+            if (sg.orbA < 0 || sg.two) continue;
             sg.progOff = (int)prog.size();
             uint32_t bitsA[2] = {0, 0}, bitsB[2] = {0, 0};
             {
@@ -232,9 +253,218 @@ struct EngineImpl {
                 }
         }
         if (prog.empty()) prog.push_back(0);
-        if (!d_prog.ensure(prog.size(), err)) return false;
-        HCHK(hipMemcpy(d_prog.p, prog.data(), prog.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!dprog.ensure(prog.size(), err)) return false;
+        HCHK(hipMemcpy(dprog.p, prog.data(), prog.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         return true;
+    }
+
+    int groupset_for(uint32_t g1, uint32_t g2, bool beta, std::string &err)
+    {
+        for (size_t i = 0; i < gsets.size(); i++)
+            if (gsets[i]->key1 == g1 && gsets[i]->key2 == g2 && gsets[i]->beta == beta) return (int)i;
+        auto t = std::make_unique<GroupSet>();
+        t->key1 = g1;
+        t->key2 = g2;
+        t->beta = beta;
+        const std::vector<uint32_t> &un = beta ? unB : unA;
+        const uint32_t U = g1 | g2;
+        std::map<std::array<uint32_t, 3>, int> cls;
+        std::vector<int4> r;
+        std::vector<uint32_t> info;
+        auto shape = [](int c) { return c == 1 ? 0u : (c == 0 ? 1u : 2u); };
+        for (int rk = 0; rk < (int)un.size(); rk++) {
+            const uint32_t str = un[rk];
+            const int c1 = __builtin_popcount(str & g1), c2 = __builtin_popcount(str & g2);
+            const std::array<uint32_t, 3> key = {str & ~U, (uint32_t)c1, (uint32_t)c2};
+            auto it = cls.find(key);
+            int ci;
+            if (it == cls.end()) {
+                ci = (int)r.size();
+                cls[key] = ci;
+                r.push_back(make_int4(-1, -1, -1, -1));
+                info.push_back((str & ~U) | (shape(c1) << 24) | (shape(c2) << 26));
+            } else {
+                ci = it->second;
+            }
+            // member index per group: 0 = the group's lower bit set (or 0 / 2 electrons), 1 = upper bit
+            const uint32_t lo1 = g1 & (~g1 + 1), lo2 = g2 & (~g2 + 1);
+            const int m1 = (c1 == 1 && !(str & lo1)) ? 1 : 0, m2 = (c2 == 1 && !(str & lo2)) ? 1 : 0;
+            int *rr = &r[ci].x;
+            rr[m1 + 2 * m2] = rk;
+        }
+        t->ncls = (int)r.size();
+        if (!t->ranks.ensure(r.size(), err) || !t->info.ensure(info.size(), err)) return -1;
+        HCHK(hipMemcpy(t->ranks.p, r.data(), r.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(t->info.p, info.data(), info.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        gsets.push_back(std::move(t));
+        return (int)gsets.size() - 1;
+    }
+
+    // Lane-layout segmentation (S_z sector): two-group orbit segments where
+    // operators pair up on disjoint spatial pairs, one-group orbit segments
+    // otherwise, and the vector-major segments' LDS tiles and generic
+    // operators run one operator per pass (k_opv).
+    bool build_lane_segments(std::string &err)
+    {
+        lsegs.clear();
+        if (mode != kSz || sweepKind == kSweepTile) {
+            lsegs = segs;
+            for (Seg &sg : lsegs) sg.two = 0;
+            return build_orbit_programs(lsegs, d_lprog, err);
+        }
+        const uint32_t amask = (1u << n) - 1;
+        auto bitsA = [&](int kk) { return ((hops[kk].cre | hops[kk].ann) >> n) & amask; };
+        auto bitsB = [&](int kk) { return (hops[kk].cre | hops[kk].ann) & amask; };
+        const int maxOps = std::min(kOrb2MaxOps, fuseDepth > 0 ? fuseDepth : kOrb2MaxOps);
+        std::vector<unsigned long long> prog2;
+        std::vector<uint32_t> opgh;
+        // try to add a group mask u to the list g (size <= maxGroups): returns slot or -1
+        auto join = [&](std::vector<uint32_t> &g, uint32_t u) -> int {
+            if (u == 0) return 0;
+            if (__builtin_popcount(u) != 2) return -1;
+            for (size_t i = 0; i < g.size(); i++) {
+                if (g[i] == u) return (int)i;
+                if (g[i] & u) return -1;
+            }
+            if ((int)g.size() >= maxGroups) return -1;
+            g.push_back(u);
+            return (int)g.size() - 1;
+        };
+        int k = 0;
+        while (k < M) {
+            std::vector<uint32_t> ga, gb;
+            int k1 = k;
+            while (k1 < M && k1 - k < maxOps) {
+                std::vector<uint32_t> ta = ga, tb = gb;
+                if (join(ta, bitsA(k1)) < 0 || join(tb, bitsB(k1)) < 0 || (bitsA(k1) | bitsB(k1)) == 0) break;
+                ga.swap(ta);
+                gb.swap(tb);
+                k1++;
+            }
+            if (k1 == k) {
+                // not an orbit operator: one generic pass
+                Seg sg{k, k + 1, -1};
+                lsegs.push_back(sg);
+                k++;
+                continue;
+            }
+            if (ga.size() <= 1 && gb.size() <= 1) k1 = std::min(k1, k + kOrbMaxOps);  // k_orbv's limit
+            Seg sg{k, k1, -1};
+            if (ga.size() <= 1 && gb.size() <= 1) {
+                const uint32_t Ua = ga.empty() ? 0 : ga[0], Ub = gb.empty() ? 0 : gb[0];
+                sg.ua = Ua;
+                sg.ub = Ub;
+                sg.orbA = tileset_for(Ua, err, false);
+                sg.orbB = tileset_for(Ub, err, true);
+                if (sg.orbA < 0 || sg.orbB < 0) return false;
+                sg.orbits = (long long)tilesets[sg.orbA]->ntiles * tilesets[sg.orbB]->ntiles;
+                lsegs.push_back(sg);
+                k = k1;
+                continue;
+            }
+            sg.two = 1;
+            for (size_t i = 0; i < 2; i++) {
+                sg.grA[i] = i < ga.size() ? ga[i] : 0;
+                sg.grB[i] = i < gb.size() ? gb[i] : 0;
+            }
+            sg.gA = groupset_for(sg.grA[0], sg.grA[1], false, err);
+            sg.gB = groupset_for(sg.grB[0], sg.grB[1], true, err);
+            if (sg.gA < 0 || sg.gB < 0) return false;
+            sg.orbits = (long long)gsets[sg.gA]->ncls * gsets[sg.gB]->ncls;
+            // programs: 81 types x kOrb2MaxOps words, and (g, h) per operator
+            sg.prog2Off = (int)prog2.size();
+            const int ghOff = (int)opgh.size();
+            for (int kk = 0; kk < kOrb2MaxOps; kk++) {
+                uint32_t gh = 0;
+                if (k + kk < k1) {
+                    const uint32_t ua = bitsA(k + kk), ub = bitsB(k + kk);
+                    const int g = (ua && ua == sg.grA[1]) ? 1 : 0, h = (ub && ub == sg.grB[1]) ? 1 : 0;
+                    gh = (uint32_t)g | ((uint32_t)h << 1);
+                }
+                opgh.push_back(gh);
+            }
+            auto pat = [](uint32_t grp, int shp, int m) -> uint32_t {
+                if (shp == 1 || grp == 0) return 0u;
+                if (shp == 2) return grp;
+                const uint32_t lo = grp & (~grp + 1);
+                return m ? (grp ^ lo) : lo;
+            };
+            static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+            for (int type = 0; type < 81; type++) {
+                const int sa1 = type % 3, sa2 = (type / 3) % 3, sb1 = (type / 9) % 3, sb2 = (type / 27) % 3;
+                auto valid = [&](int slot) {
+                    const int ma1 = slot & 1, ma2 = (slot >> 1) & 1, mb1 = (slot >> 2) & 1, mb2 = (slot >> 3) & 1;
+                    const bool twoA1 = sa1 == 0 && sg.grA[0], twoA2 = sa2 == 0 && sg.grA[1];
+                    const bool twoB1 = sb1 == 0 && sg.grB[0], twoB2 = sb2 == 0 && sg.grB[1];
+                    return (ma1 == 0 || twoA1) && (ma2 == 0 || twoA2) && (mb1 == 0 || twoB1) && (mb2 == 0 || twoB2);
+                };
+                auto str = [&](int slot) {
+                    const int ma1 = slot & 1, ma2 = (slot >> 1) & 1, mb1 = (slot >> 2) & 1, mb2 = (slot >> 3) & 1;
+                    const uint32_t a = pat(sg.grA[0], sa1, ma1) | pat(sg.grA[1], sa2, ma2);
+                    const uint32_t b = pat(sg.grB[0], sb1, mb1) | pat(sg.grB[1], sb2, mb2);
+                    return (a << n) | b;
+                };
+                for (int kk = 0; kk < kOrb2MaxOps; kk++) {
+                    unsigned long long w = 0;
+                    if (k + kk < k1) {
+                        const DevOp &o = hops[k + kk];
+                        const int g = (int)(opgh[ghOff + kk] & 1), h = (int)((opgh[ghOff + kk] >> 1) & 1);
+                        for (int c = 0; c < 4; c++) {
+                            const int oa = c & 1, ob = c >> 1;
+                            auto slot = [&](int i, int j) {
+                                return ((i << g) | (oa << (1 - g))) + 4 * ((j << h) | (ob << (1 - h)));
+                            };
+                            for (int r = 0; r < 6; r++) {
+                                const int sp = slot(PP[r] >> 1, PP[r] & 1), sq = slot(QQ[r] >> 1, QQ[r] & 1);
+                                if (!valid(sp) || !valid(sq)) continue;
+                                uint32_t b2 = 0;
+                                const int sgn = host_pair_sign(o, str(sp), b2);
+                                if (sgn == 0 || b2 != str(sq)) continue;
+                                w |= 1ull << (12 * c + r);
+                                if (sgn < 0) w |= 1ull << (12 * c + 6 + r);
+                            }
+                        }
+                    }
+                    prog2.push_back(w);
+                }
+            }
+            sg.progOff = ghOff;  // offset of the segment's (g, h) list in d_opgh
+            lsegs.push_back(sg);
+            k = k1;
+        }
+        if (prog2.empty()) prog2.push_back(0);
+        if (opgh.empty()) opgh.push_back(0);
+        if (!d_prog2.ensure(prog2.size(), err) || !d_opgh.ensure(opgh.size(), err)) return false;
+        HCHK(hipMemcpy(d_prog2.p, prog2.data(), prog2.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(d_opgh.p, opgh.data(), opgh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        // one-group segments of the lane list get their own programs
+        std::vector<uint32_t> save(lsegs.size());
+        for (size_t i = 0; i < lsegs.size(); i++) save[i] = (uint32_t)lsegs[i].progOff;
+        if (!build_orbit_programs(lsegs, d_lprog, err)) return false;
+        for (size_t i = 0; i < lsegs.size(); i++)
+            if (lsegs[i].two) lsegs[i].progOff = (int)save[i];
+        return true;
+    }
+
+    Orb2Args orb2args(const Seg &sg, bool reverse) const
+    {
+        const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
+        Orb2Args a;
+        a.n = n;
+        a.nb = nb;
+        a.aRanks = A.ranks.p;
+        a.bRanks = B.ranks.p;
+        a.aInfo = A.info.p;
+        a.bInfo = B.info.p;
+        a.nAc = A.ncls;
+        a.nBc = B.ncls;
+        a.ops = d_ops.p;
+        a.k0 = sg.k0;
+        a.k1 = sg.k1;
+        a.reverse = reverse ? 1 : 0;
+        a.prog = d_prog2.p + sg.prog2Off;
+        a.opgh = d_opgh.p + sg.progOff;
+        return a;
     }
 
     void drop_graphs()
@@ -363,7 +593,7 @@ struct EngineImpl {
         return (int)std::max<long long>(1, std::min<long long>((sg.orbits + kBlock - 1) / kBlock, 2048));
     }
 
-    OrbArgs orbargs(const Seg &sg, bool reverse) const
+    OrbArgs orbargs(const Seg &sg, bool reverse, const uint32_t *progBase = nullptr) const
     {
         const TileSet &A = *tilesets[sg.orbA], &B = *tilesets[sg.orbB];
         OrbArgs a;
@@ -384,7 +614,7 @@ struct EngineImpl {
         a.k0 = sg.k0;
         a.k1 = sg.k1;
         a.reverse = reverse ? 1 : 0;
-        a.prog = d_prog.p + (sg.progOff >= 0 ? sg.progOff : 0);
+        a.prog = (progBase ? progBase : d_prog.p) + (sg.progOff >= 0 ? sg.progOff : 0);
         a.uaLocal = sg.ua;
         a.ubLocal = sg.ub;
         const uint32_t full = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1);
@@ -567,11 +797,17 @@ struct EngineImpl {
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
-        for (const Seg &sg : segs) {
-            if (sg.orbA >= 0) {
+        for (const Seg &sg : lsegs) {
+            if (sg.two) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, orbargs(sg, false),
+                hipLaunchKernelGGL((k_orb2v<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, orb2args(sg, false),
                                    v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                mark_end(mk);
+                LCHK();
+            } else if (sg.orbA >= 0) {
+                const int mk = mark_begin(1);
+                hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream,
+                                   orbargs(sg, false, d_lprog.p), v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
                 mark_end(mk);
                 LCHK();
             } else {
@@ -592,12 +828,19 @@ struct EngineImpl {
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
-        for (int si = (int)segs.size() - 1; si >= 0; si--) {
-            const Seg &sg = segs[si];
-            if (sg.orbA >= 0) {
+        for (int si = (int)lsegs.size() - 1; si >= 0; si--) {
+            const Seg &sg = lsegs[si];
+            if (sg.two) {
                 const int mk = mark_begin(2);
-                hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, orbargs(sg, true), phi,
+                hipLaunchKernelGGL((k_orb2v<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, orb2args(sg, true), phi,
                                    lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                mark_end(mk);
+                LCHK();
+            } else if (sg.orbA >= 0) {
+                const int mk = mark_begin(2);
+                hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream,
+                                   orbargs(sg, true, d_lprog.p), phi, lam, cstride, bl_csneg.p, M, bl_partial.p,
+                                   lslots);
                 mark_end(mk);
                 LCHK();
             } else {
@@ -936,9 +1179,10 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     }
     e.slots = std::max(maxTiles, e.Ggen);
     if (!e.build_orbit_programs(err)) return false;
+    if (!e.build_lane_segments(err)) return false;
     {
         long long work = 1;
-        for (const Seg &sg : e.segs) work = std::max(work, sg.orbA >= 0 ? sg.orbits : e.N);
+        for (const Seg &sg : e.lsegs) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : e.N);
         // enough waves to keep the chip's HBM pipeline full: every resident
         // wave (4 per SIMD at the adjoint kernel's register use, 6 forward)
         // holds two orbits' loads in flight
@@ -1416,8 +1660,27 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.bnvec = 0;
         return true;
     }
+    case kOptLaneGroups:
+        if (value < 1 || value > 2) {
+            err = "lane groups must be 1 or 2";
+            return false;
+        }
+        if (value == e.maxGroups) return true;
+        e.maxGroups = value;
+        if (e.M > 0) {
+            e.drop_graphs();
+            if (!e.build_lane_segments(err)) return false;
+            long long work = 1;
+            for (const Seg &sg : e.lsegs) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : e.N);
+            const long long waves = (work + 1) / 2;
+            e.lslots = (int)std::max<long long>(1, std::min<long long>(1024, (waves + 3) / 4));
+            e.lslotsF = (int)std::max<long long>(1, std::min<long long>(1536, (waves + 3) / 4));
+        }
+        e.bnvec = 0;
+        return true;
     case kOptHamKind:
     case kOptSegments:
+    case kOptLaneSegments:
         err = "read-only engine property";
         return false;
     default:
@@ -1437,6 +1700,8 @@ int Engine::option(int key) const
     case kOptLdsBudget: return (int)e.ldsMax;
     case kOptHamKind: return e.hkind;
     case kOptSegments: return (int)e.segs.size();
+    case kOptLaneGroups: return e.maxGroups;
+    case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }
 }

@@ -519,6 +519,123 @@ __global__ void __launch_bounds__(kBlock)
     if (BR) lanes_block_partials<kOrbMaxOps>(acc, nk, a.k0, a.k1, a.reverse, red, partial, c, M, slots);
 }
 
+// Two-group orbit segment in lane layout (see kernels.hpp): wave per orbit
+// of up to 16 members, lane per vector; operators in a runtime loop with the
+// segment's sin/cos staged in LDS and brackets accumulated per wave in LDS.
+template <int G, int H>
+__device__ __forceinline__ void orb2_apply(unsigned long long w, bool neg, double C, double S, double (&x)[16],
+                                           double (&l)[16], bool br_on, double &br)
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const unsigned cw = (unsigned)(w >> (12 * c)) & 0xfffu;
+        if (!(cw & 0x3fu)) continue;
+        const int oa = c & 1, ob = c >> 1;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            if (!((cw >> r) & 1u)) continue;
+            const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
+            const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
+            const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
+            const double s = ((((cw >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
+            const double xp = x[sp], xq = x[sq];
+            if (br_on) {
+                const double lp = l[sp], lq = l[sq];
+                br += s * (lp * xq - lq * xp);
+                l[sp] = s * lq * S + lp * C;
+                l[sq] = -s * lp * S + lq * C;
+            }
+            x[sp] = s * xq * S + xp * C;
+            x[sq] = -s * xp * S + xq * C;
+        }
+    }
+}
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+            const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    __shared__ double2 csL[kOrb2MaxOps][kLanes];
+    __shared__ double accL[BR ? WPB * kOrb2MaxOps * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    const int nk = a.k1 - a.k0;
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+        csL[kk][l] = cs[((long long)c * M + k) * kLanes + l];
+    }
+    if (BR)
+        for (int e = threadIdx.x; e < WPB * kOrb2MaxOps * kLanes; e += kBlock) accL[e] = 0.0;
+    __syncthreads();
+    v0 += c * cstride + lane;
+    if (BR) v1 += c * cstride + lane;
+    const int total = a.nAc * a.nBc;
+    for (int t = blockIdx.x * WPB + wv; t < total; t += gridDim.x * WPB) {
+        const int ia = t / a.nBc, ib = t - ia * a.nBc;
+        const int4 ra = a.aRanks[ia], rb = a.bRanks[ib];
+        const uint32_t fa = a.aInfo[ia], fb = a.bInfo[ib];
+        const int rA[4] = {ra.x, ra.y, ra.z, ra.w}, rB[4] = {rb.x, rb.y, rb.z, rb.w};
+        const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
+        const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
+        if (na_ * nb_ < 2) continue;  // a lone determinant: nothing acts on it
+        const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+        const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        bool ok[16];
+        long long idx[16];
+        double x[16], l[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const int ma = s & 3, mb = s >> 2;
+            ok[s] = rA[ma] >= 0 && rB[mb] >= 0;
+            idx[s] = ((long long)rA[ma] * a.nb + rB[mb]) * kLanes;
+            x[s] = ok[s] ? v0[idx[s]] : 0.0;
+            l[s] = (BR && ok[s]) ? v1[idx[s]] : 0.0;
+        }
+        const unsigned long long *prog = a.prog + type * kOrb2MaxOps;
+        for (int kk = 0; kk < nk; kk++) {
+            const int jk = a.reverse ? nk - 1 - kk : kk;
+            const unsigned long long w = prog[jk];
+            if (!w) continue;
+            const int k = a.k0 + jk;
+            const bool neg = __popc(spect & a.ops[k].mask) & 1;
+            const double2 t2 = csL[kk][lane];
+            double br = 0;
+            switch (a.opgh[jk]) {
+            case 0: orb2_apply<0, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+            case 1: orb2_apply<1, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+            case 2: orb2_apply<0, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+            default: orb2_apply<1, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+            }
+            if (BR) accL[(wv * kOrb2MaxOps + kk) * kLanes + lane] += br;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++)
+            if (ok[s]) {
+                v0[idx[s]] = x[s];
+                if (BR) v1[idx[s]] = l[s];
+            }
+    }
+    if (BR) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+            const int kk = e / kLanes, l = e - kk * kLanes;
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < WPB; w++) s += accL[(w * kOrb2MaxOps + kk) * kLanes + l];
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            partial[(((long long)c * M + k) * slots + blockIdx.x) * kLanes + l] = s;
+        }
+    }
+}
+
+template __global__ void k_orb2v<false>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orb2v<true>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
     k_opv(double *__restrict__ v0, double *__restrict__ v1, long long cstride, long long N, DevSector sec,

@@ -141,10 +141,11 @@ def main():
         rep = a.autotune(thetas, reps=2)
         tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
-                                round(c["ms"], 3)] for c in rep.get("candidates", [])],
-                "candidate_fields": ["sweep", "depth", "lane_min", "ms"]}
+                                c["choice"]["groups"], round(c["ms"], 3)] for c in rep.get("candidates", [])],
+                "candidate_fields": ["sweep", "depth", "lane_min", "groups", "ms"]}
     opts = {"sweep": a.get_option(a.OPT_SWEEP), "fuse_depth": a.get_option(a.OPT_FUSE_DEPTH),
-            "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH)}
+            "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH),
+            "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
@@ -204,8 +205,9 @@ def main():
 
     # sweeps: every segment reads and writes each vector once (forward: psi;
     # adjoint: psi and lambda), 8 B per amplitude per access
-    k_fwd = "k_orbv<false> (lane layout)" if lanes else "k_orb<false>"
-    k_rev = "k_orbv<true> (lane layout)" if lanes else "k_orb<true>"
+    two = lanes and opts["lane_groups"] == 2
+    k_fwd = ("k_orb2v<false>" if two else "k_orbv<false>") + " (lane layout)" if lanes else "k_orb<false>"
+    k_rev = ("k_orb2v<true>" if two else "k_orbv<true>") + " (lane layout)" if lanes else "k_orb<true>"
     rooflines["forward_sweep"] = hbm(k_fwd, 16.0 * N * args.nvec, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
                                      fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
     rooflines["adjoint_sweep"] = hbm(k_rev, 32.0 * N * args.nvec, per_launch(rev_ms, rev_n, 1 if lanes else nseg),

@@ -78,6 +78,27 @@ def test_lane_layout_generic_operators(gpu, storage):
             assert np.max(np.abs(Gl[v] - go)) <= G_TOL
 
 
+def test_two_group_segments_fuse_units(gpu):
+    """TUPS on 8 orbitals: with two orbital groups per pass the lane sweeps
+    take half the passes (consecutive units, also across half-layers, act on
+    disjoint spatial pairs), and give the same batch as one group per pass
+    and as the single-vector path."""
+    w, thetas = _tups_case(8, 4, 1212, 40)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 1)
+        E1, G1 = a.energy_gradients(thetas)
+        n1 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
+        E2, G2 = a.energy_gradients(thetas)
+        n2 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        assert n1 == 28 and n2 == 14, (n1, n2)  # 28 units; consecutive units are disjoint, so they pair up
+        assert np.max(np.abs(E1 - E2)) <= SAME and np.max(np.abs(G1 - G2)) <= SAME
+        for v in (0, 39):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E2[v] - Es) <= SAME and np.max(np.abs(G2[v] - gs)) <= SAME
+
+
 def test_lane_layout_number_sector(gpu):
     """Operators that move an electron between spins: the particle-number
     sector (no S_z tiles), explicit triplet Hamiltonian."""
@@ -137,14 +158,15 @@ def test_options_and_fusion_depth(gpu):
     w, thetas = _tups_case(6, 2, 31, 20)
     with from_workload(w) as a:
         base = a.energy_gradients(thetas)
-        for sweep, depth, lanes in itertools.product((0, 1, 2), (0, 1, 2, 3), (0, 1)):
+        for sweep, depth, lanes, groups in itertools.product((0, 1, 2), (0, 1, 2, 3, 7), (0, 1), (1, 2)):
             a.set_option(AnsatzSynth.OPT_SWEEP, sweep)
             a.set_option(AnsatzSynth.OPT_FUSE_DEPTH, depth)
+            a.set_option(AnsatzSynth.OPT_LANE_GROUPS, groups)
             a.set_option(AnsatzSynth.OPT_LANE_MIN, lanes)
             assert a.get_option(AnsatzSynth.OPT_SWEEP) == sweep
             E, G = a.energy_gradients(thetas)
-            assert np.max(np.abs(E - base[0])) <= SAME, (sweep, depth, lanes)
-            assert np.max(np.abs(G - base[1])) <= SAME, (sweep, depth, lanes)
+            assert np.max(np.abs(E - base[0])) <= SAME, (sweep, depth, lanes, groups)
+            assert np.max(np.abs(G - base[1])) <= SAME, (sweep, depth, lanes, groups)
 
 
 def test_autotune_invariance_and_cache(gpu, tmp_path):
