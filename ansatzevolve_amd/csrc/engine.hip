@@ -146,9 +146,10 @@ struct EngineImpl {
     DBuf<int64_t> d_rowptr;
     DBuf<double> d_h, d_g, d_diag, d_FA, d_FB;  // matrix-free tables
     // lane-layout matrix-free sigma (k_sigv_sparse + two DGEMMs)
-    bool sigv = false;
+    bool sigv = false;  // lane-layout split sigma available (n <= 12)
+    bool sig1 = false;  // vector-major split sigma available (n <= 14)
     DBuf<double> d_Vaa, d_Vbb;
-    DBuf<uint32_t> d_ts;
+    DBuf<uint32_t> d_ts, d_ts1;
     int ntp = 0;
     rocblas_handle blas = nullptr;
     DBuf<double> bl_epart;
@@ -931,6 +932,41 @@ struct EngineImpl {
             }
             return true;
         }
+        if (hkind == 3 && sig1) {
+            // split sigma: sparse part, then the same-spin parts as DGEMMs
+            // over whole vectors, then the energy partials
+            int mk = mark_begin(0);
+            Sig1Args s;
+            s.m = mfargs();
+            s.ts = d_ts1.p;
+            s.ntp = ntp;
+            s.x = x;
+            s.y = y;
+            s.vstride = N;
+            hipLaunchKernelGGL(k_sig1_sparse, dim3(na, nvec), dim3(kSig1Threads), kSig1Lds, stream, s);
+            mark_end(mk);
+            LCHK();
+            mk = mark_begin(3);
+            const double one = 1.0;
+            // alpha part: Y_v (na x nb) += V_aa X_v   (column-major: Y^T += X^T V_aa^T)
+            if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, nb, na, na, &one,
+                                              x, nb, (rocblas_stride)N, d_Vaa.p, na, 0, &one, y, nb,
+                                              (rocblas_stride)N, nvec) != rocblas_status_success) {
+                err = "rocblas_dgemm_strided_batched (alpha part) failed";
+                return false;
+            }
+            // beta part: Y_v += X_v V_bb^T   (column-major: Y^T += V_bb X^T)
+            if (rocblas_dgemm_strided_batched(blas, rocblas_operation_transpose, rocblas_operation_none, nb, na, nb,
+                                              &one, d_Vbb.p, nb, 0, x, nb, (rocblas_stride)N, &one, y, nb,
+                                              (rocblas_stride)N, nvec) != rocblas_status_success) {
+                err = "rocblas_dgemm_strided_batched (beta part) failed";
+                return false;
+            }
+            mark_end(mk);
+            hipLaunchKernelGGL(k_dot_rows, dim3(eslots(), nvec), dim3(kBlock), 0, stream, x, y, N, N, epart);
+            LCHK();
+            return true;
+        }
         if (hkind == 3) {
             const int mk = mark_begin(0);
             hipLaunchKernelGGL(k_mf_sigma, dim3(na, nvec), dim3(kBlock), mf_lds(), stream, mfargs(), x, y, N, epart);
@@ -1284,12 +1320,13 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
         e.nnz = W * e.N;  // implicit entries
         e.hkind = 3;
         e.sigv = false;
-        if (n <= 12 && e.nb <= kSigMaxNb) {
-            // tables of the lane-layout sigma: dense same-spin rows, beta entries
+        e.sig1 = false;
+        if (n <= 14 && e.nb <= kSig1MaxNb) {
+            // tables of the split sigma: dense same-spin rows, beta entries
             const size_t na = e.na, nb = e.nb;
             e.ntp = (kd * (n - kd) + kd + 3) / 4 * 4;
             if (!e.d_Vaa.ensure(na * na, err) || !e.d_Vbb.ensure(nb * nb, err) ||
-                !e.d_ts.ensure(nb * e.ntp, err))
+                !e.d_ts.ensure(nb * e.ntp, err) || !e.d_ts1.ensure(nb * e.ntp, err))
                 return false;
             HCHK(hipMemsetAsync(e.d_Vaa.p, 0, na * na * sizeof(double), e.stream));
             HCHK(hipMemsetAsync(e.d_Vbb.p, 0, nb * nb * sizeof(double), e.stream));
@@ -1299,12 +1336,20 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             hipLaunchKernelGGL(k_dense_spin, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
                                a, 0, e.d_Vbb.p);
             LCHK();
+            const bool lanesOk = n <= 12 && e.nb <= kSigMaxNb;
+            if (lanesOk) {
+                hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                                   e.stream, a, e.d_ts.p, e.ntp, kSigVC * 8, kSigNN);
+                LCHK();
+            }
             hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
-                               a, e.d_ts.p, e.ntp);
+                               a, e.d_ts1.p, e.ntp, 8, kSig1NN);
             LCHK();
             HCHK(hipStreamSynchronize(e.stream));
             HCHK(hipFuncSetAttribute((const void *)k_sigv_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSigLds));
+            HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kSig1Lds));
             if (!e.blas) {
                 if (rocblas_create_handle(&e.blas) != rocblas_status_success) {
                     e.blas = nullptr;
@@ -1316,7 +1361,8 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
                 err = "rocblas_set_stream failed";
                 return false;
             }
-            e.sigv = true;
+            e.sigv = lanesOk;
+            e.sig1 = true;
         }
         return true;
     }
@@ -1405,7 +1451,11 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
     if (e.useGraph && !e.timingOn) {
         if (!e.egGraph) {
             // captured once: the sin/cos tables and every buffer keep their
-            // device addresses, so the graph replays for any angles
+            // device addresses, so the graph replays for any angles.  One
+            // eager run first: library GEMM kernels load lazily and must not
+            // load under capture.
+            if (!enqueue()) return false;
+            HCHK(hipStreamSynchronize(e.stream));
             hipGraph_t g = nullptr;
             HCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
             const bool ok = enqueue();

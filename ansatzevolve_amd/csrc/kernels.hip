@@ -1239,7 +1239,8 @@ __global__ void __launch_bounds__(kBlock) k_dense_spin(MfArgs a, int spin, doubl
 // coefficient slot is zero.  Entry = (byte offset of the source beta J in a
 // staged row of kSigVC doubles) | (byte offset of the coefficient
 // (e + parity * kSigNN), e = r * n + q, in a coefficient row) << 16.
-__global__ void __launch_bounds__(kBlock) k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp)
+__global__ void __launch_bounds__(kBlock)
+    k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff)
 {
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= a.nb) return;
@@ -1251,14 +1252,14 @@ __global__ void __launch_bounds__(kBlock) k_beta_table(MfArgs a, uint32_t *__res
         for (uint32_t w = ~s & full; w; w &= w - 1) {
             const int r = __ffs(o) - 1, q = __ffs(w) - 1;
             const int J = a.rkB[s ^ (1u << r) ^ (1u << q)];
-            const int e = r * n + q + (between_parity(s, r, q) ? kSigNN : 0);
-            row[t++] = (uint32_t)(J * kSigVC * 8) | ((uint32_t)(e * 8) << 16);
+            const int e = r * n + q + (between_parity(s, r, q) ? nnoff : 0);
+            row[t++] = (uint32_t)(J * xunit) | ((uint32_t)(e * 8) << 16);
         }
     for (uint32_t o = s; o; o &= o - 1) {
         const int r = __ffs(o) - 1;
-        row[t++] = (uint32_t)(b * kSigVC * 8) | ((uint32_t)((r * n + r) * 8) << 16);
+        row[t++] = (uint32_t)(b * xunit) | ((uint32_t)((r * n + r) * 8) << 16);
     }
-    for (; t < ntp; t++) row[t] = (uint32_t)(2 * kSigNN * 8) << 16;
+    for (; t < ntp; t++) row[t] = (uint32_t)(2 * nnoff * 8) << 16;
 }
 
 // Lane-layout sigma, sparse part: for output (alpha, beta), the alpha-beta
@@ -1374,6 +1375,115 @@ __global__ void __launch_bounds__(kSigThreads) k_sigv_sparse(SigvArgs a)
         const int beta = slot + kSigSlots * i;
         if (beta < nb) y[((long long)ar * nb + beta) * kLanes + v] = acc[i];
     }
+}
+
+// Matrix-free sigma, sparse part, vector-major vectors (single vectors and
+// small batches, n <= 14): the same split and entry table as k_sigv_sparse
+// with one vector per workgroup, threads over output beta strings, source
+// rows staged two at a time.  LDS: [coef row 0 | coef row 1 | x row 0 |
+// x row 1], coef row = kSig1GS doubles, x row = kSig1MaxNb doubles.
+__global__ void __launch_bounds__(kSig1Threads) k_sig1_sparse(Sig1Args a)
+{
+    extern __shared__ double sm[];
+    const MfArgs &m = a.m;
+    const int n = m.n, nb = m.nb, nn = n * n;
+    const int ar = blockIdx.x;
+    const double *__restrict__ x = a.x + (long long)blockIdx.y * a.vstride;
+    double *__restrict__ y = a.y + (long long)blockIdx.y * a.vstride;
+    char *lds = (char *)sm;
+    double *gs0 = sm, *gs1 = sm + kSig1GS;
+    double *xs0 = sm + 2 * kSig1GS, *xs1 = xs0 + kSig1MaxNb;
+    const int tid = threadIdx.x;
+    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const int ka = __popc(alpha), ne = n - ka;
+    const int R = 1 + ka * ne;
+    double acc[kSig1BI];
+#pragma unroll
+    for (int i = 0; i < kSig1BI; i++) acc[i] = 0;
+    for (int r0 = 0; r0 < R; r0 += 2) {
+        const bool two = r0 + 1 < R;
+        __syncthreads();
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && !two) break;
+            const int r = r0 + h;
+            double *gs = h ? gs1 : gs0;
+            double *xs = h ? xs1 : xs0;
+            for (int e = tid; e < kSig1GS; e += kSig1Threads) gs[e] = 0.0;
+            __syncthreads();
+            int J;
+            if (r == 0) {
+                J = ar;
+                for (int e = tid; e < nn; e += kSig1Threads) {
+                    double vj = 0;
+                    if (e / n != e % n)
+                        for (uint32_t o = alpha; o; o &= o - 1) {
+                            const int xo = __ffs(o) - 1;
+                            vj += thr13(m.g[(size_t)(xo * n + xo) * nn + e]);
+                        }
+                    gs[e] = vj;
+                    gs[kSig1NN + e] = -vj;
+                }
+            } else {
+                const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
+                uint32_t oo = alpha, ww = ~alpha & full;
+                for (int i = 0; i < ir; i++) oo &= oo - 1;
+                for (int i = 0; i < iq; i++) ww &= ww - 1;
+                const int rA = __ffs(oo) - 1, qA = __ffs(ww) - 1;
+                J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
+                const bool neg = between_parity(alpha, rA, qA);
+                const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+                for (int e = tid; e < nn; e += kSig1Threads) {
+                    const double cf = neg ? -thr13(gb[e]) : thr13(gb[e]);
+                    gs[e] = cf;
+                    gs[kSig1NN + e] = -cf;
+                }
+            }
+            const double *src = x + (long long)J * nb;
+            for (int i = tid; i < nb; i += kSig1Threads) xs[i] = src[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSig1BI; i++) {
+            const int beta = tid + kSig1Threads * i;
+            if (beta < nb) {
+                const uint4 *te = (const uint4 *)(a.ts + (size_t)beta * a.ntp);
+                double s0 = 0, s1 = 0;
+                for (int t4 = 0; t4 < a.ntp / 4; t4++) {
+                    const uint4 q = te[t4];
+                    const uint32_t ee[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t xb = ee[u] & 0xffffu, cb = ee[u] >> 16;
+                        s0 += *(const double *)(lds + cb) * *(const double *)(lds + 2 * kSig1GS * 8 + xb);
+                        if (two)
+                            s1 += *(const double *)(lds + kSig1GS * 8 + cb) *
+                                  *(const double *)(lds + (2 * kSig1GS + kSig1MaxNb) * 8 + xb);
+                    }
+                }
+                acc[i] += s0 + s1;
+                if (r0 == 0) acc[i] += m.diag[(long long)ar * nb + beta] * xs0[beta];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSig1BI; i++) {
+        const int beta = tid + kSig1Threads * i;
+        if (beta < nb) y[(long long)ar * nb + beta] = acc[i];
+    }
+}
+
+// part[v * G + block] = block's share of <x_v|y_v>, vectors at stride vstride.
+__global__ void __launch_bounds__(kBlock)
+    k_dot_rows(const double *__restrict__ x, const double *__restrict__ y, long long N, long long vstride,
+               double *__restrict__ part)
+{
+    __shared__ double red[kBlock / 64];
+    const long long o = (long long)blockIdx.y * vstride;
+    double s = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock)
+        s += x[o + i] * y[o + i];
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
 
 // Per-lane dot products of lane-layout chunks: part[(c * G + block) * 64 + l].

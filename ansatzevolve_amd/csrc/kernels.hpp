@@ -304,10 +304,31 @@ struct SigvArgs {
     long long cstride;   // N * 64
 };
 __global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
-__global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp);
+// entries: source byte offset J * xunit, coefficient slot e + parity * nnoff
+__global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff);
 __global__ void k_sigv_sparse(SigvArgs a);  // grid (ceil(na / 8) * 64, chunks), kSigThreads, kSigLds
 __global__ void k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N,
                             double *__restrict__ part);  // grid (G, chunks)
+
+// The same split for vector-major vectors (n <= 14): sparse part per (alpha
+// row, vector), same-spin parts as DGEMMs over whole vectors.
+constexpr int kSig1Threads = 1024;
+constexpr int kSig1MaxNb = 3432;  // C(14, 7)
+constexpr int kSig1BI = (kSig1MaxNb + kSig1Threads - 1) / kSig1Threads;
+constexpr int kSig1NN = 196;      // n <= 14
+constexpr int kSig1GS = 2 * kSig1NN + 8;
+constexpr size_t kSig1Lds = (size_t)(2 * kSig1GS + 2 * kSig1MaxNb) * sizeof(double);
+struct Sig1Args {
+    MfArgs m;
+    const uint32_t *ts;  // nb x ntp entries, xunit 8, nnoff kSig1NN
+    int ntp;
+    const double *x;
+    double *y;
+    long long vstride;
+};
+__global__ void k_sig1_sparse(Sig1Args a);  // grid (na, nvec), kSig1Threads, kSig1Lds
+__global__ void k_dot_rows(const double *__restrict__ x, const double *__restrict__ y, long long N, long long vstride,
+                           double *__restrict__ part);  // grid (G, nvec)
 
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>
