@@ -47,6 +47,11 @@ C_API = [
     ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
+    ("multiBatchCreate", _vp, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("multiBatchUpload", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    ("multiBatchRun", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("multiBatchDownload", ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    ("multiBatchDestroy", ctypes.c_int, [_vp]),
     ("setEngineOption", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp]),
     ("getEngineOption", ctypes.c_int, [ctypes.c_int, _ip, _vp]),
     ("autotune", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
@@ -299,6 +304,53 @@ class AnsatzSynth:
         self._chk("optimise", self.lib.optimise(len(a), a.ctypes.data_as(_dp), ctypes.byref(E), float(stop_energy),
                                                 max_trace, tr.ctypes.data_as(_dp), ctypes.byref(nt), self.ctx))
         return a, E.value, tr[:nt.value].copy()
+
+
+class MultiBatch:
+    """Angle batches of several AnsatzSynth contexts (uneven Hamiltonians)
+    evaluated by one graph launch (multiBatch* entry points)."""
+
+    def __init__(self, contexts):
+        self.lib = load_library()
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.ctx.value for c in self.contexts])
+        self.h = self.lib.multiBatchCreate(len(self.contexts), arr)
+        if not self.h:
+            raise AnsatzError("multiBatchCreate", 5)
+        self.nvec = [0] * len(self.contexts)
+        self.nparam = [c.dimensions()[2] for c in self.contexts]
+
+    def close(self):
+        if self.h:
+            self.lib.multiBatchDestroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, i, thetas):
+        t = np.ascontiguousarray(np.atleast_2d(thetas), dtype=np.float64)
+        r = self.lib.multiBatchUpload(self.h, i, t.shape[0], t.shape[1], t.ctypes.data_as(_dp))
+        if r != 0:
+            raise AnsatzError("multiBatchUpload", r)
+        self.nvec[i] = t.shape[0]
+
+    def run(self, with_gradient=True):
+        r = self.lib.multiBatchRun(self.h, 1 if with_gradient else 0)
+        if r != 0:
+            raise AnsatzError("multiBatchRun", r)
+
+    def download(self, i, with_gradient=True):
+        E = np.zeros(self.nvec[i])
+        G = np.zeros((self.nvec[i], self.nparam[i])) if with_gradient else None
+        r = self.lib.multiBatchDownload(self.h, i, E.ctypes.data_as(_dp), G.ctypes.data_as(_dp) if G is not None
+                                        else None)
+        if r != 0:
+            raise AnsatzError("multiBatchDownload", r)
+        return (E, G) if with_gradient else E
 
 
 def from_workload(w, storage=0):

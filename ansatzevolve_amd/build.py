@@ -11,7 +11,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libansatzsynth.so")
 ARCH = os.environ.get("AE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "engine.hip", "mlp.hip"]
+HIP_SOURCES = ["kernels.hip", "engine.hip", "mlp.hip", "multi.hip"]
 CXX_SOURCES = ["manager.cpp", "newton.cpp", "autotune.cpp", "abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

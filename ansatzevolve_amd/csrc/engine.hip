@@ -468,8 +468,11 @@ struct EngineImpl {
         return a;
     }
 
+    int generation = 0;  // bumped whenever captured sequences become stale
+
     void drop_graphs()
     {
+        generation++;
         if (egGraph) (void)hipGraphExecDestroy(egGraph);
         if (bGraph) (void)hipGraphExecDestroy(bGraph);
         egGraph = bGraph = nullptr;
@@ -1547,6 +1550,70 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
 // least laneMin vectors sweep in lane layout and transpose around the
 // Hamiltonian application.  The sequence is captured once per (batch size,
 // gradient, layout) and replayed as one hipGraph launch.
+// The batch sequence for the uploaded batch on the engine's stream (no graph).
+bool Engine::batch_enqueue(bool with_gradient, std::string &err)
+{
+    EngineImpl &e = *d_;
+    const int nvec = e.bnvec;
+    if (nvec < 1) {
+        err = "no angle batch uploaded";
+        return false;
+    }
+    const bool lanes = e.use_lanes(nvec);
+    const int es = e.eslots();
+    if (lanes) {
+        const int nch = (nvec + kLanes - 1) / kLanes;
+        const long long total = (long long)nch * e.N * kLanes;
+        const dim3 tgrid((unsigned)((e.N + kLanes - 1) / kLanes), nch);
+        hipLaunchKernelGGL(k_bcast_lanes, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                           dim3(kBlock), 0, e.stream, e.d_start.p, e.bl_psi.p, e.N, nch);
+        LCHK();
+        if (!e.sweep_forward_lanes(e.bl_psi.p, nch, err)) return false;
+        if (e.hkind == 3 && e.sigv) {
+            // matrix-free sigma in lane layout: no transposes
+            if (!e.ham_apply_lanes(e.bl_psi.p, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
+            if (with_gradient) {
+                if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
+                hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
+                                   e.M, e.lslots, nvec, e.b_out.p + nvec, 2.0);
+                LCHK();
+            }
+            return true;
+        }
+        hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.bl_psi.p, e.b_psi.p, e.N, nvec);
+        LCHK();
+        if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es,
+                           1LL, e.b_out.p, 1.0);
+        LCHK();
+        if (with_gradient) {
+            hipLaunchKernelGGL(k_to_lanes, tgrid, dim3(kBlock), 0, e.stream, e.b_lam.p, e.bl_lam.p, e.N, nvec);
+            LCHK();
+            if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
+            hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p, e.M,
+                               e.lslots, nvec, e.b_out.p + nvec, 2.0);
+            LCHK();
+        }
+        return true;
+    }
+    const long long total = (long long)nvec * e.N;
+    hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                       dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
+    LCHK();
+    if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
+    if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
+                       e.b_out.p, 1.0);
+    LCHK();
+    if (with_gradient) {
+        if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
+                           e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
+        LCHK();
+    }
+    return true;
+}
+
 bool Engine::batch_run(bool with_gradient, std::string &err)
 {
     EngineImpl &e = *d_;
@@ -1556,60 +1623,7 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
         return false;
     }
     const bool lanes = e.use_lanes(nvec);
-    auto enqueue = [&]() -> bool {
-        const int es = e.eslots();
-        if (lanes) {
-            const int nch = (nvec + kLanes - 1) / kLanes;
-            const long long total = (long long)nch * e.N * kLanes;
-            const dim3 tgrid((unsigned)((e.N + kLanes - 1) / kLanes), nch);
-            hipLaunchKernelGGL(k_bcast_lanes, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
-                               dim3(kBlock), 0, e.stream, e.d_start.p, e.bl_psi.p, e.N, nch);
-            LCHK();
-            if (!e.sweep_forward_lanes(e.bl_psi.p, nch, err)) return false;
-            if (e.hkind == 3 && e.sigv) {
-                // matrix-free sigma in lane layout: no transposes
-                if (!e.ham_apply_lanes(e.bl_psi.p, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
-                if (with_gradient) {
-                    if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
-                    hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
-                                       e.M, e.lslots, nvec, e.b_out.p + nvec, 2.0);
-                    LCHK();
-                }
-                return true;
-            }
-            hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.bl_psi.p, e.b_psi.p, e.N, nvec);
-            LCHK();
-            if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
-            hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es,
-                               1LL, e.b_out.p, 1.0);
-            LCHK();
-            if (with_gradient) {
-                hipLaunchKernelGGL(k_to_lanes, tgrid, dim3(kBlock), 0, e.stream, e.b_lam.p, e.bl_lam.p, e.N, nvec);
-                LCHK();
-                if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
-                hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p, e.M,
-                                   e.lslots, nvec, e.b_out.p + nvec, 2.0);
-                LCHK();
-            }
-            return true;
-        }
-        const long long total = (long long)nvec * e.N;
-        hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
-                           dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
-        LCHK();
-        if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
-        if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
-        hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
-                           e.b_out.p, 1.0);
-        LCHK();
-        if (with_gradient) {
-            if (!e.sweep_reverse(e.b_psi.p, e.b_lam.p, nvec, e.b_csneg.p, e.b_partial.p, err)) return false;
-            hipLaunchKernelGGL(k_reduce, dim3(nvec * e.M), dim3(kBlock), 0, e.stream, e.b_partial.p, nvec * e.M,
-                               e.slots, (long long)e.slots, 1LL, e.b_out.p + nvec, 2.0);
-            LCHK();
-        }
-        return true;
-    };
+    auto enqueue = [&]() -> bool { return batch_enqueue(with_gradient, err); };
     if (!e.useGraph || e.timingOn) return enqueue();
     if (e.bGraph && (e.bGraphNvec != nvec || e.bGraphGrad != with_gradient || e.bGraphLanes != lanes)) {
         (void)hipGraphExecDestroy(e.bGraph);
@@ -1654,6 +1668,12 @@ bool Engine::batch_download(double *E, double *deriv, std::string &err)
 }
 
 int Engine::batch_size() const { return d_->bnvec; }
+
+void *Engine::stream() const { return (void *)d_->stream; }
+
+bool Engine::timing_on() const { return d_->timingOn; }
+
+int Engine::generation() const { return d_->generation; }
 
 bool Engine::set_option(int key, int value, std::string &err)
 {

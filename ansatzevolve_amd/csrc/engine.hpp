@@ -89,6 +89,12 @@ public:
     // copy energies and per-operator derivatives).
     bool batch_upload(int nvec, const double *angles, std::string &err);
     bool batch_run(bool with_gradient, std::string &err);
+    // The batch sequence enqueued on the engine's stream without a graph
+    // (batch_run replays it from a hipGraph); for callers that capture several
+    // engines into one graph.
+    bool batch_enqueue(bool with_gradient, std::string &err);
+    // The engine's HIP stream (hipStream_t).
+    void *stream() const;
     bool batch_download(double *E, double *deriv, std::string &err);
     int batch_size() const;
     bool synchronize(std::string &err);
@@ -98,6 +104,10 @@ public:
     // sweep, 3 dense same-spin GEMMs of the lane-layout sigma.  Lane-layout
     // sweeps record one interval per launch.
     void set_timing(bool on);
+    bool timing_on() const;
+    // Changes whenever previously captured launch sequences are stale
+    // (new operators, Hamiltonian, start state or engine options).
+    int generation() const;
     bool timing(int cls, double *ms, long long *launches, std::string &err);
 
     bool set_option(int key, int value, std::string &err);

@@ -56,6 +56,9 @@ public:
 
     int numberOfParameters() const { return m_numberOfUniqueParameters; }
     int batchAngleCount() const { return m_batchNangles; }
+    // The device engine after construction (nullptr before); used by the
+    // multi-instance batch, which holds every involved manager's lock.
+    Engine *engine() { return m_isConstructed ? m_engine.get() : nullptr; }
 
 private:
     bool validateToRun();

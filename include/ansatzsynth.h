@@ -204,6 +204,23 @@ int getEngineOption(int key, int *value, void *ctx);
 int autotune(int nvec, int nangles, const double *angles, const char *cachefile, int reps, char *report,
              int reportlen, void *ctx);
 
+/* Several contexts evaluated by one launch: each context (its own
+ * Hamiltonian, sector size and operators; constructed) gets an angle batch
+ * with multiBatchUpload, multiBatchRun enqueues all of them as parallel
+ * branches of one hipGraph (captured once per shape) and returns without
+ * waiting, multiBatchDownload waits and copies one context's energies and
+ * per-parameter gradients (may be NULL).  A context may belong to one
+ * multi-batch at a time; the contexts stay owned by the caller and must
+ * outlive the multi-batch.  No reference counterpart (the reference
+ * evaluates one Hamiltonian per manager).
+ * Codes: 1 NULL handle, 2 bad index or NULL pointer, 3 nvec or nangles < 1,
+ * 5 failure, -6 gradients requested with nangles != number of parameters. */
+void *multiBatchCreate(int nctx, void *const *ctxs);
+int multiBatchUpload(void *mb, int index, int nvec, int nangles, const double *angles);
+int multiBatchRun(void *mb, int withgradient);
+int multiBatchDownload(void *mb, int index, double *energies, double *gradients);
+int multiBatchDestroy(void *mb);
+
 /* ---------------- Part 3: learned warm start (no reference counterpart) ----
  * A 3-layer FP32 network x -> tanh(x W0 + b0) -> tanh(. W1 + b1) -> . W2 + b2
  * proposing initial angles from integral features, with device kernels for

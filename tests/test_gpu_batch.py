@@ -209,3 +209,34 @@ def test_lane_sigma_config2_against_reference(gpu):
             i = v % len(th)
             assert abs(E[v] - gz["cfg2_E"][i]) <= E_TOL
             assert np.max(np.abs(G[v] - gz["cfg2_grad"][i])) <= G_TOL
+
+
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
+def test_multi_instance_batch_uneven(gpu, storage):
+    """Three Hamiltonians of uneven sector size (4, 6 and 8 orbitals: 36,
+    400 and 4,900 determinants) with batches of 5, 17 and 70 angle vectors,
+    evaluated by one graph launch; every instance equals its own batched
+    evaluation, on the first launch (capture) and on replays with new angles."""
+    from ansatzevolve_amd import MultiBatch
+    specs = [(4, 2, 61, 5), (6, 3, 62, 17), (8, 4, 63, 70)]
+    cases = [_tups_case(n, L, seed, nvec) for n, L, seed, nvec in specs]
+    ctxs = [from_workload(w, storage=storage) for w, _ in cases]
+    try:
+        refs = [a.energy_gradients(th) for a, (_, th) in zip(ctxs, cases)]
+        others = [seeded_angles(th.shape[1], 900 + i, count=th.shape[0]) for i, (_, th) in enumerate(cases)]
+        refs2 = [a.energy_gradients(th) for a, th in zip(ctxs, others)]
+        with MultiBatch(ctxs) as mb:
+            for rep in range(3):
+                use = cases if rep != 1 else [(None, o) for o in others]
+                exp = refs if rep != 1 else refs2
+                for i, (_, th) in enumerate(use):
+                    mb.upload(i, th)
+                mb.run(True)
+                for i in range(len(ctxs)):
+                    E, G = mb.download(i)
+                    assert np.max(np.abs(E - exp[i][0])) <= SAME, (rep, i)
+                    assert np.max(np.abs(G - exp[i][1])) <= SAME, (rep, i)
+                ctxs[0].energy_gradient(cases[0][1][0])  # other device work between launches
+    finally:
+        for a in ctxs:
+            a.close()
