@@ -26,7 +26,7 @@ void log_msg(const char *fmt, ...);
 namespace {
 
 struct Candidate {
-    int sweep, depth, laneMin, groups;
+    int sweep, depth, laneMin, groups, groupsFwd;
 };
 
 std::string shape_key(const Engine &eng, int nvec, int nops)
@@ -40,7 +40,7 @@ std::string cand_json(const Candidate &c)
 {
     std::ostringstream o;
     o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
-      << ", \"groups\": " << c.groups << "}";
+      << ", \"groups\": " << c.groups << ", \"groups_fwd\": " << c.groupsFwd << "}";
     return o.str();
 }
 
@@ -56,8 +56,9 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         if (line.find(tag) == std::string::npos) continue;
         Candidate t{};
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
-                        "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d}",
-                        &t.sweep, &t.depth, &t.laneMin, &t.groups) == 4) {
+                        "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d, "
+                        "\"groups_fwd\": %d}",
+                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd) == 5) {
             c = t;
             found = true;
         }
@@ -68,7 +69,8 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
 bool apply(Engine &eng, const Candidate &c, std::string &err)
 {
     return eng.set_option(kOptSweep, c.sweep, err) && eng.set_option(kOptFuseDepth, c.depth, err) &&
-           eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneMin, c.laneMin, err) &&
+           eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneGroupsFwd, c.groupsFwd, err) &&
+           eng.set_option(kOptLaneMin, c.laneMin, err) &&
            eng.set_option(kOptGraph, 1, err);
 }
 
@@ -105,14 +107,18 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
             if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
             for (int d : depths) {
                 if (!sz && d != 0) continue;
-                cands.push_back({s, d, lanes ? 1 : 0, 1});
+                cands.push_back({s, d, lanes ? 1 : 0, 1, 1});
                 // the lane layout can also fuse two disjoint groups per pass
-                if (lanes && sz && s == kSweepOrbit && d == 0) cands.push_back({s, d, 1, 2});
+                // (forward only, or both sweeps)
+                if (lanes && sz && s == kSweepOrbit && d == 0) {
+                    cands.push_back({s, d, 1, 1, 2});
+                    cands.push_back({s, d, 1, 2, 2});
+                }
             }
         }
 
     const Candidate saved{eng.option(kOptSweep), eng.option(kOptFuseDepth), eng.option(kOptLaneMin),
-                          eng.option(kOptLaneGroups)};
+                          eng.option(kOptLaneGroups), eng.option(kOptLaneGroupsFwd)};
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";

@@ -124,12 +124,19 @@ struct EngineImpl {
     DBuf<DevOp> d_ops;
     DBuf<uint32_t> d_prog;  // orbit programs of the orbit segments (OrbArgs::prog)
     std::vector<Seg> segs;
-    std::vector<Seg> lsegs;  // segmentation of the lane-layout sweeps
+    // segmentation of the lane-layout sweeps: adjoint (lsegs) and forward
+    // (lsegsF) may fuse different numbers of orbital groups per pass
+    struct LaneList {
+        std::vector<Seg> segs;
+        DBuf<uint32_t> lprog;             // one-group orbit programs
+        DBuf<unsigned long long> prog2;   // two-group orbit programs
+        DBuf<uint32_t> opgh;
+    };
+    LaneList laneA, laneF;
+    std::vector<Seg> &lsegs = laneA.segs;
     std::vector<std::unique_ptr<GroupSet>> gsets;
-    DBuf<uint32_t> d_lprog;                 // one-group orbit programs of lsegs
-    DBuf<unsigned long long> d_prog2;       // two-group orbit programs of lsegs
-    DBuf<uint32_t> d_opgh;
-    int maxGroups = 1;  // groups per spin of a lane-layout orbit segment (1 or 2; the autotuner measures both)
+    int maxGroups = 1;   // adjoint: groups per spin of a lane-layout orbit segment (1 or 2; tuned)
+    int maxGroupsF = 1;  // forward
     std::vector<std::unique_ptr<TileSet>> tilesets;
     int slots = 1, Ggen = 1;
     int ngeneric = 0;
@@ -307,11 +314,29 @@ struct EngineImpl {
     // operators run one operator per pass (k_opv).
     bool build_lane_segments(std::string &err)
     {
+        if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
+        // enough waves to keep the chip's HBM pipeline full: every resident
+        // wave holds two orbits' loads in flight (one 16-member orbit in the
+        // two-group kernel)
+        auto blocks = [&](const std::vector<Seg> &l, long long cap) {
+            long long work = 1;
+            for (const Seg &sg : l) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : N);
+            const long long waves = (work + 1) / 2;
+            return (int)std::max<long long>(1, std::min<long long>(cap, (waves + 3) / 4));
+        };
+        lslots = blocks(laneA.segs, 1024);
+        lslotsF = blocks(laneF.segs, 1536);
+        return true;
+    }
+
+    bool build_lane_list(LaneList &L, int maxG, std::string &err)
+    {
+        std::vector<Seg> &lsegs = L.segs;
         lsegs.clear();
         if (mode != kSz || sweepKind == kSweepTile) {
             lsegs = segs;
             for (Seg &sg : lsegs) sg.two = 0;
-            return build_orbit_programs(lsegs, d_lprog, err);
+            return build_orbit_programs(lsegs, L.lprog, err);
         }
         const uint32_t amask = (1u << n) - 1;
         auto bitsA = [&](int kk) { return ((hops[kk].cre | hops[kk].ann) >> n) & amask; };
@@ -327,7 +352,7 @@ struct EngineImpl {
                 if (g[i] == u) return (int)i;
                 if (g[i] & u) return -1;
             }
-            if ((int)g.size() >= maxGroups) return -1;
+            if ((int)g.size() >= maxG) return -1;
             g.push_back(u);
             return (int)g.size() - 1;
         };
@@ -410,13 +435,13 @@ struct EngineImpl {
                     if (k + kk < k1) {
                         const DevOp &o = hops[k + kk];
                         const int g = (int)(opgh[ghOff + kk] & 1), h = (int)((opgh[ghOff + kk] >> 1) & 1);
-                        for (int c = 0; c < 4; c++) {
+                        auto slotc = [&](int c, int i, int j) {
                             const int oa = c & 1, ob = c >> 1;
-                            auto slot = [&](int i, int j) {
-                                return ((i << g) | (oa << (1 - g))) + 4 * ((j << h) | (ob << (1 - h)));
-                            };
+                            return ((i << g) | (oa << (1 - g))) + 4 * ((j << h) | (ob << (1 - h)));
+                        };
+                        for (int c = 0; c < 4; c++)
                             for (int r = 0; r < 6; r++) {
-                                const int sp = slot(PP[r] >> 1, PP[r] & 1), sq = slot(QQ[r] >> 1, QQ[r] & 1);
+                                const int sp = slotc(c, PP[r] >> 1, PP[r] & 1), sq = slotc(c, QQ[r] >> 1, QQ[r] & 1);
                                 if (!valid(sp) || !valid(sq)) continue;
                                 uint32_t b2 = 0;
                                 const int sgn = host_pair_sign(o, str(sp), b2);
@@ -424,7 +449,23 @@ struct EngineImpl {
                                 w |= 1ull << (12 * c + r);
                                 if (sgn < 0) w |= 1ull << (12 * c + 6 + r);
                             }
+                        // uniform form (bit 63, word in bits 0..11): one 12-bit word serves all four
+                        // sub-orbits when every combo either has exactly that word or has both
+                        // members of each of its pairs absent (absent amplitudes are zero in
+                        // registers and never stored, so rotating them changes nothing)
+                        unsigned long long m = 0;
+                        for (int c = 0; c < 4; c++) m |= (w >> (12 * c)) & 0xfffull;
+                        bool uni = m != 0;
+                        for (int c = 0; uni && c < 4; c++) {
+                            const unsigned long long wc = (w >> (12 * c)) & 0xfffull;
+                            if (wc == m) continue;
+                            for (int r = 0; r < 6; r++) {
+                                if (!((m >> r) & 1)) continue;
+                                const int sp = slotc(c, PP[r] >> 1, PP[r] & 1), sq = slotc(c, QQ[r] >> 1, QQ[r] & 1);
+                                if (valid(sp) || valid(sq)) uni = false;
+                            }
                         }
+                        if (uni) w = (1ull << 63) | m;
                     }
                     prog2.push_back(w);
                 }
@@ -435,19 +476,19 @@ struct EngineImpl {
         }
         if (prog2.empty()) prog2.push_back(0);
         if (opgh.empty()) opgh.push_back(0);
-        if (!d_prog2.ensure(prog2.size(), err) || !d_opgh.ensure(opgh.size(), err)) return false;
-        HCHK(hipMemcpy(d_prog2.p, prog2.data(), prog2.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
-        HCHK(hipMemcpy(d_opgh.p, opgh.data(), opgh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!L.prog2.ensure(prog2.size(), err) || !L.opgh.ensure(opgh.size(), err)) return false;
+        HCHK(hipMemcpy(L.prog2.p, prog2.data(), prog2.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(L.opgh.p, opgh.data(), opgh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         // one-group segments of the lane list get their own programs
         std::vector<uint32_t> save(lsegs.size());
         for (size_t i = 0; i < lsegs.size(); i++) save[i] = (uint32_t)lsegs[i].progOff;
-        if (!build_orbit_programs(lsegs, d_lprog, err)) return false;
+        if (!build_orbit_programs(lsegs, L.lprog, err)) return false;
         for (size_t i = 0; i < lsegs.size(); i++)
             if (lsegs[i].two) lsegs[i].progOff = (int)save[i];
         return true;
     }
 
-    Orb2Args orb2args(const Seg &sg, bool reverse) const
+    Orb2Args orb2args(const LaneList &L, const Seg &sg, bool reverse) const
     {
         const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
         Orb2Args a;
@@ -463,8 +504,8 @@ struct EngineImpl {
         a.k0 = sg.k0;
         a.k1 = sg.k1;
         a.reverse = reverse ? 1 : 0;
-        a.prog = d_prog2.p + sg.prog2Off;
-        a.opgh = d_opgh.p + sg.progOff;
+        a.prog = L.prog2.p + sg.prog2Off;
+        a.opgh = L.opgh.p + sg.progOff;
         return a;
     }
 
@@ -801,17 +842,18 @@ struct EngineImpl {
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
-        for (const Seg &sg : lsegs) {
+        for (const Seg &sg : laneF.segs) {
             if (sg.two) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orb2v<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, orb2args(sg, false),
-                                   v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                hipLaunchKernelGGL((k_orb2v<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream,
+                                   orb2args(laneF, sg, false), v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream,
-                                   orbargs(sg, false, d_lprog.p), v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                                   orbargs(sg, false, laneF.lprog.p), v, nullptr, cstride, bl_cs.p, M, nullptr,
+                                   lslotsF);
                 mark_end(mk);
                 LCHK();
             } else {
@@ -836,14 +878,14 @@ struct EngineImpl {
             const Seg &sg = lsegs[si];
             if (sg.two) {
                 const int mk = mark_begin(2);
-                hipLaunchKernelGGL((k_orb2v<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, orb2args(sg, true), phi,
-                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                hipLaunchKernelGGL((k_orb2v<true>), dim3(lslots, nch), dim3(kBlock), 0, stream,
+                                   orb2args(laneA, sg, true), phi, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(2);
                 hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream,
-                                   orbargs(sg, true, d_lprog.p), phi, lam, cstride, bl_csneg.p, M, bl_partial.p,
+                                   orbargs(sg, true, laneA.lprog.p), phi, lam, cstride, bl_csneg.p, M, bl_partial.p,
                                    lslots);
                 mark_end(mk);
                 LCHK();
@@ -1219,16 +1261,6 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     e.slots = std::max(maxTiles, e.Ggen);
     if (!e.build_orbit_programs(err)) return false;
     if (!e.build_lane_segments(err)) return false;
-    {
-        long long work = 1;
-        for (const Seg &sg : e.lsegs) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : e.N);
-        // enough waves to keep the chip's HBM pipeline full: every resident
-        // wave (4 per SIMD at the adjoint kernel's register use, 6 forward)
-        // holds two orbits' loads in flight
-        const long long waves = (work + 1) / 2;
-        e.lslots = (int)std::max<long long>(1, std::min<long long>(1024, (waves + 3) / 4));
-        e.lslotsF = (int)std::max<long long>(1, std::min<long long>(1536, (waves + 3) / 4));
-    }
     if (!e.d_cs.ensure(M, err) || !e.d_csneg.ensure(M, err) || !e.d_partial.ensure((size_t)M * e.slots, err) ||
         !e.d_epart.ensure(e.eslots_alloc(), err) || !e.d_out.ensure(M + 1, err))
         return false;
@@ -1731,23 +1763,21 @@ bool Engine::set_option(int key, int value, std::string &err)
         return true;
     }
     case kOptLaneGroups:
+    case kOptLaneGroupsFwd: {
         if (value < 1 || value > 2) {
             err = "lane groups must be 1 or 2";
             return false;
         }
-        if (value == e.maxGroups) return true;
-        e.maxGroups = value;
+        int &dst = key == kOptLaneGroups ? e.maxGroups : e.maxGroupsF;
+        if (value == dst) return true;
+        dst = value;
         if (e.M > 0) {
             e.drop_graphs();
             if (!e.build_lane_segments(err)) return false;
-            long long work = 1;
-            for (const Seg &sg : e.lsegs) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : e.N);
-            const long long waves = (work + 1) / 2;
-            e.lslots = (int)std::max<long long>(1, std::min<long long>(1024, (waves + 3) / 4));
-            e.lslotsF = (int)std::max<long long>(1, std::min<long long>(1536, (waves + 3) / 4));
         }
         e.bnvec = 0;
         return true;
+    }
     case kOptHamKind:
     case kOptSegments:
     case kOptLaneSegments:
@@ -1771,6 +1801,7 @@ int Engine::option(int key) const
     case kOptHamKind: return e.hkind;
     case kOptSegments: return (int)e.segs.size();
     case kOptLaneGroups: return e.maxGroups;
+    case kOptLaneGroupsFwd: return e.maxGroupsF;
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

@@ -32,8 +32,9 @@ enum EngineOption : int {
     kOptLdsBudget = 4,  // bytes of LDS an LDS-tile segment may use (0: the device's limit)
     kOptHamKind = 5,    // read only: 1 rows, 2 uniform-width rows, 3 matrix-free
     kOptSegments = 6,   // read only: fused segments per sweep (vector-major sweeps)
-    kOptLaneGroups = 7, // lane-layout orbit segments: disjoint 2-bit groups per spin (1 or 2)
-    kOptLaneSegments = 8,  // read only: fused segments per lane-layout sweep
+    kOptLaneGroups = 7, // lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin and pass (1 or 2)
+    kOptLaneSegments = 8,  // read only: fused segments per lane-layout adjoint sweep
+    kOptLaneGroupsFwd = 9, // the same for the lane-layout forward sweep
 };
 
 class Engine {

@@ -552,6 +552,37 @@ __device__ __forceinline__ void orb2_apply(unsigned long long w, bool neg, doubl
     }
 }
 
+// Uniform operator word: the same 6 pair bits and phases in all four
+// sub-orbits; each active pair is rotated in every sub-orbit.
+template <int G, int H>
+__device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double C, double S, double (&x)[16],
+                                                   double (&l)[16], bool br_on, double &br)
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (!((w >> r) & 1u)) continue;
+        const double s = ((((w >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
+        const double sS = s * S;
+        const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int oa = c & 1, ob = c >> 1;
+            const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
+            const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
+            const double xp = x[sp], xq = x[sq];
+            if (br_on) {
+                const double lp = l[sp], lq = l[sq];
+                br += s * (lp * xq - lq * xp);
+                l[sp] = lq * sS + lp * C;
+                l[sq] = -lp * sS + lq * C;
+            }
+            x[sp] = xq * sS + xp * C;
+            x[sq] = -xp * sS + xq * C;
+        }
+    }
+}
+
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
     k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
@@ -585,14 +616,15 @@ __global__ void __launch_bounds__(kBlock)
         if (na_ * nb_ < 2) continue;  // a lone determinant: nothing acts on it
         const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
         const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
         bool ok[16];
-        long long idx[16];
+        uint32_t idx[16];
         double x[16], l[16];
 #pragma unroll
         for (int s = 0; s < 16; s++) {
             const int ma = s & 3, mb = s >> 2;
             ok[s] = rA[ma] >= 0 && rB[mb] >= 0;
-            idx[s] = ((long long)rA[ma] * a.nb + rB[mb]) * kLanes;
+            idx[s] = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
             x[s] = ok[s] ? v0[idx[s]] : 0.0;
             l[s] = (BR && ok[s]) ? v1[idx[s]] : 0.0;
         }
@@ -605,11 +637,21 @@ __global__ void __launch_bounds__(kBlock)
             const bool neg = __popc(spect & a.ops[k].mask) & 1;
             const double2 t2 = csL[kk][lane];
             double br = 0;
-            switch (a.opgh[jk]) {
-            case 0: orb2_apply<0, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-            case 1: orb2_apply<1, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-            case 2: orb2_apply<0, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-            default: orb2_apply<1, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+            if (w >> 63) {
+                const unsigned wu = (unsigned)w & 0xfffu;
+                switch (a.opgh[jk]) {
+                case 0: orb2_apply_uniform<0, 0>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
+                case 1: orb2_apply_uniform<1, 0>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
+                case 2: orb2_apply_uniform<0, 1>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
+                default: orb2_apply_uniform<1, 1>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
+                }
+            } else {
+                switch (a.opgh[jk]) {
+                case 0: orb2_apply<0, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+                case 1: orb2_apply<1, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+                case 2: orb2_apply<0, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+                default: orb2_apply<1, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
+                }
             }
             if (BR) accL[(wv * kOrb2MaxOps + kk) * kLanes + lane] += br;
         }

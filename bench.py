@@ -45,6 +45,8 @@ def parse():
                    help="Hamiltonian form: 0 automatic, 1 explicit rows, 2 matrix-free sigma")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the library's default sweep kind, fusion depth and batch layout")
+    p.add_argument("--lane-groups", type=int, default=0,
+                   help="force the lane-layout orbit groups per pass (1 or 2) after autotuning")
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain 2 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -142,11 +144,16 @@ def main():
         rep = a.autotune(thetas, reps=2)
         tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
-                                c["choice"]["groups"], round(c["ms"], 3)] for c in rep.get("candidates", [])],
-                "candidate_fields": ["sweep", "depth", "lane_min", "groups", "ms"]}
+                                c["choice"]["groups"], c["choice"]["groups_fwd"], round(c["ms"], 3)]
+                               for c in rep.get("candidates", [])],
+                "candidate_fields": ["sweep", "depth", "lane_min", "groups", "groups_fwd", "ms"]}
+    if args.lane_groups:
+        a.set_option(a.OPT_LANE_GROUPS, args.lane_groups)
+        a.set_option(a.OPT_LANE_GROUPS_FWD, args.lane_groups)
     opts = {"sweep": a.get_option(a.OPT_SWEEP), "fuse_depth": a.get_option(a.OPT_FUSE_DEPTH),
             "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH),
-            "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS)}
+            "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_groups_fwd": a.get_option(a.OPT_LANE_GROUPS_FWD),
+            "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
@@ -206,9 +213,10 @@ def main():
 
     # sweeps: every segment reads and writes each vector once (forward: psi;
     # adjoint: psi and lambda), 8 B per amplitude per access
-    two = lanes and opts["lane_groups"] == 2
-    k_fwd = ("k_orb2v<false>" if two else "k_orbv<false>") + " (lane layout)" if lanes else "k_orb<false>"
-    k_rev = ("k_orb2v<true>" if two else "k_orbv<true>") + " (lane layout)" if lanes else "k_orb<true>"
+    k_fwd = ("k_orb2v<false>" if opts["lane_groups_fwd"] == 2 else "k_orbv<false>") + " (lane layout)" if lanes \
+        else "k_orb<false>"
+    k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
+        else "k_orb<true>"
     rooflines["forward_sweep"] = hbm(k_fwd, 16.0 * N * args.nvec, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
                                      fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
     rooflines["adjoint_sweep"] = hbm(k_rev, 32.0 * N * args.nvec, per_launch(rev_ms, rev_n, 1 if lanes else nseg),

@@ -183,10 +183,11 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *   5 (read only) Hamiltonian form: 1 rows, 2 uniform-width rows,
  *     3 matrix-free;
  *   6 (read only) fused segments per sweep (vector-major sweeps);
- *   7 lane-layout orbit segments: disjoint 2-bit orbital groups per spin
+ *   7 lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin
  *     fused into one pass (1 or 2, default 1);
- *   8 (read only) fused segments per lane-layout sweep.
- * Changing keys 0, 3, 4 or 7 re-segments the operators; an uploaded angle
+ *   8 (read only) fused segments per lane-layout adjoint sweep;
+ *   9 the same as 7 for the lane-layout forward sweep.
+ * Changing keys 0, 3, 4, 7 or 9 re-segments the operators; an uploaded angle
  * batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */
 int setEngineOption(int key, int value, void *ctx);

@@ -89,9 +89,12 @@ def test_two_group_segments_fuse_units(gpu):
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 1)
         E1, G1 = a.energy_gradients(thetas)
         n1 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
+        Ef, Gf = a.energy_gradients(thetas)  # two-group forward, one-group adjoint
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
         E2, G2 = a.energy_gradients(thetas)
         n2 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        assert np.max(np.abs(Ef - E1)) <= SAME and np.max(np.abs(Gf - G1)) <= SAME
         assert n1 == 28 and n2 == 14, (n1, n2)  # 28 units; consecutive units are disjoint, so they pair up
         assert np.max(np.abs(E1 - E2)) <= SAME and np.max(np.abs(G1 - G2)) <= SAME
         for v in (0, 39):
