@@ -222,3 +222,66 @@ def test_graph_replay_matches_eager(gpu):
                 Eo, go = o.energy_gradient(th)
                 assert abs(E - Eo) <= E_TOL and np.max(np.abs(g - go)) <= G_TOL
                 a.energies(thetas[:2])            # other device work between replays
+
+
+def _triplets_from_dense(states, Hd, thr=0.0):
+    ii, jj = np.nonzero(np.abs(Hd) > thr)
+    return np.array([states[i] for i in ii]), np.array([states[j] for j in jj]), Hd[ii, jj]
+
+
+@pytest.mark.parametrize("case", ["superposition_sz", "full_space", "one_parameter"])
+def test_sector_modes_and_start_states(gpu, case):
+    """Start states and sectors beyond a single determinant: a normalised
+    superposition within one S_z sector; a start that mixes particle numbers
+    (the full Fock space, as the reference's compressor-free path); and an
+    ansatz with a single parameter shared by every operator (scales 1, -0.5,
+    2).  Energy, gradient, Hessian and the final state against the oracle."""
+    nq = 8
+    rng = np.random.default_rng({"superposition_sz": 1, "full_space": 2, "one_parameter": 3}[case])
+    if case == "superposition_sz":
+        start = [0b00110011, 0b01010101, 0b10011001]  # 2 up + 2 down electrons each
+        coeff = np.array([0.6, -0.48, 0.64])
+        states = [b for b in range(256) if bin(b & 0xF).count("1") == 2 and bin(b >> 4).count("1") == 2]
+        ops = np.array([[0, 1, -1, -1], [4, 5, -1, -1], [0, 4, 2, 6], [1, 3, -1, -1], [5, 7, -1, -1],
+                        [2, 6, 3, 7]])
+        order = np.array([0, 0, 1, 2, 2, 3])
+        scale = None
+    elif case == "full_space":
+        start = [0b00000011, 0b00000111]  # two and three electrons
+        coeff = np.array([0.8, 0.6])
+        states = list(range(256))
+        ops = np.array([[2, 0, -1, -1], [4, 1, -1, -1], [3, 5, 0, 1], [6, 2, -1, -1], [7, 3, 4, 5]])
+        order = np.array([0, 1, 2, 3, 4])
+        scale = None
+    else:
+        start = [0b00110011]
+        coeff = np.array([1.0])
+        states = [b for b in range(256) if bin(b & 0xF).count("1") == 2 and bin(b >> 4).count("1") == 2]
+        ops = np.array([[0, 2, -1, -1], [4, 6, -1, -1], [1, 5, 2, 6]])
+        order = np.array([0, 0, 0])
+        scale = np.array([1.0, -0.5, 2.0])
+    Hd = rng.normal(size=(len(states), len(states)))
+    Hd = 0.5 * (Hd + Hd.T)
+    ti, tj, tv = _triplets_from_dense(states, Hd, 0.3)
+    P = int(order.max()) + 1
+    o = OracleEngine(nq, start, coeff, ops, order, scale)
+    o.set_triplets(ti, tj, tv)
+    with AnsatzSynth() as a:
+        a.set_initial_state(nq, [s + 1 for s in start], coeff)
+        a.set_hamiltonian(ti + 1, tj + 1, tv)
+        a.set_excitation(ops + 1, order + 1, scale)
+        for th in seeded_angles(P, 70, count=3):
+            E, g = a.energy_gradient(th)
+            Eo, go = o.energy_gradient(th)
+            assert abs(E - Eo) <= E_TOL
+            assert np.max(np.abs(g - go)) <= G_TOL
+            H = a.hessian(th)
+            Ho, _, _ = o.hessian(th)
+            assert np.max(np.abs(H - Ho)) <= H_TOL
+            st = a.final_state(th, nq)
+            so = np.zeros(256)
+            so[o.states()] = o.evolve(th)
+            assert np.max(np.abs(st - so)) <= 1e-12
+        Eb = a.energies(seeded_angles(P, 71, count=20))
+        for v, th in enumerate(seeded_angles(P, 71, count=20)):
+            assert abs(Eb[v] - o.energy(th)) <= E_TOL
