@@ -130,7 +130,6 @@ struct EngineImpl {
         std::vector<Seg> segs;
         DBuf<uint32_t> lprog;             // one-group orbit programs
         DBuf<unsigned long long> prog2;   // two-group orbit programs
-        DBuf<uint32_t> opgh;
     };
     LaneList laneA, laneF;
     std::vector<Seg> &lsegs = laneA.segs;
@@ -343,79 +342,33 @@ struct EngineImpl {
         auto bitsB = [&](int kk) { return (hops[kk].cre | hops[kk].ann) & amask; };
         const int maxOps = std::min(kOrb2MaxOps, fuseDepth > 0 ? fuseDepth : kOrb2MaxOps);
         std::vector<unsigned long long> prog2;
-        std::vector<uint32_t> opgh;
-        // try to add a group mask u to the list g (size <= maxGroups): returns slot or -1
-        auto join = [&](std::vector<uint32_t> &g, uint32_t u) -> int {
-            if (u == 0) return 0;
-            if (__builtin_popcount(u) != 2) return -1;
-            for (size_t i = 0; i < g.size(); i++) {
-                if (g[i] == u) return (int)i;
-                if (g[i] & u) return -1;
-            }
-            if ((int)g.size() >= maxG) return -1;
-            g.push_back(u);
-            return (int)g.size() - 1;
-        };
-        int k = 0;
-        while (k < M) {
-            std::vector<uint32_t> ga, gb;
-            int k1 = k;
-            while (k1 < M && k1 - k < maxOps) {
-                std::vector<uint32_t> ta = ga, tb = gb;
-                if (join(ta, bitsA(k1)) < 0 || join(tb, bitsB(k1)) < 0 || (bitsA(k1) | bitsB(k1)) == 0) break;
-                ga.swap(ta);
-                gb.swap(tb);
+        // a one-group run from k0: operators whose alpha bits are empty or one
+        // fixed 2-bit group, and likewise beta (a TUPS unit)
+        auto run = [&](int k0, uint32_t &ua, uint32_t &ub) {
+            ua = ub = 0;
+            int k1 = k0;
+            while (k1 < M && k1 - k0 < std::min(kOrbMaxOps, maxOps)) {
+                const uint32_t a = bitsA(k1), b = bitsB(k1);
+                if ((a | b) == 0) break;
+                if (a && (__builtin_popcount(a) != 2 || (ua && ua != a))) break;
+                if (b && (__builtin_popcount(b) != 2 || (ub && ub != b))) break;
+                if (a) ua = a;
+                if (b) ub = b;
                 k1++;
             }
-            if (k1 == k) {
-                // not an orbit operator: one generic pass
-                Seg sg{k, k + 1, -1};
-                lsegs.push_back(sg);
-                k++;
-                continue;
-            }
-            if (ga.size() <= 1 && gb.size() <= 1) k1 = std::min(k1, k + kOrbMaxOps);  // k_orbv's limit
-            Seg sg{k, k1, -1};
-            if (ga.size() <= 1 && gb.size() <= 1) {
-                const uint32_t Ua = ga.empty() ? 0 : ga[0], Ub = gb.empty() ? 0 : gb[0];
-                sg.ua = Ua;
-                sg.ub = Ub;
-                sg.orbA = tileset_for(Ua, err, false);
-                sg.orbB = tileset_for(Ub, err, true);
-                if (sg.orbA < 0 || sg.orbB < 0) return false;
-                sg.orbits = (long long)tilesets[sg.orbA]->ntiles * tilesets[sg.orbB]->ntiles;
-                lsegs.push_back(sg);
-                k = k1;
-                continue;
-            }
-            sg.two = 1;
-            for (size_t i = 0; i < 2; i++) {
-                sg.grA[i] = i < ga.size() ? ga[i] : 0;
-                sg.grB[i] = i < gb.size() ? gb[i] : 0;
-            }
-            sg.gA = groupset_for(sg.grA[0], sg.grA[1], false, err);
-            sg.gB = groupset_for(sg.grB[0], sg.grB[1], true, err);
-            if (sg.gA < 0 || sg.gB < 0) return false;
-            sg.orbits = (long long)gsets[sg.gA]->ncls * gsets[sg.gB]->ncls;
-            // programs: 81 types x kOrb2MaxOps words, and (g, h) per operator
-            sg.prog2Off = (int)prog2.size();
-            const int ghOff = (int)opgh.size();
-            for (int kk = 0; kk < kOrb2MaxOps; kk++) {
-                uint32_t gh = 0;
-                if (k + kk < k1) {
-                    const uint32_t ua = bitsA(k + kk), ub = bitsB(k + kk);
-                    const int g = (ua && ua == sg.grA[1]) ? 1 : 0, h = (ub && ub == sg.grB[1]) ? 1 : 0;
-                    gh = (uint32_t)g | ((uint32_t)h << 1);
-                }
-                opgh.push_back(gh);
-            }
-            auto pat = [](uint32_t grp, int shp, int m) -> uint32_t {
-                if (shp == 1 || grp == 0) return 0u;
-                if (shp == 2) return grp;
-                const uint32_t lo = grp & (~grp + 1);
-                return m ? (grp ^ lo) : lo;
-            };
-            static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+            return k1;
+        };
+        auto pat = [](uint32_t grp, int shp, int m) -> uint32_t {
+            if (shp == 1 || grp == 0) return 0u;
+            if (shp == 2) return grp;
+            const uint32_t lo = grp & (~grp + 1);
+            return m ? (grp ^ lo) : lo;
+        };
+        static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+        // programs of a two-group segment [k, k2) split at k1: false if some
+        // operator's pair pattern differs between its sub-orbits
+        auto two_group_prog = [&](const Seg &sg, int k, int k1, int k2, std::vector<unsigned long long> &out) {
+            out.clear();
             for (int type = 0; type < 81; type++) {
                 const int sa1 = type % 3, sa2 = (type / 3) % 3, sb1 = (type / 9) % 3, sb2 = (type / 27) % 3;
                 auto valid = [&](int slot) {
@@ -432,13 +385,14 @@ struct EngineImpl {
                 };
                 for (int kk = 0; kk < kOrb2MaxOps; kk++) {
                     unsigned long long w = 0;
-                    if (k + kk < k1) {
+                    if (k + kk < k2) {
                         const DevOp &o = hops[k + kk];
-                        const int g = (int)(opgh[ghOff + kk] & 1), h = (int)((opgh[ghOff + kk] >> 1) & 1);
+                        const int g = (k + kk < k1) ? 0 : 1, h = g;
                         auto slotc = [&](int c, int i, int j) {
                             const int oa = c & 1, ob = c >> 1;
                             return ((i << g) | (oa << (1 - g))) + 4 * ((j << h) | (ob << (1 - h)));
                         };
+                        unsigned long long words[4] = {0, 0, 0, 0};
                         for (int c = 0; c < 4; c++)
                             for (int r = 0; r < 6; r++) {
                                 const int sp = slotc(c, PP[r] >> 1, PP[r] & 1), sq = slotc(c, QQ[r] >> 1, QQ[r] & 1);
@@ -446,39 +400,78 @@ struct EngineImpl {
                                 uint32_t b2 = 0;
                                 const int sgn = host_pair_sign(o, str(sp), b2);
                                 if (sgn == 0 || b2 != str(sq)) continue;
-                                w |= 1ull << (12 * c + r);
-                                if (sgn < 0) w |= 1ull << (12 * c + 6 + r);
+                                words[c] |= 1ull << r;
+                                if (sgn < 0) words[c] |= 1ull << (6 + r);
                             }
-                        // uniform form (bit 63, word in bits 0..11): one 12-bit word serves all four
-                        // sub-orbits when every combo either has exactly that word or has both
-                        // members of each of its pairs absent (absent amplitudes are zero in
-                        // registers and never stored, so rotating them changes nothing)
+                        // one word for all four sub-orbits: every sub-orbit has it, or
+                        // has both members of each of its pairs absent (absent amplitudes
+                        // are zero in registers and never stored)
                         unsigned long long m = 0;
-                        for (int c = 0; c < 4; c++) m |= (w >> (12 * c)) & 0xfffull;
-                        bool uni = m != 0;
-                        for (int c = 0; uni && c < 4; c++) {
-                            const unsigned long long wc = (w >> (12 * c)) & 0xfffull;
-                            if (wc == m) continue;
+                        for (int c = 0; c < 4; c++) m |= words[c];
+                        for (int c = 0; c < 4; c++) {
+                            if (words[c] == m) continue;
                             for (int r = 0; r < 6; r++) {
                                 if (!((m >> r) & 1)) continue;
                                 const int sp = slotc(c, PP[r] >> 1, PP[r] & 1), sq = slotc(c, QQ[r] >> 1, QQ[r] & 1);
-                                if (valid(sp) || valid(sq)) uni = false;
+                                if (valid(sp) || valid(sq)) return false;
                             }
                         }
-                        if (uni) w = (1ull << 63) | m;
+                        w = m;
                     }
-                    prog2.push_back(w);
+                    out.push_back(w);
                 }
             }
-            sg.progOff = ghOff;  // offset of the segment's (g, h) list in d_opgh
+            return true;
+        };
+        std::vector<unsigned long long> words;
+        int k = 0;
+        while (k < M) {
+            uint32_t ua = 0, ub = 0;
+            const int k1 = run(k, ua, ub);
+            if (k1 == k) {
+                // not an orbit operator: one generic pass
+                Seg sg{k, k + 1, -1};
+                lsegs.push_back(sg);
+                k++;
+                continue;
+            }
+            if (maxG >= 2 && k1 < M) {
+                uint32_t ua2 = 0, ub2 = 0;
+                const int k2 = run(k1, ua2, ub2);
+                if (k2 > k1 && k2 - k <= std::min(kOrb2MaxOps, maxOps) && !(ua & ua2) && !(ub & ub2)) {
+                    Seg sg{k, k2, -1};
+                    sg.two = 1;
+                    sg.grA[0] = ua;
+                    sg.grA[1] = ua2;
+                    sg.grB[0] = ub;
+                    sg.grB[1] = ub2;
+                    if (two_group_prog(sg, k, k1, k2, words)) {
+                        sg.gA = groupset_for(ua, ua2, false, err);
+                        sg.gB = groupset_for(ub, ub2, true, err);
+                        if (sg.gA < 0 || sg.gB < 0) return false;
+                        sg.orbits = (long long)gsets[sg.gA]->ncls * gsets[sg.gB]->ncls;
+                        sg.prog2Off = (int)prog2.size();
+                        prog2.insert(prog2.end(), words.begin(), words.end());
+                        sg.progOff = k1 - k;  // the split: operators on the first groups
+                        lsegs.push_back(sg);
+                        k = k2;
+                        continue;
+                    }
+                }
+            }
+            Seg sg{k, k1, -1};
+            sg.ua = ua;
+            sg.ub = ub;
+            sg.orbA = tileset_for(ua, err, false);
+            sg.orbB = tileset_for(ub, err, true);
+            if (sg.orbA < 0 || sg.orbB < 0) return false;
+            sg.orbits = (long long)tilesets[sg.orbA]->ntiles * tilesets[sg.orbB]->ntiles;
             lsegs.push_back(sg);
             k = k1;
         }
         if (prog2.empty()) prog2.push_back(0);
-        if (opgh.empty()) opgh.push_back(0);
-        if (!L.prog2.ensure(prog2.size(), err) || !L.opgh.ensure(opgh.size(), err)) return false;
+        if (!L.prog2.ensure(prog2.size(), err)) return false;
         HCHK(hipMemcpy(L.prog2.p, prog2.data(), prog2.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
-        HCHK(hipMemcpy(L.opgh.p, opgh.data(), opgh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         // one-group segments of the lane list get their own programs
         std::vector<uint32_t> save(lsegs.size());
         for (size_t i = 0; i < lsegs.size(); i++) save[i] = (uint32_t)lsegs[i].progOff;
@@ -505,7 +498,7 @@ struct EngineImpl {
         a.k1 = sg.k1;
         a.reverse = reverse ? 1 : 0;
         a.prog = L.prog2.p + sg.prog2Off;
-        a.opgh = L.opgh.p + sg.progOff;
+        a.split = sg.progOff;
         return a;
     }
 

@@ -520,38 +520,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Two-group orbit segment in lane layout (see kernels.hpp): wave per orbit
-// of up to 16 members, lane per vector; operators in a runtime loop with the
-// segment's sin/cos staged in LDS and brackets accumulated per wave in LDS.
-template <int G, int H>
-__device__ __forceinline__ void orb2_apply(unsigned long long w, bool neg, double C, double S, double (&x)[16],
-                                           double (&l)[16], bool br_on, double &br)
-{
-    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const unsigned cw = (unsigned)(w >> (12 * c)) & 0xfffu;
-        if (!(cw & 0x3fu)) continue;
-        const int oa = c & 1, ob = c >> 1;
-#pragma unroll
-        for (int r = 0; r < 6; r++) {
-            if (!((cw >> r) & 1u)) continue;
-            const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
-            const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
-            const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
-            const double s = ((((cw >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
-            const double xp = x[sp], xq = x[sq];
-            if (br_on) {
-                const double lp = l[sp], lq = l[sq];
-                br += s * (lp * xq - lq * xp);
-                l[sp] = s * lq * S + lp * C;
-                l[sq] = -s * lp * S + lq * C;
-            }
-            x[sp] = s * xq * S + xp * C;
-            x[sq] = -s * xp * S + xq * C;
-        }
-    }
-}
-
+// of up to 16 members, lane per vector, the two runs of operators unrolled.
 // Uniform operator word: the same 6 pair bits and phases in all four
 // sub-orbits; each active pair is rotated in every sub-orbit.
 template <int G, int H>
@@ -583,28 +552,44 @@ __device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double 
     }
 }
 
+// one operator of a run: its word, angle (per lane, from LDS) and bracket slot
+template <bool BR, int G, int H>
+__device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long long *pw, int kk, uint32_t spect,
+                                        const double2 (*csL)[kLanes], double *accW, int lane, double (&x)[16],
+                                        double (&l)[16])
+{
+    const unsigned w = (unsigned)pw[kk] & 0xfffu;
+    if (!w) return;
+    const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
+    const double2 t = csL[kk][lane];
+    double br = 0;
+    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br);
+    if (BR) accW[kk * kLanes + lane] += br;
+}
+
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
     k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
             const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
     constexpr int WPB = kBlock / kLanes;
-    __shared__ double2 csL[kOrb2MaxOps][kLanes];
-    __shared__ double accL[BR ? WPB * kOrb2MaxOps * kLanes : 1];
+    constexpr int K = kOrb2MaxOps;
+    __shared__ double2 csL[K][kLanes];
+    __shared__ double accL[BR ? WPB * K * kLanes : 1];
     const int lane = threadIdx.x & (kLanes - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     const int c = blockIdx.y;
-    const int nk = a.k1 - a.k0;
+    const int nk = a.k1 - a.k0, m = a.split;
     for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
         const int kk = e / kLanes, l = e - kk * kLanes;
-        const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
-        csL[kk][l] = cs[((long long)c * M + k) * kLanes + l];
+        csL[kk][l] = cs[((long long)c * M + a.k0 + kk) * kLanes + l];
     }
     if (BR)
-        for (int e = threadIdx.x; e < WPB * kOrb2MaxOps * kLanes; e += kBlock) accL[e] = 0.0;
+        for (int e = threadIdx.x; e < WPB * K * kLanes; e += kBlock) accL[e] = 0.0;
     __syncthreads();
-    v0 += c * cstride + lane;
-    if (BR) v1 += c * cstride + lane;
+    double *accW = BR ? accL + wv * K * kLanes : nullptr;
+    v0 += c * cstride;
+    if (BR) v1 += c * cstride;
     const int total = a.nAc * a.nBc;
     for (int t = blockIdx.x * WPB + wv; t < total; t += gridDim.x * WPB) {
         const int ia = t / a.nBc, ib = t - ia * a.nBc;
@@ -616,51 +601,51 @@ __global__ void __launch_bounds__(kBlock)
         if (na_ * nb_ < 2) continue;  // a lone determinant: nothing acts on it
         const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
         const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        const unsigned long long *pw = a.prog + type * K;
         // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
-        bool ok[16];
-        uint32_t idx[16];
         double x[16], l[16];
 #pragma unroll
         for (int s = 0; s < 16; s++) {
             const int ma = s & 3, mb = s >> 2;
-            ok[s] = rA[ma] >= 0 && rB[mb] >= 0;
-            idx[s] = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
-            x[s] = ok[s] ? v0[idx[s]] : 0.0;
-            l[s] = (BR && ok[s]) ? v1[idx[s]] : 0.0;
+            const bool ok = rA[ma] >= 0 && rB[mb] >= 0;
+            const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
+            x[s] = ok ? (v0 + off)[lane] : 0.0;
+            l[s] = (BR && ok) ? (v1 + off)[lane] : 0.0;
         }
-        const unsigned long long *prog = a.prog + type * kOrb2MaxOps;
-        for (int kk = 0; kk < nk; kk++) {
-            const int jk = a.reverse ? nk - 1 - kk : kk;
-            const unsigned long long w = prog[jk];
-            if (!w) continue;
-            const int k = a.k0 + jk;
-            const bool neg = __popc(spect & a.ops[k].mask) & 1;
-            const double2 t2 = csL[kk][lane];
-            double br = 0;
-            if (w >> 63) {
-                const unsigned wu = (unsigned)w & 0xfffu;
-                switch (a.opgh[jk]) {
-                case 0: orb2_apply_uniform<0, 0>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
-                case 1: orb2_apply_uniform<1, 0>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
-                case 2: orb2_apply_uniform<0, 1>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
-                default: orb2_apply_uniform<1, 1>(wu, neg, t2.x, t2.y, x, l, BR, br); break;
-                }
-            } else {
-                switch (a.opgh[jk]) {
-                case 0: orb2_apply<0, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-                case 1: orb2_apply<1, 0>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-                case 2: orb2_apply<0, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-                default: orb2_apply<1, 1>(w, neg, t2.x, t2.y, x, l, BR, br); break;
-                }
+        if (!a.reverse) {
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                if (kk >= m) break;
+                orb2_op<BR, 0, 0>(a, pw, kk, spect, csL, accW, lane, x, l);
             }
-            if (BR) accL[(wv * kOrb2MaxOps + kk) * kLanes + lane] += br;
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                if (kk < m || kk >= nk) continue;
+                orb2_op<BR, 1, 1>(a, pw, kk, spect, csL, accW, lane, x, l);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int kk = K - 1 - j;
+                if (kk < m || kk >= nk) continue;
+                orb2_op<BR, 1, 1>(a, pw, kk, spect, csL, accW, lane, x, l);
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int kk = K - 1 - j;
+                if (kk >= m) continue;
+                orb2_op<BR, 0, 0>(a, pw, kk, spect, csL, accW, lane, x, l);
+            }
         }
 #pragma unroll
-        for (int s = 0; s < 16; s++)
-            if (ok[s]) {
-                v0[idx[s]] = x[s];
-                if (BR) v1[idx[s]] = l[s];
+        for (int s = 0; s < 16; s++) {
+            const int ma = s & 3, mb = s >> 2;
+            if (rA[ma] >= 0 && rB[mb] >= 0) {
+                const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
+                (v0 + off)[lane] = x[s];
+                if (BR) (v1 + off)[lane] = l[s];
             }
+        }
     }
     if (BR) {
         __syncthreads();
@@ -668,9 +653,8 @@ __global__ void __launch_bounds__(kBlock)
             const int kk = e / kLanes, l = e - kk * kLanes;
             double s = 0;
 #pragma unroll
-            for (int w = 0; w < WPB; w++) s += accL[(w * kOrb2MaxOps + kk) * kLanes + l];
-            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
-            partial[(((long long)c * M + k) * slots + blockIdx.x) * kLanes + l] = s;
+            for (int q = 0; q < WPB; q++) s += accL[(q * K + kk) * kLanes + l];
+            partial[(((long long)c * M + a.k0 + kk) * slots + blockIdx.x) * kLanes + l] = s;
         }
     }
 }

@@ -177,15 +177,18 @@ __global__ void k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ 
 // electron count of each group, so an orbit is the product of per-group
 // classes: alpha members ma = ma1 + 2 ma2, beta members mb = mb1 + 2 mb2
 // (m = 0 the group's lower bit set, 1 its upper bit; groups holding 0 or 2
-// electrons have only m = 0), slot = ma + 4 mb, at most 16 members.
+// electrons have only m = 0), slot = ma + 4 mb, at most 16 members.  A
+// segment is two consecutive runs: operators on the first groups, then
+// operators on the second groups (two TUPS units on disjoint pairs), and
+// every operator's pair pattern is the same in its four sub-orbits.
 // Class descriptors: ranks[4] (-1 absent) and info = spectator bits |
 // shape1 << 24 | shape2 << 26 (shape 0 two members, 1 empty, 2 full).
-// Orbit type = shapeA1 + 3 shapeA2 + 9 (shapeB1 + 3 shapeB2).  An operator on
-// alpha group g and beta group h (0 if it touches none) acts within the 4
-// sub-orbits that fix the other groups' members (combo c = other alpha member
-// + 2 other beta member); prog[type * kOrb2MaxOps + kk] holds per combo c, at
-// bits 12c, the 6 member-pair bits and at 12c + 6 their phases (first member's
-// view, spectators cleared); opgh[kk] = g | h << 1.
+// Orbit type = shapeA1 + 3 shapeA2 + 9 (shapeB1 + 3 shapeB2).  An operator of
+// run g (g = 0 first, 1 second) acts within the 4 sub-orbits that fix the
+// other groups' members (combo c = other alpha member + 2 other beta member);
+// prog[type * kOrb2MaxOps + kk] holds its 6 member-pair bits and at bits 6..11
+// their phases (first member's view, spectators cleared), the same in all
+// four sub-orbits.
 constexpr int kOrb2MaxOps = 10;
 struct Orb2Args {
     int n, nb;
@@ -195,7 +198,7 @@ struct Orb2Args {
     const DevOp *ops;
     int k0, k1, reverse;
     const unsigned long long *prog;
-    const uint32_t *opgh;
+    int split;  // operators k0 .. k0+split-1 act on groups (0, 0), the rest on (1, 1)
 };
 template <bool BR>
 __global__ void k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
