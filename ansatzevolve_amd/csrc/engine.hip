@@ -96,6 +96,7 @@ struct Seg {
     int gA = -1, gB = -1;      // two-group segment: alpha / beta group sets
     uint32_t grA[2] = {0, 0}, grB[2] = {0, 0};
     int prog2Off = -1;
+    int grid = 1;  // lane layout: blocks of its launches
 };
 
 static long long binom_ll(int n, int k)
@@ -173,8 +174,10 @@ struct EngineImpl {
     DBuf<double> bl_psi, bl_lam, bl_partial;
     DBuf<double2> bl_cs, bl_csneg;
     std::vector<double2> h_csl, h_cslneg;
-    int lslots = 1;   // blocks of every lane-layout adjoint launch (= bracket partial rows per operator)
-    int lslotsF = 1;  // blocks of a lane-layout forward launch
+    int lslots = 1;   // most blocks of a lane-layout adjoint launch (= stride of the bracket partial rows)
+    int lslotsF = 1;  // most blocks of a lane-layout forward launch
+    DBuf<int> d_lrows;  // bracket partial rows written per operator (its launch's grid)
+    int numCU = 1;
 
     // tunable execution choices (Engine::set_option)
     int sweepKind = kSweepOrbit;
@@ -314,17 +317,38 @@ struct EngineImpl {
     bool build_lane_segments(std::string &err)
     {
         if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
-        // enough waves to keep the chip's HBM pipeline full: every resident
-        // wave holds two orbits' loads in flight (one 16-member orbit in the
-        // two-group kernel)
-        auto blocks = [&](const std::vector<Seg> &l, long long cap) {
-            long long work = 1;
-            for (const Seg &sg : l) work = std::max(work, (sg.orbA >= 0 || sg.two) ? sg.orbits : N);
-            const long long waves = (work + 1) / 2;
-            return (int)std::max<long long>(1, std::min<long long>(cap, (waves + 3) / 4));
+        // Every launch is one grid-stride pass of exactly as many blocks as
+        // the chip holds at once for that kernel (its occupancy x CUs): a
+        // grid a little larger than that leaves a second, sparsely occupied
+        // round of blocks that each still carry a full share of the orbits.
+        // Fewer blocks only when the segment has less work than that.
+        auto resident = [&](const void *fn) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+            return (long long)nb * numCU;
         };
-        lslots = blocks(laneA.segs, 1024);
-        lslotsF = blocks(laneF.segs, 1536);
+        auto assign = [&](std::vector<Seg> &l, bool adj) {
+            int mx = 1;
+            for (Seg &sg : l) {
+                const void *fn = sg.two ? (adj ? (const void *)k_orb2v<true> : (const void *)k_orb2v<false>)
+                                 : sg.orbA >= 0 ? (adj ? (const void *)k_orbv<true> : (const void *)k_orbv<false>)
+                                                : (adj ? (const void *)k_opv<true> : (const void *)k_opv<false>);
+                // waves with work: one orbit (or one 64-determinant tile) each
+                const long long work = (sg.orbA >= 0 || sg.two) ? sg.orbits : (N + kLanes - 1) / kLanes;
+                const long long want = (work + kBlock / kLanes - 1) / (kBlock / kLanes);
+                sg.grid = (int)std::max<long long>(1, std::min(resident(fn), want));
+                mx = std::max(mx, sg.grid);
+            }
+            return mx;
+        };
+        lslots = assign(laneA.segs, true);
+        lslotsF = assign(laneF.segs, false);
+        // partial rows per operator: the grid of the adjoint launch that owns it
+        std::vector<int> rows(M, 0);
+        for (const Seg &sg : laneA.segs)
+            for (int k = sg.k0; k < sg.k1; k++) rows[k] = sg.grid;
+        if (!d_lrows.ensure(std::max(M, 1), err)) return false;
+        HCHK(hipMemcpy(d_lrows.p, rows.data(), M * sizeof(int), hipMemcpyHostToDevice));
         return true;
     }
 
@@ -838,13 +862,13 @@ struct EngineImpl {
         for (const Seg &sg : laneF.segs) {
             if (sg.two) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orb2v<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orb2v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orb2args(laneF, sg, false), v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orbv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orbv<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orbargs(sg, false, laneF.lprog.p), v, nullptr, cstride, bl_cs.p, M, nullptr,
                                    lslotsF);
                 mark_end(mk);
@@ -852,7 +876,7 @@ struct EngineImpl {
             } else {
                 for (int k = sg.k0; k < sg.k1; k++) {
                     const int mk = mark_begin(1);
-                    hipLaunchKernelGGL((k_opv<false>), dim3(lslotsF, nch), dim3(kBlock), 0, stream, v, nullptr,
+                    hipLaunchKernelGGL((k_opv<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, v, nullptr,
                                        cstride, N, sec(), d_ops.p, k, bl_cs.p, M, nullptr, lslotsF);
                     mark_end(mk);
                     LCHK();
@@ -863,7 +887,7 @@ struct EngineImpl {
     }
 
     // Adjoint sweep of lane-layout (phi, lambda) chunks; every operator's
-    // partial rows are written by exactly one launch of lslots blocks.
+    // partial rows are written by exactly one launch (rows 0 .. its grid - 1).
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
@@ -871,13 +895,13 @@ struct EngineImpl {
             const Seg &sg = lsegs[si];
             if (sg.two) {
                 const int mk = mark_begin(2);
-                hipLaunchKernelGGL((k_orb2v<true>), dim3(lslots, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orb2v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orb2args(laneA, sg, true), phi, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(2);
-                hipLaunchKernelGGL((k_orbv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orbv<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orbargs(sg, true, laneA.lprog.p), phi, lam, cstride, bl_csneg.p, M, bl_partial.p,
                                    lslots);
                 mark_end(mk);
@@ -885,7 +909,7 @@ struct EngineImpl {
             } else {
                 for (int k = sg.k1 - 1; k >= sg.k0; k--) {
                     const int mk = mark_begin(2);
-                    hipLaunchKernelGGL((k_opv<true>), dim3(lslots, nch), dim3(kBlock), 0, stream, phi, lam, cstride,
+                    hipLaunchKernelGGL((k_opv<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, phi, lam, cstride,
                                        N, sec(), d_ops.p, k, bl_csneg.p, M, bl_partial.p, lslots);
                     mark_end(mk);
                     LCHK();
@@ -931,7 +955,7 @@ struct EngineImpl {
         const int G = 512;
         hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, bl_epart.p);
         LCHK();
-        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, stream, bl_epart.p, 1, G, nvec, out, 1.0);
+        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, stream, bl_epart.p, 1, G, nullptr, nvec, out, 1.0);
         LCHK();
         return true;
     }
@@ -1063,6 +1087,7 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
     HCHK(hipGetDeviceProperties(&prop, dev));
     e.ldsMax = prop.sharedMemPerBlock;
     e.ldsDevice = prop.sharedMemPerBlock;
+    e.numCU = std::max(1, prop.multiProcessorCount);
     if (const char *b = getenv("AE_LDS_BUDGET")) e.ldsMax = std::min<size_t>(e.ldsMax, (size_t)atol(b));
     HCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
     if (const char *g = getenv("AE_GRAPH")) e.useGraph = std::string(g) != "0";
@@ -1600,7 +1625,7 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
             if (with_gradient) {
                 if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
                 hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
-                                   e.M, e.lslots, nvec, e.b_out.p + nvec, 2.0);
+                                   e.M, e.lslots, e.d_lrows.p, nvec, e.b_out.p + nvec, 2.0);
                 LCHK();
             }
             return true;
@@ -1616,7 +1641,7 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
             LCHK();
             if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
             hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p, e.M,
-                               e.lslots, nvec, e.b_out.p + nvec, 2.0);
+                               e.lslots, e.d_lrows.p, nvec, e.b_out.p + nvec, 2.0);
             LCHK();
         }
         return true;

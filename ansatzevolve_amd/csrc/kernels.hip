@@ -591,61 +591,88 @@ __global__ void __launch_bounds__(kBlock)
     v0 += c * cstride;
     if (BR) v1 += c * cstride;
     const int total = a.nAc * a.nBc;
-    for (int t = blockIdx.x * WPB + wv; t < total; t += gridDim.x * WPB) {
-        const int ia = t / a.nBc, ib = t - ia * a.nBc;
-        const int4 ra = a.aRanks[ia], rb = a.bRanks[ib];
-        const uint32_t fa = a.aInfo[ia], fb = a.bInfo[ib];
-        const int rA[4] = {ra.x, ra.y, ra.z, ra.w}, rB[4] = {rb.x, rb.y, rb.z, rb.w};
-        const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
-        const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
-        if (na_ * nb_ < 2) continue;  // a lone determinant: nothing acts on it
-        const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
-        const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
-        const unsigned long long *pw = a.prog + type * K;
+    const int stride = gridDim.x * WPB;
+    // the forward sweep takes two orbits per iteration (both loaded before
+    // either is computed: twice the bytes in flight per wave); the adjoint
+    // sweep, with twice the registers per orbit, one
+    constexpr int NO = BR ? 1 : 2;
+    for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
+        int rA[NO][4], rB[NO][4], type[NO];
+        uint32_t spect[NO];
+        bool live[NO];
+#pragma unroll
+        for (int o = 0; o < NO; o++) {
+            const int t = t0 + o * stride;
+            live[o] = t < total;
+            const int tt = live[o] ? t : t0;
+            const int ia = tt / a.nBc, ib = tt - ia * a.nBc;
+            const int4 ra = a.aRanks[ia], rb = a.bRanks[ib];
+            const uint32_t fa = a.aInfo[ia], fb = a.bInfo[ib];
+            rA[o][0] = ra.x; rA[o][1] = ra.y; rA[o][2] = ra.z; rA[o][3] = ra.w;
+            rB[o][0] = rb.x; rB[o][1] = rb.y; rB[o][2] = rb.z; rB[o][3] = rb.w;
+            const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
+            const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
+            live[o] = live[o] && na_ * nb_ >= 2;  // a lone determinant: nothing acts on it
+            type[o] = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+            spect[o] = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        }
         // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
-        double x[16], l[16];
+        double x[NO][16], l[NO][16];
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const int ma = s & 3, mb = s >> 2;
-            const bool ok = rA[ma] >= 0 && rB[mb] >= 0;
-            const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
-            x[s] = ok ? (v0 + off)[lane] : 0.0;
-            l[s] = (BR && ok) ? (v1 + off)[lane] : 0.0;
+        for (int o = 0; o < NO; o++)
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int ma = s & 3, mb = s >> 2;
+                const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
+                const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                x[o][s] = ok ? (v0 + off)[lane] : 0.0;
+                l[o][s] = (BR && ok) ? (v1 + off)[lane] : 0.0;
+            }
+#pragma unroll
+        for (int o = 0; o < NO; o++) {
+            if (!live[o]) continue;
+            const unsigned long long *pw = a.prog + type[o] * K;
+            // the forward sweep (no brackets) runs forward, the adjoint sweep in
+            // reverse: a compile-time direction keeps the 16-member arrays in
+            // one register assignment (a runtime branch made the compiler copy
+            // them at the merge)
+            if (!BR) {
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    if (kk >= m) break;
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    if (kk < m || kk >= nk) continue;
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int kk = K - 1 - j;
+                    if (kk < m || kk >= nk) continue;
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int kk = K - 1 - j;
+                    if (kk >= m) continue;
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+            }
         }
-        if (!a.reverse) {
 #pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                if (kk >= m) break;
-                orb2_op<BR, 0, 0>(a, pw, kk, spect, csL, accW, lane, x, l);
-            }
+        for (int o = 0; o < NO; o++)
 #pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                if (kk < m || kk >= nk) continue;
-                orb2_op<BR, 1, 1>(a, pw, kk, spect, csL, accW, lane, x, l);
+            for (int s = 0; s < 16; s++) {
+                const int ma = s & 3, mb = s >> 2;
+                if (live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0) {
+                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                    (v0 + off)[lane] = x[o][s];
+                    if (BR) (v1 + off)[lane] = l[o][s];
+                }
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int kk = K - 1 - j;
-                if (kk < m || kk >= nk) continue;
-                orb2_op<BR, 1, 1>(a, pw, kk, spect, csL, accW, lane, x, l);
-            }
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int kk = K - 1 - j;
-                if (kk >= m) continue;
-                orb2_op<BR, 0, 0>(a, pw, kk, spect, csL, accW, lane, x, l);
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const int ma = s & 3, mb = s >> 2;
-            if (rA[ma] >= 0 && rB[mb] >= 0) {
-                const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
-                (v0 + off)[lane] = x[s];
-                if (BR) (v1 + off)[lane] = l[s];
-            }
-        }
     }
     if (BR) {
         __syncthreads();
@@ -754,8 +781,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_reduce_lanes(const double *__restrict__ part, int M, int slots, int nvec, double *__restrict__ out,
-                   double scale)
+    k_reduce_lanes(const double *__restrict__ part, int M, int slots, const int *__restrict__ rows, int nvec,
+                   double *__restrict__ out, double scale)
 {
     // block per (chunk, operator); thread = lane x slot group
     const int c = blockIdx.x / M, k = blockIdx.x - c * M;
@@ -764,7 +791,8 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ double red[G][kLanes];
     const double *p = part + ((long long)c * M + k) * slots * kLanes;
     double s = 0;
-    for (int j = g; j < slots; j += G) s += p[(long long)j * kLanes + lane];
+    const int nrow = rows ? rows[k] : slots;
+    for (int j = g; j < nrow; j += G) s += p[(long long)j * kLanes + lane];
     red[g][lane] = s;
     __syncthreads();
     if (g == 0) {

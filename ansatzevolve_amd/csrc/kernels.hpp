@@ -200,6 +200,8 @@ struct Orb2Args {
     const unsigned long long *prog;
     int split;  // operators k0 .. k0+split-1 act on groups (0, 0), the rest on (1, 1)
 };
+// BR = adjoint sweep: operators in reverse order with brackets (a.reverse is
+// ignored); !BR: forward order.
 template <bool BR>
 __global__ void k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
@@ -213,9 +215,10 @@ __global__ void k_to_lanes(const double *__restrict__ src, double *__restrict__ 
 __global__ void k_from_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nvec);
 // dst (lane layout, nchunks chunks) = src in every lane
 __global__ void k_bcast_lanes(const double *__restrict__ src, double *__restrict__ dst, long long N, int nchunks);
-// out[v*M + k] = scale * sum over blocks of the partial rows; grid chunks*M
-__global__ void k_reduce_lanes(const double *__restrict__ part, int M, int slots, int nvec, double *__restrict__ out,
-                               double scale);
+// out[v*M + k] = scale * sum of operator k's partial rows (rows[k] of them,
+// every row when rows is null; row stride slots); grid chunks*M
+__global__ void k_reduce_lanes(const double *__restrict__ part, int M, int slots, const int *__restrict__ rows,
+                               int nvec, double *__restrict__ out, double scale);
 
 // One operator on whole vectors (any sector); BR = also rotate v1 and
 // accumulate the adjoint bracket s*(v1_b v0_b' - v1_b' v0_b) before the
