@@ -158,6 +158,7 @@ struct EngineImpl {
     DBuf<double> d_Vaa, d_Vbb;
     DBuf<uint32_t> d_ts, d_ts1;
     int ntp = 0;
+    int ntp16 = 0;  // entries per beta of the lane-layout table (multiple of 16)
     rocblas_handle blas = nullptr;
     DBuf<double> bl_epart;
     DBuf<int32_t> d_col;
@@ -927,12 +928,12 @@ struct EngineImpl {
         SigvArgs s;
         s.m = mfargs();
         s.ts = d_ts.p;
-        s.ntp = ntp;
+        s.ntp = ntp16;
         s.x = x;
         s.y = y;
         s.cstride = N * kLanes;
-        const unsigned gx = (unsigned)((na + 7) / 8 * 64);
-        hipLaunchKernelGGL(k_sigv_sparse, dim3(gx, nch), dim3(kSigThreads), kSigLds, stream, s);
+        const unsigned gx = (unsigned)((na + 7) / 8 * 32);
+        hipLaunchKernelGGL(k_sigw_sparse, dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
         mark_end(mk);
         LCHK();
         const int mk3 = mark_begin(3);
@@ -1378,8 +1379,9 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             // tables of the split sigma: dense same-spin rows, beta entries
             const size_t na = e.na, nb = e.nb;
             e.ntp = (kd * (n - kd) + kd + 3) / 4 * 4;
+            e.ntp16 = kSigWNtp;
             if (!e.d_Vaa.ensure(na * na, err) || !e.d_Vbb.ensure(nb * nb, err) ||
-                !e.d_ts.ensure(nb * e.ntp, err) || !e.d_ts1.ensure(nb * e.ntp, err))
+                !e.d_ts.ensure(nb * e.ntp16, err) || !e.d_ts1.ensure(nb * e.ntp, err))
                 return false;
             HCHK(hipMemsetAsync(e.d_Vaa.p, 0, na * na * sizeof(double), e.stream));
             HCHK(hipMemsetAsync(e.d_Vbb.p, 0, nb * nb * sizeof(double), e.stream));
@@ -1389,18 +1391,18 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             hipLaunchKernelGGL(k_dense_spin, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
                                a, 0, e.d_Vbb.p);
             LCHK();
-            const bool lanesOk = n <= 12 && e.nb <= kSigMaxNb;
+            const bool lanesOk = n <= 12 && e.nb <= kSigMaxNb && kd * (n - kd) + kd <= kSigWNtp;
             if (lanesOk) {
                 hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                                   e.stream, a, e.d_ts.p, e.ntp, kSigVC * 8, kSigNN);
+                                   e.stream, a, e.d_ts.p, e.ntp16, kSigWVC * 8, kSigNN, 17);
                 LCHK();
             }
             hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
-                               a, e.d_ts1.p, e.ntp, 8, kSig1NN);
+                               a, e.d_ts1.p, e.ntp, 8, kSig1NN, 16);
             LCHK();
             HCHK(hipStreamSynchronize(e.stream));
-            HCHK(hipFuncSetAttribute((const void *)k_sigv_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)kSigLds));
+            HCHK(hipFuncSetAttribute((const void *)k_sigw_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kSigWLds));
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));
             if (!e.blas) {

@@ -1287,14 +1287,14 @@ __global__ void __launch_bounds__(kBlock) k_dense_spin(MfArgs a, int spin, doubl
     }
 }
 
-// Beta entry table of the lane-layout sigma: per beta string b, its singles
+// Beta entry table of the matrix-free sigma: per beta string b, its singles
 // (occupied r ascending x empty q ascending) and then its diagonal entries
-// (r = q over occupied r), padded to a multiple of 4 with an entry whose
-// coefficient slot is zero.  Entry = (byte offset of the source beta J in a
-// staged row of kSigVC doubles) | (byte offset of the coefficient
-// (e + parity * kSigNN), e = r * n + q, in a coefficient row) << 16.
+// (r = q over occupied r), padded to ntp entries with an entry whose
+// coefficient slot is zero.  Entry = (byte offset J * xunit of the source
+// beta J in a staged row) | (byte offset of the coefficient
+// (e + parity * nnoff), e = r * n + q, in a coefficient row) << cshift.
 __global__ void __launch_bounds__(kBlock)
-    k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff)
+    k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift)
 {
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= a.nb) return;
@@ -1307,13 +1307,13 @@ __global__ void __launch_bounds__(kBlock)
             const int r = __ffs(o) - 1, q = __ffs(w) - 1;
             const int J = a.rkB[s ^ (1u << r) ^ (1u << q)];
             const int e = r * n + q + (between_parity(s, r, q) ? nnoff : 0);
-            row[t++] = (uint32_t)(J * xunit) | ((uint32_t)(e * 8) << 16);
+            row[t++] = (uint32_t)(J * xunit) | ((uint32_t)(e * 8) << cshift);
         }
     for (uint32_t o = s; o; o &= o - 1) {
         const int r = __ffs(o) - 1;
-        row[t++] = (uint32_t)(b * xunit) | ((uint32_t)((r * n + r) * 8) << 16);
+        row[t++] = (uint32_t)(b * xunit) | ((uint32_t)((r * n + r) * 8) << cshift);
     }
-    for (; t < ntp; t++) row[t] = (uint32_t)(2 * nnoff * 8) << 16;
+    for (; t < ntp; t++) row[t] = (uint32_t)(2 * nnoff * 8) << cshift;
 }
 
 // Lane-layout sigma, sparse part: for output (alpha, beta), the alpha-beta
@@ -1428,6 +1428,144 @@ __global__ void __launch_bounds__(kSigThreads) k_sigv_sparse(SigvArgs a)
     for (int i = 0; i < kSigBI; i++) {
         const int beta = slot + kSigSlots * i;
         if (beta < nb) y[((long long)ar * nb + beta) * kLanes + v] = acc[i];
+    }
+}
+
+// Lane-layout sigma, sparse part, 16 vectors per workgroup: the same terms
+// as k_sigv_sparse with one staged source row (nb x 16 doubles) and the
+// per-entry work cut to one LDS read.  A wave holds four target betas, one
+// per 16-lane DPP row (lane = vector).  Each lane loads the table entry of
+// its row's target with index (lane mod 16) and that entry's coefficient;
+// the 16 entries of a chunk are then broadcast across the row with
+// row_newbcast (the source offset folds into the address add), so every
+// entry costs one conflict-free 128-byte row read of the staged source and
+// one FMA.  The next source row is fetched into registers while the current
+// one is summed.
+typedef __attribute__((address_space(3))) double lds_f64;
+__device__ __forceinline__ double lds_at(uint32_t byteaddr) { return *(const lds_f64 *)(size_t)byteaddr; }
+template <int J>
+__device__ __forceinline__ double row_bcast(double v)
+{
+    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + J, 0xf, 0xf, true);
+}
+template <int J>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v)
+{
+    return __builtin_amdgcn_update_dpp(0u, v, 0x150 + J, 0xf, 0xf, true);
+}
+
+// 16 entries of one chunk: four partial sums, so four row reads can be in
+// flight per wave
+template <int J>
+__device__ __forceinline__ void sigw_entries(double cl, uint32_t xb, uint32_t lb, double (&s)[4])
+{
+    // four row reads issued before the four FMAs that use them
+    const double x0 = lds_at(row_bcast<J>(xb) + lb), x1 = lds_at(row_bcast<J + 1>(xb) + lb);
+    const double x2 = lds_at(row_bcast<J + 2>(xb) + lb), x3 = lds_at(row_bcast<J + 3>(xb) + lb);
+    s[0] = __builtin_fma(row_bcast<J>(cl), x0, s[0]);
+    s[1] = __builtin_fma(row_bcast<J + 1>(cl), x1, s[1]);
+    s[2] = __builtin_fma(row_bcast<J + 2>(cl), x2, s[2]);
+    s[3] = __builtin_fma(row_bcast<J + 3>(cl), x3, s[3]);
+    if constexpr (J + 4 < 16) sigw_entries<J + 4>(cl, xb, lb, s);
+}
+
+__global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
+{
+    extern __shared__ double sm[];
+    const MfArgs &m = a.m;
+    const int n = m.n, nb = m.nb, nn = n * n;
+    // XCD-aware order: the 4 sub-chunks of one alpha row run on one XCD
+    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
+    const int ar = xcd + 8 * (k >> 2), sub = k & 3;
+    if (ar >= m.na) return;
+    const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigWVC;
+    const double *__restrict__ x = a.x + coff;
+    double *__restrict__ y = a.y + coff;
+    double *gs = sm, *xs = sm + kSigGS;
+    const int tid = threadIdx.x, lane = tid & (kLanes - 1), w = tid / kLanes;
+    const int v = lane & 15, trow = w * 4 + (lane >> 4);
+    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const int ka = __popc(alpha), ne = n - ka;
+    const int R = 1 + ka * ne;
+    const int nx = nb * kSigWVC;
+    const uint32_t gsb = (uint32_t)(size_t)(lds_f64 *)gs;
+    const uint32_t lb = (uint32_t)(size_t)(lds_f64 *)xs + v * 8;
+    constexpr uint32_t kPad = (uint32_t)(2 * kSigNN * 8) << 17;  // zero coefficient, source beta 0
+    const uint32_t toff = (uint32_t)(trow * kSigWNtp + v);
+    double acc[kSigWNI];
+#pragma unroll
+    for (int i = 0; i < kSigWNI; i++) acc[i] = 0;
+    for (int r = 0; r < R; r++) {
+        // source row: r = 0 the identity row (J = ar), else the alpha single
+        // tA = r - 1 (occupied rA, empty qA by rank) with its coefficients
+        int J = ar, rA = 0, qA = 0;
+        if (r > 0) {
+            const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
+            uint32_t oo = alpha, ww = ~alpha & full;
+            for (int i = 0; i < ir; i++) oo &= oo - 1;
+            for (int i = 0; i < iq; i++) ww &= ww - 1;
+            rA = __ffs(oo) - 1;
+            qA = __ffs(ww) - 1;
+            J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
+        }
+        __syncthreads();  // every read of the previous row is done
+        const double *src = x + (long long)J * nb * kLanes;
+        for (int i = tid; i < nx; i += kSigWThreads) xs[i] = src[(long long)(i >> 4) * kLanes + (i & 15)];
+        if (r == 0) {
+            // identity row: J_alpha[e] = sum over occupied alpha o of g[oo][e] (r != q only)
+            for (int e = tid; e < nn; e += kSigWThreads) {
+                double vj = 0;
+                if (e / n != e % n)
+                    for (uint32_t o = alpha; o; o &= o - 1) {
+                        const int xo = __ffs(o) - 1;
+                        vj += thr13(m.g[(size_t)(xo * n + xo) * nn + e]);
+                    }
+                gs[e] = vj;
+                gs[kSigNN + e] = -vj;
+            }
+        } else {
+            const bool neg = between_parity(alpha, rA, qA);
+            const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+            for (int e = tid; e < nn; e += kSigWThreads) {
+                const double cf = neg ? -thr13(gb[e]) : thr13(gb[e]);
+                gs[e] = cf;
+                gs[kSigNN + e] = -cf;
+            }
+        }
+        if (tid < kSigGS - 2 * kSigNN) gs[2 * kSigNN + tid] = 0.0;
+        __syncthreads();
+        // table entries of the next pass are loaded while this pass sums
+        // (32-bit element offsets from the table base); the empty asm keeps
+        // the compiler from hoisting every pass's entries out of the row
+        // loop, which would hold 45 more registers
+        uint32_t tof = toff;
+        asm volatile("" : "+v"(tof));
+        uint32_t ent[kSigWH], entn[kSigWH];
+#pragma unroll
+        for (int h = 0; h < kSigWH; h++) ent[h] = trow < nb ? a.ts[tof + 16 * h] : kPad;
+#pragma unroll
+        for (int i = 0; i < kSigWNI; i++) {
+            if (64 * i >= nb) break;
+            const int tgt = trow + 64 * i, tn = tgt + 64;
+#pragma unroll
+            for (int h = 0; h < kSigWH; h++)
+                entn[h] = tn < nb ? a.ts[tof + (uint32_t)(64 * kSigWNtp * (i + 1) + 16 * h)] : kPad;
+            double s[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int h = 0; h < kSigWH; h++) {
+                const double cl = lds_at(gsb + (ent[h] >> 17));
+                sigw_entries<0>(cl, ent[h] & 0x1ffffu, lb, s);
+            }
+            acc[i] += (s[0] + s[1]) + (s[2] + s[3]);
+            if (r == 0 && tgt < nb) acc[i] += m.diag[(long long)ar * nb + tgt] * xs[tgt * kSigWVC + v];
+#pragma unroll
+            for (int h = 0; h < kSigWH; h++) ent[h] = entn[h];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSigWNI; i++) {
+        const int tgt = trow + 64 * i;
+        if (tgt < nb) y[((long long)ar * nb + tgt) * kLanes + v] = acc[i];
     }
 }
 
