@@ -99,6 +99,22 @@ struct Seg {
     int grid = 1;  // lane layout: blocks of its launches
 };
 
+// the lane-layout sparse sigma kernel for ng groups of 4 single entries
+static void (*sigw_kernel(int ng))(SigvArgs)
+{
+    switch (ng) {
+    case 1: return k_sigw_sparse<1>;
+    case 2: return k_sigw_sparse<2>;
+    case 3: return k_sigw_sparse<3>;
+    case 4: return k_sigw_sparse<4>;
+    case 5: return k_sigw_sparse<5>;
+    case 6: return k_sigw_sparse<6>;
+    case 7: return k_sigw_sparse<7>;
+    case 8: return k_sigw_sparse<8>;
+    default: return k_sigw_sparse<9>;
+    }
+}
+
 static long long binom_ll(int n, int k)
 {
     if (k < 0 || k > n) return 0;
@@ -158,7 +174,7 @@ struct EngineImpl {
     DBuf<double> d_Vaa, d_Vbb;
     DBuf<uint32_t> d_ts, d_ts1;
     int ntp = 0;
-    int ntp16 = 0;  // entries per beta of the lane-layout table (multiple of 16)
+    int ntp16 = 0;  // lane-layout table: singles per beta rounded up to 4 (rows of kSigWNtp)
     rocblas_handle blas = nullptr;
     DBuf<double> bl_epart;
     DBuf<int32_t> d_col;
@@ -933,7 +949,7 @@ struct EngineImpl {
         s.y = y;
         s.cstride = N * kLanes;
         const unsigned gx = (unsigned)((na + 7) / 8 * 32);
-        hipLaunchKernelGGL(k_sigw_sparse, dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
+        hipLaunchKernelGGL(sigw_kernel(ntp16 / 4), dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
         mark_end(mk);
         LCHK();
         const int mk3 = mark_begin(3);
@@ -1379,9 +1395,9 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             // tables of the split sigma: dense same-spin rows, beta entries
             const size_t na = e.na, nb = e.nb;
             e.ntp = (kd * (n - kd) + kd + 3) / 4 * 4;
-            e.ntp16 = kSigWNtp;
+            e.ntp16 = (kd * (n - kd) + 3) / 4 * 4;
             if (!e.d_Vaa.ensure(na * na, err) || !e.d_Vbb.ensure(nb * nb, err) ||
-                !e.d_ts.ensure(nb * e.ntp16, err) || !e.d_ts1.ensure(nb * e.ntp, err))
+                !e.d_ts.ensure(nb * kSigWNtp, err) || !e.d_ts1.ensure(nb * e.ntp, err))
                 return false;
             HCHK(hipMemsetAsync(e.d_Vaa.p, 0, na * na * sizeof(double), e.stream));
             HCHK(hipMemsetAsync(e.d_Vbb.p, 0, nb * nb * sizeof(double), e.stream));
@@ -1391,18 +1407,19 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             hipLaunchKernelGGL(k_dense_spin, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
                                a, 0, e.d_Vbb.p);
             LCHK();
-            const bool lanesOk = n <= 12 && e.nb <= kSigMaxNb && kd * (n - kd) + kd <= kSigWNtp;
+            const bool lanesOk = n <= 12 && e.nb <= kSigMaxNb && e.ntp16 >= 4 && e.ntp16 <= 36;
             if (lanesOk) {
                 hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                                   e.stream, a, e.d_ts.p, e.ntp16, kSigWVC * 8, kSigNN, 17);
+                                   e.stream, a, e.d_ts.p, kSigWNtp, kSigWVC * 8, kSigNN, 17, 1);
                 LCHK();
             }
             hipLaunchKernelGGL(k_beta_table, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, e.stream,
-                               a, e.d_ts1.p, e.ntp, 8, kSig1NN, 16);
+                               a, e.d_ts1.p, e.ntp, 8, kSig1NN, 16, 0);
             LCHK();
             HCHK(hipStreamSynchronize(e.stream));
-            HCHK(hipFuncSetAttribute((const void *)k_sigw_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)kSigWLds));
+            if (lanesOk)
+                HCHK(hipFuncSetAttribute((const void *)sigw_kernel(e.ntp16 / 4),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSigWLds));
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));
             if (!e.blas) {

@@ -1290,11 +1290,12 @@ __global__ void __launch_bounds__(kBlock) k_dense_spin(MfArgs a, int spin, doubl
 // Beta entry table of the matrix-free sigma: per beta string b, its singles
 // (occupied r ascending x empty q ascending) and then its diagonal entries
 // (r = q over occupied r), padded to ntp entries with an entry whose
-// coefficient slot is zero.  Entry = (byte offset J * xunit of the source
+// coefficient slot is zero.  With sepdiag the diagonal entries are left out
+// and the last slot holds the string b.  Entry = (byte offset J * xunit of the source
 // beta J in a staged row) | (byte offset of the coefficient
 // (e + parity * nnoff), e = r * n + q, in a coefficient row) << cshift.
 __global__ void __launch_bounds__(kBlock)
-    k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift)
+    k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift, int sepdiag)
 {
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= a.nb) return;
@@ -1309,11 +1310,13 @@ __global__ void __launch_bounds__(kBlock)
             const int e = r * n + q + (between_parity(s, r, q) ? nnoff : 0);
             row[t++] = (uint32_t)(J * xunit) | ((uint32_t)(e * 8) << cshift);
         }
-    for (uint32_t o = s; o; o &= o - 1) {
-        const int r = __ffs(o) - 1;
-        row[t++] = (uint32_t)(b * xunit) | ((uint32_t)((r * n + r) * 8) << cshift);
-    }
+    if (!sepdiag)
+        for (uint32_t o = s; o; o &= o - 1) {
+            const int r = __ffs(o) - 1;
+            row[t++] = (uint32_t)(b * xunit) | ((uint32_t)((r * n + r) * 8) << cshift);
+        }
     for (; t < ntp; t++) row[t] = (uint32_t)(2 * nnoff * 8) << cshift;
+    if (sepdiag) row[ntp - 1] = s;  // the string itself, for the diagonal sums
 }
 
 // Lane-layout sigma, sparse part: for output (alpha, beta), the alpha-beta
@@ -1454,10 +1457,10 @@ __device__ __forceinline__ uint32_t row_bcast(uint32_t v)
     return __builtin_amdgcn_update_dpp(0u, v, 0x150 + J, 0xf, 0xf, true);
 }
 
-// 16 entries of one chunk: four partial sums, so four row reads can be in
-// flight per wave
+// four entries of a 16-entry chunk (J, J+1, J+2, J+3): four partial sums,
+// so four row reads can be in flight per wave
 template <int J>
-__device__ __forceinline__ void sigw_entries(double cl, uint32_t xb, uint32_t lb, double (&s)[4])
+__device__ __forceinline__ void sigw_group(double cl, uint32_t xb, uint32_t lb, double (&s)[4])
 {
     // four row reads issued before the four FMAs that use them
     const double x0 = lds_at(row_bcast<J>(xb) + lb), x1 = lds_at(row_bcast<J + 1>(xb) + lb);
@@ -1466,9 +1469,21 @@ __device__ __forceinline__ void sigw_entries(double cl, uint32_t xb, uint32_t lb
     s[1] = __builtin_fma(row_bcast<J + 1>(cl), x1, s[1]);
     s[2] = __builtin_fma(row_bcast<J + 2>(cl), x2, s[2]);
     s[3] = __builtin_fma(row_bcast<J + 3>(cl), x3, s[3]);
-    if constexpr (J + 4 < 16) sigw_entries<J + 4>(cl, xb, lb, s);
 }
 
+// one chunk of 16 table entries, the first min(G, 4) groups of 4
+template <int G>
+__device__ __forceinline__ void sigw_chunk(uint32_t gsb, uint32_t ent, uint32_t lb, double (&s)[4])
+{
+    const double cl = lds_at(gsb + (ent >> 17));
+    const uint32_t xb = ent & 0x1ffffu;
+    sigw_group<0>(cl, xb, lb, s);
+    if constexpr (G > 1) sigw_group<4>(cl, xb, lb, s);
+    if constexpr (G > 2) sigw_group<8>(cl, xb, lb, s);
+    if constexpr (G > 3) sigw_group<12>(cl, xb, lb, s);
+}
+
+template <int NG>
 __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
 {
     extern __shared__ double sm[];
@@ -1481,7 +1496,7 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
     const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigWVC;
     const double *__restrict__ x = a.x + coff;
     double *__restrict__ y = a.y + coff;
-    double *gs = sm, *xs = sm + kSigGS;
+    double *gs = sm, *dt = sm + kSigGS, *xs = dt + 128;
     const int tid = threadIdx.x, lane = tid & (kLanes - 1), w = tid / kLanes;
     const int v = lane & 15, trow = w * 4 + (lane >> 4);
     const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
@@ -1533,6 +1548,22 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
             }
         }
         if (tid < kSigGS - 2 * kSigNN) gs[2 * kSigNN + tid] = 0.0;
+        // beta-diagonal sums over the occupied orbitals of a string, by its
+        // low and high 6 bits: dt[t] (bits 0-5), dt[64 + t] (bits 6-11); zero
+        // for the identity row, whose diagonal is the diag array
+        if (tid < 128) {
+            const int hi = tid >> 6, t = tid & 63;
+            double d = 0;
+            if (r > 0) {
+                const bool neg = between_parity(alpha, rA, qA);
+                const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+                for (int b = 0; b < 6; b++) {
+                    const int q = b + 6 * hi;
+                    if (q < n && ((t >> b) & 1)) d += neg ? -thr13(gb[q * n + q]) : thr13(gb[q * n + q]);
+                }
+            }
+            dt[tid] = d;
+        }
         __syncthreads();
         // table entries of the next pass are loaded while this pass sums
         // (32-bit element offsets from the table base); the empty asm keeps
@@ -1551,12 +1582,15 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
             for (int h = 0; h < kSigWH; h++)
                 entn[h] = tn < nb ? a.ts[tof + (uint32_t)(64 * kSigWNtp * (i + 1) + 16 * h)] : kPad;
             double s[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int h = 0; h < kSigWH; h++) {
-                const double cl = lds_at(gsb + (ent[h] >> 17));
-                sigw_entries<0>(cl, ent[h] & 0x1ffffu, lb, s);
-            }
+            // the singles: NG groups of 4 entries in chunks of 16
+            sigw_chunk<NG>(gsb, ent[0], lb, s);
+            if constexpr (NG > 4) sigw_chunk<NG - 4>(gsb, ent[1], lb, s);
+            if constexpr (NG > 8) sigw_chunk<NG - 8>(gsb, ent[2], lb, s);
             acc[i] += (s[0] + s[1]) + (s[2] + s[3]);
+            // beta diagonal: the sums over the target string's occupied
+            // orbitals (the string is the last table slot)
+            const uint32_t bs = row_bcast<15>(ent[kSigWH - 1]);
+            if (tgt < nb) acc[i] += (dt[bs & 63] + dt[64 + ((bs >> 6) & 63)]) * xs[tgt * kSigWVC + v];
             if (r == 0 && tgt < nb) acc[i] += m.diag[(long long)ar * nb + tgt] * xs[tgt * kSigWVC + v];
 #pragma unroll
             for (int h = 0; h < kSigWH; h++) ent[h] = entn[h];
@@ -1568,6 +1602,16 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
         if (tgt < nb) y[((long long)ar * nb + tgt) * kLanes + v] = acc[i];
     }
 }
+
+template __global__ void k_sigw_sparse<1>(SigvArgs);
+template __global__ void k_sigw_sparse<2>(SigvArgs);
+template __global__ void k_sigw_sparse<3>(SigvArgs);
+template __global__ void k_sigw_sparse<4>(SigvArgs);
+template __global__ void k_sigw_sparse<5>(SigvArgs);
+template __global__ void k_sigw_sparse<6>(SigvArgs);
+template __global__ void k_sigw_sparse<7>(SigvArgs);
+template __global__ void k_sigw_sparse<8>(SigvArgs);
+template __global__ void k_sigw_sparse<9>(SigvArgs);
 
 // Matrix-free sigma, sparse part, vector-major vectors (single vectors and
 // small batches, n <= 14): the same split and entry table as k_sigv_sparse

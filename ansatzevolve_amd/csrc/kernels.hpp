@@ -312,17 +312,21 @@ struct SigvArgs {
 __global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
 // entries: source byte offset J * xunit | coefficient byte offset (slot
 // e + parity * nnoff) << cshift
-__global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift);
+__global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift,
+                             int sepdiag);
 __global__ void k_sigv_sparse(SigvArgs a);  // grid (ceil(na / 8) * 64, chunks), kSigThreads, kSigLds
 // 16 vectors per workgroup, one staged source row; table with xunit 128,
-// cshift 17 and kSigWNtp entries per beta
+// cshift 17, kSigWNtp entries per beta and the diagonal separate (SigvArgs
+// ntp = singles per beta rounded up to 4)
 constexpr int kSigWVC = 16;
 constexpr int kSigWThreads = 1024;
 constexpr int kSigWNI = (kSigMaxNb + 63) / 64;  // target passes (64 betas per pass)
 constexpr int kSigWPre = (kSigMaxNb * kSigWVC + kSigWThreads - 1) / kSigWThreads;
 constexpr int kSigWH = 3;                       // entry chunks of 16 (n <= 12: at most 42 entries)
 constexpr int kSigWNtp = 16 * kSigWH;
-constexpr size_t kSigWLds = (size_t)(kSigGS + kSigMaxNb * kSigWVC) * sizeof(double);
+constexpr size_t kSigWLds = (size_t)(kSigGS + 128 + kSigMaxNb * kSigWVC) * sizeof(double);
+// NG = groups of 4 single entries per beta (1..9; a.ntp = 4 NG)
+template <int NG>
 __global__ void k_sigw_sparse(SigvArgs a);  // grid (ceil(na / 8) * 32, chunks), kSigWThreads, kSigWLds
 __global__ void k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N,
                             double *__restrict__ part);  // grid (G, chunks)
