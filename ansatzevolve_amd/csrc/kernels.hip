@@ -1457,6 +1457,17 @@ __device__ __forceinline__ uint32_t row_bcast(uint32_t v)
     return __builtin_amdgcn_update_dpp(0u, v, 0x150 + J, 0xf, 0xf, true);
 }
 
+// acc += (c of lane J of this 16-lane row) * x: the broadcast rides on the
+// FMA (DPP row_newbcast on a 64-bit VALU op); c comes from an LDS read, so no
+// VALU result feeds the DPP operand
+template <int J>
+__device__ __forceinline__ void fma_row_bcast(double &acc, double c, double x)
+{
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+v"(acc)
+        : "v"(c), "v"(x), "i"(J));
+}
+
 // four entries of a 16-entry chunk (J, J+1, J+2, J+3): four partial sums,
 // so four row reads can be in flight per wave
 template <int J>
@@ -1465,10 +1476,11 @@ __device__ __forceinline__ void sigw_group(double cl, uint32_t xb, uint32_t lb, 
     // four row reads issued before the four FMAs that use them
     const double x0 = lds_at(row_bcast<J>(xb) + lb), x1 = lds_at(row_bcast<J + 1>(xb) + lb);
     const double x2 = lds_at(row_bcast<J + 2>(xb) + lb), x3 = lds_at(row_bcast<J + 3>(xb) + lb);
-    s[0] = __builtin_fma(row_bcast<J>(cl), x0, s[0]);
-    s[1] = __builtin_fma(row_bcast<J + 1>(cl), x1, s[1]);
-    s[2] = __builtin_fma(row_bcast<J + 2>(cl), x2, s[2]);
-    s[3] = __builtin_fma(row_bcast<J + 3>(cl), x3, s[3]);
+    __builtin_amdgcn_sched_barrier(0);  // keep the four reads ahead of the FMAs
+    fma_row_bcast<J>(s[0], cl, x0);
+    fma_row_bcast<J + 1>(s[1], cl, x1);
+    fma_row_bcast<J + 2>(s[2], cl, x2);
+    fma_row_bcast<J + 3>(s[3], cl, x3);
 }
 
 // one chunk of 16 table entries, the first min(G, 4) groups of 4
