@@ -11,7 +11,10 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libansatzsynth.so")
 ARCH = os.environ.get("AE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "engine.hip", "mlp.hip", "multi.hip"]
+HIP_SOURCES = ["kernels.hip", "lanes.hip", "engine.hip", "mlp.hip", "multi.hip"]
+# per-file device flags: the lane sweeps keep their orbit arrays as separate
+# registers (no promotion of private arrays to one vector register tuple)
+HIP_FILE_FLAGS = {"lanes.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"]}
 CXX_SOURCES = ["manager.cpp", "newton.cpp", "autotune.cpp", "abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -36,7 +39,7 @@ def build(verbose=False):
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s + ".o")
         objs.append(o)
-        jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([HIPCC] + HIP_FLAGS + HIP_FILE_FLAGS.get(s, []) + ["-c", os.path.join(CSRC, s), "-o", o])
     for s in CXX_SOURCES:
         o = os.path.join(BUILD, s + ".o")
         objs.append(o)

@@ -1,0 +1,422 @@
+// Lane-layout sweep kernels (batches of 64 vectors, lane = vector): orbit
+// segments (k_orbv, k_orb2v) and one-operator passes (k_opv).  See
+// kernels.hpp.
+//
+// Built without the promotion of private arrays to vector registers
+// (-mllvm -disable-promote-alloca-to-vector): the 16-member orbit arrays
+// would otherwise become one 32-register tuple whose every conditional
+// update copies the whole tuple (thousands of register moves and 150+
+// VGPRs in the adjoint two-group kernel instead of 95).
+#include "kernels.hpp"
+
+namespace ae {
+
+// ------------------------------------------------ lane layout (batches)
+// Per-wave bracket sums of a segment's operators -> one partial row of 64
+// lanes per (operator, block).
+template <int NK>
+__device__ __forceinline__ void lanes_block_partials(const double (&acc)[NK], int nk, int k0, int k1, int reverse,
+                                                     double *red, double *__restrict__ partial, int c, int M,
+                                                     int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    const int lane = threadIdx.x & (kLanes - 1), wv = threadIdx.x / kLanes;
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++)
+        if (kk < nk) red[(kk * WPB + wv) * kLanes + lane] = acc[kk];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < WPB; w++) s += red[(kk * WPB + w) * kLanes + l];
+        const int k = reverse ? k1 - 1 - kk : k0 + kk;
+        partial[(((long long)c * M + k) * slots + blockIdx.x) * kLanes + l] = s;
+    }
+}
+
+// One orbit of a lane-layout wave: member indices, presence, type and the
+// spectator bits are wave-uniform; amplitudes are per lane (per vector).
+struct LaneOrbit {
+    long long idx[4];
+    bool ok[4];
+    bool live;
+    int type;
+    uint32_t spect;
+};
+
+__device__ __forceinline__ void lane_orbit_open(const OrbArgs &a, int t, LaneOrbit &o)
+{
+    const int ia = t / a.nBc, ib = t - ia * a.nBc;
+    const int4 da = a.aDesc[ia], db = a.bDesc[ib];
+    const int Ra = da.y >= 0 ? 2 : 1, Rb = db.y >= 0 ? 2 : 1;
+    o.live = Ra * Rb > 1;  // a lone determinant: no operator of the segment acts on it
+    const int ta = Ra == 2 ? 0 : (((uint32_t)da.z & a.uaLocal) ? 2 : 1);
+    const int tb = Rb == 2 ? 0 : (((uint32_t)db.z & a.ubLocal) ? 2 : 1);
+    o.type = ta * 3 + tb;
+    o.spect = (((uint32_t)da.z << a.n) | (uint32_t)db.z) & a.spect;
+    const int ra[2] = {da.x, da.y >= 0 ? da.y : da.x};
+    const int rb[2] = {db.x, db.y >= 0 ? db.y : db.x};
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int i = p >> 1, j = p & 1;
+        o.ok[p] = o.live && (i < Ra) && (j < Rb);
+        o.idx[p] = ((long long)ra[i] * a.nb + rb[j]) * kLanes;
+    }
+}
+
+template <bool BR>
+__device__ __forceinline__ void lane_orbit_apply(const OrbArgs &a, int nk, const LaneOrbit &o, double (&x)[4],
+                                                 double (&l)[4], const double (&C)[kOrbMaxOps],
+                                                 const double (&S)[kOrbMaxOps], double (&acc)[kOrbMaxOps])
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+    const uint32_t *prog = a.prog + o.type * kOrbMaxOps;
+#pragma unroll
+    for (int kk = 0; kk < kOrbMaxOps; kk++) {
+        if (kk >= nk) break;
+        const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+        const uint32_t w = prog[a.reverse ? nk - 1 - kk : kk];
+        if (!(w & 0x3f)) continue;
+        const bool neg = __popc(o.spect & a.ops[k].mask) & 1;
+        double br = 0;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            if (!((w >> r) & 1)) continue;
+            const int p = PP[r], q = QQ[r];
+            const double s = (((w >> (8 + r)) & 1) != neg) ? -1.0 : 1.0;
+            const double xp = x[p], xq = x[q];
+            if (BR) {
+                const double lp = l[p], lq = l[q];
+                br += s * (lp * xq - lq * xp);
+                l[p] = s * lq * S[kk] + lp * C[kk];
+                l[q] = -s * lp * S[kk] + lq * C[kk];
+            }
+            x[p] = s * xq * S[kk] + xp * C[kk];
+            x[q] = -s * xp * S[kk] + xq * C[kk];
+        }
+        acc[kk] += br;
+    }
+}
+
+// Orbit segment in lane layout.  Each wave takes two orbits per iteration
+// and issues the loads of both before computing either, so that twice the
+// bytes are in flight per wave (the kernel is bound by HBM latency x bytes
+// in flight, not by arithmetic).
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+           const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    __shared__ double red[BR ? kOrbMaxOps * WPB * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    v0 += c * cstride + lane;
+    if (BR) v1 += c * cstride + lane;
+    cs += (long long)c * M * kLanes + lane;
+    const int nk = a.k1 - a.k0;
+    double C[kOrbMaxOps], S[kOrbMaxOps], acc[kOrbMaxOps];
+#pragma unroll
+    for (int kk = 0; kk < kOrbMaxOps; kk++) {
+        acc[kk] = 0;
+        C[kk] = 1;
+        S[kk] = 0;
+        if (kk < nk) {
+            const int k = a.reverse ? a.k1 - 1 - kk : a.k0 + kk;
+            const double2 t = cs[(long long)k * kLanes];
+            C[kk] = t.x;
+            S[kk] = t.y;
+        }
+    }
+    const int total = a.nAc * a.nBc;
+    const int stride = gridDim.x * WPB;
+    for (int t = blockIdx.x * WPB + wv; t < total; t += 2 * stride) {
+        LaneOrbit o0, o1;
+        lane_orbit_open(a, t, o0);
+        const bool has1 = t + stride < total;
+        if (has1) lane_orbit_open(a, t + stride, o1);
+        else o1.live = false;
+        if (!o0.live && !o1.live) continue;
+        if (!o1.live) {
+#pragma unroll
+            for (int p = 0; p < 4; p++) o1.ok[p] = false;
+        }
+        double x0[4], l0[4], x1[4], l1[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            x0[p] = o0.ok[p] ? v0[o0.idx[p]] : 0.0;
+            x1[p] = o1.ok[p] ? v0[o1.idx[p]] : 0.0;
+            if (BR) {
+                l0[p] = o0.ok[p] ? v1[o0.idx[p]] : 0.0;
+                l1[p] = o1.ok[p] ? v1[o1.idx[p]] : 0.0;
+            }
+        }
+        if (o0.live) lane_orbit_apply<BR>(a, nk, o0, x0, l0, C, S, acc);
+        if (o1.live) lane_orbit_apply<BR>(a, nk, o1, x1, l1, C, S, acc);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (o0.ok[p]) {
+                v0[o0.idx[p]] = x0[p];
+                if (BR) v1[o0.idx[p]] = l0[p];
+            }
+            if (o1.ok[p]) {
+                v0[o1.idx[p]] = x1[p];
+                if (BR) v1[o1.idx[p]] = l1[p];
+            }
+        }
+    }
+    if (BR) lanes_block_partials<kOrbMaxOps>(acc, nk, a.k0, a.k1, a.reverse, red, partial, c, M, slots);
+}
+
+// Two-group orbit segment in lane layout (see kernels.hpp): wave per orbit
+// of up to 16 members, lane per vector, the two runs of operators unrolled.
+// Uniform operator word: the same 6 pair bits and phases in all four
+// sub-orbits; each active pair is rotated in every sub-orbit.
+// (p, q) <- (C p + sS q, C q - sS p) with both results written back into
+// the registers of p and q.  Written out so that a conditional rotation
+// leaves the 16-member arrays where they were: with compiler-chosen
+// registers every taken pair block ended in ~32 register copies.
+__device__ __forceinline__ void rot_inplace(double &p, double &q, double C, double sS)
+{
+    double t, u;
+    // operands: %0 p, %1 q, %2 %3 temporaries, %4 C, %5 sS
+    asm("v_mul_f64 %2, %5, %1\n\t"
+        "v_mul_f64 %3, %5, %0\n\t"
+        "v_fma_f64 %0, %4, %0, %2\n\t"
+        "v_fma_f64 %1, %4, %1, -%3"
+        : "+v"(p), "+v"(q), "=&v"(t), "=&v"(u)
+        : "v"(C), "v"(sS));
+}
+
+// Adjoint step on a pair: brr += lp xq - lq xp (before the step), then the
+// rotation of (xp, xq) and of (lp, lq), every result in place.
+__device__ __forceinline__ void rot2_br_inplace(double &xp, double &xq, double &lp, double &lq, double &brr,
+                                                double C, double sS)
+{
+    double t1, t2, u2, t3, u3;
+    // operands: %0 xp, %1 xq, %2 lp, %3 lq, %4 brr, %5-%9 temporaries,
+    // %10 C, %11 sS
+    asm("v_mul_f64 %5, %2, %1\n\t"
+        "v_mul_f64 %6, %11, %3\n\t"
+        "v_mul_f64 %7, %11, %2\n\t"
+        "v_mul_f64 %8, %11, %1\n\t"
+        "v_mul_f64 %9, %11, %0\n\t"
+        "v_fma_f64 %5, -%3, %0, %5\n\t"
+        "v_fma_f64 %2, %10, %2, %6\n\t"
+        "v_fma_f64 %3, %10, %3, -%7\n\t"
+        "v_fma_f64 %0, %10, %0, %8\n\t"
+        "v_fma_f64 %1, %10, %1, -%9\n\t"
+        "v_add_f64 %4, %4, %5"
+        : "+v"(xp), "+v"(xq), "+v"(lp), "+v"(lq), "+v"(brr), "=&v"(t1), "=&v"(t2), "=&v"(u2), "=&v"(t3),
+          "=&v"(u3)
+        : "v"(C), "v"(sS));
+}
+
+template <int G, int H>
+__device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double C, double S, double (&x)[16],
+                                                   double (&l)[16], bool br_on, double &br)
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (!((w >> r) & 1u)) continue;
+        const double s = ((((w >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
+        const double sS = s * S;
+        double brr = 0;
+        const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int oa = c & 1, ob = c >> 1;
+            const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
+            const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
+            if (br_on) rot2_br_inplace(x[sp], x[sq], l[sp], l[sq], brr, C, sS);
+            else rot_inplace(x[sp], x[sq], C, sS);
+        }
+        if (br_on) br += s * brr;
+    }
+}
+
+// one operator of a run: its word, angle (per lane, from LDS) and bracket slot
+template <bool BR, int G, int H>
+__device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long long *pw, int kk, uint32_t spect,
+                                        const double2 (*csL)[kLanes], double *accW, int lane, double (&x)[16],
+                                        double (&l)[16])
+{
+    const unsigned w = (unsigned)pw[kk] & 0xfffu;
+    if (!w) return;
+    const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
+    const double2 t = csL[kk][lane];
+    double br = 0;
+    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br);
+    if (BR) accW[kk * kLanes + lane] += br;
+}
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+            const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    constexpr int K = kOrb2MaxOps;
+    __shared__ double2 csL[K][kLanes];
+    __shared__ double accL[BR ? WPB * K * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    const int nk = a.k1 - a.k0, m = a.split;
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        csL[kk][l] = cs[((long long)c * M + a.k0 + kk) * kLanes + l];
+    }
+    if (BR)
+        for (int e = threadIdx.x; e < WPB * K * kLanes; e += kBlock) accL[e] = 0.0;
+    __syncthreads();
+    double *accW = BR ? accL + wv * K * kLanes : nullptr;
+    v0 += c * cstride;
+    if (BR) v1 += c * cstride;
+    const int total = a.nAc * a.nBc;
+    const int stride = gridDim.x * WPB;
+    // the forward sweep takes two orbits per iteration (both loaded before
+    // either is computed: twice the bytes in flight per wave); the adjoint
+    // sweep, with twice the registers per orbit, one
+    constexpr int NO = BR ? 1 : 2;
+    for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
+        int rA[NO][4], rB[NO][4], type[NO];
+        uint32_t spect[NO];
+        bool live[NO];
+#pragma unroll
+        for (int o = 0; o < NO; o++) {
+            const int t = t0 + o * stride;
+            live[o] = t < total;
+            const int tt = live[o] ? t : t0;
+            const int ia = tt / a.nBc, ib = tt - ia * a.nBc;
+            const int4 ra = a.aRanks[ia], rb = a.bRanks[ib];
+            const uint32_t fa = a.aInfo[ia], fb = a.bInfo[ib];
+            rA[o][0] = ra.x; rA[o][1] = ra.y; rA[o][2] = ra.z; rA[o][3] = ra.w;
+            rB[o][0] = rb.x; rB[o][1] = rb.y; rB[o][2] = rb.z; rB[o][3] = rb.w;
+            const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
+            const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
+            live[o] = live[o] && na_ * nb_ >= 2;  // a lone determinant: nothing acts on it
+            type[o] = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+            spect[o] = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        }
+        // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
+        double x[NO][16], l[NO][16];
+#pragma unroll
+        for (int o = 0; o < NO; o++)
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int ma = s & 3, mb = s >> 2;
+                const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
+                const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                x[o][s] = ok ? (v0 + off)[lane] : 0.0;
+                l[o][s] = (BR && ok) ? (v1 + off)[lane] : 0.0;
+            }
+#pragma unroll
+        for (int o = 0; o < NO; o++) {
+            if (!live[o]) continue;
+            const unsigned long long *pw = a.prog + type[o] * K;
+            // the forward sweep (no brackets) runs forward, the adjoint sweep in
+            // reverse: a compile-time direction keeps the 16-member arrays in
+            // one register assignment (a runtime branch made the compiler copy
+            // them at the merge)
+            if (!BR) {
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    if (kk >= m) break;
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    if (kk < m || kk >= nk) continue;
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int kk = K - 1 - j;
+                    if (kk < m || kk >= nk) continue;
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int kk = K - 1 - j;
+                    if (kk >= m) continue;
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < NO; o++)
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int ma = s & 3, mb = s >> 2;
+                if (live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0) {
+                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                    (v0 + off)[lane] = x[o][s];
+                    if (BR) (v1 + off)[lane] = l[o][s];
+                }
+            }
+    }
+    if (BR) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+            const int kk = e / kLanes, l = e - kk * kLanes;
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < WPB; q++) s += accL[(q * K + kk) * kLanes + l];
+            partial[(((long long)c * M + a.k0 + kk) * slots + blockIdx.x) * kLanes + l] = s;
+        }
+    }
+}
+
+template __global__ void k_orb2v<false>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orb2v<true>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_opv(double *__restrict__ v0, double *__restrict__ v1, long long cstride, long long N, DevSector sec,
+          const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs, int M, double *__restrict__ partial,
+          int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    __shared__ double red[BR ? WPB * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    v0 += c * cstride + lane;
+    if (BR) v1 += c * cstride + lane;
+    const DevOp o = ops[k];
+    const double2 t = cs[((long long)c * M + k) * kLanes + lane];
+    const double C = t.x, S = t.y;
+    double acc[1] = {0};
+    for (long long i = blockIdx.x * WPB + wv; i < N; i += (long long)gridDim.x * WPB) {
+        uint32_t b2;
+        const uint32_t b = sec.states[i];
+        const int s = pair_sign(o, b, b2);
+        if (s == 0 || b2 < b) continue;
+        const long long j = sec.rank(b2);
+        if (j < 0) continue;
+        const double x = v0[i * kLanes], y = v0[j * kLanes];
+        if (BR) {
+            const double lx = v1[i * kLanes], ly = v1[j * kLanes];
+            acc[0] += s * (lx * y - ly * x);
+            v1[i * kLanes] = s * ly * S + lx * C;
+            v1[j * kLanes] = -s * lx * S + ly * C;
+        }
+        v0[i * kLanes] = s * y * S + x * C;
+        v0[j * kLanes] = -s * x * S + y * C;
+    }
+    if (BR) lanes_block_partials<1>(acc, 1, k, k + 1, 0, red, partial, c, M, slots);
+}
+
+template __global__ void k_orbv<false>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orbv<true>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_opv<false>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                      const double2 *, int, double *, int);
+template __global__ void k_opv<true>(double *, double *, long long, long long, DevSector, const DevOp *, int,
+                                     const double2 *, int, double *, int);
+
+} // namespace ae
