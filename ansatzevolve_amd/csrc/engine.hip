@@ -194,6 +194,12 @@ struct EngineImpl {
     int lslots = 1;   // most blocks of a lane-layout adjoint launch (= stride of the bracket partial rows)
     int lslotsF = 1;  // most blocks of a lane-layout forward launch
     DBuf<int> d_lrows;  // bracket partial rows written per operator (its launch's grid)
+    // stored forward states (lane layout): state s (before segment s) at
+    // bl_ck + s * nch * N * 64, s = 0 .. segments
+    int ckOpt = 1;        // kOptCheckpoint
+    bool ckShape = false;  // forward and adjoint lists share their segments, all orbit segments
+    bool ckOn = false;     // the uploaded batch uses stored states
+    DBuf<double> bl_ck;
     int numCU = 1;
 
     // tunable execution choices (Engine::set_option)
@@ -360,6 +366,11 @@ struct EngineImpl {
         };
         lslots = assign(laneA.segs, true);
         lslotsF = assign(laneF.segs, false);
+        ckShape = laneA.segs.size() == laneF.segs.size() && !laneA.segs.empty();
+        for (size_t i = 0; ckShape && i < laneA.segs.size(); i++) {
+            const Seg &a = laneA.segs[i], &f = laneF.segs[i];
+            ckShape = a.k0 == f.k0 && a.k1 == f.k1 && (a.two || a.orbA >= 0) && (f.two || f.orbA >= 0);
+        }
         // partial rows per operator: the grid of the adjoint launch that owns it
         std::vector<int> rows(M, 0);
         for (const Seg &sg : laneA.segs)
@@ -871,23 +882,31 @@ struct EngineImpl {
         return true;
     }
 
-    // Forward sweep of lane-layout chunks.
+    // state s of the stored forward states (checkpoint mode)
+    double *ck_state(int s, int nch) { return bl_ck.p + (size_t)s * nch * N * kLanes; }
+
+    // Forward sweep of lane-layout chunks: in place on v, or (checkpoint
+    // mode) segment s from state s to state s + 1, v being state 0.
     // (with timing on, every launch is bracketed by its own events)
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
-        for (const Seg &sg : laneF.segs) {
+        for (size_t si = 0; si < laneF.segs.size(); si++) {
+            const Seg &sg = laneF.segs[si];
+            const double *in = ckOn ? ck_state((int)si, nch) : v;
+            double *out = ckOn ? ck_state((int)si + 1, nch) : v;
             if (sg.two) {
                 const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orb2v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
-                                   orb2args(laneF, sg, false), v, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                                   orb2args(laneF, sg, false), in, out, nullptr, cstride, bl_cs.p, M, nullptr,
+                                   lslotsF);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orbv<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
-                                   orbargs(sg, false, laneF.lprog.p), v, nullptr, cstride, bl_cs.p, M, nullptr,
-                                   lslotsF);
+                                   orbargs(sg, false, laneF.lprog.p), in, out, nullptr, cstride, bl_cs.p, M,
+                                   nullptr, lslotsF);
                 mark_end(mk);
                 LCHK();
             } else {
@@ -905,22 +924,27 @@ struct EngineImpl {
 
     // Adjoint sweep of lane-layout (phi, lambda) chunks; every operator's
     // partial rows are written by exactly one launch (rows 0 .. its grid - 1).
+    // Checkpoint mode: segment s reads phi from stored state s + 1 and does
+    // not write it back (only lambda is updated in place).
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
         for (int si = (int)lsegs.size() - 1; si >= 0; si--) {
             const Seg &sg = lsegs[si];
+            const double *in = ckOn ? ck_state(si + 1, nch) : phi;
+            double *out = ckOn ? nullptr : phi;
             if (sg.two) {
                 const int mk = mark_begin(2);
                 hipLaunchKernelGGL((k_orb2v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
-                                   orb2args(laneA, sg, true), phi, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                                   orb2args(laneA, sg, true), in, out, lam, cstride, bl_csneg.p, M, bl_partial.p,
+                                   lslots);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
                 const int mk = mark_begin(2);
                 hipLaunchKernelGGL((k_orbv<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
-                                   orbargs(sg, true, laneA.lprog.p), phi, lam, cstride, bl_csneg.p, M, bl_partial.p,
-                                   lslots);
+                                   orbargs(sg, true, laneA.lprog.p), in, out, lam, cstride, bl_csneg.p, M,
+                                   bl_partial.p, lslots);
                 mark_end(mk);
                 LCHK();
             } else {
@@ -1605,6 +1629,20 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         if (!e.bl_psi.ensure(nch * e.N * kLanes, err) || !e.bl_lam.ensure(nch * e.N * kLanes, err) ||
             !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err))
             return false;
+        // stored forward states 0 .. segments, when they fit in half the
+        // free memory
+        bool ck = e.ckOpt && e.ckShape && e.hkind == 3 && e.sigv;
+        const double *ckWas = e.bl_ck.p;
+        if (ck) {
+            const size_t need = (e.laneF.segs.size() + 1) * nch * (size_t)e.N * kLanes;
+            size_t freeB = 0, totalB = 0;
+            HCHK(hipMemGetInfo(&freeB, &totalB));
+            const size_t have = e.bl_ck.n;
+            ck = need <= have || (need - have) * sizeof(double) <= freeB / 2;
+            if (ck && !e.bl_ck.ensure(need, err)) return false;
+        }
+        if (ck != e.ckOn || e.bl_ck.p != ckWas) e.drop_graphs();
+        e.ckOn = ck;
         if (!e.upload_cs_lanes(angles, nvec, err)) return false;
     } else {
         if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
@@ -1634,13 +1672,15 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
         const int nch = (nvec + kLanes - 1) / kLanes;
         const long long total = (long long)nch * e.N * kLanes;
         const dim3 tgrid((unsigned)((e.N + kLanes - 1) / kLanes), nch);
+        double *psi0 = e.ckOn ? e.ck_state(0, nch) : e.bl_psi.p;
         hipLaunchKernelGGL(k_bcast_lanes, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
-                           dim3(kBlock), 0, e.stream, e.d_start.p, e.bl_psi.p, e.N, nch);
+                           dim3(kBlock), 0, e.stream, e.d_start.p, psi0, e.N, nch);
         LCHK();
-        if (!e.sweep_forward_lanes(e.bl_psi.p, nch, err)) return false;
+        if (!e.sweep_forward_lanes(psi0, nch, err)) return false;
         if (e.hkind == 3 && e.sigv) {
             // matrix-free sigma in lane layout: no transposes
-            if (!e.ham_apply_lanes(e.bl_psi.p, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
+            const double *psi = e.ckOn ? e.ck_state((int)e.laneF.segs.size(), nch) : e.bl_psi.p;
+            if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
             if (with_gradient) {
                 if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
                 hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
@@ -1815,9 +1855,20 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.bnvec = 0;
         return true;
     }
+    case kOptCheckpoint:
+        if (value < 0 || value > 1) {
+            err = "checkpoint option must be 0 or 1";
+            return false;
+        }
+        if (value == e.ckOpt) return true;
+        e.ckOpt = value;
+        e.bnvec = 0;  // decided at the next upload
+        e.drop_graphs();
+        return true;
     case kOptHamKind:
     case kOptSegments:
     case kOptLaneSegments:
+    case kOptCheckpointActive:
         err = "read-only engine property";
         return false;
     default:
@@ -1839,6 +1890,8 @@ int Engine::option(int key) const
     case kOptSegments: return (int)e.segs.size();
     case kOptLaneGroups: return e.maxGroups;
     case kOptLaneGroupsFwd: return e.maxGroupsF;
+    case kOptCheckpoint: return e.ckOpt;
+    case kOptCheckpointActive: return e.ckOn ? 1 : 0;
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

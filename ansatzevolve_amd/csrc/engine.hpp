@@ -35,6 +35,10 @@ enum EngineOption : int {
     kOptLaneGroups = 7, // lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin and pass (1 or 2)
     kOptLaneSegments = 8,  // read only: fused segments per lane-layout adjoint sweep
     kOptLaneGroupsFwd = 9, // the same for the lane-layout forward sweep
+    kOptCheckpoint = 10,   // 1: the lane-layout gradient keeps every segment's forward state (out-of-place
+                           // forward, adjoint without state write-back) when the two sweeps share their
+                           // segments and the states fit in half the free memory; 0: in-place sweeps
+    kOptCheckpointActive = 11,  // read only: 1 when the uploaded batch runs with stored states
 };
 
 class Engine {

@@ -168,8 +168,12 @@ __global__ void k_bcast(const double *__restrict__ src, double *__restrict__ dst
 // of the segment (every block of the fixed `slots`-block grid writes its row).
 constexpr int kLanes = 64;
 
+// Amplitudes x are read from v0 and written to xout: xout == v0 in place;
+// another buffer out of place (every member written, lone determinants
+// copied); nullptr not written (adjoint sweep over a stored forward state).
+// The same holds for k_orb2v.
 template <bool BR>
-__global__ void k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+__global__ void k_orbv(OrbArgs a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                        const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 // Two-group orbit segments (lane layout): consecutive operators whose alpha
 // bits are one of at most two disjoint 2-bit groups (or none) and likewise for
@@ -203,7 +207,7 @@ struct Orb2Args {
 // BR = adjoint sweep: operators in reverse order with brackets (a.reverse is
 // ignored); !BR: forward order.
 template <bool BR>
-__global__ void k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+__global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 // one operator on whole lane-layout vectors (any sector), wave per amplitude
 template <bool BR>

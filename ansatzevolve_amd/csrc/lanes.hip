@@ -40,17 +40,19 @@ __device__ __forceinline__ void lanes_block_partials(const double (&acc)[NK], in
 struct LaneOrbit {
     long long idx[4];
     bool ok[4];
-    bool live;
+    bool live;  // members are loaded and stored
+    bool act;   // operators act (more than one member)
     int type;
     uint32_t spect;
 };
 
-__device__ __forceinline__ void lane_orbit_open(const OrbArgs &a, int t, LaneOrbit &o)
+__device__ __forceinline__ void lane_orbit_open(const OrbArgs &a, int t, LaneOrbit &o, bool copyAll)
 {
     const int ia = t / a.nBc, ib = t - ia * a.nBc;
     const int4 da = a.aDesc[ia], db = a.bDesc[ib];
     const int Ra = da.y >= 0 ? 2 : 1, Rb = db.y >= 0 ? 2 : 1;
-    o.live = Ra * Rb > 1;  // a lone determinant: no operator of the segment acts on it
+    o.act = Ra * Rb > 1;  // a lone determinant: no operator of the segment acts on it
+    o.live = o.act || copyAll;
     const int ta = Ra == 2 ? 0 : (((uint32_t)da.z & a.uaLocal) ? 2 : 1);
     const int tb = Rb == 2 ? 0 : (((uint32_t)db.z & a.ubLocal) ? 2 : 1);
     o.type = ta * 3 + tb;
@@ -105,7 +107,7 @@ __device__ __forceinline__ void lane_orbit_apply(const OrbArgs &a, int nk, const
 // in flight, not by arithmetic).
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
-    k_orbv(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+    k_orbv(OrbArgs a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
            const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
     constexpr int WPB = kBlock / kLanes;
@@ -114,7 +116,9 @@ __global__ void __launch_bounds__(kBlock)
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     const int c = blockIdx.y;
     v0 += c * cstride + lane;
+    if (xout) xout += c * cstride + lane;
     if (BR) v1 += c * cstride + lane;
+    const bool copyAll = xout && xout != v0;  // out of place: lone determinants are copied too
     cs += (long long)c * M * kLanes + lane;
     const int nk = a.k1 - a.k0;
     double C[kOrbMaxOps], S[kOrbMaxOps], acc[kOrbMaxOps];
@@ -134,10 +138,10 @@ __global__ void __launch_bounds__(kBlock)
     const int stride = gridDim.x * WPB;
     for (int t = blockIdx.x * WPB + wv; t < total; t += 2 * stride) {
         LaneOrbit o0, o1;
-        lane_orbit_open(a, t, o0);
+        lane_orbit_open(a, t, o0, copyAll);
         const bool has1 = t + stride < total;
-        if (has1) lane_orbit_open(a, t + stride, o1);
-        else o1.live = false;
+        if (has1) lane_orbit_open(a, t + stride, o1, copyAll);
+        else o1.live = o1.act = false;
         if (!o0.live && !o1.live) continue;
         if (!o1.live) {
 #pragma unroll
@@ -153,16 +157,16 @@ __global__ void __launch_bounds__(kBlock)
                 l1[p] = o1.ok[p] ? v1[o1.idx[p]] : 0.0;
             }
         }
-        if (o0.live) lane_orbit_apply<BR>(a, nk, o0, x0, l0, C, S, acc);
-        if (o1.live) lane_orbit_apply<BR>(a, nk, o1, x1, l1, C, S, acc);
+        if (o0.act) lane_orbit_apply<BR>(a, nk, o0, x0, l0, C, S, acc);
+        if (o1.act) lane_orbit_apply<BR>(a, nk, o1, x1, l1, C, S, acc);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (o0.ok[p]) {
-                v0[o0.idx[p]] = x0[p];
+                if (xout) xout[o0.idx[p]] = x0[p];
                 if (BR) v1[o0.idx[p]] = l0[p];
             }
             if (o1.ok[p]) {
-                v0[o1.idx[p]] = x1[p];
+                if (xout) xout[o1.idx[p]] = x1[p];
                 if (BR) v1[o1.idx[p]] = l1[p];
             }
         }
@@ -255,7 +259,7 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
 
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
-    k_orb2v(Orb2Args a, double *__restrict__ v0, double *__restrict__ v1, long long cstride,
+    k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
             const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
     constexpr int WPB = kBlock / kLanes;
@@ -275,7 +279,10 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     double *accW = BR ? accL + wv * K * kLanes : nullptr;
     v0 += c * cstride;
+    if (xout) xout += c * cstride;
     if (BR) v1 += c * cstride;
+    // out of place: every member is written, lone determinants included
+    const bool copyAll = xout && xout != v0;
     const int total = a.nAc * a.nBc;
     const int stride = gridDim.x * WPB;
     // the forward sweep takes two orbits per iteration (both loaded before
@@ -285,7 +292,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
         int rA[NO][4], rB[NO][4], type[NO];
         uint32_t spect[NO];
-        bool live[NO];
+        bool live[NO], act[NO];
 #pragma unroll
         for (int o = 0; o < NO; o++) {
             const int t = t0 + o * stride;
@@ -298,7 +305,8 @@ __global__ void __launch_bounds__(kBlock)
             rB[o][0] = rb.x; rB[o][1] = rb.y; rB[o][2] = rb.z; rB[o][3] = rb.w;
             const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
             const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
-            live[o] = live[o] && na_ * nb_ >= 2;  // a lone determinant: nothing acts on it
+            act[o] = live[o] && na_ * nb_ >= 2;  // a lone determinant: nothing acts on it
+            if (!copyAll) live[o] = act[o];
             type[o] = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
             spect[o] = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
         }
@@ -316,7 +324,7 @@ __global__ void __launch_bounds__(kBlock)
             }
 #pragma unroll
         for (int o = 0; o < NO; o++) {
-            if (!live[o]) continue;
+            if (!act[o]) continue;
             const unsigned long long *pw = a.prog + type[o] * K;
             // the forward sweep (no brackets) runs forward, the adjoint sweep in
             // reverse: a compile-time direction keeps the 16-member arrays in
@@ -355,7 +363,7 @@ __global__ void __launch_bounds__(kBlock)
                 const int ma = s & 3, mb = s >> 2;
                 if (live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0) {
                     const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                    (v0 + off)[lane] = x[o][s];
+                    if (xout) (xout + off)[lane] = x[o][s];
                     if (BR) (v1 + off)[lane] = l[o][s];
                 }
             }
@@ -372,8 +380,10 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-template __global__ void k_orb2v<false>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
-template __global__ void k_orb2v<true>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orb2v<false>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
+                                        double *, int);
+template __global__ void k_orb2v<true>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
+                                       double *, int);
 
 template <bool BR>
 __global__ void __launch_bounds__(kBlock)
@@ -412,8 +422,10 @@ __global__ void __launch_bounds__(kBlock)
     if (BR) lanes_block_partials<1>(acc, 1, k, k + 1, 0, red, partial, c, M, slots);
 }
 
-template __global__ void k_orbv<false>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
-template __global__ void k_orbv<true>(OrbArgs, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orbv<false>(OrbArgs, const double *, double *, double *, long long, const double2 *, int,
+                                       double *, int);
+template __global__ void k_orbv<true>(OrbArgs, const double *, double *, double *, long long, const double2 *, int,
+                                      double *, int);
 template __global__ void k_opv<false>(double *, double *, long long, long long, DevSector, const DevOp *, int,
                                       const double2 *, int, double *, int);
 template __global__ void k_opv<true>(double *, double *, long long, long long, DevSector, const DevOp *, int,

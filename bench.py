@@ -144,19 +144,21 @@ def main():
         rep = a.autotune(thetas, reps=2)
         tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
-                                c["choice"]["groups"], c["choice"]["groups_fwd"], round(c["ms"], 3)]
+                                c["choice"]["groups"], c["choice"]["groups_fwd"], c["choice"]["checkpoint"],
+                                round(c["ms"], 3)]
                                for c in rep.get("candidates", [])],
-                "candidate_fields": ["sweep", "depth", "lane_min", "groups", "groups_fwd", "ms"]}
+                "candidate_fields": ["sweep", "depth", "lane_min", "groups", "groups_fwd", "checkpoint", "ms"]}
     if args.lane_groups:
         a.set_option(a.OPT_LANE_GROUPS, args.lane_groups)
         a.set_option(a.OPT_LANE_GROUPS_FWD, args.lane_groups)
     opts = {"sweep": a.get_option(a.OPT_SWEEP), "fuse_depth": a.get_option(a.OPT_FUSE_DEPTH),
             "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH),
             "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_groups_fwd": a.get_option(a.OPT_LANE_GROUPS_FWD),
-            "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS)}
+            "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS), "checkpoint": a.get_option(a.OPT_CHECKPOINT)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
+    opts["checkpoint_active"] = a.get_option(a.OPT_CHECKPOINT_ACTIVE)
     for _ in range(args.warmup):
         a.run_batch(True)
     a.download_batch(True)  # drains the stream
@@ -212,14 +214,16 @@ def main():
                 "note": note}
 
     # sweeps: every segment reads and writes each vector once (forward: psi;
-    # adjoint: psi and lambda), 8 B per amplitude per access
+    # adjoint: psi and lambda), 8 B per amplitude per access; with stored
+    # forward states the adjoint reads psi and does not write it back
     k_fwd = ("k_orb2v<false>" if opts["lane_groups_fwd"] == 2 else "k_orbv<false>") + " (lane layout)" if lanes \
         else "k_orb<false>"
     k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
         else "k_orb<true>"
     rooflines["forward_sweep"] = hbm(k_fwd, 16.0 * N * args.nvec, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
                                      fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
-    rooflines["adjoint_sweep"] = hbm(k_rev, 32.0 * N * args.nvec, per_launch(rev_ms, rev_n, 1 if lanes else nseg),
+    rev_bytes = (24.0 if opts.get("checkpoint_active") else 32.0) * N * args.nvec
+    rooflines["adjoint_sweep"] = hbm(k_rev, rev_bytes, per_launch(rev_ms, rev_n, 1 if lanes else nseg),
                                      rev_n if lanes else rev_n * nseg, rev_ms / nbd)
     if ham_kind == 3 and gemm_n > 0:
         # lane-layout matrix-free sigma: the sparse kernel's algorithmic work
@@ -227,7 +231,7 @@ def main():
         # singles + alpha-dependent parts of beta singles + diagonal, 2 flops
         # each per vector; the DGEMMs cover the same-spin rows densely
         per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
-        rooflines["hamiltonian"] = fp64("k_sigv_sparse (lane-layout sigma, %d vectors)" % args.nvec,
+        rooflines["hamiltonian"] = fp64("k_sigw_sparse (lane-layout sigma, %d vectors)" % args.nvec,
                                         2.0 * N * per_el * args.nvec, ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
                                         "FP64 vector-ALU/LDS work priced against the FP64 peak")
         na = math.comb(n_, ku)

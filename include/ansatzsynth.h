@@ -186,9 +186,15 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *   7 lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin
  *     fused into one pass (1 or 2, default 1);
  *   8 (read only) fused segments per lane-layout adjoint sweep;
- *   9 the same as 7 for the lane-layout forward sweep.
- * Changing keys 0, 3, 4, 7 or 9 re-segments the operators; an uploaded angle
- * batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
+ *   9 the same as 7 for the lane-layout forward sweep;
+ *  10 stored forward states for the lane-layout gradient: 1 (default) keeps
+ *     the state after every segment (out-of-place forward sweep, adjoint
+ *     sweep reading them instead of writing psi back) when both sweeps share
+ *     their segments and the states fit in half the free device memory;
+ *     0 in-place sweeps;
+ *  11 (read only) 1 when the uploaded batch runs with stored states.
+ * Changing keys 0, 3, 4, 7, 9 or 10 re-segments the operators or changes the
+ * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */
 int setEngineOption(int key, int value, void *ctx);
 int getEngineOption(int key, int *value, void *ctx);

@@ -102,6 +102,35 @@ def test_two_group_segments_fuse_units(gpu):
             assert abs(E2[v] - Es) <= SAME and np.max(np.abs(G2[v] - gs)) <= SAME
 
 
+@pytest.mark.parametrize("groups", [1, 2])
+def test_stored_forward_states(gpu, groups):
+    """Gradient from stored forward states (out-of-place forward sweep, the
+    adjoint sweep reading each segment's state instead of un-rotating psi and
+    writing it back) against the in-place sweeps, over three 64-vector chunks;
+    the forward states differ only by rounding of the backward un-rotation."""
+    w, thetas = _tups_case(8, 4, 4242, 130)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, groups)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, groups)
+        a.set_option(AnsatzSynth.OPT_CHECKPOINT, 0)
+        E0, G0 = a.energy_gradients(thetas)
+        assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 0
+        a.set_option(AnsatzSynth.OPT_CHECKPOINT, 1)
+        E1, G1 = a.energy_gradients(thetas)
+        assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1
+        assert np.max(np.abs(E1 - E0)) <= SAME and np.max(np.abs(G1 - G0)) <= SAME
+        # mixed segmentations (two-group forward, one-group adjoint) fall back
+        if groups == 1:
+            a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
+            E2, G2 = a.energy_gradients(thetas)
+            assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 0
+            assert np.max(np.abs(E2 - E0)) <= SAME and np.max(np.abs(G2 - G0)) <= SAME
+        for v in (0, 129):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E1[v] - Es) <= SAME and np.max(np.abs(G1[v] - gs)) <= SAME
+
+
 def test_lane_layout_number_sector(gpu):
     """Operators that move an electron between spins: the particle-number
     sector (no S_z tiles), explicit triplet Hamiltonian."""
