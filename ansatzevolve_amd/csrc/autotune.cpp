@@ -26,7 +26,7 @@ void log_msg(const char *fmt, ...);
 namespace {
 
 struct Candidate {
-    int sweep, depth, laneMin, groups, groupsFwd, ckpt;
+    int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves;
 };
 
 std::string shape_key(const Engine &eng, int nvec, int nops)
@@ -41,7 +41,7 @@ std::string cand_json(const Candidate &c)
     std::ostringstream o;
     o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
       << ", \"groups\": " << c.groups << ", \"groups_fwd\": " << c.groupsFwd << ", \"checkpoint\": " << c.ckpt
-      << "}";
+      << ", \"sigma_halves\": " << c.halves << "}";
     return o.str();
 }
 
@@ -58,8 +58,8 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         Candidate t{};
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
                         "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d, "
-                        "\"groups_fwd\": %d, \"checkpoint\": %d}",
-                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt) == 6) {
+                        "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d}",
+                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves) == 7) {
             c = t;
             found = true;
         }
@@ -72,6 +72,7 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
     return eng.set_option(kOptSweep, c.sweep, err) && eng.set_option(kOptFuseDepth, c.depth, err) &&
            eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneGroupsFwd, c.groupsFwd, err) &&
            eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptCheckpoint, c.ckpt, err) &&
+           eng.set_option(kOptSigmaHalves, c.halves, err) &&
            eng.set_option(kOptGraph, 1, err);
 }
 
@@ -108,21 +109,23 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
             if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
             for (int d : depths) {
                 if (!sz && d != 0) continue;
-                cands.push_back({s, d, lanes ? 1 : 0, 1, 1, 0});
+                cands.push_back({s, d, lanes ? 1 : 0, 1, 1, 0, 0});
                 // the lane layout can also fuse two disjoint groups per pass
-                // (forward only, or both sweeps), and keep the forward states
-                // for the gradient
+                // (forward only, or both sweeps), keep the forward states for
+                // the gradient, and use the two-half sigma where it applies
                 if (lanes && sz && s == kSweepOrbit && d == 0) {
-                    cands.push_back({s, d, 1, 1, 2, 0});
-                    cands.push_back({s, d, 1, 2, 2, 0});
-                    cands.push_back({s, d, 1, 1, 1, 1});
-                    cands.push_back({s, d, 1, 2, 2, 1});
+                    cands.push_back({s, d, 1, 1, 2, 0, 0});
+                    cands.push_back({s, d, 1, 2, 2, 0, 0});
+                    cands.push_back({s, d, 1, 1, 1, 1, 0});
+                    cands.push_back({s, d, 1, 2, 2, 1, 0});
+                    if (eng.option(kOptSigmaHalvesAvailable) == 1) cands.push_back({s, d, 1, 2, 2, 1, 1});
                 }
             }
         }
 
-    const Candidate saved{eng.option(kOptSweep),      eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
-                          eng.option(kOptLaneGroups), eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint)};
+    const Candidate saved{eng.option(kOptSweep),         eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
+                          eng.option(kOptLaneGroups),    eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint),
+                          eng.option(kOptSigmaHalves)};
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";

@@ -99,6 +99,18 @@ struct Seg {
     int grid = 1;  // lane layout: blocks of its launches
 };
 
+// the two-half lane-layout sparse sigma kernel for (same-half, cross-half)
+// entry counts, or null
+static void (*sigh_kernel(int S, int X))(SighArgs)
+{
+    if (S == 2 && X == 2) return k_sigh_sparse<2, 2>;
+    if (S == 6 && X == 3) return k_sigh_sparse<6, 3>;
+    if (S == 12 && X == 4) return k_sigh_sparse<12, 4>;
+    if (S == 20 && X == 5) return k_sigh_sparse<20, 5>;
+    if (S == 30 && X == 6) return k_sigh_sparse<30, 6>;
+    return nullptr;
+}
+
 // the lane-layout sparse sigma kernel for ng groups of 4 single entries
 static void (*sigw_kernel(int ng))(SigvArgs)
 {
@@ -175,6 +187,12 @@ struct EngineImpl {
     DBuf<uint32_t> d_ts, d_ts1;
     int ntp = 0;
     int ntp16 = 0;  // lane-layout table: singles per beta rounded up to 4 (rows of kSigWNtp)
+    // two-half lane sigma (k_sigh_sparse, n = 2 kd): target-slot table, slot ranks
+    bool sigh = false;     // available
+    int sighOpt = 0;       // kOptSigmaHalves (autotuned; the one-row kernel is faster at 12 orbitals)
+    int sighS = 0, sighX = 0, sighNA = 0, sighNB = 0, sighTA = 0, sighT = 0;
+    DBuf<uint32_t> d_tsh;
+    DBuf<int> d_hl;
     rocblas_handle blas = nullptr;
     DBuf<double> bl_epart;
     DBuf<int32_t> d_col;
@@ -530,6 +548,84 @@ struct EngineImpl {
         if (!build_orbit_programs(lsegs, L.lprog, err)) return false;
         for (size_t i = 0; i < lsegs.size(); i++)
             if (lsegs[i].two) lsegs[i].progOff = (int)save[i];
+        return true;
+    }
+
+    // Tables of the two-half lane sigma (k_sigh_sparse): beta strings split
+    // by the occupation of orbital kSigHOrb (half A occupied), slot = position
+    // within the half; target slots A (padded to 4) then B; per target its
+    // A-phase entries then its B-phase entries, then rank, own slot offset,
+    // string.  Entry coefficient slots as k_beta_table's (r occupied, q empty
+    // in the target string; sign half when an odd number of occupied
+    // orbitals lies between them).
+    bool build_sigh_tables(std::string &err)
+    {
+        const int kd = __builtin_popcount(unB[0]);
+        const int S = (kd - 1) * (n - kd), X = n - kd;
+        const uint32_t pbit = 1u << kSigHOrb, full = (1u << n) - 1;
+        std::vector<int> hl(2 * kSigHMax, 0), slotOf(nb);
+        int nA = 0, nB = 0;
+        for (int b = 0; b < nb; b++) {
+            if (unB[b] & pbit) {
+                slotOf[b] = nA;
+                hl[nA++] = b;
+            } else {
+                slotOf[b] = nB;
+                hl[kSigHMax + nB++] = b;
+            }
+        }
+        if (nA > kSigHMax || nB > kSigHMax) {
+            err = "two-half sigma: a half exceeds the LDS buffer";
+            return false;
+        }
+        const int TA = (nA + 3) / 4 * 4, T = TA + (nB + 3) / 4 * 4;
+        const uint32_t pad = (uint32_t)(2 * kSigNN * 8) << 17;
+        std::vector<uint32_t> tab((size_t)T * kSigWNtp, pad);
+        auto parity = [](uint32_t st, int r, int q) {
+            const int lo = std::min(r, q), hi = std::max(r, q);
+            const uint32_t msk = ((1u << hi) - 1) & ~((1u << (lo + 1)) - 1);
+            return __builtin_popcount(st & msk) & 1;
+        };
+        for (int t = 0; t < T; t++) {
+            uint32_t *row = &tab[(size_t)t * kSigWNtp];
+            const int b = t < TA ? (t < nA ? hl[t] : -1) : (t - TA < nB ? hl[kSigHMax + t - TA] : -1);
+            row[kSigWNtp - 3] = 0xffffffffu;
+            row[kSigWNtp - 2] = 0;
+            row[kSigWNtp - 1] = 0;
+            if (b < 0) continue;
+            const uint32_t st = unB[b];
+            const bool isA = (st & pbit) != 0;
+            std::vector<uint32_t> same, cross;
+            for (uint32_t o = st; o; o &= o - 1)
+                for (uint32_t wq = ~st & full; wq; wq &= wq - 1) {
+                    const int r = __builtin_ctz(o), q = __builtin_ctz(wq);
+                    const int J = rkB[st ^ (1u << r) ^ (1u << q)];
+                    const int ecf = r * n + q + (parity(st, r, q) ? kSigNN : 0);
+                    const uint32_t ent = (uint32_t)(slotOf[J] * kSigWVC * 8) | ((uint32_t)(ecf * 8) << 17);
+                    (((unB[J] & pbit) != 0) == isA ? same : cross).push_back(ent);
+                }
+            if ((int)same.size() != S || (int)cross.size() != X) {
+                err = "two-half sigma: unexpected entry counts";
+                return false;
+            }
+            int k = 0;
+            for (uint32_t ent : isA ? same : cross) row[k++] = ent;
+            for (uint32_t ent : isA ? cross : same) row[k++] = ent;
+            row[kSigWNtp - 3] = (uint32_t)b;
+            row[kSigWNtp - 2] = (uint32_t)(slotOf[b] * kSigWVC * 8);
+            row[kSigWNtp - 1] = st;
+        }
+        if (!d_tsh.ensure(tab.size(), err) || !d_hl.ensure(hl.size(), err)) return false;
+        HCHK(hipMemcpy(d_tsh.p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(d_hl.p, hl.data(), hl.size() * sizeof(int), hipMemcpyHostToDevice));
+        HCHK(hipFuncSetAttribute((const void *)sigh_kernel(S, X), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kSigHLds));
+        sighS = S;
+        sighX = X;
+        sighNA = nA;
+        sighNB = nB;
+        sighTA = TA;
+        sighT = T;
         return true;
     }
 
@@ -973,7 +1069,22 @@ struct EngineImpl {
         s.y = y;
         s.cstride = N * kLanes;
         const unsigned gx = (unsigned)((na + 7) / 8 * 32);
-        hipLaunchKernelGGL(sigw_kernel(ntp16 / 4), dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
+        if (sigh && sighOpt) {
+            SighArgs h;
+            h.m = s.m;
+            h.ts = d_tsh.p;
+            h.hl = d_hl.p;
+            h.nA = sighNA;
+            h.nB = sighNB;
+            h.TA = sighTA;
+            h.T = sighT;
+            h.x = x;
+            h.y = y;
+            h.cstride = s.cstride;
+            hipLaunchKernelGGL(sigh_kernel(sighS, sighX), dim3(gx, nch), dim3(kSigWThreads), kSigHLds, stream, h);
+        } else {
+            hipLaunchKernelGGL(sigw_kernel(ntp16 / 4), dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
+        }
         mark_end(mk);
         LCHK();
         const int mk3 = mark_begin(3);
@@ -1444,6 +1555,8 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             if (lanesOk)
                 HCHK(hipFuncSetAttribute((const void *)sigw_kernel(e.ntp16 / 4),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSigWLds));
+            e.sigh = lanesOk && n == 2 * kd && sigh_kernel((kd - 1) * (n - kd), n - kd) != nullptr;
+            if (e.sigh && !e.build_sigh_tables(err)) return false;
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));
             if (!e.blas) {
@@ -1855,6 +1968,15 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.bnvec = 0;
         return true;
     }
+    case kOptSigmaHalves:
+        if (value < 0 || value > 1) {
+            err = "sigma halves option must be 0 or 1";
+            return false;
+        }
+        if (value == e.sighOpt) return true;
+        e.sighOpt = value;
+        e.drop_graphs();
+        return true;
     case kOptCheckpoint:
         if (value < 0 || value > 1) {
             err = "checkpoint option must be 0 or 1";
@@ -1869,6 +1991,7 @@ bool Engine::set_option(int key, int value, std::string &err)
     case kOptSegments:
     case kOptLaneSegments:
     case kOptCheckpointActive:
+    case kOptSigmaHalvesAvailable:
         err = "read-only engine property";
         return false;
     default:
@@ -1892,6 +2015,8 @@ int Engine::option(int key) const
     case kOptLaneGroupsFwd: return e.maxGroupsF;
     case kOptCheckpoint: return e.ckOpt;
     case kOptCheckpointActive: return e.ckOn ? 1 : 0;
+    case kOptSigmaHalves: return e.sighOpt;
+    case kOptSigmaHalvesAvailable: return e.sigh ? 1 : 0;
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

@@ -39,6 +39,9 @@ enum EngineOption : int {
                            // forward, adjoint without state write-back) when the two sweeps share their
                            // segments and the states fit in half the free memory; 0: in-place sweeps
     kOptCheckpointActive = 11,  // read only: 1 when the uploaded batch runs with stored states
+    kOptSigmaHalves = 12,       // 1: lane-layout sigma with double-buffered source halves when available,
+                                // 0 (default): the one-row kernel
+    kOptSigmaHalvesAvailable = 13,  // read only: 1 when the two-half sigma applies (n = 2 kd <= 12)
 };
 
 class Engine {

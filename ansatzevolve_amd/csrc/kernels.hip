@@ -1242,6 +1242,220 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
     }
 }
 
+// Lane-layout sigma, sparse part, with the source row split in two halves
+// by the occupation of beta orbital kSigHOrb and double-buffered: while one
+// half is summed, the next half is copied from HBM into the other LDS buffer
+// by asynchronous global-to-LDS loads (no registers held).  A single with
+// r, q != kSigHOrb keeps the half, one moving that orbital crosses it, so for
+// n = 2 kd every target has S same-half and X cross-half entries; targets are
+// ordered by half (each half padded to a multiple of 4), so a wave's four
+// targets share their counts.  Per phase a target sums its entries for the
+// staged half; its diagonal terms come in the phase of its own half.
+template <int J>
+__device__ __forceinline__ void sigw_pair(double cl, uint32_t xb, uint32_t lb, double (&s)[4])
+{
+    const double x0 = lds_at(row_bcast<J>(xb) + lb), x1 = lds_at(row_bcast<J + 1>(xb) + lb);
+    __builtin_amdgcn_sched_barrier(0);
+    fma_row_bcast<J>(s[0], cl, x0);
+    fma_row_bcast<J + 1>(s[1], cl, x1);
+}
+template <int J>
+__device__ __forceinline__ void sigw_single(double cl, uint32_t xb, uint32_t lb, double (&s)[4])
+{
+    const double x0 = lds_at(row_bcast<J>(xb) + lb);
+    fma_row_bcast<J>(s[2], cl, x0);
+}
+// the first NIN (1..16) entries of a chunk
+template <int NIN>
+__device__ __forceinline__ void sigh_chunk(uint32_t gsb, uint32_t ent, uint32_t lb, double (&s)[4])
+{
+    const double cl = lds_at(gsb + (ent >> 17));
+    const uint32_t xb = ent & 0x1ffffu;
+    if constexpr (NIN >= 4) sigw_group<0>(cl, xb, lb, s);
+    if constexpr (NIN >= 8) sigw_group<4>(cl, xb, lb, s);
+    if constexpr (NIN >= 12) sigw_group<8>(cl, xb, lb, s);
+    if constexpr (NIN >= 16) sigw_group<12>(cl, xb, lb, s);
+    constexpr int G = NIN & ~3;
+    if constexpr ((NIN & 3) >= 2) sigw_pair<G>(cl, xb, lb, s);
+    if constexpr (NIN & 1) sigw_single<NIN - 1>(cl, xb, lb, s);
+}
+// C entries from two prefetched chunk entries
+template <int C>
+__device__ __forceinline__ void sigh_run(uint32_t gsb, const uint32_t (&ent)[2], uint32_t lb, double (&s)[4])
+{
+    sigh_chunk<(C < 16 ? C : 16)>(gsb, ent[0], lb, s);
+    if constexpr (C > 16) sigh_chunk<C - 16>(gsb, ent[1], lb, s);
+}
+
+template <int S, int X>
+__global__ void __launch_bounds__(kSigWThreads) k_sigh_sparse(SighArgs a)
+{
+    static_assert(S <= 32 && X <= 16, "at most two chunks of same-half entries");
+    extern __shared__ double sm[];
+    const MfArgs &m = a.m;
+    const int n = m.n, nb = m.nb, nn = n * n;
+    // XCD-aware order: the 4 sub-chunks of one alpha row run on one XCD
+    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
+    const int ar = xcd + 8 * (k >> 2), sub = k & 3;
+    if (ar >= m.na) return;
+    const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigWVC;
+    const double *__restrict__ x = a.x + coff;
+    double *__restrict__ y = a.y + coff;
+    double *buf0 = sm, *buf1 = sm + kSigHMax * kSigWVC;
+    double *cf0 = buf1 + kSigHMax * kSigWVC, *cf1 = cf0 + kSigGS, *dt0 = cf1 + kSigGS, *dt1 = dt0 + 128;
+    int *hls = (int *)(dt1 + 128);  // slot -> beta rank of both halves
+    const int tid = threadIdx.x, lane = tid & (kLanes - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid / kLanes);
+    // waves 0 .. kSigHWaves-1 sum; the last wave only copies the next half
+    // (its outstanding copies then never delay the summing waves' own loads)
+    const bool stager = w == kSigHWaves;
+    const int v = lane & 15, trow = w * 4 + (lane >> 4);
+    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const int ka = __popc(alpha), ne = n - ka;
+    const int R = 1 + ka * ne;
+    constexpr uint32_t kPad = (uint32_t)(2 * kSigNN * 8) << 17;
+    for (int i = tid; i < 2 * kSigHMax; i += kSigWThreads) hls[i] = a.hl[i];
+    // source row r: 0 the identity row (J = ar), else the alpha single r - 1
+    auto source = [&](int r, int &rA, int &qA) {
+        if (r == 0) return ar;
+        const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
+        uint32_t oo = alpha, ww = ~alpha & full;
+        for (int i = 0; i < ir; i++) oo &= oo - 1;
+        for (int i = 0; i < iq; i++) ww &= ww - 1;
+        rA = __ffs(oo) - 1;
+        qA = __ffs(ww) - 1;
+        return (int)m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
+    };
+    // signed coefficient row and beta-diagonal sums of source row r (as in
+    // k_sigw_sparse), by the summing waves
+    auto coefficients = [&](int r, int rA, int qA, double *gs, double *dt) {
+        const bool neg = r > 0 && between_parity(alpha, rA, qA);
+        const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+        for (int e = tid; e < nn; e += kSigHWaves * kLanes) {
+            double c = 0;
+            if (r == 0) {
+                if (e / n != e % n)
+                    for (uint32_t o = alpha; o; o &= o - 1) {
+                        const int xo = __ffs(o) - 1;
+                        c += thr13(m.g[(size_t)(xo * n + xo) * nn + e]);
+                    }
+            } else {
+                c = neg ? -thr13(gb[e]) : thr13(gb[e]);
+            }
+            gs[e] = c;
+            gs[kSigNN + e] = -c;
+        }
+        if (tid < kSigGS - 2 * kSigNN) gs[2 * kSigNN + tid] = 0.0;
+        if (tid < 128) {
+            const int hi = tid >> 6, t = tid & 63;
+            double d = 0;
+            if (r > 0)
+                for (int b = 0; b < 6; b++) {
+                    const int q = b + 6 * hi;
+                    if (q < n && ((t >> b) & 1)) d += neg ? -thr13(gb[q * n + q]) : thr13(gb[q * n + q]);
+                }
+            dt[tid] = d;
+        }
+    };
+    // half h of source row J into dst, by the staging wave: 8 slots (1 KB)
+    // per instruction, lane l -> slot 8c + l/8, vectors 2 (l mod 8) and
+    // 2 (l mod 8) + 1; slot ranks come from LDS, so no load waits between
+    // the copies
+    auto stage = [&](int J, int h, double *dst) {
+        const int cnt = h ? a.nB : a.nA;
+        const double *src = x + (long long)J * nb * kLanes + (lane & 7) * 2;
+        for (int c = 0; c * 8 < cnt; c++) {
+            const int slot = c * 8 + (lane >> 3);
+            const int beta = hls[h * kSigHMax + (slot < cnt ? slot : 0)];
+            __builtin_amdgcn_global_load_lds((const void *)(src + (long long)beta * kLanes),
+                                             (__attribute__((address_space(3))) void *)(dst + c * 128), 16, 0, 0);
+        }
+    };
+    double acc[kSigHNI];
+#pragma unroll
+    for (int i = 0; i < kSigHNI; i++) acc[i] = 0;
+    int rA = 0, qA = 0, Jcur = ar;
+    if (!stager) coefficients(0, 0, 0, cf0, dt0);
+    __syncthreads();  // slot ranks in LDS
+    if (stager) stage(ar, 0, buf0);
+    for (int ph = 0; ph < 2 * R; ph++) {
+        const int r = ph >> 1, half = ph & 1;
+        // the staging wave's copies of this phase's half have landed; after
+        // the barrier the other buffer and coefficient row are free
+        if (stager) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        __syncthreads();
+        if (half == 0) {
+            if (stager) stage(Jcur, 1, buf1);
+        } else if (r + 1 < R) {
+            Jcur = source(r + 1, rA, qA);
+            if (stager) stage(Jcur, 0, buf0);
+            else coefficients(r + 1, rA, qA, ((r + 1) & 1) ? cf1 : cf0, ((r + 1) & 1) ? dt1 : dt0);
+        }
+        if (stager) continue;
+        const double *gs = (r & 1) ? cf1 : cf0, *dt = (r & 1) ? dt1 : dt0;
+        const double *xs = half ? buf1 : buf0;
+        const uint32_t gsb = (uint32_t)(size_t)(lds_f64 *)gs;
+        const uint32_t lb = (uint32_t)(size_t)(lds_f64 *)xs + v * 8;
+        // entries of the next pass are loaded while this pass sums; the empty
+        // asm keeps them from being hoisted out of the phase loop
+        uint32_t tb = (uint32_t)(trow * kSigWNtp + v);
+        asm volatile("" : "+v"(tb));
+        auto load = [&](int t, uint32_t (&ent)[2], uint32_t (&meta)[3]) {
+            const bool isA = t < a.TA;
+            const int off = half == 0 ? 0 : (isA ? S : X);
+            const int cnt = (half == 0) == isA ? S : X;  // a second chunk only when there are > 16 entries
+            const uint32_t base = tb + (uint32_t)((t - trow) * kSigWNtp);
+            const bool ok = t < a.T;
+            ent[0] = ok ? a.ts[base + off] : kPad;
+            ent[1] = ok && cnt > 16 ? a.ts[base + off + 16] : kPad;
+            meta[0] = ok ? a.ts[base - v + kSigWNtp - 3] : 0xffffffffu;  // beta rank (or -1)
+            meta[1] = ok ? a.ts[base - v + kSigWNtp - 2] : 0u;           // own slot byte offset
+            meta[2] = ok ? a.ts[base - v + kSigWNtp - 1] : 0u;           // the string
+        };
+        uint32_t ent[2], meta[3], entn[2], metan[3];
+        load(trow, ent, meta);
+#pragma unroll
+        for (int i = 0; i < kSigHNI; i++) {
+            if (kSigHPass * i >= a.T) break;
+            const int t = trow + kSigHPass * i;
+            load(t + kSigHPass, entn, metan);
+            const bool isA = __builtin_amdgcn_readfirstlane(t) < a.TA;  // the same for the wave's four targets
+            const bool same = (half == 0) == isA;
+            double s[4] = {0, 0, 0, 0};
+            if (same) sigh_run<S>(gsb, ent, lb, s);
+            else sigh_run<X>(gsb, ent, lb, s);
+            acc[i] += (s[0] + s[1]) + (s[2] + s[3]);
+            if (same && meta[0] != 0xffffffffu) {
+                // the target's own half: beta-diagonal sums and, for the
+                // identity row, the diagonal
+                const double xt = lds_at(lb + meta[1]);
+                acc[i] += (dt[meta[2] & 63] + dt[64 + ((meta[2] >> 6) & 63)]) * xt;
+                if (r == 0) acc[i] += m.diag[(long long)ar * nb + meta[0]] * xt;
+            }
+            ent[0] = entn[0];
+            ent[1] = entn[1];
+            meta[0] = metan[0];
+            meta[1] = metan[1];
+            meta[2] = metan[2];
+        }
+    }
+    if (stager) return;
+#pragma unroll
+    for (int i = 0; i < kSigHNI; i++) {
+        const int t = trow + kSigHPass * i;
+        if (t < a.T) {
+            const uint32_t rank = a.ts[(size_t)t * kSigWNtp + kSigWNtp - 3];
+            if (rank != 0xffffffffu) y[((long long)ar * nb + rank) * kLanes + v] = acc[i];
+        }
+    }
+}
+
+template __global__ void k_sigh_sparse<2, 2>(SighArgs);
+template __global__ void k_sigh_sparse<6, 3>(SighArgs);
+template __global__ void k_sigh_sparse<12, 4>(SighArgs);
+template __global__ void k_sigh_sparse<20, 5>(SighArgs);
+template __global__ void k_sigh_sparse<30, 6>(SighArgs);
+
 template __global__ void k_sigw_sparse<1>(SigvArgs);
 template __global__ void k_sigw_sparse<2>(SigvArgs);
 template __global__ void k_sigw_sparse<3>(SigvArgs);

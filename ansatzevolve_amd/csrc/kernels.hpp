@@ -329,6 +329,34 @@ constexpr int kSigWPre = (kSigMaxNb * kSigWVC + kSigWThreads - 1) / kSigWThreads
 constexpr int kSigWH = 3;                       // entry chunks of 16 (n <= 12: at most 42 entries)
 constexpr int kSigWNtp = 16 * kSigWH;
 constexpr size_t kSigWLds = (size_t)(kSigGS + 128 + kSigMaxNb * kSigWVC) * sizeof(double);
+// Two-half variant (n = 2 kd): the source row split by the occupation of
+// beta orbital kSigHOrb, halves double-buffered in LDS.  Target slots: the
+// strings with that orbital occupied (half A, nA of them, padded to TA, a
+// multiple of 4), then the others (half B) up to T.  Table row per target
+// slot (kSigWNtp entries): its entries for the A phase, then for the B phase
+// (A targets: S same-half then X cross; B targets: X cross then S same;
+// entry = source slot * 128 | coefficient byte offset << 17), and in the
+// last three slots the beta rank (0xffffffff for padding), the byte offset
+// of its own slot in its half, and the string.  hl[h * kSigHMax + slot] =
+// beta rank of a half's slot.
+constexpr int kSigHOrb = 0;
+constexpr int kSigHMax = 464;  // slots per half buffer (C(11, 5) = 462, padded to 8)
+struct SighArgs {
+    MfArgs m;
+    const uint32_t *ts;
+    const int *hl;
+    int nA, nB, TA, T;
+    const double *x;
+    double *y;
+    long long cstride;  // N * 64
+};
+constexpr int kSigHWaves = kSigWThreads / 64 - 1;  // summing waves (the last wave stages)
+constexpr int kSigHPass = 4 * kSigHWaves;              // targets per pass
+constexpr int kSigHNI = (2 * kSigHMax + kSigHPass - 1) / kSigHPass;
+constexpr size_t kSigHLds =
+    (size_t)(2 * kSigHMax * kSigWVC + 2 * kSigGS + 2 * 128) * sizeof(double) + 2 * kSigHMax * sizeof(int);
+template <int S, int X>
+__global__ void k_sigh_sparse(SighArgs a);  // grid (ceil(na / 8) * 32, chunks), kSigWThreads, kSigHLds
 // NG = groups of 4 single entries per beta (1..9; a.ntp = 4 NG)
 template <int NG>
 __global__ void k_sigw_sparse(SigvArgs a);  // grid (ceil(na / 8) * 32, chunks), kSigWThreads, kSigWLds

@@ -192,7 +192,12 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     sweep reading them instead of writing psi back) when both sweeps share
  *     their segments and the states fit in half the free device memory;
  *     0 in-place sweeps;
- *  11 (read only) 1 when the uploaded batch runs with stored states.
+ *  11 (read only) 1 when the uploaded batch runs with stored states;
+ *  12 lane-layout matrix-free sigma: 1 the variant with the source rows
+ *     split in two double-buffered halves when it applies, 0 (default) the
+ *     one-row variant;
+ *  13 (read only) 1 when the two-half variant applies (12 or fewer
+ *     orbitals, as many beta electrons as empty beta orbitals).
  * Changing keys 0, 3, 4, 7, 9 or 10 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

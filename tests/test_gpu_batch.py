@@ -131,6 +131,24 @@ def test_stored_forward_states(gpu, groups):
             assert abs(E1[v] - Es) <= SAME and np.max(np.abs(G1[v] - gs)) <= SAME
 
 
+@pytest.mark.parametrize("n", [6, 8, 10])
+def test_sigma_halves_match_one_row(gpu, n):
+    """The lane-layout sigma with the source row split in two double-buffered
+    halves (asynchronous global-to-LDS copies by a staging wave) against the
+    one-row variant: the same sums in another order."""
+    w, thetas = _tups_case(n, 2, 900 + n, 70)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_HALVES_AVAILABLE) == 1
+        a.set_option(AnsatzSynth.OPT_SIGMA_HALVES, 0)
+        E0, G0 = a.energy_gradients(thetas)
+        a.set_option(AnsatzSynth.OPT_SIGMA_HALVES, 1)
+        E1, G1 = a.energy_gradients(thetas)
+        assert np.max(np.abs(E1 - E0)) <= SAME and np.max(np.abs(G1 - G0)) <= SAME
+        Es, gs = a.energy_gradient(thetas[69])
+        assert abs(E1[69] - Es) <= SAME and np.max(np.abs(G1[69] - gs)) <= SAME
+
+
 def test_lane_layout_number_sector(gpu):
     """Operators that move an electron between spins: the particle-number
     sector (no S_z tiles), explicit triplet Hamiltonian."""
