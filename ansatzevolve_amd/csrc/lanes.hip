@@ -258,7 +258,7 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
 }
 
 template <bool BR>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 5)  // at most 96 VGPRs: 5 waves per SIMD
     k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
             const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
@@ -319,8 +319,8 @@ __global__ void __launch_bounds__(kBlock)
                 const int ma = s & 3, mb = s >> 2;
                 const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
                 const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                x[o][s] = ok ? (v0 + off)[lane] : 0.0;
-                l[o][s] = (BR && ok) ? (v1 + off)[lane] : 0.0;
+                x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
+                l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
             }
 #pragma unroll
         for (int o = 0; o < NO; o++) {
@@ -363,8 +363,8 @@ __global__ void __launch_bounds__(kBlock)
                 const int ma = s & 3, mb = s >> 2;
                 if (live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0) {
                     const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                    if (xout) (xout + off)[lane] = x[o][s];
-                    if (BR) (v1 + off)[lane] = l[o][s];
+                    if (xout) __builtin_nontemporal_store(x[o][s], xout + off + lane);
+                    if (BR) __builtin_nontemporal_store(l[o][s], v1 + off + lane);
                 }
             }
     }
