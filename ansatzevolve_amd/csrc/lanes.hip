@@ -258,7 +258,8 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
 }
 
 template <bool BR>
-__global__ void __launch_bounds__(kBlock, 5)  // at most 96 VGPRs: 5 waves per SIMD
+// forward: at most 80 VGPRs (6 waves per SIMD); adjoint: 96 (5 waves)
+__global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
             const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
