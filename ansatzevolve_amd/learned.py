@@ -119,19 +119,36 @@ class WarmStartMLP:
 
 # ------------------------------------------------------------------ pipeline
 # Instances of the learned warm-start workload: n spatial orbitals at half
-# filling, site spacing s ~ U(1.0, 2.5), the seeded integral generator, TUPS
-# with L = n / 2 layers, the alternating determinant as the start.
+# filling, TUPS with L = n / 2 layers, the alternating determinant as the
+# start.  Two families:
+#   "random": the seeded integral generator with its own seed per instance
+#     (site spacing s ~ U(1.0, 2.5)): every instance has its own random
+#     two-electron factors;
+#   "smooth": the generator at one fixed seed per orbital count, so an
+#     instance is fixed by two continuous parameters, the site spacing
+#     s ~ U(1.0, 2.5) and a scale of the two-electron integrals
+#     lambda ~ U(0.6, 1.4) (a chain stretched or compressed, with stronger or
+#     weaker interaction).
 WARM_SEED = 20260500
+WARM_SMOOTH_SEED = 20260700
 
 
-def warm_instance(index, n):
+def warm_instance(index, n, family="smooth"):
     from .workloads import hf_index, synthetic_integrals, tups_operators
     rng = np.random.default_rng(WARM_SEED + 1000 * n + index)
     spacing = float(rng.uniform(1.0, 2.5))
-    h, g = synthetic_integrals(n, WARM_SEED + 100000 + 1000 * n + index, spacing=spacing)
+    if family == "random":
+        lam = 1.0
+        h, g = synthetic_integrals(n, WARM_SEED + 100000 + 1000 * n + index, spacing=spacing)
+    elif family == "smooth":
+        lam = float(rng.uniform(0.6, 1.4))
+        h, g = synthetic_integrals(n, WARM_SMOOTH_SEED + n, spacing=spacing)
+        g = lam * g
+    else:
+        raise ValueError("unknown instance family %r" % family)
     ops, order, scale = tups_operators(n, n // 2)
     return dict(n=n, nq=2 * n, h=h, g=g, ops=ops, order=order, scale=scale, start_idx=hf_index(n),
-                P=int(order.max()) + 1, spacing=spacing, index=index)
+                P=int(order.max()) + 1, spacing=spacing, lam=lam, index=index)
 
 
 def wrap_angles(t):
@@ -198,6 +215,29 @@ def solve(w, theta0, stop_energy=float("nan")):
         return a.optimise(np.asarray(theta0, dtype=np.float64), stop_energy=stop_energy)
 
 
+def consistent_targets(insts, sols, neighbours=2, log=None):
+    """Training targets that vary smoothly across the family: for every
+    training instance the plain optimiser is also started from the optimised
+    angles of its nearest training neighbours in (spacing, lambda) (same
+    orbital count), and the lowest minimum found is kept.  The optimiser from
+    the standard guess alone lands in different local minima on neighbouring
+    instances, which no regression can fit."""
+    out = [(np.asarray(t), float(E)) for t, E, _ in sols]
+    params = np.array([[w["spacing"], w["lam"]] for w in insts])
+    for i, w in enumerate(insts):
+        same = [j for j in range(len(insts)) if j != i and insts[j]["n"] == w["n"]]
+        if not same:
+            continue
+        d = np.linalg.norm(params[same] - params[i], axis=1)
+        for j in [same[k] for k in np.argsort(d)[:neighbours]]:
+            th, E, _ = solve(w, sols[j][0])
+            if E < out[i][1] - 1e-9:
+                out[i] = (np.asarray(th), float(E))
+        if log and i % 20 == 0:
+            log("[warmstart] consistent targets %d/%d" % (i + 1, len(insts)))
+    return out
+
+
 def start_energies(insts, starts):
     """Exact FP64 energies of candidate starts, starts[i] (k x P_i) for
     instance i: one launch over all instances (uneven sizes)."""
@@ -255,9 +295,11 @@ def finetune_energy(model, insts, steps=300, lr=3e-4, log=None):
     return hist
 
 
-def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidden=256, log=print):
-    """Generate (plain optimiser from the standard guess theta = 0), train
-    (supervised on the optimised angles, then energy fine-tuning), and
+def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidden=256, log=print,
+                 family="smooth", neighbours=2):
+    """Generate (plain optimiser from the standard guess theta = 0; for the
+    training set also from neighbours' optima, keeping the lowest minimum),
+    train (supervised on the optimised angles, then energy fine-tuning), and
     evaluate time-to-1e-6 Ha on the held-out instances with the fallback
     rule: the network's proposal is used only if its exact energy is not
     above the standard guess's.  The target energy of an instance is the
@@ -266,7 +308,7 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
     import time
     insts = []
     for n in ns:
-        insts += [warm_instance(i, n) for i in range(count)]
+        insts += [warm_instance(i, n, family) for i in range(count)]
     rng = np.random.default_rng(WARM_SEED)
     order = rng.permutation(len(insts))
     insts = [insts[i] for i in order]
@@ -278,10 +320,14 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
         plain.append(solve(w, np.zeros(w["P"])))
         if k % 20 == 0:
             log("[warmstart] plain solves %d/%d (%.0f s)" % (k + 1, len(insts), time.time() - t0))
+    targets = [(plain[k][0], plain[k][1]) for k in range(ntr)]
+    if neighbours:
+        targets = consistent_targets(train, plain[:ntr], neighbours=neighbours, log=log)
+    improved = int(sum(targets[k][1] < plain[k][1] - 1e-9 for k in range(ntr)))
     gen_s = time.time() - t0
     model = WarmStart(max(ns), max(w["P"] for w in insts), hidden=hidden)
     t1 = time.time()
-    losses = model.fit(train, [plain[k][0] for k in range(ntr)], epochs=epochs)
+    losses = model.fit(train, [t for t, _ in targets], epochs=epochs)
     log("[warmstart] supervised stage: %d steps, loss %.4f -> %.4f" % (len(losses), losses[0], losses[-1]))
     ehist = finetune_energy(model, train, steps=energy_steps, log=log) if energy_steps else []
     train_s = time.time() - t1
@@ -298,6 +344,8 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
         best = min(E0, E1)
         t_plain, i_plain = time_to(tr0, best + 1e-6)
         t_mlp, i_mlp = time_to(tr1, best + 1e-6)
+        if k % 10 == 0:
+            log("[warmstart] held-out solves %d/%d" % (k + 1, len(held)))
         rows.append(dict(n=w["n"], index=w["index"], E_std_start=float(Es[0]), E_mlp_start=float(Es[1]),
                          used_mlp=use, E_plain=float(E0), E_mlp=float(E1), t_plain=t_plain, t_mlp=t_mlp + choose_s,
                          it_plain=i_plain, it_mlp=i_mlp, t_plain_conv=float(tr0[-1][0]) if len(tr0) else 0.0,
@@ -317,7 +365,8 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
                     mlp_misses=int(sum(np.isinf(r["t_mlp"]) for r in rs)),
                     proposals_used=int(sum(r["used_mlp"] for r in rs)),
                     median_start_gap_std=med("gap_std", sel), median_start_gap_chosen=med("gap_start", sel))
-    summary = dict(ns=list(ns), instances=len(insts), train=ntr, held_out=len(held), generate_s=gen_s,
+    summary = dict(ns=list(ns), family=family, neighbours=neighbours, targets_improved=improved,
+                   instances=len(insts), train=ntr, held_out=len(held), generate_s=gen_s,
                    train_s=train_s, supervised_loss=[float(losses[0]), float(losses[-1])],
                    energy_finetune_mean_E=[ehist[0], ehist[-1]] if ehist else None, all=summ(), rows=rows)
     for n in ns:
@@ -334,10 +383,13 @@ def _main():
     p.add_argument("--count", type=int, default=80, help="instances per orbital count")
     p.add_argument("--epochs", type=int, default=300, help="supervised epochs")
     p.add_argument("--energy-steps", type=int, default=300, help="energy fine-tuning steps")
+    p.add_argument("--family", default="smooth", choices=["smooth", "random"], help="instance family")
+    p.add_argument("--neighbours", type=int, default=2,
+                   help="training targets: also start from this many neighbours' optima (0: plain only)")
     p.add_argument("--out", default="")
     a = p.parse_args()
-    s = run_pipeline(a.ns, a.count, epochs=a.epochs, energy_steps=a.energy_steps,
-                     log=lambda m: print(m, file=sys.stderr, flush=True))
+    s = run_pipeline(a.ns, a.count, epochs=a.epochs, energy_steps=a.energy_steps, family=a.family,
+                     neighbours=a.neighbours, log=lambda m: print(m, file=sys.stderr, flush=True))
     txt = json.dumps(s)
     if a.out:
         with open(a.out, "w") as f:
