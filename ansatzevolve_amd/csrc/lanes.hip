@@ -218,9 +218,12 @@ __device__ __forceinline__ void rot2_br_inplace(double &xp, double &xq, double &
         : "v"(C), "v"(sS));
 }
 
+// cm: the sub-orbits (c = other alpha member + 2 other beta member) whose
+// members exist in this orbit type; the others hold only absent (zero)
+// amplitudes and are skipped (most orbits have 2 or 4 of 16 members)
 template <int G, int H>
 __device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double C, double S, double (&x)[16],
-                                                   double (&l)[16], bool br_on, double &br)
+                                                   double (&l)[16], bool br_on, double &br, unsigned cm)
 {
     constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
 #pragma unroll
@@ -232,6 +235,7 @@ __device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double 
         const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
+            if (c > 0 && !((cm >> c) & 1u)) continue;
             const int oa = c & 1, ob = c >> 1;
             const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
             const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
@@ -246,14 +250,16 @@ __device__ __forceinline__ void orb2_apply_uniform(unsigned w, bool neg, double 
 template <bool BR, int G, int H>
 __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long long *pw, int kk, uint32_t spect,
                                         const double2 (*csL)[kLanes], double *accW, int lane, double (&x)[16],
-                                        double (&l)[16])
+                                        double (&l)[16], unsigned cmask)
 {
     const unsigned w = (unsigned)pw[kk] & 0xfffu;
     if (!w) return;
     const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
     const double2 t = csL[kk][lane];
     double br = 0;
-    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br);
+    // the forward sweep rotates all four sub-orbits (skipping them did not
+    // pay there: the branches cost as much as the rotations they save)
+    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br, BR ? (cmask >> (4 * G)) & 0xfu : 0xfu);
     if (BR) accW[kk * kLanes + lane] += br;
 }
 
@@ -292,7 +298,7 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     constexpr int NO = BR ? 1 : 2;
     for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
         int rA[NO][4], rB[NO][4], type[NO];
-        uint32_t spect[NO];
+        uint32_t spect[NO], cmask[NO];
         bool live[NO], act[NO];
 #pragma unroll
         for (int o = 0; o < NO; o++) {
@@ -310,6 +316,14 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
             if (!copyAll) live[o] = act[o];
             type[o] = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
             spect[o] = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+            // existing sub-orbits per run: run 0 (first groups) varies the second
+            // groups' members, run 1 the first groups' (shape 0 = two members)
+            {
+                const bool a2 = ((fa >> 26) & 3) == 0, b2 = ((fb >> 26) & 3) == 0;
+                const bool a1 = ((fa >> 24) & 3) == 0, b1 = ((fb >> 24) & 3) == 0;
+                cmask[o] = 1u | (a2 ? 2u : 0u) | (b2 ? 4u : 0u) | (a2 && b2 ? 8u : 0u) |
+                           ((1u | (a1 ? 2u : 0u) | (b1 ? 4u : 0u) | (a1 && b1 ? 8u : 0u)) << 4);
+            }
         }
         // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
         double x[NO][16], l[NO][16];
@@ -335,25 +349,25 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
 #pragma unroll
                 for (int kk = 0; kk < K; kk++) {
                     if (kk >= m) break;
-                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o], cmask[o]);
                 }
 #pragma unroll
                 for (int kk = 0; kk < K; kk++) {
                     if (kk < m || kk >= nk) continue;
-                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o], cmask[o]);
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < K; j++) {
                     const int kk = K - 1 - j;
                     if (kk < m || kk >= nk) continue;
-                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                    orb2_op<BR, 1, 1>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o], cmask[o]);
                 }
 #pragma unroll
                 for (int j = 0; j < K; j++) {
                     const int kk = K - 1 - j;
                     if (kk >= m) continue;
-                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o]);
+                    orb2_op<BR, 0, 0>(a, pw, kk, spect[o], csL, accW, lane, x[o], l[o], cmask[o]);
                 }
             }
         }
