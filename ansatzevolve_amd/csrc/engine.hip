@@ -180,7 +180,7 @@ struct EngineImpl {
     int W = 0;
     DBuf<int64_t> d_rowptr;
     DBuf<double> d_h, d_g, d_diag, d_FA, d_FB;  // matrix-free tables
-    // lane-layout matrix-free sigma (k_sigv_sparse + two DGEMMs)
+    // lane-layout matrix-free sigma (k_sigw_sparse or k_sigh_sparse + two DGEMMs)
     bool sigv = false;  // lane-layout split sigma available (n <= 12)
     bool sig1 = false;  // vector-major split sigma available (n <= 14)
     DBuf<double> d_Vaa, d_Vbb;

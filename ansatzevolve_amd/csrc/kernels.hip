@@ -947,123 +947,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Lane-layout sigma, sparse part: for output (alpha, beta), the alpha-beta
-// doubles and the beta-diagonal part of the alpha singles through every
-// alpha single's source row, the alpha-dependent spectator part of the beta
-// singles (identity row), and the diagonal.  One workgroup per (alpha,
-// 8-vector sub-chunk); source rows are staged in LDS two at a time with their
-// signed coefficient rows; thread = (vector v, beta slot).  The LDS layout
-// is fixed so every read uses a compile-time offset:
-//   [coef row 0 | coef row 1 | x row 0 | x row 1], coef row = kSigGS doubles
-//   (+coef[e], -coef[e] at kSigNN + e, zero at 2 kSigNN), x row = kSigMaxNb x
-//   kSigVC doubles.
-__global__ void __launch_bounds__(kSigThreads) k_sigv_sparse(SigvArgs a)
-{
-    extern __shared__ double sm[];
-    const MfArgs &m = a.m;
-    const int n = m.n, nb = m.nb, nn = n * n;
-    // XCD-aware order: the 8 sub-chunks of one alpha row run on one XCD
-    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
-    const int ar = xcd + 8 * (k >> 3), sub = k & 7;
-    if (ar >= m.na) return;
-    const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigVC;
-    const double *__restrict__ x = a.x + coff;
-    double *__restrict__ y = a.y + coff;
-    char *lds = (char *)sm;
-    double *gs0 = sm, *gs1 = sm + kSigGS;
-    double *xs0 = sm + 2 * kSigGS, *xs1 = xs0 + kSigMaxNb * kSigVC;
-    const int tid = threadIdx.x, v = tid & (kSigVC - 1), slot = tid / kSigVC;
-    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
-    const int ka = __popc(alpha), ne = n - ka;
-    const int R = 1 + ka * ne;
-    double acc[kSigBI];
-#pragma unroll
-    for (int i = 0; i < kSigBI; i++) acc[i] = 0;
-    for (int r0 = 0; r0 < R; r0 += 2) {
-        const bool two = r0 + 1 < R;
-        __syncthreads();
-        for (int h = 0; h < 2; h++) {
-            if (h == 1 && !two) break;
-            const int r = r0 + h;
-            double *gs = h ? gs1 : gs0;
-            double *xs = h ? xs1 : xs0;
-            int J;
-            if (r == 0) {
-                // identity row: J_alpha[e] = sum over occupied alpha o of g[oo][e] (r != q only)
-                J = ar;
-                for (int e = tid; e < kSigGS; e += kSigThreads) gs[e] = 0.0;
-                __syncthreads();
-                for (int e = tid; e < nn; e += kSigThreads) {
-                    double vj = 0;
-                    if (e / n != e % n)
-                        for (uint32_t o = alpha; o; o &= o - 1) {
-                            const int xo = __ffs(o) - 1;
-                            vj += thr13(m.g[(size_t)(xo * n + xo) * nn + e]);
-                        }
-                    gs[e] = vj;
-                    gs[kSigNN + e] = -vj;
-                }
-            } else {
-                const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
-                uint32_t oo = alpha, ww = ~alpha & full;
-                for (int i = 0; i < ir; i++) oo &= oo - 1;
-                for (int i = 0; i < iq; i++) ww &= ww - 1;
-                const int rA = __ffs(oo) - 1, qA = __ffs(ww) - 1;
-                J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
-                const bool neg = between_parity(alpha, rA, qA);
-                const double *gb = m.g + (size_t)(rA * n + qA) * nn;
-                for (int e = tid; e < kSigGS; e += kSigThreads) gs[e] = 0.0;
-                __syncthreads();
-                for (int e = tid; e < nn; e += kSigThreads) {
-                    const double cf = neg ? -thr13(gb[e]) : thr13(gb[e]);
-                    gs[e] = cf;
-                    gs[kSigNN + e] = -cf;
-                }
-            }
-            const double *src = x + (long long)J * nb * kLanes;
-            for (int i = tid; i < nb * kSigVC; i += kSigThreads) {
-                const int bb = i / kSigVC, vv = i - bb * kSigVC;
-                xs[i] = src[(long long)bb * kLanes + vv];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kSigBI; i++) {
-            const int beta = slot + kSigSlots * i;
-            if (beta < nb) {
-                const uint4 *te = (const uint4 *)(a.ts + (size_t)beta * a.ntp);
-                double s0 = 0, s1 = 0;
-                const int vo = v * 8;
-                for (int t4 = 0; t4 < a.ntp / 4; t4++) {
-                    const uint4 q = te[t4];
-                    const uint32_t ee[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const uint32_t xb = (ee[u] & 0xffffu) + vo, cb = ee[u] >> 16;
-                        const double c0 = *(const double *)(lds + cb);
-                        const double x0 = *(const double *)(lds + 2 * kSigGS * 8 + xb);
-                        s0 += c0 * x0;
-                        if (two) {
-                            const double c1 = *(const double *)(lds + kSigGS * 8 + cb);
-                            const double x1 = *(const double *)(lds + (2 * kSigGS + kSigMaxNb * kSigVC) * 8 + xb);
-                            s1 += c1 * x1;
-                        }
-                    }
-                }
-                acc[i] += s0 + s1;
-                if (r0 == 0) acc[i] += m.diag[(long long)ar * nb + beta] * xs0[beta * kSigVC + v];
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < kSigBI; i++) {
-        const int beta = slot + kSigSlots * i;
-        if (beta < nb) y[((long long)ar * nb + beta) * kLanes + v] = acc[i];
-    }
-}
-
-// Lane-layout sigma, sparse part, 16 vectors per workgroup: the same terms
-// as k_sigv_sparse with one staged source row (nb x 16 doubles) and the
-// per-entry work cut to one LDS read.  A wave holds four target betas, one
+// doubles and the beta-diagonal part of the alpha singles through every alpha
+// single's source row, the alpha-dependent spectator part of the beta singles
+// (identity row), and the diagonal.  16 vectors per workgroup, one staged
+// source row (nb x 16 doubles), one LDS read per entry.  A wave holds four target betas, one
 // per 16-lane DPP row (lane = vector).  Each lane loads the table entry of
 // its row's target with index (lane mod 16) and that entry's coefficient;
 // the 16 entries of a chunk are then broadcast across the row with
@@ -1467,7 +1354,7 @@ template __global__ void k_sigw_sparse<8>(SigvArgs);
 template __global__ void k_sigw_sparse<9>(SigvArgs);
 
 // Matrix-free sigma, sparse part, vector-major vectors (single vectors and
-// small batches, n <= 14): the same split and entry table as k_sigv_sparse
+// small batches, n <= 14): the same split and entry table as k_sigw_sparse
 // with one vector per workgroup, threads over output beta strings, source
 // rows staged two at a time.  LDS: [coef row 0 | coef row 1 | x row 0 |
 // x row 1], coef row = kSig1GS doubles, x row = kSig1MaxNb doubles.

@@ -290,21 +290,16 @@ __global__ void k_mf_sigma(MfArgs a, const double *__restrict__ x, double *__res
                            double *__restrict__ epart);
 
 // Matrix-free sigma in lane layout (batches, n <= 12, N_down <= 924):
-//   y = sparse part (k_sigv_sparse: alpha-beta doubles, the beta-diagonal
+//   y = sparse part (k_sigw_sparse: alpha-beta doubles, the beta-diagonal
 //       parts of alpha singles, the alpha-dependent parts of beta singles,
 //       the diagonal)
 //     + V_bb applied to every alpha row (DGEMM, beta singles' alpha-free
 //       parts and beta-beta doubles)
 //     + V_aa applied across alpha rows (DGEMM, alpha singles' beta-free parts
 //       and alpha-alpha doubles).
-constexpr int kSigVC = 8;          // vectors per workgroup (a sub-chunk of the 64 lanes)
-constexpr int kSigThreads = 1024;
-constexpr int kSigSlots = kSigThreads / kSigVC;
 constexpr int kSigMaxNb = 924;
-constexpr int kSigBI = (kSigMaxNb + kSigSlots - 1) / kSigSlots;
 constexpr int kSigNN = 144;        // coefficient slots per sign (n <= 12)
-constexpr int kSigGS = 2 * kSigNN + 8;
-constexpr size_t kSigLds = (size_t)(2 * kSigGS + 2 * kSigMaxNb * kSigVC) * sizeof(double);
+constexpr int kSigGS = 2 * kSigNN + 8;  // coefficient row: +coef[e], -coef[e] at kSigNN + e, zeros from 2 kSigNN
 struct SigvArgs {
     MfArgs m;
     const uint32_t *ts;  // nb x ntp beta entries (k_beta_table)
@@ -318,7 +313,6 @@ __global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
 // e + parity * nnoff) << cshift
 __global__ void k_beta_table(MfArgs a, uint32_t *__restrict__ ts, int ntp, int xunit, int nnoff, int cshift,
                              int sepdiag);
-__global__ void k_sigv_sparse(SigvArgs a);  // grid (ceil(na / 8) * 64, chunks), kSigThreads, kSigLds
 // 16 vectors per workgroup, one staged source row; table with xunit 128,
 // cshift 17, kSigWNtp entries per beta and the diagonal separate (SigvArgs
 // ntp = singles per beta rounded up to 4)
