@@ -1025,10 +1025,10 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
     double *gs = sm, *dt = sm + kSigGS, *xs = dt + 128;
     const int tid = threadIdx.x, lane = tid & (kLanes - 1), w = tid / kLanes;
     const int v = lane & 15, trow = w * 4 + (lane >> 4);
+    const int wu = __builtin_amdgcn_readfirstlane(w);
     const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
     const int ka = __popc(alpha), ne = n - ka;
     const int R = 1 + ka * ne;
-    const int nx = nb * kSigWVC;
     const uint32_t gsb = (uint32_t)(size_t)(lds_f64 *)gs;
     const uint32_t lb = (uint32_t)(size_t)(lds_f64 *)xs + v * 8;
     constexpr uint32_t kPad = (uint32_t)(2 * kSigNN * 8) << 17;  // zero coefficient, source beta 0
@@ -1050,8 +1050,19 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
             J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
         }
         __syncthreads();  // every read of the previous row is done
-        const double *src = x + (long long)J * nb * kLanes;
-        for (int i = tid; i < nx; i += kSigWThreads) xs[i] = src[(long long)(i >> 4) * kLanes + (i & 15)];
+        {
+            // the source row into LDS by asynchronous global-to-LDS copies, all
+            // issued before any wait: 8 betas (1 KB) per wave instruction,
+            // lane l -> beta 8c + l/8, vectors 2 (l mod 8) and 2 (l mod 8) + 1
+            // (a load/store loop waited for each load in turn)
+            const double *src = x + (long long)J * nb * kLanes + (lane & 7) * 2;
+            for (int c = wu; c * 8 < nb; c += kSigWThreads / kLanes) {
+                const int beta = c * 8 + (lane >> 3);
+                __builtin_amdgcn_global_load_lds((const void *)(src + (long long)(beta < nb ? beta : 0) * kLanes),
+                                                 (__attribute__((address_space(3))) void *)(xs + c * 128), 16, 0,
+                                                 0);
+            }
+        }
         if (r == 0) {
             // identity row: J_alpha[e] = sum over occupied alpha o of g[oo][e] (r != q only)
             for (int e = tid; e < nn; e += kSigWThreads) {
@@ -1083,13 +1094,21 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
             if (r > 0) {
                 const bool neg = between_parity(alpha, rA, qA);
                 const double *gb = m.g + (size_t)(rA * n + qA) * nn;
+                // the six loads first, then the sums (a conditional load per
+                // term waited for each in turn)
+                double gv[6];
+#pragma unroll
                 for (int b = 0; b < 6; b++) {
                     const int q = b + 6 * hi;
-                    if (q < n && ((t >> b) & 1)) d += neg ? -thr13(gb[q * n + q]) : thr13(gb[q * n + q]);
+                    gv[b] = gb[(q < n ? q : 0) * (n + 1)];
                 }
+#pragma unroll
+                for (int b = 0; b < 6; b++)
+                    if (b + 6 * hi < n && ((t >> b) & 1)) d += neg ? -thr13(gv[b]) : thr13(gv[b]);
             }
             dt[tid] = d;
         }
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's row copies have landed
         __syncthreads();
         // table entries of the next pass are loaded while this pass sums
         // (32-bit element offsets from the table base); the empty asm keeps
@@ -1102,7 +1121,9 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigw_sparse(SigvArgs a)
         for (int h = 0; h < kSigWH; h++) ent[h] = trow < nb ? a.ts[tof + 16 * h] : kPad;
 #pragma unroll
         for (int i = 0; i < kSigWNI; i++) {
-            if (64 * i >= nb) break;
+            // no target of this wave in this pass (or later ones): the partial
+            // last pass leaves its LDS bandwidth to the waves that have some
+            if (64 * i + 4 * wu >= nb) break;
             const int tgt = trow + 64 * i, tn = tgt + 64;
 #pragma unroll
             for (int h = 0; h < kSigWH; h++)

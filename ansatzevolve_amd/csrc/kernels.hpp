@@ -322,7 +322,8 @@ constexpr int kSigWNI = (kSigMaxNb + 63) / 64;  // target passes (64 betas per p
 constexpr int kSigWPre = (kSigMaxNb * kSigWVC + kSigWThreads - 1) / kSigWThreads;
 constexpr int kSigWH = 3;                       // entry chunks of 16 (n <= 12: at most 42 entries)
 constexpr int kSigWNtp = 16 * kSigWH;
-constexpr size_t kSigWLds = (size_t)(kSigGS + 128 + kSigMaxNb * kSigWVC) * sizeof(double);
+constexpr int kSigWSlots = (kSigMaxNb + 7) / 8 * 8;  // staged betas (copied 8 at a time)
+constexpr size_t kSigWLds = (size_t)(kSigGS + 128 + kSigWSlots * kSigWVC) * sizeof(double);
 // Two-half variant (n = 2 kd): the source row split by the occupation of
 // beta orbital kSigHOrb, halves double-buffered in LDS.  Target slots: the
 // strings with that orbital occupied (half A, nA of them, padded to TA, a
