@@ -11,7 +11,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libansatzsynth.so")
 ARCH = os.environ.get("AE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "lanes.hip", "engine.hip", "mlp.hip", "multi.hip"]
+HIP_SOURCES = ["kernels.hip", "lanes.hip", "sigg.hip", "engine.hip", "mlp.hip", "multi.hip"]
 # per-file device flags: the lane sweeps keep their orbit arrays as separate
 # registers (no promotion of private arrays to one vector register tuple)
 HIP_FILE_FLAGS = {"lanes.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"]}
@@ -34,6 +34,9 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
+    # the grouped-target sigma programs are generated (sigg.py)
+    from . import sigg
+    sigg.emit()
     jobs = []
     objs = []
     for s in HIP_SOURCES:

@@ -26,7 +26,7 @@ void log_msg(const char *fmt, ...);
 namespace {
 
 struct Candidate {
-    int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves;
+    int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves, grouped;
 };
 
 std::string shape_key(const Engine &eng, int nvec, int nops)
@@ -41,7 +41,7 @@ std::string cand_json(const Candidate &c)
     std::ostringstream o;
     o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
       << ", \"groups\": " << c.groups << ", \"groups_fwd\": " << c.groupsFwd << ", \"checkpoint\": " << c.ckpt
-      << ", \"sigma_halves\": " << c.halves << "}";
+      << ", \"sigma_halves\": " << c.halves << ", \"sigma_grouped\": " << c.grouped << "}";
     return o.str();
 }
 
@@ -58,8 +58,9 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         Candidate t{};
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
                         "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d, "
-                        "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d}",
-                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves) == 7) {
+                        "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d, \"sigma_grouped\": %d}",
+                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves,
+                        &t.grouped) == 8) {
             c = t;
             found = true;
         }
@@ -72,7 +73,7 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
     return eng.set_option(kOptSweep, c.sweep, err) && eng.set_option(kOptFuseDepth, c.depth, err) &&
            eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneGroupsFwd, c.groupsFwd, err) &&
            eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptCheckpoint, c.ckpt, err) &&
-           eng.set_option(kOptSigmaHalves, c.halves, err) &&
+           eng.set_option(kOptSigmaHalves, c.halves, err) && eng.set_option(kOptSigmaGrouped, c.grouped, err) &&
            eng.set_option(kOptGraph, 1, err);
 }
 
@@ -103,29 +104,31 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
     const bool sz = eng.mode() == kSz;
     const int sweeps[3] = {kSweepOrbit, kSweepOrbitLds, kSweepTile};
     const int depths[3] = {0, 2, 1};
+    const int gr = eng.option(kOptSigmaGroupedAvailable) == 1 ? 1 : 0;  // grouped sigma where it applies
     for (int lanes = 0; lanes < (nvec > 1 ? 2 : 1); lanes++)
         for (int s : sweeps) {
             if (!sz && s != kSweepOrbit) continue;          // other sectors sweep one operator per pass
             if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
             for (int d : depths) {
                 if (!sz && d != 0) continue;
-                cands.push_back({s, d, lanes ? 1 : 0, 1, 1, 0, 0});
+                cands.push_back({s, d, lanes ? 1 : 0, 1, 1, 0, 0, gr});
                 // the lane layout can also fuse two disjoint groups per pass
                 // (forward only, or both sweeps), keep the forward states for
-                // the gradient, and use the two-half sigma where it applies
+                // the gradient, and use the other sigma variants where they apply
                 if (lanes && sz && s == kSweepOrbit && d == 0) {
-                    cands.push_back({s, d, 1, 1, 2, 0, 0});
-                    cands.push_back({s, d, 1, 2, 2, 0, 0});
-                    cands.push_back({s, d, 1, 1, 1, 1, 0});
-                    cands.push_back({s, d, 1, 2, 2, 1, 0});
-                    if (eng.option(kOptSigmaHalvesAvailable) == 1) cands.push_back({s, d, 1, 2, 2, 1, 1});
+                    cands.push_back({s, d, 1, 1, 2, 0, 0, gr});
+                    cands.push_back({s, d, 1, 2, 2, 0, 0, gr});
+                    cands.push_back({s, d, 1, 1, 1, 1, 0, gr});
+                    cands.push_back({s, d, 1, 2, 2, 1, 0, gr});
+                    if (gr) cands.push_back({s, d, 1, 2, 2, 1, 0, 0});
+                    if (eng.option(kOptSigmaHalvesAvailable) == 1) cands.push_back({s, d, 1, 2, 2, 1, 1, 0});
                 }
             }
         }
 
     const Candidate saved{eng.option(kOptSweep),         eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
                           eng.option(kOptLaneGroups),    eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint),
-                          eng.option(kOptSigmaHalves)};
+                          eng.option(kOptSigmaHalves),   eng.option(kOptSigmaGrouped)};
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";

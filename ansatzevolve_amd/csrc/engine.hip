@@ -190,6 +190,8 @@ struct EngineImpl {
     // two-half lane sigma (k_sigh_sparse, n = 2 kd): target-slot table, slot ranks
     bool sigh = false;     // available
     int sighOpt = 0;       // kOptSigmaHalves (autotuned; the one-row kernel is faster at 12 orbitals)
+    bool sigg = false;     // grouped-target sigma available (12 orbitals, 6 beta electrons)
+    int siggOpt = 1;       // kOptSigmaGrouped
     int sighS = 0, sighX = 0, sighNA = 0, sighNB = 0, sighTA = 0, sighT = 0;
     DBuf<uint32_t> d_tsh;
     DBuf<int> d_hl;
@@ -1069,7 +1071,9 @@ struct EngineImpl {
         s.y = y;
         s.cstride = N * kLanes;
         const unsigned gx = (unsigned)((na + 7) / 8 * 32);
-        if (sigh && sighOpt) {
+        if (sigg && siggOpt) {
+            hipLaunchKernelGGL(k_sigg_sparse<6>, dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
+        } else if (sigh && sighOpt) {
             SighArgs h;
             h.m = s.m;
             h.ts = d_tsh.p;
@@ -1556,6 +1560,10 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
                 HCHK(hipFuncSetAttribute((const void *)sigw_kernel(e.ntp16 / 4),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSigWLds));
             e.sigh = lanesOk && n == 2 * kd && sigh_kernel((kd - 1) * (n - kd), n - kd) != nullptr;
+            e.sigg = lanesOk && n == 12 && kd == 6 && e.nb == 924;
+            if (e.sigg)
+                HCHK(hipFuncSetAttribute((const void *)k_sigg_sparse<6>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)kSigWLds));
             if (e.sigh && !e.build_sigh_tables(err)) return false;
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));
@@ -1977,6 +1985,15 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.sighOpt = value;
         e.drop_graphs();
         return true;
+    case kOptSigmaGrouped:
+        if (value < 0 || value > 1) {
+            err = "sigma grouped option must be 0 or 1";
+            return false;
+        }
+        if (value == e.siggOpt) return true;
+        e.siggOpt = value;
+        e.drop_graphs();
+        return true;
     case kOptCheckpoint:
         if (value < 0 || value > 1) {
             err = "checkpoint option must be 0 or 1";
@@ -1992,6 +2009,7 @@ bool Engine::set_option(int key, int value, std::string &err)
     case kOptLaneSegments:
     case kOptCheckpointActive:
     case kOptSigmaHalvesAvailable:
+    case kOptSigmaGroupedAvailable:
         err = "read-only engine property";
         return false;
     default:
@@ -2017,6 +2035,8 @@ int Engine::option(int key) const
     case kOptCheckpointActive: return e.ckOn ? 1 : 0;
     case kOptSigmaHalves: return e.sighOpt;
     case kOptSigmaHalvesAvailable: return e.sigh ? 1 : 0;
+    case kOptSigmaGrouped: return e.siggOpt;
+    case kOptSigmaGroupedAvailable: return e.sigg ? 1 : 0;
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

@@ -42,6 +42,9 @@ enum EngineOption : int {
     kOptSigmaHalves = 12,       // 1: lane-layout sigma with double-buffered source halves when available,
                                 // 0 (default): the one-row kernel
     kOptSigmaHalvesAvailable = 13,  // read only: 1 when the two-half sigma applies (n = 2 kd <= 12)
+    kOptSigmaGrouped = 14,      // 1 (default): lane-layout sigma with targets grouped by their upper six
+                                // beta orbitals when available (takes precedence over kOptSigmaHalves)
+    kOptSigmaGroupedAvailable = 15,  // read only: 1 when the grouped sigma applies (12 orbitals, 6 beta)
 };
 
 class Engine {

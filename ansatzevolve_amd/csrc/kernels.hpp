@@ -355,6 +355,10 @@ __global__ void k_sigh_sparse(SighArgs a);  // grid (ceil(na / 8) * 32, chunks),
 // NG = groups of 4 single entries per beta (1..9; a.ntp = 4 NG)
 template <int NG>
 __global__ void k_sigw_sparse(SigvArgs a);  // grid (ceil(na / 8) * 32, chunks), kSigWThreads, kSigWLds
+// 12 orbitals, K beta electrons (K = 6): targets grouped by their upper
+// six orbitals, one group per 16-lane row (sigg.hip); same grid, block and LDS
+template <int K>
+__global__ void k_sigg_sparse(SigvArgs a);
 __global__ void k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N,
                             double *__restrict__ part);  // grid (G, chunks)
 

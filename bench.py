@@ -145,10 +145,10 @@ def main():
         tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
                                 c["choice"]["groups"], c["choice"]["groups_fwd"], c["choice"]["checkpoint"],
-                                c["choice"]["sigma_halves"], round(c["ms"], 3)]
+                                c["choice"]["sigma_halves"], c["choice"]["sigma_grouped"], round(c["ms"], 3)]
                                for c in rep.get("candidates", [])],
                 "candidate_fields": ["sweep", "depth", "lane_min", "groups", "groups_fwd", "checkpoint",
-                                     "sigma_halves", "ms"]}
+                                     "sigma_halves", "sigma_grouped", "ms"]}
     if args.lane_groups:
         a.set_option(a.OPT_LANE_GROUPS, args.lane_groups)
         a.set_option(a.OPT_LANE_GROUPS_FWD, args.lane_groups)
@@ -156,7 +156,7 @@ def main():
             "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH),
             "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_groups_fwd": a.get_option(a.OPT_LANE_GROUPS_FWD),
             "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS), "checkpoint": a.get_option(a.OPT_CHECKPOINT),
-            "sigma_halves": a.get_option(a.OPT_SIGMA_HALVES)}
+            "sigma_halves": a.get_option(a.OPT_SIGMA_HALVES), "sigma_grouped": a.get_option(a.OPT_SIGMA_GROUPED)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
@@ -234,8 +234,9 @@ def main():
         # each per vector; the DGEMMs cover the same-spin rows densely
         per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
         halves = a.get_option(a.OPT_SIGMA_HALVES) and a.get_option(a.OPT_SIGMA_HALVES_AVAILABLE)
-        rooflines["hamiltonian"] = fp64("%s (lane-layout sigma, %d vectors)"
-                                        % ("k_sigh_sparse" if halves else "k_sigw_sparse", args.nvec),
+        grouped = a.get_option(a.OPT_SIGMA_GROUPED) and a.get_option(a.OPT_SIGMA_GROUPED_AVAILABLE)
+        sig_name = "k_sigg_sparse" if grouped else "k_sigh_sparse" if halves else "k_sigw_sparse"
+        rooflines["hamiltonian"] = fp64("%s (lane-layout sigma, %d vectors)" % (sig_name, args.nvec),
                                         2.0 * N * per_el * args.nvec, ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
                                         "FP64 vector-ALU/LDS work priced against the FP64 peak")
         na = math.comb(n_, ku)

@@ -197,7 +197,13 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     split in two double-buffered halves when it applies, 0 (default) the
  *     one-row variant;
  *  13 (read only) 1 when the two-half variant applies (12 or fewer
- *     orbitals, as many beta electrons as empty beta orbitals).
+ *     orbitals, as many beta electrons as empty beta orbitals);
+ *  14 lane-layout matrix-free sigma: 1 (default) the variant whose 16-lane
+ *     rows each hold all targets with one occupation of the upper six beta
+ *     orbitals, when it applies (it then takes precedence over key 12),
+ *     0 the variants of key 12;
+ *  15 (read only) 1 when the grouped variant applies (12 orbitals, 6 beta
+ *     electrons).
  * Changing keys 0, 3, 4, 7, 9 or 10 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

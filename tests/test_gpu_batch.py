@@ -149,6 +149,25 @@ def test_sigma_halves_match_one_row(gpu, n):
         assert abs(E1[69] - Es) <= SAME and np.max(np.abs(G1[69] - gs)) <= SAME
 
 
+def test_sigma_grouped_matches_one_row(gpu):
+    """12 orbitals, 6 beta electrons: the lane-layout sigma whose 16-lane rows
+    hold all targets with one occupation of the upper six beta orbitals (one
+    LDS read feeding several FMAs) against the one-row variant, on one full
+    and one partial chunk of 64 vectors, and against the single-vector path."""
+    w, thetas = _tups_case(12, 1, 1212, 65)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED_AVAILABLE) == 1
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED) == 1
+        E1, G1 = a.energy_gradients(thetas)
+        a.set_option(AnsatzSynth.OPT_SIGMA_GROUPED, 0)
+        E0, G0 = a.energy_gradients(thetas)
+        assert np.max(np.abs(E1 - E0)) <= SAME and np.max(np.abs(G1 - G0)) <= SAME
+        for v in (0, 64):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E1[v] - Es) <= SAME and np.max(np.abs(G1[v] - gs)) <= SAME
+
+
 def test_lane_layout_number_sector(gpu):
     """Operators that move an electron between spins: the particle-number
     sector (no S_z tiles), explicit triplet Hamiltonian."""
