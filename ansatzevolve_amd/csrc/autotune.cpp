@@ -120,6 +120,7 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
                     cands.push_back({s, d, 1, 2, 2, 0, 0, gr});
                     cands.push_back({s, d, 1, 1, 1, 1, 0, gr});
                     cands.push_back({s, d, 1, 2, 2, 1, 0, gr});
+                    cands.push_back({s, d, 1, 3, 3, 1, 0, gr});
                     if (gr) cands.push_back({s, d, 1, 2, 2, 1, 0, 0});
                     if (eng.option(kOptSigmaHalvesAvailable) == 1) cands.push_back({s, d, 1, 2, 2, 1, 1, 0});
                 }

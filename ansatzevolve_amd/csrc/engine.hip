@@ -78,9 +78,10 @@ struct TileSet {
 // Classes of one spin for a pair of disjoint bit groups (g1, g2): strings with
 // the same bits outside the groups and the same electron count in each group.
 struct GroupSet {
-    uint32_t key1 = 0, key2 = 0;
+    uint32_t key1 = 0, key2 = 0, key3 = 0;  // key3: third group (three-group sets), else 0
     bool beta = false;
     int ncls = 0;
+    int n2 = 0;  // three-group sets: the first n2 classes have two group-3 members
     DBuf<int4> ranks;
     DBuf<uint32_t> info;
 };
@@ -92,10 +93,13 @@ struct Seg {
     long long orbits = 0;      // alpha classes x beta classes
     uint32_t ua = 0, ub = 0;   // orbit segment: its alpha / beta bits (within one spin string)
     int progOff = -1;          // orbit segment: offset of its program (two-group: of its (g, h) list)
-    int two = 0;               // lane layout: two-group orbit segment (k_orb2v)
-    int gA = -1, gB = -1;      // two-group segment: alpha / beta group sets
-    uint32_t grA[2] = {0, 0}, grB[2] = {0, 0};
+    int two = 0;               // lane layout: two- (or three-) group orbit segment (k_orb2v, k_orb3v)
+    int three = 0;             // lane layout: three-group orbit segment (k_orb3v)
+    int gA = -1, gB = -1;      // two-/three-group segment: alpha / beta group sets
+    uint32_t grA[3] = {0, 0, 0}, grB[3] = {0, 0, 0};
     int prog2Off = -1;
+    int prog3Off = -1;         // three-group segment: offset of its run-2 words
+    int split2 = 0;            // three-group segment: operators before run 2
     int grid = 1;  // lane layout: blocks of its launches
 };
 
@@ -158,12 +162,13 @@ struct EngineImpl {
     struct LaneList {
         std::vector<Seg> segs;
         DBuf<uint32_t> lprog;             // one-group orbit programs
-        DBuf<unsigned long long> prog2;   // two-group orbit programs
+        DBuf<unsigned long long> prog2;   // two-group orbit programs (and run 0/1 words of three-group ones)
+        DBuf<unsigned long long> prog3;   // three-group segments: run-2 words
     };
     LaneList laneA, laneF;
     std::vector<Seg> &lsegs = laneA.segs;
     std::vector<std::unique_ptr<GroupSet>> gsets;
-    int maxGroups = 1;   // adjoint: groups per spin of a lane-layout orbit segment (1 or 2; tuned)
+    int maxGroups = 1;   // adjoint: groups per spin of a lane-layout orbit segment (1 to 3; tuned)
     int maxGroupsF = 1;  // forward
     std::vector<std::unique_ptr<TileSet>> tilesets;
     int slots = 1, Ggen = 1;
@@ -314,7 +319,8 @@ struct EngineImpl {
     int groupset_for(uint32_t g1, uint32_t g2, bool beta, std::string &err)
     {
         for (size_t i = 0; i < gsets.size(); i++)
-            if (gsets[i]->key1 == g1 && gsets[i]->key2 == g2 && gsets[i]->beta == beta) return (int)i;
+            if (gsets[i]->key1 == g1 && gsets[i]->key2 == g2 && gsets[i]->key3 == 0 && gsets[i]->beta == beta)
+                return (int)i;
         auto t = std::make_unique<GroupSet>();
         t->key1 = g1;
         t->key2 = g2;
@@ -353,6 +359,69 @@ struct EngineImpl {
         return (int)gsets.size() - 1;
     }
 
+    // Class set of a three-group segment (k_orb3v): strings agreeing outside
+    // g1 | g2 | g3 with the same electron count in each group; members by
+    // (m1, m2, m3), m = 0 the group's lower bit (or 0 / 2 electrons), 1 the
+    // upper; two int4 per class (m3 = 0 then 1, each indexed m1 + 2 m2).
+    // Classes with one electron in g3 (two group-3 members) come first.
+    int groupset3_for(uint32_t g1, uint32_t g2, uint32_t g3, bool beta, std::string &err)
+    {
+        for (size_t i = 0; i < gsets.size(); i++)
+            if (gsets[i]->key1 == g1 && gsets[i]->key2 == g2 && gsets[i]->key3 == g3 && gsets[i]->beta == beta)
+                return (int)i;
+        auto t = std::make_unique<GroupSet>();
+        t->key1 = g1;
+        t->key2 = g2;
+        t->key3 = g3;
+        t->beta = beta;
+        const std::vector<uint32_t> &un = beta ? unB : unA;
+        const uint32_t U = g1 | g2 | g3;
+        auto shape = [](int c) { return c == 1 ? 0u : (c == 0 ? 1u : 2u); };
+        std::map<std::array<uint32_t, 4>, int> cls;
+        std::vector<std::array<int, 8>> mem;
+        std::vector<uint32_t> info;
+        for (int rk = 0; rk < (int)un.size(); rk++) {
+            const uint32_t str = un[rk];
+            const int c1 = __builtin_popcount(str & g1), c2 = __builtin_popcount(str & g2),
+                      c3 = __builtin_popcount(str & g3);
+            const std::array<uint32_t, 4> key = {str & ~U, (uint32_t)c1, (uint32_t)c2, (uint32_t)c3};
+            auto it = cls.find(key);
+            int ci;
+            if (it == cls.end()) {
+                ci = (int)mem.size();
+                cls[key] = ci;
+                mem.push_back({-1, -1, -1, -1, -1, -1, -1, -1});
+                info.push_back((str & ~U) | (shape(c1) << 24) | (shape(c2) << 26) | (shape(c3) << 28));
+            } else {
+                ci = it->second;
+            }
+            const uint32_t lo1 = g1 & (~g1 + 1), lo2 = g2 & (~g2 + 1), lo3 = g3 & (~g3 + 1);
+            const int m1 = (c1 == 1 && !(str & lo1)) ? 1 : 0, m2 = (c2 == 1 && !(str & lo2)) ? 1 : 0,
+                      m3 = (c3 == 1 && !(str & lo3)) ? 1 : 0;
+            mem[ci][4 * m3 + m1 + 2 * m2] = rk;
+        }
+        // classes with two group-3 members first (stable within each part)
+        std::vector<int> order;
+        for (int pass = 0; pass < 2; pass++)
+            for (int ci = 0; ci < (int)mem.size(); ci++)
+                if ((((info[ci] >> 28) & 3) == 0) == (pass == 0)) order.push_back(ci);
+        std::vector<int4> r;
+        std::vector<uint32_t> inf;
+        for (int ci : order) {
+            const std::array<int, 8> &m = mem[ci];
+            r.push_back(make_int4(m[0], m[1], m[2], m[3]));
+            r.push_back(make_int4(m[4], m[5], m[6], m[7]));
+            inf.push_back(info[ci]);
+            if (((info[ci] >> 28) & 3) == 0) t->n2++;
+        }
+        t->ncls = (int)inf.size();
+        if (!t->ranks.ensure(r.size(), err) || !t->info.ensure(inf.size(), err)) return -1;
+        HCHK(hipMemcpy(t->ranks.p, r.data(), r.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(t->info.p, inf.data(), inf.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        gsets.push_back(std::move(t));
+        return (int)gsets.size() - 1;
+    }
+
     // Lane-layout segmentation (S_z sector): two-group orbit segments where
     // operators pair up on disjoint spatial pairs, one-group orbit segments
     // otherwise, and the vector-major segments' LDS tiles and generic
@@ -373,7 +442,8 @@ struct EngineImpl {
         auto assign = [&](std::vector<Seg> &l, bool adj) {
             int mx = 1;
             for (Seg &sg : l) {
-                const void *fn = sg.two ? (adj ? (const void *)k_orb2v<true> : (const void *)k_orb2v<false>)
+                const void *fn = sg.three ? (adj ? (const void *)k_orb3v<true> : (const void *)k_orb3v<false>)
+                                 : sg.two ? (adj ? (const void *)k_orb2v<true> : (const void *)k_orb2v<false>)
                                  : sg.orbA >= 0 ? (adj ? (const void *)k_orbv<true> : (const void *)k_orbv<false>)
                                                 : (adj ? (const void *)k_opv<true> : (const void *)k_opv<false>);
                 // waves with work: one orbit (or one 64-determinant tile) each
@@ -406,14 +476,15 @@ struct EngineImpl {
         lsegs.clear();
         if (mode != kSz || sweepKind == kSweepTile) {
             lsegs = segs;
-            for (Seg &sg : lsegs) sg.two = 0;
+            for (Seg &sg : lsegs) sg.two = sg.three = 0;
             return build_orbit_programs(lsegs, L.lprog, err);
         }
         const uint32_t amask = (1u << n) - 1;
         auto bitsA = [&](int kk) { return ((hops[kk].cre | hops[kk].ann) >> n) & amask; };
         auto bitsB = [&](int kk) { return (hops[kk].cre | hops[kk].ann) & amask; };
         const int maxOps = std::min(kOrb2MaxOps, fuseDepth > 0 ? fuseDepth : kOrb2MaxOps);
-        std::vector<unsigned long long> prog2;
+        const int maxOps3 = std::min(kOrb3MaxOps, fuseDepth > 0 ? fuseDepth : kOrb3MaxOps);
+        std::vector<unsigned long long> prog2, prog3;
         // a one-group run from k0: operators whose alpha bits are empty or one
         // fixed 2-bit group, and likewise beta (a TUPS unit)
         auto run = [&](int k0, uint32_t &ua, uint32_t &ub) {
@@ -439,7 +510,8 @@ struct EngineImpl {
         static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
         // programs of a two-group segment [k, k2) split at k1: false if some
         // operator's pair pattern differs between its sub-orbits
-        auto two_group_prog = [&](const Seg &sg, int k, int k1, int k2, std::vector<unsigned long long> &out) {
+        auto two_group_prog = [&](const Seg &sg, int k, int k1, int k2, std::vector<unsigned long long> &out,
+                                  int K) {
             out.clear();
             for (int type = 0; type < 81; type++) {
                 const int sa1 = type % 3, sa2 = (type / 3) % 3, sb1 = (type / 9) % 3, sb2 = (type / 27) % 3;
@@ -455,7 +527,7 @@ struct EngineImpl {
                     const uint32_t b = pat(sg.grB[0], sb1, mb1) | pat(sg.grB[1], sb2, mb2);
                     return (a << n) | b;
                 };
-                for (int kk = 0; kk < kOrb2MaxOps; kk++) {
+                for (int kk = 0; kk < K; kk++) {
                     unsigned long long w = 0;
                     if (k + kk < k2) {
                         const DevOp &o = hops[k + kk];
@@ -495,7 +567,45 @@ struct EngineImpl {
             }
             return true;
         };
-        std::vector<unsigned long long> words;
+        // run-2 words of a three-group segment per group-3 type (shapeA3 + 3
+        // shapeB3) over the four group-3 slots (a3 + 2 b3): false if a slot is
+        // in two pairs, the pairs differ in their slot xor, or something acts
+        // where both group-3 classes have one member (k_orb3v skips run 2 there)
+        auto three_group_prog3 = [&](const Seg &sg, int k, int k2, int k3, std::vector<unsigned long long> &out) {
+            out.clear();
+            const uint32_t g3a = sg.grA[2], g3b = sg.grB[2];
+            for (int type3 = 0; type3 < 9; type3++) {
+                const int sa = type3 % 3, sb = type3 / 3;
+                auto valid = [&](int slot) {
+                    return ((slot & 1) == 0 || (sa == 0 && g3a)) && ((slot >> 1) == 0 || (sb == 0 && g3b));
+                };
+                auto str = [&](int slot) { return (pat(g3a, sa, slot & 1) << n) | pat(g3b, sb, slot >> 1); };
+                for (int kk = 0; kk < kOrb3MaxOps; kk++) {
+                    unsigned long long w = 0;
+                    if (k + kk >= k2 && k + kk < k3) {
+                        const DevOp &o = hops[k + kk];
+                        int used = 0, dx = -1;
+                        for (int r = 0; r < 6; r++) {
+                            const int p = PP[r], q = QQ[r];
+                            if (!valid(p) || !valid(q)) continue;
+                            uint32_t b2 = 0;
+                            const int sgn = host_pair_sign(o, str(p), b2);
+                            if (sgn == 0 || b2 != str(q)) continue;
+                            if (((used >> p) & 1) || ((used >> q) & 1)) return false;
+                            if (dx >= 0 && dx != (p ^ q)) return false;
+                            used |= (1 << p) | (1 << q);
+                            dx = p ^ q;
+                            w |= 1ull << r;
+                            if (sgn < 0) w |= 1ull << (6 + r);
+                        }
+                        if (w && sa != 0 && sb != 0) return false;
+                    }
+                    out.push_back(w);
+                }
+            }
+            return true;
+        };
+        std::vector<unsigned long long> words, words3;
         int k = 0;
         while (k < M) {
             uint32_t ua = 0, ub = 0;
@@ -507,6 +617,50 @@ struct EngineImpl {
                 k++;
                 continue;
             }
+            if (maxG >= 3 && k1 < M) {
+                uint32_t ua2 = 0, ub2 = 0, ua3 = 0, ub3 = 0;
+                const int k2 = run(k1, ua2, ub2);
+                const int k3 = (k2 > k1 && k2 < M) ? run(k2, ua3, ub3) : k2;
+                const uint32_t A12 = ua | ua2, B12 = ub | ub2;
+                if (k2 > k1 && k3 > k2 && k3 - k <= maxOps3 && !(ua & ua2) && !(ub & ub2) && !(A12 & ua3) &&
+                    !(B12 & ub3)) {
+                    // no operator's sign mask may cover another group's orbitals:
+                    // the run-0/1 words hold for every group-3 member and the run-2
+                    // words for every group-1/2 member
+                    bool masks = true;
+                    for (int kk = k; kk < k3; kk++) {
+                        const uint32_t other = kk < k2 ? ((ua3 << n) | ub3) : ((A12 << n) | B12);
+                        if (hops[kk].mask & other) masks = false;
+                    }
+                    Seg sg{k, k3, -1};
+                    sg.two = 1;
+                    sg.three = 1;
+                    sg.grA[0] = ua;
+                    sg.grA[1] = ua2;
+                    sg.grA[2] = ua3;
+                    sg.grB[0] = ub;
+                    sg.grB[1] = ub2;
+                    sg.grB[2] = ub3;
+                    if (masks && two_group_prog(sg, k, k1, k2, words, kOrb3MaxOps) &&
+                        three_group_prog3(sg, k, k2, k3, words3)) {
+                        sg.gA = groupset3_for(ua, ua2, ua3, false, err);
+                        sg.gB = groupset3_for(ub, ub2, ub3, true, err);
+                        if (sg.gA < 0 || sg.gB < 0) return false;
+                        const GroupSet &GA = *gsets[sg.gA], &GB = *gsets[sg.gB];
+                        const long long a2 = GA.n2, a1 = GA.ncls - GA.n2, b2 = GB.n2, b1 = GB.ncls - GB.n2;
+                        sg.orbits = 4 * a2 * b2 + 2 * a2 * b1 + 2 * a1 * b2 + a1 * b1;  // work items
+                        sg.prog2Off = (int)prog2.size();
+                        prog2.insert(prog2.end(), words.begin(), words.end());
+                        sg.prog3Off = (int)prog3.size();
+                        prog3.insert(prog3.end(), words3.begin(), words3.end());
+                        sg.progOff = k1 - k;
+                        sg.split2 = k2 - k;
+                        lsegs.push_back(sg);
+                        k = k3;
+                        continue;
+                    }
+                }
+            }
             if (maxG >= 2 && k1 < M) {
                 uint32_t ua2 = 0, ub2 = 0;
                 const int k2 = run(k1, ua2, ub2);
@@ -517,7 +671,7 @@ struct EngineImpl {
                     sg.grA[1] = ua2;
                     sg.grB[0] = ub;
                     sg.grB[1] = ub2;
-                    if (two_group_prog(sg, k, k1, k2, words)) {
+                    if (two_group_prog(sg, k, k1, k2, words, kOrb2MaxOps)) {
                         sg.gA = groupset_for(ua, ua2, false, err);
                         sg.gB = groupset_for(ub, ub2, true, err);
                         if (sg.gA < 0 || sg.gB < 0) return false;
@@ -542,8 +696,10 @@ struct EngineImpl {
             k = k1;
         }
         if (prog2.empty()) prog2.push_back(0);
-        if (!L.prog2.ensure(prog2.size(), err)) return false;
+        if (prog3.empty()) prog3.push_back(0);
+        if (!L.prog2.ensure(prog2.size(), err) || !L.prog3.ensure(prog3.size(), err)) return false;
         HCHK(hipMemcpy(L.prog2.p, prog2.data(), prog2.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(L.prog3.p, prog3.data(), prog3.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
         // one-group segments of the lane list get their own programs
         std::vector<uint32_t> save(lsegs.size());
         for (size_t i = 0; i < lsegs.size(); i++) save[i] = (uint32_t)lsegs[i].progOff;
@@ -649,6 +805,35 @@ struct EngineImpl {
         a.reverse = reverse ? 1 : 0;
         a.prog = L.prog2.p + sg.prog2Off;
         a.split = sg.progOff;
+        return a;
+    }
+
+    Orb3Args orb3args(const LaneList &L, const Seg &sg) const
+    {
+        const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
+        Orb3Args a;
+        a.n = n;
+        a.nb = nb;
+        a.aRanks = A.ranks.p;
+        a.bRanks = B.ranks.p;
+        a.aInfo = A.info.p;
+        a.bInfo = B.info.p;
+        a.nA = A.ncls;
+        a.nB = B.ncls;
+        a.nA2 = A.n2;
+        a.nB2 = B.n2;
+        const long long a1 = A.ncls - A.n2, b1 = B.ncls - B.n2;
+        a.w[0] = 4LL * A.n2 * B.n2;
+        a.w[1] = a.w[0] + 2LL * A.n2 * b1;
+        a.w[2] = a.w[1] + 2LL * a1 * B.n2;
+        a.w[3] = a.w[2] + a1 * b1;
+        a.ops = d_ops.p;
+        a.k0 = sg.k0;
+        a.k1 = sg.k1;
+        a.prog = L.prog2.p + sg.prog2Off;
+        a.prog3 = L.prog3.p + sg.prog3Off;
+        a.s1 = sg.progOff;
+        a.s2 = sg.split2;
         return a;
     }
 
@@ -993,7 +1178,13 @@ struct EngineImpl {
             const Seg &sg = laneF.segs[si];
             const double *in = ckOn ? ck_state((int)si, nch) : v;
             double *out = ckOn ? ck_state((int)si + 1, nch) : v;
-            if (sg.two) {
+            if (sg.three) {
+                const int mk = mark_begin(1);
+                hipLaunchKernelGGL((k_orb3v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, orb3args(laneF, sg),
+                                   in, out, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
+                mark_end(mk);
+                LCHK();
+            } else if (sg.two) {
                 const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orb2v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orb2args(laneF, sg, false), in, out, nullptr, cstride, bl_cs.p, M, nullptr,
@@ -1031,7 +1222,13 @@ struct EngineImpl {
             const Seg &sg = lsegs[si];
             const double *in = ckOn ? ck_state(si + 1, nch) : phi;
             double *out = ckOn ? nullptr : phi;
-            if (sg.two) {
+            if (sg.three) {
+                const int mk = mark_begin(2);
+                hipLaunchKernelGGL((k_orb3v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, orb3args(laneA, sg),
+                                   in, out, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                mark_end(mk);
+                LCHK();
+            } else if (sg.two) {
                 const int mk = mark_begin(2);
                 hipLaunchKernelGGL((k_orb2v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
                                    orb2args(laneA, sg, true), in, out, lam, cstride, bl_csneg.p, M, bl_partial.p,
@@ -1962,8 +2159,8 @@ bool Engine::set_option(int key, int value, std::string &err)
     }
     case kOptLaneGroups:
     case kOptLaneGroupsFwd: {
-        if (value < 1 || value > 2) {
-            err = "lane groups must be 1 or 2";
+        if (value < 1 || value > 3) {
+            err = "lane groups must be 1, 2 or 3";
             return false;
         }
         int &dst = key == kOptLaneGroups ? e.maxGroups : e.maxGroupsF;

@@ -32,7 +32,7 @@ enum EngineOption : int {
     kOptLdsBudget = 4,  // bytes of LDS an LDS-tile segment may use (0: the device's limit)
     kOptHamKind = 5,    // read only: 1 rows, 2 uniform-width rows, 3 matrix-free
     kOptSegments = 6,   // read only: fused segments per sweep (vector-major sweeps)
-    kOptLaneGroups = 7, // lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin and pass (1 or 2)
+    kOptLaneGroups = 7, // lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin and pass (1 to 3)
     kOptLaneSegments = 8,  // read only: fused segments per lane-layout adjoint sweep
     kOptLaneGroupsFwd = 9, // the same for the lane-layout forward sweep
     kOptCheckpoint = 10,   // 1: the lane-layout gradient keeps every segment's forward state (out-of-place

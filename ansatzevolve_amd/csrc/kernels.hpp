@@ -209,6 +209,38 @@ struct Orb2Args {
 template <bool BR>
 __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
+// Three-group orbit segment in lane layout: three consecutive runs of
+// operators on pairwise disjoint orbital groups (three TUPS units on
+// disjoint spatial pairs).  Classes have up to 8 members per spin; ranks are
+// two int4 per class (group-3 member 0, then 1; within each, group-1 member
+// + 2 group-2 member) and info = spectator bits | shape1 << 24 | shape2 << 26
+// | shape3 << 28.  Classes with two group-3 members come first (nA2, nB2 of
+// them).  Groups 1 and 2 are held in registers as in k_orb2v (16 members);
+// the group-3 members of an orbit sit in 1, 2 or 4 lanes of one vector (so a
+// wave covers 64, 32 or 16 vectors of it), and the third run's operators
+// rotate across those lanes.  Work items: 4 per orbit with two group-3
+// members in both spins (16 vectors each), 2 with two in one spin, 1
+// otherwise; w[0..3] are the cumulative ends of the four item ranges.
+// prog[type12 * kOrb3MaxOps + kk] are run 0/1 words as in k_orb2v (type12 from
+// shapes 1 and 2); prog3[type3 * kOrb3MaxOps + kk] are the run-2 words over the
+// four group-3 slots (a3 + 2 b3), type3 = shapeA3 + 3 shapeB3.  The host
+// checks that no operator's sign mask covers another group's orbitals and
+// that the active pairs of a run-2 operator share one slot xor.
+constexpr int kOrb3MaxOps = 16;
+struct Orb3Args {
+    int n, nb;
+    const int4 *aRanks, *bRanks;
+    const uint32_t *aInfo, *bInfo;
+    int nA, nB, nA2, nB2;
+    long long w[4];
+    const DevOp *ops;
+    int k0, k1;
+    const unsigned long long *prog, *prog3;
+    int s1, s2;  // operators k0 .. k0+s1-1: run 0; .. k0+s2-1: run 1; the rest run 2
+};
+template <bool BR>
+__global__ void k_orb3v(Orb3Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
+                        const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 // one operator on whole lane-layout vectors (any sector), wave per amplitude
 template <bool BR>
 __global__ void k_opv(double *__restrict__ v0, double *__restrict__ v1, long long cstride, long long N, DevSector sec,

@@ -1,5 +1,5 @@
 // Lane-layout sweep kernels (batches of 64 vectors, lane = vector): orbit
-// segments (k_orbv, k_orb2v) and one-operator passes (k_opv).  See
+// segments (k_orbv, k_orb2v, k_orb3v) and one-operator passes (k_opv).  See
 // kernels.hpp.
 //
 // Built without the promotion of private arrays to vector registers
@@ -417,6 +417,333 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
 template __global__ void k_orb2v<false>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
                                         double *, int);
 template __global__ void k_orb2v<true>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
+                                       double *, int);
+
+// ------------------------------------------------ three-group segments
+// f(Idx<I>) for I = B .. E-1, written out at compile time (no loop)
+template <int I>
+struct Idx {
+    static constexpr int value = I;
+};
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(Idx<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// The value of the lane (this lane xor L) of its quad (L = 1, 2, 3): two
+// 32-bit DPP quad_perm moves (DPP on 64-bit ALU operations takes only row
+// broadcasts).
+template <int L>
+__device__ __forceinline__ double quad_xor(double v)
+{
+    constexpr int ctrl = L == 1 ? 0xb1 : (L == 2 ? 0x4e : 0x1b);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// sum over the Q lanes of a vector (lanes Q v .. Q v + Q - 1)
+template <int Q>
+__device__ __forceinline__ double quad_sum(double v)
+{
+    if constexpr (Q >= 2) v += quad_xor<1>(v);
+    if constexpr (Q == 4) v += quad_xor<2>(v);
+    return v;
+}
+
+// bracket of one operator for this item: summed over the lanes of a vector,
+// added by the first of them to the wave's row of the vector
+template <int Q>
+__device__ __forceinline__ void orb3_bracket(double *accW, int kk, int vv, int sl, double br)
+{
+    br = quad_sum<Q>(br);
+    if (sl == 0) accW[kk * kLanes + vv] += br;
+}
+
+// operator of run 0 or 1 (groups 1, 2 in registers, as in k_orb2v)
+template <bool BR, int G, int Q>
+__device__ __forceinline__ void orb3_op12(const Orb3Args &a, const unsigned long long *pw, int kk, uint32_t spect,
+                                          const double2 (*csL)[kLanes], double *accW, int vv, int sl,
+                                          double (&x)[16], double (&l)[16], unsigned cmask)
+{
+    const unsigned w = (unsigned)pw[kk] & 0xfffu;
+    if (!w) return;
+    const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
+    const double2 t = csL[kk][vv];
+    double br = 0;
+    orb2_apply_uniform<G, G>(w, neg, t.x, t.y, x, l, BR, br, BR ? (cmask >> (4 * G)) & 0xfu : 0xfu);
+    if (BR) orb3_bracket<Q>(accW, kk, vv, sl, br);
+}
+
+// x <- cd x + cf xq, in place (as rot_inplace: a conditional update must
+// leave the member arrays in their registers)
+__device__ __forceinline__ void rot3_inplace(double &x, double xq, double cd, double cf)
+{
+    double t;
+    // operands: %0 x, %1 temporary, %2 xq, %3 cf, %4 cd
+    asm("v_mul_f64 %1, %3, %2\n\t"
+        "v_fma_f64 %0, %4, %0, %1"
+        : "+v"(x), "=&v"(t)
+        : "v"(xq), "v"(cf), "v"(cd));
+}
+
+// adjoint: brr += pm (l xq - lq x), then x <- cd x + cf xq, l <- cd l + cf lq,
+// in place
+__device__ __forceinline__ void rot3_br_inplace(double &x, double &l, double &brr, double xq, double lq, double cd,
+                                                double cf, double pm)
+{
+    double t1, t2, t3;
+    // operands: %0 x, %1 l, %2 brr, %3-%5 temporaries, %6 xq, %7 lq, %8 cd,
+    // %9 cf, %10 pm
+    asm("v_mul_f64 %3, %1, %6\n\t"
+        "v_fma_f64 %3, -%7, %0, %3\n\t"
+        "v_fma_f64 %2, %10, %3, %2\n\t"
+        "v_mul_f64 %4, %9, %6\n\t"
+        "v_mul_f64 %5, %9, %7\n\t"
+        "v_fma_f64 %0, %8, %0, %4\n\t"
+        "v_fma_f64 %1, %8, %1, %5"
+        : "+v"(x), "+v"(l), "+v"(brr), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+        : "v"(xq), "v"(lq), "v"(cd), "v"(cf), "v"(pm));
+}
+
+// operator of run 2: pairs of group-3 slots, one slot per lane; partner
+// lane = this lane xor d.  The branch on d only picks the partner moves:
+// around the in-place updates it made the compiler copy all 32 members at
+// the merge.
+template <bool BR>
+__device__ __forceinline__ void orb3_rot3(int d, double cd, double cf, double pm, double (&x)[16],
+                                          double (&l)[16], unsigned ex, double &brr)
+{
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        if (BR && !((ex >> j) & 1u)) continue;
+        double xq, lq = 0;
+        if (d == 1) {
+            xq = quad_xor<1>(x[j]);
+            if (BR) lq = quad_xor<1>(l[j]);
+        } else if (d == 2) {
+            xq = quad_xor<2>(x[j]);
+            if (BR) lq = quad_xor<2>(l[j]);
+        } else {
+            xq = quad_xor<3>(x[j]);
+            if (BR) lq = quad_xor<3>(l[j]);
+        }
+        if (BR) rot3_br_inplace(x[j], l[j], brr, xq, lq, cd, cf, pm);
+        else rot3_inplace(x[j], xq, cd, cf);
+    }
+}
+
+template <bool BR, int QA, int QB>
+__device__ __forceinline__ void orb3_op3(const Orb3Args &a, const unsigned long long *pw3, int kk, uint32_t spect,
+                                         const double2 (*csL)[kLanes], double *accW, int vv, int sl, int slot,
+                                         double (&x)[16], double (&l)[16], unsigned ex)
+{
+    constexpr int Q = QA * QB;
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+    const unsigned w = (unsigned)pw3[kk] & 0xfffu;
+    if (!w) return;
+    const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
+    const double2 t = csL[kk][vv];
+    // this lane's pair: (p, q) <- (C p + sS q, C q - sS p); bracket s (l_p x_q - l_q x_p) on the p side
+    double cd = 1.0, cf = 0.0, pm = 0.0;
+    int d = 0;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (!((w >> r) & 1u)) continue;
+        d = PP[r] ^ QQ[r];
+        const double s = ((((w >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
+        if (slot == PP[r]) {
+            cd = t.x;
+            cf = s * t.y;
+            pm = s;
+        } else if (slot == QQ[r]) {
+            cd = t.x;
+            cf = -s * t.y;
+        }
+    }
+    double brr = 0;
+    // slot bits (a3, b3) are lane bits 0, 1 with both spread, lane bit 0 with one
+    orb3_rot3<BR>(Q == 4 ? d : 1, cd, cf, pm, x, l, ex, brr);
+    if (BR) orb3_bracket<Q>(accW, kk, vv, sl, brr);
+}
+
+// one work item: orbit (ia, ib), vectors part * 64 / Q ..
+template <bool BR, int QA, int QB>
+__device__ __forceinline__ void orb3_item(const Orb3Args &a, int ia, int ib, int part, const double *v0, double *xout,
+                                          double *v1, bool copyAll, const double2 (*csL)[kLanes], double *accW,
+                                          int lane)
+{
+    constexpr int Q = QA * QB, VPI = kLanes / Q, K = kOrb3MaxOps;
+    const int sl = lane & (Q - 1);
+    const int a3 = QA == 2 ? (sl & 1) : 0;
+    const int b3 = QB == 2 ? (QA == 2 ? (sl >> 1) : sl) : 0;
+    const int slot = a3 + 2 * b3;
+    const int vv = part * VPI + lane / Q;
+    const int4 ra0 = ld_const(a.aRanks, 2 * ia), rb0 = ld_const(a.bRanks, 2 * ib);
+    int rA[4] = {ra0.x, ra0.y, ra0.z, ra0.w}, rB[4] = {rb0.x, rb0.y, rb0.z, rb0.w};
+    if constexpr (QA == 2) {
+        const int4 r1 = ld_const(a.aRanks, 2 * ia + 1);
+        rA[0] = a3 ? r1.x : rA[0];
+        rA[1] = a3 ? r1.y : rA[1];
+        rA[2] = a3 ? r1.z : rA[2];
+        rA[3] = a3 ? r1.w : rA[3];
+    }
+    if constexpr (QB == 2) {
+        const int4 r1 = ld_const(a.bRanks, 2 * ib + 1);
+        rB[0] = b3 ? r1.x : rB[0];
+        rB[1] = b3 ? r1.y : rB[1];
+        rB[2] = b3 ? r1.z : rB[2];
+        rB[3] = b3 ? r1.w : rB[3];
+    }
+    const uint32_t fa = ld_const(a.aInfo, ia), fb = ld_const(a.bInfo, ib);
+    const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+    const int type3 = ((fa >> 28) & 3) + 3 * ((fb >> 28) & 3);
+    const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+    // members present per spin in groups 1, 2 (the same for every group-3 member)
+    const bool a1 = ((fa >> 24) & 3) == 0, a2 = ((fa >> 26) & 3) == 0;
+    const bool b1 = ((fb >> 24) & 3) == 0, b2 = ((fb >> 26) & 3) == 0;
+    const unsigned exA = 1u | (a1 ? 2u : 0u) | (a2 ? 4u : 0u) | (a1 && a2 ? 8u : 0u);
+    const unsigned exB = 1u | (b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b1 && b2 ? 8u : 0u);
+    unsigned ex = 0;
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+        if ((exB >> mb) & 1u) ex |= exA << (4 * mb);
+    const unsigned cmask = 1u | (a2 ? 2u : 0u) | (b2 ? 4u : 0u) | (a2 && b2 ? 8u : 0u) |
+                           ((1u | (a1 ? 2u : 0u) | (b1 ? 4u : 0u) | (a1 && b1 ? 8u : 0u)) << 4);
+    // one member in every group: nothing acts on it
+    const bool act = Q > 1 || __popc(ex) >= 2;
+    const bool live = act || copyAll;
+    if (!live) return;
+    double x[16], l[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int ma = s & 3, mb = s >> 2;
+        const bool ok = rA[ma] >= 0 && rB[mb] >= 0;
+        const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
+        x[s] = ok ? __builtin_nontemporal_load(v0 + off + vv) : 0.0;
+        l[s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + vv) : 0.0;
+    }
+    if (act) {
+        // runtime operator loops (the bodies are the same for every operator;
+        // unrolled, the 48 bodies of the three runs overflowed the instruction
+        // cache and the register allocator)
+        const int nk = a.k1 - a.k0, s1 = a.s1, s2 = a.s2;
+        const unsigned long long *pw = a.prog + type * K, *pw3 = a.prog3 + type3 * K;
+        // every run (at most kOrbMaxOps operators) written out position by
+        // position with a compile-time direction, as in k_orb2v: the member
+        // arrays keep one register assignment throughout.  (A loop with the
+        // partner moves, which are convergent, stayed a loop and copied all
+        // 32 members at every iteration.)
+        constexpr int R = kOrbMaxOps;
+        if (!BR) {
+            static_for<0, R>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if (i < s1) orb3_op12<BR, 0, Q>(a, pw, i, spect, csL, accW, vv, sl, x, l, cmask);
+            });
+            static_for<0, R>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if (s1 + i < s2) orb3_op12<BR, 1, Q>(a, pw, s1 + i, spect, csL, accW, vv, sl, x, l, cmask);
+            });
+            if constexpr (Q > 1)
+                static_for<0, R>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    if (s2 + i < nk) orb3_op3<BR, QA, QB>(a, pw3, s2 + i, spect, csL, accW, vv, sl, slot, x, l, ex);
+                });
+        } else {
+            if constexpr (Q > 1)
+                static_for<0, R>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    if (nk - 1 - i >= s2)
+                        orb3_op3<BR, QA, QB>(a, pw3, nk - 1 - i, spect, csL, accW, vv, sl, slot, x, l, ex);
+                });
+            static_for<0, R>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if (s2 - 1 - i >= s1) orb3_op12<BR, 1, Q>(a, pw, s2 - 1 - i, spect, csL, accW, vv, sl, x, l, cmask);
+            });
+            static_for<0, R>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if (s1 - 1 - i >= 0) orb3_op12<BR, 0, Q>(a, pw, s1 - 1 - i, spect, csL, accW, vv, sl, x, l, cmask);
+            });
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int ma = s & 3, mb = s >> 2;
+        if (rA[ma] >= 0 && rB[mb] >= 0) {
+            const uint32_t off = ((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes;
+            if (xout) __builtin_nontemporal_store(x[s], xout + off + vv);
+            if (BR) __builtin_nontemporal_store(l[s], v1 + off + vv);
+        }
+    }
+}
+
+template <bool BR>
+// forward: at most 128 VGPRs (4 waves per SIMD); adjoint: 168 (3 waves, as
+// many as its 48 KB of LDS per block allow)
+__global__ void __launch_bounds__(kBlock, BR ? 3 : 4)
+    k_orb3v(Orb3Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
+            const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int WPB = kBlock / kLanes;
+    constexpr int K = kOrb3MaxOps;
+    __shared__ double2 csL[K][kLanes];
+    __shared__ double accL[BR ? WPB * K * kLanes : 1];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    const int nk = a.k1 - a.k0;
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        csL[kk][l] = cs[((long long)c * M + a.k0 + kk) * kLanes + l];
+    }
+    if (BR)
+        for (int e = threadIdx.x; e < WPB * K * kLanes; e += kBlock) accL[e] = 0.0;
+    __syncthreads();
+    double *accW = BR ? accL + wv * K * kLanes : nullptr;
+    v0 += c * cstride;
+    if (xout) xout += c * cstride;
+    if (BR) v1 += c * cstride;
+    const bool copyAll = xout && xout != v0;
+    const int nB1 = a.nB - a.nB2;
+    const long long stride = (long long)gridDim.x * WPB;
+    for (long long t = blockIdx.x * (long long)WPB + wv; t < a.w[3]; t += stride) {
+        if (t < a.w[0]) {
+            const int o = (int)(t >> 2), ia = o / a.nB2;
+            orb3_item<BR, 2, 2>(a, ia, o - ia * a.nB2, (int)(t & 3), v0, xout, v1, copyAll, csL, accW, lane);
+        } else if (t < a.w[1]) {
+            const long long u = t - a.w[0];
+            const int o = (int)(u >> 1), ia = o / nB1;
+            orb3_item<BR, 2, 1>(a, ia, a.nB2 + o - ia * nB1, (int)(u & 1), v0, xout, v1, copyAll, csL, accW,
+                                lane);
+        } else if (t < a.w[2]) {
+            const long long u = t - a.w[1];
+            const int o = (int)(u >> 1), i = o / a.nB2;
+            orb3_item<BR, 1, 2>(a, a.nA2 + i, o - i * a.nB2, (int)(u & 1), v0, xout, v1, copyAll, csL, accW,
+                                lane);
+        } else {
+            const int o = (int)(t - a.w[2]), i = o / nB1;
+            orb3_item<BR, 1, 1>(a, a.nA2 + i, a.nB2 + o - i * nB1, 0, v0, xout, v1, copyAll, csL, accW, lane);
+        }
+    }
+    if (BR) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+            const int kk = e / kLanes, l = e - kk * kLanes;
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < WPB; q++) s += accL[(q * K + kk) * kLanes + l];
+            partial[(((long long)c * M + a.k0 + kk) * slots + blockIdx.x) * kLanes + l] = s;
+        }
+    }
+}
+
+template __global__ void k_orb3v<false>(Orb3Args, const double *, double *, double *, long long, const double2 *, int,
+                                        double *, int);
+template __global__ void k_orb3v<true>(Orb3Args, const double *, double *, double *, long long, const double2 *, int,
                                        double *, int);
 
 template <bool BR>

@@ -184,7 +184,7 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     3 matrix-free;
  *   6 (read only) fused segments per sweep (vector-major sweeps);
  *   7 lane-layout adjoint sweep: disjoint 2-bit orbital groups per spin
- *     fused into one pass (1 or 2, default 1);
+ *     fused into one pass (1, 2 or 3, default 1);
  *   8 (read only) fused segments per lane-layout adjoint sweep;
  *   9 the same as 7 for the lane-layout forward sweep;
  *  10 stored forward states for the lane-layout gradient: 1 (default) keeps

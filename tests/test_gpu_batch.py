@@ -102,6 +102,34 @@ def test_two_group_segments_fuse_units(gpu):
             assert abs(E2[v] - Es) <= SAME and np.max(np.abs(G2[v] - gs)) <= SAME
 
 
+def test_three_group_segments(gpu):
+    """TUPS on 8 orbitals with three units on pairwise disjoint spatial pairs
+    per pass: the third group's members sit in 1, 2 or 4 lanes of a vector and
+    its operators rotate across those lanes.  Against one group per pass and
+    the single-vector path, in place and with stored forward states, over one
+    full and one partial chunk."""
+    w, thetas = _tups_case(8, 4, 1313, 70)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        a.set_option(AnsatzSynth.OPT_CHECKPOINT, 0)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 1)
+        E1, G1 = a.energy_gradients(thetas)
+        n1 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 3)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 3)
+        E3, G3 = a.energy_gradients(thetas)
+        n3 = a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS)
+        assert n1 == 28 and n3 == 10, (n1, n3)  # nine triples of units and a last single unit
+        assert np.max(np.abs(E3 - E1)) <= SAME and np.max(np.abs(G3 - G1)) <= SAME
+        a.set_option(AnsatzSynth.OPT_CHECKPOINT, 1)
+        Ec, Gc = a.energy_gradients(thetas)
+        assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1
+        assert np.max(np.abs(Ec - E1)) <= SAME and np.max(np.abs(Gc - G1)) <= SAME
+        for v in (0, 69):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(Ec[v] - Es) <= SAME and np.max(np.abs(Gc[v] - gs)) <= SAME
+
+
 @pytest.mark.parametrize("groups", [1, 2])
 def test_stored_forward_states(gpu, groups):
     """Gradient from stored forward states (out-of-place forward sweep, the
