@@ -1168,9 +1168,19 @@ struct EngineImpl {
     // state s of the stored forward states (checkpoint mode)
     double *ck_state(int s, int nch) { return bl_ck.p + (size_t)s * nch * N * kLanes; }
 
+    // Whether the first forward segment reads the initial state directly (one
+    // plain vector broadcast to every lane) instead of a lane-layout state 0:
+    // checkpoint mode with a two-group first segment.
+    bool first_reads_start() const
+    {
+        return ckOn && !laneF.segs.empty() && laneF.segs[0].two && !laneF.segs[0].three;
+    }
+
     // Forward sweep of lane-layout chunks: in place on v, or (checkpoint
-    // mode) segment s from state s to state s + 1, v being state 0.
-    // (with timing on, every launch is bracketed by its own events)
+    // mode) segment s from state s to state s + 1, v being state 0 (with
+    // first_reads_start(), segment 0 reads d_start instead and state 0 is
+    // never written).  With timing on, every launch is bracketed by its own
+    // events.
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
         const long long cstride = N * kLanes;
@@ -1178,7 +1188,14 @@ struct EngineImpl {
             const Seg &sg = laneF.segs[si];
             const double *in = ckOn ? ck_state((int)si, nch) : v;
             double *out = ckOn ? ck_state((int)si + 1, nch) : v;
-            if (sg.three) {
+            if (si == 0 && first_reads_start()) {
+                const int mk = mark_begin(1);
+                hipLaunchKernelGGL((k_orb2v<false, true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
+                                   orb2args(laneF, sg, false), d_start.p, out, nullptr, cstride, bl_cs.p, M, nullptr,
+                                   lslotsF);
+                mark_end(mk);
+                LCHK();
+            } else if (sg.three) {
                 const int mk = mark_begin(1);
                 hipLaunchKernelGGL((k_orb3v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, orb3args(laneF, sg),
                                    in, out, nullptr, cstride, bl_cs.p, M, nullptr, lslotsF);
@@ -1991,9 +2008,12 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
         const long long total = (long long)nch * e.N * kLanes;
         const dim3 tgrid((unsigned)((e.N + kLanes - 1) / kLanes), nch);
         double *psi0 = e.ckOn ? e.ck_state(0, nch) : e.bl_psi.p;
-        hipLaunchKernelGGL(k_bcast_lanes, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
-                           dim3(kBlock), 0, e.stream, e.d_start.p, psi0, e.N, nch);
-        LCHK();
+        if (!e.first_reads_start()) {
+            hipLaunchKernelGGL(k_bcast_lanes,
+                               dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)), dim3(kBlock),
+                               0, e.stream, e.d_start.p, psi0, e.N, nch);
+            LCHK();
+        }
         if (!e.sweep_forward_lanes(psi0, nch, err)) return false;
         if (e.hkind == 3 && e.sigv) {
             // matrix-free sigma in lane layout: no transposes

@@ -205,8 +205,10 @@ struct Orb2Args {
     int split;  // operators k0 .. k0+split-1 act on groups (0, 0), the rest on (1, 1)
 };
 // BR = adjoint sweep: operators in reverse order with brackets (a.reverse is
-// ignored); !BR: forward order.
-template <bool BR>
+// ignored); !BR: forward order.  BC (forward, out of place): v0 is one plain
+// vector of N amplitudes read by every lane (the initial state), not a lane-
+// layout chunk.
+template <bool BR, bool BC = false>
 __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 // Three-group orbit segment in lane layout: three consecutive runs of

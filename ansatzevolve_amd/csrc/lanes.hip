@@ -278,7 +278,7 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
     if (BR) accW[kk * kLanes + lane] += br;
 }
 
-template <bool BR>
+template <bool BR, bool BC>
 // forward: at most 80 VGPRs (6 waves per SIMD); adjoint: 96 (5 waves)
 __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
@@ -300,11 +300,11 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
         for (int e = threadIdx.x; e < WPB * K * kLanes; e += kBlock) accL[e] = 0.0;
     __syncthreads();
     double *accW = BR ? accL + wv * K * kLanes : nullptr;
-    v0 += c * cstride;
+    if (!BC) v0 += c * cstride;  // BC: v0 is one plain vector, the same for every lane
     if (xout) xout += c * cstride;
     if (BR) v1 += c * cstride;
     // out of place: every member is written, lone determinants included
-    const bool copyAll = xout && xout != v0;
+    const bool copyAll = BC || (xout && xout != v0);
     const int total = a.nAc * a.nBc;
     const int stride = gridDim.x * WPB;
     // the forward sweep takes two orbits per iteration (both loaded before
@@ -353,7 +353,8 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
                 const int ma = s & 3, mb = s >> 2;
                 const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
                 const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
+                if (BC) x[o][s] = ok ? v0[off / kLanes] : 0.0;
+                else x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
                 l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
             }
 #pragma unroll
@@ -414,10 +415,12 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     }
 }
 
-template __global__ void k_orb2v<false>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
-                                        double *, int);
-template __global__ void k_orb2v<true>(Orb2Args, const double *, double *, double *, long long, const double2 *, int,
-                                       double *, int);
+template __global__ void k_orb2v<false, false>(Orb2Args, const double *, double *, double *, long long,
+                                               const double2 *, int, double *, int);
+template __global__ void k_orb2v<false, true>(Orb2Args, const double *, double *, double *, long long,
+                                              const double2 *, int, double *, int);
+template __global__ void k_orb2v<true, false>(Orb2Args, const double *, double *, double *, long long,
+                                              const double2 *, int, double *, int);
 
 // ------------------------------------------------ three-group segments
 // f(Idx<I>) for I = B .. E-1, written out at compile time (no loop)
