@@ -27,6 +27,7 @@ namespace {
 
 struct Candidate {
     int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves, grouped;
+    int gemmB = 0, gemmA = 0;  // rocBLAS solutions of the sigma DGEMMs (0: default; tuned after the choice)
 };
 
 std::string shape_key(const Engine &eng, int nvec, int nops)
@@ -41,7 +42,8 @@ std::string cand_json(const Candidate &c)
     std::ostringstream o;
     o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
       << ", \"groups\": " << c.groups << ", \"groups_fwd\": " << c.groupsFwd << ", \"checkpoint\": " << c.ckpt
-      << ", \"sigma_halves\": " << c.halves << ", \"sigma_grouped\": " << c.grouped << "}";
+      << ", \"sigma_halves\": " << c.halves << ", \"sigma_grouped\": " << c.grouped << ", \"gemm_beta\": " << c.gemmB
+      << ", \"gemm_alpha\": " << c.gemmA << "}";
     return o.str();
 }
 
@@ -58,9 +60,10 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         Candidate t{};
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
                         "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d, "
-                        "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d, \"sigma_grouped\": %d}",
-                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves,
-                        &t.grouped) == 8) {
+                        "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d, \"sigma_grouped\": %d, "
+                        "\"gemm_beta\": %d, \"gemm_alpha\": %d}",
+                        &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves, &t.grouped,
+                        &t.gemmB, &t.gemmA) == 10) {
             c = t;
             found = true;
         }
@@ -74,6 +77,7 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
            eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneGroupsFwd, c.groupsFwd, err) &&
            eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptCheckpoint, c.ckpt, err) &&
            eng.set_option(kOptSigmaHalves, c.halves, err) && eng.set_option(kOptSigmaGrouped, c.grouped, err) &&
+           eng.set_option(kOptGemmSolutionBeta, c.gemmB, err) && eng.set_option(kOptGemmSolutionAlpha, c.gemmA, err) &&
            eng.set_option(kOptGraph, 1, err);
 }
 
@@ -127,9 +131,11 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
             }
         }
 
-    const Candidate saved{eng.option(kOptSweep),         eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
-                          eng.option(kOptLaneGroups),    eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint),
-                          eng.option(kOptSigmaHalves),   eng.option(kOptSigmaGrouped)};
+    Candidate saved{eng.option(kOptSweep),         eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
+                    eng.option(kOptLaneGroups),    eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint),
+                    eng.option(kOptSigmaHalves),   eng.option(kOptSigmaGrouped)};
+    saved.gemmB = eng.option(kOptGemmSolutionBeta);
+    saved.gemmA = eng.option(kOptGemmSolutionAlpha);
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";
@@ -175,7 +181,18 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
         log_msg("Autotune: applying the choice failed: %s", err.c_str());
         return false;
     }
-    rep << "], \"max_dev\": " << worstDev << ", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs << "}";
+    // rocBLAS solutions of the lane sigma's DGEMMs for the chosen layout
+    // (every candidate above ran the library's default ones)
+    std::string gemmRep;
+    if (!eng.batch_upload(nvec, dec.data(), err) || !eng.tune_sigma_gemms(gemmRep, err)) {
+        log_msg("Autotune: tuning the sigma DGEMMs failed: %s", err.c_str());
+        return false;
+    }
+    best.gemmB = eng.option(kOptGemmSolutionBeta);
+    best.gemmA = eng.option(kOptGemmSolutionAlpha);
+    rep << "], \"max_dev\": " << worstDev << ", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs;
+    if (!gemmRep.empty()) rep << ", \"sigma_gemms\": " << gemmRep;
+    rep << "}";
     report = rep.str();
     if (cacheFile && *cacheFile) {
         std::ofstream out(cacheFile, std::ios::app);
@@ -203,6 +220,9 @@ bool Manager::getEngineOption(int key, int &value)
 {
     if (!validateToRun()) return false;
     value = m_engine->option(key);
+    // rocBLAS solution indices may be negative; every other option is >= 0
+    // and -1 marks an unknown key
+    if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha) return true;
     return value >= 0;
 }
 

@@ -5,6 +5,10 @@
 // operator table and the sin/cos table -- is resident in HBM; a call moves
 // only the per-operator sin/cos (16 B per operator) in and the results out.
 #include <hip/hip_runtime.h>
+// rocblas_gemm_strided_batched_ex_get_solutions (solution choice of the
+// same-spin DGEMMs) is part of rocBLAS's beta API
+#define ROCBLAS_BETA_FEATURES_API
+#define ROCBLAS_NO_DEPRECATED_WARNINGS
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -197,6 +201,7 @@ struct EngineImpl {
     int sighOpt = 0;       // kOptSigmaHalves (autotuned; the one-row kernel is faster at 12 orbitals)
     bool sigg = false;     // grouped-target sigma available (12 orbitals, 6 beta electrons)
     int siggOpt = 1;       // kOptSigmaGrouped
+    int gemmSol[2] = {0, 0};  // rocBLAS solution of the beta / alpha same-spin DGEMM (0: library default)
     int sighS = 0, sighX = 0, sighNA = 0, sighNB = 0, sighTA = 0, sighT = 0;
     DBuf<uint32_t> d_tsh;
     DBuf<int> d_hl;
@@ -1272,6 +1277,31 @@ struct EngineImpl {
         return true;
     }
 
+    // The same-spin DGEMMs of the lane-layout sigma with rocBLAS solution sol
+    // (0: the library's default).  which 0, beta part: per alpha row,
+    // Y_a (nb x 64) += V_bb X_a (column-major view: Y_a^T += X_a^T V_bb^T);
+    // which 1, alpha part: per chunk, Y (na x nb*64) += V_aa X.
+    rocblas_status sigma_gemm(int which, const double *x, double *y, int nch, int sol, bool list = false,
+                              rocblas_int *sols = nullptr, rocblas_int *nsol = nullptr)
+    {
+        const double one = 1.0;
+        const rocblas_datatype f = rocblas_datatype_f64_r;
+        const rocblas_gemm_algo algo = (sol != 0 || list) ? rocblas_gemm_algo_solution_index
+                                                          : rocblas_gemm_algo_standard;
+        const int m = which ? nb * kLanes : kLanes, nk = which ? na : nb;
+        const int ld = which ? nb * kLanes : kLanes;
+        const rocblas_stride st = which ? (rocblas_stride)N * kLanes : (rocblas_stride)nb * kLanes;
+        const double *V = which ? d_Vaa.p : d_Vbb.p;
+        const int batch = which ? nch : na * nch;
+        if (list)
+            return rocblas_gemm_strided_batched_ex_get_solutions(
+                blas, rocblas_operation_none, rocblas_operation_none, m, nk, nk, &one, x, f, ld, st, V, f, nk, 0, &one,
+                y, f, ld, st, y, f, ld, st, batch, f, algo, 0, sols, nsol);
+        return rocblas_gemm_strided_batched_ex(blas, rocblas_operation_none, rocblas_operation_none, m, nk, nk, &one,
+                                               x, f, ld, st, V, f, nk, 0, &one, y, f, ld, st, y, f, ld, st, batch, f,
+                                               algo, sol, 0);
+    }
+
     // Lane-layout sigma for nch chunks and the per-vector energies
     // out[v] = <x_v|y_v> (matrix-free storage, n <= 12).
     bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err)
@@ -1306,20 +1336,17 @@ struct EngineImpl {
         mark_end(mk);
         LCHK();
         const int mk3 = mark_begin(3);
-        const double one = 1.0;
-        // beta part: per alpha row, Y_a (nb x 64) += V_bb X_a  (column-major view: Y_a^T += X_a^T V_bb^T)
-        if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, kLanes, nb, nb, &one,
-                                          x, kLanes, (rocblas_stride)nb * kLanes, d_Vbb.p, nb, 0, &one, y, kLanes,
-                                          (rocblas_stride)nb * kLanes, na * nch) != rocblas_status_success) {
-            err = "rocblas_dgemm_strided_batched (beta part) failed";
-            return false;
-        }
-        // alpha part: per chunk, Y (na x nb*64) += V_aa X
-        if (rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, nb * kLanes, na, na,
-                                          &one, x, nb * kLanes, (rocblas_stride)N * kLanes, d_Vaa.p, na, 0, &one, y,
-                                          nb * kLanes, (rocblas_stride)N * kLanes, nch) != rocblas_status_success) {
-            err = "rocblas_dgemm_strided_batched (alpha part) failed";
-            return false;
+        for (int which = 0; which < 2; which++) {
+            rocblas_status st = sigma_gemm(which, x, y, nch, gemmSol[which]);
+            if (st != rocblas_status_success && gemmSol[which] != 0) {
+                // a recorded solution this library does not offer: the default
+                gemmSol[which] = 0;
+                st = sigma_gemm(which, x, y, nch, 0);
+            }
+            if (st != rocblas_status_success) {
+                err = which ? "rocBLAS DGEMM (alpha part) failed" : "rocBLAS DGEMM (beta part) failed";
+                return false;
+            }
         }
         mark_end(mk3);
         const int G = 512;
@@ -2202,6 +2229,14 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.sighOpt = value;
         e.drop_graphs();
         return true;
+    case kOptGemmSolutionBeta:
+    case kOptGemmSolutionAlpha: {
+        int &dst = e.gemmSol[key == kOptGemmSolutionAlpha ? 1 : 0];
+        if (value == dst) return true;
+        dst = value;
+        e.drop_graphs();
+        return true;
+    }
     case kOptSigmaGrouped:
         if (value < 0 || value > 1) {
             err = "sigma grouped option must be 0 or 1";
@@ -2254,9 +2289,71 @@ int Engine::option(int key) const
     case kOptSigmaHalvesAvailable: return e.sigh ? 1 : 0;
     case kOptSigmaGrouped: return e.siggOpt;
     case kOptSigmaGroupedAvailable: return e.sigg ? 1 : 0;
+    case kOptGemmSolutionBeta: return e.gemmSol[0];
+    case kOptGemmSolutionAlpha: return e.gemmSol[1];
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }
+}
+
+bool Engine::tune_sigma_gemms(std::string &report, std::string &err)
+{
+    EngineImpl &e = *d_;
+    report.clear();
+    if (!(e.hkind == 3 && e.sigv && e.bnvec >= 1 && e.use_lanes(e.bnvec) && e.blas)) return true;
+    const int nch = (e.bnvec + kLanes - 1) / kLanes;
+    const double *x = e.bl_psi.p;
+    double *y = e.bl_lam.p;
+    hipEvent_t t0, t1;
+    HCHK(hipEventCreate(&t0));
+    HCHK(hipEventCreate(&t1));
+    std::string rep = "{";
+    bool ok = true;
+    for (int which = 0; which < 2 && ok; which++) {
+        rocblas_int cnt = 0;
+        std::vector<rocblas_int> list;
+        if (e.sigma_gemm(which, x, y, nch, 0, true, nullptr, &cnt) == rocblas_status_success && cnt > 0) {
+            list.resize(cnt);
+            if (e.sigma_gemm(which, x, y, nch, 0, true, list.data(), &cnt) != rocblas_status_success) list.clear();
+            list.resize(std::min<size_t>(list.size(), (size_t)std::max<rocblas_int>(cnt, 0)));
+        }
+        list.insert(list.begin(), 0);  // the library's default first
+        int best = 0;
+        float bestMs = 1e30f, defMs = 0;
+        for (rocblas_int sol : list) {
+            // one warm-up call, then the mean of three
+            if (e.sigma_gemm(which, x, y, nch, sol) != rocblas_status_success) continue;
+            if (hipEventRecord(t0, e.stream) != hipSuccess) { ok = false; break; }
+            bool good = true;
+            for (int r = 0; r < 3 && good; r++) good = e.sigma_gemm(which, x, y, nch, sol) == rocblas_status_success;
+            if (hipEventRecord(t1, e.stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) {
+                ok = false;
+                break;
+            }
+            float ms = 0;
+            if (!good || hipEventElapsedTime(&ms, t0, t1) != hipSuccess) continue;
+            ms /= 3;
+            if (sol == 0) defMs = ms;
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = sol;
+            }
+        }
+        e.gemmSol[which] = best;
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "%s\"%s\": {\"solutions\": %d, \"default_ms\": %.4f, \"chosen\": %d, \"ms\": %.4f}",
+                      which ? ", " : "", which ? "alpha" : "beta", (int)list.size() - 1, defMs, best, bestMs);
+        rep += buf;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (!ok) {
+        err = "timing the sigma DGEMMs failed";
+        return false;
+    }
+    e.drop_graphs();
+    report = rep + "}";
+    return true;
 }
 
 bool Engine::synchronize(std::string &err)

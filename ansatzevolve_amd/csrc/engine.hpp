@@ -45,6 +45,8 @@ enum EngineOption : int {
     kOptSigmaGrouped = 14,      // 1 (default): lane-layout sigma with targets grouped by their upper six
                                 // beta orbitals when available (takes precedence over kOptSigmaHalves)
     kOptSigmaGroupedAvailable = 15,  // read only: 1 when the grouped sigma applies (12 orbitals, 6 beta)
+    kOptGemmSolutionBeta = 16,   // rocBLAS solution index of the lane sigma's beta-part DGEMM (0: default)
+    kOptGemmSolutionAlpha = 17,  // the same for the alpha-part DGEMM (an index the library lacks: default)
 };
 
 class Engine {
@@ -108,6 +110,11 @@ public:
     void *stream() const;
     bool batch_download(double *E, double *deriv, std::string &err);
     int batch_size() const;
+    // Times every rocBLAS solution of the lane-layout sigma's two same-spin
+    // DGEMMs on the uploaded batch's buffers (contents are overwritten) and
+    // keeps the fastest; report: JSON with the counts and times.  Does
+    // nothing unless the batch runs the lane-layout matrix-free sigma.
+    bool tune_sigma_gemms(std::string &report, std::string &err);
     bool synchronize(std::string &err);
 
     // HIP-event timing of kernel classes on the engine's stream:

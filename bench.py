@@ -143,6 +143,7 @@ def main():
         print("[bench] autotuning the batch (sweep kind x fusion depth x layout)", file=sys.stderr, flush=True)
         rep = a.autotune(thetas, reps=2)
         tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
+                "sigma_gemms": rep.get("sigma_gemms"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
                                 c["choice"]["groups"], c["choice"]["groups_fwd"], c["choice"]["checkpoint"],
                                 c["choice"]["sigma_halves"], c["choice"]["sigma_grouped"], round(c["ms"], 3)]
@@ -156,7 +157,9 @@ def main():
             "lane_min": a.get_option(a.OPT_LANE_MIN), "graph": a.get_option(a.OPT_GRAPH),
             "lane_groups": a.get_option(a.OPT_LANE_GROUPS), "lane_groups_fwd": a.get_option(a.OPT_LANE_GROUPS_FWD),
             "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS), "checkpoint": a.get_option(a.OPT_CHECKPOINT),
-            "sigma_halves": a.get_option(a.OPT_SIGMA_HALVES), "sigma_grouped": a.get_option(a.OPT_SIGMA_GROUPED)}
+            "sigma_halves": a.get_option(a.OPT_SIGMA_HALVES), "sigma_grouped": a.get_option(a.OPT_SIGMA_GROUPED),
+            "gemm_solution_beta": a.get_option(a.OPT_GEMM_SOLUTION_BETA),
+            "gemm_solution_alpha": a.get_option(a.OPT_GEMM_SOLUTION_ALPHA)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)

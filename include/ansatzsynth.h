@@ -203,7 +203,11 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     orbitals, when it applies (it then takes precedence over key 12),
  *     0 the variants of key 12;
  *  15 (read only) 1 when the grouped variant applies (12 orbitals, 6 beta
- *     electrons).
+ *     electrons);
+ *  16, 17 rocBLAS solution index of the lane-layout sigma's beta-part and
+ *     alpha-part DGEMMs (0, the default: the library's own choice; the
+ *     autotuner times every solution and keeps the fastest; an index the
+ *     library does not offer falls back to 0).
  * Changing keys 0, 3, 4, 7, 9 or 10 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

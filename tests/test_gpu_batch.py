@@ -290,6 +290,29 @@ def test_autotune_invariance_and_cache(gpu, tmp_path):
         assert len(f.readlines()) == 1
 
 
+def test_autotune_sigma_gemm_solutions(gpu):
+    """Matrix-free storage at 10 orbitals: after choosing the layout, the
+    autotuner times every rocBLAS solution of the lane sigma's two same-spin
+    DGEMMs and keeps the fastest; the batch with the chosen solutions equals
+    the single-vector path."""
+    w = make_workload(2)
+    rng = np.random.default_rng(6)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-np.pi, np.pi, size=(64, w["P"]))
+    with from_workload(w, storage=2) as a:
+        rep = a.autotune(thetas, reps=1)
+        ch = rep["choice"]
+        assert a.get_option(AnsatzSynth.OPT_GEMM_SOLUTION_BETA) == ch["gemm_beta"]
+        assert a.get_option(AnsatzSynth.OPT_GEMM_SOLUTION_ALPHA) == ch["gemm_alpha"]
+        if ch["lane_min"] == 1:
+            g = rep["sigma_gemms"]
+            for part in ("beta", "alpha"):
+                assert g[part]["solutions"] >= 1 and g[part]["ms"] <= g[part]["default_ms"] * 1.05
+        E, G = a.energy_gradients(thetas)
+        for v in (0, 63):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E[v] - Es) <= SAME and np.max(np.abs(G[v] - gs)) <= SAME
+
+
 def test_lane_sigma_config2_against_reference(gpu):
     """10 orbitals, 63,504 determinants, matrix-free storage: a 64-vector
     batch through the lane-layout sigma against the reference build's
