@@ -86,6 +86,7 @@ struct GroupSet {
     bool beta = false;
     int ncls = 0;
     int n2 = 0;  // three-group sets: the first n2 classes have two group-3 members
+    int nm = 0;  // two-group sets: the first nm classes have two members in some group
     DBuf<int4> ranks;
     DBuf<uint32_t> info;
 };
@@ -355,6 +356,23 @@ struct EngineImpl {
             const int m1 = (c1 == 1 && !(str & lo1)) ? 1 : 0, m2 = (c2 == 1 && !(str & lo2)) ? 1 : 0;
             int *rr = &r[ci].x;
             rr[m1 + 2 * m2] = rk;
+        }
+        // classes with two members in some group first, single-member ones
+        // last (their orbits with single-member classes of the other spin are
+        // lone determinants)
+        {
+            std::vector<int4> r2;
+            std::vector<uint32_t> i2;
+            for (int pass = 0; pass < 2; pass++)
+                for (size_t ci = 0; ci < r.size(); ci++) {
+                    const bool multi = ((info[ci] >> 24) & 3) == 0 || ((info[ci] >> 26) & 3) == 0;
+                    if (multi != (pass == 0)) continue;
+                    r2.push_back(r[ci]);
+                    i2.push_back(info[ci]);
+                    if (multi) t->nm++;
+                }
+            r.swap(r2);
+            info.swap(i2);
         }
         t->ncls = (int)r.size();
         if (!t->ranks.ensure(r.size(), err) || !t->info.ensure(info.size(), err)) return -1;
@@ -804,6 +822,8 @@ struct EngineImpl {
         a.bInfo = B.info.p;
         a.nAc = A.ncls;
         a.nBc = B.ncls;
+        a.nAm = A.nm;
+        a.nBm = B.nm;
         a.ops = d_ops.p;
         a.k0 = sg.k0;
         a.k1 = sg.k1;

@@ -199,6 +199,11 @@ struct Orb2Args {
     const int4 *aRanks, *bRanks;
     const uint32_t *aInfo, *bInfo;
     int nAc, nBc;
+    // classes with two members in some group come first (nAm, nBm of them):
+    // orbits of a single-member alpha and beta class (lone determinants, no
+    // operator acts) are the rectangle [nAm, nAc) x [nBm, nBc), left out of
+    // the orbit loop and only copied (out of place)
+    int nAm, nBm;
     const DevOp *ops;
     int k0, k1, reverse;
     const unsigned long long *prog;

@@ -305,7 +305,10 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     if (BR) v1 += c * cstride;
     // out of place: every member is written, lone determinants included
     const bool copyAll = BC || (xout && xout != v0);
-    const int total = a.nAc * a.nBc;
+    // orbits with some two-member class: alpha classes [0, nAm) with every
+    // beta class, then the single-member alpha classes with beta [0, nBm)
+    const int mainA = a.nAm * a.nBc;
+    const int total = mainA + (a.nAc - a.nAm) * a.nBm;
     const int stride = gridDim.x * WPB;
     // the forward sweep takes two orbits per iteration (both loaded before
     // either is computed: twice the bytes in flight per wave); the adjoint
@@ -320,7 +323,15 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
             const int t = t0 + o * stride;
             live[o] = t < total;
             const int tt = live[o] ? t : t0;
-            const int ia = tt / a.nBc, ib = tt - ia * a.nBc;
+            int ia, ib;
+            if (tt < mainA) {
+                ia = tt / a.nBc;
+                ib = tt - ia * a.nBc;
+            } else {
+                const int u = tt - mainA, i = u / a.nBm;
+                ia = a.nAm + i;
+                ib = u - i * a.nBm;
+            }
             // adjoint: descriptors through the scalar unit (ld_const); the
             // forward kernel ran slower that way (9.5 against 7.4 ms per step)
             const int4 ra = BR ? ld_const(a.aRanks, ia) : a.aRanks[ia];
@@ -402,6 +413,25 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
                     if (BR) __builtin_nontemporal_store(l[o][s], v1 + off + lane);
                 }
             }
+    }
+    if (copyAll) {
+        // lone determinants: copied, four per wave iteration (all loads first)
+        const int nAl = a.nAc - a.nAm, nBl = a.nBc - a.nBm, nl = nAl * nBl;
+        for (int t0 = blockIdx.x * WPB + wv; t0 < nl; t0 += 4 * stride) {
+            uint32_t off[4];
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int t = t0 + q * stride, tt = t < nl ? t : t0;
+                const int i = tt / nBl, ia = a.nAm + i, ib = a.nBm + (tt - i * nBl);
+                const int ra = a.aRanks[ia].x, rb = a.bRanks[ib].x;
+                off[q] = ((uint32_t)ra * (uint32_t)a.nb + (uint32_t)rb) * kLanes;
+                if (t < nl) v[q] = BC ? v0[off[q] / kLanes] : __builtin_nontemporal_load(v0 + off[q] + lane);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (t0 + q * stride < nl) __builtin_nontemporal_store(v[q], xout + off[q] + lane);
+        }
     }
     if (BR) {
         __syncthreads();
