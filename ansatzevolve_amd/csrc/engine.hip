@@ -195,6 +195,7 @@ struct EngineImpl {
     bool sig1 = false;  // vector-major split sigma available (n <= 14)
     DBuf<double> d_Vaa, d_Vbb;
     DBuf<uint32_t> d_ts, d_ts1;
+    DBuf<int4> d_sigsrc;  // grouped sigma: source rows of every alpha row, ascending
     int ntp = 0;
     int ntp16 = 0;  // lane-layout table: singles per beta rounded up to 4 (rows of kSigWNtp)
     // two-half lane sigma (k_sigh_sparse, n = 2 kd): target-slot table, slot ranks
@@ -1330,6 +1331,7 @@ struct EngineImpl {
         SigvArgs s;
         s.m = mfargs();
         s.ts = d_ts.p;
+        s.src = d_sigsrc.p;
         s.ntp = ntp16;
         s.x = x;
         s.y = y;
@@ -1822,9 +1824,30 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSigWLds));
             e.sigh = lanesOk && n == 2 * kd && sigh_kernel((kd - 1) * (n - kd), n - kd) != nullptr;
             e.sigg = lanesOk && n == 12 && kd == 6 && e.nb == 924;
-            if (e.sigg)
+            if (e.sigg) {
                 HCHK(hipFuncSetAttribute((const void *)k_sigg_sparse<6>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)kSigWLds));
+                // source rows per alpha row in ascending rank: concurrently
+                // running workgroups with nearby targets stage nearby (often
+                // the same) source rows at about the same time
+                const int ku = __builtin_popcount(e.unA[0]);
+                const int R = 1 + ku * (n - ku);
+                std::vector<int4> src((size_t)na * R);
+                for (size_t ar = 0; ar < na; ar++) {
+                    const uint32_t al = e.unA[ar], full = (1u << n) - 1;
+                    std::vector<int4> row;
+                    row.push_back(make_int4((int)ar, -1, -1, 0));
+                    for (uint32_t o = al; o; o &= o - 1)
+                        for (uint32_t w = ~al & full; w; w &= w - 1) {
+                            const int rA = __builtin_ctz(o), qA = __builtin_ctz(w);
+                            row.push_back(make_int4(e.rkA[al ^ (1u << rA) ^ (1u << qA)], rA, qA, 0));
+                        }
+                    std::sort(row.begin(), row.end(), [](const int4 &p, const int4 &q) { return p.x < q.x; });
+                    std::copy(row.begin(), row.end(), src.begin() + ar * R);
+                }
+                if (!e.d_sigsrc.ensure(src.size(), err)) return false;
+                HCHK(hipMemcpy(e.d_sigsrc.p, src.data(), src.size() * sizeof(int4), hipMemcpyHostToDevice));
+            }
             if (e.sigh && !e.build_sigh_tables(err)) return false;
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));

@@ -342,6 +342,9 @@ constexpr int kSigGS = 2 * kSigNN + 8;  // coefficient row: +coef[e], -coef[e] a
 struct SigvArgs {
     MfArgs m;
     const uint32_t *ts;  // nb x ntp beta entries (k_beta_table)
+    // k_sigg_sparse: per alpha row its source rows in ascending rank,
+    // {J, rA, qA, 0} (rA = -1: the identity row), 1 + k(n - k) per row
+    const int4 *src;
     int ntp;
     const double *x;
     double *y;

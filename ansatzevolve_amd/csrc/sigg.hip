@@ -124,7 +124,7 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
     const int n = m.n, nb = m.nb, nn = n * n;
     const int tid = threadIdx.x, lane = tid & (kLanes - 1), v = lane & 15;
     const int wu = __builtin_amdgcn_readfirstlane(tid / kLanes);
-    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const uint32_t alpha = m.unA[ar];
     const int ka = __popc(alpha), ne = n - ka;
     const int R = 1 + ka * ne;
     const uint32_t gsb = (uint32_t)(size_t)(sg_lds_f64 *)gs;
@@ -134,18 +134,11 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
 #pragma unroll
     for (int i = 0; i < kSiggAcc; i++) acc[i] = 0;
     for (int r = 0; r < R; r++) {
-        // source row: r = 0 the identity row (J = ar), else the alpha single
-        // tA = r - 1 (occupied rA, empty qA by rank) with its coefficients
-        int J = ar, rA = 0, qA = 0;
-        if (r > 0) {
-            const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
-            uint32_t oo = alpha, ww = ~alpha & full;
-            for (int i = 0; i < ir; i++) oo &= oo - 1;
-            for (int i = 0; i < iq; i++) ww &= ww - 1;
-            rA = __ffs(oo) - 1;
-            qA = __ffs(ww) - 1;
-            J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
-        }
+        // source row r in ascending rank: the identity row (J = ar) or the
+        // alpha single (occupied rA, empty qA) with its coefficients
+        const int4 se = a.src[(long long)ar * R + r];
+        const int J = se.x, rA = se.y < 0 ? 0 : se.y, qA = se.z < 0 ? 0 : se.z;
+        const bool ident = se.y < 0;
         __syncthreads();  // every read of the previous row is done
         {
             // the source row into LDS by asynchronous global-to-LDS copies, all
@@ -158,7 +151,7 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
                                                  0);
             }
         }
-        if (r == 0) {
+        if (ident) {
             // identity row: J_alpha[e] = sum over occupied alpha o of g[oo][e] (r != q only)
             for (int e = tid; e < nn; e += kSigWThreads) {
                 double vj = 0;
@@ -185,7 +178,7 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
         if (tid < 128) {
             const int hi = tid >> 6, t = tid & 63;
             double d = 0;
-            if (r > 0) {
+            if (!ident) {
                 const bool neg = sg_between_parity(alpha, rA, qA);
                 const double *gb = m.g + (size_t)(rA * n + qA) * nn;
                 double gv[6];
@@ -199,8 +192,8 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
         }
         __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's row copies have landed
         __syncthreads();
-        sigg_task<K, MA, 0>(hA, lb, gsb, dt, diag, r == 0, acc);
-        if constexpr (MB >= 0) sigg_task<K, MB, 10>(hB, lb, gsb, dt, diag, r == 0, acc);
+        sigg_task<K, MA, 0>(hA, lb, gsb, dt, diag, ident, acc);
+        if constexpr (MB >= 0) sigg_task<K, MB, 10>(hB, lb, gsb, dt, diag, ident, acc);
     }
     const uint32_t *bt = SiggTables<K>::base();
     if (hA >= 0) sigg_store<0, sg_comb6(MA)>(acc, y, (long long)ar * nb + bt[hA * 16] / 128, v);
@@ -214,9 +207,12 @@ template <int K>
 __global__ void __launch_bounds__(kSigWThreads) k_sigg_sparse(SigvArgs a)
 {
     extern __shared__ double sm[];
-    // XCD-aware order: the 4 sub-chunks of one alpha row run on one XCD
-    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3;
-    const int ar = xcd + 8 * (k >> 2), sub = k & 3;
+    // Block order: every alpha row of sub-chunk 0 (vectors 0-15), then of
+    // sub-chunk 1, ...: a sub-chunk's 128-byte lines of the chunk (N x 128 B,
+    // 109 MB at 12 orbitals) stay in the 256 MB last-level cache while its 37
+    // readers per source row stage them.  Rows are spread over the 8 XCDs.
+    const int bid = blockIdx.x, xcd = bid & 7, k = bid >> 3, nr = (a.m.na + 7) / 8;
+    const int sub = k / nr, ar = xcd + 8 * (k - sub * nr);
     if (ar >= a.m.na) return;
     const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigWVC;
     const double *__restrict__ x = a.x + coff;
