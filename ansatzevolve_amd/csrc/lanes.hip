@@ -214,22 +214,22 @@ __device__ __forceinline__ void rot_inplace(double &p, double &q, double C, doub
 __device__ __forceinline__ void rot2_br_inplace(double &xp, double &xq, double &lp, double &lq, double &brr,
                                                 double C, double sS)
 {
-    double t1, t2, u2, t3, u3;
-    // operands: %0 xp, %1 xq, %2 lp, %3 lq, %4 brr, %5-%9 temporaries,
-    // %10 C, %11 sS
-    asm("v_mul_f64 %5, %2, %1\n\t"
-        "v_mul_f64 %6, %11, %3\n\t"
-        "v_mul_f64 %7, %11, %2\n\t"
-        "v_mul_f64 %8, %11, %1\n\t"
-        "v_mul_f64 %9, %11, %0\n\t"
-        "v_fma_f64 %5, -%3, %0, %5\n\t"
-        "v_fma_f64 %2, %10, %2, %6\n\t"
-        "v_fma_f64 %3, %10, %3, -%7\n\t"
-        "v_fma_f64 %0, %10, %0, %8\n\t"
-        "v_fma_f64 %1, %10, %1, -%9\n\t"
-        "v_add_f64 %4, %4, %5"
-        : "+v"(xp), "+v"(xq), "+v"(lp), "+v"(lq), "+v"(brr), "=&v"(t1), "=&v"(t2), "=&v"(u2), "=&v"(t3),
-          "=&v"(u3)
+    double t2, u2, t3, u3;
+    // operands: %0 xp, %1 xq, %2 lp, %3 lq, %4 brr, %5-%8 temporaries,
+    // %9 C, %10 sS.  The bracket goes straight into brr by two FMAs on the
+    // old values (10 FP64 instructions per pair instead of 11: the sweeps
+    // are about as much FP64-issue as memory bound)
+    asm("v_fma_f64 %4, %2, %1, %4\n\t"
+        "v_mul_f64 %5, %10, %3\n\t"
+        "v_mul_f64 %6, %10, %2\n\t"
+        "v_mul_f64 %7, %10, %1\n\t"
+        "v_mul_f64 %8, %10, %0\n\t"
+        "v_fma_f64 %4, -%3, %0, %4\n\t"
+        "v_fma_f64 %2, %9, %2, %5\n\t"
+        "v_fma_f64 %3, %9, %3, -%6\n\t"
+        "v_fma_f64 %0, %9, %0, %7\n\t"
+        "v_fma_f64 %1, %9, %1, -%8"
+        : "+v"(xp), "+v"(xq), "+v"(lp), "+v"(lq), "+v"(brr), "=&v"(t2), "=&v"(u2), "=&v"(t3), "=&v"(u3)
         : "v"(C), "v"(sS));
 }
 
@@ -272,15 +272,16 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
     const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
     const double2 t = csL[kk][lane];
     double br = 0;
-    // the forward sweep rotates all four sub-orbits (skipping them did not
-    // pay there: the branches cost as much as the rotations they save)
-    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br, BR ? (cmask >> (4 * G)) & 0xfu : 0xfu);
+    // both sweeps skip the sub-orbits whose members the orbit type lacks
+    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br, (cmask >> (4 * G)) & 0xfu);
     if (BR) accW[kk * kLanes + lane] += br;
 }
 
 template <bool BR, bool BC>
-// forward: at most 80 VGPRs (6 waves per SIMD); adjoint: 96 (5 waves)
-__global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
+// forward: at most 64 VGPRs (8 waves per SIMD, one orbit per wave, absent
+// sub-orbits skipped: 6.6 ms per step, against 7.2 ms with two orbits per
+// wave, every sub-orbit rotated and 6 waves); adjoint: 96 (5 waves)
+__global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
             const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
 {
@@ -310,10 +311,9 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 6)
     const int mainA = a.nAm * a.nBc;
     const int total = mainA + (a.nAc - a.nAm) * a.nBm;
     const int stride = gridDim.x * WPB;
-    // the forward sweep takes two orbits per iteration (both loaded before
-    // either is computed: twice the bytes in flight per wave); the adjoint
-    // sweep, with twice the registers per orbit, one
-    constexpr int NO = BR ? 1 : 2;
+    // one orbit per wave iteration (two per iteration, loaded together,
+    // needed more registers than 8 waves per SIMD allow)
+    constexpr int NO = 1;
     for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
         int rA[NO][4], rB[NO][4], type[NO];
         uint32_t spect[NO], cmask[NO];
