@@ -144,6 +144,13 @@ def warm_instance(index, n, family="smooth"):
         lam = float(rng.uniform(0.6, 1.4))
         h, g = synthetic_integrals(n, WARM_SMOOTH_SEED + n, spacing=spacing)
         g = lam * g
+    elif family == "line":
+        # one parameter: the smooth family's base integrals at site spacings
+        # spread quasi-randomly over [1.0, 2.5] (held-out instances fall
+        # between training ones)
+        lam = 1.0
+        spacing = 1.0 + 1.5 * float(np.mod((index + 1) * 0.6180339887498949, 1.0))
+        h, g = synthetic_integrals(n, WARM_SMOOTH_SEED + n, spacing=spacing)
     else:
         raise ValueError("unknown instance family %r" % family)
     ops, order, scale = tups_operators(n, n // 2)
@@ -238,6 +245,35 @@ def consistent_targets(insts, sols, neighbours=2, log=None):
     return out
 
 
+def continuation_targets(insts, sols, log=None):
+    """Training targets along one branch of minima: per orbital count the
+    training instances in order of site spacing, the first from its plain
+    optimum, every next one optimised from the previous one's angles.  The
+    optimiser from the standard guess lands in different minima on
+    neighbouring instances; a branch followed by continuation varies
+    smoothly with the spacing, so a regression can interpolate it.  The
+    branch is followed from both ends of the spacing range; per orbital
+    count the one with the lower total energy is kept (taking the lower of
+    the two per instance would jump where they cross)."""
+    out = [(np.asarray(t), float(E)) for t, E, _ in sols]
+    for n in sorted({w["n"] for w in insts}):
+        idx = sorted([i for i, w in enumerate(insts) if w["n"] == n], key=lambda i: insts[i]["spacing"])
+        best = None
+        for order in (idx, idx[::-1]):
+            br = {order[0]: out[order[0]]}
+            for a, b in zip(order[:-1], order[1:]):
+                th, E, _ = solve(insts[b], br[a][0])
+                br[b] = (np.asarray(th), float(E))
+            total = sum(E for _, E in br.values())
+            if best is None or total < best[0]:
+                best = (total, br)
+        for i, t in best[1].items():
+            out[i] = t
+        if log:
+            log("[warmstart] continuation targets n=%d: %d instances, total E %.6f" % (n, len(idx), best[0]))
+    return out
+
+
 def start_energies(insts, starts):
     """Exact FP64 energies of candidate starts, starts[i] (k x P_i) for
     instance i: one launch over all instances (uneven sizes)."""
@@ -296,7 +332,7 @@ def finetune_energy(model, insts, steps=300, lr=3e-4, log=None):
 
 
 def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidden=256, log=print,
-                 family="smooth", neighbours=2):
+                 family="smooth", neighbours=2, targets_kind="consistent"):
     """Generate (plain optimiser from the standard guess theta = 0; for the
     training set also from neighbours' optima, keeping the lowest minimum),
     train (supervised on the optimised angles, then energy fine-tuning), and
@@ -321,7 +357,9 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
         if k % 20 == 0:
             log("[warmstart] plain solves %d/%d (%.0f s)" % (k + 1, len(insts), time.time() - t0))
     targets = [(plain[k][0], plain[k][1]) for k in range(ntr)]
-    if neighbours:
+    if targets_kind == "continuation":
+        targets = continuation_targets(train, plain[:ntr], log=log)
+    elif neighbours:
         targets = consistent_targets(train, plain[:ntr], neighbours=neighbours, log=log)
     improved = int(sum(targets[k][1] < plain[k][1] - 1e-9 for k in range(ntr)))
     gen_s = time.time() - t0
@@ -360,12 +398,14 @@ def run_pipeline(ns, count, train_frac=0.75, epochs=300, energy_steps=300, hidde
     def summ(sel=None):
         rs = [r for r in rows if sel is None or sel(r)]
         return dict(count=len(rs), median_t_plain=med("t_plain", sel), median_t_mlp=med("t_mlp", sel),
+                    median_t_plain_conv=med("t_plain_conv", sel), median_t_mlp_conv=med("t_mlp_conv", sel),
                     median_it_plain=med("it_plain", sel), median_it_mlp=med("it_mlp", sel),
                     plain_misses=int(sum(np.isinf(r["t_plain"]) for r in rs)),
                     mlp_misses=int(sum(np.isinf(r["t_mlp"]) for r in rs)),
                     proposals_used=int(sum(r["used_mlp"] for r in rs)),
                     median_start_gap_std=med("gap_std", sel), median_start_gap_chosen=med("gap_start", sel))
-    summary = dict(ns=list(ns), family=family, neighbours=neighbours, targets_improved=improved,
+    summary = dict(ns=list(ns), family=family, neighbours=neighbours, targets=targets_kind,
+                   targets_improved=improved,
                    instances=len(insts), train=ntr, held_out=len(held), generate_s=gen_s,
                    train_s=train_s, supervised_loss=[float(losses[0]), float(losses[-1])],
                    energy_finetune_mean_E=[ehist[0], ehist[-1]] if ehist else None, all=summ(), rows=rows)
@@ -382,14 +422,18 @@ def _main():
     p.add_argument("--ns", type=int, nargs="+", default=[6, 8])
     p.add_argument("--count", type=int, default=80, help="instances per orbital count")
     p.add_argument("--epochs", type=int, default=300, help="supervised epochs")
-    p.add_argument("--energy-steps", type=int, default=300, help="energy fine-tuning steps")
-    p.add_argument("--family", default="smooth", choices=["smooth", "random"], help="instance family")
+    p.add_argument("--energy-steps", type=int, default=0,
+                   help="energy fine-tuning steps (it moves proposals off the branch the targets follow)")
+    p.add_argument("--family", default="line", choices=["smooth", "random", "line"], help="instance family")
+    p.add_argument("--targets", default="continuation", choices=["consistent", "continuation"],
+                   help="training targets: lowest of neighbours' optima, or one branch by continuation")
     p.add_argument("--neighbours", type=int, default=2,
                    help="training targets: also start from this many neighbours' optima (0: plain only)")
     p.add_argument("--out", default="")
     a = p.parse_args()
     s = run_pipeline(a.ns, a.count, epochs=a.epochs, energy_steps=a.energy_steps, family=a.family,
-                     neighbours=a.neighbours, log=lambda m: print(m, file=sys.stderr, flush=True))
+                     neighbours=a.neighbours, targets_kind=a.targets,
+                     log=lambda m: print(m, file=sys.stderr, flush=True))
     txt = json.dumps(s)
     if a.out:
         with open(a.out, "w") as f:

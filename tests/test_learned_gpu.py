@@ -114,3 +114,25 @@ def test_warm_start_pipeline_gain(gpu):
     print("\nwarm start (4 orbitals, 16 held out): %s" % {k: a[k] for k in a})
     assert a["proposals_used"] >= len(rows) // 2
     assert a["median_start_gap_chosen"] < a["median_start_gap_std"]
+
+
+def test_warm_start_continuation_targets(gpu):
+    """The one-parameter family (site spacing only) with training targets
+    that follow one branch of minima by continuation (each training
+    instance optimised from its spacing neighbour's optimum, the branch with
+    the lower total energy of the two directions): the targets are smooth in
+    the spacing, the network fits them closely, and its proposals start much
+    nearer a minimum than the standard guess."""
+    from ansatzevolve_amd.learned import run_pipeline
+    s = run_pipeline([4], 48, epochs=150, energy_steps=0, family="line", targets_kind="continuation",
+                     log=lambda m: None)
+    rows = s["rows"]
+    assert len(rows) == 12 and s["targets"] == "continuation"
+    for r in rows:
+        assert r["used_mlp"] == (r["E_mlp_start"] <= r["E_std_start"])
+    lo, hi = s["supervised_loss"]
+    assert hi < 0.2 * lo
+    a = s["all"]
+    print("\nwarm start, continuation targets (4 orbitals, 12 held out): %s" % {k: a[k] for k in a})
+    assert a["median_start_gap_chosen"] < a["median_start_gap_std"]
+
