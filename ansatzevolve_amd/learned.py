@@ -120,7 +120,7 @@ class WarmStartMLP:
 # ------------------------------------------------------------------ pipeline
 # Instances of the learned warm-start workload: n spatial orbitals at half
 # filling, TUPS with L = n / 2 layers, the alternating determinant as the
-# start.  Two families:
+# start.  Three families:
 #   "random": the seeded integral generator with its own seed per instance
 #     (site spacing s ~ U(1.0, 2.5)): every instance has its own random
 #     two-electron factors;
@@ -128,7 +128,9 @@ class WarmStartMLP:
 #     instance is fixed by two continuous parameters, the site spacing
 #     s ~ U(1.0, 2.5) and a scale of the two-electron integrals
 #     lambda ~ U(0.6, 1.4) (a chain stretched or compressed, with stronger or
-#     weaker interaction).
+#     weaker interaction);
+#   "line": the smooth family at lambda = 1, so the site spacing alone fixes
+#     an instance.
 WARM_SEED = 20260500
 WARM_SMOOTH_SEED = 20260700
 
