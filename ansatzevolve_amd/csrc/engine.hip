@@ -106,6 +106,11 @@ struct Seg {
     int prog3Off = -1;         // three-group segment: offset of its run-2 words
     int split2 = 0;            // three-group segment: operators before run 2
     int grid = 1;  // lane layout: blocks of its launches
+    // lane-layout forward two-group segment with reachable amplitudes only:
+    // its orbit and lone lists in d_suplist (supOff < 0: every orbit) and
+    // its grid; adjoint: its orbit bits in d_supmask (maskOff < 0: none)
+    int supOff = -1, supN = 0, supLOff = 0, supLN = 0, gridSup = 1;
+    long long maskOff = -1;
 };
 
 // the two-half lane-layout sparse sigma kernel for (same-half, cross-half)
@@ -233,6 +238,21 @@ struct EngineImpl {
     bool ckOn = false;     // the uploaded batch uses stored states
     DBuf<double> bl_ck;
     int numCU = 1;
+    // kOptReachable: lane-layout sweeps skip the amplitudes that the start
+    // state and the operators so far cannot reach (exact zeros).  supDirty:
+    // the lists must be recomputed (new segments or start state); supGen
+    // counts recomputations; the stored states are cleared once per (buffer,
+    // chunks, lists) key, since the forward sweep then writes only the
+    // reachable part of each
+    int reachOpt = 1;
+    bool supDirty = true;
+    long long supGen = 0;
+    std::vector<double> hstart;
+    DBuf<int> d_suplist;
+    DBuf<uint32_t> d_supmask;
+    const double *ckZeroP = nullptr;
+    size_t ckZeroNch = 0;
+    long long ckZeroGen = -1;
 
     // tunable execution choices (Engine::set_option)
     int sweepKind = kSweepOrbit;
@@ -452,6 +472,7 @@ struct EngineImpl {
     // operators run one operator per pass (k_opv).
     bool build_lane_segments(std::string &err)
     {
+        supDirty = true;
         if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
         // Every launch is one grid-stride pass of exactly as many blocks as
         // the chip holds at once for that kernel (its occupancy x CUs): a
@@ -491,6 +512,127 @@ struct EngineImpl {
             for (int k = sg.k0; k < sg.k1; k++) rows[k] = sg.grid;
         if (!d_lrows.ensure(std::max(M, 1), err)) return false;
         HCHK(hipMemcpy(d_lrows.p, rows.data(), M * sizeof(int), hipMemcpyHostToDevice));
+        return true;
+    }
+
+    // Reachable amplitudes of the lane-layout sweeps.  S_0 = the start
+    // state's nonzero determinants; a two-group forward segment s acts on
+    // an orbit only if some member is in S_s (the others hold exact zeros
+    // and stay zero), and S_{s+1} = S_s + every member of those orbits.  Its
+    // list holds those orbits and the lone determinants in S_s.  An
+    // adjoint segment (stored states, the same segments) reads the stored
+    // state s + 1 only on orbits meeting S_{s+1}.  A segment of another
+    // kind makes everything reachable from there on.
+    bool compute_support(std::string &err)
+    {
+        supDirty = false;
+        supGen++;
+        for (Seg &sg : laneF.segs) sg.supOff = -1;
+        for (Seg &sg : laneA.segs) sg.maskOff = -1;
+        if (!reachOpt || (long long)hstart.size() != N) return true;
+        std::vector<uint8_t> S(N);
+        for (long long i = 0; i < N; i++) S[i] = hstart[i] != 0.0;
+        std::vector<std::vector<int4>> rk(gsets.size());
+        auto ranks = [&](int g) -> const std::vector<int4> & {
+            std::vector<int4> &r = rk[g];
+            if (r.empty() && gsets[g]->ncls > 0) {
+                r.resize(gsets[g]->ncls);
+                if (hipMemcpy(r.data(), gsets[g]->ranks.p, r.size() * sizeof(int4), hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+                    r.clear();
+            }
+            return r;
+        };
+        // the orbit index space of k_orb2v: alpha classes [0, nAm) x every
+        // beta class, then [nAm, nAc) x [0, nBm); lone: [nAm, nAc) x [nBm, nBc)
+        auto for_orbit = [&](const GroupSet &A, const GroupSet &B, const std::vector<int4> &ra,
+                             const std::vector<int4> &rb, int t, auto &&f) {
+            const int mainA = A.nm * B.ncls;
+            int ia, ib;
+            if (t < mainA) {
+                ia = t / B.ncls;
+                ib = t - ia * B.ncls;
+            } else {
+                const int u = t - mainA;
+                ia = A.nm + u / B.nm;
+                ib = u - (ia - A.nm) * B.nm;
+            }
+            const int a4[4] = {ra[ia].x, ra[ia].y, ra[ia].z, ra[ia].w};
+            const int b4[4] = {rb[ib].x, rb[ib].y, rb[ib].z, rb[ib].w};
+            for (int ma = 0; ma < 4; ma++)
+                for (int mb = 0; mb < 4; mb++)
+                    if (a4[ma] >= 0 && b4[mb] >= 0) f((long long)a4[ma] * nb + b4[mb]);
+        };
+        std::vector<int> lists;
+        std::vector<uint32_t> masks;
+        bool full = false;
+        const bool adj = ckShape;
+        for (size_t si = 0; si < laneF.segs.size(); si++) {
+            Seg &sg = laneF.segs[si];
+            if (!full && (!sg.two || sg.three)) {
+                full = true;
+                std::fill(S.begin(), S.end(), (uint8_t)1);
+            }
+            if (full) break;
+            const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
+            const std::vector<int4> &ra = ranks(sg.gA), &rb = ranks(sg.gB);
+            if (ra.empty() || rb.empty()) {
+                err = "reading class ranks failed";
+                return false;
+            }
+            const int total = A.nm * B.ncls + (A.ncls - A.nm) * B.nm;
+            std::vector<int> act;
+            for (int t = 0; t < total; t++) {
+                bool any = false;
+                for_orbit(A, B, ra, rb, t, [&](long long d) { any = any || S[d]; });
+                if (any) act.push_back(t);
+            }
+            const int nBl = B.ncls - B.nm, nl = (A.ncls - A.nm) * nBl;
+            std::vector<int> lone;
+            for (int t = 0; t < nl; t++) {
+                const int i = t / nBl, ia = A.nm + i, ib = B.nm + (t - i * nBl);
+                if (S[(long long)ra[ia].x * nb + rb[ib].x]) lone.push_back(t);
+            }
+            for (int t : act) for_orbit(A, B, ra, rb, t, [&](long long d) { S[d] = 1; });
+            if ((int)act.size() < total || (int)lone.size() < nl) {
+                sg.supOff = (int)lists.size();
+                sg.supN = (int)act.size();
+                lists.insert(lists.end(), act.begin(), act.end());
+                sg.supLOff = (int)lists.size();
+                sg.supLN = (int)lone.size();
+                lists.insert(lists.end(), lone.begin(), lone.end());
+                const int wpb = kBlock / kLanes;
+                const long long want = (std::max<long long>(sg.supN, (sg.supLN + 3) / 4) + wpb - 1) / wpb;
+                sg.gridSup = (int)std::max<long long>(1, std::min<long long>(sg.grid, want));
+            }
+            // the adjoint segment reading the state this one writes
+            if (adj && si < laneA.segs.size() && laneA.segs[si].two && !laneA.segs[si].three) {
+                Seg &sa = laneA.segs[si];
+                const GroupSet &AA = *gsets[sa.gA], &BB = *gsets[sa.gB];
+                const std::vector<int4> &raa = ranks(sa.gA), &rbb = ranks(sa.gB);
+                const int tot = AA.nm * BB.ncls + (AA.ncls - AA.nm) * BB.nm;
+                std::vector<uint32_t> bits((tot + 31) / 32, 0u);
+                bool all = true;
+                for (int t = 0; t < tot; t++) {
+                    bool any = false;
+                    for_orbit(AA, BB, raa, rbb, t, [&](long long d) { any = any || S[d]; });
+                    if (any) bits[t >> 5] |= 1u << (t & 31);
+                    else all = false;
+                }
+                if (!all) {
+                    sa.maskOff = (long long)masks.size();
+                    masks.insert(masks.end(), bits.begin(), bits.end());
+                }
+            }
+        }
+        if (!lists.empty()) {
+            if (!d_suplist.ensure(lists.size(), err)) return false;
+            HCHK(hipMemcpy(d_suplist.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (!masks.empty()) {
+            if (!d_supmask.ensure(masks.size(), err)) return false;
+            HCHK(hipMemcpy(d_supmask.p, masks.data(), masks.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
         return true;
     }
 
@@ -831,6 +973,16 @@ struct EngineImpl {
         a.reverse = reverse ? 1 : 0;
         a.prog = L.prog2.p + sg.prog2Off;
         a.split = sg.progOff;
+        a.list = a.llist = nullptr;
+        a.nlist = a.nllist = 0;
+        a.live = nullptr;
+        if (!reverse && sg.supOff >= 0) {
+            a.list = d_suplist.p + sg.supOff;
+            a.llist = d_suplist.p + sg.supLOff;
+            a.nlist = sg.supN;
+            a.nllist = sg.supLN;
+        }
+        if (reverse && ckOn && sg.maskOff >= 0) a.live = d_supmask.p + sg.maskOff;
         return a;
     }
 
@@ -1193,6 +1345,8 @@ struct EngineImpl {
 
     // state s of the stored forward states (checkpoint mode)
     double *ck_state(int s, int nch) { return bl_ck.p + (size_t)s * nch * N * kLanes; }
+    // forward two-group launch grid (smaller with a reachable-orbit list)
+    static int fgrid(const Seg &sg) { return sg.supOff >= 0 ? sg.gridSup : sg.grid; }
 
     // Whether the first forward segment reads the initial state directly (one
     // plain vector broadcast to every lane) instead of a lane-layout state 0:
@@ -1216,7 +1370,7 @@ struct EngineImpl {
             double *out = ckOn ? ck_state((int)si + 1, nch) : v;
             if (si == 0 && first_reads_start()) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orb2v<false, true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orb2v<false, true>), dim3(fgrid(sg), nch), dim3(kBlock), 0, stream,
                                    orb2args(laneF, sg, false), d_start.p, out, nullptr, cstride, bl_cs.p, M, nullptr,
                                    lslotsF);
                 mark_end(mk);
@@ -1229,7 +1383,7 @@ struct EngineImpl {
                 LCHK();
             } else if (sg.two) {
                 const int mk = mark_begin(1);
-                hipLaunchKernelGGL((k_orb2v<false>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_orb2v<false>), dim3(fgrid(sg), nch), dim3(kBlock), 0, stream,
                                    orb2args(laneF, sg, false), in, out, nullptr, cstride, bl_cs.p, M, nullptr,
                                    lslotsF);
                 mark_end(mk);
@@ -1273,9 +1427,13 @@ struct EngineImpl {
                 LCHK();
             } else if (sg.two) {
                 const int mk = mark_begin(2);
-                hipLaunchKernelGGL((k_orb2v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream,
-                                   orb2args(laneA, sg, true), in, out, lam, cstride, bl_csneg.p, M, bl_partial.p,
-                                   lslots);
+                const Orb2Args oa = orb2args(laneA, sg, true);
+                if (oa.live)
+                    hipLaunchKernelGGL((k_orb2v<true, false, true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, oa,
+                                       in, out, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                else
+                    hipLaunchKernelGGL((k_orb2v<true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, oa, in, out,
+                                       lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
                 mark_end(mk);
                 LCHK();
             } else if (sg.orbA >= 0) {
@@ -1515,6 +1673,7 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
         e.sweepKind = v == "tile" ? kSweepTile : v == "orblds" ? kSweepOrbitLds : kSweepOrbit;
     }
     if (const char *l = getenv("AE_LANES")) e.laneMin = std::max(0, atoi(l));
+    if (const char *r = getenv("AE_REACH")) e.reachOpt = atoi(r) ? 1 : 0;
     e.nq = nq;
     e.mode = mode;
     e.nel = nel;
@@ -1714,6 +1873,8 @@ bool Engine::set_start(const std::vector<double> &start, std::string &err)
     e.drop_graphs();
     HCHK(hipMemcpy(e.d_start.p, start.data(), 8 * e.N, hipMemcpyHostToDevice));
     e.haveStart = true;
+    e.hstart = start;
+    e.supDirty = true;
     return true;
 }
 
@@ -2048,6 +2209,18 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         }
         if (ck != e.ckOn || e.bl_ck.p != ckWas) e.drop_graphs();
         e.ckOn = ck;
+        if (e.supDirty) {
+            if (!e.compute_support(err)) return false;
+            e.drop_graphs();
+        }
+        if (ck && (e.ckZeroP != e.bl_ck.p || e.ckZeroNch != nch || e.ckZeroGen != e.supGen)) {
+            // the forward sweep writes only the reachable part of each state
+            HCHK(hipMemsetAsync(e.bl_ck.p, 0, e.bl_ck.n * sizeof(double), e.stream));
+            HCHK(hipStreamSynchronize(e.stream));
+            e.ckZeroP = e.bl_ck.p;
+            e.ckZeroNch = nch;
+            e.ckZeroGen = e.supGen;
+        }
         if (!e.upload_cs_lanes(angles, nvec, err)) return false;
     } else {
         if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
@@ -2289,6 +2462,17 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.siggOpt = value;
         e.drop_graphs();
         return true;
+    case kOptReachable:
+        if (value < 0 || value > 1) {
+            err = "reachable option must be 0 or 1";
+            return false;
+        }
+        if (value == e.reachOpt) return true;
+        e.reachOpt = value;
+        e.supDirty = true;
+        e.bnvec = 0;  // lists and stored states are renewed at the next upload
+        e.drop_graphs();
+        return true;
     case kOptCheckpoint:
         if (value < 0 || value > 1) {
             err = "checkpoint option must be 0 or 1";
@@ -2334,6 +2518,7 @@ int Engine::option(int key) const
     case kOptSigmaGroupedAvailable: return e.sigg ? 1 : 0;
     case kOptGemmSolutionBeta: return e.gemmSol[0];
     case kOptGemmSolutionAlpha: return e.gemmSol[1];
+    case kOptReachable: return e.reachOpt;
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

@@ -47,6 +47,8 @@ enum EngineOption : int {
     kOptSigmaGroupedAvailable = 15,  // read only: 1 when the grouped sigma applies (12 orbitals, 6 beta)
     kOptGemmSolutionBeta = 16,   // rocBLAS solution index of the lane sigma's beta-part DGEMM (0: default)
     kOptGemmSolutionAlpha = 17,  // the same for the alpha-part DGEMM (an index the library lacks: default)
+    kOptReachable = 18,          // 1 (default): lane-layout sweeps skip orbits that the start state and the
+                                 // operators before them cannot reach (exact zeros); 0: every orbit
 };
 
 class Engine {

@@ -208,12 +208,23 @@ struct Orb2Args {
     int k0, k1, reverse;
     const unsigned long long *prog;
     int split;  // operators k0 .. k0+split-1 act on groups (0, 0), the rest on (1, 1)
+    // forward, reachable amplitudes only (list != nullptr): the orbits
+    // list[0 .. nlist) and the lone determinants llist[0 .. nllist) (indices
+    // into the orbit and lone index spaces); the others hold zero on input
+    // and output and are not touched
+    const int *list, *llist;
+    int nlist, nllist;
+    // adjoint (live != nullptr): bit t of live is 0 when every member of
+    // orbit t holds zero in the stored forward state (v0), which is then
+    // not read
+    const uint32_t *live;
 };
 // BR = adjoint sweep: operators in reverse order with brackets (a.reverse is
 // ignored); !BR: forward order.  BC (forward, out of place): v0 is one plain
 // vector of N amplitudes read by every lane (the initial state), not a lane-
 // layout chunk.
-template <bool BR, bool BC = false>
+// LV (adjoint): a.live marks the orbits whose stored state is read.
+template <bool BR, bool BC = false, bool LV = false>
 __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 // Three-group orbit segment in lane layout: three consecutive runs of

@@ -277,7 +277,7 @@ __device__ __forceinline__ void orb2_op(const Orb2Args &a, const unsigned long l
     if (BR) accW[kk * kLanes + lane] += br;
 }
 
-template <bool BR, bool BC>
+template <bool BR, bool BC, bool LV>
 // forward: at most 64 VGPRs (8 waves per SIMD, one orbit per wave, absent
 // sub-orbits skipped: 6.6 ms per step, against 7.2 ms with two orbits per
 // wave, every sub-orbit rotated and 6 waves); adjoint: 96 (5 waves)
@@ -309,7 +309,8 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     // orbits with some two-member class: alpha classes [0, nAm) with every
     // beta class, then the single-member alpha classes with beta [0, nBm)
     const int mainA = a.nAm * a.nBc;
-    const int total = mainA + (a.nAc - a.nAm) * a.nBm;
+    const int *list = BR ? nullptr : a.list;  // reachable-orbit lists: forward only
+    const int total = list ? a.nlist : mainA + (a.nAc - a.nAm) * a.nBm;
     const int stride = gridDim.x * WPB;
     // one orbit per wave iteration (two per iteration, loaded together,
     // needed more registers than 8 waves per SIMD allow)
@@ -317,12 +318,14 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
         int rA[NO][4], rB[NO][4], type[NO];
         uint32_t spect[NO], cmask[NO];
-        bool live[NO], act[NO];
+        bool live[NO], act[NO], psiIn[NO];
 #pragma unroll
         for (int o = 0; o < NO; o++) {
             const int t = t0 + o * stride;
             live[o] = t < total;
-            const int tt = live[o] ? t : t0;
+            const int tt = list ? list[live[o] ? t : t0] : (live[o] ? t : t0);
+            // adjoint: whether the stored state is nonzero on this orbit
+            psiIn[o] = !LV || ((ld_const(a.live, tt >> 5) >> (tt & 31)) & 1u);
             int ia, ib;
             if (tt < mainA) {
                 ia = tt / a.nBc;
@@ -358,16 +361,33 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
         // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
         double x[NO][16], l[NO][16];
 #pragma unroll
-        for (int o = 0; o < NO; o++)
+        for (int o = 0; o < NO; o++) {
+            // one wave-uniform branch per orbit, in the LV kernel only (a
+            // per-member condition on the psi loads alone split every
+            // member's loads in two regions and slowed the adjoint sweep by
+            // 4%; the branch in every segment's kernel, by 2%)
+            if (!LV || psiIn[o]) {
 #pragma unroll
-            for (int s = 0; s < 16; s++) {
-                const int ma = s & 3, mb = s >> 2;
-                const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
-                const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                if (BC) x[o][s] = ok ? v0[off / kLanes] : 0.0;
-                else x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
-                l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
+                for (int s = 0; s < 16; s++) {
+                    const int ma = s & 3, mb = s >> 2;
+                    const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
+                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                    if (BC) x[o][s] = ok ? v0[off / kLanes] : 0.0;
+                    else x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
+                    l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
+                }
+            } else {
+                // adjoint, stored state zero on this orbit: lambda only
+#pragma unroll
+                for (int s = 0; s < 16; s++) {
+                    const int ma = s & 3, mb = s >> 2;
+                    const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
+                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                    x[o][s] = 0.0;
+                    l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
+                }
             }
+        }
 #pragma unroll
         for (int o = 0; o < NO; o++) {
             if (!act[o]) continue;
@@ -416,13 +436,13 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     }
     if (copyAll) {
         // lone determinants: copied, four per wave iteration (all loads first)
-        const int nAl = a.nAc - a.nAm, nBl = a.nBc - a.nBm, nl = nAl * nBl;
+        const int nAl = a.nAc - a.nAm, nBl = a.nBc - a.nBm, nl = list ? a.nllist : nAl * nBl;
         for (int t0 = blockIdx.x * WPB + wv; t0 < nl; t0 += 4 * stride) {
             uint32_t off[4];
             double v[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int t = t0 + q * stride, tt = t < nl ? t : t0;
+                const int t = t0 + q * stride, tt = list ? a.llist[t < nl ? t : t0] : (t < nl ? t : t0);
                 const int i = tt / nBl, ia = a.nAm + i, ib = a.nBm + (tt - i * nBl);
                 const int ra = a.aRanks[ia].x, rb = a.bRanks[ib].x;
                 off[q] = ((uint32_t)ra * (uint32_t)a.nb + (uint32_t)rb) * kLanes;
@@ -445,12 +465,14 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     }
 }
 
-template __global__ void k_orb2v<false, false>(Orb2Args, const double *, double *, double *, long long,
-                                               const double2 *, int, double *, int);
-template __global__ void k_orb2v<false, true>(Orb2Args, const double *, double *, double *, long long,
-                                              const double2 *, int, double *, int);
-template __global__ void k_orb2v<true, false>(Orb2Args, const double *, double *, double *, long long,
-                                              const double2 *, int, double *, int);
+template __global__ void k_orb2v<false, false, false>(Orb2Args, const double *, double *, double *, long long,
+                                                      const double2 *, int, double *, int);
+template __global__ void k_orb2v<false, true, false>(Orb2Args, const double *, double *, double *, long long,
+                                                     const double2 *, int, double *, int);
+template __global__ void k_orb2v<true, false, false>(Orb2Args, const double *, double *, double *, long long,
+                                                     const double2 *, int, double *, int);
+template __global__ void k_orb2v<true, false, true>(Orb2Args, const double *, double *, double *, long long,
+                                                    const double2 *, int, double *, int);
 
 // ------------------------------------------------ three-group segments
 // f(Idx<I>) for I = B .. E-1, written out at compile time (no loop)

@@ -207,8 +207,13 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *  16, 17 rocBLAS solution index of the lane-layout sigma's beta-part and
  *     alpha-part DGEMMs (0, the default: the library's own choice; the
  *     autotuner times every solution and keeps the fastest; an index the
- *     library does not offer falls back to 0).
- * Changing keys 0, 3, 4, 7, 9 or 10 re-segments the operators or changes the
+ *     library does not offer falls back to 0);
+ *  18 1 (default): the lane-layout forward sweep acts only on the orbits
+ *     that the start state and the operators before each segment can reach
+ *     (all other amplitudes are exact zeros), and the adjoint sweep reads
+ *     stored forward states only there; 0: every orbit.  Results are the
+ *     same bit for bit.
+ * Changing keys 0, 3, 4, 7, 9, 10 or 18 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */
 int setEngineOption(int key, int value, void *ctx);

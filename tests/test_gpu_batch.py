@@ -159,6 +159,38 @@ def test_stored_forward_states(gpu, groups):
             assert abs(E1[v] - Es) <= SAME and np.max(np.abs(G1[v] - gs)) <= SAME
 
 
+@pytest.mark.parametrize("ck", [0, 1], ids=["inplace", "stored"])
+def test_reachable_orbits_exact(gpu, ck):
+    """Forward sweeps that act only on the orbits which the start state and
+    the operators before them can reach, and adjoint sweeps that read the
+    stored states only there, against sweeps over every orbit: the skipped
+    amplitudes are exact zeros, so the results are the same bit for bit.
+    Again after switching back (stored states written in full, then cleared
+    for the lists) and from a start state of two determinants."""
+    n = 10
+    w, thetas = _tups_case(n, 3, 5151, 70)
+    hf = hf_index(n)
+    other = hf ^ (1 << (n - 2)) ^ (1 << (n - 1))  # an alpha electron moved up, same sector
+    for idx, cf in (([hf + 1], [1.0]), ([hf + 1, other + 1], [0.8, 0.6])):
+        with from_workload(w, storage=2) as a:
+            a.set_initial_state(w["nq"], idx, cf)
+            a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+            a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
+            a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
+            a.set_option(AnsatzSynth.OPT_CHECKPOINT, ck)
+            assert a.get_option(AnsatzSynth.OPT_REACHABLE) == 1
+            E1, G1 = a.energy_gradients(thetas)
+            assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == ck
+            a.set_option(AnsatzSynth.OPT_REACHABLE, 0)
+            E0, G0 = a.energy_gradients(thetas)
+            a.set_option(AnsatzSynth.OPT_REACHABLE, 1)
+            E2, G2 = a.energy_gradients(thetas)
+            assert np.array_equal(E1, E0) and np.array_equal(G1, G0)
+            assert np.array_equal(E2, E0) and np.array_equal(G2, G0)
+            Es, gs = a.energy_gradient(thetas[69])
+            assert abs(E1[69] - Es) <= SAME and np.max(np.abs(G1[69] - gs)) <= SAME
+
+
 @pytest.mark.parametrize("n", [6, 8, 10])
 def test_sigma_halves_match_one_row(gpu, n):
     """The lane-layout sigma with the source row split in two double-buffered
