@@ -274,7 +274,8 @@ class AnsatzSynth:
     (OPT_SWEEP, OPT_LANE_MIN, OPT_GRAPH, OPT_FUSE_DEPTH, OPT_LDS_BUDGET, OPT_HAM_KIND, OPT_SEGMENTS,
      OPT_LANE_GROUPS, OPT_LANE_SEGMENTS, OPT_LANE_GROUPS_FWD, OPT_CHECKPOINT, OPT_CHECKPOINT_ACTIVE,
      OPT_SIGMA_HALVES, OPT_SIGMA_HALVES_AVAILABLE, OPT_SIGMA_GROUPED, OPT_SIGMA_GROUPED_AVAILABLE,
-     OPT_GEMM_SOLUTION_BETA, OPT_GEMM_SOLUTION_ALPHA, OPT_REACHABLE) = range(19)
+     OPT_GEMM_SOLUTION_BETA, OPT_GEMM_SOLUTION_ALPHA, OPT_REACHABLE, OPT_SWEEP_AMP_FORWARD,
+     OPT_SWEEP_AMP_ADJOINT_LAMBDA, OPT_SWEEP_AMP_ADJOINT_STATE) = range(22)
 
     def set_option(self, key, value):
         self._chk("setEngineOption", self.lib.setEngineOption(int(key), int(value), self.ctx))

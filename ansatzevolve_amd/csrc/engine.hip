@@ -250,6 +250,10 @@ struct EngineImpl {
     std::vector<double> hstart;
     DBuf<int> d_suplist;
     DBuf<uint32_t> d_supmask;
+    // amplitudes per vector and step that the lane-layout sweeps move
+    // (summed over segments): forward reads and writes, adjoint lambda reads
+    // and writes, adjoint stored-state reads (kOptSweepAmp*)
+    long long ampF = 0, ampAL = 0, ampAP = 0;
     const double *ckZeroP = nullptr;
     size_t ckZeroNch = 0;
     long long ckZeroGen = -1;
@@ -529,9 +533,13 @@ struct EngineImpl {
         supGen++;
         for (Seg &sg : laneF.segs) sg.supOff = -1;
         for (Seg &sg : laneA.segs) sg.maskOff = -1;
-        if (!reachOpt || (long long)hstart.size() != N) return true;
-        std::vector<uint8_t> S(N);
-        for (long long i = 0; i < N; i++) S[i] = hstart[i] != 0.0;
+        // without lists every amplitude counts as reachable (the amplitude
+        // counts are still computed)
+        const bool lists_on = reachOpt && (long long)hstart.size() == N;
+        std::vector<uint8_t> S(N, (uint8_t)1);
+        if (lists_on)
+            for (long long i = 0; i < N; i++) S[i] = hstart[i] != 0.0;
+        ampF = ampAL = ampAP = 0;
         std::vector<std::vector<int4>> rk(gsets.size());
         auto ranks = [&](int g) -> const std::vector<int4> & {
             std::vector<int4> &r = rk[g];
@@ -567,13 +575,29 @@ struct EngineImpl {
         std::vector<uint32_t> masks;
         bool full = false;
         const bool adj = ckShape;
+        // adjoint lambda: every determinant but the lone ones of two-group segments
+        std::vector<long long> loneA(laneA.segs.size(), 0);
+        for (size_t i = 0; i < laneA.segs.size(); i++) {
+            const Seg &sa = laneA.segs[i];
+            if (sa.two && !sa.three) {
+                const GroupSet &AA = *gsets[sa.gA], &BB = *gsets[sa.gB];
+                loneA[i] = (long long)(AA.ncls - AA.nm) * (BB.ncls - BB.nm);
+            }
+            ampAL += N - loneA[i];
+            if (!adj || !sa.two || sa.three) ampAP += N - loneA[i];  // no mask: every acting orbit
+        }
         for (size_t si = 0; si < laneF.segs.size(); si++) {
             Seg &sg = laneF.segs[si];
             if (!full && (!sg.two || sg.three)) {
                 full = true;
                 std::fill(S.begin(), S.end(), (uint8_t)1);
             }
-            if (full) break;
+            if (full) {
+                ampF += N;
+                if (adj && si < laneA.segs.size() && laneA.segs[si].two && !laneA.segs[si].three)
+                    ampAP += N - loneA[si];
+                continue;
+            }
             const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
             const std::vector<int4> &ra = ranks(sg.gA), &rb = ranks(sg.gB);
             if (ra.empty() || rb.empty()) {
@@ -593,7 +617,11 @@ struct EngineImpl {
                 const int i = t / nBl, ia = A.nm + i, ib = B.nm + (t - i * nBl);
                 if (S[(long long)ra[ia].x * nb + rb[ib].x]) lone.push_back(t);
             }
-            for (int t : act) for_orbit(A, B, ra, rb, t, [&](long long d) { S[d] = 1; });
+            for (int t : act) for_orbit(A, B, ra, rb, t, [&](long long d) {
+                    S[d] = 1;
+                    ampF++;
+                });
+            ampF += (long long)lone.size();
             if ((int)act.size() < total || (int)lone.size() < nl) {
                 sg.supOff = (int)lists.size();
                 sg.supN = (int)act.size();
@@ -615,11 +643,19 @@ struct EngineImpl {
                 bool all = true;
                 for (int t = 0; t < tot; t++) {
                     bool any = false;
-                    for_orbit(AA, BB, raa, rbb, t, [&](long long d) { any = any || S[d]; });
-                    if (any) bits[t >> 5] |= 1u << (t & 31);
-                    else all = false;
+                    long long cnt = 0;
+                    for_orbit(AA, BB, raa, rbb, t, [&](long long d) {
+                        any = any || S[d];
+                        cnt++;
+                    });
+                    if (any) {
+                        bits[t >> 5] |= 1u << (t & 31);
+                        ampAP += cnt;
+                    } else {
+                        all = false;
+                    }
                 }
-                if (!all) {
+                if (!all && lists_on) {
                     sa.maskOff = (long long)masks.size();
                     masks.insert(masks.end(), bits.begin(), bits.end());
                 }
@@ -2489,6 +2525,9 @@ bool Engine::set_option(int key, int value, std::string &err)
     case kOptCheckpointActive:
     case kOptSigmaHalvesAvailable:
     case kOptSigmaGroupedAvailable:
+    case kOptSweepAmpForward:
+    case kOptSweepAmpAdjointLambda:
+    case kOptSweepAmpAdjointState:
         err = "read-only engine property";
         return false;
     default:
@@ -2519,6 +2558,9 @@ int Engine::option(int key) const
     case kOptGemmSolutionBeta: return e.gemmSol[0];
     case kOptGemmSolutionAlpha: return e.gemmSol[1];
     case kOptReachable: return e.reachOpt;
+    case kOptSweepAmpForward: return (int)std::min<long long>(e.ampF, INT32_MAX);
+    case kOptSweepAmpAdjointLambda: return (int)std::min<long long>(e.ampAL, INT32_MAX);
+    case kOptSweepAmpAdjointState: return (int)std::min<long long>(e.ampAP, INT32_MAX);
     case kOptLaneSegments: return (int)e.lsegs.size();
     default: return -1;
     }

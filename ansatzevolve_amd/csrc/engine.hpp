@@ -49,6 +49,10 @@ enum EngineOption : int {
     kOptGemmSolutionAlpha = 17,  // the same for the alpha-part DGEMM (an index the library lacks: default)
     kOptReachable = 18,          // 1 (default): lane-layout sweeps skip orbits that the start state and the
                                  // operators before them cannot reach (exact zeros); 0: every orbit
+    // read only, after a lane-layout upload: amplitudes per vector and step (summed over the segments)
+    kOptSweepAmpForward = 19,        // read and written by the forward sweep
+    kOptSweepAmpAdjointLambda = 20,  // of lambda read and written by the adjoint sweep
+    kOptSweepAmpAdjointState = 21,   // of the forward state read by the adjoint sweep
 };
 
 class Engine {

@@ -166,6 +166,10 @@ def main():
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
     opts["checkpoint_active"] = a.get_option(a.OPT_CHECKPOINT_ACTIVE)
+    opts["reachable"] = a.get_option(a.OPT_REACHABLE)
+    amp = {"forward": a.get_option(a.OPT_SWEEP_AMP_FORWARD),
+           "adjoint_lambda": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_LAMBDA),
+           "adjoint_state": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_STATE)}
     for _ in range(args.warmup):
         a.run_batch(True)
     a.download_batch(True)  # drains the stream
@@ -220,16 +224,28 @@ def main():
                 "launches": launches, "algorithmic_flops_per_launch": flops_launch, "ms_per_step": per_step_ms,
                 "note": note}
 
-    # sweeps: every segment reads and writes each vector once (forward: psi;
-    # adjoint: psi and lambda), 8 B per amplitude per access; with stored
-    # forward states the adjoint reads psi and does not write it back
+    # sweeps: 8 B per amplitude per access.  Vector-major: every segment
+    # reads and writes each vector once (forward: psi; adjoint: psi and
+    # lambda).  Lane layout: the amplitudes the engine reports per step
+    # (summed over the segments): the forward sweep reads and writes the
+    # reachable part of each state, the adjoint sweep reads and writes lambda
+    # on every determinant some operator of the segment acts on, and reads
+    # the stored state there where it can be nonzero (with stored states;
+    # in place it reads and writes psi like lambda)
     k_fwd = ("k_orb2v<false>" if opts["lane_groups_fwd"] == 2 else "k_orbv<false>") + " (lane layout)" if lanes \
         else "k_orb<false>"
     k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
         else "k_orb<true>"
-    rooflines["forward_sweep"] = hbm(k_fwd, 16.0 * N * args.nvec, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
+    if lanes:
+        fwd_launches, rev_launches = max(1, fwd_n // nbd), max(1, rev_n // nbd)
+        fwd_bytes = 16.0 * amp["forward"] * args.nvec / fwd_launches
+        rev_state = 8.0 * amp["adjoint_state"] if opts.get("checkpoint_active") else 16.0 * amp["adjoint_lambda"]
+        rev_bytes = (16.0 * amp["adjoint_lambda"] + rev_state) * args.nvec / rev_launches
+    else:
+        fwd_bytes = 16.0 * N * args.nvec
+        rev_bytes = (24.0 if opts.get("checkpoint_active") else 32.0) * N * args.nvec
+    rooflines["forward_sweep"] = hbm(k_fwd, fwd_bytes, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
                                      fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
-    rev_bytes = (24.0 if opts.get("checkpoint_active") else 32.0) * N * args.nvec
     rooflines["adjoint_sweep"] = hbm(k_rev, rev_bytes, per_launch(rev_ms, rev_n, 1 if lanes else nseg),
                                      rev_n if lanes else rev_n * nseg, rev_ms / nbd)
     if ham_kind == 3 and gemm_n > 0:
@@ -292,6 +308,7 @@ def main():
         "breakdown_ms_per_step": {"hamiltonian": ham_ms / nbd, "same_spin_dgemm": gemm_ms / nbd,
                                   "forward_sweep": fwd_ms / nbd, "adjoint_sweep": rev_ms / nbd},
         "engine_options": opts,
+        "sweep_amplitudes_per_vector_step": amp,
         "autotune": tune,
         "single_vector_egrad_per_s": single_rate,
         "single_vector_breakdown_ms": single_breakdown,

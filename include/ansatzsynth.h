@@ -212,7 +212,11 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     that the start state and the operators before each segment can reach
  *     (all other amplitudes are exact zeros), and the adjoint sweep reads
  *     stored forward states only there; 0: every orbit.  Results are the
- *     same bit for bit.
+ *     same bit for bit;
+ *  19, 20, 21 (read only, after a lane-layout batch upload) amplitudes per
+ *     vector and step, summed over the segments: read and written by the
+ *     forward sweep (19), of the adjoint vector read and written by the
+ *     adjoint sweep (20), of the stored forward states read by it (21).
  * Changing keys 0, 3, 4, 7, 9, 10 or 18 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

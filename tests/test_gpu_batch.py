@@ -191,6 +191,55 @@ def test_reachable_orbits_exact(gpu, ck):
             assert abs(E1[69] - Es) <= SAME and np.max(np.abs(G1[69] - gs)) <= SAME
 
 
+def _reachable_model(n, layers, start):
+    """Amplitude counts of two-unit TUPS segments from a string-level model:
+    a spin string reaches the string with the electron of a half-filled pair
+    moved to the pair's other orbital; the reachable determinants are the
+    products of reachable alpha and beta strings (one start determinant
+    with equal alpha and beta strings); a determinant is lone in a segment
+    when every pair of the segment is empty or full in both strings."""
+    units = [(p, p + 1) for _ in range(layers) for st in (0, 1) for p in range(st, n - 1, 2)]
+    segs = [units[i:i + 2] for i in range(0, len(units), 2)]
+    strs = [s for s in range(1 << n) if bin(s).count("1") == n // 2]
+    N = len(strs) ** 2
+    S = {start}
+    fwd = lam = state = 0
+    for sg in segs:
+        for p, q in sg:
+            S |= {s ^ ((1 << p) | (1 << q)) for s in S if ((s >> p) & 1) != ((s >> q) & 1)}
+        lone = {s for s in strs if all(((s >> p) & 1) == ((s >> q) & 1) for p, q in sg)}
+        fwd += len(S) ** 2
+        lam += N - len(lone) ** 2
+        state += len(S) ** 2 - len(S & lone) ** 2
+    return len(segs), fwd, lam, state, N
+
+
+def test_reachable_amplitude_counts(gpu):
+    """The amplitudes the lane-layout sweeps move per vector and step, as the
+    engine reports them, against the string-level model of the reachable
+    determinants (alternating start, two-unit segments), and against the
+    full counts with the lists switched off."""
+    n, layers = 10, 2  # 18 units: nine two-unit segments
+    w, thetas = _tups_case(n, layers, 5353, 64)
+    nseg, fwd, lam, state, N = _reachable_model(n, layers, sum(1 << i for i in range(0, n, 2)))
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
+        a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
+        a.upload_batch(thetas)
+        assert a.get_option(AnsatzSynth.OPT_LANE_SEGMENTS) == nseg
+        assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1
+        got = [a.get_option(k) for k in (AnsatzSynth.OPT_SWEEP_AMP_FORWARD, AnsatzSynth.OPT_SWEEP_AMP_ADJOINT_LAMBDA,
+                                         AnsatzSynth.OPT_SWEEP_AMP_ADJOINT_STATE)]
+        assert got == [fwd, lam, state]
+        assert fwd < nseg * N and state < lam
+        a.set_option(AnsatzSynth.OPT_REACHABLE, 0)
+        a.upload_batch(thetas)
+        got = [a.get_option(k) for k in (AnsatzSynth.OPT_SWEEP_AMP_FORWARD, AnsatzSynth.OPT_SWEEP_AMP_ADJOINT_LAMBDA,
+                                         AnsatzSynth.OPT_SWEEP_AMP_ADJOINT_STATE)]
+        assert got == [nseg * N, lam, lam]
+
+
 @pytest.mark.parametrize("n", [6, 8, 10])
 def test_sigma_halves_match_one_row(gpu, n):
     """The lane-layout sigma with the source row split in two double-buffered
