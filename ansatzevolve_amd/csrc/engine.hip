@@ -240,12 +240,14 @@ struct EngineImpl {
     int numCU = 1;
     // kOptReachable: lane-layout sweeps skip the amplitudes that the start
     // state and the operators so far cannot reach (exact zeros).  supDirty:
-    // the lists must be recomputed (new segments or start state); supGen
+    // the lists must be recomputed (new segments or start state; supCk: the
+    // stored-state mode they were made for); supGen
     // counts recomputations; the stored states are cleared once per (buffer,
     // chunks, lists) key, since the forward sweep then writes only the
     // reachable part of each
     int reachOpt = 1;
     bool supDirty = true;
+    bool supCk = false;
     long long supGen = 0;
     std::vector<double> hstart;
     DBuf<int> d_suplist;
@@ -524,12 +526,16 @@ struct EngineImpl {
     // an orbit only if some member is in S_s (the others hold exact zeros
     // and stay zero), and S_{s+1} = S_s + every member of those orbits.  Its
     // list holds those orbits and the lone determinants in S_s.  An
-    // adjoint segment (stored states, the same segments) reads the stored
-    // state s + 1 only on orbits meeting S_{s+1}.  A segment of another
-    // kind makes everything reachable from there on.
-    bool compute_support(std::string &err)
+    // adjoint segment (stored states, the same segments) acts only on the
+    // orbits meeting S_{s+1}: elsewhere the state is zero, so the brackets
+    // are, and the lambda there never meets a nonzero state again (the
+    // segments after it in the adjoint order see S_s, S_{s-1}, ..., subsets
+    // of S_{s+1}, which is a union of this segment's orbits).  A segment of
+    // another kind makes everything reachable from there on.
+    bool compute_support(bool ck, std::string &err)
     {
         supDirty = false;
+        supCk = ck;
         supGen++;
         for (Seg &sg : laneF.segs) sg.supOff = -1;
         for (Seg &sg : laneA.segs) sg.maskOff = -1;
@@ -574,7 +580,7 @@ struct EngineImpl {
         std::vector<int> lists;
         std::vector<uint32_t> masks;
         bool full = false;
-        const bool adj = ckShape;
+        const bool adj = ckShape && ck;
         // adjoint lambda: every determinant but the lone ones of two-group segments
         std::vector<long long> loneA(laneA.segs.size(), 0);
         for (size_t i = 0; i < laneA.segs.size(); i++) {
@@ -583,8 +589,10 @@ struct EngineImpl {
                 const GroupSet &AA = *gsets[sa.gA], &BB = *gsets[sa.gB];
                 loneA[i] = (long long)(AA.ncls - AA.nm) * (BB.ncls - BB.nm);
             }
-            ampAL += N - loneA[i];
-            if (!adj || !sa.two || sa.three) ampAP += N - loneA[i];  // no mask: every acting orbit
+            if (!adj || !sa.two || sa.three) {  // no mask: every acting orbit
+                ampAL += N - loneA[i];
+                ampAP += N - loneA[i];
+            }
         }
         for (size_t si = 0; si < laneF.segs.size(); si++) {
             Seg &sg = laneF.segs[si];
@@ -594,8 +602,10 @@ struct EngineImpl {
             }
             if (full) {
                 ampF += N;
-                if (adj && si < laneA.segs.size() && laneA.segs[si].two && !laneA.segs[si].three)
+                if (adj && si < laneA.segs.size() && laneA.segs[si].two && !laneA.segs[si].three) {
+                    ampAL += N - loneA[si];
                     ampAP += N - loneA[si];
+                }
                 continue;
             }
             const GroupSet &A = *gsets[sg.gA], &B = *gsets[sg.gB];
@@ -650,6 +660,7 @@ struct EngineImpl {
                     });
                     if (any) {
                         bits[t >> 5] |= 1u << (t & 31);
+                        ampAL += cnt;
                         ampAP += cnt;
                     } else {
                         all = false;
@@ -2245,8 +2256,8 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         }
         if (ck != e.ckOn || e.bl_ck.p != ckWas) e.drop_graphs();
         e.ckOn = ck;
-        if (e.supDirty) {
-            if (!e.compute_support(err)) return false;
+        if (e.supDirty || e.supCk != ck) {
+            if (!e.compute_support(ck, err)) return false;
             e.drop_graphs();
         }
         if (ck && (e.ckZeroP != e.bl_ck.p || e.ckZeroNch != nch || e.ckZeroGen != e.supGen)) {

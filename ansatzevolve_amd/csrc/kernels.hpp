@@ -214,9 +214,10 @@ struct Orb2Args {
     // and output and are not touched
     const int *list, *llist;
     int nlist, nllist;
-    // adjoint (live != nullptr): bit t of live is 0 when every member of
-    // orbit t holds zero in the stored forward state (v0), which is then
-    // not read
+    // adjoint with stored states (LV kernel): bit t of live is 0 when every
+    // member of orbit t holds zero in the stored forward state (v0); such
+    // an orbit is skipped (its brackets are zero and its lambda is not used
+    // by any later segment)
     const uint32_t *live;
 };
 // BR = adjoint sweep: operators in reverse order with brackets (a.reverse is

@@ -318,14 +318,18 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     for (int t0 = blockIdx.x * WPB + wv; t0 < total; t0 += NO * stride) {
         int rA[NO][4], rB[NO][4], type[NO];
         uint32_t spect[NO], cmask[NO];
-        bool live[NO], act[NO], psiIn[NO];
+        bool live[NO], act[NO];
 #pragma unroll
         for (int o = 0; o < NO; o++) {
             const int t = t0 + o * stride;
             live[o] = t < total;
             const int tt = list ? list[live[o] ? t : t0] : (live[o] ? t : t0);
-            // adjoint: whether the stored state is nonzero on this orbit
-            psiIn[o] = !LV || ((ld_const(a.live, tt >> 5) >> (tt & 31)) & 1u);
+            // LV adjoint: an orbit where the stored state is zero is left
+            // alone.  Its brackets are zero, and the determinants whose
+            // lambda later segments combine with a nonzero state (the
+            // reachable set before them, a subset of this segment's) lie in
+            // the other orbits, so its lambda is never used again.
+            if (LV && !((ld_const(a.live, tt >> 5) >> (tt & 31)) & 1u)) live[o] = false;
             int ia, ib;
             if (tt < mainA) {
                 ia = tt / a.nBc;
@@ -361,33 +365,16 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
         // element offsets fit in 32 bits (N * 64 < 2^32 for n <= 14)
         double x[NO][16], l[NO][16];
 #pragma unroll
-        for (int o = 0; o < NO; o++) {
-            // one wave-uniform branch per orbit, in the LV kernel only (a
-            // per-member condition on the psi loads alone split every
-            // member's loads in two regions and slowed the adjoint sweep by
-            // 4%; the branch in every segment's kernel, by 2%)
-            if (!LV || psiIn[o]) {
+        for (int o = 0; o < NO; o++)
 #pragma unroll
-                for (int s = 0; s < 16; s++) {
-                    const int ma = s & 3, mb = s >> 2;
-                    const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
-                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                    if (BC) x[o][s] = ok ? v0[off / kLanes] : 0.0;
-                    else x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
-                    l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
-                }
-            } else {
-                // adjoint, stored state zero on this orbit: lambda only
-#pragma unroll
-                for (int s = 0; s < 16; s++) {
-                    const int ma = s & 3, mb = s >> 2;
-                    const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
-                    const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
-                    x[o][s] = 0.0;
-                    l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
-                }
+            for (int s = 0; s < 16; s++) {
+                const int ma = s & 3, mb = s >> 2;
+                const bool ok = live[o] && rA[o][ma] >= 0 && rB[o][mb] >= 0;
+                const uint32_t off = ((uint32_t)rA[o][ma] * (uint32_t)a.nb + (uint32_t)rB[o][mb]) * kLanes;
+                if (BC) x[o][s] = ok ? v0[off / kLanes] : 0.0;
+                else x[o][s] = ok ? __builtin_nontemporal_load(v0 + off + lane) : 0.0;
+                l[o][s] = (BR && ok) ? __builtin_nontemporal_load(v1 + off + lane) : 0.0;
             }
-        }
 #pragma unroll
         for (int o = 0; o < NO; o++) {
             if (!act[o]) continue;
