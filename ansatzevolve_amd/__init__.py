@@ -56,6 +56,10 @@ C_API = [
     ("getEngineOption", ctypes.c_int, [ctypes.c_int, _ip, _vp]),
     ("autotune", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                 ctypes.c_int, _vp]),
+    # run files
+    ("setRunPath", ctypes.c_int, [ctypes.c_char_p, _vp]),
+    ("loadRunFiles", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _vp]),
+    ("readParameterSets", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp]),
     # learned warm start
     ("mlpCreate", _vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]),
     ("mlpDestroy", ctypes.c_int, [_vp]),
@@ -297,6 +301,17 @@ class AnsatzSynth:
         return json.loads(buf.value.decode())
 
 
+    # ---- run files (<prefix>_Initial.dat, _Operators.dat, _Order.dat, ...)
+    def set_run_path(self, prefix):
+        """The Hamiltonian is read from <prefix>_Ham_Coeff/_Ham_Index.dat or
+        <prefix>_oneEInts/_twoEInts.bin at construction when none was set."""
+        self._chk("setRunPath", self.lib.setRunPath(os.fsencode(prefix), self.ctx))
+
+    def load_run_files(self, prefix, with_nuclear=False):
+        """The whole set-up from the run files of `prefix`; with_nuclear adds
+        <prefix>_Nuclear_Energy.dat to every energy."""
+        self._chk("loadRunFiles", self.lib.loadRunFiles(os.fsencode(prefix), 1 if with_nuclear else 0, self.ctx))
+
     def optimise(self, angles, stop_energy=float("nan"), max_trace=100000):
         """Plain Newton optimisation from `angles`; returns (angles, E, trace)
         with trace rows (seconds, energy, |g|, Hessian evals, energy evals)."""
@@ -354,6 +369,24 @@ class MultiBatch:
         if r != 0:
             raise AnsatzError("multiBatchDownload", r)
         return (E, G) if with_gradient else E
+
+
+def read_parameter_sets(path, lib=None):
+    """(energies, sets) of a _Parameters.dat file: the recorded energy of
+    every set and its values (nsets x nparams, zero padded)."""
+    lib = lib or load_library()
+    ns, npar = ctypes.c_int(0), ctypes.c_int(0)
+    code = lib.readParameterSets(os.fsencode(path), 0, 0, ctypes.byref(ns), ctypes.byref(npar), None, None)
+    if code != 0:
+        raise AnsatzError("readParameterSets", code)
+    E = np.zeros(ns.value)
+    P = np.zeros((ns.value, npar.value))
+    if ns.value:
+        code = lib.readParameterSets(os.fsencode(path), ns.value, npar.value, ctypes.byref(ns), ctypes.byref(npar),
+                                     E.ctypes.data_as(_dp), P.ctypes.data_as(_dp))
+        if code != 0:
+            raise AnsatzError("readParameterSets", code)
+    return E, P
 
 
 def from_workload(w, storage=0):

@@ -15,7 +15,9 @@ HIP_SOURCES = ["kernels.hip", "lanes.hip", "sigg.hip", "engine.hip", "mlp.hip", 
 # per-file device flags: the lane sweeps keep their orbit arrays as separate
 # registers (no promotion of private arrays to one vector register tuple)
 HIP_FILE_FLAGS = {"lanes.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"]}
-CXX_SOURCES = ["manager.cpp", "newton.cpp", "autotune.cpp", "abi.cpp"]
+CXX_SOURCES = ["manager.cpp", "newton.cpp", "autotune.cpp", "abi.cpp", "runfiles.cpp"]
+# command-line front end over the C ABI (cli.cpp), beside the library
+CLI = os.path.join(HERE, "ansatzsynth")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
@@ -57,6 +59,10 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    _run(["g++"] + CXX_FLAGS[:-1] + ["-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, "cli.cpp"),
+                                     "-o", tmp, "-L" + HERE, "-lansatzsynth", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
     return LIB
 
 

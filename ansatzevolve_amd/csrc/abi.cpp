@@ -3,6 +3,8 @@
 // call on a ctx takes that ctx's lock.  No exception crosses the boundary.
 #include "../../include/ansatzsynth.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -11,6 +13,7 @@
 #include <vector>
 
 #include "manager.hpp"
+#include "runfiles.hpp"
 
 using ae::log_msg;
 using ae::Manager;
@@ -309,6 +312,89 @@ int setNuclearEnergy(double energy, void *ctx)
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     if (!m->storeNuclearEnergy(energy)) return 3;
+    return 0;
+}
+
+int setRunPath(const char *runpath, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (runpath == nullptr || runpath[0] == 0) {
+        log_msg("Run path has zero size");
+        return 2;
+    }
+    if (traceInterfaceCalls) log_msg("setRunPath called with %s", runpath);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->storeRunPath(runpath)) return 3;
+    return 0;
+}
+
+int loadRunFiles(const char *prefix, int withnuclear, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (prefix == nullptr || prefix[0] == 0) {
+        log_msg("Run path has zero size");
+        return 2;
+    }
+    if (traceInterfaceCalls) log_msg("loadRunFiles called with %s, withnuclear %d", prefix, withnuclear);
+    const std::string pre(prefix);
+    int nq = 0;
+    std::vector<int> idx;
+    std::vector<double> coeff;
+    std::vector<std::array<int8_t, 4>> ops;
+    std::vector<std::pair<int, double>> dep;
+    double nuc = 0;
+    bool haveNuc = false;
+    std::string err;
+    if (!ae::files::read_initial(pre + "_Initial.dat", nq, idx, coeff, err) ||
+        !ae::files::read_operators(pre + "_Operators.dat", ops, err) ||
+        !ae::files::read_order(pre + "_Order.dat", dep, err) ||
+        (withnuclear && !ae::files::read_nuclear_energy(pre + "_Nuclear_Energy.dat", nuc, haveNuc, err))) {
+        log_msg("loadRunFiles: %s", err.c_str());
+        return 3;
+    }
+    if (dep.size() != ops.size()) {
+        log_msg("loadRunFiles: %zu operators but %zu order entries", ops.size(), dep.size());
+        return 3;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->storeInitial(nq, idx, coeff) || !m->storeOperators(ops) || !m->storeParameterDependencies(dep) ||
+        (withnuclear && !m->storeNuclearEnergy(nuc)) || !m->storeRunPath(pre))
+        return 4;
+    return 0;
+}
+
+int readParameterSets(const char *path, int maxsets, int maxparams, int *nsets, int *nparams, double *energies,
+                      double *params)
+{
+    if (path == nullptr || nsets == nullptr || nparams == nullptr) {
+        log_msg("readParameterSets: NULL argument");
+        return 2;
+    }
+    std::vector<double> E;
+    std::vector<std::vector<double>> sets;
+    std::string err;
+    if (!ae::files::read_parameter_sets(path, E, sets, err)) {
+        log_msg("readParameterSets: %s", err.c_str());
+        return 3;
+    }
+    int np = 0;
+    for (const auto &v : sets) np = std::max(np, (int)v.size());
+    *nsets = (int)sets.size();
+    *nparams = np;
+    if (maxsets <= 0) return 0;  // counts only
+    if ((int)sets.size() > maxsets || np > maxparams || energies == nullptr || params == nullptr) return 6;
+    for (size_t s = 0; s < sets.size(); s++) {
+        energies[s] = E[s];
+        for (int i = 0; i < maxparams; i++) params[s * maxparams + i] = i < (int)sets[s].size() ? sets[s][i] : 0.0;
+    }
     return 0;
 }
 

@@ -1,5 +1,6 @@
 // Host-side mirror of stateAnsatzManager (src/AnsatzManager.cpp).
 #include "manager.hpp"
+#include "runfiles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -147,6 +148,21 @@ bool Manager::storeIntegrals(int norb, const double *h, const double *g)
     return true;
 }
 
+bool Manager::storeRunPath(const std::string &runPath)
+{
+    bool success = true;
+    if (m_isConstructed) {
+        success = false;
+        log_msg("Already constructed, setup cant be modified now");
+    }
+    if (runPath.empty()) {
+        success = false;
+        log_msg("Run path has zero size");
+    }
+    if (success) m_runPath = runPath;
+    return success;
+}
+
 bool Manager::storeNuclearEnergy(double e)
 {
     if (m_isConstructed) {
@@ -249,6 +265,24 @@ bool Manager::construct()
     if (m_parameterDependency.size() != m_excitations.size()) {
         success = false;
         log_msg("More operators than parameters dependencies given");
+    }
+    if (success && m_iIndexes.empty() && m_h.empty() && !m_runPath.empty()) {
+        // the Hamiltonian from the run path's files
+        int kind = 0;
+        std::vector<int> i0, j0;
+        std::vector<double> v, h, g;
+        std::string ferr;
+        if (!files::read_run_hamiltonian(m_runPath, m_numberOfQubits, kind, i0, j0, v, h, g, ferr)) {
+            log_msg("Could not construct Hamiltonian: %s", ferr.c_str());
+        } else if (kind == 1) {
+            m_iIndexes = std::move(i0);
+            m_jIndexes = std::move(j0);
+            m_coeffs = std::move(v);
+        } else {
+            m_norb = m_numberOfQubits / 2;
+            m_h = std::move(h);
+            m_g = std::move(g);
+        }
     }
     if (m_iIndexes.empty() && m_h.empty()) {
         success = false;

@@ -27,6 +27,10 @@ public:
     bool storeNuclearEnergy(double e);
     bool storeParameterDependencies(const std::vector<std::pair<int, double>> &dep);
     bool storeHamiltonianStorage(int mode);
+    // Run path prefix (src/AnsatzManager.cpp:447-466): with no Hamiltonian
+    // stored, construction loads it from <prefix>_Ham_Coeff/_Ham_Index.dat
+    // or <prefix>_oneEInts/_twoEInts.bin (src/AnsatzManager.cpp:185-199).
+    bool storeRunPath(const std::string &runPath);
 
     bool getExpectationValue(const std::vector<double> &angles, double &E);
     bool getFinalState(const std::vector<double> &angles, std::vector<double> &full);
@@ -84,6 +88,7 @@ private:
     int m_norb = 0;
     std::vector<double> m_h, m_g;
     int m_hamStorage = 0;
+    std::string m_runPath;
 
     std::unique_ptr<Engine> m_engine;
     std::vector<double> m_angles;  // per-operator angles of the current psi

@@ -252,6 +252,37 @@ int multiBatchRun(void *mb, int withgradient);
 int multiBatchDownload(void *mb, int index, double *energies, double *gradients);
 int multiBatchDestroy(void *mb);
 
+/* Run files.  The reference reads them in its command-line program
+ * (src/main.cpp:233-380 with src/TUPSLoadingUtils.cpp) and, for the
+ * Hamiltonian only, through stateAnsatzManager::storeRunPath
+ * (src/AnsatzManager.cpp:447-466, 185-199), which has no C-ABI binding.
+ *
+ * setRunPath: with no Hamiltonian set, construction loads it from
+ * <runpath>_Ham_Coeff.dat + _Ham_Index.dat (text triplets, one-based
+ * full-space indices) when both exist, else from _oneEInts.bin +
+ * _twoEInts.bin (FP64 integrals, n^2 and n^4, n = qubits / 2), as the
+ * reference's file constructor (src/hamiltonianmatrix.cpp:660-705).
+ * 1 ctx NULL, 2 runpath NULL or empty, 3 already constructed. */
+int setRunPath(const char *runpath, void *ctx);
+/* The whole set-up from <prefix>_Initial.dat (bitstring, coefficient),
+ * _Operators.dat (four one-based indices per operator), _Order.dat
+ * ("parameter[,scale]" per operator), the Hamiltonian as in setRunPath,
+ * and (withnuclear != 0) _Nuclear_Energy.dat, added to every energy (a
+ * missing file counts as 0, src/TUPSLoadingUtils.cpp:156-174).
+ * 1 ctx NULL, 2 prefix NULL or empty, 3 a file missing or malformed (or
+ * operator and order counts differ), 4 a set-up step failed (e.g. already
+ * constructed).  The Hamiltonian files are read at construction. */
+int loadRunFiles(const char *prefix, int withnuclear, void *ctx);
+/* The parameter sets of a _Parameters.dat file (a count line, an "Energy of
+ * minimum ... = E" line, then the values; src/TUPSLoadingUtils.cpp:176-234).
+ * Sets whose value count differs from their count line are skipped.  Sets
+ * *nsets and *nparams (the longest set); with maxsets > 0 fills energies[s]
+ * and params[s * maxparams + i] (zero padded).  No ctx.  2 NULL path or
+ * count pointer, 3 file missing or malformed, 6 more sets or parameters
+ * than the buffers hold (counts still set) or NULL buffers. */
+int readParameterSets(const char *path, int maxsets, int maxparams, int *nsets, int *nparams, double *energies,
+                      double *params);
+
 /* ---------------- Part 3: learned warm start (no reference counterpart) ----
  * A 3-layer FP32 network x -> tanh(x W0 + b0) -> tanh(. W1 + b1) -> . W2 + b2
  * proposing initial angles from integral features, with device kernels for
