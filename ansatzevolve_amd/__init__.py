@@ -35,6 +35,7 @@ C_API = [
     ("setTraceInterfaceCalls", None, [ctypes.c_int]),
     # extensions
     ("setHamiltonianIntegrals", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("getGradientOperators", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
     ("setNuclearEnergy", ctypes.c_int, [ctypes.c_double, _vp]),
     ("setHamiltonianStorage", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getEnergyGradient", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _vp]),
@@ -150,6 +151,7 @@ class AnsatzSynth:
         i, ip = _i(indexes_1based)
         c, cp = _d(coeffs)
         self._chk("setInitialState", self.lib.setInitialState(num_qubits, len(i), ip, cp, self.ctx))
+        self.num_qubits = int(num_qubits)
 
     def set_hamiltonian(self, i_1based, j_1based, coeffs):
         a, ap = _i(i_1based)
@@ -204,6 +206,14 @@ class AnsatzSynth:
         a, ap = _d(angles)
         g = np.zeros(len(a))
         self._chk("getGradient_COMP", self.lib.getGradient_COMP(len(a), ap, g.ctypes.data_as(_dp), self.ctx))
+        return g
+
+    def gradient_operators(self, angles):
+        """Gradient per operator (dE/dx_k for operator k's angle)."""
+        a, ap = _d(angles)
+        M = self.dimensions()[1]
+        g = np.zeros(M)
+        self._chk("getGradientOperators", self.lib.getGradientOperators(len(a), ap, g.ctypes.data_as(_dp), self.ctx))
         return g
 
     def hessian(self, angles):
@@ -311,6 +321,8 @@ class AnsatzSynth:
         """The whole set-up from the run files of `prefix`; with_nuclear adds
         <prefix>_Nuclear_Energy.dat to every energy."""
         self._chk("loadRunFiles", self.lib.loadRunFiles(os.fsencode(prefix), 1 if with_nuclear else 0, self.ctx))
+        with open(os.fsdecode(prefix) + "_Initial.dat") as f:
+            self.num_qubits = len(f.read().split(",")[0].strip())
 
     def optimise(self, angles, stop_energy=float("nan"), max_trace=100000):
         """Plain Newton optimisation from `angles`; returns (angles, E, trace)

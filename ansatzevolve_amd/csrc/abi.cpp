@@ -252,6 +252,33 @@ int getGradient_COMP(int nangles, const double *angles, double *gradient, void *
     return 0;
 }
 
+int getGradientOperators(int nangles, const double *angles, double *gradient, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("nullptr angles passed");
+        return 2;
+    }
+    if (gradient == nullptr) {
+        log_msg("nullptr gradient passed");
+        return 3;
+    }
+    if (nangles < 1) {
+        log_msg("NAngles < 1");
+        return 4;
+    }
+    if (traceInterfaceCalls) log_msg("getGradientOperators called with NAngles %d", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> g;
+    if (!m->getGradient(std::vector<double>(angles, angles + nangles), g)) return 5;
+    std::memcpy(gradient, g.data(), g.size() * sizeof(double));
+    return 0;
+}
+
 int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx)
 {
     if (ctx == nullptr) {

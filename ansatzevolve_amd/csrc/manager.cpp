@@ -491,6 +491,11 @@ bool Manager::getGradientComp(const std::vector<double> &angles, std::vector<dou
     return true;
 }
 
+bool Manager::getGradient(const std::vector<double> &angles, std::vector<double> &grad)
+{
+    return gradientAt(angles, nullptr, grad);
+}
+
 bool Manager::getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad)
 {
     std::vector<double> deriv;

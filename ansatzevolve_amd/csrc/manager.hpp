@@ -35,6 +35,8 @@ public:
     bool getExpectationValue(const std::vector<double> &angles, double &E);
     bool getFinalState(const std::vector<double> &angles, std::vector<double> &full);
     bool getGradientComp(const std::vector<double> &angles, std::vector<double> &grad);
+    // per operator (decompressed; src/AnsatzManager.h:221)
+    bool getGradient(const std::vector<double> &angles, std::vector<double> &grad);
     bool getHessianComp(const std::vector<double> &angles, std::vector<double> &hessRowMajor);
     bool getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad);
     bool getExpectationValues(int nvec, int nangles, const double *angles, std::vector<double> &E);

@@ -167,6 +167,14 @@ int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, 
 int optimise(int nangles, double *angles, double *energy, double stopenergy, int maxtrace, double *trace,
              int *ntrace, void *ctx);
 
+/* Gradient per operator (the decompressed form of the C++/Python
+ * stateAnsatzManager::getGradient, src/AnsatzManager.h:221): dE/dx_k for
+ * the angle x_k of operator k; gradient receives one value per operator
+ * (getDimensions).  angles: per operator or per parameter.  Codes as
+ * getGradient_COMP (1 ctx, 2 angles, 3 gradient NULL, 4 nangles < 1,
+ * 5 failure). */
+int getGradientOperators(int nangles, const double *angles, double *gradient, void *ctx);
+
 /* Sector dimension, number of operators and parameters after construction
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);
