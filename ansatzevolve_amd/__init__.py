@@ -36,6 +36,7 @@ C_API = [
     # extensions
     ("setHamiltonianIntegrals", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
     ("getGradientOperators", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
+    ("getOverlapGradient", ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _vp]),
     ("setNuclearEnergy", ctypes.c_int, [ctypes.c_double, _vp]),
     ("setHamiltonianStorage", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getEnergyGradient", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _vp]),
@@ -215,6 +216,17 @@ class AnsatzSynth:
         g = np.zeros(M)
         self._chk("getGradientOperators", self.lib.getGradientOperators(len(a), ap, g.ctypes.data_as(_dp), self.ctx))
         return g
+
+    def overlap_gradient(self, angles, proj_full):
+        """(<proj|psi(angles)>, its per-parameter gradient); proj over the
+        full space of 2^qubits basis states."""
+        a, ap = _d(angles)
+        pr, pp = _d(proj_full)
+        g = np.zeros(len(a))
+        ov = ctypes.c_double(0)
+        self._chk("getOverlapGradient", self.lib.getOverlapGradient(len(a), ap, len(pr), pp, ctypes.byref(ov),
+                                                                    g.ctypes.data_as(_dp), self.ctx))
+        return ov.value, g
 
     def hessian(self, angles):
         a, ap = _d(angles)

@@ -279,6 +279,44 @@ int getGradientOperators(int nangles, const double *angles, double *gradient, vo
     return 0;
 }
 
+int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, const double *projvector,
+                       double *overlap, double *gradient, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("nullptr ctx passed");
+        return 1;
+    }
+    if (angles == nullptr || projvector == nullptr) {
+        log_msg("nullptr angles or projection vector passed");
+        return 2;
+    }
+    if (gradient == nullptr) {
+        log_msg("nullptr gradient passed");
+        return 3;
+    }
+    if (nangles < 1 || nbasisvectors < 1) {
+        log_msg("NAngles < 1 or nbasisvectors < 1");
+        return 4;
+    }
+    if (traceInterfaceCalls) log_msg("getOverlapGradient called with NAngles %d", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    long long dim = 0;
+    int nops = 0, nparams = 0;
+    if (!m->getDimensions(dim, nops, nparams)) return 5;
+    if (nangles != nparams) {
+        log_msg("Gradient is the wrong size");
+        return -6;
+    }
+    std::vector<double> g;
+    double ov = 0;
+    const std::vector<double> proj(projvector, projvector + nbasisvectors);
+    if (!m->getOverlapGradient(std::vector<double>(angles, angles + nangles), proj, ov, g)) return 5;
+    std::memcpy(gradient, g.data(), g.size() * sizeof(double));
+    if (overlap) *overlap = ov;
+    return 0;
+}
+
 int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx)
 {
     if (ctx == nullptr) {

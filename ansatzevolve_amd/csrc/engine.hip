@@ -2214,6 +2214,40 @@ bool Engine::gradient(const double *angles, double *E, double *deriv, std::strin
     return true;
 }
 
+bool Engine::overlap_gradient(const double *angles, const double *proj, double *overlap, double *deriv,
+                              std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
+    HCHK(hipMemcpyAsync(e.d_psi.p, e.d_start.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.sweep_forward(e.d_psi.p, 1, e.d_cs.p, err)) return false;
+    HCHK(hipMemcpyAsync(e.d_phi.p, e.d_psi.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
+    HCHK(hipMemcpyAsync(e.d_lam.p, proj, 8 * e.N, hipMemcpyHostToDevice, e.stream));
+    // overlap partials in the energy partial rows (eslots_alloc() >= this grid)
+    const int G = (int)((e.N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_dot_rows, dim3(G, 1), dim3(kBlock), 0, e.stream, e.d_phi.p, e.d_lam.p, e.N, e.N,
+                       e.d_epart.p);
+    LCHK();
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, G, 0LL, 1LL, e.d_out.p + e.M,
+                       1.0);
+    LCHK();
+    if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, e.d_csneg.p, e.d_partial.p, err)) return false;
+    // brackets without the factor 2 of the energy gradient (src/fusedevolve.cpp:2882-2884)
+    hipLaunchKernelGGL(k_reduce, dim3(e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, e.M, e.slots,
+                       (long long)e.slots, 1LL, e.d_out.p, 1.0);
+    LCHK();
+    std::vector<double> out(e.M + 1);
+    HCHK(hipMemcpyAsync(out.data(), e.d_out.p, 8 * (e.M + 1), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    std::memcpy(deriv, out.data(), 8 * e.M);
+    if (overlap) *overlap = out[e.M];
+    return true;
+}
+
 bool Engine::download_state(double *out, std::string &err)
 {
     EngineImpl &e = *d_;

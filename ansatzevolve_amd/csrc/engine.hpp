@@ -96,6 +96,12 @@ public:
     // psi = U(angles) psi_0, then E and the adjoint sweep, as one device
     // sequence (replayed from a hipGraph after the first call).
     bool energy_gradient(const double *angles, double *E, double *deriv, std::string &err);
+    // psi = U(angles) psi_0, overlap = <proj|psi> and, by the adjoint sweep
+    // with lambda = proj, deriv[k] = d<proj|psi>/d(angle_k) per operator
+    // (FusedEvolve::evolveDerivativeProj, src/fusedevolve.cpp:2814-2889);
+    // proj: N amplitudes in sector order.
+    bool overlap_gradient(const double *angles, const double *proj, double *overlap, double *deriv,
+                          std::string &err);
     // Per-parameter Hessian (row-major P x P), gradient and energy.
     bool hessian(const double *angles, double *hess, double *grad, double *E, std::string &err);
     // The current psi over the sector.

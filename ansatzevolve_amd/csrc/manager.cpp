@@ -496,6 +496,43 @@ bool Manager::getGradient(const std::vector<double> &angles, std::vector<double>
     return gradientAt(angles, nullptr, grad);
 }
 
+bool Manager::getOverlapGradient(const std::vector<double> &angles, const std::vector<double> &projFull,
+                                 double &overlap, std::vector<double> &grad)
+{
+    if (!validateToRun()) return false;
+    const size_t M = m_excitations.size();
+    std::vector<double> x;
+    if (angles.size() == M) {
+        x = angles;
+    } else if ((int)angles.size() == m_numberOfUniqueParameters) {
+        x.resize(M);
+        for (size_t k = 0; k < M; k++)
+            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
+    } else {
+        log_msg("Incorrect number of angles given: %zu", angles.size());
+        return false;
+    }
+    if (projFull.size() != ((size_t)1 << m_numberOfQubits)) {
+        log_msg("Projection vector has %zu entries, expected 2^%d", projFull.size(), m_numberOfQubits);
+        return false;
+    }
+    // the sector's amplitudes of proj (entries outside the sector meet zeros of psi)
+    const auto &st = m_engine->states();
+    std::vector<double> comp(st.size());
+    for (size_t i = 0; i < st.size(); i++) comp[i] = projFull[st[i]];
+    std::vector<double> deriv(M);
+    std::string err;
+    if (!m_engine->overlap_gradient(x.data(), comp.data(), &overlap, deriv.data(), err)) {
+        log_msg("Overlap gradient failed: %s", err.c_str());
+        m_angles.assign(M, std::nan(""));
+        return false;
+    }
+    m_angles = x;  // psi is now U(x) psi_0
+    grad.resize(m_numberOfUniqueParameters);
+    compressGradient(deriv.data(), grad.data());
+    return true;
+}
+
 bool Manager::getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad)
 {
     std::vector<double> deriv;

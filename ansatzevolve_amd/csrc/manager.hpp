@@ -37,6 +37,10 @@ public:
     bool getGradientComp(const std::vector<double> &angles, std::vector<double> &grad);
     // per operator (decompressed; src/AnsatzManager.h:221)
     bool getGradient(const std::vector<double> &angles, std::vector<double> &grad);
+    // overlap <proj|psi(angles)> and its per-parameter gradient, proj over
+    // the full space of 2^qubits basis states (src/fusedevolve.cpp:2814-2889)
+    bool getOverlapGradient(const std::vector<double> &angles, const std::vector<double> &projFull,
+                            double &overlap, std::vector<double> &grad);
     bool getHessianComp(const std::vector<double> &angles, std::vector<double> &hessRowMajor);
     bool getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad);
     bool getExpectationValues(int nvec, int nangles, const double *angles, std::vector<double> &E);

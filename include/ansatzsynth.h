@@ -175,6 +175,18 @@ int optimise(int nangles, double *angles, double *energy, double stopenergy, int
  * 5 failure). */
 int getGradientOperators(int nangles, const double *angles, double *gradient, void *ctx);
 
+/* Overlap <proj|psi(angles)> and its per-parameter gradient, by the adjoint
+ * sweep with the projection vector in place of H psi
+ * (FusedEvolve::evolveDerivativeProj, src/fusedevolve.cpp:2814-2889; the
+ * reference's projected optimiser uses it, src/tupsquantities.cpp:2075-2095).
+ * projvector: dense over the full space, nbasisvectors = 2^numqubits (as
+ * getFinalState); overlap may be NULL.  angles per parameter.
+ * 1 ctx NULL, 2 angles or projvector NULL, 3 gradient NULL, 4 nangles or
+ * nbasisvectors < 1, 5 failure (also nbasisvectors != 2^numqubits),
+ * -6 nangles != number of parameters. */
+int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, const double *projvector,
+                       double *overlap, double *gradient, void *ctx);
+
 /* Sector dimension, number of operators and parameters after construction
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx);
