@@ -108,9 +108,9 @@ struct Seg {
     int grid = 1;  // lane layout: blocks of its launches
     // lane-layout forward two-group segment with reachable amplitudes only:
     // its orbit and lone lists in d_suplist (supOff < 0: every orbit) and
-    // its grid; adjoint: its orbit bits in d_supmask (maskOff < 0: none)
+    // its grid; adjoint (stored states): the orbits where the state can be
+    // nonzero, also at supOff (supN of them)
     int supOff = -1, supN = 0, supLOff = 0, supLN = 0, gridSup = 1;
-    long long maskOff = -1;
 };
 
 // the two-half lane-layout sparse sigma kernel for (same-half, cross-half)
@@ -251,7 +251,6 @@ struct EngineImpl {
     long long supGen = 0;
     std::vector<double> hstart;
     DBuf<int> d_suplist;
-    DBuf<uint32_t> d_supmask;
     // amplitudes per vector and step that the lane-layout sweeps move
     // (summed over segments): forward reads and writes, adjoint lambda reads
     // and writes, adjoint stored-state reads (kOptSweepAmp*)
@@ -538,7 +537,7 @@ struct EngineImpl {
         supCk = ck;
         supGen++;
         for (Seg &sg : laneF.segs) sg.supOff = -1;
-        for (Seg &sg : laneA.segs) sg.maskOff = -1;
+        for (Seg &sg : laneA.segs) sg.supOff = -1;
         // without lists every amplitude counts as reachable (the amplitude
         // counts are still computed)
         const bool lists_on = reachOpt && (long long)hstart.size() == N;
@@ -578,7 +577,6 @@ struct EngineImpl {
                     if (a4[ma] >= 0 && b4[mb] >= 0) f((long long)a4[ma] * nb + b4[mb]);
         };
         std::vector<int> lists;
-        std::vector<uint32_t> masks;
         bool full = false;
         const bool adj = ckShape && ck;
         // adjoint lambda: every determinant but the lone ones of two-group segments
@@ -649,8 +647,7 @@ struct EngineImpl {
                 const GroupSet &AA = *gsets[sa.gA], &BB = *gsets[sa.gB];
                 const std::vector<int4> &raa = ranks(sa.gA), &rbb = ranks(sa.gB);
                 const int tot = AA.nm * BB.ncls + (AA.ncls - AA.nm) * BB.nm;
-                std::vector<uint32_t> bits((tot + 31) / 32, 0u);
-                bool all = true;
+                std::vector<int> liveOrb;
                 for (int t = 0; t < tot; t++) {
                     bool any = false;
                     long long cnt = 0;
@@ -659,26 +656,21 @@ struct EngineImpl {
                         cnt++;
                     });
                     if (any) {
-                        bits[t >> 5] |= 1u << (t & 31);
+                        liveOrb.push_back(t);
                         ampAL += cnt;
                         ampAP += cnt;
-                    } else {
-                        all = false;
                     }
                 }
-                if (!all && lists_on) {
-                    sa.maskOff = (long long)masks.size();
-                    masks.insert(masks.end(), bits.begin(), bits.end());
+                if ((int)liveOrb.size() < tot && lists_on) {
+                    sa.supOff = (int)lists.size();
+                    sa.supN = (int)liveOrb.size();
+                    lists.insert(lists.end(), liveOrb.begin(), liveOrb.end());
                 }
             }
         }
         if (!lists.empty()) {
             if (!d_suplist.ensure(lists.size(), err)) return false;
             HCHK(hipMemcpy(d_suplist.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
-        }
-        if (!masks.empty()) {
-            if (!d_supmask.ensure(masks.size(), err)) return false;
-            HCHK(hipMemcpy(d_supmask.p, masks.data(), masks.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         }
         return true;
     }
@@ -1022,14 +1014,16 @@ struct EngineImpl {
         a.split = sg.progOff;
         a.list = a.llist = nullptr;
         a.nlist = a.nllist = 0;
-        a.live = nullptr;
         if (!reverse && sg.supOff >= 0) {
             a.list = d_suplist.p + sg.supOff;
             a.llist = d_suplist.p + sg.supLOff;
             a.nlist = sg.supN;
             a.nllist = sg.supLN;
         }
-        if (reverse && ckOn && sg.maskOff >= 0) a.live = d_supmask.p + sg.maskOff;
+        if (reverse && ckOn && sg.supOff >= 0) {
+            a.list = d_suplist.p + sg.supOff;
+            a.nlist = sg.supN;
+        }
         return a;
     }
 
@@ -1475,7 +1469,7 @@ struct EngineImpl {
             } else if (sg.two) {
                 const int mk = mark_begin(2);
                 const Orb2Args oa = orb2args(laneA, sg, true);
-                if (oa.live)
+                if (oa.list)
                     hipLaunchKernelGGL((k_orb2v<true, false, true>), dim3(sg.grid, nch), dim3(kBlock), 0, stream, oa,
                                        in, out, lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
                 else

@@ -211,20 +211,18 @@ struct Orb2Args {
     // forward, reachable amplitudes only (list != nullptr): the orbits
     // list[0 .. nlist) and the lone determinants llist[0 .. nllist) (indices
     // into the orbit and lone index spaces); the others hold zero on input
-    // and output and are not touched
+    // and output and are not touched.  Adjoint with stored states (LV
+    // kernel): list[0 .. nlist) = the orbits where the stored forward state
+    // can be nonzero; the others are skipped (their brackets are zero and
+    // their lambda is not used by any later segment)
     const int *list, *llist;
     int nlist, nllist;
-    // adjoint with stored states (LV kernel): bit t of live is 0 when every
-    // member of orbit t holds zero in the stored forward state (v0); such
-    // an orbit is skipped (its brackets are zero and its lambda is not used
-    // by any later segment)
-    const uint32_t *live;
 };
 // BR = adjoint sweep: operators in reverse order with brackets (a.reverse is
 // ignored); !BR: forward order.  BC (forward, out of place): v0 is one plain
 // vector of N amplitudes read by every lane (the initial state), not a lane-
 // layout chunk.
-// LV (adjoint): a.live marks the orbits whose stored state is read.
+// LV (adjoint): a.list holds the orbits to act on.
 template <bool BR, bool BC = false, bool LV = false>
 __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);

@@ -309,7 +309,13 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
     // orbits with some two-member class: alpha classes [0, nAm) with every
     // beta class, then the single-member alpha classes with beta [0, nBm)
     const int mainA = a.nAm * a.nBc;
-    const int *list = BR ? nullptr : a.list;  // reachable-orbit lists: forward only
+    // orbit lists: the forward sweep's reachable orbits; in the LV adjoint
+    // the orbits where the stored state can be nonzero.  The adjoint leaves
+    // the others alone: their brackets are zero, and the determinants whose
+    // lambda later segments combine with a nonzero state (the reachable set
+    // before them, a subset of this segment's) lie in the listed orbits, so
+    // their lambda is never used again.
+    const int *list = (BR && !LV) ? nullptr : a.list;
     const int total = list ? a.nlist : mainA + (a.nAc - a.nAm) * a.nBm;
     const int stride = gridDim.x * WPB;
     // one orbit per wave iteration (two per iteration, loaded together,
@@ -324,12 +330,6 @@ __global__ void __launch_bounds__(kBlock, BR ? 5 : 8)
             const int t = t0 + o * stride;
             live[o] = t < total;
             const int tt = list ? list[live[o] ? t : t0] : (live[o] ? t : t0);
-            // LV adjoint: an orbit where the stored state is zero is left
-            // alone.  Its brackets are zero, and the determinants whose
-            // lambda later segments combine with a nonzero state (the
-            // reachable set before them, a subset of this segment's) lie in
-            // the other orbits, so its lambda is never used again.
-            if (LV && !((ld_const(a.live, tt >> 5) >> (tt & 31)) & 1u)) live[o] = false;
             int ia, ib;
             if (tt < mainA) {
                 ia = tt / a.nBc;
