@@ -38,3 +38,16 @@ def test_setup_methods():
     assert m.storeRunPath(H4)
     assert not m.storeRunPath("")
     assert m.storeNuclearEnergy(1.5)
+
+
+def test_file_function_edge_cases(tmp_path):
+    # a bitstring alone means coefficient 1 (as the reference's reader assumes)
+    f = tmp_path / "S_Initial.dat"
+    f.write_text("0011\n1100,-0.5\n")
+    st = pae.readCsvState(str(f))
+    assert len(st) == 16 and st[3] == 1.0 and st[12] == -0.5 and sum(x != 0 for x in st) == 2
+    # no parameter file: only the template path; the count comes from the order file
+    (tmp_path / "S_Order.dat").write_text("1\n2,-1\n2\n")
+    paths, order, P = pae.loadParameters(str(tmp_path / "S_Order.dat"), str(tmp_path / "none.dat"),
+                                         [(0, 0.0)] * 3)
+    assert paths == [[(0, 0.0)] * 3] and order == [(0, 1.0), (1, -1.0), (1, 1.0)] and P == 2
