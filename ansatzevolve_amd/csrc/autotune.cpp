@@ -5,11 +5,14 @@
 // fusion depth (most operators per pass) and the batch layout (lane layout
 // or one vector per grid slice).  Each candidate runs the caller's angle
 // batch once to warm up (and capture its hipGraph), then `reps` timed
-// replays; the median wall time decides.  Every candidate's energies and
-// per-operator derivatives are compared with the first candidate's, so a
-// choice is only accepted when the results agree (the report carries the
-// largest deviation).  A choice can be recorded per problem shape in a
-// text file of JSON lines and re-applied without measuring.
+// replays; the median wall time decides among the candidates whose results
+// agree.  Every candidate's energies and per-operator derivatives are
+// compared with the first candidate's: a candidate whose largest deviation,
+// relative to max(1, |first value|), exceeds kAgreeTol, or whose results are
+// not finite, is reported as rejected and never chosen or recorded.  The
+// first candidate must itself give finite results.  A choice can be
+// recorded per problem shape in a text file of JSON lines and re-applied
+// without measuring.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -24,6 +27,33 @@ namespace ae {
 void log_msg(const char *fmt, ...);
 
 namespace {
+
+// largest accepted deviation of a candidate's results from the first
+// candidate's (relative to max(1, |value|)); execution choices change only
+// the summation order, which moves results by ~1e-15
+constexpr double kAgreeTol = 1e-10;
+
+// largest relative deviation of (E, D) from (E0, D0); +inf when any value is
+// not finite
+double deviation(const std::vector<double> &E, const std::vector<double> &E0, const std::vector<double> &D,
+                 const std::vector<double> &D0)
+{
+    double dev = 0;
+    auto acc = [&](double x, double x0) {
+        const double d = std::fabs(x - x0) / std::max(1.0, std::fabs(x0));
+        dev = std::isfinite(d) ? std::max(dev, d) : INFINITY;
+    };
+    for (size_t i = 0; i < E.size(); i++) acc(E[i], E0[i]);
+    for (size_t i = 0; i < D.size(); i++) acc(D[i], D0[i]);
+    return dev;
+}
+
+bool all_finite(const std::vector<double> &a)
+{
+    for (double x : a)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
 
 struct Candidate {
     int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves, grouped;
@@ -140,6 +170,8 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
     std::ostringstream rep;
     rep << "{\"key\": \"" << key << "\", \"source\": \"measured\", \"reps\": " << reps << ", \"candidates\": [";
     double bestMs = 1e300, worstDev = 0;
+    int nrejected = 0;
+    bool haveBest = false;
     for (size_t ci = 0; ci < cands.size(); ci++) {
         const Candidate &c = cands[ci];
         if (!apply(eng, c, err) || !eng.batch_upload(nvec, dec.data(), err) || !eng.batch_run(true, err) ||
@@ -163,19 +195,36 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
         const double med = ms[ms.size() / 2];
         double dev = 0;
         if (ci == 0) {
+            if (!all_finite(E) || !all_finite(D)) {
+                log_msg("Autotune: the first candidate %s gave non-finite results", cand_json(c).c_str());
+                std::string e2;
+                apply(eng, saved, e2);
+                return false;
+            }
             E0 = E;
             D0 = D;
         } else {
-            for (int v = 0; v < nvec; v++) dev = std::max(dev, std::fabs(E[v] - E0[v]));
-            for (size_t i = 0; i < D.size(); i++) dev = std::max(dev, std::fabs(D[i] - D0[i]));
+            dev = deviation(E, E0, D, D0);
         }
-        worstDev = std::max(worstDev, dev);
-        rep << (ci ? ", " : "") << "{\"choice\": " << cand_json(c) << ", \"ms\": " << med << ", \"max_dev\": " << dev
-            << "}";
-        if (med < bestMs) {
+        const bool rejected = !(dev <= kAgreeTol);
+        if (rejected) {
+            nrejected++;
+            log_msg("Autotune: candidate %s rejected, results deviate by %g", cand_json(c).c_str(), dev);
+        } else {
+            worstDev = std::max(worstDev, dev);
+        }
+        rep << (ci ? ", " : "") << "{\"choice\": " << cand_json(c) << ", \"ms\": " << med << ", \"max_dev\": "
+            << (std::isfinite(dev) ? dev : 1e308) << ", \"rejected\": " << (rejected ? "true" : "false") << "}";
+        if (!rejected && med < bestMs) {
             bestMs = med;
             best = c;
+            haveBest = true;
         }
+    }
+    if (!haveBest) {  // cannot happen: the first candidate agrees with itself
+        std::string e2;
+        apply(eng, saved, e2);
+        return false;
     }
     if (!apply(eng, best, err)) {
         log_msg("Autotune: applying the choice failed: %s", err.c_str());
@@ -190,7 +239,8 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
     }
     best.gemmB = eng.option(kOptGemmSolutionBeta);
     best.gemmA = eng.option(kOptGemmSolutionAlpha);
-    rep << "], \"max_dev\": " << worstDev << ", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs;
+    rep << "], \"max_dev\": " << worstDev << ", \"rejected\": " << nrejected << ", \"tolerance\": " << kAgreeTol
+        << ", \"choice\": " << cand_json(best) << ", \"ms\": " << bestMs;
     if (!gemmRep.empty()) rep << ", \"sigma_gemms\": " << gemmRep;
     rep << "}";
     report = rep.str();

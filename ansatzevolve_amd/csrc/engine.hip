@@ -274,6 +274,22 @@ struct EngineImpl {
     int bGraphNvec = 0;
     bool bGraphGrad = false;
     bool bGraphLanes = false;
+    // Addresses of every buffer the batch sequence reads or writes.  A
+    // captured graph holds them as kernel arguments, so a buffer that
+    // ensure() moved makes it stale: batch_upload drops the graphs (and bumps
+    // the generation, which multi-instance graphs key on) when the list
+    // changes, and batch_run re-checks it against the captured list.
+    std::vector<const void *> uploadPtrs, bGraphPtrs, egGraphPtrs;
+    // the same for the single-vector E+grad graph
+    std::vector<const void *> eg_ptrs() const
+    {
+        return {d_start.p, d_psi.p, d_phi.p, d_lam.p, d_cs.p, d_csneg.p, d_partial.p, d_epart.p, d_out.p};
+    }
+    std::vector<const void *> batch_ptrs() const
+    {
+        return {b_psi.p,      b_lam.p,   b_partial.p, b_epart.p, b_out.p,  b_cs.p,    b_csneg.p, bl_psi.p,
+                bl_lam.p,     bl_partial.p, bl_epart.p, bl_cs.p,  bl_csneg.p, bl_ck.p, d_suplist.p, d_lrows.p};
+    }
 
     bool use_lanes(int nvec) const { return laneMin > 0 && nvec >= laneMin; }
 
@@ -2152,6 +2168,10 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
         return true;
     };
     if (e.useGraph && !e.timingOn) {
+        if (e.egGraph && e.egGraphPtrs != e.eg_ptrs()) {
+            (void)hipGraphExecDestroy(e.egGraph);
+            e.egGraph = nullptr;
+        }
         if (!e.egGraph) {
             // captured once: the sin/cos tables and every buffer keep their
             // device addresses, so the graph replays for any angles.  One
@@ -2171,6 +2191,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
             const hipError_t ei = hipGraphInstantiate(&e.egGraph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             HCHK(ei);
+            e.egGraphPtrs = e.eg_ptrs();
         }
         HCHK(hipGraphLaunch(e.egGraph, e.stream));
     } else if (!enqueue()) {
@@ -2301,6 +2322,11 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
         if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
     }
+    std::vector<const void *> ptrs = e.batch_ptrs();
+    if (ptrs != e.uploadPtrs) {
+        e.drop_graphs();
+        e.uploadPtrs.swap(ptrs);
+    }
     e.bnvec = nvec;
     return true;
 }
@@ -2390,14 +2416,15 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
     const bool lanes = e.use_lanes(nvec);
     auto enqueue = [&]() -> bool { return batch_enqueue(with_gradient, err); };
     if (!e.useGraph || e.timingOn) return enqueue();
-    if (e.bGraph && (e.bGraphNvec != nvec || e.bGraphGrad != with_gradient || e.bGraphLanes != lanes)) {
+    if (e.bGraph && (e.bGraphNvec != nvec || e.bGraphGrad != with_gradient || e.bGraphLanes != lanes ||
+                     e.bGraphPtrs != e.batch_ptrs())) {
         (void)hipGraphExecDestroy(e.bGraph);
         e.bGraph = nullptr;
     }
     if (!e.bGraph) {
-        // buffers only grow between uploads of a batch size, so the captured
-        // addresses stay valid; a new size recaptures.  One eager run first:
-        // library GEMM kernels load lazily and must not load under capture.
+        // the graph is keyed on (size, gradient, layout) and on the buffer
+        // addresses it captures.  One eager run first: library GEMM kernels
+        // load lazily and must not load under capture.
         if (!enqueue()) return false;
         HCHK(hipStreamSynchronize(e.stream));
         hipGraph_t g = nullptr;
@@ -2415,6 +2442,7 @@ bool Engine::batch_run(bool with_gradient, std::string &err)
         e.bGraphNvec = nvec;
         e.bGraphGrad = with_gradient;
         e.bGraphLanes = lanes;
+        e.bGraphPtrs = e.batch_ptrs();
     }
     HCHK(hipGraphLaunch(e.bGraph, e.stream));
     return true;
@@ -2611,13 +2639,35 @@ bool Engine::tune_sigma_gemms(std::string &report, std::string &err)
     report.clear();
     if (!(e.hkind == 3 && e.sigv && e.bnvec >= 1 && e.use_lanes(e.bnvec) && e.blas)) return true;
     const int nch = (e.bnvec + kLanes - 1) / kLanes;
-    const double *x = e.bl_psi.p;
+    const long long ny = (long long)nch * e.N * kLanes;
+    double *x = e.bl_psi.p;
     double *y = e.bl_lam.p;
-    hipEvent_t t0, t1;
-    HCHK(hipEventCreate(&t0));
-    HCHK(hipEventCreate(&t1));
+    const unsigned fgrid = (unsigned)std::min<long long>((ny + kBlock - 1) / kBlock, 8192);
+    constexpr int kCmpBlocks = 1024;
+    // every solution's output is checked against the default solution's on
+    // a fixed input before it may be timed: a solution whose result differs
+    // by more than kTol relative to max |y|, or is not finite, is rejected
+    constexpr double kTol = 1e-10;
+    double *yref = nullptr, *cmp = nullptr;
+    HCHK(hipMalloc(&yref, ny * sizeof(double)));
+    if (hipMalloc(&cmp, 2 * kCmpBlocks * sizeof(double)) != hipSuccess) {
+        (void)hipFree(yref);
+        err = "hipMalloc of the comparison buffer failed";
+        return false;
+    }
+    std::vector<double> hcmp(2 * kCmpBlocks);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     std::string rep = "{";
-    bool ok = true;
+    bool ok = hipEventCreate(&t0) == hipSuccess && hipEventCreate(&t1) == hipSuccess;
+    // y = V x with solution sol into a cleared y; false when the library refuses it
+    auto run_once = [&](int which, rocblas_int sol) -> bool {
+        hipLaunchKernelGGL(k_fill, dim3(fgrid), dim3(kBlock), 0, e.stream, y, ny, 0.0);
+        return hipGetLastError() == hipSuccess && e.sigma_gemm(which, x, y, nch, sol) == rocblas_status_success;
+    };
+    if (ok) {
+        hipLaunchKernelGGL(k_pattern, dim3(fgrid), dim3(kBlock), 0, e.stream, x, ny);
+        ok = hipGetLastError() == hipSuccess;
+    }
     for (int which = 0; which < 2 && ok; which++) {
         rocblas_int cnt = 0;
         std::vector<rocblas_int> list;
@@ -2627,9 +2677,39 @@ bool Engine::tune_sigma_gemms(std::string &report, std::string &err)
             list.resize(std::min<size_t>(list.size(), (size_t)std::max<rocblas_int>(cnt, 0)));
         }
         list.insert(list.begin(), 0);  // the library's default first
-        int best = 0;
+        // the default solution's output is the reference
+        if (!run_once(which, 0)) {
+            ok = false;
+            break;
+        }
+        hipLaunchKernelGGL(k_copy, dim3(fgrid), dim3(kBlock), 0, e.stream, y, yref, ny);
+        if (hipGetLastError() != hipSuccess) {
+            ok = false;
+            break;
+        }
+        int best = 0, rejected = 0;
         float bestMs = 1e30f, defMs = 0;
         for (rocblas_int sol : list) {
+            if (sol != 0) {
+                if (!run_once(which, sol)) continue;
+                hipLaunchKernelGGL(k_absdiff_max, dim3(kCmpBlocks), dim3(kBlock), 0, e.stream, y, yref, ny, cmp);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(hcmp.data(), cmp, hcmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   e.stream) != hipSuccess ||
+                    hipStreamSynchronize(e.stream) != hipSuccess) {
+                    ok = false;
+                    break;
+                }
+                double dmax = 0, rmax = 0;
+                for (int b = 0; b < kCmpBlocks; b++) {
+                    dmax = std::max(dmax, hcmp[2 * b]);
+                    rmax = std::max(rmax, hcmp[2 * b + 1]);
+                }
+                if (!(dmax <= kTol * std::max(1.0, rmax))) {
+                    rejected++;
+                    continue;
+                }
+            }
             // one warm-up call, then the mean of three
             if (e.sigma_gemm(which, x, y, nch, sol) != rocblas_status_success) continue;
             if (hipEventRecord(t0, e.stream) != hipSuccess) { ok = false; break; }
@@ -2649,13 +2729,18 @@ bool Engine::tune_sigma_gemms(std::string &report, std::string &err)
             }
         }
         e.gemmSol[which] = best;
-        char buf[160];
-        std::snprintf(buf, sizeof(buf), "%s\"%s\": {\"solutions\": %d, \"default_ms\": %.4f, \"chosen\": %d, \"ms\": %.4f}",
-                      which ? ", " : "", which ? "alpha" : "beta", (int)list.size() - 1, defMs, best, bestMs);
+        char buf[200];
+        std::snprintf(buf, sizeof(buf),
+                      "%s\"%s\": {\"solutions\": %d, \"rejected\": %d, \"default_ms\": %.4f, \"chosen\": %d, "
+                      "\"ms\": %.4f}",
+                      which ? ", " : "", which ? "alpha" : "beta", (int)list.size() - 1, rejected, defMs, best, bestMs);
         rep += buf;
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    (void)hipStreamSynchronize(e.stream);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(yref);
+    (void)hipFree(cmp);
     if (!ok) {
         err = "timing the sigma DGEMMs failed";
         return false;

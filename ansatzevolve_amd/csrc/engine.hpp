@@ -124,8 +124,10 @@ public:
     int batch_size() const;
     // Times every rocBLAS solution of the lane-layout sigma's two same-spin
     // DGEMMs on the uploaded batch's buffers (contents are overwritten) and
-    // keeps the fastest; report: JSON with the counts and times.  Does
-    // nothing unless the batch runs the lane-layout matrix-free sigma.
+    // keeps the fastest of those whose output on a fixed input agrees with
+    // the default solution's (1e-10 relative); report: JSON with the counts,
+    // rejections and times.  Does nothing unless the batch runs the
+    // lane-layout matrix-free sigma.
     bool tune_sigma_gemms(std::string &report, std::string &err);
     bool synchronize(std::string &err);
 

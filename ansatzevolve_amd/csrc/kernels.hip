@@ -1,4 +1,6 @@
 // HIP kernels of the statevector engine (gfx950).  See kernels.hpp.
+#include <cfloat>
+
 #include "kernels.hpp"
 
 namespace ae {
@@ -7,6 +9,44 @@ __global__ void __launch_bounds__(kBlock) k_fill(double *__restrict__ p, long lo
 {
     for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
         p[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_absdiff_max(const double *__restrict__ a, const double *__restrict__ ref, long long n, double *__restrict__ out)
+{
+    __shared__ double sd[kBlock], sr[kBlock];
+    double d = 0, r = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const double x = fabs(a[i] - ref[i]);
+        d = x <= DBL_MAX ? fmax(d, x) : INFINITY;  // NaN and inf both count as infinitely far
+        r = fmax(r, fabs(ref[i]));
+    }
+    sd[threadIdx.x] = d;
+    sr[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + w]);
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sd[0];
+        out[2 * blockIdx.x + 1] = sr[0];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pattern(double *__restrict__ p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        p[i] = (double)((unsigned)(i * 2654435761ull) % 2048u) * (1.0 / 1024.0) - 1.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy(const double *__restrict__ src, double *__restrict__ dst, long long n)
+{
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        dst[i] = src[i];
 }
 
 __global__ void __launch_bounds__(kBlock)

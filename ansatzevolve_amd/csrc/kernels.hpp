@@ -142,6 +142,13 @@ struct OrbArgs {
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_fill(double *__restrict__ p, long long n, double v);
+// Per block b: out[2b] = max |a_i - ref_i| (+inf where that is not finite),
+// out[2b+1] = max |ref_i|, over a grid-stride range of n elements; kBlock threads.
+__global__ void k_absdiff_max(const double *__restrict__ a, const double *__restrict__ ref, long long n,
+                              double *__restrict__ out);
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, long long n);
+// p[i] = a fixed pseudo-random value in [-1, 1) (test inputs on the device)
+__global__ void k_pattern(double *__restrict__ p, long long n);
 
 template <bool BR>
 __global__ void k_orb(OrbArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,

@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# LDS: 256 CUs x 128 bytes per clock x 2.4 GHz peak engine clock
+LDS_PEAK_GBS = 256 * 128 * 2.4
+# results must agree with the reference build / the single-vector path
+E_TOL, G_TOL, SAME_TOL = 1e-10, 1e-9, 1e-10
 
 
 def parse():
@@ -54,10 +58,15 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(config, threads):
+def cpu_baseline(config, threads, reps=3):
     """The reference's own CPU build (oracle/_ref, compiled from its sources):
     FusedEvolve::evolve + evolveDerivative(&E) on one angle vector, its
-    matrix-free Hamiltonian path, in a child process with `threads` threads."""
+    matrix-free Hamiltonian path, in a child process with `threads` threads.
+    Set-up (integral files read, operator list built) and one warm-up E+grad
+    (the reference builds its fused-block cache lazily on the first evolve,
+    src/fusedevolve.cpp:2111-2112) are timed apart; the rate is the median of
+    `reps` further E+grads, as the reference's benchmark mode repeats its
+    evaluation (src/main.cpp:404-435)."""
     code = r"""
 import json, sys, time, os
 sys.path.insert(0, %r)
@@ -68,17 +77,33 @@ w = make_workload(%d)
 t0 = time.time()
 fe = RefFusedEvolve(w["nq"], w["nup"], w["ndown"], [w["start_idx"]], [1.0], w["ops"], w["order"],
                     w["scale"], w["h"], w["g"])
+setup = time.time() - t0
 x = w["scale"] * w["theta"][w["order"]]
-E, g, t = fe.egrad(x, reps=1)
-print("RESULT " + json.dumps({"t": t, "setup": time.time() - t0 - t, "threads": fe.threads(), "E": E}))
-""" % (ROOT, config)
+E, g, t_first = fe.egrad(x, reps=1)
+times = []
+for r in range(%d):
+    E, g, t = fe.egrad(x, reps=1)
+    times.append(t)
+    print("[cpu] E+grad %%d: %%.2f s" %% (r, t), file=sys.stderr, flush=True)
+print("RESULT " + json.dumps({"times": times, "first": t_first, "setup": setup, "threads": fe.threads(), "E": E}))
+""" % (ROOT, config, reps)
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = str(threads)
-    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=1200)
     for line in r.stdout.splitlines():
         if line.startswith("RESULT "):
             return json.loads(line[7:])
     raise RuntimeError("reference CPU run failed: " + r.stderr[-2000:])
+
+
+def host_cpu_share():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the
+    GPU hosts here allot each GPU a fixed CPU share and export it there),
+    else every logical CPU of the host."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v)
+    return os.cpu_count() or 1
 
 
 def main():
@@ -136,9 +161,25 @@ def main():
     a.set_kernel_timing(True)
     for i in range(nsingle):
         a.energy_gradient(singles[i + 3])
-    single_breakdown = {name: a.kernel_timing(c)[0] / nsingle
-                        for c, name in ((0, "hamiltonian"), (1, "forward_sweep"), (2, "adjoint_sweep"))}
+    single_breakdown = {}
+    single_launches = 0
+    for c, name in ((0, "hamiltonian"), (1, "forward_sweep"), (2, "adjoint_sweep"), (3, "same_spin_dgemm")):
+        ms, cnt = a.kernel_timing(c)
+        single_breakdown[name] = ms / nsingle
+        single_launches += cnt
     a.set_kernel_timing(False)
+    single_ms = 1000.0 / single_rate
+    # one vector of 853,776 amplitudes is 6.8 MB: it stays in the on-chip
+    # caches, and the sequence is a chain of short dependent launches; its
+    # bound is launch latency, reported as the average time per launch
+    single_bound = {
+        "bound": "launch latency",
+        "ms_per_egrad": single_ms,
+        "sweep_segments_per_sweep": a.get_option(a.OPT_SEGMENTS),
+        "timed_intervals_per_egrad": single_launches / nsingle,
+        "ideal_hbm_bytes_per_egrad": 40.0 * N,
+        "hbm_frac_of_ideal_bytes": 40.0 * N / (single_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
 
     tune = None
     if not args.no_autotune:
@@ -211,11 +252,18 @@ def main():
         # lane-layout sweeps time each launch; other paths one interval per sweep
         return ms / max(1, cnt * per_interval)
 
-    def hbm(kernel, bytes_launch, avg_ms, launches, per_step_ms):
+    def hbm(kernel, bytes_launch, avg_ms, launches, per_step_ms, per_pass_bytes=None):
         ach = bytes_launch / (avg_ms * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": kernel, "avg_launch_ms": avg_ms, "launches": launches,
-                "algorithmic_bytes_per_launch": bytes_launch, "ms_per_step": per_step_ms}
+        r = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+             "traffic": None, "kernel": kernel, "avg_launch_ms": avg_ms, "launches": launches,
+             "algorithmic_bytes_per_launch": bytes_launch, "ms_per_step": per_step_ms}
+        if per_pass_bytes is not None:
+            # the bytes each launch actually has to move (its own pass over
+            # the amplitudes it touches): the streaming efficiency, apart
+            # from the fusion gap that `frac` includes
+            pa = per_pass_bytes / (avg_ms * 1e-3) / 1e9
+            r["per_pass"] = {"bytes_per_launch": per_pass_bytes, "achieved": pa, "frac": pa / HBM_PEAK_GBS}
+        return r
 
     def fp64(kernel, flops_launch, avg_ms, launches, per_step_ms, note):
         ach = flops_launch / (avg_ms * 1e-3) / 1e12
@@ -224,52 +272,79 @@ def main():
                 "launches": launches, "algorithmic_flops_per_launch": flops_launch, "ms_per_step": per_step_ms,
                 "note": note}
 
-    # sweeps: 8 B per amplitude per access.  Vector-major: every segment
-    # reads and writes each vector once (forward: psi; adjoint: psi and
-    # lambda).  Lane layout: the amplitudes the engine reports per step
-    # (summed over the segments): the forward sweep reads and writes the
+    # sweeps.  `frac` counts the ideal-fusion bytes: every vector read and
+    # written once per sweep, 16 B per amplitude per vector, spread over the
+    # sweep's launches.  `per_pass` counts what each launch has to move: 8 B
+    # per amplitude per access over the amplitudes the engine reports per
+    # step (summed over the segments): the forward sweep reads and writes the
     # reachable part of each state, the adjoint sweep reads and writes lambda
     # on every determinant some operator of the segment acts on, and reads
     # the stored state there where it can be nonzero (with stored states;
-    # in place it reads and writes psi like lambda)
+    # in place it reads and writes psi like lambda).  Vector-major: every
+    # segment reads and writes each vector once (forward: psi; adjoint: psi
+    # and lambda).
     k_fwd = ("k_orb2v<false>" if opts["lane_groups_fwd"] == 2 else "k_orbv<false>") + " (lane layout)" if lanes \
         else "k_orb<false>"
     k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
         else "k_orb<true>"
     if lanes:
         fwd_launches, rev_launches = max(1, fwd_n // nbd), max(1, rev_n // nbd)
-        fwd_bytes = 16.0 * amp["forward"] * args.nvec / fwd_launches
+        fwd_pass = 16.0 * amp["forward"] * args.nvec / fwd_launches
         rev_state = 8.0 * amp["adjoint_state"] if opts.get("checkpoint_active") else 16.0 * amp["adjoint_lambda"]
-        rev_bytes = (16.0 * amp["adjoint_lambda"] + rev_state) * args.nvec / rev_launches
+        rev_pass = (16.0 * amp["adjoint_lambda"] + rev_state) * args.nvec / rev_launches
     else:
-        fwd_bytes = 16.0 * N * args.nvec
-        rev_bytes = (24.0 if opts.get("checkpoint_active") else 32.0) * N * args.nvec
-    rooflines["forward_sweep"] = hbm(k_fwd, fwd_bytes, per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
-                                     fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd)
-    rooflines["adjoint_sweep"] = hbm(k_rev, rev_bytes, per_launch(rev_ms, rev_n, 1 if lanes else nseg),
-                                     rev_n if lanes else rev_n * nseg, rev_ms / nbd)
+        fwd_launches = rev_launches = max(1, nseg)
+        fwd_pass = 16.0 * N * args.nvec
+        rev_pass = (24.0 if opts.get("checkpoint_active") else 32.0) * N * args.nvec
+    ideal_sweep = 16.0 * N * args.nvec  # one read + one write of every vector per sweep
+    rooflines["forward_sweep"] = hbm(k_fwd, ideal_sweep / fwd_launches,
+                                     per_launch(fwd_ms, fwd_n, 1 if lanes else nseg),
+                                     fwd_n if lanes else fwd_n * nseg, fwd_ms / nbd, fwd_pass)
+    rooflines["adjoint_sweep"] = hbm(k_rev, ideal_sweep / rev_launches,
+                                     per_launch(rev_ms, rev_n, 1 if lanes else nseg),
+                                     rev_n if lanes else rev_n * nseg, rev_ms / nbd, rev_pass)
     if ham_kind == 3 and gemm_n > 0:
-        # lane-layout matrix-free sigma: the sparse kernel's algorithmic work
-        # per element = alpha-beta doubles + beta-diagonal parts of alpha
-        # singles + alpha-dependent parts of beta singles + diagonal, 2 flops
-        # each per vector; the DGEMMs cover the same-spin rows densely
-        per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
         halves = a.get_option(a.OPT_SIGMA_HALVES) and a.get_option(a.OPT_SIGMA_HALVES_AVAILABLE)
         grouped = a.get_option(a.OPT_SIGMA_GROUPED) and a.get_option(a.OPT_SIGMA_GROUPED_AVAILABLE)
         sig_name = "k_sigg_sparse" if grouped else "k_sigh_sparse" if halves else "k_sigw_sparse"
-        rooflines["hamiltonian"] = fp64("%s (lane-layout sigma, %d vectors)" % (sig_name, args.nvec),
-                                        2.0 * N * per_el * args.nvec, ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
-                                        "FP64 vector-ALU/LDS work priced against the FP64 peak")
+        # the sparse lane sigma is bound by LDS reads: one workgroup per
+        # (alpha row, 16-vector sub-chunk) stages 1 + ku (n - ku) source rows
+        # (identity and the alpha singles), each 128 B per beta string, and
+        # reads them back as 512-byte wave reads (64 lanes x 8 B); the
+        # grouped kernel's generated programs make a fixed number of reads
+        # per staged row, the one-row kernel one read per entry and target
+        per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
         na = math.comb(n_, ku)
         nb = math.comb(n_, kd)
+        nsub = 4 * math.ceil(args.nvec / 64)
+        nsrc = 1 + ku * (n_ - ku)
+        if grouped:
+            from ansatzevolve_amd import sigg
+            reads = sum(len(sigg.program(kd, m)) for wv in sigg.wave_plan(kd) for (m, _, _) in wv)
+        else:
+            # one read per (entry, four targets): the beta singles and the diagonal of every target
+            reads = (nb + 3) // 4 * (kd * (n_ - kd) + 1)
+        lds_bytes = na * nsub * nsrc * (reads * 512.0 + nb * 128.0)
+        lds_peak = LDS_PEAK_GBS
+        avg = ham_ms / max(ham_n, 1)
+        ach = lds_bytes / (avg * 1e-3) / 1e9
+        flops = 2.0 * N * per_el * args.nvec
+        rooflines["hamiltonian"] = {
+            "bound": "lds", "achieved": ach, "peak": lds_peak, "unit": "GB/s", "frac": ach / lds_peak,
+            "traffic": None, "kernel": "%s (lane-layout sigma, %d vectors)" % (sig_name, args.nvec),
+            "avg_launch_ms": avg, "launches": ham_n, "algorithmic_lds_bytes_per_launch": lds_bytes,
+            "lds_wave_reads_per_staged_row": reads, "ms_per_step": ham_ms / nbd,
+            "fp64_tflops": flops / (avg * 1e-3) / 1e12,
+            "note": "LDS peak = 256 CUs x 128 B/clock x 2.4 GHz; FP64 work (no MFMA) for reference"}
         gflops = 2.0 * na * na * nb * 64 * math.ceil(args.nvec / 64) + 2.0 * nb * nb * 64 * na * math.ceil(args.nvec / 64)
         rooflines["same_spin_dgemm"] = fp64("rocBLAS dgemm_strided_batched x2 (dense same-spin rows)", gflops,
                                             gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd,
                                             "dense FLOPs of both GEMMs on FP64 MFMA")
     elif ham_kind == 3:
-        rooflines["hamiltonian"] = fp64("k_mf_sigma (matrix-free sigma)", 2.0 * nnz * args.nvec / max(1.0, ham_n / nbd),
-                                        ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd,
-                                        "FP64 vector-ALU work priced against the FP64 peak")
+        r = fp64("k_mf_sigma (matrix-free sigma)", 2.0 * nnz * args.nvec / max(1.0, ham_n / nbd),
+                 ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd, "FP64 vector-ALU work priced against the FP64 peak")
+        r["bound"] = "valu"
+        rooflines["hamiltonian"] = r
     else:
         # explicit rows: 12 B per stored entry (8 B value + 4 B column) per
         # launch + 16 B per amplitude per vector (x read, y written)
@@ -312,23 +387,33 @@ def main():
         "autotune": tune,
         "single_vector_egrad_per_s": single_rate,
         "single_vector_breakdown_ms": single_breakdown,
+        "single_vector_bound": single_bound,
         "setup_s": setup_s,
         "parity": parity,
         "batch_vs_single_max_diff": batch_consistency,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref_ansatz.so")):
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpu_share()
         print("[bench] reference CPU baseline on %d threads" % threads, file=sys.stderr, flush=True)
         try:
             cb = cpu_baseline(args.config, threads)
+            med = float(np.median(cb["times"]))
             result["cpu_baseline"] = {
-                "value": 1.0 / cb["t"],
+                "value": 1.0 / med,
                 "unit": "E+grad evals/s",
                 "cores": cb["threads"],
                 "kind": "reference",
-                "sample": "one E+grad (FusedEvolve::evolve + evolveDerivative) at theta_0 of the same workload, "
-                          "reference CPU build, its matrix-free Hamiltonian path; %.1f s" % cb["t"],
+                "sample": "median of %d E+grads (FusedEvolve::evolve + evolveDerivative) at theta_0 of the same "
+                          "workload after one warm-up E+grad, reference CPU build, its matrix-free Hamiltonian "
+                          "path, %d threads (host logical CPUs: %d): %s s; set-up %.1f s and warm-up %.1f s "
+                          "not counted"
+                          % (len(cb["times"]), cb["threads"], os.cpu_count() or 0,
+                             ", ".join("%.2f" % t for t in cb["times"]), cb["setup"], cb["first"]),
+                "times_s": cb["times"],
+                "setup_s": cb["setup"],
+                "warmup_s": cb["first"],
+                "host_logical_cpus": os.cpu_count(),
                 "dE_vs_gpu": abs(cb["E"] - a.energy(w["theta"])),
             }
         except Exception as ex:  # report, do not hide the GPU number
@@ -360,6 +445,16 @@ def main():
         except Exception as ex:
             result["warm_start"] = {"error": str(ex)[:300]}
     print(json.dumps(_finite(result)))
+    # a throughput figure only counts when the results are right
+    bad = []
+    if not batch_consistency <= SAME_TOL:
+        bad.append("batch vs single-vector results differ by %g" % batch_consistency)
+    if parity is not None and not (parity["dE"] <= E_TOL and parity["max_dgrad"] <= G_TOL):
+        bad.append("results differ from the reference CPU build's golden vector: dE %g, dgrad %g"
+                   % (parity["dE"], parity["max_dgrad"]))
+    if bad:
+        print("[bench] FAILED: " + "; ".join(bad), file=sys.stderr, flush=True)
+        sys.exit(3)
 
 
 def _finite(o):

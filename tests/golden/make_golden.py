@@ -5,6 +5,8 @@ built from the reference sources by `make -C oracle ref`).
 Inputs are regenerated from ansatzevolve_amd.workloads (seeded); each
 integral set is stored with a checksum so a changed generator is caught.
 Run in the build container:  python tests/golden/make_golden.py [--big]
+    python tests/golden/make_golden.py --cfg4   (adds only the 14-orbital
+    workload, 11,778,624 determinants, to the existing file; tens of minutes)
 """
 import hashlib
 import json
@@ -51,7 +53,40 @@ def integral_case(n, layers, seed, nsets, hessian=True):
     return out
 
 
+def cfg4_only():
+    """E and gradient of the 14-orbital workload at theta_0 and one more
+    seeded angle vector, by the reference's matrix-free path (its
+    FusedEvolve built as its benchmark mode does), merged into OUT."""
+    old = np.load(OUT, allow_pickle=False)
+    data = {key: old[key] for key in old.files}
+    meta = json.loads(str(old["meta_json"]))
+    w = make_workload(4)
+    fe = RefFusedEvolve(w["nq"], w["nup"], w["ndown"], [w["start_idx"]], [1.0], w["ops"], w["order"],
+                        w["scale"], w["h"], w["g"])
+    th = np.stack([w["theta"], seeded_angles(w["P"], w["seed"] + 2)])
+    E, G, T = [], [], []
+    for t in th:
+        t0 = time.time()
+        e, g, tt = fe.egrad(w["scale"] * t[w["order"]], reps=1)
+        E.append(e)
+        G.append(g)
+        T.append(tt)
+        print("cfg4 E=%.15f %.1fs" % (e, time.time() - t0), file=sys.stderr, flush=True)
+    data["cfg4_theta"] = th
+    data["cfg4_E"] = np.array(E)
+    data["cfg4_grad"] = np.array(G)
+    meta["cfg4"] = {"config": 4, "seed": w["seed"], "digest": digest(w["h"], w["g"]),
+                    "ref_egrad_seconds": T, "ref_threads": fe.threads()}
+    fe.close()
+    data["meta_json"] = np.array(json.dumps(meta))
+    np.savez_compressed(OUT, **data)
+    print("wrote", OUT, file=sys.stderr)
+
+
 def main():
+    if "--cfg4" in sys.argv:
+        cfg4_only()
+        return
     big = "--big" in sys.argv
     data = {}
     meta = {}
@@ -114,12 +149,13 @@ def main():
         # keep previously generated large cases
         old = np.load(OUT, allow_pickle=False)
         for key in old.files:
-            if key.startswith("cfg3") and key not in data:
+            if key.startswith(("cfg3", "cfg4")) and key not in data:
                 data[key] = old[key]
         om = json.loads(str(old["meta_json"]))
-        if "cfg3" in om:
-            meta["cfg3"] = om["cfg3"]
-            data["meta_json"] = np.array(json.dumps(meta))
+        for c in ("cfg3", "cfg4"):
+            if c in om and c not in meta:
+                meta[c] = om[c]
+        data["meta_json"] = np.array(json.dumps(meta))
     np.savez_compressed(OUT, **data)
     print("wrote", OUT, file=sys.stderr)
 

@@ -449,3 +449,63 @@ def test_multi_instance_batch_uneven(gpu, storage):
     finally:
         for a in ctxs:
             a.close()
+
+
+@pytest.mark.parametrize("storage", [1, 2], ids=["rows", "matrixfree"])
+def test_reupload_after_larger_batch_recaptures(gpu, storage):
+    """A batch graph captures its buffers' addresses.  Uploading a larger
+    batch grows (moves) those buffers; re-uploading the first size and
+    replaying must then recapture, not replay the stale graph: the result
+    equals single-vector calls.  The same through a multi-instance graph
+    whose context was used alone at a larger size in between."""
+    from ansatzevolve_amd import MultiBatch
+    w, thetas = _tups_case(8, 3, 777, 64)
+    small = thetas[:16]
+    with from_workload(w, storage=storage) as a:
+        ref = [a.energy_gradient(t) for t in small]
+        a.upload_batch(small)
+        a.run_batch(True)
+        E, G = a.download_batch(True)
+        a.upload_batch(thetas)          # grows the batch buffers, no run
+        a.upload_batch(small[::-1])     # back to 16 vectors: same graph key as before
+        a.run_batch(True)
+        E2, G2 = a.download_batch(True)
+        for v in range(16):
+            assert abs(E[v] - ref[v][0]) <= SAME and np.max(np.abs(G[v] - ref[v][1])) <= SAME
+            assert abs(E2[15 - v] - ref[v][0]) <= SAME and np.max(np.abs(G2[15 - v] - ref[v][1])) <= SAME
+    w2, th2 = _tups_case(6, 2, 778, 64)
+    ctxs = [from_workload(w, storage=storage), from_workload(w2, storage=storage)]
+    try:
+        refs = [[c.energy_gradient(t) for t in th[:16]] for c, th in zip(ctxs, (thetas, th2))]
+        with MultiBatch(ctxs) as mb:
+            for rep in range(2):
+                mb.upload(0, thetas[:16])
+                mb.upload(1, th2[:16])
+                mb.run(True)
+                for i in range(2):
+                    E, G = mb.download(i)
+                    for v in range(16):
+                        assert abs(E[v] - refs[i][v][0]) <= SAME, (rep, i, v)
+                        assert np.max(np.abs(G[v] - refs[i][v][1])) <= SAME, (rep, i, v)
+                # context 0 alone at 64 vectors: its buffers move
+                ctxs[0].energy_gradients(thetas)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_autotune_rejects_nothing_and_reports_tolerance(gpu):
+    """The autotuner's report carries its agreement tolerance and rejected
+    count; on a correct library no candidate and no DGEMM solution is
+    rejected, and every candidate's deviation is within the tolerance."""
+    w = make_workload(2)
+    rng = np.random.default_rng(8)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-np.pi, np.pi, size=(64, w["P"]))
+    with from_workload(w, storage=2) as a:
+        rep = a.autotune(thetas, reps=1)
+        assert rep["tolerance"] == 1e-10
+        assert rep["rejected"] == 0
+        assert all(not c["rejected"] and c["max_dev"] <= SAME for c in rep["candidates"])
+        if rep["choice"]["lane_min"] == 1:
+            for part in ("beta", "alpha"):
+                assert rep["sigma_gemms"][part]["rejected"] == 0

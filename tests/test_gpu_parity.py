@@ -149,6 +149,67 @@ def test_config3_energy_gradient(gpu, storage):
         assert abs(Eb[0] - E) <= 1e-12 and np.max(np.abs(Gb[0] - g)) <= 1e-12
 
 
+def test_config3_benchmarked_batch_path(gpu):
+    """The path the throughput benchmark times: 12 orbitals, 853,776
+    determinants, TUPS L=6, a 64-vector batch after the autotuner (lane
+    layout, stored forward states, reachable-amplitude lists, grouped-target
+    sigma, tuned DGEMM solutions), replayed from its hipGraph.  Vector 0 is
+    theta_0 and is checked against the reference CPU build's golden vector;
+    other vectors against single-vector calls."""
+    gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
+    if "cfg3_E" not in gz.files:
+        pytest.skip("no 12-orbital golden vector")
+    w = make_workload(3)
+    assert np.array_equal(gz["cfg3_theta"][0], w["theta"])
+    rng = np.random.default_rng(w["seed"] + 3)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-np.pi, np.pi, size=(64, w["P"]))
+    thetas[0] = w["theta"]
+    with from_workload(w) as a:
+        rep = a.autotune(thetas, reps=1)
+        assert rep["rejected"] == 0
+        # the benchmarked choice whatever the timing said: lane layout, two
+        # orbital groups per pass, stored states (the tuned DGEMMs stay)
+        for key, val in ((AnsatzSynth.OPT_LANE_MIN, 1), (AnsatzSynth.OPT_LANE_GROUPS, 2),
+                         (AnsatzSynth.OPT_LANE_GROUPS_FWD, 2), (AnsatzSynth.OPT_CHECKPOINT, 1)):
+            a.set_option(key, val)
+        a.upload_batch(thetas)
+        assert a.get_option(AnsatzSynth.OPT_HAM_KIND) == 3
+        assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1
+        assert a.get_option(AnsatzSynth.OPT_REACHABLE) == 1
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED_AVAILABLE) == 1
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED) == 1
+        for _ in range(3):  # capture, then replays
+            a.run_batch(True)
+        E, G = a.download_batch(True)
+        assert abs(E[0] - gz["cfg3_E"][0]) <= E_TOL
+        assert np.max(np.abs(G[0] - gz["cfg3_grad"][0])) <= G_TOL
+        for v in (1, 17, 40, 63):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E[v] - Es) <= 1e-12, v
+            assert np.max(np.abs(G[v] - gs)) <= 1e-11, v
+
+
+def test_config4_energy_gradient(gpu):
+    """14 orbitals, 11,778,624 determinants, TUPS L=7 (273 parameters, 455
+    operators), matrix-free sigma: energy and gradient at theta_0 and at a
+    second seeded angle vector (far from any stationary point) against the
+    reference's own CPU build."""
+    gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
+    if "cfg4_E" not in gz.files:
+        pytest.skip("no 14-orbital golden vector")
+    w = make_workload(4)
+    assert np.array_equal(gz["cfg4_theta"][0], w["theta"])
+    with from_workload(w) as a:
+        d, M, P = a.dimensions()
+        assert (d, M, P) == (11778624, 455, 273)
+        assert a.get_option(AnsatzSynth.OPT_HAM_KIND) == 3
+        for i, th in enumerate(gz["cfg4_theta"]):
+            E, g = a.energy_gradient(th)
+            assert abs(E - gz["cfg4_E"][i]) <= E_TOL, (i, E)
+            assert np.max(np.abs(g - gz["cfg4_grad"][i])) <= G_TOL, i
+            assert np.linalg.norm(g) > 1e-2  # not a stationary point
+
+
 HAMS_KEYS = ["H4_L1_", "H4_L5_", "H6_L1_", "H6_L5_", "H8_L1_", "H10_L1_", "H10_L5_", "H14_L5_"]
 
 
