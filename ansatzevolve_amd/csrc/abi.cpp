@@ -40,15 +40,15 @@ void setTraceInterfaceCalls(int val) { traceInterfaceCalls = val != 0; }
 void *init(void)
 {
     Manager *ptr = new (std::nothrow) Manager;
-    if (traceInterfaceCalls) log_msg("Constructed object %p", (void *)ptr);
+    if (traceInterfaceCalls) log_msg("init: new context %p", (void *)ptr);
     return ptr;
 }
 
 int cleanup(void *ctx)
 {
-    if (traceInterfaceCalls) log_msg("cleanup called with void** %p", ctx);
+    if (traceInterfaceCalls) log_msg("cleanup(%p)", ctx);
     if (ctx == nullptr) {
-        log_msg("(void**) ctx is nullptr, something has gone wrong. Potential Leak");
+        log_msg("cleanup: the handle pointer is null, so the context (if any) cannot be released");
         return 1;
     }
     void **myCtx = (void **)ctx;
@@ -66,17 +66,17 @@ int setExcitation(int nparams, const int *operators, const int *orderfile, void 
 int setExcitationScale(int nparams, const int *operators, const int *orderfile, const double *scale, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (nparams < 1) {
-        log_msg("nparams < 1");
+        log_msg("setExcitation: the operator count is below 1");
         return 2;
     }
     if (traceInterfaceCalls) {
-        log_msg("setExcitationScale called with nParams %d operators %s orderfile %s", nparams,
+        log_msg("setExcitationScale(nparams=%d, operators=[%s], order=[%s])", nparams,
                 join(operators, 4 * nparams).c_str(), join(orderfile, nparams).c_str());
-        if (scale) log_msg("scale %s", join(scale, nparams).c_str());
+        if (scale) log_msg("  scales [%s]", join(scale, nparams).c_str());
     }
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
@@ -84,7 +84,7 @@ int setExcitationScale(int nparams, const int *operators, const int *orderfile, 
     for (int i = 0; i < nparams; i++)
         for (int j = 0; j < 4; j++) {
             if (operators[nparams * j + i] - 1 > std::numeric_limits<int8_t>::max()) {
-                log_msg("Excitation out of bounds, max is 128: %d", operators[nparams * j + i]);
+                log_msg("setExcitation: orbital index %d is above 128", operators[nparams * j + i]);
                 return 3;
             }
             excs[i][j] = (int8_t)(operators[nparams * j + i] - 1);  // Fortran order
@@ -93,7 +93,7 @@ int setExcitationScale(int nparams, const int *operators, const int *orderfile, 
     std::vector<std::pair<int, double>> dep;
     for (int i = 0; i < nparams; i++) {
         if (orderfile[i] > nparams) {
-            log_msg("orderfile out of bounds: %d", orderfile[i]);
+            log_msg("setExcitation: parameter index %d is above the operator count", orderfile[i]);
             return 5;
         }
         dep.push_back({orderfile[i] - 1, scale ? scale[i] : 1.0});
@@ -105,16 +105,16 @@ int setExcitationScale(int nparams, const int *operators, const int *orderfile, 
 int setHamiltonian(int n, const int *iindexes, const int *jindexes, const double *coeffs, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (traceInterfaceCalls)
-        log_msg("setHamiltonian called with N %d iIndexes %s jIndexes %s", n, join(iindexes, n).c_str(),
+        log_msg("setHamiltonian(n=%d, i=[%s], j=[%s], ...)", n, join(iindexes, n).c_str(),
                 join(jindexes, n).c_str());
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     if (n < 1) {
-        log_msg("N < 1");
+        log_msg("the entry count is below 1");
         return 2;
     }
     std::vector<int> iv(iindexes, iindexes + n), jv(jindexes, jindexes + n);
@@ -128,20 +128,20 @@ int setHamiltonian(int n, const int *iindexes, const int *jindexes, const double
 int setInitialState(int numqubits, int n, const int *iindexes, const double *coeffs, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (traceInterfaceCalls)
-        log_msg("setInitialState called with numQubits %d N %d iIndexes %s coeffs %s", numqubits, n,
+        log_msg("setInitialState(qubits=%d, n=%d, indexes=[%s], coeffs=[%s])", numqubits, n,
                 join(iindexes, n).c_str(), join(coeffs, n).c_str());
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     if (n < 1) {
-        log_msg("N < 1");
+        log_msg("the entry count is below 1");
         return 2;
     }
     if (numqubits < 1) {
-        log_msg("numQubits < 1");
+        log_msg("the qubit count is below 1");
         return 2;
     }
     std::vector<int> iv(iindexes, iindexes + n);
@@ -153,25 +153,25 @@ int setInitialState(int numqubits, int n, const int *iindexes, const double *coe
 
 int setInitialStateComplex(int, int, const int *, const ANSATZ_DOUBLE_COMPLEX *, void *)
 {
-    log_msg("Cannot set a complex initial state without building in complex mode");
+    log_msg("setInitialStateComplex: this library evaluates real amplitudes only");
     return 4;
 }
 
 int getEnergy(int nangles, const double *angles, double *energy, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 3;
     }
-    if (traceInterfaceCalls) log_msg("getEnergy called with NAngles %d", nangles);
+    if (traceInterfaceCalls) log_msg("getEnergy(nangles=%d)", nangles);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     double E = 0;
@@ -183,32 +183,32 @@ int getEnergy(int nangles, const double *angles, double *energy, void *ctx)
 int getFinalState(int nangles, const double *angles, int nbasisvectors, double *finalstate, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (finalstate == nullptr) {
-        log_msg("nullptr finalState passed");
+        log_msg("the final-state output array is null");
         return 3;
     }
     if (nbasisvectors < 1) {
-        log_msg("NBasisVectors < 1");
+        log_msg("getFinalState: the output length is below 1");
         return 4;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 5;
     }
-    if (traceInterfaceCalls) log_msg("getFinalState called with NAngles %d NBasisVectors %d", nangles, nbasisvectors);
+    if (traceInterfaceCalls) log_msg("getFinalState(nangles=%d, nbasis=%d)", nangles, nbasisvectors);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     std::vector<double> state;
     if (!m->getFinalState(std::vector<double>(angles, angles + nangles), state)) return 6;
     if (state.size() != (size_t)nbasisvectors) {
-        log_msg("Final state does not have the right number allocated, NBasisVectors should be %zu", state.size());
+        log_msg("getFinalState: the output holds %d amplitudes, the full space has %zu", nbasisvectors, state.size());
         return 7;
     }
     std::memcpy(finalstate, state.data(), state.size() * sizeof(double));
@@ -217,35 +217,35 @@ int getFinalState(int nangles, const double *angles, int nbasisvectors, double *
 
 int getFinalStateComplex(int, const double *, int, ANSATZ_DOUBLE_COMPLEX *, void *)
 {
-    log_msg("Cannot get a complex final state without building in complex mode");
+    log_msg("getFinalStateComplex: this library evaluates real amplitudes only");
     return 4;
 }
 
 int getGradient_COMP(int nangles, const double *angles, double *gradient, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (gradient == nullptr) {
-        log_msg("nullptr gradient passed");
+        log_msg("the gradient output array is null");
         return 3;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 4;
     }
-    if (traceInterfaceCalls) log_msg("getGradient_COMP called with NAngles %d", nangles);
+    if (traceInterfaceCalls) log_msg("getGradient_COMP(nangles=%d)", nangles);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     std::vector<double> g;
     if (!m->getGradientComp(std::vector<double>(angles, angles + nangles), g)) return 5;
     if ((size_t)nangles != g.size()) {
-        log_msg("Gradient is the wrong size, This is a Bug on the implementation side");
+        log_msg("internal error: the gradient came back with the wrong length");
         return -6;
     }
     std::memcpy(gradient, g.data(), g.size() * sizeof(double));
@@ -255,22 +255,22 @@ int getGradient_COMP(int nangles, const double *angles, double *gradient, void *
 int getGradientOperators(int nangles, const double *angles, double *gradient, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (gradient == nullptr) {
-        log_msg("nullptr gradient passed");
+        log_msg("the gradient output array is null");
         return 3;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 4;
     }
-    if (traceInterfaceCalls) log_msg("getGradientOperators called with NAngles %d", nangles);
+    if (traceInterfaceCalls) log_msg("getGradientOperators(nangles=%d)", nangles);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     std::vector<double> g;
@@ -283,29 +283,29 @@ int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, con
                        double *overlap, double *gradient, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr || projvector == nullptr) {
-        log_msg("nullptr angles or projection vector passed");
+        log_msg("the angle or projection array is null");
         return 2;
     }
     if (gradient == nullptr) {
-        log_msg("nullptr gradient passed");
+        log_msg("the gradient output array is null");
         return 3;
     }
     if (nangles < 1 || nbasisvectors < 1) {
-        log_msg("NAngles < 1 or nbasisvectors < 1");
+        log_msg("the angle count or the basis size is below 1");
         return 4;
     }
-    if (traceInterfaceCalls) log_msg("getOverlapGradient called with NAngles %d", nangles);
+    if (traceInterfaceCalls) log_msg("getOverlapGradient(nangles=%d)", nangles);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     long long dim = 0;
     int nops = 0, nparams = 0;
     if (!m->getDimensions(dim, nops, nparams)) return 5;
     if (nangles != nparams) {
-        log_msg("Gradient is the wrong size");
+        log_msg("gradient length does not match the parameter count");
         return -6;
     }
     std::vector<double> g;
@@ -320,29 +320,29 @@ int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, con
 int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (hessian == nullptr) {
-        log_msg("nullptr hessian passed");
+        log_msg("the Hessian output array is null");
         return 3;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 4;
     }
-    if (traceInterfaceCalls) log_msg("getHessian_COMP called with NAngles %d", nangles);
+    if (traceInterfaceCalls) log_msg("getHessian_COMP(nangles=%d)", nangles);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     std::vector<double> h;
     if (!m->getHessianComp(std::vector<double>(angles, angles + nangles), h)) return 5;
     const int P = m->numberOfParameters();
     if (nangles != P) {
-        log_msg("Hessian has wrong dimensions, This is a bug on the implementation side");
+        log_msg("internal error: the Hessian came back with the wrong size");
         return -6;
     }
     for (int i = 0; i < P; i++)
@@ -354,14 +354,14 @@ int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ct
 int setHamiltonianIntegrals(int norb, const double *h, const double *g, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (norb < 1 || h == nullptr || g == nullptr) {
         log_msg("norb < 1 or integrals missing");
         return 2;
     }
-    if (traceInterfaceCalls) log_msg("setHamiltonianIntegrals called with norb %d", norb);
+    if (traceInterfaceCalls) log_msg("setHamiltonianIntegrals(norb=%d)", norb);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     if (!m->storeIntegrals(norb, h, g)) return 3;
@@ -371,7 +371,7 @@ int setHamiltonianIntegrals(int norb, const double *h, const double *g, void *ct
 int setNuclearEnergy(double energy, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -383,14 +383,14 @@ int setNuclearEnergy(double energy, void *ctx)
 int setRunPath(const char *runpath, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (runpath == nullptr || runpath[0] == 0) {
-        log_msg("Run path has zero size");
+        log_msg("setRunPath: empty path");
         return 2;
     }
-    if (traceInterfaceCalls) log_msg("setRunPath called with %s", runpath);
+    if (traceInterfaceCalls) log_msg("setRunPath(%s)", runpath);
     Manager *m = static_cast<Manager *>(ctx);
     std::lock_guard<std::mutex> lock(m->lock);
     if (!m->storeRunPath(runpath)) return 3;
@@ -400,11 +400,11 @@ int setRunPath(const char *runpath, void *ctx)
 int loadRunFiles(const char *prefix, int withnuclear, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (prefix == nullptr || prefix[0] == 0) {
-        log_msg("Run path has zero size");
+        log_msg("setRunPath: empty path");
         return 2;
     }
     if (traceInterfaceCalls) log_msg("loadRunFiles called with %s, withnuclear %d", prefix, withnuclear);
@@ -466,7 +466,7 @@ int readParameterSets(const char *path, int maxsets, int maxparams, int *nsets, 
 int setHamiltonianStorage(int mode, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (mode < 0 || mode > 2) {
@@ -482,19 +482,19 @@ int setHamiltonianStorage(int mode, void *ctx)
 int getEnergyGradient(int nangles, const double *angles, double *energy, double *gradient, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (gradient == nullptr) {
-        log_msg("nullptr gradient passed");
+        log_msg("the gradient output array is null");
         return 3;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 4;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -514,7 +514,7 @@ int getEnergyGradient(int nangles, const double *angles, double *energy, double 
 int getEnergies(int nvec, int nangles, const double *angles, double *energies, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr || energies == nullptr) {
@@ -536,7 +536,7 @@ int getEnergies(int nvec, int nangles, const double *angles, double *energies, v
 int getEnergyGradients(int nvec, int nangles, const double *angles, double *energies, double *gradients, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr || energies == nullptr || gradients == nullptr) {
@@ -563,7 +563,7 @@ int getEnergyGradients(int nvec, int nangles, const double *angles, double *ener
 int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -580,7 +580,7 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
 int setEngineOption(int key, int value, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -591,7 +591,7 @@ int setEngineOption(int key, int value, void *ctx)
 int getEngineOption(int key, int *value, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (value == nullptr) {
@@ -610,11 +610,11 @@ int autotune(int nvec, int nangles, const double *angles, const char *cachefile,
              int reportlen, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (nvec < 1 || nangles < 1) {
@@ -640,11 +640,11 @@ extern "C" {
 int uploadAngleBatch(int nvec, int nangles, const double *angles, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr) {
-        log_msg("nullptr angles passed");
+        log_msg("the angle array is null");
         return 2;
     }
     if (nvec < 1 || nangles < 1) {
@@ -659,7 +659,7 @@ int uploadAngleBatch(int nvec, int nangles, const double *angles, void *ctx)
 int runAngleBatch(int withgradient, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -670,7 +670,7 @@ int runAngleBatch(int withgradient, void *ctx)
 int downloadAngleBatch(double *energies, double *gradients, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (energies == nullptr) {
@@ -689,7 +689,7 @@ int downloadAngleBatch(double *energies, double *gradients, void *ctx)
 int setKernelTiming(int on, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     Manager *m = static_cast<Manager *>(ctx);
@@ -700,7 +700,7 @@ int setKernelTiming(int on, void *ctx)
 int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (kernelclass < 0 || kernelclass > 3 || milliseconds == nullptr || launches == nullptr) {
@@ -723,7 +723,7 @@ extern "C" int optimise(int nangles, double *angles, double *energy, double stop
                         int *ntrace, void *ctx)
 {
     if (ctx == nullptr) {
-        log_msg("nullptr ctx passed");
+        log_msg("the context is null");
         return 1;
     }
     if (angles == nullptr || energy == nullptr) {
@@ -731,7 +731,7 @@ extern "C" int optimise(int nangles, double *angles, double *energy, double stop
         return 2;
     }
     if (nangles < 1) {
-        log_msg("NAngles < 1");
+        log_msg("the angle count is below 1");
         return 3;
     }
     Manager *m = static_cast<Manager *>(ctx);

@@ -116,11 +116,11 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
 bool Manager::autotune(int nvec, int nangles, const double *angles, const char *cacheFile, int reps,
                        std::string &report)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::vector<double> dec;
     if (!decompressBatch(nvec, nangles, angles, dec)) return false;
-    Engine &eng = *m_engine;
-    const int M = (int)m_excitations.size();
+    Engine &eng = *eng_;
+    const int M = (int)ops_.size();
     const std::string key = shape_key(eng, nvec, M);
     std::string err;
     Candidate best{};
@@ -257,9 +257,9 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
 
 bool Manager::setEngineOption(int key, int value)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::string err;
-    if (!m_engine->set_option(key, value, err)) {
+    if (!eng_->set_option(key, value, err)) {
         log_msg("Engine option %d = %d rejected: %s", key, value, err.c_str());
         return false;
     }
@@ -268,8 +268,8 @@ bool Manager::setEngineOption(int key, int value)
 
 bool Manager::getEngineOption(int key, int &value)
 {
-    if (!validateToRun()) return false;
-    value = m_engine->option(key);
+    if (!ensureBuilt()) return false;
+    value = eng_->option(key);
     // rocBLAS solution indices may be negative; every other option is >= 0
     // and -1 marks an unknown key
     if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha) return true;

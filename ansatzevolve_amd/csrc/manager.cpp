@@ -1,4 +1,6 @@
-// Host-side mirror of stateAnsatzManager (src/AnsatzManager.cpp).
+// Host-side counterpart of stateAnsatzManager (src/AnsatzManager.cpp): the
+// stored set-up, the sector choice, the lazily built device engine and the
+// angle conversions.
 #include "manager.hpp"
 #include "runfiles.hpp"
 
@@ -20,418 +22,383 @@ void log_msg(const char *fmt, ...)
     va_end(ap);
 }
 
-bool Manager::storeOperators(const std::vector<std::array<int8_t, 4>> &excs)
+bool Manager::frozen(const char *what) const
 {
-    bool success = true;
-    if (m_isConstructed) {
-        success = false;
-        log_msg("Already constructed, setup cant be modified now");
-    }
-    if (excs.empty()) {
-        success = false;
-        log_msg("Excitations given has zero size");
-    }
-    if (success) m_excitations = excs;
-    return success;
+    if (!eng_) return false;
+    log_msg("%s rejected: the set-up is fixed once the first evaluation has run", what);
+    return true;
 }
 
-// src/AnsatzManager.cpp:271-374: particle number and S_z of the non-zero
-// components decide which sector the state lives in.
+// ---------------------------------------------------------------- set-up
+
+bool Manager::storeOperators(const std::vector<std::array<int8_t, 4>> &excs)
+{
+    const bool locked = frozen("Operators");
+    if (excs.empty()) log_msg("Operators rejected: the list is empty");
+    if (locked || excs.empty()) return false;
+    ops_ = excs;
+    return true;
+}
+
+// The start state, and which occupations all of its nonzero components
+// share (they decide the sector: src/AnsatzManager.cpp:271-374).  Bits
+// 0 .. q/2-1 are spin down, q/2 .. q-1 spin up.
 bool Manager::storeInitial(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs)
 {
-    bool success = true;
-    if (m_isConstructed) {
-        success = false;
-        log_msg("Already constructed, setup cant be modified now");
-    }
-    if (numberOfQubits < 1) {
-        success = false;
-        log_msg("Number of qubits < 1: %d", numberOfQubits);
-    }
-    if (numberOfQubits > 30) {
-        success = false;
-        log_msg("Number of qubits > 30 is not supported: %d", numberOfQubits);
+    std::vector<std::string> why;
+    if (frozen("Initial state")) why.push_back("set-up fixed");
+    if (numberOfQubits < 1 || numberOfQubits > 30) {
+        log_msg("Initial state rejected: %d qubits (1 to 30 are supported)", numberOfQubits);
+        why.push_back("qubits");
     }
     if (indexes.empty() || coeffs.empty()) {
-        success = false;
-        log_msg("Initial state has no elements");
+        log_msg("Initial state rejected: no components");
+        why.push_back("empty");
+    } else if (indexes.size() != coeffs.size()) {
+        log_msg("Initial state rejected: %zu indexes against %zu coefficients", indexes.size(), coeffs.size());
+        why.push_back("sizes");
     }
-    if (coeffs.size() != indexes.size()) {
-        success = false;
-        log_msg("Initial state indexes and coeffs are a different size");
-    }
-    for (int idx : indexes)
-        if (success && (idx < 0 || (numberOfQubits <= 30 && (long long)idx >= (1ll << numberOfQubits)))) {
-            success = false;
-            log_msg("Initial state index out of range: %d", idx + 1);
-        }
-    if (!success) return false;
-    // the dense start vector of the reference: later duplicates overwrite
-    std::map<int, double> dense;
-    for (size_t i = 0; i < indexes.size(); i++) dense[indexes[i]] = coeffs[i];
-    double mag = 0;
-    for (auto &kv : dense) mag += kv.second * kv.second;
-    mag = std::sqrt(mag);
-    if (std::abs(mag - 1) > 1e-14) log_msg("Warning: Initial vector is not normalised, the error is %g", std::abs(mag - 1));
-    m_startIdx = indexes;
-    m_startCoeff = coeffs;
-    m_numberOfQubits = numberOfQubits;
+    if (why.empty())
+        for (int b : indexes)
+            if (b < 0 || (long long)b >= (1ll << numberOfQubits)) {
+                log_msg("Initial state rejected: basis index %d is outside 1 .. 2^%d", b + 1, numberOfQubits);
+                why.push_back("range");
+                break;
+            }
+    if (!why.empty()) return false;
 
-    int numberOfParticles = -1, spinUp = -1, spinDown = -1;
-    bool allSame = true, SZSym = (numberOfQubits % 2 == 0);
-    const uint32_t low = (numberOfQubits / 2 >= 32) ? 0xffffffffu : ((1u << (numberOfQubits / 2)) - 1);
-    for (size_t i = 0; i < indexes.size(); i++) {
-        if (coeffs[i] == 0.) continue;
-        const uint32_t s = (uint32_t)indexes[i];
-        const int c = __builtin_popcount(s);
-        if (numberOfParticles == -1) numberOfParticles = c;
-        if (c != numberOfParticles) {
-            allSame = false;
-            break;
+    // a later entry for the same basis state replaces an earlier one
+    std::map<int, double> amp;
+    for (size_t k = 0; k < indexes.size(); k++) amp[indexes[k]] = coeffs[k];
+    double norm2 = 0;
+    for (const auto &e : amp) norm2 += e.second * e.second;
+    const double dev = std::fabs(std::sqrt(norm2) - 1.0);
+    if (dev > 1e-14) log_msg("Note: the initial state's norm differs from 1 by %g", dev);
+    start_.qubits = numberOfQubits;
+    start_.bits = indexes;
+    start_.amps = coeffs;
+
+    const int half = numberOfQubits / 2;
+    const uint32_t downMask = (1u << half) - 1;
+    Occupation o;
+    o.fixedTotal = true;
+    o.fixedSpins = numberOfQubits % 2 == 0;
+    bool any = false;
+    for (size_t k = 0; k < indexes.size() && o.fixedTotal; k++) {
+        if (coeffs[k] == 0.0) continue;
+        const uint32_t b = (uint32_t)indexes[k];
+        const int tot = __builtin_popcount(b), dn = __builtin_popcount(b & downMask),
+                  up = __builtin_popcount(b >> half);
+        if (!any) {
+            o.total = tot;
+            o.up = up;
+            o.down = dn;
+            any = true;
+            continue;
         }
-        const int cu = __builtin_popcount(s >> (numberOfQubits / 2));
-        const int cd = __builtin_popcount(s & low);
-        if (spinUp == -1) spinUp = cu;
-        if (spinDown == -1) spinDown = cd;
-        if (cu != spinUp || cd != spinDown) SZSym = false;
+        if (tot != o.total) o.fixedTotal = false;
+        else if (up != o.up || dn != o.down) o.fixedSpins = false;
     }
-    if (numberOfParticles == -1) log_msg("Warning: Could not determine particle number");
-    else if (!allSame) log_msg("Warning: not all the same particle number");
-    m_numberOfParticles = -1;
-    m_spinUp = m_spinDown = -1;
-    m_InitialSZSym = false;
-    m_particleSym = false;
-    if (allSame && numberOfParticles != -1) {
-        m_numberOfParticles = numberOfParticles;
-        m_particleSym = true;
-        if (SZSym && spinUp != -1 && spinDown != -1) {
-            m_InitialSZSym = true;
-            m_spinUp = spinUp;
-            m_spinDown = spinDown;
-        }
+    if (!any) {
+        log_msg("Note: the initial state has no nonzero component; the full space is used");
+        o = Occupation{};
+    } else if (!o.fixedTotal) {
+        log_msg("Note: the initial state mixes particle numbers; the full space is used");
+        o = Occupation{};
+    } else if (!o.fixedSpins) {
+        o.up = o.down = -1;
     }
+    occ_ = o;
     return true;
 }
 
 bool Manager::storeHamiltonian(std::vector<int> &&i0, std::vector<int> &&j0, std::vector<double> &&v)
 {
-    bool success = true;
-    if (m_isConstructed) {
-        success = false;
-        log_msg("Already constructed, setup cant be modified now");
-    }
+    bool ok = !frozen("Hamiltonian");
     if (i0.empty() || j0.empty() || v.empty()) {
-        success = false;
-        log_msg("Hamiltonian Matrix has no elements");
+        log_msg("Hamiltonian rejected: no entries");
+        ok = false;
+    } else if (i0.size() != j0.size() || i0.size() != v.size()) {
+        log_msg("Hamiltonian rejected: %zu row, %zu column and %zu value entries", i0.size(), j0.size(), v.size());
+        ok = false;
     }
-    if (i0.size() != j0.size() || i0.size() != v.size()) {
-        success = false;
-        log_msg("Hamiltonian Matrix indexes and coeffs are a different size");
-    }
-    if (!success) return false;
-    m_iIndexes = std::move(i0);
-    m_jIndexes = std::move(j0);
-    m_coeffs = std::move(v);
+    if (!ok) return false;
+    trip_.i = std::move(i0);
+    trip_.j = std::move(j0);
+    trip_.v = std::move(v);
     return true;
 }
 
 bool Manager::storeIntegrals(int norb, const double *h, const double *g)
 {
-    if (m_isConstructed) {
-        log_msg("Already constructed, setup cant be modified now");
-        return false;
-    }
-    m_norb = norb;
+    if (frozen("Integrals")) return false;
+    norb_ = norb;
     const size_t n2 = (size_t)norb * norb;
-    m_h.assign(h, h + n2);
-    m_g.assign(g, g + n2 * n2);
+    h1_.assign(h, h + n2);
+    g2_.assign(g, g + n2 * n2);
     return true;
 }
 
 bool Manager::storeRunPath(const std::string &runPath)
 {
-    bool success = true;
-    if (m_isConstructed) {
-        success = false;
-        log_msg("Already constructed, setup cant be modified now");
-    }
-    if (runPath.empty()) {
-        success = false;
-        log_msg("Run path has zero size");
-    }
-    if (success) m_runPath = runPath;
-    return success;
+    const bool locked = frozen("Run path");
+    if (runPath.empty()) log_msg("Run path rejected: empty");
+    if (locked || runPath.empty()) return false;
+    runPrefix_ = runPath;
+    return true;
 }
 
 bool Manager::storeNuclearEnergy(double e)
 {
-    if (m_isConstructed) {
-        log_msg("Already constructed, setup cant be modified now");
-        return false;
-    }
-    m_nuclearEnergy = e;
+    if (frozen("Nuclear energy")) return false;
+    enuc_ = e;
     return true;
 }
 
 bool Manager::storeHamiltonianStorage(int mode)
 {
-    if (m_isConstructed) {
-        log_msg("Already constructed, setup cant be modified now");
-        return false;
-    }
-    m_hamStorage = mode;
+    if (frozen("Hamiltonian storage")) return false;
+    storage_ = mode;
     return true;
 }
 
-// src/AnsatzManager.cpp:415-445
+// Operator k takes angle scale_k * theta[param_k]; there are as many
+// parameters as the largest index + 1, and no more parameters than
+// operators (src/AnsatzManager.cpp:415-445).
 bool Manager::storeParameterDependencies(const std::vector<std::pair<int, double>> &dep)
 {
-    bool success = true;
-    if (m_isConstructed) {
-        success = false;
-        log_msg("Already constructed, setup cant be modified now");
+    const bool locked = frozen("Parameter map");
+    int largest = -1;
+    bool negative = false;
+    for (const auto &d : dep) {
+        largest = std::max(largest, d.first);
+        negative = negative || d.first < 0;
     }
-    if (dep.empty()) {
-        success = false;
-        log_msg("Parameter Dependencies, (order file) has zero size");
-    }
-    for (auto &d : dep)
-        if (d.first < 0) {
-            success = false;
-            log_msg("Parameter index < 1 in the order file");
-            break;
-        }
-    if (!success) return false;
-    m_parameterDependency = dep;
-    m_numberOfUniqueParameters = -1;
-    for (auto &d : dep) m_numberOfUniqueParameters = std::max(d.first + 1, m_numberOfUniqueParameters);
-    if ((size_t)m_numberOfUniqueParameters > dep.size()) {
-        m_numberOfUniqueParameters = -1;
-        m_parameterDependency.clear();
+    if (dep.empty()) log_msg("Parameter map rejected: empty");
+    if (negative) log_msg("Parameter map rejected: a parameter index below 1");
+    if (locked || dep.empty() || negative) return false;
+    paramIdx_.clear();
+    paramScale_.clear();
+    nparams_ = -1;
+    if ((size_t)largest + 1 > dep.size()) {
+        log_msg("Parameter map rejected: parameter %d for only %zu operators", largest + 1, dep.size());
         return false;
     }
+    for (const auto &d : dep) {
+        paramIdx_.push_back(d.first);
+        paramScale_.push_back(d.second);
+    }
+    nparams_ = largest + 1;
     return true;
 }
 
-// src/AnsatzManager.cpp:202-237
-void Manager::setOperatorSymmetry()
+// ---------------------------------------------------------------- build
+
+int Manager::chooseSector() const
 {
-    m_OperatorSZSym = true;
-    const int nq = m_numberOfQubits;
-    const uint32_t down = (1u << (nq / 2)) - 1;
-    const uint32_t up = ((nq >= 32) ? 0xffffffffu : ((1u << nq) - 1)) ^ down;
-    for (const auto &e : m_excitations) {
-        uint32_t cr = 0, de = 0;
-        if (e[0] >= nq || e[1] >= nq || e[2] >= nq || e[3] >= nq) {
-            m_OperatorSZSym = false;  // out of range: rejected later by the engine
-            continue;
-        }
-        if (e[0] >= 0 && e[1] >= 0 && e[2] < 0 && e[3] < 0) {
-            cr = 1u << e[0];
-            de = 1u << e[1];
-        } else if (e[0] >= 0 && e[1] >= 0 && e[2] >= 0 && e[3] >= 0) {
-            cr = (1u << e[0]) | (1u << e[1]);
-            de = (1u << e[2]) | (1u << e[3]);
-        } else {
-            m_OperatorSZSym = false;  // rejected later by the engine
-            continue;
-        }
-        m_OperatorSZSym = m_OperatorSZSym && __builtin_popcount(cr & up) == __builtin_popcount(de & up) &&
-                          __builtin_popcount(cr & down) == __builtin_popcount(de & down);
+    if (!occ_.fixedTotal) return kFull;
+    if (!occ_.fixedSpins || start_.qubits % 2 != 0) return kNumber;
+    // every operator must move as many up (down) electrons in as out
+    // (src/AnsatzManager.cpp:202-237); an operator of another shape keeps
+    // the S_z sector out and is rejected by the engine later
+    const int q = start_.qubits;
+    const uint32_t down = (1u << (q / 2)) - 1;
+    for (const auto &e : ops_) {
+        const bool single = e[0] >= 0 && e[1] >= 0 && e[2] < 0 && e[3] < 0;
+        const bool dbl = e[0] >= 0 && e[1] >= 0 && e[2] >= 0 && e[3] >= 0;
+        if ((!single && !dbl) || std::max(std::max(e[0], e[1]), std::max(e[2], e[3])) >= q) return kNumber;
+        // created and annihilated orbitals
+        const uint32_t in = dbl ? (1u << e[0]) | (1u << e[1]) : 1u << e[0];
+        const uint32_t out = dbl ? (1u << e[2]) | (1u << e[3]) : 1u << e[1];
+        if (__builtin_popcount(in & down) != __builtin_popcount(out & down) ||
+            __builtin_popcount(in & ~down) != __builtin_popcount(out & ~down))
+            return kNumber;
     }
-    if (m_excitations.empty()) m_OperatorSZSym = false;
+    return ops_.empty() ? kNumber : kSz;
 }
 
-bool Manager::construct()
+bool Manager::build()
 {
-    if (m_isConstructed) return true;
-    bool success = true;
-    if (m_numberOfQubits == -1) {
-        success = false;
-        log_msg("Number of Qubits is not set. Make sure the initial state has been provided and has elements");
+    std::vector<const char *> missing;
+    if (start_.qubits < 0 || start_.bits.empty()) missing.push_back("the initial state");
+    if (ops_.empty()) missing.push_back("the operators");
+    if (nparams_ < 0) missing.push_back("the parameter map (order file)");
+    if (nparams_ >= 0 && !ops_.empty() && paramIdx_.size() != ops_.size())
+        log_msg("Cannot build: %zu operators but %zu parameter map entries", ops_.size(), paramIdx_.size());
+    if (!missing.empty()) {
+        std::string m = missing[0];
+        for (size_t k = 1; k < missing.size(); k++) m += std::string(", ") + missing[k];
+        log_msg("Cannot build: missing %s", m.c_str());
     }
-    if (m_startIdx.empty()) {
-        success = false;
-        log_msg("Make sure the initial state has been provided");
-    }
-    if (m_excitations.empty()) {
-        success = false;
-        log_msg("No excitations have been set");
-    }
-    if (m_parameterDependency.empty() || m_numberOfUniqueParameters == -1) {
-        success = false;
-        log_msg("Parameter Dependencies, (order file) has zero size");
-    }
-    if (m_parameterDependency.size() != m_excitations.size()) {
-        success = false;
-        log_msg("More operators than parameters dependencies given");
-    }
-    if (success && m_iIndexes.empty() && m_h.empty() && !m_runPath.empty()) {
-        // the Hamiltonian from the run path's files
+    if (!missing.empty() || paramIdx_.size() != ops_.size()) return false;
+
+    // a run path stands in for a Hamiltonian that was not given directly
+    if (trip_.i.empty() && h1_.empty() && !runPrefix_.empty()) {
         int kind = 0;
         std::vector<int> i0, j0;
         std::vector<double> v, h, g;
         std::string ferr;
-        if (!files::read_run_hamiltonian(m_runPath, m_numberOfQubits, kind, i0, j0, v, h, g, ferr)) {
-            log_msg("Could not construct Hamiltonian: %s", ferr.c_str());
+        if (!files::read_run_hamiltonian(runPrefix_, start_.qubits, kind, i0, j0, v, h, g, ferr)) {
+            log_msg("Cannot build: the run path's Hamiltonian: %s", ferr.c_str());
         } else if (kind == 1) {
-            m_iIndexes = std::move(i0);
-            m_jIndexes = std::move(j0);
-            m_coeffs = std::move(v);
+            trip_.i = std::move(i0);
+            trip_.j = std::move(j0);
+            trip_.v = std::move(v);
         } else {
-            m_norb = m_numberOfQubits / 2;
-            m_h = std::move(h);
-            m_g = std::move(g);
+            norb_ = start_.qubits / 2;
+            h1_ = std::move(h);
+            g2_ = std::move(g);
         }
     }
-    if (m_iIndexes.empty() && m_h.empty()) {
-        success = false;
-        log_msg("No Hamiltonian has been set");
+    if (trip_.i.empty() && h1_.empty()) {
+        log_msg("Cannot build: missing the Hamiltonian");
+        return false;
     }
-    if (!success) return false;
-    setOperatorSymmetry();
-    int mode = kFull;
-    if (m_particleSym) mode = (m_InitialSZSym && m_OperatorSZSym) ? kSz : kNumber;
+
     auto eng = std::make_unique<Engine>();
     std::string err;
-    if (!eng->init(m_numberOfQubits, mode, m_numberOfParticles, m_spinUp, m_spinDown, err)) {
+    if (!eng->init(start_.qubits, chooseSector(), occ_.total, occ_.up, occ_.down, err)) {
         log_msg("Device set-up failed: %s", err.c_str());
         return false;
     }
     // operators and the parameter map (src/tupsquantities.cpp:643-667)
-    std::vector<int8_t> ops4;
-    std::vector<int> order;
-    std::vector<double> scale;
-    for (size_t k = 0; k < m_excitations.size(); k++) {
-        for (int j = 0; j < 4; j++) ops4.push_back(m_excitations[k][j]);
-        order.push_back(m_parameterDependency[k].first);
-        scale.push_back(m_parameterDependency[k].second);
-    }
-    if (!eng->set_operators(ops4, order, scale, m_numberOfUniqueParameters, err)) {
+    std::vector<int8_t> flat;
+    flat.reserve(4 * ops_.size());
+    for (const auto &e : ops_) flat.insert(flat.end(), e.begin(), e.end());
+    if (!eng->set_operators(flat, paramIdx_, paramScale_, nparams_, err)) {
         log_msg("Operator set-up failed: %s", err.c_str());
         return false;
     }
-    // compressVector of the start (sparsematrix.h:83-105)
-    std::vector<double> start(eng->dim(), 0.0);
-    for (size_t i = 0; i < m_startIdx.size(); i++) {
-        const long long r = eng->rank((uint32_t)m_startIdx[i]);
-        if (r >= 0) start[r] = m_startCoeff[i];
+    // the start state over the sector (components outside it are dropped,
+    // as compressVector does: src/sparsematrix.h:83-105)
+    std::vector<double> psi0(eng->dim(), 0.0);
+    for (size_t k = 0; k < start_.bits.size(); k++) {
+        const long long r = eng->rank((uint32_t)start_.bits[k]);
+        if (r >= 0) psi0[r] = start_.amps[k];
     }
-    if (!eng->set_start(start, err)) {
-        log_msg("Start vector upload failed: %s", err.c_str());
+    if (!eng->set_start(psi0, err)) {
+        log_msg("Start state upload failed: %s", err.c_str());
         return false;
     }
-    // Hamiltonian: triplets (src/hamiltonianmatrix.cpp:628-659) or integrals
-    if (!m_iIndexes.empty()) {
+    if (!trip_.i.empty()) {
+        // rows of y = M^T x over the sector: output row j holds (i, v) for
+        // every stored (i, j, v), repeated (i, j) summed, entries with an
+        // end outside the sector dropped (src/hamiltonianmatrix.cpp:628-659)
         const long long N = eng->dim();
-        std::vector<std::pair<long long, std::pair<int32_t, double>>> t;  // (row j, (col i, v))
-        t.reserve(m_iIndexes.size());
-        for (size_t q = 0; q < m_iIndexes.size(); q++) {
-            if (m_iIndexes[q] < 0 || m_jIndexes[q] < 0) continue;
-            const long long ci = eng->rank((uint32_t)m_iIndexes[q]);
-            const long long cj = eng->rank((uint32_t)m_jIndexes[q]);
-            if (ci < 0 || cj < 0) continue;
-            t.push_back({cj, {(int32_t)ci, m_coeffs[q]}});
+        std::vector<std::pair<long long, std::pair<int32_t, double>>> ent;
+        ent.reserve(trip_.i.size());
+        for (size_t k = 0; k < trip_.i.size(); k++) {
+            if (trip_.i[k] < 0 || trip_.j[k] < 0) continue;
+            const long long ri = eng->rank((uint32_t)trip_.i[k]), rj = eng->rank((uint32_t)trip_.j[k]);
+            if (ri >= 0 && rj >= 0) ent.push_back({rj, {(int32_t)ri, trip_.v[k]}});
         }
-        std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) {
-            return a.first != b.first ? a.first < b.first : a.second.first < b.second.first;
+        std::stable_sort(ent.begin(), ent.end(), [](const auto &x, const auto &y) {
+            return x.first != y.first ? x.first < y.first : x.second.first < y.second.first;
         });
         std::vector<int64_t> rowptr(N + 1, 0);
         std::vector<int32_t> col;
         std::vector<double> val;
-        for (size_t q = 0; q < t.size(); q++) {
-            if (q > 0 && t[q].first == t[q - 1].first && t[q].second.first == t[q - 1].second.first) {
-                val.back() += t[q].second.second;  // duplicates summed
+        for (size_t k = 0; k < ent.size(); k++) {
+            const bool repeat = k > 0 && ent[k].first == ent[k - 1].first && ent[k].second.first == ent[k - 1].second.first;
+            if (repeat) {
+                val.back() += ent[k].second.second;
                 continue;
             }
-            col.push_back(t[q].second.first);
-            val.push_back(t[q].second.second);
-            rowptr[t[q].first + 1]++;
+            col.push_back(ent[k].second.first);
+            val.push_back(ent[k].second.second);
+            rowptr[ent[k].first + 1]++;
         }
         for (long long r = 0; r < N; r++) rowptr[r + 1] += rowptr[r];
         if (!eng->set_ham_rows(rowptr, col, val, err)) {
             log_msg("Hamiltonian upload failed: %s", err.c_str());
             return false;
         }
-        m_iIndexes.clear();
-        m_jIndexes.clear();
-        m_coeffs.clear();
+        trip_ = Triplets{};
     } else {
-        if (2 * m_norb != m_numberOfQubits) {
-            log_msg("Integrals for %d orbitals do not match %d qubits", m_norb, m_numberOfQubits);
+        if (2 * norb_ != start_.qubits) {
+            log_msg("Cannot build: integrals for %d orbitals against %d qubits", norb_, start_.qubits);
             return false;
         }
-        if (!eng->set_ham_integrals(m_norb, m_h, m_g, m_hamStorage, err)) {
+        if (!eng->set_ham_integrals(norb_, h1_, g2_, storage_, err)) {
             log_msg("Integral Hamiltonian set-up failed: %s", err.c_str());
             return false;
         }
     }
-    m_engine = std::move(eng);
-    // evolve at zero angles, as the reference does on construction
-    m_angles.assign(m_excitations.size(), 0.0);
-    if (!m_engine->evolve(m_angles.data(), err)) {
+    // the state at zero angles, as the reference evolves on construction
+    std::vector<double> zero(ops_.size(), 0.0);
+    if (!eng->evolve(zero.data(), err)) {
         log_msg("Evolution failed: %s", err.c_str());
-        m_engine.reset();
         return false;
     }
-    m_isConstructed = true;
+    cur_ = zero;
+    eng_ = std::move(eng);
     return true;
 }
 
-bool Manager::validateToRun()
+bool Manager::ensureBuilt()
 {
-    if (!m_isConstructed && !construct()) {
-        log_msg("Not yet Constructed, something is missing");
-        return false;
-    }
-    return true;
+    if (eng_) return true;
+    if (build()) return true;
+    log_msg("Evaluation refused: the set-up is incomplete or could not be built");
+    return false;
 }
 
-// src/AnsatzManager.cpp:467-506: per-operator angles are used as given,
-// per-parameter angles are decompressed (x = D theta); psi is re-evolved
-// only when the per-operator angles change.
+// ---------------------------------------------------------------- angles
+
+bool Manager::toOperatorAngles(const double *in, size_t n, double *out) const
+{
+    const size_t M = ops_.size();
+    if (n == M) {
+        std::copy(in, in + M, out);
+        return true;
+    }
+    if ((int)n == nparams_) {
+        for (size_t k = 0; k < M; k++) out[k] = paramScale_[k] * in[paramIdx_[k]];
+        return true;
+    }
+    log_msg("Angles rejected: %zu given, %zu per operator or %d per parameter expected", n, M, nparams_);
+    return false;
+}
+
+bool Manager::toOperatorAngles(const std::vector<double> &in, std::vector<double> &out) const
+{
+    out.resize(ops_.size());
+    return toOperatorAngles(in.data(), in.size(), out.data());
+}
+
+// Per-operator or per-parameter angles (src/AnsatzManager.cpp:467-506); psi
+// is evolved again only when the per-operator angles change.
 bool Manager::setAngles(std::vector<double> angles)
 {
-    if (!validateToRun()) return false;
-    const size_t M = m_excitations.size();
-    if (angles.size() == M) {
-    } else if ((int)angles.size() == m_numberOfUniqueParameters) {
-        std::vector<double> dec(M);
-        for (size_t k = 0; k < M; k++)
-            dec[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
-        angles.swap(dec);
-    } else {
-        log_msg("Incorrect number of angles given: %zu", angles.size());
+    if (!ensureBuilt()) return false;
+    std::vector<double> x;
+    if (!toOperatorAngles(angles, x)) return false;
+    if (x == cur_) return true;
+    std::string err;
+    if (!eng_->evolve(x.data(), err)) {
+        log_msg("Evolution failed: %s", err.c_str());
+        cur_.assign(ops_.size(), std::nan(""));
         return false;
     }
-    if (angles != m_angles) {
-        std::string err;
-        m_angles = angles;
-        if (!m_engine->evolve(m_angles.data(), err)) {
-            log_msg("Evolution failed: %s", err.c_str());
-            m_angles.assign(M, std::nan(""));
-            return false;
-        }
-    }
+    cur_ = x;
     return true;
 }
 
 void Manager::compressGradient(const double *deriv, double *grad) const
 {
-    for (int p = 0; p < m_numberOfUniqueParameters; p++) grad[p] = 0;
-    for (size_t k = 0; k < m_parameterDependency.size(); k++)
-        grad[m_parameterDependency[k].first] += m_parameterDependency[k].second * deriv[k];
+    std::fill(grad, grad + nparams_, 0.0);
+    for (size_t k = 0; k < paramIdx_.size(); k++) grad[paramIdx_[k]] += paramScale_[k] * deriv[k];
 }
 
 bool Manager::getExpectationValue(const std::vector<double> &angles, double &E)
 {
     if (!setAngles(angles)) return false;
     std::string err;
-    if (!m_engine->energy(&E, err)) {
+    if (!eng_->energy(&E, err)) {
         log_msg("Energy failed: %s", err.c_str());
         return false;
     }
-    E += m_nuclearEnergy;
+    E += enuc_;
     return true;
 }
 
@@ -439,13 +406,13 @@ bool Manager::getFinalState(const std::vector<double> &angles, std::vector<doubl
 {
     if (!setAngles(angles)) return false;
     std::string err;
-    std::vector<double> comp(m_engine->dim());
-    if (!m_engine->download_state(comp.data(), err)) {
+    std::vector<double> comp(eng_->dim());
+    if (!eng_->download_state(comp.data(), err)) {
         log_msg("State download failed: %s", err.c_str());
         return false;
     }
-    full.assign((size_t)1 << m_numberOfQubits, 0.0);
-    const auto &st = m_engine->states();
+    full.assign((size_t)1 << start_.qubits, 0.0);
+    const auto &st = eng_->states();
     for (size_t i = 0; i < comp.size(); i++) full[st[i]] = comp[i];
     return true;
 }
@@ -456,27 +423,18 @@ bool Manager::getFinalState(const std::vector<double> &angles, std::vector<doubl
 // src/fusedevolve.cpp:2263-2346).
 bool Manager::gradientAt(const std::vector<double> &angles, double *E, std::vector<double> &deriv)
 {
-    if (!validateToRun()) return false;
-    const size_t M = m_excitations.size();
+    if (!ensureBuilt()) return false;
+    const size_t M = ops_.size();
     std::vector<double> x;
-    if (angles.size() == M) {
-        x = angles;
-    } else if ((int)angles.size() == m_numberOfUniqueParameters) {
-        x.resize(M);
-        for (size_t k = 0; k < M; k++)
-            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
-    } else {
-        log_msg("Incorrect number of angles given: %zu", angles.size());
-        return false;
-    }
+    if (!toOperatorAngles(angles, x)) return false;
     std::string err;
     deriv.resize(M);
     bool ok;
-    if (x != m_angles) {
-        ok = m_engine->energy_gradient(x.data(), E, deriv.data(), err);
-        m_angles = ok ? x : std::vector<double>(M, std::nan(""));
+    if (x != cur_) {
+        ok = eng_->energy_gradient(x.data(), E, deriv.data(), err);
+        cur_ = ok ? x : std::vector<double>(M, std::nan(""));
     } else {
-        ok = m_engine->gradient(x.data(), E, deriv.data(), err);
+        ok = eng_->gradient(x.data(), E, deriv.data(), err);
     }
     if (!ok) log_msg("Gradient failed: %s", err.c_str());
     return ok;
@@ -486,7 +444,7 @@ bool Manager::getGradientComp(const std::vector<double> &angles, std::vector<dou
 {
     std::vector<double> deriv;
     if (!gradientAt(angles, nullptr, deriv)) return false;
-    grad.resize(m_numberOfUniqueParameters);
+    grad.resize(nparams_);
     compressGradient(deriv.data(), grad.data());
     return true;
 }
@@ -499,36 +457,27 @@ bool Manager::getGradient(const std::vector<double> &angles, std::vector<double>
 bool Manager::getOverlapGradient(const std::vector<double> &angles, const std::vector<double> &projFull,
                                  double &overlap, std::vector<double> &grad)
 {
-    if (!validateToRun()) return false;
-    const size_t M = m_excitations.size();
+    if (!ensureBuilt()) return false;
+    const size_t M = ops_.size();
     std::vector<double> x;
-    if (angles.size() == M) {
-        x = angles;
-    } else if ((int)angles.size() == m_numberOfUniqueParameters) {
-        x.resize(M);
-        for (size_t k = 0; k < M; k++)
-            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
-    } else {
-        log_msg("Incorrect number of angles given: %zu", angles.size());
-        return false;
-    }
-    if (projFull.size() != ((size_t)1 << m_numberOfQubits)) {
-        log_msg("Projection vector has %zu entries, expected 2^%d", projFull.size(), m_numberOfQubits);
+    if (!toOperatorAngles(angles, x)) return false;
+    if (projFull.size() != ((size_t)1 << start_.qubits)) {
+        log_msg("Projection vector has %zu entries, expected 2^%d", projFull.size(), start_.qubits);
         return false;
     }
     // the sector's amplitudes of proj (entries outside the sector meet zeros of psi)
-    const auto &st = m_engine->states();
+    const auto &st = eng_->states();
     std::vector<double> comp(st.size());
     for (size_t i = 0; i < st.size(); i++) comp[i] = projFull[st[i]];
     std::vector<double> deriv(M);
     std::string err;
-    if (!m_engine->overlap_gradient(x.data(), comp.data(), &overlap, deriv.data(), err)) {
+    if (!eng_->overlap_gradient(x.data(), comp.data(), &overlap, deriv.data(), err)) {
         log_msg("Overlap gradient failed: %s", err.c_str());
-        m_angles.assign(M, std::nan(""));
+        cur_.assign(M, std::nan(""));
         return false;
     }
-    m_angles = x;  // psi is now U(x) psi_0
-    grad.resize(m_numberOfUniqueParameters);
+    cur_ = x;  // psi is now U(x) psi_0
+    grad.resize(nparams_);
     compressGradient(deriv.data(), grad.data());
     return true;
 }
@@ -537,8 +486,8 @@ bool Manager::getEnergyGradient(const std::vector<double> &angles, double &E, st
 {
     std::vector<double> deriv;
     if (!gradientAt(angles, &E, deriv)) return false;
-    E += m_nuclearEnergy;
-    grad.resize(m_numberOfUniqueParameters);
+    E += enuc_;
+    grad.resize(nparams_);
     compressGradient(deriv.data(), grad.data());
     return true;
 }
@@ -547,11 +496,11 @@ bool Manager::getHessianComp(const std::vector<double> &angles, std::vector<doub
 {
     if (!setAngles(angles)) return false;
     std::string err;
-    const int P = m_numberOfUniqueParameters;
+    const int P = nparams_;
     hess.assign((size_t)P * P, 0.0);
     std::vector<double> g(P);
     double E;
-    if (!m_engine->hessian(m_angles.data(), hess.data(), g.data(), &E, err)) {
+    if (!eng_->hessian(cur_.data(), hess.data(), g.data(), &E, err)) {
         log_msg("Hessian failed: %s", err.c_str());
         return false;
     }
@@ -560,83 +509,73 @@ bool Manager::getHessianComp(const std::vector<double> &angles, std::vector<doub
 
 bool Manager::decompressBatch(int nvec, int nangles, const double *angles, std::vector<double> &out)
 {
-    const size_t M = m_excitations.size();
+    const size_t M = ops_.size();
     out.resize((size_t)nvec * M);
-    if ((size_t)nangles == M) {
-        std::copy(angles, angles + (size_t)nvec * M, out.begin());
-        return true;
-    }
-    if (nangles == m_numberOfUniqueParameters) {
-        for (int v = 0; v < nvec; v++)
-            for (size_t k = 0; k < M; k++)
-                out[v * M + k] = m_parameterDependency[k].second *
-                                 angles[(size_t)v * nangles + m_parameterDependency[k].first];
-        return true;
-    }
-    log_msg("Incorrect number of angles given: %d", nangles);
-    return false;
+    for (int v = 0; v < nvec; v++)
+        if (!toOperatorAngles(angles + (size_t)v * nangles, (size_t)nangles, out.data() + (size_t)v * M)) return false;
+    return true;
 }
 
 // src/AnsatzManager.cpp:709-754
 bool Manager::getExpectationValues(int nvec, int nangles, const double *angles, std::vector<double> &E)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::vector<double> dec;
     if (!decompressBatch(nvec, nangles, angles, dec)) return false;
     E.resize(nvec);
     std::string err;
-    if (!m_engine->batch(nvec, dec.data(), E.data(), nullptr, err)) {
+    if (!eng_->batch(nvec, dec.data(), E.data(), nullptr, err)) {
         log_msg("Batched energies failed: %s", err.c_str());
         return false;
     }
-    for (auto &x : E) x += m_nuclearEnergy;
+    for (auto &x : E) x += enuc_;
     return true;
 }
 
 bool Manager::getEnergyGradients(int nvec, int nangles, const double *angles, std::vector<double> &E,
                                  std::vector<double> &grads)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::vector<double> dec;
     if (!decompressBatch(nvec, nangles, angles, dec)) return false;
-    const size_t M = m_excitations.size();
+    const size_t M = ops_.size();
     E.resize(nvec);
     std::vector<double> deriv((size_t)nvec * M);
     std::string err;
-    if (!m_engine->batch(nvec, dec.data(), E.data(), deriv.data(), err)) {
+    if (!eng_->batch(nvec, dec.data(), E.data(), deriv.data(), err)) {
         log_msg("Batched gradients failed: %s", err.c_str());
         return false;
     }
-    grads.resize((size_t)nvec * m_numberOfUniqueParameters);
+    grads.resize((size_t)nvec * nparams_);
     for (int v = 0; v < nvec; v++) {
-        E[v] += m_nuclearEnergy;
-        compressGradient(deriv.data() + (size_t)v * M, grads.data() + (size_t)v * m_numberOfUniqueParameters);
+        E[v] += enuc_;
+        compressGradient(deriv.data() + (size_t)v * M, grads.data() + (size_t)v * nparams_);
     }
     return true;
 }
 
 bool Manager::uploadAngleBatch(int nvec, int nangles, const double *angles)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::vector<double> dec;
     if (!decompressBatch(nvec, nangles, angles, dec)) return false;
     std::string err;
-    if (!m_engine->batch_upload(nvec, dec.data(), err)) {
+    if (!eng_->batch_upload(nvec, dec.data(), err)) {
         log_msg("Batch upload failed: %s", err.c_str());
         return false;
     }
-    m_batchNangles = nangles;
+    batchNangles_ = nangles;
     return true;
 }
 
 bool Manager::runAngleBatch(bool withGradient)
 {
-    if (!m_isConstructed || m_engine->batch_size() < 1) {
+    if (!eng_ || eng_->batch_size() < 1) {
         log_msg("No angle batch uploaded");
         return false;
     }
     std::string err;
-    if (!m_engine->batch_run(withGradient, err)) {
+    if (!eng_->batch_run(withGradient, err)) {
         log_msg("Batch run failed: %s", err.c_str());
         return false;
     }
@@ -645,38 +584,38 @@ bool Manager::runAngleBatch(bool withGradient)
 
 bool Manager::downloadAngleBatch(double *energies, double *gradients)
 {
-    if (!m_isConstructed || m_engine->batch_size() < 1) {
+    if (!eng_ || eng_->batch_size() < 1) {
         log_msg("No angle batch uploaded");
         return false;
     }
-    const int nvec = m_engine->batch_size();
-    const size_t M = m_excitations.size();
+    const int nvec = eng_->batch_size();
+    const size_t M = ops_.size();
     std::vector<double> deriv(gradients ? (size_t)nvec * M : 0);
     std::string err;
-    if (!m_engine->batch_download(energies, gradients ? deriv.data() : nullptr, err)) {
+    if (!eng_->batch_download(energies, gradients ? deriv.data() : nullptr, err)) {
         log_msg("Batch download failed: %s", err.c_str());
         return false;
     }
     for (int v = 0; v < nvec; v++) {
-        energies[v] += m_nuclearEnergy;
+        energies[v] += enuc_;
         if (gradients)
-            compressGradient(deriv.data() + (size_t)v * M, gradients + (size_t)v * m_numberOfUniqueParameters);
+            compressGradient(deriv.data() + (size_t)v * M, gradients + (size_t)v * nparams_);
     }
     return true;
 }
 
 bool Manager::setKernelTiming(bool on)
 {
-    if (!validateToRun()) return false;
-    m_engine->set_timing(on);
+    if (!ensureBuilt()) return false;
+    eng_->set_timing(on);
     return true;
 }
 
 bool Manager::getKernelTiming(int cls, double &ms, long long &launches)
 {
-    if (!validateToRun()) return false;
+    if (!ensureBuilt()) return false;
     std::string err;
-    if (!m_engine->timing(cls, &ms, &launches, err)) {
+    if (!eng_->timing(cls, &ms, &launches, err)) {
         log_msg("Timing query failed: %s", err.c_str());
         return false;
     }
@@ -685,10 +624,10 @@ bool Manager::getKernelTiming(int cls, double &ms, long long &launches)
 
 bool Manager::getDimensions(long long &dim, int &nops, int &nparams)
 {
-    if (!validateToRun()) return false;
-    dim = m_engine->dim();
-    nops = (int)m_excitations.size();
-    nparams = m_numberOfUniqueParameters;
+    if (!ensureBuilt()) return false;
+    dim = eng_->dim();
+    nops = (int)ops_.size();
+    nparams = nparams_;
     return true;
 }
 
@@ -699,46 +638,32 @@ namespace ae {
 bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
                        bool useStop)
 {
-    if (!validateToRun()) return false;
-    const size_t M = m_excitations.size();
-    const int P = m_numberOfUniqueParameters;
+    if (!ensureBuilt()) return false;
+    const size_t M = ops_.size();
+    const int P = nparams_;
     const bool perParam = (int)angles.size() == P && angles.size() != M;
     std::vector<double> x;
-    if (angles.size() == M) x = angles;
-    else if (perParam) {
-        x.resize(M);
-        for (size_t k = 0; k < M; k++)
-            x[k] = m_parameterDependency[k].second * angles[m_parameterDependency[k].first];
-    } else {
-        log_msg("Incorrect number of angles given: %zu", angles.size());
-        return false;
-    }
-    std::vector<int> order(M);
-    std::vector<double> scale(M);
-    for (size_t k = 0; k < M; k++) {
-        order[k] = m_parameterDependency[k].first;
-        scale[k] = m_parameterDependency[k].second;
-    }
-    Engine *eng = m_engine.get();
+    if (!toOperatorAngles(angles, x)) return false;
+    Engine *eng = eng_.get();
     std::string err;
     EvalFns f;
     f.hessian = [&](const std::vector<double> &a, std::vector<double> &H, std::vector<double> &g, double &En) {
         if (!eng->hessian(a.data(), H.data(), g.data(), &En, err)) return false;
-        En += m_nuclearEnergy;
+        En += enuc_;
         return true;
     };
     f.egrad = [&](const std::vector<double> &a, double &En, std::vector<double> &g) {
         std::vector<double> deriv(M);
         if (!eng->evolve(a.data(), err) || !eng->gradient(a.data(), &En, deriv.data(), err)) return false;
-        En += m_nuclearEnergy;
+        En += enuc_;
         compressGradient(deriv.data(), g.data());
         return true;
     };
-    NewtonDriver nd(order, scale, P, f);
+    NewtonDriver nd(paramIdx_, paramScale_, P, f);
     if (useStop) nd.set_stop_energy(stopEnergy);
     const bool ok = nd.optimise(x, E);
     // psi now belongs to the last evaluated angles; force a re-evolve next time
-    m_angles.assign(M, std::nan(""));
+    cur_.assign(M, std::nan(""));
     if (!ok) {
         log_msg("Optimisation failed: %s", err.c_str());
         return false;
@@ -756,9 +681,9 @@ bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<doubl
         // (src/tupsquantities.cpp:655-666)
         std::vector<bool> seen(P, false);
         for (size_t k = 0; k < M; k++) {
-            const int p = m_parameterDependency[k].first;
+            const int p = paramIdx_[k];
             if (!seen[p]) {
-                angles[p] = x[k] / m_parameterDependency[k].second;
+                angles[p] = x[k] / paramScale_[k];
                 seen[p] = true;
             }
         }

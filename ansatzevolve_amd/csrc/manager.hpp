@@ -1,8 +1,9 @@
-// Host-side mirror of the reference's stateAnsatzManager
-// (src/AnsatzManager.{h,cpp}): stores the set-up, constructs lazily on the
-// first evaluation, detects the particle-number / S_z symmetry that picks
-// the sector, converts angle formats and caches the last angles.  The
-// evaluations themselves run on the device engine.
+// Host-side counterpart of the reference's stateAnsatzManager
+// (src/AnsatzManager.{h,cpp}), with the same public operations: it records
+// the set-up, builds the device engine on the first evaluation, picks the
+// sector from the conserved quantities of the start state and the
+// operators, converts angle formats and remembers the angles of the current
+// state.  The evaluations run on the device engine.
 #pragma once
 #include <array>
 #include <memory>
@@ -18,7 +19,7 @@ namespace ae {
 
 class Manager {
 public:
-    std::mutex lock;  // taken by every C-ABI call (src/AnsatzManager.h:121)
+    std::mutex lock;  // taken by every C-ABI call (the reference's per-manager mutex, src/AnsatzManager.h:121)
 
     bool storeOperators(const std::vector<std::array<int8_t, 4>> &excs);
     bool storeInitial(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs);
@@ -64,41 +65,63 @@ public:
     bool getEngineOption(int key, int &value);
     bool autotune(int nvec, int nangles, const double *angles, const char *cacheFile, int reps, std::string &report);
 
-    int numberOfParameters() const { return m_numberOfUniqueParameters; }
-    int batchAngleCount() const { return m_batchNangles; }
-    // The device engine after construction (nullptr before); used by the
+    int numberOfParameters() const { return nparams_; }
+    int batchAngleCount() const { return batchNangles_; }
+    // The device engine once built (nullptr before); used by the
     // multi-instance batch, which holds every involved manager's lock.
-    Engine *engine() { return m_isConstructed ? m_engine.get() : nullptr; }
+    Engine *engine() { return eng_.get(); }
 
 private:
-    bool validateToRun();
-    bool construct();
-    void setOperatorSymmetry();
+    // Builds the device engine on the first evaluation; false (with the
+    // reason logged) while something the evaluation needs is missing.
+    bool ensureBuilt();
+    bool build();
+    // True, with a log line naming `what`, once the engine is built: the
+    // set-up is fixed from the first evaluation on.
+    bool frozen(const char *what) const;
+    // The sector the engine works in: S_z when the start state has a fixed
+    // (up, down) occupation and every operator conserves it; particle number
+    // when only the total is fixed; else the full space.
+    int chooseSector() const;
+    // Per-operator angles from angles given per operator (M) or per
+    // parameter (P, x_k = scale_k theta_order_k); false for other lengths.
+    bool toOperatorAngles(const double *in, size_t n, double *out) const;
+    bool toOperatorAngles(const std::vector<double> &in, std::vector<double> &out) const;
     bool setAngles(std::vector<double> angles);
     bool gradientAt(const std::vector<double> &angles, double *E, std::vector<double> &deriv);
     bool decompressBatch(int nvec, int nangles, const double *angles, std::vector<double> &out);
     void compressGradient(const double *deriv, double *grad) const;
 
-    bool m_isConstructed = false;
-    double m_nuclearEnergy = 0;
-    int m_numberOfUniqueParameters = -1;
-    std::vector<std::array<int8_t, 4>> m_excitations;
-    std::vector<std::pair<int, double>> m_parameterDependency;
-    std::vector<int> m_startIdx;
-    std::vector<double> m_startCoeff;
-    int m_numberOfQubits = -1;
-    int m_numberOfParticles = -1, m_spinUp = -1, m_spinDown = -1;
-    bool m_InitialSZSym = false, m_OperatorSZSym = false, m_particleSym = false;
-    std::vector<int> m_iIndexes, m_jIndexes;
-    std::vector<double> m_coeffs;
-    int m_norb = 0;
-    std::vector<double> m_h, m_g;
-    int m_hamStorage = 0;
-    std::string m_runPath;
+    // --- the stored set-up
+    struct Start {
+        int qubits = -1;
+        std::vector<int> bits;      // zero-based basis states
+        std::vector<double> amps;
+    } start_;
+    // occupation shared by every nonzero component of the start state
+    struct Occupation {
+        bool fixedTotal = false;  // every component has `total` electrons
+        bool fixedSpins = false;  // ... and the same (up, down) split
+        int total = -1, up = -1, down = -1;
+    } occ_;
+    std::vector<std::array<int8_t, 4>> ops_;  // zero-based, -1 absent
+    std::vector<int> paramIdx_;               // parameter of each operator
+    std::vector<double> paramScale_;          // its scale
+    int nparams_ = -1;
+    struct Triplets {
+        std::vector<int> i, j;  // zero-based full-space indices
+        std::vector<double> v;
+    } trip_;
+    int norb_ = 0;
+    std::vector<double> h1_, g2_;  // one- and two-electron integrals
+    int storage_ = 0;               // HamStorage
+    std::string runPrefix_;
+    double enuc_ = 0;
 
-    std::unique_ptr<Engine> m_engine;
-    std::vector<double> m_angles;  // per-operator angles of the current psi
-    int m_batchNangles = 0;        // angle count of the uploaded batch
+    // --- the built state
+    std::unique_ptr<Engine> eng_;
+    std::vector<double> cur_;  // per-operator angles of the engine's current psi
+    int batchNangles_ = 0;     // angle count of the uploaded batch
 };
 
 } // namespace ae
