@@ -37,6 +37,7 @@ C_API = [
     ("setHamiltonianIntegrals", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
     ("getGradientOperators", ctypes.c_int, [ctypes.c_int, _dp, _dp, _vp]),
     ("getOverlapGradient", ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _vp]),
+    ("getHessianProj", ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _vp]),
     ("setNuclearEnergy", ctypes.c_int, [ctypes.c_double, _vp]),
     ("setHamiltonianStorage", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getEnergyGradient", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _vp]),
@@ -49,6 +50,10 @@ C_API = [
     ("setKernelTiming", ctypes.c_int, [ctypes.c_int, _vp]),
     ("getKernelTiming", ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _vp]),
     ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
+    ("iterativeOptimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _ip, _vp]),
+    ("subspaceDiagonalise", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _vp]),
+    ("getProperties", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                      _dp, _vp]),
     ("multiBatchCreate", _vp, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("multiBatchUpload", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     ("multiBatchRun", ctypes.c_int, [_vp, ctypes.c_int]),
@@ -228,6 +233,19 @@ class AnsatzSynth:
                                                                     g.ctypes.data_as(_dp), self.ctx))
         return ov.value, g
 
+    def hessian_proj(self, angles, proj_full):
+        """(Hessian, gradient, value) of the overlap <proj|psi(angles)>; proj
+        over the full space of 2^qubits basis states."""
+        a, ap = _d(angles)
+        pr, pp = _d(proj_full)
+        P = len(a)
+        H = np.zeros(P * P)
+        g = np.zeros(P)
+        ov = ctypes.c_double(0)
+        self._chk("getHessianProj", self.lib.getHessianProj(P, ap, len(pr), pp, H.ctypes.data_as(_dp),
+                                                            g.ctypes.data_as(_dp), ctypes.byref(ov), self.ctx))
+        return H.reshape(P, P).T.copy(), g, ov.value
+
     def hessian(self, angles):
         a, ap = _d(angles)
         P = len(a)
@@ -345,6 +363,56 @@ class AnsatzSynth:
         nt = ctypes.c_int()
         self._chk("optimise", self.lib.optimise(len(a), a.ctypes.data_as(_dp), ctypes.byref(E), float(stop_energy),
                                                 max_trace, tr.ctypes.data_as(_dp), ctypes.byref(nt), self.ctx))
+        return a, E.value, tr[:nt.value].copy()
+
+    def subspace_diagonalise(self, angles2d):
+        """(eigenvalues, H, S, null rank) of the Hamiltonian in the span of
+        the states of the angle vectors (rows)."""
+        A = np.ascontiguousarray(np.atleast_2d(angles2d), dtype=np.float64)
+        n = A.shape[0]
+        ev, H, S = np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
+        nr = ctypes.c_int()
+        self._chk("subspaceDiagonalise", self.lib.subspaceDiagonalise(
+            n, A.shape[1], A.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), H.ctypes.data_as(_dp),
+            S.ctypes.data_as(_dp), ctypes.byref(nr), self.ctx))
+        return ev, H, S, nr.value
+
+    PROPERTY_COLUMNS = ["energy", "energy_with_nuclear", "hess_neg", "hess_zero", "hess_pos", "hess_diag_neg",
+                        "hess_diag_zero", "hess_diag_pos", "metric_pos", "metric_zero", "metric_diag_zero",
+                        "grad_norm"]
+
+    def properties(self, angles2d, noverlaps=1, no_hessian=False):
+        """writeProperties: (rows as dicts, Hessians, metrics, lowest
+        eigenvalues of H)."""
+        A = np.ascontiguousarray(np.atleast_2d(angles2d), dtype=np.float64)
+        n = A.shape[0]
+        P = self.dimensions()[2]
+        cols = 12 + 2 * noverlaps
+        R = np.zeros((n, cols))
+        H = np.zeros((n, P, P))
+        G = np.zeros((n, P, P))
+        ev = np.zeros(max(noverlaps, 1))
+        self._chk("getProperties", self.lib.getProperties(
+            n, A.shape[1], A.ctypes.data_as(_dp), int(noverlaps), 1 if no_hessian else 0, R.ctypes.data_as(_dp),
+            H.ctypes.data_as(_dp), G.ctypes.data_as(_dp), ev.ctypes.data_as(_dp), self.ctx))
+        rows = []
+        for r in R:
+            d = {k: float(v) for k, v in zip(self.PROPERTY_COLUMNS, r[:12])}
+            d["overlaps"] = [float(r[12 + 2 * t]) for t in range(noverlaps)]
+            rows.append(d)
+        return rows, H, G, ev[:noverlaps]
+
+    def iterative_optimise(self, angles, ediis=False, max_trace=10000):
+        """The iterative projected optimiser (iterativeTups) from `angles`;
+        returns (angles, E, trace) with one trace row per outer iteration."""
+        a = np.array(angles, dtype=np.float64)
+        E = ctypes.c_double()
+        tr = np.zeros((max_trace, 5))
+        nt = ctypes.c_int()
+        self._chk("iterativeOptimise", self.lib.iterativeOptimise(len(a), a.ctypes.data_as(_dp), ctypes.byref(E),
+                                                                  1 if ediis else 0, max_trace,
+                                                                  tr.ctypes.data_as(_dp), ctypes.byref(nt),
+                                                                  self.ctx))
         return a, E.value, tr[:nt.value].copy()
 
 

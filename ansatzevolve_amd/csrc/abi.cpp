@@ -317,6 +317,47 @@ int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, con
     return 0;
 }
 
+int getHessianProj(int nangles, const double *angles, int nbasisvectors, const double *projvector,
+                   double *hessian, double *gradient, double *overlap, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr || projvector == nullptr) {
+        log_msg("the angle or projection array is null");
+        return 2;
+    }
+    if (hessian == nullptr) {
+        log_msg("the Hessian output array is null");
+        return 3;
+    }
+    if (nangles < 1 || nbasisvectors < 1) {
+        log_msg("the angle count or the basis size is below 1");
+        return 4;
+    }
+    if (traceInterfaceCalls) log_msg("getHessianProj(nangles=%d)", nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    long long dim = 0;
+    int nops = 0, nparams = 0;
+    if (!m->getDimensions(dim, nops, nparams)) return 5;
+    if (nangles != nparams) {
+        log_msg("gradient length does not match the parameter count");
+        return -6;
+    }
+    std::vector<double> H, g;
+    double ov = 0;
+    const std::vector<double> proj(projvector, projvector + nbasisvectors);
+    if (!m->getHessianProj(std::vector<double>(angles, angles + nangles), proj, H, g, ov)) return 5;
+    // row-major P x P from the engine; the interface is column-major (symmetric anyway)
+    for (int i = 0; i < nparams; i++)
+        for (int j = 0; j < nparams; j++) hessian[(size_t)nparams * j + i] = H[(size_t)i * nparams + j];
+    if (gradient) std::memcpy(gradient, g.data(), g.size() * sizeof(double));
+    if (overlap) *overlap = ov;
+    return 0;
+}
+
 int getHessian_COMP(int nangles, const double *angles, double *hessian, void *ctx)
 {
     if (ctx == nullptr) {
@@ -748,5 +789,95 @@ extern "C" int optimise(int nangles, double *angles, double *energy, double stop
         std::memcpy(trace, tr.data(), sizeof(double) * 5 * (size_t)(w > 0 ? w : 0));
         *ntrace = w;
     }
+    return 0;
+}
+
+extern "C" int iterativeOptimise(int nangles, double *angles, double *energy, int ediis, int maxtrace, double *trace,
+                                 int *ntrace, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr || energy == nullptr) {
+        log_msg("the angle or energy array is null");
+        return 2;
+    }
+    if (nangles < 1) {
+        log_msg("the angle count is below 1");
+        return 3;
+    }
+    if (traceInterfaceCalls) log_msg("iterativeOptimise(nangles=%d, ediis=%d)", nangles, ediis);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> a(angles, angles + nangles), tr;
+    double E = 0;
+    if (!m->iterativeOptimise(a, E, tr, ediis != 0)) return 5;
+    std::memcpy(angles, a.data(), sizeof(double) * nangles);
+    *energy = E;
+    if (trace && ntrace) {
+        const int rows = (int)(tr.size() / 5);
+        const int w = rows < maxtrace ? rows : maxtrace;
+        std::memcpy(trace, tr.data(), sizeof(double) * 5 * (size_t)(w > 0 ? w : 0));
+        *ntrace = w;
+    }
+    return 0;
+}
+
+extern "C" int subspaceDiagonalise(int nsets, int nangles, const double *angles, double *eigenvalues, double *hmat,
+                                   double *smat, int *nullrank, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr || eigenvalues == nullptr) {
+        log_msg("the angle or eigenvalue array is null");
+        return 2;
+    }
+    if (nsets < 1 || nangles < 1) {
+        log_msg("the set count or the angle count is below 1");
+        return 3;
+    }
+    if (traceInterfaceCalls) log_msg("subspaceDiagonalise(nsets=%d, nangles=%d)", nsets, nangles);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> ev, H, S;
+    int nr = 0;
+    if (!m->subspaceDiagonalise(nsets, nangles, angles, ev, H, S, nr)) return 5;
+    std::memcpy(eigenvalues, ev.data(), sizeof(double) * nsets);
+    // symmetric: row-major = column-major
+    if (hmat) std::memcpy(hmat, H.data(), sizeof(double) * H.size());
+    if (smat) std::memcpy(smat, S.data(), sizeof(double) * S.size());
+    if (nullrank) *nullrank = nr;
+    return 0;
+}
+
+extern "C" int getProperties(int nsets, int nangles, const double *angles, int noverlaps, int nohess, double *props,
+                             double *hessians, double *metrics, double *eigenvalues, void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr || props == nullptr || (noverlaps > 0 && eigenvalues == nullptr)) {
+        log_msg("the angle, property or eigenvalue array is null");
+        return 2;
+    }
+    if (nsets < 1 || nangles < 1 || noverlaps < 0 || noverlaps > 16) {
+        log_msg("the set count or angle count is below 1, or the overlap count outside 0 .. 16");
+        return 3;
+    }
+    if (traceInterfaceCalls) log_msg("getProperties(nsets=%d, nangles=%d, noverlaps=%d)", nsets, nangles, noverlaps);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> rows, H, G, low;
+    if (!m->properties(nsets, nangles, angles, noverlaps, nohess != 0, rows, H, G, low)) return 5;
+    std::memcpy(props, rows.data(), sizeof(double) * rows.size());
+    const int P = m->numberOfParameters();
+    // per set P x P, symmetric: row-major = column-major
+    if (!nohess && hessians) std::memcpy(hessians, H.data(), sizeof(double) * (size_t)nsets * P * P);
+    if (!nohess && metrics) std::memcpy(metrics, G.data(), sizeof(double) * (size_t)nsets * P * P);
+    if (noverlaps > 0) std::memcpy(eigenvalues, low.data(), sizeof(double) * noverlaps);
     return 0;
 }

@@ -221,6 +221,7 @@ struct EngineImpl {
     DBuf<double> b_psi, b_lam, b_partial, b_epart, b_out;
     DBuf<double2> b_cs, b_csneg;
     DBuf<double> hz_B, hz_HB, hz_psi2, hz_lam2, hz_xpart, hz_dpart, hz_gram, hz_out, hz_epart;
+    DBuf<double2> sub_cs, sub_csneg;  // angle tables of subspace() (apart from the graph-captured batch tables)
 
     int bnvec = 0;  // size of the uploaded angle batch
 
@@ -2271,6 +2272,71 @@ bool Engine::download_state(double *out, std::string &err)
     return true;
 }
 
+bool Engine::download_ham_state(double *out, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.ham_apply(e.d_psi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
+    HCHK(hipMemcpyAsync(out, e.d_lam.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::apply_ham(const double *x, double *y, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (e.hkind == 0) {
+        err = "no Hamiltonian";
+        return false;
+    }
+    HCHK(hipMemcpyAsync(e.d_phi.p, x, 8 * e.N, hipMemcpyHostToDevice, e.stream));
+    if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
+    HCHK(hipMemcpyAsync(y, e.d_lam.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+bool Engine::subspace(int nvec, const double *angles, double *Hmat, double *Smat, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    if (nvec < 1) {
+        err = "empty set of angle vectors";
+        return false;
+    }
+    const long long N = e.N;
+    const int chunks = 64;
+    const size_t nn = (size_t)nvec * nvec;
+    if (!e.hz_B.ensure((size_t)nvec * N, err) || !e.hz_HB.ensure((size_t)nvec * N, err) ||
+        !e.hz_epart.ensure((size_t)nvec * e.eslots_alloc(), err) || !e.hz_gram.ensure(chunks * nn, err) ||
+        !e.hz_out.ensure(2 * nn, err))
+        return false;
+    if (!e.upload_cs(angles, nvec, e.sub_cs, e.sub_csneg, err)) return false;
+    const long long total = (long long)nvec * N;
+    hipLaunchKernelGGL(k_bcast, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 16384)),
+                       dim3(kBlock), 0, e.stream, e.d_start.p, e.hz_B.p, N, nvec);
+    LCHK();
+    if (!e.sweep_forward(e.hz_B.p, nvec, e.sub_cs.p, err)) return false;
+    if (!e.ham_apply(e.hz_B.p, e.hz_HB.p, nvec, e.hz_epart.p, err)) return false;
+    const int tiles = (nvec + 15) / 16;
+    for (int which = 0; which < 2; which++) {
+        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p,
+                           which ? e.hz_B.p : e.hz_HB.p, N, nvec, N, chunks, e.hz_gram.p);
+        LCHK();
+        hipLaunchKernelGGL(k_reduce, dim3((unsigned)nn), dim3(kBlock), 0, e.stream, e.hz_gram.p, (int)nn, chunks,
+                           1LL, (long long)nn, e.hz_out.p + which * nn, 1.0);
+        LCHK();
+    }
+    std::vector<double> out(2 * nn);
+    HCHK(hipMemcpyAsync(out.data(), e.hz_out.p, 8 * out.size(), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    std::memcpy(Hmat, out.data(), 8 * nn);
+    std::memcpy(Smat, out.data() + nn, 8 * nn);
+    return true;
+}
+
 bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
 {
     EngineImpl &e = *d_;
@@ -2796,21 +2862,43 @@ bool Engine::batch(int nvec, const double *angles, double *E, double *deriv, std
 //      Hessian = THT + 2 (Y + Y^T) - 2 diag(sum_{k in mu} scale_k^2 D[k]).
 // This is the reference's quantity (src/fusedevolve.cpp:2437-2780) obtained
 // with forward-propagated tangents instead of one adjoint sweep per angle.
-bool Engine::hessian(const double *angles, double *hess, double *grad, double *E, std::string &err)
+bool Engine::hessian(const double *angles, double *hess, double *grad, double *E, std::string &err, double *metric)
+{
+    return second_derivatives(angles, nullptr, hess, grad, E, metric, err);
+}
+
+bool Engine::hessian_proj(const double *angles, const double *proj, double *hess, double *grad, double *overlap,
+                          std::string &err)
+{
+    return second_derivatives(angles, proj, hess, grad, overlap, nullptr, err);
+}
+
+// Second derivatives by one forward sweep of tangents.  proj == nullptr:
+// E = <psi|H|psi> (lambda_0 = U^T H psi, factor 2, plus the <dpsi|H|dpsi>
+// term); otherwise f = <proj|psi> (lambda_0 = U^T proj, factor 1, no
+// such term; src/fusedevolve.cpp:2892-3183).  metric (optional, energy
+// only): g[mu][nu] = <dpsi/dtheta_mu|dpsi/dtheta_nu>.
+bool Engine::second_derivatives(const double *angles, const double *proj, double *hess, double *grad, double *val,
+                                double *metric, std::string &err)
 {
     EngineImpl &e = *d_;
-    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+    if (!e.haveStart || e.M == 0 || (!proj && e.hkind == 0)) {
         err = "engine not set up";
         return false;
     }
     const int M = e.M, P = e.P;
     const long long N = e.N;
-    // 1.
-    if (!evolve(angles, err)) return false;
+    // 1. the first derivatives; afterwards d_lam = lambda_0 (the adjoint
+    // state at the start) and d_phi ~ psi_0
     std::vector<double> deriv(M);
     double E0 = 0;
-    if (!gradient(angles, &E0, deriv.data(), err)) return false;
-    // d_phi ~ psi_0 now, d_lam = lambda_0.
+    if (proj) {
+        if (!overlap_gradient(angles, proj, &E0, deriv.data(), err)) return false;
+    } else {
+        if (!evolve(angles, err)) return false;
+        if (!gradient(angles, &E0, deriv.data(), err)) return false;
+    }
+    const double fac = proj ? 1.0 : 2.0;
     // parameters ordered by their first operator, so the active set at
     // operator k is a prefix
     std::vector<int> first(P, M);
@@ -2824,7 +2912,7 @@ bool Engine::hessian(const double *angles, double *hess, double *grad, double *E
     const size_t PN = (size_t)P * N;
     if (!e.hz_B.ensure(PN, err) || !e.hz_HB.ensure(PN, err) || !e.hz_psi2.ensure(2 * N, err) ||
         !e.hz_lam2.ensure(2 * N, err) || !e.hz_xpart.ensure((size_t)M * P * G, err) ||
-        !e.hz_dpart.ensure((size_t)M * G, err) || !e.hz_out.ensure((size_t)M * P + M + (size_t)P * P, err) ||
+        !e.hz_dpart.ensure((size_t)M * G, err) ||
         !e.hz_epart.ensure((size_t)P * e.eslots_alloc(), err))
         return false;
     auto fill = [&](double *p, size_t cnt) {
@@ -2850,25 +2938,42 @@ bool Engine::hessian(const double *angles, double *hess, double *grad, double *E
         std::swap(psiA, psiB);
         std::swap(lamA, lamB);
     }
-    // 3. H B, Gram, reductions
-    if (!e.ham_apply(e.hz_B.p, e.hz_HB.p, P, e.hz_epart.p, err)) return false;
+    // 3. (energy) H B and the Gram matrix 2 B (H B)^T; the metric B B^T;
+    // reductions
     const int chunks = 64;
     const int tiles = (P + 15) / 16;
-    if (!e.hz_gram.ensure((size_t)chunks * P * P, err)) return false;
-    hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_HB.p, N, P, N,
-                       chunks, e.hz_gram.p);
-    LCHK();
-    double *outX = e.hz_out.p, *outD = outX + (size_t)M * P, *outT = outD + M;
+    const bool withT = !proj, withG = metric && !proj;
+    if (!e.hz_out.ensure((size_t)M * P + M + 2 * (size_t)P * P, err)) return false;
+    double *outX = e.hz_out.p, *outD = outX + (size_t)M * P, *outT = outD + M, *outG = outT + (size_t)P * P;
+    if (withT || withG) {
+        if (!e.hz_gram.ensure((size_t)chunks * P * P, err)) return false;
+    }
+    if (withT) {
+        if (!e.ham_apply(e.hz_B.p, e.hz_HB.p, P, e.hz_epart.p, err)) return false;
+        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_HB.p, N, P,
+                           N, chunks, e.hz_gram.p);
+        LCHK();
+        hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
+                           (long long)P * P, outT, 2.0);
+        LCHK();
+    }
+    if (withG) {
+        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_B.p, N, P,
+                           N, chunks, e.hz_gram.p);
+        LCHK();
+        hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
+                           (long long)P * P, outG, 1.0);
+        LCHK();
+    }
     hipLaunchKernelGGL(k_reduce, dim3(M * P), dim3(kBlock), 0, e.stream, e.hz_xpart.p, M * P, G, (long long)G, 1LL,
                        outX, 1.0);
     hipLaunchKernelGGL(k_reduce, dim3(M), dim3(kBlock), 0, e.stream, e.hz_dpart.p, M, G, (long long)G, 1LL, outD, 1.0);
-    hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
-                       (long long)P * P, outT, 2.0);
     LCHK();
-    std::vector<double> hout((size_t)M * P + M + (size_t)P * P);
-    HCHK(hipMemcpyAsync(hout.data(), e.hz_out.p, 8 * hout.size(), hipMemcpyDeviceToHost, e.stream));
+    std::vector<double> hout((size_t)M * P + M + 2 * (size_t)P * P, 0.0);
+    const size_t nread = (size_t)M * P + M + (withG ? 2 : withT ? 1 : 0) * (size_t)P * P;
+    HCHK(hipMemcpyAsync(hout.data(), e.hz_out.p, 8 * nread, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
-    const double *X = hout.data(), *D = X + (size_t)M * P, *T = D + M;
+    const double *X = hout.data(), *D = X + (size_t)M * P, *T = D + M, *Gm = T + (size_t)P * P;
     // assemble in parameter order
     std::vector<double> Y((size_t)P * P, 0.0), Dg(P, 0.0);
     for (int k = 0; k < M; k++) {
@@ -2879,15 +2984,16 @@ bool Engine::hessian(const double *angles, double *hess, double *grad, double *E
     for (int i = 0; i < P; i++)
         for (int j = 0; j < P; j++) {
             const int a = std::min(pos[i], pos[j]), b = std::max(pos[i], pos[j]);
-            double v = T[(size_t)a * P + b] + 2.0 * (Y[(size_t)i * P + j] + Y[(size_t)j * P + i]);
-            if (i == j) v -= 2.0 * Dg[i];
+            double v = (withT ? T[(size_t)a * P + b] : 0.0) + fac * (Y[(size_t)i * P + j] + Y[(size_t)j * P + i]);
+            if (i == j) v -= fac * Dg[i];
             hess[(size_t)i * P + j] = v;
+            if (withG) metric[(size_t)i * P + j] = Gm[(size_t)a * P + b];
         }
     if (grad) {
         for (int p = 0; p < P; p++) grad[p] = 0;
         for (int k = 0; k < M; k++) grad[e.order[k]] += e.scale[k] * deriv[k];
     }
-    if (E) *E = E0;
+    if (val) *val = E0;
     return true;
 }
 

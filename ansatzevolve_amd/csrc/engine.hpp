@@ -102,10 +102,25 @@ public:
     // proj: N amplitudes in sector order.
     bool overlap_gradient(const double *angles, const double *proj, double *overlap, double *deriv,
                           std::string &err);
-    // Per-parameter Hessian (row-major P x P), gradient and energy.
-    bool hessian(const double *angles, double *hess, double *grad, double *E, std::string &err);
+    // Per-parameter Hessian (row-major P x P), gradient and energy; metric
+    // (optional, P x P): g[mu][nu] = <dpsi/dtheta_mu|dpsi/dtheta_nu>.
+    bool hessian(const double *angles, double *hess, double *grad, double *E, std::string &err,
+                 double *metric = nullptr);
+    // The same for the overlap f = <proj|psi(angles)> (proj: N amplitudes in
+    // sector order): d2f/dtheta_mu dtheta_nu, df/dtheta, f
+    // (FusedEvolve::evolveHessianProj, src/fusedevolve.cpp:2892-3183).
+    bool hessian_proj(const double *angles, const double *proj, double *hess, double *grad, double *overlap,
+                      std::string &err);
     // The current psi over the sector.
     bool download_state(double *out, std::string &err);
+    // H psi of the current psi over the sector.
+    bool download_ham_state(double *out, std::string &err);
+    // y = H x for one host vector over the sector.
+    bool apply_ham(const double *x, double *y, std::string &err);
+    // The states psi_v = U(angles_v) psi_0 of nvec per-operator angle
+    // vectors (row-major nvec x M) and their Gram matrices (row-major
+    // nvec x nvec): Hmat[v][w] = <psi_v|H|psi_w>, Smat[v][w] = <psi_v|psi_w>.
+    bool subspace(int nvec, const double *angles, double *Hmat, double *Smat, std::string &err);
     // Batched: nvec angle vectors (row-major nvec x M).  deriv may be null.
     bool batch(int nvec, const double *angles, double *E, double *deriv, std::string &err);
     // The same in three steps, so that repeated evaluations run with their
@@ -152,6 +167,8 @@ public:
     int num_generic_ops() const;
 
 private:
+    bool second_derivatives(const double *angles, const double *proj, double *hess, double *grad, double *val,
+                            double *metric, std::string &err);
     std::unique_ptr<EngineImpl> d_;
 };
 

@@ -3,6 +3,7 @@
 // angle conversions.
 #include "manager.hpp"
 #include "runfiles.hpp"
+#include "spectral.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -507,6 +508,33 @@ bool Manager::getHessianComp(const std::vector<double> &angles, std::vector<doub
     return true;
 }
 
+bool Manager::getHessianProj(const std::vector<double> &angles, const std::vector<double> &projFull,
+                             std::vector<double> &hess, std::vector<double> &grad, double &overlap)
+{
+    if (!ensureBuilt()) return false;
+    std::vector<double> x;
+    if (!toOperatorAngles(angles, x)) return false;
+    if (projFull.size() != ((size_t)1 << start_.qubits)) {
+        log_msg("Projection vector has %zu entries, expected 2^%d", projFull.size(), start_.qubits);
+        return false;
+    }
+    const auto &st = eng_->states();
+    std::vector<double> comp(st.size());
+    for (size_t i = 0; i < st.size(); i++) comp[i] = projFull[st[i]];
+    const int P = nparams_;
+    hess.assign((size_t)P * P, 0.0);
+    grad.assign(P, 0.0);
+    std::string err;
+    // the projected Hessian runs its own forward sweep; psi ends as U(x) psi_0
+    if (!eng_->hessian_proj(x.data(), comp.data(), hess.data(), grad.data(), &overlap, err)) {
+        log_msg("Projected Hessian failed: %s", err.c_str());
+        cur_.assign(ops_.size(), std::nan(""));
+        return false;
+    }
+    cur_ = x;
+    return true;
+}
+
 bool Manager::decompressBatch(int nvec, int nangles, const double *angles, std::vector<double> &out)
 {
     const size_t M = ops_.size();
@@ -689,6 +717,179 @@ bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<doubl
         }
     } else {
         angles = x;
+    }
+    return true;
+}
+
+bool Manager::iterativeOptimise(std::vector<double> &angles, double &E, std::vector<double> &trace, bool ediis)
+{
+    if (!ensureBuilt()) return false;
+    const size_t M = ops_.size();
+    const int P = nparams_;
+    const bool perParam = (int)angles.size() == P && angles.size() != M;
+    std::vector<double> x;
+    if (!toOperatorAngles(angles, x)) return false;
+    Engine *eng = eng_.get();
+    const auto &st = eng->states();
+    std::vector<double> proj(st.size());  // H psi of the last outer iterate, sector order
+    std::string err;
+    ProjFns f;
+    f.setProjection = [&](const std::vector<double> &a, std::vector<double> &psi) {
+        psi.resize(st.size());
+        return eng->evolve(a.data(), err) && eng->download_state(psi.data(), err) &&
+               eng->download_ham_state(proj.data(), err);
+    };
+    f.hessianProj = [&](const std::vector<double> &a, std::vector<double> &H, std::vector<double> &g, double &v) {
+        H.assign((size_t)P * P, 0.0);
+        g.assign(P, 0.0);
+        return eng->hessian_proj(a.data(), proj.data(), H.data(), g.data(), &v, err);
+    };
+    f.gradProj = [&](const std::vector<double> &a, double &v, std::vector<double> &g) {
+        std::vector<double> deriv(M);
+        if (!eng->overlap_gradient(a.data(), proj.data(), &v, deriv.data(), err)) return false;
+        g.resize(P);
+        compressGradient(deriv.data(), g.data());
+        return true;
+    };
+    ProjectedDriver pd(paramIdx_, paramScale_, P, f);
+    pd.useEdiis = ediis;
+    const bool ok = pd.optimise(x, E);
+    cur_.assign(M, std::nan(""));  // psi belongs to the last evaluated angles
+    if (!ok) {
+        log_msg("Iterative optimisation failed: %s", err.c_str());
+        return false;
+    }
+    E += enuc_;
+    trace.clear();
+    for (const auto &tp : pd.trace) {
+        trace.push_back(tp.seconds);
+        trace.push_back(tp.energy + enuc_);
+        trace.push_back(tp.gradNorm);
+        trace.push_back((double)tp.hessianEvals);
+        trace.push_back((double)tp.energyEvals);
+    }
+    if (perParam) {
+        std::vector<bool> seen(P, false);
+        for (size_t k = 0; k < M; k++)
+            if (!seen[paramIdx_[k]]) {
+                angles[paramIdx_[k]] = x[k] / paramScale_[k];
+                seen[paramIdx_[k]] = true;
+            }
+    } else {
+        angles = x;
+    }
+    return true;
+}
+
+bool Manager::subspaceDiagonalise(int nsets, int nangles, const double *angles, std::vector<double> &evals,
+                                  std::vector<double> &H, std::vector<double> &S, int &nullRank)
+{
+    if (!ensureBuilt()) return false;
+    std::vector<double> dec;
+    if (!decompressBatch(nsets, nangles, angles, dec)) return false;
+    H.assign((size_t)nsets * nsets, 0.0);
+    S.assign((size_t)nsets * nsets, 0.0);
+    std::string err;
+    if (!eng_->subspace(nsets, dec.data(), H.data(), S.data(), err)) {
+        log_msg("Subspace Gram matrices failed: %s", err.c_str());
+        return false;
+    }
+    // exact symmetry: <i|H|j> and <j|H|i> are summed in different orders
+    for (int i = 0; i < nsets; i++)
+        for (int j = 0; j < i; j++) {
+            const double h = 0.5 * (H[(size_t)i * nsets + j] + H[(size_t)j * nsets + i]);
+            const double o = 0.5 * (S[(size_t)i * nsets + j] + S[(size_t)j * nsets + i]);
+            H[(size_t)i * nsets + j] = H[(size_t)j * nsets + i] = h;
+            S[(size_t)i * nsets + j] = S[(size_t)j * nsets + i] = o;
+        }
+    subspace_eigen(nsets, H, S, evals, nullRank);
+    return true;
+}
+
+// Per angle vector (src/tupsquantities.cpp:185-568): energies, counts of
+// negative / near-zero / positive eigenvalues and diagonal entries of the
+// Hessian (|e| <= 1e-10 is near zero), of positive and near-zero
+// eigenvalues of the metric g_mu,nu = <dpsi/dtheta_mu|dpsi/dtheta_nu> and of
+// its near-zero diagonal entries, the gradient norm, and the overlaps with
+// the lowest eigenvectors of H.
+bool Manager::properties(int nsets, int nangles, const double *angles, int noverlaps, bool noHess,
+                         std::vector<double> &rows, std::vector<double> &hessians, std::vector<double> &metrics,
+                         std::vector<double> &lowest)
+{
+    if (!ensureBuilt()) return false;
+    const int P = nparams_;
+    const long long N = eng_->dim();
+    const int cols = kPropColumns + 2 * noverlaps;
+    std::string err;
+    std::vector<double> vecs;
+    lowest.clear();
+    if (noverlaps > 0) {
+        std::vector<double> start(N, 0.0);
+        for (size_t k = 0; k < start_.bits.size(); k++) {
+            const long long r = eng_->rank((uint32_t)start_.bits[k]);
+            if (r >= 0) start[r] = start_.amps[k];
+        }
+        auto applyH = [&](const double *x, double *y) { return eng_->apply_ham(x, y, err); };
+        if (!lowest_eigenpairs(noverlaps, N, applyH, start, lowest, vecs, 400, 1e-10)) {
+            log_msg("Lowest eigenvectors did not converge: %s", err.c_str());
+            return false;
+        }
+    }
+    rows.assign((size_t)nsets * cols, 0.0);
+    hessians.assign(noHess ? 0 : (size_t)nsets * P * P, 0.0);
+    metrics.assign(noHess ? 0 : (size_t)nsets * P * P, 0.0);
+    const double zero = 1e-10;
+    std::vector<double> x, psi(N), g(P), Hm((size_t)P * P), Gm((size_t)P * P), w, V;
+    for (int s = 0; s < nsets; s++) {
+        x.resize(ops_.size());
+        if (!toOperatorAngles(angles + (size_t)s * nangles, (size_t)nangles, x.data())) return false;
+        double *row = rows.data() + (size_t)s * cols;
+        double E = 0;
+        if (!noHess) {
+            if (!eng_->hessian(x.data(), Hm.data(), g.data(), &E, err, Gm.data())) {
+                log_msg("Hessian failed: %s", err.c_str());
+                cur_.assign(ops_.size(), std::nan(""));
+                return false;
+            }
+            std::copy(Hm.begin(), Hm.end(), hessians.begin() + (size_t)s * P * P);
+            std::copy(Gm.begin(), Gm.end(), metrics.begin() + (size_t)s * P * P);
+            symmetric_eigen(P, Hm, w, V);
+            for (double e : w) row[e < -zero ? 2 : e > zero ? 4 : 3] += 1;
+            for (int i = 0; i < P; i++) {
+                const double d = Hm[(size_t)i * P + i];
+                row[d < -zero ? 5 : d > zero ? 7 : 6] += 1;
+            }
+            symmetric_eigen(P, Gm, w, V);
+            for (double e : w) row[e > zero ? 8 : 9] += 1;  // a metric has no negative eigenvalues
+            for (int i = 0; i < P; i++) row[10] += std::fabs(Gm[(size_t)i * P + i]) <= zero ? 1 : 0;
+        } else {
+            std::vector<double> deriv(ops_.size());
+            if (!eng_->energy_gradient(x.data(), &E, deriv.data(), err)) {
+                log_msg("Gradient failed: %s", err.c_str());
+                cur_.assign(ops_.size(), std::nan(""));
+                return false;
+            }
+            compressGradient(deriv.data(), g.data());
+        }
+        cur_.assign(ops_.size(), std::nan(""));
+        row[0] = E;
+        row[1] = E + enuc_;
+        double gn = 0;
+        for (double v : g) gn += v * v;
+        row[11] = std::sqrt(gn);
+        if (noverlaps > 0) {
+            if (!eng_->evolve(x.data(), err) || !eng_->download_state(psi.data(), err)) {
+                log_msg("State failed: %s", err.c_str());
+                return false;
+            }
+            cur_ = x;
+            for (int t = 0; t < noverlaps; t++) {
+                double o = 0;
+                for (long long i = 0; i < N; i++) o += psi[i] * vecs[(size_t)t * N + i];
+                row[kPropColumns + 2 * t] = o;
+                row[kPropColumns + 2 * t + 1] = std::fabs(o);
+            }
+        }
     }
     return true;
 }

@@ -43,6 +43,10 @@ public:
     bool getOverlapGradient(const std::vector<double> &angles, const std::vector<double> &projFull,
                             double &overlap, std::vector<double> &grad);
     bool getHessianComp(const std::vector<double> &angles, std::vector<double> &hessRowMajor);
+    // Hessian (row-major), gradient and value of the overlap <proj|psi>, proj
+    // over the full space of 2^qubits basis states (src/fusedevolve.cpp:2892-3183)
+    bool getHessianProj(const std::vector<double> &angles, const std::vector<double> &projFull,
+                        std::vector<double> &hessRowMajor, std::vector<double> &grad, double &overlap);
     bool getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad);
     bool getExpectationValues(int nvec, int nangles, const double *angles, std::vector<double> &E);
     bool getEnergyGradients(int nvec, int nangles, const double *angles, std::vector<double> &E,
@@ -58,6 +62,21 @@ public:
     // Hessian evaluations, energy evaluations.
     bool optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
                   bool useStop);
+    // The Hamiltonian in the span of the states of nsets angle vectors
+    // (doSubspaceDiagonalisation): Loewdin-symmetrised eigenvalues, the
+    // Gram matrices (row-major) and the number of dropped directions.
+    bool subspaceDiagonalise(int nsets, int nangles, const double *angles, std::vector<double> &evals,
+                             std::vector<double> &H, std::vector<double> &S, int &nullRank);
+    // writeProperties: one row of kPropColumns + 2 noverlaps values per
+    // angle vector (see the C interface), the Hessians and metric tensors
+    // (row-major P x P each, unless noHess) and the lowest noverlaps
+    // eigenvalues of H (when noverlaps > 0).
+    static constexpr int kPropColumns = 12;
+    bool properties(int nsets, int nangles, const double *angles, int noverlaps, bool noHess,
+                    std::vector<double> &rows, std::vector<double> &hessians, std::vector<double> &metrics,
+                    std::vector<double> &lowest);
+    // The iterative projected optimiser (projected.cpp); angles and trace as optimise.
+    bool iterativeOptimise(std::vector<double> &angles, double &E, std::vector<double> &trace, bool ediis);
     bool getKernelTiming(int cls, double &ms, long long &launches);
     // Execution choices of the device engine (EngineOption) and the
     // autotuner that picks them for a batch shape (autotune.cpp).

@@ -165,6 +165,115 @@ def wrap_angles(t):
     return np.pi - np.mod(np.pi - np.asarray(t, dtype=np.float64), 2 * np.pi)
 
 
+# ------------------------------------------------------------ TUPS angles
+# Parameter order of the TUPS workload (workloads.tups_operators): per layer,
+# units on pairs (p, p+1), first p = 0, 2, 4, ... then p = 1, 3, ...
+# (0-based lower orbital), three parameters per unit: theta_a (the two
+# singles a+_p a_p+1 of both spins), theta_b (the pair double), theta_c (the
+# singles again).
+def tups_units(n, layers):
+    """[(layer, lower orbital p, index of theta_a)] in parameter order."""
+    out, par = [], 0
+    for layer in range(layers):
+        for start in (0, 1):
+            for p in range(start, n - 1, 2):
+                out.append((layer, p, par))
+                par += 3
+    return out
+
+
+def output_slots(n, layers, nmax):
+    """Network output index of every parameter: (layer, pair, kind) on the
+    grid of the largest orbital count, so that a unit on the same pair in
+    the same layer shares its outputs across orbital counts."""
+    slots = np.zeros(3 * len(tups_units(n, layers)), dtype=np.int64)
+    for layer, p, i in tups_units(n, layers):
+        for k in range(3):
+            slots[i + k] = (layer * (nmax - 1) + p) * 3 + k
+    return slots
+
+
+def _pair_single(t):
+    """The two singles of a unit at angle t on the two-electron states of
+    one spatial pair, basis (LL, LH, HL, HH) = (beta, alpha) in the lower (L)
+    or upper (H) orbital.  Each electron's orbital turns by t:
+    |L> -> cos t |L> + sin t |H>, |H> -> cos t |H> - sin t |L> (the rotation
+    rule psi_b <- C psi_b + s S psi_b', psi_b' <- -s S psi_b + C psi_b' with
+    b = H, b' = L and sign s = +1: the two orbitals are adjacent, so no
+    occupied orbital lies between them)."""
+    c, s = np.cos(t), np.sin(t)
+    r = np.array([[c, -s], [s, c]])
+    return np.kron(r, r)
+
+
+def _pair_double(t):
+    """The pair double (both electrons L <-> H): LL -> cos t LL + sin t HH,
+    HH -> cos t HH - sin t LL (sign +1: its sign mask is empty)."""
+    c, s = np.cos(t), np.sin(t)
+    d = np.eye(4)
+    d[0, 0], d[3, 0], d[0, 3], d[3, 3] = c, s, -s, c
+    return d
+
+
+def _unit_state(ta, tb, tc):
+    x = np.array([1.0, 0.0, 0.0, 0.0])
+    return _pair_single(tc) @ _pair_double(tb) @ _pair_single(ta) @ x
+
+
+def _canonical_first_unit(ta, tb, tc):
+    """A unit acting on its pair's start configuration (both electrons in
+    the lower orbital) reaches the unit sphere of the 3-dimensional singlet
+    space {LL, (LH + HL) / sqrt 2, HH}: three angles for two degrees of
+    freedom, and the state's sign is free (it only flips the total state).
+    Canonical form: theta_c = -theta_a = -t1 with t1 in (-pi/4, pi/4], the
+    sign that makes the LL amplitude positive.  The singles rotate about the
+    axis A = (LL + HH) / sqrt 2 (LL + HH is invariant), so S(-t1) D(t2) S(t1)
+    is a rotation about the axis n = S(-t1) y, with n(t1) = (sin 2t1 / sqrt 2,
+    cos 2t1, -sin 2t1 / sqrt 2); it carries LL to r exactly when n is
+    orthogonal to both A and LL - r."""
+    r = _unit_state(ta, tb, tc)
+    if r[0] < 0 or (abs(r[0]) < 1e-14 and r[1] + r[2] < 0):
+        r = -r
+    r3 = np.array([r[0], (r[1] + r[2]) / np.sqrt(2.0), r[3]])
+    A = np.array([1.0, 0.0, 1.0]) / np.sqrt(2.0)
+    nv = np.cross(A, np.array([1.0, 0.0, 0.0]) - r3)
+    if np.linalg.norm(nv) < 1e-12:
+        nv = np.array([0.0, 1.0, 0.0])
+    nv /= np.linalg.norm(nv)
+    if nv[1] < 0 or (nv[1] == 0 and nv[0] < 0):
+        nv = -nv
+    t1 = 0.5 * np.arctan2(np.sqrt(2.0) * nv[0], nv[1])
+    u = _pair_single(t1) @ np.array([1.0, 0.0, 0.0, 0.0])
+    v = _pair_single(t1) @ r
+    t2 = float(wrap_angles(np.arctan2(v[3], v[0]) - np.arctan2(u[3], u[0])))
+    if np.max(np.abs(_unit_state(t1, t2, -t1) - r)) > 1e-9:
+        return ta, tb, tc  # not reached (kept as a guard)
+    return t1, t2, -t1
+
+
+def canonical_angles(theta, n, layers):
+    """Angles of the same energy in one canonical form per symmetry class:
+    * every unit: (theta_a + pi, theta_c + pi) gives the same operator (the
+      singles at theta + pi differ by (-1)^(electrons of that spin in the
+      pair), which commutes with the whole unit), so theta_a is brought into
+      (-pi/2, pi/2];
+    * units of the first half-layer, acting on the alternating start
+      determinant's pairs (lower orbital occupied in both spins): the form
+      of _canonical_first_unit;
+    * all angles in (-pi, pi]."""
+    t = np.array(theta, dtype=np.float64).copy()
+    for layer, p, i in tups_units(n, layers):
+        ta, tb, tc = t[i], t[i + 1], t[i + 2]
+        if layer == 0 and p % 2 == 0:
+            ta, tb, tc = _canonical_first_unit(ta, tb, tc)
+        k = np.round(ta / np.pi)
+        ta, tc = ta - k * np.pi, tc - k * np.pi
+        if ta <= -np.pi / 2:
+            ta, tc = ta + np.pi, tc + np.pi
+        t[i], t[i + 1], t[i + 2] = ta, tb, tc
+    return wrap_angles(t)
+
+
 def time_to(trace, target):
     """(seconds, Newton iterations) until the trace's energy <= target, or
     (inf, inf) if it never gets there.  Trace rows: seconds, energy, |g|,

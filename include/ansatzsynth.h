@@ -167,6 +167,49 @@ int getKernelTiming(int kernelclass, double *milliseconds, long long *launches, 
 int optimise(int nangles, double *angles, double *energy, double stopenergy, int maxtrace, double *trace,
              int *ntrace, void *ctx);
 
+/* The reference's iterative optimiser (TUPSQuantities::iterativeTups with
+ * runNewtonMethodProjected, src/tupsquantities.cpp:2044-2145, 1238-1465;
+ * the CLI's iterativeoptimise mode, C++-only in the reference): outer
+ * iterations freeze the projection p = H psi(previous angles) and minimise
+ * <p|psi(theta)> by Newton steps (|eigenvalue|^-1 inverse Hessian of the
+ * overlap, cubic-model line search with forward steps), until the projected
+ * gradient's norm is below 1e-5.  ediis != 0 extrapolates the outer
+ * iterates by energy-DIIS over the last 15 (off in the reference).  angles,
+ * energy, trace as optimise (one row per outer iteration).
+ * 1 ctx NULL, 2 pointer NULL, 3 nangles < 1, 5 failure. */
+int iterativeOptimise(int nangles, double *angles, double *energy, int ediis, int maxtrace, double *trace,
+                      int *ntrace, void *ctx);
+
+/* The Hamiltonian in the span of the states of nsets angle vectors
+ * (TUPSQuantities::doSubspaceDiagonalisation, src/tupsquantities.cpp:
+ * 1600-1768; the CLI's subspacediag mode): H_ij = <psi_i|H|psi_j> and
+ * S_ij = <psi_i|psi_j>, Loewdin-symmetrised S^-1/2 H S^-1/2 with the
+ * eigenvalues of S below 1e-14 dropped (those directions give zero
+ * eigenvalues).  angles: nsets rows of nangles (per operator or per
+ * parameter).  eigenvalues: nsets, ascending; hmat, smat (nsets x nsets)
+ * and nullrank (the dropped directions) may be NULL.
+ * 1 ctx NULL, 2 angles or eigenvalues NULL, 3 nsets or nangles < 1,
+ * 5 failure. */
+int subspaceDiagonalise(int nsets, int nangles, const double *angles, double *eigenvalues, double *hmat,
+                        double *smat, int *nullrank, void *ctx);
+
+/* Properties of the states of nsets angle vectors
+ * (TUPSQuantities::writeProperties, src/tupsquantities.cpp:185-568; the
+ * CLI's writeproperties mode).  props: nsets rows of 12 + 2 noverlaps
+ * values: 0 energy (electronic), 1 energy + nuclear energy, 2..4 Hessian
+ * eigenvalues below -1e-10 / within 1e-10 of zero / above 1e-10, 5..7 the
+ * same for the Hessian's diagonal, 8..9 eigenvalues of the metric
+ * g_mu,nu = <dpsi/dtheta_mu|dpsi/dtheta_nu> above 1e-10 / within 1e-10,
+ * 10 metric diagonal entries within 1e-10 of zero, 11 |gradient|, then per
+ * eigenvector t of H (lowest first) the overlap <v_t|psi> and its magnitude.
+ * nohess != 0 skips the Hessian and metric (columns 2..10 stay 0).
+ * hessians, metrics (nsets x P x P, may be NULL): the matrices; eigenvalues
+ * (noverlaps): the lowest eigenvalues of H (Lanczos with the Hamiltonian on
+ * the device).  1 ctx NULL, 2 pointer NULL, 3 nsets or nangles < 1 or
+ * noverlaps outside 0..16, 5 failure. */
+int getProperties(int nsets, int nangles, const double *angles, int noverlaps, int nohess, double *props,
+                  double *hessians, double *metrics, double *eigenvalues, void *ctx);
+
 /* Gradient per operator (the decompressed form of the C++/Python
  * stateAnsatzManager::getGradient, src/AnsatzManager.h:221): dE/dx_k for
  * the angle x_k of operator k; gradient receives one value per operator
@@ -186,6 +229,16 @@ int getGradientOperators(int nangles, const double *angles, double *gradient, vo
  * -6 nangles != number of parameters. */
 int getOverlapGradient(int nangles, const double *angles, int nbasisvectors, const double *projvector,
                        double *overlap, double *gradient, void *ctx);
+
+/* Hessian of the overlap f = <proj|psi(angles)>: hessian[nangles*j+i] =
+ * d2f/dtheta_i dtheta_j (column-major, as getHessian_COMP), its gradient and
+ * f itself (FusedEvolve::evolveHessianProj, src/fusedevolve.cpp:2892-3183;
+ * the Hessian of the reference's projected Newton step,
+ * src/tupsquantities.cpp:1238-1465).  projvector as getOverlapGradient;
+ * gradient and overlap may be NULL.  Codes as getOverlapGradient, with
+ * 3 = hessian NULL. */
+int getHessianProj(int nangles, const double *angles, int nbasisvectors, const double *projvector,
+                   double *hessian, double *gradient, double *overlap, void *ctx);
 
 /* Sector dimension, number of operators and parameters after construction
  * (constructs if needed).  Any pointer may be NULL.  1 ctx NULL, 5 failure. */

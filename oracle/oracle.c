@@ -545,3 +545,80 @@ int or_hessian(const or_engine *e, const double *angles, double *hess, double *g
     free(Ts); free(psi); free(TC); free(HTC); free(THT); free(Hunc); free(X); free(lam);
     return rc;
 }
+
+/* FusedEvolve::evolveDerivativeProj (src/fusedevolve.cpp:2814-2889): the
+ * reverse sweep of (psi, proj) with negated angles; brackets without the
+ * factor 2; *ov = proj . psi. */
+int or_overlap_gradient(const or_engine *e, const double *angles, const double *proj, double *ov, double *grad_comp)
+{
+    long N = e->N;
+    double *phi = malloc(sizeof(double) * N), *lam = malloc(sizeof(double) * N);
+    double *deriv = calloc(e->nops, sizeof(double));
+    int rc = or_evolve(e, angles, phi);
+    if (!rc) {
+        memcpy(lam, proj, sizeof(double) * N);
+        *ov = dotp(lam, phi, N);
+        double *v[2] = {phi, lam};
+        for (int k = e->nops - 1; k >= 0 && !rc; k--)
+            rc = rotate_op(e, k, -angles[k], v, 2, phi, lam, &deriv[k]);
+    }
+    if (!rc) {
+        for (int p = 0; p < e->nparams; p++) grad_comp[p] = 0;
+        for (int k = 0; k < e->nops; k++) grad_comp[e->order[k]] += e->scale[k] * deriv[k];
+    }
+    free(phi); free(lam); free(deriv);
+    return rc;
+}
+
+/* FusedEvolve::evolveHessianProj (src/fusedevolve.cpp:2892-3183): the
+ * tangents of evolveHessian; the gradient T_C . proj and the value
+ * psi . proj (:3023-3037); per-angle reverse sweeps of (T_a, proj) with
+ * brackets for k >= a, factor 1 (:3120-3125); Hessian = C Hunc C^T, no
+ * <dpsi|H|dpsi> term (:3135).  Column-major output. */
+int or_hessian_proj(const or_engine *e, const double *angles, const double *proj, double *hess, double *grad_comp,
+                    double *ov)
+{
+    long N = e->N;
+    int M = e->nops, P = e->nparams;
+    double *Ts = calloc((size_t)M * N, sizeof(double));
+    double *psi = malloc(sizeof(double) * N);
+    memcpy(psi, e->start, sizeof(double) * N);
+    int rc = 0;
+    for (int k = 0; k < M && !rc; k++) {
+        generator_apply(e, k, psi, Ts + (size_t)k * N);
+        double *v[1] = {psi};
+        rc = rotate_op(e, k, angles[k], v, 1, NULL, NULL, NULL);
+    }
+    for (int k = 0; k < M && !rc; k++) {
+        double *v[1] = {Ts + (size_t)k * N};
+        for (int m = k; m < M && !rc; m++) rc = rotate_op(e, m, angles[m], v, 1, NULL, NULL, NULL);
+    }
+    if (rc) { free(Ts); free(psi); return rc; }
+    for (int p = 0; p < P; p++) grad_comp[p] = 0;
+    for (int k = 0; k < M; k++) grad_comp[e->order[k]] += e->scale[k] * dotp(proj, Ts + (size_t)k * N, N);
+    *ov = dotp(proj, psi, N);
+    double *Hunc = calloc((size_t)M * M, sizeof(double));
+    double *X = malloc(sizeof(double) * N), *lam = malloc(sizeof(double) * N);
+    for (int a = 0; a < M && !rc; a++) {
+        memcpy(X, Ts + (size_t)a * N, sizeof(double) * N);
+        memcpy(lam, proj, sizeof(double) * N);
+        double *v[2] = {X, lam};
+        for (int k = M - 1; k >= a && !rc; k--) {
+            double br = 0;
+            rc = rotate_op(e, k, -angles[k], v, 2, X, lam, &br);
+            Hunc[(size_t)a * M + k] += br;
+            if (k != a) Hunc[(size_t)k * M + a] += br;
+        }
+    }
+    if (!rc) {
+        double *Hc = calloc((size_t)P * P, sizeof(double));
+        for (int a = 0; a < M; a++)
+            for (int k = 0; k < M; k++)
+                Hc[(size_t)e->order[a] * P + e->order[k]] += e->scale[a] * e->scale[k] * Hunc[(size_t)a * M + k];
+        for (int i = 0; i < P; i++)
+            for (int j = 0; j < P; j++) hess[(size_t)P * j + i] = Hc[(size_t)i * P + j];
+        free(Hc);
+    }
+    free(Ts); free(psi); free(Hunc); free(X); free(lam);
+    return rc;
+}

@@ -64,6 +64,11 @@ int or_evolve(const or_engine *e, const double *angles, double *psi);
 int or_energy(const or_engine *e, const double *angles, double *E);
 int or_energy_gradient(const or_engine *e, const double *angles, double *E, double *grad_comp);
 int or_hessian(const or_engine *e, const double *angles, double *hess_colmajor, double *grad_comp, double *E);
+/* overlap f = proj . psi over the sector (proj: N amplitudes), its gradient
+ * and Hessian (column-major) */
+int or_overlap_gradient(const or_engine *e, const double *angles, const double *proj, double *ov, double *grad_comp);
+int or_hessian_proj(const or_engine *e, const double *angles, const double *proj, double *hess_colmajor,
+                    double *grad_comp, double *ov);
 
 /* parameter map D (src/tupsquantities.cpp:643-667): angles_ops = D theta */
 int or_decompress_angles(const or_engine *e, const double *theta, double *angles_ops);

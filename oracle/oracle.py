@@ -48,6 +48,8 @@ def _lib():
     L.or_energy.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.or_energy_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     L.or_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+    L.or_overlap_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+    L.or_hessian_proj.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
     L.or_decompress_angles.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.or_apply_exc.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_uint32)]
     return L
@@ -178,6 +180,26 @@ class OracleEngine:
         assert lib().or_hessian(self.h, ap, H.ctypes.data_as(_dp), g.ctypes.data_as(_dp), ctypes.byref(E)) == 0
         return H.reshape(self.P, self.P).T.copy(), g, E.value
 
+    def overlap_gradient(self, angles, proj):
+        """(proj . psi, its per-parameter gradient); proj over the sector."""
+        a, ap = _d(self._ops_angles(angles))
+        pr, pp = _d(proj)
+        ov = ctypes.c_double()
+        g = np.zeros(self.P)
+        assert lib().or_overlap_gradient(self.h, ap, pp, ctypes.byref(ov), g.ctypes.data_as(_dp)) == 0
+        return ov.value, g
+
+    def hessian_proj(self, angles, proj):
+        """(Hessian, gradient, value) of proj . psi; proj over the sector."""
+        a, ap = _d(self._ops_angles(angles))
+        pr, pp = _d(proj)
+        ov = ctypes.c_double()
+        g = np.zeros(self.P)
+        H = np.zeros(self.P * self.P)
+        assert lib().or_hessian_proj(self.h, ap, pp, H.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                                     ctypes.byref(ov)) == 0
+        return H.reshape(self.P, self.P).T.copy(), g, ov.value
+
 
 # ---------------------------------------------------------------- reference
 def ref_available():
@@ -203,6 +225,9 @@ def _reflib():
     L.refd_fe_dim.restype = ctypes.c_long
     L.refd_fe_dim.argtypes = [ctypes.c_void_p]
     L.refd_num_threads.restype = ctypes.c_int
+    L.refd_fe_hessian_proj.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]
+    L.refd_fe_iterative.restype = ctypes.c_double
+    L.refd_fe_iterative.argtypes = [ctypes.c_void_p, _dp]
     return L
 
 
@@ -285,6 +310,7 @@ class RefFusedEvolve:
 
     def __init__(self, nq, nup, ndown, start_idx, start_coeff, ops, order, scale, h, g, workdir=None):
         self.L = _reflib()
+        self.nq = nq
         ops = np.asarray(ops, dtype=np.int32).reshape(-1, 4)
         self.nops = len(ops)
         self.P = int(np.max(order)) + 1
@@ -313,6 +339,25 @@ class RefFusedEvolve:
         g = np.zeros(self.P)
         t = self.L.refd_fe_egrad(self.h, ap, reps, ctypes.byref(E), g.ctypes.data_as(_dp))
         return E.value, g, t
+
+    def hessian_proj(self, angles_ops, proj_full):
+        """FusedEvolve::evolveHessianProj: (Hessian, gradient, value)."""
+        a, ap = _d(angles_ops)
+        pr, pp = _d(proj_full)
+        H = np.zeros(self.P * self.P)
+        g = np.zeros(self.P)
+        ov = ctypes.c_double()
+        rc = self.L.refd_fe_hessian_proj(self.h, ap, pp, self.nq, H.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                                         ctypes.byref(ov))
+        assert rc == 0, rc
+        return H.reshape(self.P, self.P).T.copy(), g, ov.value
+
+    def iterative(self, angles_ops):
+        """TUPSQuantities::iterativeTups from per-operator angles: (final
+        per-operator angles, energy)."""
+        a = np.array(angles_ops, dtype=np.float64)
+        E = self.L.refd_fe_iterative(self.h, a.ctypes.data_as(_dp))
+        return a, E
 
     def close(self):
         if self.h:

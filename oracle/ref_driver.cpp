@@ -174,6 +174,42 @@ double refd_fe_egrad(void* h, const double* angles_ops, int reps, double* E, dou
     return std::chrono::duration<double>(t1 - t0).count() / reps;
 }
 
+// FusedEvolve::evolveHessianProj at per-operator angles (length nops), proj
+// over the full space of 2^nq states; column-major P x P Hessian.
+int refd_fe_hessian_proj(void* h, const double* angles_ops, const double* proj_full, int nq, double* hess,
+                         double* grad_comp, double* ov)
+{
+    auto* H = static_cast<FEHandle*>(h);
+    std::vector<realNumType> ang(angles_ops, angles_ops + H->nops);
+    vector<numType> full;
+    full.resize(1ul << nq, false, nullptr);
+    for (long i = 0; i < (1l << nq); i++) full[i] = proj_full[i];
+    vector<numType> proj;
+    compressor::compressVector<numType>(full, proj, H->comp);
+    Eigen::MatrixXd hm;
+    vector<realNumType> g;
+    realNumType e = 0;
+    H->fe->evolveHessianProj(hm, g, ang, proj, nullptr, &e);
+    if (hm.rows() != H->nparams || hm.cols() != H->nparams) return 2;
+    for (int i = 0; i < H->nparams; i++)
+        for (int j = 0; j < H->nparams; j++) hess[H->nparams * j + i] = hm(i, j);
+    for (int i = 0; i < H->nparams; i++) grad_comp[i] = g[i];
+    *ov = e;
+    return 0;
+}
+
+// TUPSQuantities::iterativeTups on the FusedEvolve route from per-operator
+// angles (updated in place); returns its final energy.
+double refd_fe_iterative(void* h, double* angles_ops)
+{
+    auto* H = static_cast<FEHandle*>(h);
+    std::vector<baseAnsatz::rotationElement> rp(H->nops);
+    for (int i = 0; i < H->nops; i++) rp[i] = {0, angles_ops[i]};
+    const double E = H->tq->iterativeTups(*H->fe, rp, true);
+    for (int i = 0; i < H->nops; i++) angles_ops[i] = rp[i].second;
+    return E;
+}
+
 long refd_fe_dim(void* h) { return (long)static_cast<FEHandle*>(h)->comp->getCompressedSize(); }
 
 int refd_num_threads(void) { return (int)NUM_CORES; }

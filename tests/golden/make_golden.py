@@ -7,6 +7,8 @@ integral set is stored with a checksum so a changed generator is caught.
 Run in the build container:  python tests/golden/make_golden.py [--big]
     python tests/golden/make_golden.py --cfg4   (adds only the 14-orbital
     workload, 11,778,624 determinants, to the existing file; tens of minutes)
+    python tests/golden/make_golden.py --proj   (adds the projected Hessian
+    and iterative-optimiser cases)
 """
 import hashlib
 import json
@@ -22,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 from ansatzevolve_amd.workloads import (hf_index, make_workload, seeded_angles, synthetic_integrals,  # noqa: E402
                                         tups_operators)
-from oracle.oracle import RefEngine, RefFusedEvolve  # noqa: E402
+from oracle.oracle import OracleEngine, RefEngine, RefFusedEvolve  # noqa: E402
 
 OUT = os.path.join(HERE, "reference_vectors.npz")
 
@@ -83,9 +85,51 @@ def cfg4_only():
     print("wrote", OUT, file=sys.stderr)
 
 
+def proj_cases():
+    """The reference build's FusedEvolve::evolveHessianProj (random
+    projection vectors over the sector) and TUPSQuantities::iterativeTups
+    (from small seeded angles) on integral workloads, merged into OUT."""
+    old = np.load(OUT, allow_pickle=False)
+    data = {key: old[key] for key in old.files}
+    meta = json.loads(str(old["meta_json"]))
+    for name, n, layers, seed in (("projn4L2", 4, 2, 2101), ("projn6L2", 6, 2, 2102)):
+        h, g = synthetic_integrals(n, seed)
+        ops, order, scale = tups_operators(n, layers)
+        P = int(order.max()) + 1
+        fe = RefFusedEvolve(2 * n, n // 2, n // 2, [hf_index(n)], [1.0], ops, order, scale, h, g)
+        o = OracleEngine(2 * n, [hf_index(n)], [1.0], ops, order, scale)
+        st = o.states()
+        rng = np.random.default_rng(seed + 5)
+        thetas = seeded_angles(P, seed + 7, count=2)
+        projs = rng.standard_normal((2, len(st)))
+        H, G, V = [], [], []
+        for th, pr in zip(thetas, projs):
+            full = np.zeros(1 << (2 * n))
+            full[st] = pr
+            hh, gg, vv = fe.hessian_proj(scale * th[order], full)
+            H.append(hh)
+            G.append(gg)
+            V.append(vv)
+        data[name + "_theta"], data[name + "_proj"] = thetas, projs
+        data[name + "_hess"], data[name + "_grad"], data[name + "_ov"] = np.array(H), np.array(G), np.array(V)
+        # the iterative optimiser from small angles
+        th0 = 0.3 * seeded_angles(P, seed + 1)
+        x, E = fe.iterative(scale * th0[order])
+        data[name + "_iter_start"], data[name + "_iter_angles"], data[name + "_iter_E"] = th0, x, np.array([E])
+        meta[name] = {"n": n, "layers": layers, "seed": seed, "digest": digest(h, g)}
+        fe.close()
+        print(name, "iterative E %.15f" % E, file=sys.stderr, flush=True)
+    data["meta_json"] = np.array(json.dumps(meta))
+    np.savez_compressed(OUT, **data)
+    print("wrote", OUT, file=sys.stderr)
+
+
 def main():
     if "--cfg4" in sys.argv:
         cfg4_only()
+        return
+    if "--proj" in sys.argv:
+        proj_cases()
         return
     big = "--big" in sys.argv
     data = {}
@@ -149,10 +193,10 @@ def main():
         # keep previously generated large cases
         old = np.load(OUT, allow_pickle=False)
         for key in old.files:
-            if key.startswith(("cfg3", "cfg4")) and key not in data:
+            if key.startswith(("cfg3", "cfg4", "proj")) and key not in data:
                 data[key] = old[key]
         om = json.loads(str(old["meta_json"]))
-        for c in ("cfg3", "cfg4"):
+        for c in [k for k in om if k.startswith(("cfg3", "cfg4", "proj"))]:
             if c in om and c not in meta:
                 meta[c] = om[c]
         data["meta_json"] = np.array(json.dumps(meta))

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from ansatzevolve_amd import AnsatzError, from_workload
+from ansatzevolve_amd.learned import canonical_angles
 from ansatzevolve_amd.workloads import hf_index, seeded_angles, synthetic_integrals, tups_operators
 
 pytestmark = pytest.mark.gpu
@@ -55,3 +56,79 @@ def test_overlap_gradient_errors(gpu):
         with pytest.raises(AnsatzError) as e:
             a.overlap_gradient(np.zeros(P), np.zeros(7))
         assert e.value.code == 5
+
+
+def _golden():
+    import os
+    gz = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"),
+                 allow_pickle=False)
+    return gz
+
+
+@pytest.mark.parametrize("name,n,layers,seed", [("projn4L2", 4, 2, 2101), ("projn6L2", 6, 2, 2102)])
+def test_hessian_proj_against_reference_build(gpu, name, n, layers, seed):
+    """getHessianProj (the reference's evolveHessianProj) against the
+    reference build's golden vectors: Hessian <= 1e-8, gradient <= 1e-9,
+    overlap <= 1e-10; the gradient also equals getOverlapGradient's, and the
+    Hessian the oracle's on a projection vector nonzero outside the sector."""
+    from oracle.oracle import OracleEngine
+    gz = _golden()
+    if name + "_hess" not in gz.files:
+        pytest.skip("no projected-Hessian golden vectors")
+    w = _case(n, layers, seed)
+    with from_workload(w) as a:
+        states = None
+        for k in range(len(gz[name + "_theta"])):
+            th, pr = gz[name + "_theta"][k], gz[name + "_proj"][k]
+            o = OracleEngine(w["nq"], [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
+            states = o.states()
+            full = np.zeros(1 << w["nq"])
+            full[states] = pr
+            H, g, ov = a.hessian_proj(th, full)
+            assert np.max(np.abs(H - gz[name + "_hess"][k])) <= 1e-8
+            assert np.max(np.abs(g - gz[name + "_grad"][k])) <= 1e-9
+            assert abs(ov - gz[name + "_ov"][k]) <= 1e-10
+            ov2, g2 = a.overlap_gradient(th, full)
+            assert abs(ov2 - ov) <= 1e-12 and np.max(np.abs(g2 - g)) <= 1e-12
+        rng = np.random.default_rng(seed)
+        full = rng.standard_normal(1 << w["nq"])
+        th = gz[name + "_theta"][0] + 0.1
+        o.set_integrals(w["h"], w["g"], explicit=True)
+        Ho, go, ovo = o.hessian_proj(th, full[states])
+        H, g, ov = a.hessian_proj(th, full)
+        assert np.max(np.abs(H - Ho)) <= 1e-10 and np.max(np.abs(g - go)) <= 1e-11 and abs(ov - ovo) <= 1e-12
+
+
+@pytest.mark.parametrize("name,n,layers,seed", [("projn4L2", 4, 2, 2101), ("projn6L2", 6, 2, 2102)])
+def test_iterative_optimise_against_reference_build(gpu, name, n, layers, seed):
+    """iterativeOptimise (the reference's iterativeTups with
+    runNewtonMethodProjected) from the same start as the reference build's
+    run ends at its energy and angles; the end point is stationary."""
+    gz = _golden()
+    if name + "_iter_E" not in gz.files:
+        pytest.skip("no iterative-optimiser golden vectors")
+    w = _case(n, layers, seed)
+    with from_workload(w) as a:
+        x0 = w["scale"] * gz[name + "_iter_start"][w["order"]]
+        x, E, trace = a.iterative_optimise(x0)
+        assert abs(E - gz[name + "_iter_E"][0]) <= 1e-9, (E, gz[name + "_iter_E"][0])
+        # the first half-layer's units can move along an energy-flat gauge
+        # direction (three angles for a point on a 2-sphere), so the runs are
+        # compared by state and by canonical angles
+        xr = gz[name + "_iter_angles"]
+        s1, s2 = a.final_state(x, w["nq"]), a.final_state(xr, w["nq"])
+        assert min(np.max(np.abs(s1 - s2)), np.max(np.abs(s1 + s2))) <= 1e-6
+        first = np.unique(w["order"], return_index=True)[1]
+        c1 = canonical_angles(x[first] / w["scale"][first], n, layers)
+        c2 = canonical_angles(xr[first] / w["scale"][first], n, layers)
+        assert np.max(np.abs(np.angle(np.exp(1j * (c1 - c2))))) <= 1e-5
+        assert trace[-1, 2] < 1e-5 and len(trace) >= 1
+        E2, g2 = a.energy_gradient(x[first] / w["scale"][first])
+        assert abs(E2 - E) <= 1e-12 and np.linalg.norm(g2) < 1e-4
+        # with energy-DIIS extrapolation the iteration still ends at a
+        # stationary point of E; per-parameter angles come back per parameter
+        th0 = gz[name + "_iter_start"]
+        thd, Ed, trd = a.iterative_optimise(th0, ediis=True)
+        assert len(thd) == len(th0)
+        E3, g3 = a.energy_gradient(thd)
+        assert abs(E3 - Ed) <= 1e-12 and np.linalg.norm(g3) < 1e-4

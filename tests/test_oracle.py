@@ -272,3 +272,54 @@ def test_hams_minima_energies(key):
         E, g = o.energy_gradient(x)
         assert abs(E - E_ref) < 1e-10
         assert np.linalg.norm(g) < 1e-5  # stationary point
+
+
+@pytest.mark.parametrize("name,n,layers,seed", [("projn4L2", 4, 2, 2101), ("projn6L2", 6, 2, 2102)])
+def test_oracle_hessian_proj_against_reference_build(name, n, layers, seed):
+    """The oracle's evolveHessianProj restatement against the reference
+    build's own (golden vectors), and against second differences of
+    <proj|psi(theta)> built from the oracle's evolution alone."""
+    gz = golden()
+    if name + "_hess" not in gz.files:
+        pytest.skip("no projected-Hessian golden vectors")
+    h, g = synthetic_integrals(n, seed)
+    ops, order, scale = tups_operators(n, layers)
+    o = OracleEngine(2 * n, [hf_index(n)], [1.0], ops, order, scale)
+    o.set_integrals(h, g, explicit=True)
+    for k in range(len(gz[name + "_theta"])):
+        th, pr = gz[name + "_theta"][k], gz[name + "_proj"][k]
+        H, gr, ov = o.hessian_proj(th, pr)
+        assert np.max(np.abs(H - gz[name + "_hess"][k])) <= 1e-12
+        assert np.max(np.abs(gr - gz[name + "_grad"][k])) <= 1e-12
+        assert abs(ov - gz[name + "_ov"][k]) <= 1e-12
+        ov2, g2 = o.overlap_gradient(th, pr)
+        assert abs(ov2 - ov) <= 1e-13 and np.max(np.abs(g2 - gr)) <= 1e-13
+    th, pr = gz[name + "_theta"][0], gz[name + "_proj"][0]
+    H, gr, ov = o.hessian_proj(th, pr)
+    f = lambda t: float(pr @ o.evolve(t))  # noqa: E731
+    P, e = len(th), 1e-4
+    I = np.eye(P)
+    for i in range(P):
+        for j in range(i, P):
+            d = (f(th + e * I[i] + e * I[j]) - f(th + e * I[i] - e * I[j]) - f(th - e * I[i] + e * I[j])
+                 + f(th - e * I[i] - e * I[j])) / (4 * e * e)
+            assert abs(d - H[i, j]) <= 1e-6
+
+
+def test_oracle_iterative_fixed_point_is_stationary():
+    """The reference build's iterative optimiser ends where the oracle's
+    energy gradient is small and its energy is the oracle's there: the
+    recorded end points are stationary points of E (the fixed point of
+    minimising <H psi|psi(theta)>)."""
+    gz = golden()
+    for name, n, layers, seed in (("projn4L2", 4, 2, 2101), ("projn6L2", 6, 2, 2102)):
+        if name + "_iter_E" not in gz.files:
+            pytest.skip("no iterative-optimiser golden vectors")
+        h, g = synthetic_integrals(n, seed)
+        ops, order, scale = tups_operators(n, layers)
+        o = OracleEngine(2 * n, [hf_index(n)], [1.0], ops, order, scale)
+        o.set_integrals(h, g, explicit=True)
+        x = gz[name + "_iter_angles"]
+        E, gr = o.energy_gradient(x)
+        assert abs(E - gz[name + "_iter_E"][0]) <= 1e-12
+        assert np.linalg.norm(gr) < 1e-4
