@@ -14,8 +14,8 @@ quasi-Newton method instead:
   at most 1), Armijo constant 1e-4, halving up to 40 times; a failed search
   drops the history and retries along -g;
 * stop when |g|_inf < gtol, when the energy has decreased by less than
-  `stall` over the last 20 accepted iterations, or after `max_evals`
-  energy + gradient evaluations.
+  `stall` over the last `stall_window` accepted iterations, or after
+  `max_evals` energy + gradient evaluations.
 
 One optimisation is a state machine (`Lbfgs`): it proposes a point, takes
 its energy and gradient, proposes the next.  `minimise` drives one instance
@@ -33,9 +33,10 @@ class Lbfgs:
     """One L-BFGS optimisation.  trace rows: (seconds, energy, |g|_inf,
     evaluations) at every accepted iterate, the start included."""
 
-    def __init__(self, x0, memory=20, gtol=1e-7, stall=1e-11, max_evals=4000):
+    def __init__(self, x0, memory=20, gtol=1e-7, stall=1e-11, stall_window=20, max_evals=4000):
         self.x = np.array(x0, dtype=np.float64)
         self.m, self.gtol, self.stall, self.max_evals = memory, gtol, stall, max_evals
+        self.window = stall_window
         self.S, self.Y = [], []
         self.E = self.g = None
         self.evals = 0
@@ -96,7 +97,7 @@ class Lbfgs:
     def _check_done(self):
         if np.max(np.abs(self.g)) < self.gtol:
             self.done, self.reason = True, "gradient"
-        elif len(self.trace) > 20 and self.trace[-21][1] - self.E < self.stall:
+        elif len(self.trace) > self.window and self.trace[-1 - self.window][1] - self.E < self.stall:
             self.done, self.reason = True, "stalled"
         elif self.evals >= self.max_evals:
             self.done, self.reason = True, "evaluations"

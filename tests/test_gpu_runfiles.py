@@ -124,3 +124,48 @@ def test_cli_paths_and_optimise(gpu):
     assert "Benchmark Derivative Time per derivative (us)" in out
     out = _cli("filepath", pre, "onlyevolve")
     assert abs(float(re.search(r"norm (\S+)", out).group(1)) - 1.0) <= 1e-12
+
+
+def test_cli_iterative_subspace_properties_extracted(gpu, tmp_path):
+    """The command-line modes iterativeoptimise, subspacediag,
+    writeproperties and loadExtractedMin against the library's entry
+    points on the same run files."""
+    import shutil
+    pre = PREFIX["H6"]
+    E_rec, P = read_parameter_sets(pre + "_Parameters.dat")
+    with AnsatzSynth() as a:
+        a.load_run_files(pre)
+        npar = a.dimensions()[2]
+        paths = [np.zeros(npar)] + [p[:npar] for p in P]
+        out = _cli("filepath", pre, "iterativeoptimise")
+        E_it = float(re.search(r"iterative energy (\S+)", out).group(1))
+        th = np.array([float(x) for x in re.search(r"angles (.*)", out).group(1).split()])
+        assert abs(a.energy(th) - E_it) <= 1e-12
+        assert np.linalg.norm(a.gradient(th)) < 1e-4
+        out = _cli("filepath", pre, "subspacediag")
+        m = re.search(r"null space rank (\d+), eigenvalues (.*)", out)
+        ev_cli = np.array([float(x) for x in m.group(2).split()])
+        ev, _, _, _ = a.subspace_diagonalise(np.array(paths[1:10]))
+        assert np.max(np.abs(ev_cli - ev)) <= 1e-12
+        out = _cli("filepath", pre, "writeproperties", "NOverlap", "2")
+        rows, _, _, low = a.properties(np.array(paths), noverlaps=2)
+        line = [ln for ln in out.splitlines() if ln.startswith("Elec. Energy")][0]
+        assert np.max(np.abs(np.array([float(x) for x in line.split()[2:]]) - [r["energy"] for r in rows])) <= 1e-12
+        line = [ln for ln in out.splitlines() if ln.startswith("NegativeHessianEValues")][0]
+        assert [int(x) for x in line.split()[1:]] == [r["hess_neg"] for r in rows]
+        low_cli = [float(x) for x in re.search(r"lowest eigenvalues of H: (.*)", out).group(1).split()]
+        assert np.max(np.abs(np.array(low_cli) - low)) <= 1e-12
+        # parameter sets from an _extractedmin file (one value per parameter)
+        dst = str(tmp_path / "X")
+        for suffix in ("_Initial.dat", "_Operators.dat", "_Order.dat", "_Nuclear_Energy.dat", "_oneEInts.bin",
+                       "_twoEInts.bin"):
+            shutil.copy(pre + suffix, dst + suffix)
+        sets = np.stack([paths[1], 0.5 * paths[1]])
+        with open(dst + "_extractedmin", "w") as f:
+            for s in sets:
+                f.write(" ".join("%.17g" % x for x in s) + "\n")
+        out = _cli("filepath", dst, "loadExtractedMin")
+        got = dict((int(k), float(E)) for k, E in re.findall(r"path (\d+): energy (\S+)", out))
+        assert len(got) == 3
+        for k in (1, 2):
+            assert abs(got[k] - a.energy(sets[k - 1])) <= 1e-12
