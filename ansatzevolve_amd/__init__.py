@@ -52,6 +52,7 @@ C_API = [
     ("optimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _vp]),
     ("iterativeOptimise", ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _ip, _vp]),
     ("subspaceDiagonalise", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _vp]),
+    ("getEnergyGradientNoAlloc", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _vp]),
     ("getProperties", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                       _dp, _vp]),
     ("multiBatchCreate", _vp, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),

@@ -881,3 +881,46 @@ extern "C" int getProperties(int nsets, int nangles, const double *angles, int n
     if (noverlaps > 0) std::memcpy(eigenvalues, low.data(), sizeof(double) * noverlaps);
     return 0;
 }
+
+// One process-wide lock: every no-allocation call on a GPU shares that GPU's
+// scratch array, so calls from this process run one at a time.
+static std::mutex noAllocLock;
+
+extern "C" int getEnergyGradientNoAlloc(int nangles, const double *angles, double *energy, double *gradient,
+                                        void *ctx)
+{
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("the angle array is null");
+        return 2;
+    }
+    if (gradient == nullptr) {
+        log_msg("the gradient output array is null");
+        return 3;
+    }
+    if (nangles < 1) {
+        log_msg("the angle count is below 1");
+        return 4;
+    }
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (!m->built()) {
+        log_msg("getEnergyGradientNoAlloc: the context is not built yet (an ordinary evaluation builds it)");
+        return 6;
+    }
+    if (nangles != m->numberOfParameters()) {
+        log_msg("gradient length does not match the parameter count");
+        return -6;
+    }
+    std::lock_guard<std::mutex> glock(noAllocLock);
+    double E = 0;
+    const int rc = m->getEnergyGradientNoAlloc(angles, nangles, E, gradient);
+    if (rc == 1) return 6;
+    if (rc == 2) return 7;
+    if (rc != 0) return 5;
+    if (energy) *energy = E;
+    return 0;
+}

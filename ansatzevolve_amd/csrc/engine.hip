@@ -15,12 +15,20 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 
 #include "engine.hpp"
 #include "kernels.hpp"
 
 namespace ae {
+
+// Scratch of the no-allocation evaluation (Engine::energy_gradient_noalloc):
+// one fixed device array; every GPU the library runs on holds its own copy
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns per device.
+constexpr size_t kNoAllocScratchDoubles = (size_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 #define HCHK(expr)                                                                     \
     do {                                                                               \
@@ -2278,6 +2286,99 @@ bool Engine::download_ham_state(double *out, std::string &err)
     if (!e.ham_apply(e.d_psi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
     HCHK(hipMemcpyAsync(out, e.d_lam.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
+    return true;
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the symbol's host-side name is no device address).
+static bool noalloc_scratch(double **out, std::string &err)
+{
+    static std::mutex mu;
+    static std::map<int, double *> ptrs;
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it == ptrs.end()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return false;
+        }
+        it = ptrs.emplace(dev, static_cast<double *>(p)).first;
+    }
+    *out = it->second;
+    return true;
+}
+
+size_t Engine::noalloc_scratch_doubles() { return kNoAllocScratchDoubles; }
+
+size_t Engine::noalloc_scratch_need() const
+{
+    const EngineImpl &e = *d_;
+    const size_t M = e.M;
+    auto r = [](size_t n) { return (n + 31) & ~(size_t)31; };  // 256-byte aligned parts
+    return r(2 * M) + r(2 * M) + r(M + 1) + r(e.eslots_alloc()) + r(M * (size_t)e.slots);
+}
+
+// energy_gradient without graph, rocBLAS or device allocation: the angle
+// tables, energy partials, bracket partials and results live in `scratch`
+// (layout: cos/sin 2M | cos/-sin 2M | results M+1 | energy partials |
+// bracket partials M x slots), the states in the engine's own buffers.
+// The sigma is the explicit rows or the all-in-one matrix-free kernel.
+bool Engine::energy_gradient_noalloc(const double *angles, double *E, double *deriv, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.haveStart || e.M == 0 || e.hkind == 0) {
+        err = "engine not set up";
+        return false;
+    }
+    const size_t need = noalloc_scratch_need();
+    if (need > kNoAllocScratchDoubles) {
+        err = "this set-up needs " + std::to_string(need) + " doubles of scratch; the no-allocation entry point has " +
+              std::to_string(kNoAllocScratchDoubles);
+        return false;
+    }
+    double *base = nullptr;
+    if (!noalloc_scratch(&base, err)) return false;
+    const size_t M = e.M;
+    auto r = [](size_t n) { return (n + 31) & ~(size_t)31; };
+    double2 *cs = reinterpret_cast<double2 *>(base);
+    double2 *csneg = reinterpret_cast<double2 *>(base + r(2 * M));
+    double *out = base + 2 * r(2 * M);
+    double *epart = out + r(M + 1);
+    double *partial = epart + r(e.eslots_alloc());
+    e.fill_cs(angles, 1);
+    e.h_csneg.resize(M);
+    for (size_t k = 0; k < M; k++) e.h_csneg[k] = make_double2(e.h_cs[k].x, -e.h_cs[k].y);
+    HCHK(hipMemcpyAsync(cs, e.h_cs.data(), M * sizeof(double2), hipMemcpyHostToDevice, e.stream));
+    HCHK(hipMemcpyAsync(csneg, e.h_csneg.data(), M * sizeof(double2), hipMemcpyHostToDevice, e.stream));
+    const unsigned cgrid = (unsigned)std::min<long long>((e.N + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_start.p, e.d_psi.p, e.N, 1);
+    LCHK();
+    if (!e.sweep_forward(e.d_psi.p, 1, cs, err)) return false;
+    hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_psi.p, e.d_phi.p, e.N, 1);
+    LCHK();
+    if (e.hkind == 3) {
+        hipLaunchKernelGGL(k_mf_sigma, dim3(e.na, 1), dim3(kBlock), e.mf_lds(), e.stream, e.mfargs(), e.d_phi.p,
+                           e.d_lam.p, e.N, epart);
+        LCHK();
+    } else if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, epart, err)) {
+        return false;
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, epart, 1, e.eslots(), 0LL, 1LL, out + M, 1.0);
+    LCHK();
+    // sweep_reverse clears the bracket partials before it accumulates
+    if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, csneg, partial, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)M), dim3(kBlock), 0, e.stream, partial, (int)M, e.slots,
+                       (long long)e.slots, 1LL, out, 2.0);
+    LCHK();
+    std::vector<double> res(M + 1);
+    HCHK(hipMemcpyAsync(res.data(), out, 8 * (M + 1), hipMemcpyDeviceToHost, e.stream));
+    HCHK(hipStreamSynchronize(e.stream));
+    std::memcpy(deriv, res.data(), 8 * M);
+    if (E) *E = res[M];
     return true;
 }
 

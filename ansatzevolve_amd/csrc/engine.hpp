@@ -96,6 +96,14 @@ public:
     // psi = U(angles) psi_0, then E and the adjoint sweep, as one device
     // sequence (replayed from a hipGraph after the first call).
     bool energy_gradient(const double *angles, double *E, double *deriv, std::string &err);
+    // The same without allocating: no graph, no library call, no device
+    // allocation; the angle tables, partial sums and results live in a
+    // fixed device array shared by every call on the GPU (calls that
+    // overlap on one GPU race).  Fails when the set-up needs more than
+    // noalloc_scratch_doubles() doubles of it (noalloc_scratch_need()).
+    bool energy_gradient_noalloc(const double *angles, double *E, double *deriv, std::string &err);
+    static size_t noalloc_scratch_doubles();
+    size_t noalloc_scratch_need() const;
     // psi = U(angles) psi_0, overlap = <proj|psi> and, by the adjoint sweep
     // with lambda = proj, deriv[k] = d<proj|psi>/d(angle_k) per operator
     // (FusedEvolve::evolveDerivativeProj, src/fusedevolve.cpp:2814-2889);

@@ -493,6 +493,33 @@ bool Manager::getEnergyGradient(const std::vector<double> &angles, double &E, st
     return true;
 }
 
+int Manager::getEnergyGradientNoAlloc(const double *angles, int nangles, double &E, double *grad)
+{
+    if (!eng_) {
+        log_msg("No-allocation evaluation refused: the context is not built yet (an ordinary evaluation builds it)");
+        return 1;
+    }
+    if (eng_->noalloc_scratch_need() > Engine::noalloc_scratch_doubles()) {
+        log_msg("No-allocation evaluation refused: the set-up needs %zu doubles of scratch, at most %zu are there",
+                eng_->noalloc_scratch_need(), Engine::noalloc_scratch_doubles());
+        return 2;
+    }
+    // per-operator angles into the reused member (its capacity stays after the first call)
+    nostage_.resize(ops_.size());
+    if (!toOperatorAngles(angles, (size_t)nangles, nostage_.data())) return 3;
+    noderiv_.resize(ops_.size());
+    std::string err;
+    if (!eng_->energy_gradient_noalloc(nostage_.data(), &E, noderiv_.data(), err)) {
+        log_msg("No-allocation evaluation failed: %s", err.c_str());
+        cur_.assign(ops_.size(), std::nan(""));
+        return 3;
+    }
+    cur_ = nostage_;
+    E += enuc_;
+    compressGradient(noderiv_.data(), grad);
+    return 0;
+}
+
 bool Manager::getHessianComp(const std::vector<double> &angles, std::vector<double> &hess)
 {
     if (!setAngles(angles)) return false;

@@ -48,6 +48,11 @@ public:
     bool getHessianProj(const std::vector<double> &angles, const std::vector<double> &projFull,
                         std::vector<double> &hessRowMajor, std::vector<double> &grad, double &overlap);
     bool getEnergyGradient(const std::vector<double> &angles, double &E, std::vector<double> &grad);
+    // getEnergyGradient without any allocation (Engine::energy_gradient_noalloc).
+    // 0 done, 1 not built yet (building allocates), 2 the set-up needs more
+    // scratch than the fixed array holds, 3 other failure.
+    int getEnergyGradientNoAlloc(const double *angles, int nangles, double &E, double *grad);
+    bool built() const { return eng_ != nullptr; }
     bool getExpectationValues(int nvec, int nangles, const double *angles, std::vector<double> &E);
     bool getEnergyGradients(int nvec, int nangles, const double *angles, std::vector<double> &E,
                             std::vector<double> &grads);
@@ -141,6 +146,7 @@ private:
     std::unique_ptr<Engine> eng_;
     std::vector<double> cur_;  // per-operator angles of the engine's current psi
     int batchNangles_ = 0;     // angle count of the uploaded batch
+    std::vector<double> nostage_, noderiv_;  // host staging of the no-allocation evaluation
 };
 
 } // namespace ae

@@ -118,6 +118,22 @@ int setHamiltonianStorage(int mode, void *ctx);
  * 2263-2346).  Codes as getGradient_COMP; energy may be NULL. */
 int getEnergyGradient(int nangles, const double *angles, double *energy, double *gradient, void *ctx);
 
+/* getEnergyGradient for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The call makes no device allocation, no
+ * graph and no library call: the angle tables, the partial sums and the
+ * results live in one fixed device array of 32 MiB (4 Mi doubles) that each
+ * GPU holds until the process ends; the states use the context's own
+ * buffers, made when the context was built, so the context must have been
+ * evaluated once by an ordinary call.  Every call on a GPU shares that
+ * GPU's array: calls from this process are serialised, calls that overlap
+ * on one GPU from elsewhere race.  The Hamiltonian is applied by the
+ * explicit rows or the all-in-one matrix-free kernel.  angles per
+ * parameter.  Codes as getGradient_COMP, and 6 the context is not built
+ * yet, 7 the set-up needs more than the array holds (parameter count times
+ * the sweep's partial rows, about 455 x 4096 at most). */
+int getEnergyGradientNoAlloc(int nangles, const double *angles, double *energy, double *gradient, void *ctx);
+
 /* Batched energies for nvec angle vectors (row-major nvec x nangles), as the
  * reference's stateAnsatzManager::getExpectationValues
  * (src/AnsatzManager.cpp:709-754).  1 ctx NULL, 2 angles/energies NULL,
