@@ -355,12 +355,13 @@ def stage_train(counts, out_dir, gen_evals=3000, epochs=400, energy_steps=0, log
     return info
 
 
-def stage_evaluate(counts, out_dir, max_evals=6000, log=_log):
+def stage_evaluate(counts, out_dir, max_evals=6000, log=_log, model_dir=None):
     """Held-out comparison: proposals, fallback rule, then batched L-BFGS from
     the chosen starts and from theta = 0 together (one multi-instance launch
     per round over both); writes eval.json into out_dir."""
     train, held = instances(counts)
-    model = load_model(os.path.join(out_dir, "model.npz"))
+    model_dir = model_dir or out_dir
+    model = load_model(os.path.join(model_dir, "model.npz"))
     starts, E_std, E_prop, used, judge_s = propose(model, held)
     model.close()
     t_eval = egrad_seconds(sorted(counts))
@@ -372,7 +373,7 @@ def stage_evaluate(counts, out_dir, max_evals=6000, log=_log):
     batch_s = time.time() - t0
     plain, learned = res[:len(held)], res[len(held):]
     rows = rows_of(held, plain, learned, E_std, E_prop, used, t_eval, judge_s)
-    with open(os.path.join(out_dir, "train.json")) as f:
+    with open(os.path.join(model_dir, "train.json")) as f:
         info = json.load(f)
     summary = dict(info, held_out=len(held), eval_max_evals=max_evals, eval_batch_s=batch_s,
                    egrad_seconds={str(n): t for n, t in t_eval.items()}, judge_s_per_instance=judge_s,
@@ -394,6 +395,7 @@ def _main():
     p.add_argument("--counts", default="8:64:24,10:64:24,12:32:16",
                    help="n:training:held-out per orbital count")
     p.add_argument("--out-dir", default="gpurun_out/warmstart")
+    p.add_argument("--model-dir", default="", help="evaluate: where model.npz and train.json are (default: out-dir)")
     p.add_argument("--gen-evals", type=int, default=3000)
     p.add_argument("--eval-evals", type=int, default=6000)
     p.add_argument("--epochs", type=int, default=400)
@@ -406,7 +408,7 @@ def _main():
         info = stage_train(counts, a.out_dir, a.gen_evals, a.epochs, a.energy_steps)
         print(json.dumps(info))
         return
-    s = stage_evaluate(counts, a.out_dir, a.eval_evals)
+    s = stage_evaluate(counts, a.out_dir, a.eval_evals, model_dir=a.model_dir or None)
     print(json.dumps({k: v for k, v in s.items() if k != "rows"}))
 
 

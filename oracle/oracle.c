@@ -28,6 +28,7 @@ struct or_engine {
     double *scale;
     int nparams;
     double *start;    /* N */
+    double *start_im; /* N, complex mode (NULL: real start) */
     /* Hamiltonian: explicit rows (y[j] = sum val*x[col]) or an operator list */
     int hkind;        /* 0 none, 1 rows, 2 operator list */
     long *rowptr;
@@ -180,6 +181,7 @@ or_engine *or_engine_create(int nq, int nstart, const int *start_idx, const doub
 void or_engine_free(or_engine *e)
 {
     if (!e) return;
+    free(e->start_im);
     free(e->states); free(e->ops); free(e->order); free(e->scale); free(e->start);
     free(e->rowptr); free(e->col); free(e->val);
     free(e->hcre); free(e->hdes); free(e->hmask); free(e->hval);
@@ -620,5 +622,148 @@ int or_hessian_proj(const or_engine *e, const double *angles, const double *proj
         free(Hc);
     }
     free(Ts); free(psi); free(Hunc); free(X); free(lam);
+    return rc;
+}
+
+/* ---- complex mode (the reference's useComplex build) ----
+ * Amplitudes interleaved: entry 2i is Re psi_i, 2i + 1 is Im psi_i, i the
+ * sector index; a doubled basis state is (bits << 1) | c.  The pair rule
+ * (src/fusedevolve.cpp:27-145 with isComplex): excitations map
+ * (b, c) -> (b', c) with the real rule's phase; a number operator
+ * (creation set = annihilation set) acts only where every annihilated
+ * orbital is occupied and the phase (-1)^popc(b & mask) * flips equals +i:
+ * (b, 0) -> (b, 1) and (b, 1) -> (b, 0); any other phase leaves the state
+ * alone.  Number operators are evolved by their own routine
+ * (RunFuseNDiagonal, src/fusedevolve.cpp:627-717) as
+ * Re' = C Re - S Im, Im' = S Re + C Im (z -> e^{+i theta} z), with the
+ * bracket Im(lambda) Re(phi) - Re(lambda) Im(phi): in this file's rotation
+ * rule that is the pair sign -1 from the real entry, +1 from the imaginary
+ * one. */
+int or_apply_exc_complex(uint32_t state2, const int8_t a[4], uint32_t *out)
+{
+    *out = state2;
+    if (a[0] < 0 && a[1] < 0) return 1;
+    const uint32_t c = state2 & 1u, b = state2 >> 1;
+    int phase = 1;
+    uint32_t create, annihilate;
+    if (a[2] > -1 && a[3] > -1) {
+        if (a[0] == a[1] || a[2] == a[3]) return 1;
+        create = (1u << a[0]) | (1u << a[1]);
+        annihilate = (1u << a[2]) | (1u << a[3]);
+        uint32_t mask = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1) ^ ((1u << a[2]) - 1) ^ ((1u << a[3]) - 1);
+        mask &= ~((1u << a[0]) | (1u << a[1]) | (1u << a[2]) | (1u << a[3]));
+        if (popc(b & mask) & 1) phase = -phase;
+        if (a[0] > a[1]) phase = -phase;
+        if (a[2] > a[3]) phase = -phase;
+    } else {
+        create = 1u << a[0];
+        annihilate = 1u << a[1];
+        uint32_t mask = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1);
+        mask &= ~((1u << a[0]) | (1u << a[1]));
+        if (popc(b & mask) & 1) phase = -phase;
+    }
+    const uint32_t m = b & (create | annihilate);
+    if (create == annihilate) {
+        if ((m & annihilate) != annihilate || phase != 1) return -1; /* phase 0 or -i: no partner */
+        *out = (b << 1) | (c ^ 1u);
+        return c ? 1 : -1;
+    }
+    if (((m & annihilate) ^ annihilate) == 0 && ((m ^ annihilate) & create) == 0) {
+        *out = (((b ^ annihilate) ^ create) << 1) | c;
+        return phase;
+    }
+    if (((m & create) ^ create) == 0 && ((m ^ create) & annihilate) == 0) {
+        *out = (((b ^ create) ^ annihilate) << 1) | c;
+        return -phase;
+    }
+    return -1;
+}
+
+/* one operator on interleaved vectors (2N entries), optional brackets as in
+ * rotate_op */
+static int rotate_op_complex(const or_engine *e, int k, double theta, double **vecs, int nvec, const double *phi,
+                             const double *lam, double *bracket)
+{
+    const int8_t *a = e->ops + 4 * k;
+    double S = sin(theta), C = cos(theta), acc = 0;
+    for (long i = 0; i < 2 * e->N; i++) {
+        const uint32_t s2 = (e->states[i >> 1] << 1) | (uint32_t)(i & 1);
+        uint32_t t2;
+        const int s = or_apply_exc_complex(s2, a, &t2);
+        if (t2 == s2 || t2 < s2) continue;
+        const long r = rank_of(e, t2 >> 1);
+        if (r < 0) continue;
+        const long j = 2 * r + (long)(t2 & 1);
+        if (phi) acc += s * (lam[i] * phi[j] - lam[j] * phi[i]);
+        for (int v = 0; v < nvec; v++) {
+            const double x = vecs[v][i], y = vecs[v][j];
+            vecs[v][i] = s * y * S + x * C;
+            vecs[v][j] = -s * x * S + y * C;
+        }
+    }
+    if (bracket) *bracket = acc;
+    return 0;
+}
+
+int or_engine_set_start_complex(or_engine *e, int nstart, const int *idx, const double *re, const double *im)
+{
+    free(e->start_im);
+    e->start_im = calloc(e->N, sizeof(double));
+    memset(e->start, 0, sizeof(double) * e->N);
+    for (int i = 0; i < nstart; i++) {
+        const long r = rank_of(e, (uint32_t)idx[i]);
+        if (r >= 0) {
+            e->start[r] = re[i];
+            e->start_im[r] = im[i];
+        }
+    }
+    return 0;
+}
+
+/* psi interleaved (2N) */
+int or_evolve_complex(const or_engine *e, const double *angles, double *psi)
+{
+    for (long i = 0; i < e->N; i++) {
+        psi[2 * i] = e->start[i];
+        psi[2 * i + 1] = e->start_im ? e->start_im[i] : 0.0;
+    }
+    double *v[1] = {psi};
+    for (int k = 0; k < e->nops; k++) rotate_op_complex(e, k, angles[k], v, 1, NULL, NULL, NULL);
+    return 0;
+}
+
+/* H on an interleaved vector: the real and imaginary parts separately */
+static int ham_apply_complex(const or_engine *e, const double *x, double *y)
+{
+    long N = e->N;
+    double *a = malloc(sizeof(double) * N), *b = malloc(sizeof(double) * N);
+    int rc = 0;
+    for (int part = 0; part < 2 && !rc; part++) {
+        for (long i = 0; i < N; i++) a[i] = x[2 * i + part];
+        rc = or_ham_apply(e, a, b);
+        for (long i = 0; i < N; i++) y[2 * i + part] = b[i];
+    }
+    free(a); free(b);
+    return rc;
+}
+
+/* E = Re <psi|H|psi> and the gradient: the real build's evolveDerivative
+ * on the interleaved vectors (src/fusedevolve.cpp:2263-2346) */
+int or_energy_gradient_complex(const or_engine *e, const double *angles, double *E, double *grad_comp)
+{
+    if (!e->hkind) return 1;
+    long N2 = 2 * e->N;
+    double *phi = malloc(sizeof(double) * N2), *lam = malloc(sizeof(double) * N2);
+    double *deriv = calloc(e->nops, sizeof(double));
+    or_evolve_complex(e, angles, phi);
+    int rc = ham_apply_complex(e, phi, lam);
+    if (!rc) {
+        *E = dotp(lam, phi, N2);
+        double *v[2] = {phi, lam};
+        for (int k = e->nops - 1; k >= 0; k--) rotate_op_complex(e, k, -angles[k], v, 2, phi, lam, &deriv[k]);
+        for (int p = 0; p < e->nparams; p++) grad_comp[p] = 0;
+        for (int k = 0; k < e->nops; k++) grad_comp[e->order[k]] += e->scale[k] * 2 * deriv[k];
+    }
+    free(phi); free(lam); free(deriv);
     return rc;
 }

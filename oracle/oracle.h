@@ -70,6 +70,14 @@ int or_overlap_gradient(const or_engine *e, const double *angles, const double *
 int or_hessian_proj(const or_engine *e, const double *angles, const double *proj, double *hess_colmajor,
                     double *grad_comp, double *ov);
 
+/* Complex mode (the reference's useComplex build): interleaved amplitudes
+ * (2i = Re, 2i + 1 = Im).  or_apply_exc_complex works on doubled states
+ * (bits << 1) | part.  A complex start replaces the real one. */
+int or_apply_exc_complex(uint32_t state2, const int8_t a[4], uint32_t *out);
+int or_engine_set_start_complex(or_engine *e, int nstart, const int *idx, const double *re, const double *im);
+int or_evolve_complex(const or_engine *e, const double *angles, double *psi_interleaved);
+int or_energy_gradient_complex(const or_engine *e, const double *angles, double *E, double *grad_comp);
+
 /* parameter map D (src/tupsquantities.cpp:643-667): angles_ops = D theta */
 int or_decompress_angles(const or_engine *e, const double *theta, double *angles_ops);
 

@@ -49,6 +49,10 @@ def _lib():
     L.or_energy_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     L.or_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
     L.or_overlap_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+    L.or_engine_set_start_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]
+    L.or_evolve_complex.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.or_energy_gradient_complex.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.or_apply_exc_complex.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_uint32)]
     L.or_hessian_proj.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
     L.or_decompress_angles.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.or_apply_exc.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_uint32)]
@@ -179,6 +183,28 @@ class OracleEngine:
         H = np.zeros(self.P * self.P)
         assert lib().or_hessian(self.h, ap, H.ctypes.data_as(_dp), g.ctypes.data_as(_dp), ctypes.byref(E)) == 0
         return H.reshape(self.P, self.P).T.copy(), g, E.value
+
+    def set_start_complex(self, idx0, coeffs):
+        """Complex start state (zero-based full-space indices)."""
+        i, ip = _i(idx0)
+        c = np.asarray(coeffs, dtype=complex)
+        re, rp = _d(c.real.copy())
+        im, imp = _d(c.imag.copy())
+        assert lib().or_engine_set_start_complex(self.h, len(i), ip, rp, imp) == 0
+
+    def evolve_complex(self, angles):
+        """psi(angles) in complex mode, as a complex array over the sector."""
+        a, ap = _d(self._ops_angles(angles))
+        psi = np.zeros(2 * self.N)
+        assert lib().or_evolve_complex(self.h, ap, psi.ctypes.data_as(_dp)) == 0
+        return psi[0::2] + 1j * psi[1::2]
+
+    def energy_gradient_complex(self, angles):
+        a, ap = _d(self._ops_angles(angles))
+        E = ctypes.c_double()
+        g = np.zeros(self.P)
+        assert lib().or_energy_gradient_complex(self.h, ap, ctypes.byref(E), g.ctypes.data_as(_dp)) == 0
+        return E.value, g
 
     def overlap_gradient(self, angles, proj):
         """(proj . psi, its per-parameter gradient); proj over the sector."""
@@ -366,6 +392,73 @@ class RefFusedEvolve:
         if self._tmp:
             self._tmp.cleanup()
             self._tmp = None
+
+    def __del__(self):
+        self.close()
+
+
+# ------------------------------------------------------- reference, complex
+LIBREF_COMPLEX = os.path.join(HERE, "_ref", "libref_ansatz_complex.so")
+
+
+def ref_complex_available():
+    return os.path.exists(LIBREF_COMPLEX)
+
+
+class RefComplex:
+    """The reference's stateAnsatzManager compiled in its complex mode
+    (through oracle/ref_driver_complex.cpp): complex start, triplet
+    Hamiltonian, operators (number operators allowed)."""
+
+    def __init__(self, nq, start_idx, start_coeff, ops, order, scale, ti, tj, tv):
+        L = ctypes.CDLL(LIBREF_COMPLEX)
+        L.refc_create.restype = ctypes.c_void_p
+        L.refc_create.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int, _ip, _ip, _dp,
+                                  ctypes.c_int, _ip, _ip, _dp]
+        L.refc_destroy.argtypes = [ctypes.c_void_p]
+        L.refc_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.refc_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.refc_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_long]
+        self.L, self.nq = L, nq
+        c = np.asarray(start_coeff, dtype=complex)
+        si, sip = _i(start_idx)
+        re, rp = _d(c.real.copy())
+        im, imp = _d(c.imag.copy())
+        ops = np.asarray(ops, dtype=np.int32).reshape(-1, 4)
+        o, op = _i(ops.reshape(-1))
+        od, odp = _i(order)
+        sc, scp = _d(np.ones(len(ops)) if scale is None else scale)
+        a, ap = _i(ti)
+        b, bp = _i(tj)
+        v, vp = _d(tv)
+        self.P = int(np.max(order)) + 1
+        self.h = L.refc_create(nq, len(si), sip, rp, imp, len(ops), op, odp, scp, len(a), ap, bp, vp)
+        if not self.h:
+            raise ValueError("reference rejected the set-up")
+
+    def energy(self, angles):
+        a, ap = _d(angles)
+        E = ctypes.c_double()
+        assert self.L.refc_energy(self.h, len(a), ap, ctypes.byref(E)) == 0
+        return E.value
+
+    def gradient(self, angles):
+        a, ap = _d(angles)
+        g = np.zeros(self.P)
+        assert self.L.refc_gradient(self.h, len(a), ap, g.ctypes.data_as(_dp), self.P) == 0
+        return g
+
+    def state(self, angles):
+        a, ap = _d(angles)
+        n = 1 << self.nq
+        re, im = np.zeros(n), np.zeros(n)
+        assert self.L.refc_state(self.h, len(a), ap, re.ctypes.data_as(_dp), im.ctypes.data_as(_dp), n) == 0
+        return re + 1j * im
+
+    def close(self):
+        if self.h:
+            self.L.refc_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         self.close()

@@ -9,6 +9,9 @@ Run in the build container:  python tests/golden/make_golden.py [--big]
     workload, 11,778,624 determinants, to the existing file; tens of minutes)
     python tests/golden/make_golden.py --proj   (adds the projected Hessian
     and iterative-optimiser cases)
+    python tests/golden/make_golden.py --complex   (adds the complex-mode
+    cases, from the reference built with its complex mode: make -C oracle
+    ref-complex)
 """
 import hashlib
 import json
@@ -124,7 +127,61 @@ def proj_cases():
     print("wrote", OUT, file=sys.stderr)
 
 
+def complex_cases():
+    """The reference build in its complex mode (oracle/_ref/
+    libref_ansatz_complex.so): the test.F90 set-up, the same with number
+    operators appended (one per spin-orbital, two pair doubles, one
+    out-of-order double that never acts) and a complex start, and number
+    operators interleaved between the excitations with shared parameters.
+    Energies, gradients and final states, merged into OUT."""
+    from oracle.oracle import RefComplex
+    old = np.load(OUT, allow_pickle=False)
+    data = {key: old[key] for key in old.files}
+    with open(os.path.join(HERE, "testf90_kat.json")) as f:
+        k = json.load(f)
+    ops1, order1 = np.array(k["operators_1based"]), np.array(k["order_1based"])
+    ti, tj = np.array(k["ham_i_1based"]) - 1, np.array(k["ham_j_1based"]) - 1
+    tv = np.array(k["ham_coeffs"])
+    s0 = k["initial_index_1based"] - 1
+    extra = np.array([[p, p, 0, 0] for p in range(1, 9)] + [[1, 5, 1, 5], [2, 6, 2, 6], [5, 1, 1, 5]])
+    inter_ops, inter_ord = [], []
+    for j, (op, od) in enumerate(zip(ops1, order1)):
+        inter_ops.append(list(op))
+        inter_ord.append(od)
+        if j % 3 == 1:
+            inter_ops.append([1 + (j % 8), 1 + (j % 8), 0, 0])
+            inter_ord.append(od)
+    inter_ops.append([3, 7, 3, 7])
+    inter_ord.append(order1.max() + 1)
+    cases = [("cplx_kat", ops1, order1, [s0], [1.0]),
+             ("cplx_num", np.concatenate([ops1, extra]),
+              np.concatenate([order1, order1.max() + 1 + np.arange(len(extra))]), [s0, 172], [1.0, 0.3j]),
+             ("cplx_mix", np.array(inter_ops), np.array(inter_ord), [s0, 172], [1.0, 0.5 - 0.2j])]
+    for name, o1, od1, idx, co in cases:
+        co = np.array(co, dtype=complex)
+        co = co / np.linalg.norm(co)
+        r = RefComplex(8, idx, co, o1 - 1, od1 - 1, None, ti, tj, tv)
+        P = int(od1.max())
+        th = [np.array(k["angles_hf"] + [0.0] * (P - 9)), np.array(k["angles_min"] + [0.0] * (P - 9))] + \
+             list(seeded_angles(P, 4343, count=2))
+        data[name + "_ops1"], data[name + "_order1"] = o1, od1
+        data[name + "_start_idx0"], data[name + "_start_re"], data[name + "_start_im"] = \
+            np.array(idx), co.real.copy(), co.imag.copy()
+        data[name + "_theta"] = np.array(th)
+        data[name + "_E"] = np.array([r.energy(t) for t in th])
+        data[name + "_grad"] = np.array([r.gradient(t) for t in th])
+        st = np.array([r.state(t) for t in th])
+        data[name + "_state_re"], data[name + "_state_im"] = st.real.copy(), st.imag.copy()
+        r.close()
+        print(name, data[name + "_E"], file=sys.stderr, flush=True)
+    np.savez_compressed(OUT, **data)
+    print("wrote", OUT, file=sys.stderr)
+
+
 def main():
+    if "--complex" in sys.argv:
+        complex_cases()
+        return
     if "--cfg4" in sys.argv:
         cfg4_only()
         return
@@ -193,7 +250,7 @@ def main():
         # keep previously generated large cases
         old = np.load(OUT, allow_pickle=False)
         for key in old.files:
-            if key.startswith(("cfg3", "cfg4", "proj")) and key not in data:
+            if key.startswith(("cfg3", "cfg4", "proj", "cplx")) and key not in data:
                 data[key] = old[key]
         om = json.loads(str(old["meta_json"]))
         for c in [k for k in om if k.startswith(("cfg3", "cfg4", "proj"))]:

@@ -323,3 +323,91 @@ def test_oracle_iterative_fixed_point_is_stationary():
         E, gr = o.energy_gradient(x)
         assert abs(E - gz[name + "_iter_E"][0]) <= 1e-12
         assert np.linalg.norm(gr) < 1e-4
+
+
+# ------------------------------------------------------------- complex mode
+def _complex_oracle(gz, name):
+    k = kat()
+    o1, od1 = gz[name + "_ops1"], gz[name + "_order1"]
+    idx = gz[name + "_start_idx0"]
+    c = gz[name + "_start_re"] + 1j * gz[name + "_start_im"]
+    o = OracleEngine(8, idx, np.abs(c), o1 - 1, od1 - 1)
+    o.set_triplets(np.array(k["ham_i_1based"]) - 1, np.array(k["ham_j_1based"]) - 1, k["ham_coeffs"])
+    o.set_start_complex(idx, c)
+    return o
+
+
+@pytest.mark.parametrize("name", ["cplx_kat", "cplx_num", "cplx_mix"])
+def test_complex_against_reference_build(name):
+    """Golden vectors written by the reference built in its complex mode
+    (number operators, complex start states): energy, gradient and the
+    final state over the 256 basis states."""
+    gz = golden()
+    if name + "_E" not in gz.files:
+        pytest.skip("no complex-mode golden vectors")
+    o = _complex_oracle(gz, name)
+    for t, th in enumerate(gz[name + "_theta"]):
+        E, g = o.energy_gradient_complex(th)
+        assert abs(E - gz[name + "_E"][t]) <= 1e-12
+        assert np.max(np.abs(g - gz[name + "_grad"][t])) <= 1e-12
+        full = np.zeros(256, complex)
+        full[o.states()] = o.evolve_complex(th)
+        ref = gz[name + "_state_re"][t] + 1j * gz[name + "_state_im"][t]
+        assert np.max(np.abs(full - ref)) <= 1e-13
+
+
+def test_complex_mode_reduces_to_real_mode():
+    """A real start and no number operators: the complex-mode state is the
+    real-mode state with zero imaginary part, and energy and gradient agree
+    (the test.F90 known answers included)."""
+    k, o = kat_oracle()
+    o.set_start_complex([k["initial_index_1based"] - 1], [1.0])
+    for th in [k["angles_hf"], k["angles_min"]] + list(seeded_angles(9, 17, count=2)):
+        s = o.evolve_complex(th)
+        assert np.max(np.abs(s.imag)) == 0.0
+        assert np.max(np.abs(s.real - o.evolve(th))) <= 1e-14
+        E, g = o.energy_gradient_complex(th)
+        E0, g0 = o.energy_gradient(th)
+        assert abs(E - E0) <= 1e-13 and np.max(np.abs(g - g0)) <= 1e-13
+    assert abs(o.energy_gradient_complex(k["angles_min"])[0] - k["energy_min"]) < 1e-12
+
+
+def test_complex_number_operators_closed_form():
+    """A number operator n_p alone multiplies each basis state by exp(i theta)
+    where orbital p is occupied and leaves the rest alone; the energy is then
+    unchanged for a single-determinant start, and its gradient finite-
+    difference-correct on a superposition."""
+    k = kat()
+    ti, tj, tv = np.array(k["ham_i_1based"]) - 1, np.array(k["ham_j_1based"]) - 1, k["ham_coeffs"]
+    s0 = k["initial_index_1based"] - 1
+    o = OracleEngine(8, [s0, 172], [1.0, 1.0], np.array([[2, 2, -1, -1]]), np.array([0]))
+    o.set_triplets(ti, tj, tv)
+    c = np.array([0.6, 0.8j])
+    o.set_start_complex([s0, 172], c)
+    st = o.states()
+    for th in (0.3, -1.1):
+        s = o.evolve_complex([th])
+        expect = np.zeros(len(st), complex)
+        for i, b in zip([s0, 172], c):
+            expect[list(st).index(i)] = b * (np.exp(1j * th) if (i >> 2) & 1 else 1.0)
+        assert np.max(np.abs(s - expect)) <= 1e-15
+    E, g = o.energy_gradient_complex([0.4])
+    e = 1e-6
+    fd = (o.energy_gradient_complex([0.4 + e])[0] - o.energy_gradient_complex([0.4 - e])[0]) / (2 * e)
+    assert abs(fd - g[0]) <= 1e-8
+
+
+def test_complex_gradient_finite_difference_and_norm():
+    gz = golden()
+    if "cplx_mix_E" not in gz.files:
+        pytest.skip("no complex-mode golden vectors")
+    o = _complex_oracle(gz, "cplx_mix")
+    th = gz["cplx_mix_theta"][2]
+    E, g = o.energy_gradient_complex(th)
+    e = 1e-6
+    for i in range(len(th)):
+        d = np.zeros(len(th))
+        d[i] = e
+        fd = (o.energy_gradient_complex(th + d)[0] - o.energy_gradient_complex(th - d)[0]) / (2 * e)
+        assert abs(fd - g[i]) <= 1e-8
+    assert abs(np.linalg.norm(o.evolve_complex(th)) - 1.0) <= 1e-14
