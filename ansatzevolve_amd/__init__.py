@@ -160,6 +160,15 @@ class AnsatzSynth:
         self._chk("setInitialState", self.lib.setInitialState(num_qubits, len(i), ip, cp, self.ctx))
         self.num_qubits = int(num_qubits)
 
+    def set_initial_state_complex(self, num_qubits, indexes_1based, coeffs):
+        """Complex start state (setInitialStateComplex); the context then
+        evaluates complex amplitudes."""
+        i, ip = _i(indexes_1based)
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128).reshape(-1))
+        self._chk("setInitialStateComplex",
+                  self.lib.setInitialStateComplex(num_qubits, len(i), ip, c.ctypes.data_as(_vp), self.ctx))
+        self.num_qubits = int(num_qubits)
+
     def set_hamiltonian(self, i_1based, j_1based, coeffs):
         a, ap = _i(i_1based)
         b, bp = _i(j_1based)
@@ -259,6 +268,16 @@ class AnsatzSynth:
         n = 1 << num_qubits
         out = np.zeros(n)
         self._chk("getFinalState", self.lib.getFinalState(len(a), ap, n, out.ctypes.data_as(_dp), self.ctx))
+        return out
+
+    def final_state_complex(self, angles, num_qubits):
+        """The final state over the 2^num_qubits basis states as complex
+        amplitudes (getFinalStateComplex)."""
+        a, ap = _d(angles)
+        n = 1 << num_qubits
+        out = np.zeros(n, dtype=np.complex128)
+        self._chk("getFinalStateComplex",
+                  self.lib.getFinalStateComplex(len(a), ap, n, out.ctypes.data_as(_vp), self.ctx))
         return out
 
     def energy_gradient(self, angles):

@@ -151,10 +151,36 @@ int setInitialState(int numqubits, int n, const int *iindexes, const double *coe
     return 0;
 }
 
-int setInitialStateComplex(int, int, const int *, const ANSATZ_DOUBLE_COMPLEX *, void *)
+int setInitialStateComplex(int numqubits, int n, const int *iindexes, const ANSATZ_DOUBLE_COMPLEX *coeffs, void *ctx)
 {
-    log_msg("setInitialStateComplex: this library evaluates real amplitudes only");
-    return 4;
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (traceInterfaceCalls) log_msg("setInitialStateComplex(qubits=%d, n=%d)", numqubits, n);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    if (n < 1) {
+        log_msg("the entry count is below 1");
+        return 2;
+    }
+    if (numqubits < 1) {
+        log_msg("the qubit count is below 1");
+        return 2;
+    }
+    if (iindexes == nullptr || coeffs == nullptr) {
+        log_msg("the index or coefficient array is null");
+        return 2;
+    }
+    std::vector<int> iv(iindexes, iindexes + n);
+    for (int &x : iv) x--;
+    std::vector<double> re(n), im(n);
+    for (int k = 0; k < n; k++) {
+        re[k] = coeffs[k].real();
+        im[k] = coeffs[k].imag();
+    }
+    if (!m->storeInitialComplex(numqubits, iv, re, im)) return 3;
+    return 0;
 }
 
 int getEnergy(int nangles, const double *angles, double *energy, void *ctx)
@@ -215,10 +241,41 @@ int getFinalState(int nangles, const double *angles, int nbasisvectors, double *
     return 0;
 }
 
-int getFinalStateComplex(int, const double *, int, ANSATZ_DOUBLE_COMPLEX *, void *)
+int getFinalStateComplex(int nangles, const double *angles, int nbasisvectors, ANSATZ_DOUBLE_COMPLEX *finalstate,
+                         void *ctx)
 {
-    log_msg("getFinalStateComplex: this library evaluates real amplitudes only");
-    return 4;
+    if (ctx == nullptr) {
+        log_msg("the context is null");
+        return 1;
+    }
+    if (angles == nullptr) {
+        log_msg("the angle array is null");
+        return 2;
+    }
+    if (finalstate == nullptr) {
+        log_msg("the final-state output array is null");
+        return 3;
+    }
+    if (nbasisvectors < 1) {
+        log_msg("getFinalStateComplex: the output length is below 1");
+        return 4;
+    }
+    if (nangles < 1) {
+        log_msg("the angle count is below 1");
+        return 5;
+    }
+    if (traceInterfaceCalls) log_msg("getFinalStateComplex(nangles=%d, nbasis=%d)", nangles, nbasisvectors);
+    Manager *m = static_cast<Manager *>(ctx);
+    std::lock_guard<std::mutex> lock(m->lock);
+    std::vector<double> re, im;
+    if (!m->getFinalStateComplex(std::vector<double>(angles, angles + nangles), re, im)) return 6;
+    if (re.size() != (size_t)nbasisvectors) {
+        log_msg("getFinalStateComplex: the output holds %d amplitudes, the full space has %zu", nbasisvectors,
+                re.size());
+        return 7;
+    }
+    for (size_t i = 0; i < re.size(); i++) finalstate[i] = ANSATZ_DOUBLE_COMPLEX(re[i], im[i]);
+    return 0;
 }
 
 int getGradient_COMP(int nangles, const double *angles, double *gradient, void *ctx)

@@ -116,7 +116,7 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
 bool Manager::autotune(int nvec, int nangles, const double *angles, const char *cacheFile, int reps,
                        std::string &report)
 {
-    if (!ensureBuilt()) return false;
+    if (!ensureBuilt() || !realOnly("The autotuner")) return false;
     std::vector<double> dec;
     if (!decompressBatch(nvec, nangles, angles, dec)) return false;
     Engine &eng = *eng_;

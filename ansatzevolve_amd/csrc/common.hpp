@@ -25,15 +25,26 @@ struct DevOp {
 };
 
 // Builds the DevOp of a validated operator (a: zero-based, -1 = absent).
-// Returns false for operators the reference's real build cannot evolve:
-// anything that is not a single a^+_a a_b (a != b) or a double with four
-// distinct indices, or that addresses a spin-orbital >= nq.
-inline bool make_devop(const int8_t a[4], int nq, DevOp &o, std::string &why)
+// Returns false for operators the reference cannot evolve: anything that
+// is not a single a^+_a a_b (a != b) or a double with four distinct
+// indices, or that addresses a spin-orbital >= nq.  With numberOps (complex
+// amplitudes) number operators are accepted as well: a^+_p a_p and doubles
+// whose created pair is the annihilated pair (src/fusedevolve.cpp:82-95;
+// other repeated indices stay unsupported, src/fusedevolve.cpp:1806).  Their
+// DevOp has cre == ann == the orbitals that must be occupied, mask 0, and
+// flip the ordering phase (-1: the operator never acts).
+inline bool make_devop(const int8_t a[4], int nq, DevOp &o, std::string &why, bool numberOps = false)
 {
     auto inr = [nq](int x) { return x >= 0 && x < nq; };
     if (a[2] < 0 && a[3] < 0) {
         if (!inr(a[0]) || !inr(a[1])) { why = "single excitation index out of range"; return false; }
-        if (a[0] == a[1]) { why = "number operators need the complex build"; return false; }
+        if (a[0] == a[1] && numberOps) {
+            o.cre = o.ann = 1u << a[0];
+            o.mask = 0;
+            o.flip = 1;
+            return true;
+        }
+        if (a[0] == a[1]) { why = "number operators need complex amplitudes"; return false; }
         o.cre = 1u << a[0];
         o.ann = 1u << a[1];
         uint32_t m = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1);
@@ -44,9 +55,16 @@ inline bool make_devop(const int8_t a[4], int nq, DevOp &o, std::string &why)
     if (a[2] < 0 || a[3] < 0) { why = "operator is neither a single nor a double excitation"; return false; }
     for (int i = 0; i < 4; i++)
         if (!inr(a[i])) { why = "double excitation index out of range"; return false; }
+    if (numberOps && a[0] != a[1] && a[2] != a[3] &&
+        ((1u << a[0]) | (1u << a[1])) == ((1u << a[2]) | (1u << a[3]))) {
+        o.cre = o.ann = (1u << a[0]) | (1u << a[1]);
+        o.mask = 0;
+        o.flip = ((a[0] > a[1]) ? -1 : 1) * ((a[2] > a[3]) ? -1 : 1);
+        return true;
+    }
     for (int i = 0; i < 4; i++)
         for (int j = i + 1; j < 4; j++)
-            if (a[i] == a[j]) { why = "double excitation with repeated index (diagonal part) is unsupported in the real build"; return false; }
+            if (a[i] == a[j]) { why = "double excitation with a repeated index is not a supported operator"; return false; }
     o.cre = (1u << a[0]) | (1u << a[1]);
     o.ann = (1u << a[2]) | (1u << a[3]);
     uint32_t m = ((1u << a[0]) - 1) ^ ((1u << a[1]) - 1) ^ ((1u << a[2]) - 1) ^ ((1u << a[3]) - 1);

@@ -119,6 +119,7 @@ struct Seg {
     // its grid; adjoint (stored states): the orbits where the state can be
     // nonzero, also at supOff (supN of them)
     int supOff = -1, supN = 0, supLOff = 0, supLN = 0, gridSup = 1;
+    int diag = 0;  // a run of number operators (complex amplitudes; k_phase)
 };
 
 // the two-half lane-layout sparse sigma kernel for (same-half, cross-half)
@@ -193,6 +194,12 @@ struct EngineImpl {
     int ngeneric = 0;
 
     bool haveStart = false;
+    // complex amplitudes: d_start, d_psi, d_phi and d_lam hold a real and an
+    // imaginary plane of N each, the single-vector sequences run on the two
+    // planes as two vectors with one angle table, and number operators run
+    // as k_phase segments; planes() is the number of planes
+    bool cplx = false;
+    int planes() const { return cplx ? 2 : 1; }
     DBuf<double> d_start, d_psi, d_phi, d_lam;
     DBuf<double2> d_cs, d_csneg;
     DBuf<double> d_partial, d_epart, d_out;
@@ -1327,12 +1334,49 @@ struct EngineImpl {
         return true;
     }
 
+    // Angle tables of the single-vector sequences (d_cs, d_csneg): with
+    // complex amplitudes one table per plane, both the same.
+    bool upload_cs_single(const double *angles, std::string &err)
+    {
+        if (!cplx) return upload_cs(angles, 1, d_cs, d_csneg, err);
+        std::vector<double> two(2 * (size_t)M);
+        std::copy(angles, angles + M, two.begin());
+        std::copy(angles, angles + M, two.begin() + M);
+        return upload_cs(two.data(), 2, d_cs, d_csneg, err);
+    }
+
+    // Results of the single-vector E + gradient sequence: d_out holds
+    // planes() rows of M per-operator derivatives, then planes() energies;
+    // the planes add up.
+    bool collect_single(double *deriv, double *E, std::string &err)
+    {
+        const int pl = planes();
+        std::vector<double> out((size_t)pl * (M + 1));
+        HCHK(hipMemcpyAsync(out.data(), d_out.p, 8 * out.size(), hipMemcpyDeviceToHost, stream));
+        HCHK(hipStreamSynchronize(stream));
+        collect_marks();
+        for (int k = 0; k < M; k++) deriv[k] = pl == 2 ? out[k] + out[M + k] : out[k];
+        if (E) *E = pl == 2 ? out[2 * M] + out[2 * M + 1] : out[M];
+        return true;
+    }
+
+    // An error for the operations that work on real amplitudes only.
+    bool real_only(const char *what, std::string &err) const
+    {
+        if (!cplx) return true;
+        err = std::string(what) + " is not available for complex amplitudes";
+        return false;
+    }
+
     // Forward sweep of nvec vectors with per-vector angle tables.
     bool sweep_forward(double *v, int nvec, const double2 *cs, std::string &err)
     {
         const int mk = mark_begin(1);
         for (const Seg &sg : segs) {
-            if (sg.orbA >= 0 && orbLds) {
+            if (sg.diag) {
+                hipLaunchKernelGGL((k_phase<false>), dim3(Ggen, nvec / 2), dim3(kBlock), 0, stream, v, nullptr, N,
+                                   N, d_states.p, d_ops.p, sg.k0, sg.k1, cs, (long long)M, nullptr, 0, 0);
+            } else if (sg.orbA >= 0 && orbLds) {
                 hipLaunchKernelGGL((k_orbl<false>), dim3(tilesets[sg.orbA]->ntiles, nvec), dim3(kBlock),
                                    (size_t)2 * nb * sizeof(double), stream, orbargs(sg, false), v, nullptr, N, cs,
                                    (long long)M, nullptr, 0, 0);
@@ -1364,7 +1408,10 @@ struct EngineImpl {
         const int mk = mark_begin(2);
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
-            if (sg.orbA >= 0 && orbLds) {
+            if (sg.diag) {
+                hipLaunchKernelGGL((k_phase<true>), dim3(Ggen, nvec / 2), dim3(kBlock), 0, stream, phi, lam, N, N,
+                                   d_states.p, d_ops.p, sg.k0, sg.k1, csneg, (long long)M, partial, slots, pstride);
+            } else if (sg.orbA >= 0 && orbLds) {
                 hipLaunchKernelGGL((k_orbl<true>), dim3(tilesets[sg.orbA]->ntiles, nvec), dim3(kBlock),
                                    (size_t)4 * nb * sizeof(double), stream, orbargs(sg, true), phi, lam, N, csneg,
                                    (long long)M, partial, slots, pstride);
@@ -1821,7 +1868,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     e.scale = scale;
     e.hops.resize(M);
     for (int k = 0; k < M; k++)
-        if (!make_devop(&ops4[4 * k], e.nq, e.hops[k], err)) {
+        if (!make_devop(&ops4[4 * k], e.nq, e.hops[k], err, e.cplx)) {
             err = "operator " + std::to_string(k + 1) + ": " + err;
             return false;
         }
@@ -1834,6 +1881,18 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     e.segs.clear();
     e.ngeneric = 0;
     int maxTiles = 0;
+    // number operators (complex amplitudes only): a run of up to
+    // kPhaseMaxOps consecutive ones is one k_phase segment, and no other
+    // segment takes one in
+    auto isDiag = [&](int kk) { return e.hops[kk].cre == e.hops[kk].ann; };
+    auto diagRun = [&](int kk) {
+        int k1 = kk + 1;
+        while (k1 < M && isDiag(k1) && k1 - kk < kPhaseMaxOps) k1++;
+        Seg sg{kk, k1, -1};
+        sg.diag = 1;
+        e.segs.push_back(sg);
+        return k1;
+    };
     if (e.mode == kSz) {
         const int n = e.n;
         const uint32_t amask = (1u << n) - 1;
@@ -1861,12 +1920,16 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         auto bitsA = [&](int kk) { return ((e.hops[kk].cre | e.hops[kk].ann) >> n) & amask; };
         auto bitsB = [&](int kk) { return (e.hops[kk].cre | e.hops[kk].ann) & amask; };
         while (k < M) {
+            if (isDiag(k)) {
+                k = diagRun(k);
+                continue;
+            }
             const uint32_t Uk = bitsA(k);
             if (useOrbits && __builtin_popcount(Uk) <= 2 && __builtin_popcount(bitsB(k)) <= 2) {
                 // orbit segment: extend while both spins stay within two orbitals
                 uint32_t Ua = Uk, Ub = bitsB(k);
                 int k1 = k + 1;
-                while (k1 < M && k1 - k < maxOrb) {
+                while (k1 < M && k1 - k < maxOrb && !isDiag(k1)) {
                     const uint32_t na2 = Ua | bitsA(k1), nb2 = Ub | bitsB(k1);
                     if (__builtin_popcount(na2) > 2 || __builtin_popcount(nb2) > 2) break;
                     Ua = na2;
@@ -1893,7 +1956,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
             }
             uint32_t U = Uk;
             int k1 = k + 1;
-            while (k1 < M && k1 - k < maxOps) {
+            while (k1 < M && k1 - k < maxOps && !isDiag(k1)) {
                 const uint32_t Un = U | (((e.hops[k1].cre | e.hops[k1].ann) >> n) & amask);
                 if (!fits(Un)) break;
                 U = Un;
@@ -1917,14 +1980,29 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
             }
         }
     } else {
-        for (int k = 0; k < M; k++) e.segs.push_back({k, k + 1, -1});
-        e.ngeneric = M;
+        for (int k = 0; k < M;) {
+            if (isDiag(k)) {
+                k = diagRun(k);
+                continue;
+            }
+            e.segs.push_back({k, k + 1, -1});
+            e.ngeneric++;
+            k++;
+        }
     }
     e.slots = std::max(maxTiles, e.Ggen);
     if (!e.build_orbit_programs(err)) return false;
-    if (!e.build_lane_segments(err)) return false;
-    if (!e.d_cs.ensure(M, err) || !e.d_csneg.ensure(M, err) || !e.d_partial.ensure((size_t)M * e.slots, err) ||
-        !e.d_epart.ensure(e.eslots_alloc(), err) || !e.d_out.ensure(M + 1, err))
+    if (e.cplx) {
+        // the lane layout (batches) is not used with complex amplitudes
+        e.laneA.segs.clear();
+        e.laneF.segs.clear();
+    } else if (!e.build_lane_segments(err)) {
+        return false;
+    }
+    const int pl = e.planes();
+    if (!e.d_cs.ensure((size_t)pl * M, err) || !e.d_csneg.ensure((size_t)pl * M, err) ||
+        !e.d_partial.ensure((size_t)pl * M * e.slots, err) || !e.d_epart.ensure((size_t)pl * e.eslots_alloc(), err) ||
+        !e.d_out.ensure((size_t)pl * (M + 1), err))
         return false;
     return true;
 }
@@ -1938,9 +2016,65 @@ bool Engine::set_start(const std::vector<double> &start, std::string &err)
     }
     e.drop_graphs();
     HCHK(hipMemcpy(e.d_start.p, start.data(), 8 * e.N, hipMemcpyHostToDevice));
+    if (e.cplx) HCHK(hipMemset(e.d_start.p + e.N, 0, 8 * e.N));
     e.haveStart = true;
     e.hstart = start;
     e.supDirty = true;
+    return true;
+}
+
+bool Engine::set_complex(bool on, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (e.N == 0) {
+        err = "engine not initialised";
+        return false;
+    }
+    if (on == e.cplx) return true;
+    e.drop_graphs();
+    e.cplx = on;
+    e.haveStart = false;
+    const size_t n = (size_t)e.planes() * e.N;
+    if (!e.d_psi.ensure(n, err) || !e.d_phi.ensure(n, err) || !e.d_lam.ensure(n, err) || !e.d_start.ensure(n, err))
+        return false;
+    e.bnvec = 0;
+    // re-segment: number operators become legal (or illegal) operators
+    if (e.M > 0) return set_operators(e.ops4, std::vector<int>(e.order), std::vector<double>(e.scale), e.P, err);
+    return true;
+}
+
+bool Engine::is_complex() const { return d_->cplx; }
+
+bool Engine::set_start_complex(const std::vector<double> &re, const std::vector<double> &im, std::string &err)
+{
+    EngineImpl &e = *d_;
+    if (!e.cplx) {
+        err = "complex start state without complex amplitudes";
+        return false;
+    }
+    if ((long long)re.size() != e.N || (long long)im.size() != e.N) {
+        err = "start vector has the wrong size";
+        return false;
+    }
+    e.drop_graphs();
+    HCHK(hipMemcpy(e.d_start.p, re.data(), 8 * e.N, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(e.d_start.p + e.N, im.data(), 8 * e.N, hipMemcpyHostToDevice));
+    e.haveStart = true;
+    e.hstart = re;
+    e.supDirty = true;
+    return true;
+}
+
+bool Engine::download_state_complex(double *re, double *im, std::string &err)
+{
+    EngineImpl &e = *d_;
+    HCHK(hipMemcpyAsync(re, e.d_psi.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
+    if (e.cplx) {
+        HCHK(hipMemcpyAsync(im, e.d_psi.p + e.N, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
+    } else {
+        std::fill(im, im + e.N, 0.0);
+    }
+    HCHK(hipStreamSynchronize(e.stream));
     return true;
 }
 
@@ -2132,9 +2266,10 @@ bool Engine::evolve(const double *angles, std::string &err)
         err = "engine not set up";
         return false;
     }
-    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
-    HCHK(hipMemcpyAsync(e.d_psi.p, e.d_start.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
-    if (!e.sweep_forward(e.d_psi.p, 1, e.d_cs.p, err)) return false;
+    const int pl = e.planes();
+    if (!e.upload_cs_single(angles, err)) return false;
+    HCHK(hipMemcpyAsync(e.d_psi.p, e.d_start.p, 8 * e.N * pl, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.sweep_forward(e.d_psi.p, pl, e.d_cs.p, err)) return false;
     HCHK(hipStreamSynchronize(e.stream));
     return true;
 }
@@ -2142,12 +2277,15 @@ bool Engine::evolve(const double *angles, std::string &err)
 bool Engine::energy(double *E, std::string &err)
 {
     EngineImpl &e = *d_;
-    if (!e.ham_apply(e.d_psi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
+    const int pl = e.planes(), es = e.eslots();
+    if (!e.ham_apply(e.d_psi.p, e.d_lam.p, pl, e.d_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(pl), dim3(kBlock), 0, e.stream, e.d_epart.p, pl, es, (long long)es, 1LL,
                        e.d_out.p, 1.0);
     LCHK();
-    HCHK(hipMemcpyAsync(E, e.d_out.p, 8, hipMemcpyDeviceToHost, e.stream));
+    double parts[2] = {0, 0};
+    HCHK(hipMemcpyAsync(parts, e.d_out.p, 8 * pl, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
+    *E = parts[0] + parts[1];  // real plane, imaginary plane
     return true;
 }
 
@@ -2158,20 +2296,24 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
         err = "engine not set up";
         return false;
     }
-    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
-    const unsigned cgrid = (unsigned)std::min<long long>((e.N + kBlock - 1) / kBlock, 4096);
+    if (!e.upload_cs_single(angles, err)) return false;
+    // complex amplitudes: both planes through every step; out holds the
+    // per-operator brackets of each plane, then each plane's energy
+    const int pl = e.planes(), es = e.eslots();
+    const long long NP = e.N * pl;
+    const unsigned cgrid = (unsigned)std::min<long long>((NP + kBlock - 1) / kBlock, 4096);
     auto enqueue = [&]() -> bool {
-        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_start.p, e.d_psi.p, e.N, 1);
+        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_start.p, e.d_psi.p, NP, 1);
         LCHK();
-        if (!e.sweep_forward(e.d_psi.p, 1, e.d_cs.p, err)) return false;
-        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_psi.p, e.d_phi.p, e.N, 1);
+        if (!e.sweep_forward(e.d_psi.p, pl, e.d_cs.p, err)) return false;
+        hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_psi.p, e.d_phi.p, NP, 1);
         LCHK();
-        if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
-                           e.d_out.p + e.M, 1.0);
+        if (!e.ham_apply(e.d_phi.p, e.d_lam.p, pl, e.d_epart.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(pl), dim3(kBlock), 0, e.stream, e.d_epart.p, pl, es, (long long)es, 1LL,
+                           e.d_out.p + (size_t)pl * e.M, 1.0);
         LCHK();
-        if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, e.d_csneg.p, e.d_partial.p, err)) return false;
-        hipLaunchKernelGGL(k_reduce, dim3(e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, e.M, e.slots,
+        if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, pl, e.d_csneg.p, e.d_partial.p, err)) return false;
+        hipLaunchKernelGGL(k_reduce, dim3(pl * e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, pl * e.M, e.slots,
                            (long long)e.slots, 1LL, e.d_out.p, 2.0);
         LCHK();
         return true;
@@ -2206,13 +2348,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
     } else if (!enqueue()) {
         return false;
     }
-    std::vector<double> out(e.M + 1);
-    HCHK(hipMemcpyAsync(out.data(), e.d_out.p, 8 * (e.M + 1), hipMemcpyDeviceToHost, e.stream));
-    HCHK(hipStreamSynchronize(e.stream));
-    e.collect_marks();
-    std::memcpy(deriv, out.data(), 8 * e.M);
-    if (E) *E = out[e.M];
-    return true;
+    return e.collect_single(deriv, E, err);
 }
 
 bool Engine::gradient(const double *angles, double *E, double *deriv, std::string &err)
@@ -2220,28 +2356,25 @@ bool Engine::gradient(const double *angles, double *E, double *deriv, std::strin
     EngineImpl &e = *d_;
     // sin/cos of the current angles (the forward sweep may have used them
     // already; recomputing keeps this call self-contained)
-    if (!e.upload_cs(angles, 1, e.d_cs, e.d_csneg, err)) return false;
-    HCHK(hipMemcpyAsync(e.d_phi.p, e.d_psi.p, 8 * e.N, hipMemcpyDeviceToDevice, e.stream));
-    if (!e.ham_apply(e.d_phi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e.stream, e.d_epart.p, 1, e.eslots(), 0LL, 1LL,
-                       e.d_out.p + e.M, 1.0);
+    if (!e.upload_cs_single(angles, err)) return false;
+    const int pl = e.planes(), es = e.eslots();
+    HCHK(hipMemcpyAsync(e.d_phi.p, e.d_psi.p, 8 * e.N * pl, hipMemcpyDeviceToDevice, e.stream));
+    if (!e.ham_apply(e.d_phi.p, e.d_lam.p, pl, e.d_epart.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(pl), dim3(kBlock), 0, e.stream, e.d_epart.p, pl, es, (long long)es, 1LL,
+                       e.d_out.p + (size_t)pl * e.M, 1.0);
     LCHK();
-    if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, 1, e.d_csneg.p, e.d_partial.p, err)) return false;
-    hipLaunchKernelGGL(k_reduce, dim3(e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, e.M, e.slots,
+    if (!e.sweep_reverse(e.d_phi.p, e.d_lam.p, pl, e.d_csneg.p, e.d_partial.p, err)) return false;
+    hipLaunchKernelGGL(k_reduce, dim3(pl * e.M), dim3(kBlock), 0, e.stream, e.d_partial.p, pl * e.M, e.slots,
                        (long long)e.slots, 1LL, e.d_out.p, 2.0);
     LCHK();
-    std::vector<double> out(e.M + 1);
-    HCHK(hipMemcpyAsync(out.data(), e.d_out.p, 8 * (e.M + 1), hipMemcpyDeviceToHost, e.stream));
-    HCHK(hipStreamSynchronize(e.stream));
-    std::memcpy(deriv, out.data(), 8 * e.M);
-    if (E) *E = out[e.M];
-    return true;
+    return e.collect_single(deriv, E, err);
 }
 
 bool Engine::overlap_gradient(const double *angles, const double *proj, double *overlap, double *deriv,
                               std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("the overlap gradient", err)) return false;
     if (!e.haveStart || e.M == 0) {
         err = "engine not set up";
         return false;
@@ -2283,6 +2416,7 @@ bool Engine::download_state(double *out, std::string &err)
 bool Engine::download_ham_state(double *out, std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("H psi", err)) return false;
     if (!e.ham_apply(e.d_psi.p, e.d_lam.p, 1, e.d_epart.p, err)) return false;
     HCHK(hipMemcpyAsync(out, e.d_lam.p, 8 * e.N, hipMemcpyDeviceToHost, e.stream));
     HCHK(hipStreamSynchronize(e.stream));
@@ -2330,6 +2464,7 @@ size_t Engine::noalloc_scratch_need() const
 bool Engine::energy_gradient_noalloc(const double *angles, double *E, double *deriv, std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("the no-allocation evaluation", err)) return false;
     if (!e.haveStart || e.M == 0 || e.hkind == 0) {
         err = "engine not set up";
         return false;
@@ -2399,6 +2534,7 @@ bool Engine::apply_ham(const double *x, double *y, std::string &err)
 bool Engine::subspace(int nvec, const double *angles, double *Hmat, double *Smat, std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("subspace diagonalisation", err)) return false;
     if (!e.haveStart || e.M == 0 || e.hkind == 0) {
         err = "engine not set up";
         return false;
@@ -2441,6 +2577,7 @@ bool Engine::subspace(int nvec, const double *angles, double *Hmat, double *Smat
 bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("a batch of angle vectors", err)) return false;
     if (!e.haveStart || e.M == 0 || e.hkind == 0) {
         err = "engine not set up";
         return false;
@@ -2699,7 +2836,7 @@ bool Engine::set_option(int key, int value, std::string &err)
         int &dst = key == kOptLaneGroups ? e.maxGroups : e.maxGroupsF;
         if (value == dst) return true;
         dst = value;
-        if (e.M > 0) {
+        if (e.M > 0 && !e.cplx) {
             e.drop_graphs();
             if (!e.build_lane_segments(err)) return false;
         }
@@ -2983,6 +3120,7 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
                                 double *metric, std::string &err)
 {
     EngineImpl &e = *d_;
+    if (!e.real_only("the Hessian", err)) return false;
     if (!e.haveStart || e.M == 0 || (!proj && e.hkind == 0)) {
         err = "engine not set up";
         return false;

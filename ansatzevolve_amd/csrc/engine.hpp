@@ -78,6 +78,22 @@ public:
     // psi_0 over the sector.
     bool set_start(const std::vector<double> &start, std::string &err);
 
+    // Complex amplitudes (the reference's complex build, useComplex in
+    // src/globals.h:16-28): psi is held as a real and an imaginary plane,
+    // the excitations rotate both alike, and number operators (a^+_p a_p,
+    // a^+_p a^+_q a_p a_q) multiply the amplitudes of the states where they
+    // act by exp(+i angle) (src/fusedevolve.cpp:520-717).  Evolution,
+    // energy, gradient and the state are available; the Hessians, overlap
+    // gradients, batches, subspace products and the no-allocation
+    // evaluation refuse.  Switching re-segments the operators and drops the
+    // start state.  set_start gives a real start (imaginary plane zero).
+    bool set_complex(bool on, std::string &err);
+    bool is_complex() const;
+    bool set_start_complex(const std::vector<double> &re, const std::vector<double> &im, std::string &err);
+    // The current psi over the sector as its two planes (im all zero for
+    // real amplitudes).
+    bool download_state_complex(double *re, double *im, std::string &err);
+
     // Hamiltonian as rows of y = M^T x over the sector (output row j holds
     // (input index, value) pairs), built on the host from triplets.
     bool set_ham_rows(const std::vector<int64_t> &rowptr, const std::vector<int32_t> &col,
@@ -119,7 +135,7 @@ public:
     // (FusedEvolve::evolveHessianProj, src/fusedevolve.cpp:2892-3183).
     bool hessian_proj(const double *angles, const double *proj, double *hess, double *grad, double *overlap,
                       std::string &err);
-    // The current psi over the sector.
+    // The current psi over the sector (its real plane for complex amplitudes).
     bool download_state(double *out, std::string &err);
     // H psi of the current psi over the sector.
     bool download_ham_state(double *out, std::string &err);

@@ -102,6 +102,73 @@ template __global__ void k_op_generic<false>(double *, double *, long long, long
 template __global__ void k_op_generic<true>(double *, double *, long long, long long, DevSector, const DevOp *, int,
                                             const double2 *, long long, double *, int, long long);
 
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_phase(double *__restrict__ v0, double *__restrict__ v1, long long vstride, long long N,
+            const uint32_t *__restrict__ states, const DevOp *__restrict__ ops, int k0, int k1,
+            const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+            long long pstride)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ DevOp sop[kPhaseMaxOps];
+    __shared__ double2 scs[kPhaseMaxOps];
+    const int c = blockIdx.y, R = k1 - k0;
+    double *re = v0 + 2 * c * vstride, *im = re + vstride;
+    double *lre = BR ? v1 + 2 * c * vstride : nullptr, *lim = BR ? lre + vstride : nullptr;
+    if ((int)threadIdx.x < R) {
+        sop[threadIdx.x] = ops[k0 + threadIdx.x];
+        scs[threadIdx.x] = cs[2 * c * csStride + k0 + threadIdx.x];
+    }
+    __syncthreads();
+    double acc[kPhaseMaxOps];
+#pragma unroll
+    for (int r = 0; r < kPhaseMaxOps; r++) acc[r] = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < N; i += (long long)gridDim.x * kBlock) {
+        const uint32_t b = states[i];
+        double x = re[i], y = im[i];
+        double lx = 0, ly = 0, br = 0;
+        if (BR) {
+            lx = lre[i];
+            ly = lim[i];
+            br = ly * x - lx * y;
+        }
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < kPhaseMaxOps; r++) {
+            if (r >= R || !number_op_active(sop[r], b)) continue;
+            any = true;
+            const double C = scs[r].x, S = scs[r].y;
+            const double nx = C * x - S * y, ny = S * x + C * y;
+            x = nx;
+            y = ny;
+            if (BR) {
+                acc[r] += br;
+                const double mx = C * lx - S * ly, my = S * lx + C * ly;
+                lx = mx;
+                ly = my;
+            }
+        }
+        if (!any) continue;
+        re[i] = x;
+        im[i] = y;
+        if (BR) {
+            lre[i] = lx;
+            lim[i] = ly;
+        }
+    }
+    if (BR) {
+        for (int r = 0; r < R; r++) {
+            const double tot = block_sum(acc[r], red);
+            if (threadIdx.x == 0) partial[2 * c * pstride + (long long)(k0 + r) * slots + blockIdx.x] = tot;
+        }
+    }
+}
+
+template __global__ void k_phase<false>(double *, double *, long long, long long, const uint32_t *, const DevOp *,
+                                        int, int, const double2 *, long long, double *, int, long long);
+template __global__ void k_phase<true>(double *, double *, long long, long long, const uint32_t *, const DevOp *,
+                                       int, int, const double2 *, long long, double *, int, long long);
+
 // One workgroup per tile: the tile's rows of NV vectors are staged in LDS,
 // every operator of the segment is applied there (barrier between
 // operators), brackets are reduced per operator, and the rows are written

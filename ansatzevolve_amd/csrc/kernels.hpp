@@ -288,6 +288,33 @@ __global__ void k_op_generic(double *__restrict__ v0, double *__restrict__ v1, l
                              DevSector sec, const DevOp *__restrict__ ops, int k, const double2 *__restrict__ cs,
                              long long csStride, double *__restrict__ partial, int slots, long long pstride);
 
+// Complex amplitudes are held as two planes of N doubles, the real plane
+// then the imaginary one; the excitation operators rotate both planes
+// alike.  A number operator (cre == ann in its DevOp) acts on a basis state
+// b only when every orbital it names is occupied and its ordering phase
+// (flip) is +1; there it multiplies the amplitude by exp(+i theta)
+// (RunFuseNDiagonal, src/fusedevolve.cpp:627-717).  Runs of consecutive
+// number operators commute and are applied in one pass of k_phase.
+constexpr int kPhaseMaxOps = 16;
+
+__device__ __forceinline__ bool number_op_active(const DevOp &o, uint32_t b)
+{
+    return o.flip == 1 && (b & o.ann) == o.ann;
+}
+
+// Number operators k0 .. k1-1 (k1 - k0 <= kPhaseMaxOps) on complex vectors
+// v0 (planes at v0 + 2 c vstride and + vstride for complex vector c =
+// blockIdx.y), with the angle table of plane 2c.  BR: v1 holds lambda
+// (same layout), rotated alike, and each operator's adjoint bracket
+// sum_b Im(lambda_b) Re(v0_b) - Re(lambda_b) Im(v0_b) over its active
+// states goes to partial[2c pstride + k slots + block] (a common phase
+// leaves the bracket unchanged, so it is taken before the rotation).
+template <bool BR>
+__global__ void k_phase(double *__restrict__ v0, double *__restrict__ v1, long long vstride, long long N,
+                        const uint32_t *__restrict__ states, const DevOp *__restrict__ ops, int k0, int k1,
+                        const double2 *__restrict__ cs, long long csStride, double *__restrict__ partial, int slots,
+                        long long pstride);
+
 // A fused segment of operators on one tile held in LDS (S_z sector).
 template <int NV, bool BR>
 __global__ void k_seg(SegArgs a, double *__restrict__ v0, double *__restrict__ v1, long long vstride,

@@ -46,6 +46,21 @@ bool Manager::storeOperators(const std::vector<std::array<int8_t, 4>> &excs)
 // 0 .. q/2-1 are spin down, q/2 .. q-1 spin up.
 bool Manager::storeInitial(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs)
 {
+    return storeStart(numberOfQubits, indexes, coeffs, nullptr);
+}
+
+// setInitialStateComplex (src/AnsatzSynthInterface.cpp:251-300): the same
+// checks, amplitudes re + i im; the context then evaluates complex
+// amplitudes.
+bool Manager::storeInitialComplex(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &re,
+                                  const std::vector<double> &im)
+{
+    return storeStart(numberOfQubits, indexes, re, &im);
+}
+
+bool Manager::storeStart(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs,
+                         const std::vector<double> *imag)
+{
     std::vector<std::string> why;
     if (frozen("Initial state")) why.push_back("set-up fixed");
     if (numberOfQubits < 1 || numberOfQubits > 30) {
@@ -68,16 +83,19 @@ bool Manager::storeInitial(int numberOfQubits, const std::vector<int> &indexes, 
             }
     if (!why.empty()) return false;
 
+    auto im = [&](size_t k) { return imag ? (*imag)[k] : 0.0; };
     // a later entry for the same basis state replaces an earlier one
-    std::map<int, double> amp;
-    for (size_t k = 0; k < indexes.size(); k++) amp[indexes[k]] = coeffs[k];
+    std::map<int, double> amp2;
+    for (size_t k = 0; k < indexes.size(); k++) amp2[indexes[k]] = coeffs[k] * coeffs[k] + im(k) * im(k);
     double norm2 = 0;
-    for (const auto &e : amp) norm2 += e.second * e.second;
+    for (const auto &e : amp2) norm2 += e.second;
     const double dev = std::fabs(std::sqrt(norm2) - 1.0);
     if (dev > 1e-14) log_msg("Note: the initial state's norm differs from 1 by %g", dev);
     start_.qubits = numberOfQubits;
     start_.bits = indexes;
     start_.amps = coeffs;
+    start_.ampsIm = imag ? *imag : std::vector<double>();
+    start_.complex = imag != nullptr;
 
     const int half = numberOfQubits / 2;
     const uint32_t downMask = (1u << half) - 1;
@@ -86,7 +104,7 @@ bool Manager::storeInitial(int numberOfQubits, const std::vector<int> &indexes, 
     o.fixedSpins = numberOfQubits % 2 == 0;
     bool any = false;
     for (size_t k = 0; k < indexes.size() && o.fixedTotal; k++) {
-        if (coeffs[k] == 0.0) continue;
+        if (coeffs[k] == 0.0 && im(k) == 0.0) continue;
         const uint32_t b = (uint32_t)indexes[k];
         const int tot = __builtin_popcount(b), dn = __builtin_popcount(b & downMask),
                   up = __builtin_popcount(b >> half);
@@ -262,6 +280,11 @@ bool Manager::build()
         log_msg("Device set-up failed: %s", err.c_str());
         return false;
     }
+    const bool cplx = complexAmplitudes();
+    if (cplx && !eng->set_complex(true, err)) {
+        log_msg("Device set-up failed: %s", err.c_str());
+        return false;
+    }
     // operators and the parameter map (src/tupsquantities.cpp:643-667)
     std::vector<int8_t> flat;
     flat.reserve(4 * ops_.size());
@@ -272,12 +295,14 @@ bool Manager::build()
     }
     // the start state over the sector (components outside it are dropped,
     // as compressVector does: src/sparsematrix.h:83-105)
-    std::vector<double> psi0(eng->dim(), 0.0);
+    std::vector<double> psi0(eng->dim(), 0.0), psi0i(cplx ? eng->dim() : 0, 0.0);
     for (size_t k = 0; k < start_.bits.size(); k++) {
         const long long r = eng->rank((uint32_t)start_.bits[k]);
-        if (r >= 0) psi0[r] = start_.amps[k];
+        if (r < 0) continue;
+        psi0[r] = start_.amps[k];
+        if (cplx && !start_.ampsIm.empty()) psi0i[r] = start_.ampsIm[k];
     }
-    if (!eng->set_start(psi0, err)) {
+    if (!(cplx ? eng->set_start_complex(psi0, psi0i, err) : eng->set_start(psi0, err))) {
         log_msg("Start state upload failed: %s", err.c_str());
         return false;
     }
@@ -403,9 +428,49 @@ bool Manager::getExpectationValue(const std::vector<double> &angles, double &E)
     return true;
 }
 
+bool Manager::realOnly(const char *what) const
+{
+    if (!eng_ || !eng_->is_complex()) return true;
+    log_msg("%s is not available for complex amplitudes", what);
+    return false;
+}
+
+bool Manager::complexAmplitudes() const
+{
+    if (start_.complex) return true;
+    for (const auto &e : ops_) {
+        const bool single = e[2] < 0 && e[3] < 0;
+        const uint32_t cre = single ? 1u << e[0] : (1u << e[0]) | (1u << e[1]);
+        const uint32_t ann = single ? 1u << e[1] : (1u << e[2]) | (1u << e[3]);
+        if (e[0] >= 0 && e[1] >= 0 && cre == ann) return true;  // a number operator
+    }
+    return false;
+}
+
+bool Manager::getFinalStateComplex(const std::vector<double> &angles, std::vector<double> &re,
+                                   std::vector<double> &im)
+{
+    if (!setAngles(angles)) return false;
+    std::string err;
+    std::vector<double> cr(eng_->dim()), ci(eng_->dim());
+    if (!eng_->download_state_complex(cr.data(), ci.data(), err)) {
+        log_msg("State download failed: %s", err.c_str());
+        return false;
+    }
+    re.assign((size_t)1 << start_.qubits, 0.0);
+    im.assign(re.size(), 0.0);
+    const auto &st = eng_->states();
+    for (size_t i = 0; i < cr.size(); i++) {
+        re[st[i]] = cr[i];
+        im[st[i]] = ci[i];
+    }
+    return true;
+}
+
 bool Manager::getFinalState(const std::vector<double> &angles, std::vector<double> &full)
 {
     if (!setAngles(angles)) return false;
+    if (eng_->is_complex()) log_msg("Note: complex amplitudes; the final state returned is their real part");
     std::string err;
     std::vector<double> comp(eng_->dim());
     if (!eng_->download_state(comp.data(), err)) {
@@ -693,7 +758,7 @@ namespace ae {
 bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<double> &trace, double stopEnergy,
                        bool useStop)
 {
-    if (!ensureBuilt()) return false;
+    if (!ensureBuilt() || !realOnly("The Newton optimiser")) return false;
     const size_t M = ops_.size();
     const int P = nparams_;
     const bool perParam = (int)angles.size() == P && angles.size() != M;
@@ -750,7 +815,7 @@ bool Manager::optimise(std::vector<double> &angles, double &E, std::vector<doubl
 
 bool Manager::iterativeOptimise(std::vector<double> &angles, double &E, std::vector<double> &trace, bool ediis)
 {
-    if (!ensureBuilt()) return false;
+    if (!ensureBuilt() || !realOnly("The iterative optimiser")) return false;
     const size_t M = ops_.size();
     const int P = nparams_;
     const bool perParam = (int)angles.size() == P && angles.size() != M;
@@ -811,7 +876,7 @@ bool Manager::iterativeOptimise(std::vector<double> &angles, double &E, std::vec
 bool Manager::subspaceDiagonalise(int nsets, int nangles, const double *angles, std::vector<double> &evals,
                                   std::vector<double> &H, std::vector<double> &S, int &nullRank)
 {
-    if (!ensureBuilt()) return false;
+    if (!ensureBuilt() || !realOnly("Subspace diagonalisation")) return false;
     std::vector<double> dec;
     if (!decompressBatch(nsets, nangles, angles, dec)) return false;
     H.assign((size_t)nsets * nsets, 0.0);
@@ -843,7 +908,7 @@ bool Manager::properties(int nsets, int nangles, const double *angles, int nover
                          std::vector<double> &rows, std::vector<double> &hessians, std::vector<double> &metrics,
                          std::vector<double> &lowest)
 {
-    if (!ensureBuilt()) return false;
+    if (!ensureBuilt() || !realOnly("The properties table")) return false;
     const int P = nparams_;
     const long long N = eng_->dim();
     const int cols = kPropColumns + 2 * noverlaps;

@@ -23,6 +23,8 @@ public:
 
     bool storeOperators(const std::vector<std::array<int8_t, 4>> &excs);
     bool storeInitial(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs);
+    bool storeInitialComplex(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &re,
+                             const std::vector<double> &im);
     bool storeHamiltonian(std::vector<int> &&i0, std::vector<int> &&j0, std::vector<double> &&v);
     bool storeIntegrals(int norb, const double *h, const double *g);
     bool storeNuclearEnergy(double e);
@@ -34,7 +36,13 @@ public:
     bool storeRunPath(const std::string &runPath);
 
     bool getExpectationValue(const std::vector<double> &angles, double &E);
+    // the real part for complex amplitudes (src/AnsatzSynthInterface.cpp:402-407)
     bool getFinalState(const std::vector<double> &angles, std::vector<double> &full);
+    // both parts over the full space (imaginary part zero for real amplitudes)
+    bool getFinalStateComplex(const std::vector<double> &angles, std::vector<double> &re, std::vector<double> &im);
+    // Complex amplitudes: the start state was given with complex
+    // coefficients or some operator is a number operator.
+    bool complexAmplitudes() const;
     bool getGradientComp(const std::vector<double> &angles, std::vector<double> &grad);
     // per operator (decompressed; src/AnsatzManager.h:221)
     bool getGradient(const std::vector<double> &angles, std::vector<double> &grad);
@@ -113,6 +121,10 @@ private:
     bool toOperatorAngles(const std::vector<double> &in, std::vector<double> &out) const;
     bool setAngles(std::vector<double> angles);
     bool gradientAt(const std::vector<double> &angles, double *E, std::vector<double> &deriv);
+    // True for real amplitudes; else logs that `what` needs them.
+    bool realOnly(const char *what) const;
+    bool storeStart(int numberOfQubits, const std::vector<int> &indexes, const std::vector<double> &coeffs,
+                    const std::vector<double> *imag);
     bool decompressBatch(int nvec, int nangles, const double *angles, std::vector<double> &out);
     void compressGradient(const double *deriv, double *grad) const;
 
@@ -121,6 +133,8 @@ private:
         int qubits = -1;
         std::vector<int> bits;      // zero-based basis states
         std::vector<double> amps;
+        std::vector<double> ampsIm;  // imaginary parts (setInitialStateComplex)
+        bool complex = false;        // given with complex coefficients
     } start_;
     // occupation shared by every nonzero component of the start state
     struct Occupation {

@@ -52,7 +52,10 @@ int getEnergy(int nangles, const double *angles, double *energy, void *ctx);
  * 7 nbasisvectors != 2^numqubits.  Replaces src/AnsatzSynthInterface.cpp:362-420. */
 int getFinalState(int nangles, const double *angles, int nbasisvectors, double *finalstate, void *ctx);
 
-/* Real build: logs and returns 4 (src/AnsatzSynthInterface.cpp:487-497). */
+/* The final state over the full space as complex amplitudes (imaginary
+ * parts zero for a real context).  Codes as getFinalState.  Replaces the
+ * complex build's src/AnsatzSynthInterface.cpp:439-486 (its real build
+ * returns 4; this library has one build, and 4 means nbasisvectors < 1). */
 int getFinalStateComplex(int nangles, const double *angles, int nbasisvectors,
                          ANSATZ_DOUBLE_COMPLEX *finalstate, void *ctx);
 
@@ -86,7 +89,17 @@ int setHamiltonian(int n, const int *iindexes, const int *jindexes, const double
  * 3 not stored.  Replaces src/AnsatzSynthInterface.cpp:201-236. */
 int setInitialState(int numqubits, int n, const int *iindexes, const double *coeffs, void *ctx);
 
-/* Real build: returns 4 (src/AnsatzSynthInterface.cpp:286-298). */
+/* Complex start state: 1 ctx NULL, 2 n < 1, numqubits < 1 or a NULL
+ * array (the reference does not check the arrays), 3 rejected
+ * (as setInitialState).  The context then evaluates complex amplitudes, as
+ * the reference's complex build does: number operators (a^+_p a_p,
+ * a^+_p a^+_q a_p a_q) are allowed and multiply the states where they act
+ * by exp(+i angle); getFinalState returns the real part.  Energy and
+ * gradient entry points work as for real amplitudes; the Hessians, overlap
+ * gradients, batches, optimisers, subspace and property entry points and
+ * getEnergyGradientNoAlloc return their failure code.  A context whose
+ * operators include number operators is complex even with a real start.
+ * Replaces the complex build's src/AnsatzSynthInterface.cpp:251-285. */
 int setInitialStateComplex(int numqubits, int n, const int *iindexes, const ANSATZ_DOUBLE_COMPLEX *coeffs,
                            void *ctx);
 

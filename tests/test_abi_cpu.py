@@ -94,7 +94,18 @@ def test_set_hamiltonian_and_initial_codes(lib):
     assert lib.setInitialState(4, 0, ip, vp, c) == 2
     assert lib.setInitialState(0, 1, ip, vp, c) == 2
     assert lib.setInitialState(4, 1, ip, vp, c) == 0
-    assert lib.setInitialStateComplex(4, 1, ip, None, c) == 4
+    cz = (ctypes.c_double * 2)(0.6, 0.8)  # one complex amplitude
+    assert lib.setInitialStateComplex(4, 1, ip, cz, None) == 1
+    assert lib.setInitialStateComplex(4, 0, ip, cz, c) == 2
+    assert lib.setInitialStateComplex(0, 1, ip, cz, c) == 2
+    assert lib.setInitialStateComplex(4, 1, ip, None, c) == 2
+    assert lib.setInitialStateComplex(4, 1, ip, cz, c) == 0
+    fz = (ctypes.c_double * 32)()
+    assert lib.getFinalStateComplex(1, vp, 16, fz, None) == 1
+    assert lib.getFinalStateComplex(1, None, 16, fz, c) == 2
+    assert lib.getFinalStateComplex(1, vp, 16, None, c) == 3
+    assert lib.getFinalStateComplex(1, vp, 0, fz, c) == 4
+    assert lib.getFinalStateComplex(0, vp, 16, fz, c) == 5
     assert lib.setNuclearEnergy(1.0, None) == 1
     assert lib.setNuclearEnergy(1.0, c) == 0
     assert lib.setHamiltonianStorage(7, c) == 2
@@ -115,7 +126,7 @@ def test_getter_argument_codes(lib):
     assert lib.getFinalState(2, ap, 16, None, c) == 3
     assert lib.getFinalState(2, ap, 0, outp, c) == 4
     assert lib.getFinalState(0, ap, 16, outp, c) == 5
-    assert lib.getFinalStateComplex(2, ap, 16, None, c) == 4
+    assert lib.getFinalStateComplex(2, ap, 16, None, c) == 3  # codes as getFinalState
     for fn in (lib.getGradient_COMP, lib.getHessian_COMP):
         assert fn(2, ap, outp, None) == 1
         assert fn(2, None, outp, c) == 2
