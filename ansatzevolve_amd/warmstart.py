@@ -388,6 +388,28 @@ def stage_evaluate(counts, out_dir, max_evals=6000, log=_log, model_dir=None):
     return summary
 
 
+def sample(n=8, ntrain=24, nheld=8, gen_evals=1500, eval_evals=3000, epochs=200, log=None):
+    """A small in-memory run of the whole pipeline on one orbital count
+    (the benchmark's warm-start line): train on ntrain instances, evaluate
+    on nheld; returns the summary over the held-out instances and the
+    settings."""
+    counts = {n: (ntrain, nheld)}
+    train, held = instances(counts)
+    t0 = time.time()
+    gen, _ = generate(train, max_evals=gen_evals, log=log or (lambda m: None))
+    targets = [canonical_angles(g[0], w["n"], w["n"] // 2) for w, g in zip(train, gen)]
+    model = SlotModel()
+    model.fit(train, targets, epochs=epochs)
+    starts, E_std, E_prop, used, judge_s = propose(model, held)
+    model.close()
+    t_eval = egrad_seconds([n])
+    res = run_batched(held + held, [np.zeros(w["P"]) for w in held] + starts, eval_evals, log=log,
+                      stall=EVAL_STALL, stall_window=EVAL_WINDOW)
+    rows = rows_of(held, res[:nheld], res[nheld:], E_std, E_prop, used, t_eval, judge_s)
+    return dict(summary=summarise(rows)["all"], orbitals=n, train=ntrain, held_out=nheld, generate_max_evals=gen_evals,
+                eval_max_evals=eval_evals, epochs=epochs, seconds=time.time() - t0)
+
+
 def _main():
     import argparse
     p = argparse.ArgumentParser(description="learned warm start on per-instance seeded Hamiltonians")

@@ -136,3 +136,37 @@ def test_warm_start_continuation_targets(gpu):
     print("\nwarm start, continuation targets (4 orbitals, 12 held out): %s" % {k: a[k] for k in a})
     assert a["median_start_gap_chosen"] < a["median_start_gap_std"]
 
+
+
+def test_warmstart_generator_n8_fallback_rule(gpu):
+    """The seeded-family pipeline at 8 orbitals: batched L-BFGS targets,
+    canonical form, the width-256 network, proposals.  The fallback rule
+    holds for every held-out instance (the proposal is used exactly when its
+    exact energy is not above theta = 0's, and the chosen start's energy is
+    the lower of the two), and the batched start energies equal
+    single-context evaluations."""
+    from ansatzevolve_amd import from_workload
+    from ansatzevolve_amd.learned import canonical_angles
+    from ansatzevolve_amd.warmstart import SlotModel, generate, instances, propose
+
+    train, held = instances({8: (6, 6)})
+    assert all(w["n"] == 8 and w["P"] == 84 for w in train + held)
+    assert len({w["index"] for w in train + held}) == 12  # one seed per instance
+    gen, _ = generate(train, max_evals=150, log=lambda m: None)
+    for (x, E, ev, why), w in zip(gen, train):
+        assert ev <= 150 and np.isfinite(E)
+        with from_workload(w) as a:
+            assert E <= a.energy(np.zeros(w["P"])) + 1e-12  # descent from the standard guess
+            assert abs(a.energy(canonical_angles(x, 8, 4)) - E) <= 1e-10  # canonical form keeps the energy
+    model = SlotModel()
+    model.fit(train, [canonical_angles(g[0], 8, 4) for g in gen], epochs=20)
+    props = model.predict(held)
+    starts, E_std, E_prop, used, _ = propose(model, held)
+    model.close()
+    for w, p, s, es, ep, u in zip(held, props, starts, E_std, E_prop, used):
+        with from_workload(w) as a:
+            assert abs(a.energy(np.zeros(w["P"])) - es) <= 1e-12
+            assert abs(a.energy(p) - ep) <= 1e-12
+            assert u == (ep <= es)
+            assert np.array_equal(s, p if u else np.zeros(w["P"]))
+            assert abs(a.energy(s) - min(es, ep)) <= 1e-12
