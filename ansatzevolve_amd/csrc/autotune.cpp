@@ -136,20 +136,27 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
 
     std::vector<Candidate> cands;
     const bool sz = eng.mode() == kSz;
-    const int sweeps[3] = {kSweepOrbit, kSweepOrbitLds, kSweepTile};
+    const int sweeps[4] = {kSweepOrbit, kSweepOrbitLds, kSweepTile, kSweepOrbit2};
+    // the lane layout segments kSweepOrbit and kSweepOrbit2 alike; its
+    // candidates keep the engine's choice between the two for the
+    // vector-major (single-vector) sweeps
+    const int laneOrbit = eng.option(kOptSweep) == kSweepOrbit2 ? kSweepOrbit2 : kSweepOrbit;
     const int depths[3] = {0, 2, 1};
     const int gr = eng.option(kOptSigmaGroupedAvailable) == 1 ? 1 : 0;  // grouped sigma where it applies
     for (int lanes = 0; lanes < (nvec > 1 ? 2 : 1); lanes++)
-        for (int s : sweeps) {
-            if (!sz && s != kSweepOrbit) continue;          // other sectors sweep one operator per pass
-            if (lanes && s == kSweepOrbitLds) continue;     // the lane layout keeps orbits in registers
+        for (int s0 : sweeps) {
+            if (!sz && s0 != kSweepOrbit) continue;          // other sectors sweep one operator per pass
+            if (lanes && (s0 == kSweepOrbitLds || s0 == kSweepOrbit2)) continue;  // orbits in registers
+            if (!lanes && s0 == kSweepOrbit2 && eng.option(kOptSweepTwoGroupAvailable) != 1) continue;
+            const int s = lanes && s0 == kSweepOrbit ? laneOrbit : s0;
             for (int d : depths) {
                 if (!sz && d != 0) continue;
+                if (s0 == kSweepOrbit2 && d != 0) continue;
                 cands.push_back({s, d, lanes ? 1 : 0, 1, 1, 0, 0, gr});
                 // the lane layout can also fuse two disjoint groups per pass
                 // (forward only, or both sweeps), keep the forward states for
                 // the gradient, and use the other sigma variants where they apply
-                if (lanes && sz && s == kSweepOrbit && d == 0) {
+                if (lanes && sz && s0 == kSweepOrbit && d == 0) {
                     cands.push_back({s, d, 1, 1, 2, 0, 0, gr});
                     cands.push_back({s, d, 1, 2, 2, 0, 0, gr});
                     cands.push_back({s, d, 1, 1, 1, 1, 0, gr});

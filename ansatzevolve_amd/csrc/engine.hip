@@ -185,6 +185,13 @@ struct EngineImpl {
         DBuf<unsigned long long> prog3;   // three-group segments: run-2 words
     };
     LaneList laneA, laneF;
+    // vector-major two-group segmentation (kSweepOrbit2): two-group segments
+    // where operators pair up on disjoint orbital pairs, one-group orbit
+    // segments and single operators otherwise; used when it has some
+    // two-group segment (sweepS)
+    LaneList laneS;
+    bool sweepS = false;
+    bool use_s() const { return sweepKind == kSweepOrbit2 && sweepS; }
     std::vector<Seg> &lsegs = laneA.segs;
     std::vector<std::unique_ptr<GroupSet>> gsets;
     int maxGroups = 1;   // adjoint: groups per spin of a lane-layout orbit segment (1 to 3; tuned)
@@ -511,6 +518,19 @@ struct EngineImpl {
     {
         supDirty = true;
         if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
+        sweepS = false;
+        laneS.segs.clear();
+        if (mode == kSz && sweepKind != kSweepTile) {
+            if (!build_lane_list(laneS, 2, err)) return false;
+            sweepS = false;
+            for (const Seg &sg : laneS.segs) {
+                if (sg.three) {
+                    sweepS = false;
+                    break;
+                }
+                sweepS = sweepS || sg.two;
+            }
+        }
         // Every launch is one grid-stride pass of exactly as many blocks as
         // the chip holds at once for that kernel (its occupancy x CUs): a
         // grid a little larger than that leaves a second, sparsely occupied
@@ -1369,9 +1389,36 @@ struct EngineImpl {
     }
 
     // Forward sweep of nvec vectors with per-vector angle tables.
+    // k_orb2s grid of a vector-major two-group segment (<= slots)
+    int s_grid(const Seg &sg) const
+    {
+        return (int)std::max<long long>(1, std::min<long long>((sg.orbits + kBlock - 1) / kBlock, slots));
+    }
+
     bool sweep_forward(double *v, int nvec, const double2 *cs, std::string &err)
     {
         const int mk = mark_begin(1);
+        if (use_s()) {
+            for (const Seg &sg : laneS.segs) {
+                if (sg.two) {
+                    Orb2Args oa = orb2args(laneS, sg, false);
+                    oa.list = oa.llist = nullptr;
+                    hipLaunchKernelGGL((k_orb2s<false>), dim3(s_grid(sg), nvec), dim3(kBlock), 0, stream, oa, v,
+                                       (double *)nullptr, N, cs, M, (double *)nullptr, slots);
+                } else if (sg.orbA >= 0) {
+                    hipLaunchKernelGGL((k_orb<false>), dim3(std::min(orb_grid(sg), slots), nvec), dim3(kBlock), 0,
+                                       stream, orbargs(sg, false, laneS.lprog.p), v, nullptr, N, cs, (long long)M,
+                                       nullptr, 0, 0);
+                } else {
+                    for (int k = sg.k0; k < sg.k1; k++)
+                        hipLaunchKernelGGL((k_op_generic<false>), dim3(Ggen, nvec), dim3(kBlock), 0, stream, v,
+                                           nullptr, N, N, sec(), d_ops.p, k, cs, (long long)M, nullptr, 0, 0);
+                }
+                LCHK();
+            }
+            mark_end(mk);
+            return true;
+        }
         for (const Seg &sg : segs) {
             if (sg.diag) {
                 hipLaunchKernelGGL((k_phase<false>), dim3(Ggen, nvec / 2), dim3(kBlock), 0, stream, v, nullptr, N,
@@ -1406,6 +1453,28 @@ struct EngineImpl {
                            dim3(kBlock), 0, stream, partial, total, 0.0);
         LCHK();
         const int mk = mark_begin(2);
+        if (use_s()) {
+            for (int si = (int)laneS.segs.size() - 1; si >= 0; si--) {
+                const Seg &sg = laneS.segs[si];
+                if (sg.two) {
+                    Orb2Args oa = orb2args(laneS, sg, true);
+                    oa.list = oa.llist = nullptr;
+                    hipLaunchKernelGGL((k_orb2s<true>), dim3(s_grid(sg), nvec), dim3(kBlock), 0, stream, oa, phi,
+                                       lam, N, csneg, M, partial, slots);
+                } else if (sg.orbA >= 0) {
+                    hipLaunchKernelGGL((k_orb<true>), dim3(std::min(orb_grid(sg), slots), nvec), dim3(kBlock), 0,
+                                       stream, orbargs(sg, true, laneS.lprog.p), phi, lam, N, csneg, (long long)M,
+                                       partial, slots, pstride);
+                } else {
+                    for (int k = sg.k1 - 1; k >= sg.k0; k--)
+                        hipLaunchKernelGGL((k_op_generic<true>), dim3(Ggen, nvec), dim3(kBlock), 0, stream, phi, lam,
+                                           N, N, sec(), d_ops.p, k, csneg, (long long)M, partial, slots, pstride);
+                }
+                LCHK();
+            }
+            mark_end(mk);
+            return true;
+        }
         for (int si = (int)segs.size() - 1; si >= 0; si--) {
             const Seg &sg = segs[si];
             if (sg.diag) {
@@ -1783,7 +1852,7 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
     if (const char *g = getenv("AE_GRAPH")) e.useGraph = std::string(g) != "0";
     if (const char *sw = getenv("AE_SWEEP")) {
         const std::string v(sw);
-        e.sweepKind = v == "tile" ? kSweepTile : v == "orblds" ? kSweepOrbitLds : kSweepOrbit;
+        e.sweepKind = v == "tile" ? kSweepTile : v == "orblds" ? kSweepOrbitLds : v == "orbit2" ? kSweepOrbit2 : kSweepOrbit;
     }
     if (const char *l = getenv("AE_LANES")) e.laneMin = std::max(0, atoi(l));
     if (const char *r = getenv("AE_REACH")) e.reachOpt = atoi(r) ? 1 : 0;
@@ -1849,6 +1918,11 @@ bool Engine::init(int nq, int mode, int nel, int nup, int ndown, std::string &er
     if (!e.d_states.ensure(e.N, err)) return false;
     HCHK(hipMemcpy(e.d_states.p, e.states.data(), 4 * e.N, hipMemcpyHostToDevice));
     e.Ggen = (int)std::min<long long>((e.N + kBlock - 1) / kBlock, 1024);
+    // vector-major sweeps: two TUPS units per pass (half the launches) where
+    // a pass is long enough to outweigh the heavier thread-per-orbit kernel
+    // (measured, one vector: 853,776 determinants 3.11 -> 2.49 ms per E+grad;
+    // 63,504: 0.78 -> 0.83 ms)
+    if (!getenv("AE_SWEEP")) e.sweepKind = (mode == kSz && e.N >= (1LL << 18)) ? kSweepOrbit2 : kSweepOrbit;
     if (!e.d_psi.ensure(e.N, err) || !e.d_phi.ensure(e.N, err) || !e.d_lam.ensure(e.N, err) ||
         !e.d_start.ensure(e.N, err))
         return false;
@@ -1993,9 +2067,12 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
     e.slots = std::max(maxTiles, e.Ggen);
     if (!e.build_orbit_programs(err)) return false;
     if (e.cplx) {
-        // the lane layout (batches) is not used with complex amplitudes
+        // the lane layout (batches) and the two-group vector-major segments
+        // are not used with complex amplitudes
         e.laneA.segs.clear();
         e.laneF.segs.clear();
+        e.laneS.segs.clear();
+        e.sweepS = false;
     } else if (!e.build_lane_segments(err)) {
         return false;
     }
@@ -2777,7 +2854,7 @@ bool Engine::set_option(int key, int value, std::string &err)
     EngineImpl &e = *d_;
     switch (key) {
     case kOptSweep:
-        if (value < kSweepOrbit || value > kSweepTile) {
+        if (value < kSweepOrbit || value > kSweepOrbit2) {
             err = "unknown sweep kind";
             return false;
         }
@@ -2912,6 +2989,7 @@ int Engine::option(int key) const
     const EngineImpl &e = *d_;
     switch (key) {
     case kOptSweep: return e.sweepKind;
+    case kOptSweepTwoGroupAvailable: return e.sweepS ? 1 : 0;
     case kOptLaneMin: return e.laneMin;
     case kOptGraph: return e.useGraph ? 1 : 0;
     case kOptFuseDepth: return e.fuseDepth;

@@ -19,8 +19,12 @@ enum HamStorage : int { kHamAuto = 0, kHamExplicit = 1, kHamMatrixFree = 2 };
 //   kSweepOrbit:    runs of operators touching <= 2 alpha and <= 2 beta
 //                   orbitals as register orbits, the rest as LDS tiles;
 //   kSweepOrbitLds: the same, orbit segments staged through LDS rows;
-//   kSweepTile:     LDS tiles only.
-enum SweepKind : int { kSweepOrbit = 0, kSweepOrbitLds = 1, kSweepTile = 2 };
+//   kSweepTile:     LDS tiles only;
+//   kSweepOrbit2:   two TUPS units on disjoint orbital pairs per pass,
+//                   thread per orbit (k_orb2s), when every operator falls
+//                   into such segments (else as kSweepOrbit).
+// Batches in the lane layout use their own segmentation (kOptLaneGroups).
+enum SweepKind : int { kSweepOrbit = 0, kSweepOrbitLds = 1, kSweepTile = 2, kSweepOrbit2 = 3 };
 
 // Execution choices (Engine::set_option); none changes a result beyond
 // floating-point summation order.
@@ -53,6 +57,7 @@ enum EngineOption : int {
     kOptSweepAmpForward = 19,        // read and written by the forward sweep
     kOptSweepAmpAdjointLambda = 20,  // of lambda read and written by the adjoint sweep
     kOptSweepAmpAdjointState = 21,   // of the forward state read by the adjoint sweep
+    kOptSweepTwoGroupAvailable = 22,  // read only: 1 when kSweepOrbit2 has two-group segments to run
 };
 
 class Engine {

@@ -233,6 +233,15 @@ struct Orb2Args {
 template <bool BR, bool BC = false, bool LV = false>
 __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__restrict__ v1, long long cstride,
                         const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
+// The same two-group segments on vector-major vectors (grid.y = vector,
+// vstride apart; angle table cs[v*M + k]; in place): thread per orbit, its
+// up to 16 members gathered into registers.  Each thread's orbit has its own
+// type, so the operator words differ across a wave.  BR: v1 holds lambda,
+// rotated alike; each operator's bracket is reduced per block into
+// partial[(v*M + k)*slots + block] (grid.x <= slots).
+template <bool BR>
+__global__ void k_orb2s(Orb2Args a, double *v0, double *v1, long long vstride, const double2 *__restrict__ cs, int M,
+                        double *__restrict__ partial, int slots);
 // Three-group orbit segment in lane layout: three consecutive runs of
 // operators on pairwise disjoint orbital groups (three TUPS units on
 // disjoint spatial pairs).  Classes have up to 8 members per spin; ranks are

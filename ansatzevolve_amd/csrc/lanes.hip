@@ -461,6 +461,123 @@ template __global__ void k_orb2v<true, false, false>(Orb2Args, const double *, d
 template __global__ void k_orb2v<true, false, true>(Orb2Args, const double *, double *, double *, long long,
                                                     const double2 *, int, double *, int);
 
+// ------------------------------------------- vector-major two-group segments
+template <bool BR, int G, int H>
+__device__ __forceinline__ void orb2s_op(const Orb2Args &a, const unsigned long long *pw, int kk, uint32_t spect,
+                                         double2 t, double (&x)[16], double (&l)[16], unsigned cmask, double &acc)
+{
+    const unsigned w = (unsigned)pw[kk] & 0xfffu;
+    if (!w) return;
+    const bool neg = __popc(spect & a.ops[a.k0 + kk].mask) & 1;
+    double br = 0;
+    orb2_apply_uniform<G, H>(w, neg, t.x, t.y, x, l, BR, br, (cmask >> (4 * G)) & 0xfu);
+    if (BR) acc += br;
+}
+
+template <bool BR>
+__global__ void __launch_bounds__(kBlock)
+    k_orb2s(Orb2Args a, double *v0, double *v1, long long vstride, const double2 *__restrict__ cs, int M,
+            double *__restrict__ partial, int slots)
+{
+    constexpr int K = kOrb2MaxOps;
+    __shared__ double red[kBlock / 64];
+    const int vb = blockIdx.y;
+    v0 += vb * vstride;
+    if (BR) v1 += vb * vstride;
+    const int nk = a.k1 - a.k0, m = a.split;
+    double2 t[K];
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) t[kk] = kk < nk ? cs[(long long)vb * M + a.k0 + kk] : make_double2(1.0, 0.0);
+    double acc[K];
+#pragma unroll
+    for (int kk = 0; kk < K; kk++) acc[kk] = 0;
+    // orbit index space as in k_orb2v; the lone determinants (single-member
+    // alpha and beta classes) are left alone: nothing acts on them
+    const int mainA = a.nAm * a.nBc;
+    const int total = mainA + (a.nAc - a.nAm) * a.nBm;
+    for (int tt = blockIdx.x * kBlock + threadIdx.x; tt < total; tt += gridDim.x * kBlock) {
+        int ia, ib;
+        if (tt < mainA) {
+            ia = tt / a.nBc;
+            ib = tt - ia * a.nBc;
+        } else {
+            const int u = tt - mainA, i = u / a.nBm;
+            ia = a.nAm + i;
+            ib = u - i * a.nBm;
+        }
+        const int4 ra = a.aRanks[ia], rb = a.bRanks[ib];
+        const uint32_t fa = a.aInfo[ia], fb = a.bInfo[ib];
+        const int rA[4] = {ra.x, ra.y, ra.z, ra.w}, rB[4] = {rb.x, rb.y, rb.z, rb.w};
+        const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
+        const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
+        if (na_ * nb_ < 2) continue;
+        const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+        const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        unsigned cmask;
+        {
+            const bool a2 = ((fa >> 26) & 3) == 0, b2 = ((fb >> 26) & 3) == 0;
+            const bool a1 = ((fa >> 24) & 3) == 0, b1 = ((fb >> 24) & 3) == 0;
+            cmask = 1u | (a2 ? 2u : 0u) | (b2 ? 4u : 0u) | (a2 && b2 ? 8u : 0u) |
+                    ((1u | (a1 ? 2u : 0u) | (b1 ? 4u : 0u) | (a1 && b1 ? 8u : 0u)) << 4);
+        }
+        double x[16], l[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ma = q & 3, mb = q >> 2;
+            const bool ok = rA[ma] >= 0 && rB[mb] >= 0;
+            const long long off = (long long)rA[ma] * a.nb + rB[mb];
+            x[q] = ok ? v0[off] : 0.0;
+            l[q] = (BR && ok) ? v1[off] : 0.0;
+        }
+        const unsigned long long *pw = a.prog + type * K;
+        if (!BR) {
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                if (kk >= m) break;
+                orb2s_op<BR, 0, 0>(a, pw, kk, spect, t[kk], x, l, cmask, acc[kk]);
+            }
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                if (kk < m || kk >= nk) continue;
+                orb2s_op<BR, 1, 1>(a, pw, kk, spect, t[kk], x, l, cmask, acc[kk]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int kk = K - 1 - j;
+                if (kk < m || kk >= nk) continue;
+                orb2s_op<BR, 1, 1>(a, pw, kk, spect, t[kk], x, l, cmask, acc[kk]);
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int kk = K - 1 - j;
+                if (kk >= m) continue;
+                orb2s_op<BR, 0, 0>(a, pw, kk, spect, t[kk], x, l, cmask, acc[kk]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ma = q & 3, mb = q >> 2;
+            if (rA[ma] >= 0 && rB[mb] >= 0) {
+                const long long off = (long long)rA[ma] * a.nb + rB[mb];
+                v0[off] = x[q];
+                if (BR) v1[off] = l[q];
+            }
+        }
+    }
+    if (BR) {
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            if (kk >= nk) break;
+            const double tot = block_sum(acc[kk], red);
+            if (threadIdx.x == 0) partial[((long long)vb * M + a.k0 + kk) * slots + blockIdx.x] = tot;
+        }
+    }
+}
+
+template __global__ void k_orb2s<false>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+template __global__ void k_orb2s<true>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
+
 // ------------------------------------------------ three-group segments
 // f(Idx<I>) for I = B .. E-1, written out at compile time (no loop)
 template <int I>

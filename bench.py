@@ -109,8 +109,7 @@ def host_cpu_share():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the design for "
-              "sharding is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from ansatzevolve_amd import from_workload, load_library
     from ansatzevolve_amd.workloads import make_workload

@@ -344,7 +344,7 @@ def test_options_and_fusion_depth(gpu):
     w, thetas = _tups_case(6, 2, 31, 20)
     with from_workload(w) as a:
         base = a.energy_gradients(thetas)
-        for sweep, depth, lanes, groups in itertools.product((0, 1, 2), (0, 1, 2, 3, 7), (0, 1), (1, 2)):
+        for sweep, depth, lanes, groups in itertools.product((0, 1, 2, 3), (0, 1, 2, 3, 7), (0, 1), (1, 2)):
             a.set_option(AnsatzSynth.OPT_SWEEP, sweep)
             a.set_option(AnsatzSynth.OPT_FUSE_DEPTH, depth)
             a.set_option(AnsatzSynth.OPT_LANE_GROUPS, groups)
@@ -509,3 +509,36 @@ def test_autotune_rejects_nothing_and_reports_tolerance(gpu):
         if rep["choice"]["lane_min"] == 1:
             for part in ("beta", "alpha"):
                 assert rep["sigma_gemms"][part]["rejected"] == 0
+
+
+@pytest.mark.parametrize("n,layers", [(5, 3), (6, 2), (8, 2)])
+def test_two_group_vector_major_sweeps(gpu, n, layers):
+    """Sweep kind 3 (two TUPS units per pass, thread per orbit; one-group and
+    single-operator passes where units do not pair up, as with 5 orbital
+    pairs) against the oracle: single-vector energy and gradient, the
+    Hessian, the overlap gradient and small vector-major batches."""
+    from oracle.oracle import OracleEngine
+    w, thetas = _tups_case(n, layers, 40 + n, 6)
+    o = OracleEngine(2 * n, [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
+    o.set_integrals(w["h"], w["g"], explicit=True)
+    with from_workload(w) as a:
+        a.set_option(AnsatzSynth.OPT_SWEEP, 3)
+        assert a.get_option(AnsatzSynth.OPT_SWEEP_TWO_GROUP_AVAILABLE) == 1
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 0)
+        for th in thetas[:3]:
+            E, g = a.energy_gradient(th)
+            E0, g0 = o.energy_gradient(th)
+            assert abs(E - E0) <= 1e-10 and np.max(np.abs(g - g0)) <= 1e-9
+        H0, _, _ = o.hessian(thetas[0])
+        assert np.max(np.abs(a.hessian(thetas[0]) - H0)) <= 1e-8
+        proj = np.zeros(1 << (2 * n))
+        st = o.states()
+        proj[st] = np.cos(np.arange(len(st)) * 0.37)
+        proj /= np.linalg.norm(proj)
+        ov, gr = a.overlap_gradient(thetas[1], proj)
+        ov0, gr0 = o.overlap_gradient(thetas[1], proj[st])
+        assert abs(ov - ov0) <= 1e-12 and np.max(np.abs(gr - gr0)) <= 1e-10
+        E, G = a.energy_gradients(thetas)
+        for v in range(len(thetas)):
+            E0, g0 = o.energy_gradient(thetas[v])
+            assert abs(E[v] - E0) <= 1e-10 and np.max(np.abs(G[v] - g0)) <= 1e-9
