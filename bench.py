@@ -53,6 +53,12 @@ def parse():
                    help="keep the library's default sweep kind, fusion depth and batch layout")
     p.add_argument("--lane-groups", type=int, default=0,
                    help="force the lane-layout orbit groups per pass (1 or 2) after autotuning")
+    p.add_argument("--autotune-cache", default="",
+                   help="autotuner record file: a recorded choice for this shape is re-applied without measuring, "
+                        "a measured one is appended")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the Hessian timings and the multi-instance launch comparison reported after the "
+                        "timed region")
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain 2 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -94,6 +100,66 @@ print("RESULT " + json.dumps({"times": times, "first": t_first, "setup": setup, 
         if line.startswith("RESULT "):
             return json.loads(line[7:])
     raise RuntimeError("reference CPU run failed: " + r.stderr[-2000:])
+
+
+def extra_measurements(a, w):
+    """After the timed region: per-parameter Hessian wall time (one call
+    after a warm-up) at the benchmark configuration and at 10 orbitals, and
+    32 independent 8-orbital instances evaluated (E + gradient, one angle
+    vector each) as one multi-instance graph launch against one
+    single-instance call after another."""
+    from ansatzevolve_amd import MultiBatch, from_workload
+    from ansatzevolve_amd.learned import warm_instance
+    from ansatzevolve_amd.workloads import make_workload
+    out = {}
+    hs = {}
+    for cfg, ctx, wl in ((3, a, w), (2, None, None)):
+        own = ctx is None
+        if own:
+            wl = make_workload(cfg)
+            ctx = from_workload(wl)
+        ctx.hessian(wl["theta"])
+        t = time.perf_counter()
+        ctx.hessian(wl["theta"] + 1e-3)
+        hs["config%d" % cfg] = {"ms": 1e3 * (time.perf_counter() - t), "parameters": wl["P"],
+                                "determinants": ctx.dimensions()[0]}
+        if own:
+            ctx.close()
+    out["hessian"] = hs
+    insts = [warm_instance(i, 8, "random") for i in range(32)]
+    ctxs = [from_workload(x) for x in insts]
+    try:
+        th = [np.full(x["P"], 0.05) for x in insts]
+        for c, t0 in zip(ctxs, th):
+            c.energy_gradient(t0)
+        K = 20
+        t = time.perf_counter()
+        for r in range(K):
+            for c, t0 in zip(ctxs, th):
+                c.energy_gradient(t0 + 1e-4 * r)
+        seq = time.perf_counter() - t
+        worst = 0.0
+        with MultiBatch(ctxs) as mb:
+            for r in range(K + 2):
+                if r == 2:
+                    t = time.perf_counter()
+                for i, t0 in enumerate(th):
+                    mb.upload(i, (t0 + 1e-4 * r)[None, :])
+                mb.run(True)
+                res = [mb.download(i) for i in range(len(ctxs))]
+            multi = time.perf_counter() - t
+            for i, (c, t0) in enumerate(zip(ctxs, th)):  # last round against single calls
+                E, g = c.energy_gradient(t0 + 1e-4 * (K + 1))
+                worst = max(worst, abs(float(res[i][0][0]) - E), float(np.max(np.abs(res[i][1][0] - g))))
+        out["multi_instance"] = {
+            "instances": "32 seeded 8-orbital Hamiltonians (4,900 determinants, 84 parameters), one angle vector "
+                         "each, energy + gradient",
+            "one_launch_egrad_per_s": 32 * K / multi, "one_after_another_egrad_per_s": 32 * K / seq,
+            "max_diff_vs_single": worst}
+    finally:
+        for c in ctxs:
+            c.close()
+    return out
 
 
 def host_cpu_share():
@@ -183,8 +249,9 @@ def main():
     tune = None
     if not args.no_autotune:
         print("[bench] autotuning the batch (sweep kind x fusion depth x layout)", file=sys.stderr, flush=True)
-        rep = a.autotune(thetas, reps=2)
-        tune = {"choice": rep["choice"], "ms": rep.get("ms"), "max_dev": rep.get("max_dev"),
+        rep = a.autotune(thetas, cache=args.autotune_cache or None, reps=2)
+        tune = {"choice": rep["choice"], "source": rep.get("source"), "ms": rep.get("ms"),
+                "max_dev": rep.get("max_dev"),
                 "sigma_gemms": rep.get("sigma_gemms"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
                                 c["choice"]["groups"], c["choice"]["groups_fwd"], c["choice"]["checkpoint"],
@@ -417,6 +484,12 @@ def main():
             }
         except Exception as ex:  # report, do not hide the GPU number
             result["cpu_baseline"] = {"error": str(ex)[:300]}
+    if not args.no_extras:
+        print("[bench] Hessian timings and the multi-instance launch", file=sys.stderr, flush=True)
+        try:
+            result.update(extra_measurements(a, w))
+        except Exception as ex:
+            result["extras_error"] = str(ex)[:300]
     a.close()
     result["warm_start"] = None
     if not args.no_warm_start:
