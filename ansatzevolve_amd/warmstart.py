@@ -326,11 +326,52 @@ def parse_counts(text):
     return out
 
 
+def stage_generate(counts, out_dir, only, gen_evals=3000, log=_log):
+    """Targets of the training instances of the orbital counts in `only`
+    (batched L-BFGS from theta = 0), written to targets_n<n>.npz per count,
+    so that large runs can generate one orbital count per process."""
+    train, _ = instances(counts)
+    os.makedirs(out_dir, exist_ok=True)
+    for n in sorted(only):
+        part = [w for w in train if w["n"] == n]
+        gen, gen_s = generate(part, max_evals=gen_evals, log=log)
+        np.savez_compressed(os.path.join(out_dir, "targets_n%d.npz" % n),
+                            index=np.array([w["index"] for w in part]),
+                            angles=np.array([g[0] for g in gen]), energy=np.array([g[1] for g in gen]),
+                            evals=np.array([g[2] for g in gen]), reason=np.array([g[3] for g in gen]),
+                            seconds=np.array(gen_s), max_evals=np.array(gen_evals))
+        log("[warmstart] %d targets at %d orbitals (%.0f s)" % (len(part), n, gen_s))
+
+
+def stage_fit(counts, out_dir, target_dir=None, epochs=400, log=_log):
+    """The network trained on the targets stage_generate wrote (all orbital
+    counts of `counts`); writes model.npz and train.json into out_dir."""
+    train, held = instances(counts)
+    target_dir = target_dir or out_dir
+    gen, gen_s, gen_cap = [], 0.0, 0
+    for n in sorted(counts):
+        z = np.load(os.path.join(target_dir, "targets_n%d.npz" % n), allow_pickle=False)
+        part = [w for w in train if w["n"] == n]
+        if [w["index"] for w in part] != list(z["index"]):
+            raise ValueError("targets_n%d.npz does not match the training instances" % n)
+        gen += [(z["angles"][i], float(z["energy"][i]), int(z["evals"][i]), str(z["reason"][i]))
+                for i in range(len(part))]
+        gen_s += float(z["seconds"])
+        gen_cap = int(z["max_evals"])
+    # instances() lists the training instances by ascending orbital count,
+    # as the targets were concatenated
+    return _fit(counts, train, held, gen, gen_s, gen_cap, out_dir, epochs, 0, log)
+
+
 def stage_train(counts, out_dir, gen_evals=3000, epochs=400, energy_steps=0, log=_log):
     """Targets (batched L-BFGS from theta = 0, canonical form) and the trained
     network; writes model.npz and train.json into out_dir."""
     train, held = instances(counts)
     gen, gen_s = generate(train, max_evals=gen_evals, log=log)
+    return _fit(counts, train, held, gen, gen_s, gen_evals, out_dir, epochs, energy_steps, log)
+
+
+def _fit(counts, train, held, gen, gen_s, gen_evals, out_dir, epochs, energy_steps, log):
     targets = [canonical_angles(g[0], w["n"], w["n"] // 2) for w, g in zip(train, gen)]
     t0 = time.time()
     model = SlotModel()
@@ -355,16 +396,19 @@ def stage_train(counts, out_dir, gen_evals=3000, epochs=400, energy_steps=0, log
     return info
 
 
-def stage_evaluate(counts, out_dir, max_evals=6000, log=_log, model_dir=None):
+def stage_evaluate(counts, out_dir, max_evals=6000, log=_log, model_dir=None, only=None):
     """Held-out comparison: proposals, fallback rule, then batched L-BFGS from
     the chosen starts and from theta = 0 together (one multi-instance launch
-    per round over both); writes eval.json into out_dir."""
+    per round over both); writes eval.json into out_dir (eval_n<n>.json
+    with `only` = one orbital count, for stage_merge)."""
     train, held = instances(counts)
+    if only:
+        held = [w for w in held if w["n"] in only]
     model_dir = model_dir or out_dir
     model = load_model(os.path.join(model_dir, "model.npz"))
     starts, E_std, E_prop, used, judge_s = propose(model, held)
     model.close()
-    t_eval = egrad_seconds(sorted(counts))
+    t_eval = egrad_seconds(sorted({w["n"] for w in held}))
     log("[warmstart] single-instance E+grad seconds %s; proposals used %d of %d"
         % ({n: round(t, 5) for n, t in t_eval.items()}, sum(used), len(used)))
     t0 = time.time()
@@ -383,9 +427,35 @@ def stage_evaluate(counts, out_dir, max_evals=6000, log=_log, model_dir=None):
                                    "single-instance E+grad time of the instance's size; the learned start is "
                                    "also charged its proposal and exact-energy judgement",
                    summary=summarise(rows), rows=rows)
-    with open(os.path.join(out_dir, "eval.json"), "w") as f:
+    name = "eval.json" if not only else "eval_n%s.json" % "_".join(str(n) for n in sorted(only))
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, name), "w") as f:
         json.dump(summary, f)
     return summary
+
+
+def stage_merge(out_dir, parts):
+    """eval.json from per-orbital-count evaluations (files in `parts`)."""
+    rows, merged = [], None
+    for path in parts:
+        with open(path) as f:
+            d = json.load(f)
+        rows += d["rows"]
+        if merged is None:
+            merged = {k: v for k, v in d.items() if k not in ("rows", "summary")}
+            merged["eval_batch_s"] = 0.0
+            merged["egrad_seconds"] = {}
+            merged["judge_s_per_instance"] = {}
+        merged["eval_batch_s"] += d["eval_batch_s"]
+        merged["egrad_seconds"].update(d["egrad_seconds"])
+        for n in d["egrad_seconds"]:
+            merged["judge_s_per_instance"][n] = d["judge_s_per_instance"]
+    merged["held_out"] = len(rows)
+    merged["summary"] = summarise(rows)
+    merged["rows"] = rows
+    with open(os.path.join(out_dir, "eval.json"), "w") as f:
+        json.dump(merged, f)
+    return merged
 
 
 def sample(n=8, ntrain=24, nheld=8, gen_evals=1500, eval_evals=3000, epochs=200, log=None):
@@ -413,7 +483,8 @@ def sample(n=8, ntrain=24, nheld=8, gen_evals=1500, eval_evals=3000, epochs=200,
 def _main():
     import argparse
     p = argparse.ArgumentParser(description="learned warm start on per-instance seeded Hamiltonians")
-    p.add_argument("stage", choices=["train", "evaluate"])
+    p.add_argument("stage", choices=["train", "generate", "fit", "evaluate", "merge"],
+                   help="train = generate + fit for every orbital count in one process")
     p.add_argument("--counts", default="8:64:24,10:64:24,12:32:16",
                    help="n:training:held-out per orbital count")
     p.add_argument("--out-dir", default="gpurun_out/warmstart")
@@ -422,7 +493,15 @@ def _main():
     p.add_argument("--eval-evals", type=int, default=6000)
     p.add_argument("--epochs", type=int, default=400)
     p.add_argument("--energy-steps", type=int, default=0)
+    p.add_argument("--only", default="", help="generate / evaluate: these orbital counts only (comma list)")
+    p.add_argument("--target-dir", default="", help="fit: where targets_n<n>.npz are (default: out-dir)")
+    p.add_argument("parts", nargs="*", help="merge: the per-count eval_n<n>.json files")
     a = p.parse_args()
+    only = [int(x) for x in a.only.split(",")] if a.only else None
+    if a.stage == "merge":
+        s = stage_merge(a.out_dir, a.parts)
+        print(json.dumps({k: v for k, v in s.items() if k != "rows"}))
+        return
     from . import load_library
     load_library()
     counts = parse_counts(a.counts)
@@ -430,7 +509,13 @@ def _main():
         info = stage_train(counts, a.out_dir, a.gen_evals, a.epochs, a.energy_steps)
         print(json.dumps(info))
         return
-    s = stage_evaluate(counts, a.out_dir, a.eval_evals, model_dir=a.model_dir or None)
+    if a.stage == "generate":
+        stage_generate(counts, a.out_dir, only or sorted(counts), a.gen_evals)
+        return
+    if a.stage == "fit":
+        print(json.dumps(stage_fit(counts, a.out_dir, a.target_dir or None, a.epochs)))
+        return
+    s = stage_evaluate(counts, a.out_dir, a.eval_evals, model_dir=a.model_dir or None, only=only)
     print(json.dumps({k: v for k, v in s.items() if k != "rows"}))
 
 
