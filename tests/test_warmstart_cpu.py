@@ -94,3 +94,42 @@ def test_output_slots_align_units_across_orbital_counts():
     for key, i in u8.items():
         for k in range(3):
             assert s8[i + k] == s12[u12[key] + k]
+
+
+def test_evaluation_parts_merge_and_time_to_target(tmp_path):
+    """Host logic of the staged pipeline: the time to 1e-6 Ha of a run's own
+    final energy and of the lower one, misses, the summary per orbital
+    count, and merging per-count evaluations into one."""
+    import json
+    from ansatzevolve_amd.warmstart import rows_of, stage_merge, summarise
+
+    def trace(E_end, n_it):  # rows: seconds, energy, |g|_inf, evaluations
+        return np.array([[0.1 * i, E_end + 2.0 ** -i, 1.0, 3 * i + 1] for i in range(n_it)])
+
+    held = [dict(n=8, index=0, spacing=1.2), dict(n=8, index=1, spacing=2.0), dict(n=10, index=2, spacing=1.5)]
+    plain = [(None, -1.0, trace(-1.0, 30), "stalled"), (None, -2.0, trace(-2.0, 30), "stalled"),
+             (None, -3.0, trace(-3.0, 40), "stalled")]
+    learned = [(None, -1.0, trace(-1.0, 10), "stalled"), (None, -1.5, trace(-1.5, 30), "stalled"),
+               (None, -3.0, trace(-3.0, 20), "stalled")]
+    t_eval = {8: 0.001, 10: 0.002}
+    rows = rows_of(held, plain, learned, np.array([0.5, 0.5, 0.5]), np.array([0.4, 0.7, 0.3]),
+                   [True, False, True], t_eval, 0.01)
+    # first iterate within 1e-6 of the own final energy: 2^-i <= 1e-6 from i = 20
+    first = min(i for i in range(30) if 2.0 ** -i <= 1e-6)
+    assert rows[0]["ev_own_plain"] == 3 * first + 1
+    assert np.isinf(rows[0]["ev_own_mlp"])  # 10 iterates never get within 1e-6
+    assert np.isinf(rows[1]["t_best_mlp"]) and not np.isinf(rows[1]["t_best_plain"])  # higher minimum: a miss
+    assert rows[2]["t_own_mlp"] == rows[2]["ev_own_mlp"] * 0.002 + 0.01
+    s = summarise(rows)
+    assert s["all"]["count"] == 3 and s["n8"]["count"] == 2 and s["n10"]["count"] == 1
+    assert s["all"]["proposals_used"] == 2 and s["all"]["plain_lower_minimum"] == 1
+    parts = []
+    for n, sel in ((8, rows[:2]), (10, rows[2:])):
+        d = dict(counts={}, eval_batch_s=1.0, egrad_seconds={str(n): t_eval[n]}, judge_s_per_instance=0.01,
+                 summary=summarise(sel), rows=sel)
+        p = tmp_path / ("eval_n%d.json" % n)
+        p.write_text(json.dumps(d))
+        parts.append(str(p))
+    m = stage_merge(str(tmp_path), parts)
+    assert m["held_out"] == 3 and m["eval_batch_s"] == 2.0
+    assert json.loads((tmp_path / "eval.json").read_text())["summary"] == json.loads(json.dumps(s))
