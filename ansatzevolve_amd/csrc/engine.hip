@@ -191,6 +191,7 @@ struct EngineImpl {
     // two-group segment (sweepS)
     LaneList laneS;
     bool sweepS = false;
+    bool laneSAllTwo = false;  // every segment of laneS is a two-group one (Hessian tangents in lane layout)
     bool use_s() const { return sweepKind == kSweepOrbit2 && sweepS; }
     std::vector<Seg> &lsegs = laneA.segs;
     std::vector<std::unique_ptr<GroupSet>> gsets;
@@ -243,6 +244,9 @@ struct EngineImpl {
     DBuf<double> b_psi, b_lam, b_partial, b_epart, b_out;
     DBuf<double2> b_cs, b_csneg;
     DBuf<double> hz_B, hz_HB, hz_psi2, hz_lam2, hz_xpart, hz_dpart, hz_gram, hz_out, hz_epart;
+    DBuf<double> hz_lx, hz_ly, hz_lep;  // tangents, H B and energy partials in lane layout (Hessian, lane sigma)
+    DBuf<int> hz_inj;                   // Hessian tangent steps in lane layout: tangent position per operator
+    DBuf<double> hz_scl;                // and its scale
     DBuf<double2> sub_cs, sub_csneg;  // angle tables of subspace() (apart from the graph-captured batch tables)
 
     int bnvec = 0;  // size of the uploaded angle batch
@@ -518,12 +522,14 @@ struct EngineImpl {
     {
         supDirty = true;
         if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
-        sweepS = false;
+        sweepS = laneSAllTwo = false;
         laneS.segs.clear();
         if (mode == kSz && sweepKind != kSweepTile) {
             if (!build_lane_list(laneS, 2, err)) return false;
             sweepS = false;
+            laneSAllTwo = !laneS.segs.empty();
             for (const Seg &sg : laneS.segs) {
+                laneSAllTwo = laneSAllTwo && sg.two && !sg.three;
                 if (sg.three) {
                     sweepS = false;
                     break;
@@ -1380,6 +1386,18 @@ struct EngineImpl {
         return true;
     }
 
+    // source rows per phase of the vector-major split sigma (kOptSigma1Rows),
+    // reduced to what the device's LDS holds; 0 = the two-row kernel.
+    // Measured, one vector: 853,776 determinants 0.80 ms (two rows) ->
+    // 0.49 ms (8 rows); 11,778,624: 16.1 -> 10.8 ms (4 rows, the most that fit)
+    int sig1Rows = 8;
+    int sig1_rows() const
+    {
+        int rb = sig1Rows;
+        while (rb > 2 && (size_t)rb * (kSig1GS + nb) * sizeof(double) > ldsDevice) rb -= 2;
+        return rb;
+    }
+
     // An error for the operations that work on real amplitudes only.
     bool real_only(const char *what, std::string &err) const
     {
@@ -1665,8 +1683,11 @@ struct EngineImpl {
 
     // Lane-layout sigma for nch chunks and the per-vector energies
     // out[v] = <x_v|y_v> (matrix-free storage, n <= 12).
-    bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err)
+    // epart: 512 x nch x 64 energy partials (the batch's bl_epart when null)
+    bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err,
+                         double *epart = nullptr)
     {
+        if (!epart) epart = bl_epart.p;
         const int mk = mark_begin(0);
         SigvArgs s;
         s.m = mfargs();
@@ -1712,9 +1733,9 @@ struct EngineImpl {
         }
         mark_end(mk3);
         const int G = 512;
-        hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, bl_epart.p);
+        hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, epart);
         LCHK();
-        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, stream, bl_epart.p, 1, G, nullptr, nvec, out, 1.0);
+        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, stream, epart, 1, G, nullptr, nvec, out, 1.0);
         LCHK();
         return true;
     }
@@ -1764,7 +1785,18 @@ struct EngineImpl {
             s.x = x;
             s.y = y;
             s.vstride = N;
-            hipLaunchKernelGGL(k_sig1_sparse, dim3(na, nvec), dim3(kSig1Threads), kSig1Lds, stream, s);
+            const int rb = sig1_rows();
+            const size_t lds = (size_t)rb * (kSig1GS + nb) * sizeof(double);
+            if (rb == 8)
+                hipLaunchKernelGGL(k_sig1r<8>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
+            else if (rb == 6)
+                hipLaunchKernelGGL(k_sig1r<6>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
+            else if (rb == 4)
+                hipLaunchKernelGGL(k_sig1r<4>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
+            else if (rb == 2)
+                hipLaunchKernelGGL(k_sig1r<2>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
+            else
+                hipLaunchKernelGGL(k_sig1_sparse, dim3(na, nvec), dim3(kSig1Threads), kSig1Lds, stream, s);
             mark_end(mk);
             LCHK();
             mk = mark_begin(3);
@@ -2072,7 +2104,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         e.laneA.segs.clear();
         e.laneF.segs.clear();
         e.laneS.segs.clear();
-        e.sweepS = false;
+        e.sweepS = e.laneSAllTwo = false;
     } else if (!e.build_lane_segments(err)) {
         return false;
     }
@@ -2289,6 +2321,15 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
             if (e.sigh && !e.build_sigh_tables(err)) return false;
             HCHK(hipFuncSetAttribute((const void *)k_sig1_sparse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)kSig1Lds));
+            {
+                const void *fns[4] = {(const void *)k_sig1r<2>, (const void *)k_sig1r<4>,
+                                      (const void *)k_sig1r<6>, (const void *)k_sig1r<8>};
+                for (int q = 0; q < 4; q++) {
+                    const size_t lds = (size_t)(2 * q + 2) * (kSig1GS + e.nb) * sizeof(double);
+                    if (lds <= e.ldsDevice)
+                        HCHK(hipFuncSetAttribute(fns[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                }
+            }
             if (!e.blas) {
                 if (rocblas_create_handle(&e.blas) != rocblas_status_success) {
                     e.blas = nullptr;
@@ -2946,6 +2987,16 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.siggOpt = value;
         e.drop_graphs();
         return true;
+    case kOptSigma1Rows:
+        if (value != 0 && value != 2 && value != 4 && value != 6 && value != 8) {
+            err = "sigma rows per phase must be 0, 2, 4, 6 or 8";
+            return false;
+        }
+        if (value != e.sig1Rows) {
+            e.sig1Rows = value;
+            e.drop_graphs();
+        }
+        return true;
     case kOptReachable:
         if (value < 0 || value > 1) {
             err = "reachable option must be 0 or 1";
@@ -2990,6 +3041,7 @@ int Engine::option(int key) const
     switch (key) {
     case kOptSweep: return e.sweepKind;
     case kOptSweepTwoGroupAvailable: return e.sweepS ? 1 : 0;
+    case kOptSigma1Rows: return e.sig1Rows;
     case kOptLaneMin: return e.laneMin;
     case kOptGraph: return e.useGraph ? 1 : 0;
     case kOptFuseDepth: return e.fuseDepth;
@@ -3194,6 +3246,9 @@ bool Engine::hessian_proj(const double *angles, const double *proj, double *hess
 // term); otherwise f = <proj|psi> (lambda_0 = U^T proj, factor 1, no
 // such term; src/fusedevolve.cpp:2892-3183).  metric (optional, energy
 // only): g[mu][nu] = <dpsi/dtheta_mu|dpsi/dtheta_nu>.
+// tangent count from which the Hessian's H B runs through the lane-layout sigma
+constexpr int kHessLaneMin = 16;
+
 bool Engine::second_derivatives(const double *angles, const double *proj, double *hess, double *grad, double *val,
                                 double *metric, std::string &err)
 {
@@ -3236,24 +3291,71 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
         hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<size_t>((cnt + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0,
                            e.stream, p, (long long)cnt, 0.0);
     };
-    fill(e.hz_B.p, PN);
-    fill(e.hz_xpart.p, (size_t)M * P * G);
-    LCHK();
-    double *psiA = e.hz_psi2.p, *psiB = e.hz_psi2.p + N;
-    double *lamA = e.hz_lam2.p, *lamB = e.hz_lam2.p + N;
-    HCHK(hipMemcpyAsync(psiA, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
-    HCHK(hipMemcpyAsync(lamA, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
-    const DevSector sec = e.sec();
-    int nactive = 0;
-    for (int k = 0; k < M; k++) {
-        while (nactive < P && first[perm[nactive]] <= k) nactive++;
-        const int slices = (nactive + 3) / 4;
-        hipLaunchKernelGGL((k_tangent<4>), dim3(G, std::max(slices, 1)), dim3(kBlock), 0, e.stream, psiA, psiB, lamA,
-                           lamB, e.hz_B.p, N, nactive, N, sec, e.d_ops.p, k, e.d_cs.p, pos[e.order[k]],
-                           e.scale[k], e.hz_xpart.p + (size_t)k * P * G, G, e.hz_dpart.p + (size_t)k * G);
+    // 2. the tangents: in lane layout over two-group segments (one launch
+    // per segment for every tangent, psi and lambda advanced by the
+    // vector-major two-group kernel), else one launch per operator over the
+    // tangents active so far
+    const bool tanLanes = e.mode == kSz && e.laneSAllTwo && P >= kHessLaneMin;
+    const int nch = (P + kLanes - 1) / kLanes;
+    const int TS = std::max(1, e.numCU * 4);  // partial rows per operator of the lane tangent steps
+    const size_t L = (size_t)nch * N * kLanes;
+    if (tanLanes) {
+        if (!e.hz_lx.ensure(L, err) || !e.hz_xpart.ensure((size_t)nch * M * TS * kLanes, err) ||
+            !e.hz_dpart.ensure((size_t)M * TS, err) || !e.hz_inj.ensure(M, err) || !e.hz_scl.ensure(M, err))
+            return false;
+        std::vector<int> inj(M);
+        for (int k = 0; k < M; k++) inj[k] = pos[e.order[k]];
+        HCHK(hipMemcpy(e.hz_inj.p, inj.data(), sizeof(int) * M, hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(e.hz_scl.p, e.scale.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+        fill(e.hz_lx.p, L);
+        fill(e.hz_xpart.p, (size_t)nch * M * TS * kLanes);
+        fill(e.hz_dpart.p, (size_t)M * TS);
         LCHK();
-        std::swap(psiA, psiB);
-        std::swap(lamA, lamB);
+        double *psi = e.hz_psi2.p, *lam = e.hz_lam2.p;
+        HCHK(hipMemcpyAsync(psi, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+        HCHK(hipMemcpyAsync(lam, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+        for (const Seg &sg : e.laneS.segs) {
+            Orb2Args oa = e.orb2args(e.laneS, sg, false);
+            oa.list = oa.llist = nullptr;
+            TanArgs ta;
+            ta.psi = psi;
+            ta.lam = lam;
+            ta.inj = e.hz_inj.p;
+            ta.scale = e.hz_scl.p;
+            ta.xpart = e.hz_xpart.p;
+            ta.dpart = e.hz_dpart.p;
+            ta.slots = TS;
+            const int grid = (int)std::max<long long>(1, std::min<long long>((sg.orbits + 3) / 4, TS));
+            hipLaunchKernelGGL(k_orb2t, dim3(grid, nch), dim3(kBlock), 0, e.stream, oa, ta, e.hz_lx.p,
+                               (long long)N * kLanes, (const double2 *)e.d_cs.p, M);
+            LCHK();
+            hipLaunchKernelGGL((k_orb2s<false>), dim3(e.s_grid(sg), 1), dim3(kBlock), 0, e.stream, oa, psi,
+                               (double *)nullptr, N, (const double2 *)e.d_cs.p, M, (double *)nullptr, e.slots);
+            LCHK();
+            hipLaunchKernelGGL((k_orb2s<false>), dim3(e.s_grid(sg), 1), dim3(kBlock), 0, e.stream, oa, lam,
+                               (double *)nullptr, N, (const double2 *)e.d_cs.p, M, (double *)nullptr, e.slots);
+            LCHK();
+        }
+    } else {
+        fill(e.hz_B.p, PN);
+        fill(e.hz_xpart.p, (size_t)M * P * G);
+        LCHK();
+        double *psiA = e.hz_psi2.p, *psiB = e.hz_psi2.p + N;
+        double *lamA = e.hz_lam2.p, *lamB = e.hz_lam2.p + N;
+        HCHK(hipMemcpyAsync(psiA, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+        HCHK(hipMemcpyAsync(lamA, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+        const DevSector sec = e.sec();
+        int nactive = 0;
+        for (int k = 0; k < M; k++) {
+            while (nactive < P && first[perm[nactive]] <= k) nactive++;
+            const int slices = (nactive + 3) / 4;
+            hipLaunchKernelGGL((k_tangent<4>), dim3(G, std::max(slices, 1)), dim3(kBlock), 0, e.stream, psiA, psiB,
+                               lamA, lamB, e.hz_B.p, N, nactive, N, sec, e.d_ops.p, k, e.d_cs.p, pos[e.order[k]],
+                               e.scale[k], e.hz_xpart.p + (size_t)k * P * G, G, e.hz_dpart.p + (size_t)k * G);
+            LCHK();
+            std::swap(psiA, psiB);
+            std::swap(lamA, lamB);
+        }
     }
     // 3. (energy) H B and the Gram matrix 2 B (H B)^T; the metric B B^T;
     // reductions
@@ -3265,26 +3367,64 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
     if (withT || withG) {
         if (!e.hz_gram.ensure((size_t)chunks * P * P, err)) return false;
     }
-    if (withT) {
+    const dim3 tgrid((unsigned)((N + kLanes - 1) / kLanes), nch);
+    if (withT && e.hkind == 3 && e.sigv && P >= kHessLaneMin) {
+        // H B through the lane-layout sigma (about ten times the vector-major
+        // sigma's throughput per vector): B in lanes, sigma, H B back
+        if (!e.hz_lx.ensure(L, err) || !e.hz_ly.ensure(L, err) || !e.hz_lep.ensure((size_t)512 * nch * kLanes, err))
+            return false;
+        if (!tanLanes) {
+            hipLaunchKernelGGL(k_to_lanes, tgrid, dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_lx.p, N, P);
+            LCHK();
+        }
+        if (!e.ham_apply_lanes(e.hz_lx.p, e.hz_ly.p, nch, P, e.hz_epart.p, err, e.hz_lep.p)) return false;
+        hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.hz_ly.p, e.hz_HB.p, N, P);
+        LCHK();
+    }
+    if (tanLanes && (withT || withG)) {
+        // the tangents vector-major for the Gram matrices
+        hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.hz_lx.p, e.hz_B.p, N, P);
+        LCHK();
+    }
+    if (withT && !(e.hkind == 3 && e.sigv && P >= kHessLaneMin)) {
         if (!e.ham_apply(e.hz_B.p, e.hz_HB.p, P, e.hz_epart.p, err)) return false;
-        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_HB.p, N, P,
-                           N, chunks, e.hz_gram.p);
+    }
+    // Gram matrices out[a*P + b] = fac * sum_i X[a][i] Y[b][i]: one FP64 GEMM
+    // where the engine has a rocBLAS handle (column-major view: out = fac *
+    // Y^T X over vectors of stride N), else the tiled kernel
+    auto gram = [&](const double *X, const double *Y, double *out, double f) -> bool {
+        if (e.blas) {
+            const double zero = 0.0;
+            if (rocblas_dgemm(e.blas, rocblas_operation_transpose, rocblas_operation_none, P, P, (rocblas_int)N, &f, Y,
+                              (rocblas_int)N, X, (rocblas_int)N, &zero, out, P) != rocblas_status_success) {
+                err = "rocblas_dgemm (Gram matrix) failed";
+                return false;
+            }
+            return true;
+        }
+        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, X, Y, N, P, N, chunks,
+                           e.hz_gram.p);
         LCHK();
         hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
-                           (long long)P * P, outT, 2.0);
+                           (long long)P * P, out, f);
         LCHK();
+        return true;
+    };
+    if (withT && !gram(e.hz_B.p, e.hz_HB.p, outT, 2.0)) return false;
+    if (withG && !gram(e.hz_B.p, e.hz_B.p, outG, 1.0)) return false;
+    if (tanLanes) {
+        // outX[pos * M + k]
+        hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * M), dim3(kBlock), 0, e.stream, e.hz_xpart.p, M, TS,
+                           (const int *)nullptr, P, outX, 1.0);
+        hipLaunchKernelGGL(k_reduce, dim3(M), dim3(kBlock), 0, e.stream, e.hz_dpart.p, M, TS, (long long)TS, 1LL,
+                           outD, 1.0);
+    } else {
+        // outX[k * P + pos]
+        hipLaunchKernelGGL(k_reduce, dim3(M * P), dim3(kBlock), 0, e.stream, e.hz_xpart.p, M * P, G, (long long)G,
+                           1LL, outX, 1.0);
+        hipLaunchKernelGGL(k_reduce, dim3(M), dim3(kBlock), 0, e.stream, e.hz_dpart.p, M, G, (long long)G, 1LL, outD,
+                           1.0);
     }
-    if (withG) {
-        hipLaunchKernelGGL(k_gram, dim3(tiles, tiles, chunks), dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_B.p, N, P,
-                           N, chunks, e.hz_gram.p);
-        LCHK();
-        hipLaunchKernelGGL(k_reduce, dim3(P * P), dim3(kBlock), 0, e.stream, e.hz_gram.p, P * P, chunks, 1LL,
-                           (long long)P * P, outG, 1.0);
-        LCHK();
-    }
-    hipLaunchKernelGGL(k_reduce, dim3(M * P), dim3(kBlock), 0, e.stream, e.hz_xpart.p, M * P, G, (long long)G, 1LL,
-                       outX, 1.0);
-    hipLaunchKernelGGL(k_reduce, dim3(M), dim3(kBlock), 0, e.stream, e.hz_dpart.p, M, G, (long long)G, 1LL, outD, 1.0);
     LCHK();
     std::vector<double> hout((size_t)M * P + M + 2 * (size_t)P * P, 0.0);
     const size_t nread = (size_t)M * P + M + (withG ? 2 : withT ? 1 : 0) * (size_t)P * P;
@@ -3295,7 +3435,8 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
     std::vector<double> Y((size_t)P * P, 0.0), Dg(P, 0.0);
     for (int k = 0; k < M; k++) {
         const int nu = e.order[k];
-        for (int mu = 0; mu < P; mu++) Y[(size_t)mu * P + nu] += e.scale[k] * X[(size_t)k * P + pos[mu]];
+        for (int mu = 0; mu < P; mu++)
+            Y[(size_t)mu * P + nu] += e.scale[k] * X[tanLanes ? (size_t)pos[mu] * M + k : (size_t)k * P + pos[mu]];
         Dg[nu] += e.scale[k] * e.scale[k] * D[k];
     }
     for (int i = 0; i < P; i++)

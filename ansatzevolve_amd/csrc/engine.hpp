@@ -58,6 +58,8 @@ enum EngineOption : int {
     kOptSweepAmpAdjointLambda = 20,  // of lambda read and written by the adjoint sweep
     kOptSweepAmpAdjointState = 21,   // of the forward state read by the adjoint sweep
     kOptSweepTwoGroupAvailable = 22,  // read only: 1 when kSweepOrbit2 has two-group segments to run
+    kOptSigma1Rows = 23,  // vector-major split sigma: source rows per phase (2, 4, 6 or 8 (default); 0: the two-row kernel);
+                          // a count whose LDS exceeds the device's falls back to the largest that fits
 };
 
 class Engine {

@@ -1576,6 +1576,112 @@ __global__ void __launch_bounds__(kSig1Threads) k_sig1_sparse(Sig1Args a)
     }
 }
 
+// The same with RB source rows per phase (R = 1 + k_alpha (n - k_alpha)
+// rows in ceil(R / RB) phases): each thread reads its entry list once per
+// phase for RB rows, and staging fills every row of a phase between one
+// pair of barriers.  LDS: RB coefficient rows of kSig1GS doubles, then RB
+// source rows of nb doubles.
+template <int RB>
+__global__ void __launch_bounds__(kSig1Threads) k_sig1r(Sig1Args a)
+{
+    extern __shared__ double sm[];
+    const MfArgs &m = a.m;
+    const int n = m.n, nb = m.nb, nn = n * n;
+    const int ar = blockIdx.x;
+    const double *__restrict__ x = a.x + (long long)blockIdx.y * a.vstride;
+    double *__restrict__ y = a.y + (long long)blockIdx.y * a.vstride;
+    double *gsb = sm, *xsb = sm + RB * kSig1GS;
+    const char *gsc = (const char *)gsb, *xsc = (const char *)xsb;
+    const int tid = threadIdx.x;
+    const uint32_t alpha = m.unA[ar], full = (1u << n) - 1;
+    const int ka = __popc(alpha), ne = n - ka;
+    const int R = 1 + ka * ne;
+    double acc[kSig1BI];
+#pragma unroll
+    for (int i = 0; i < kSig1BI; i++) acc[i] = 0;
+    for (int r0 = 0; r0 < R; r0 += RB) {
+        const int nr = min(RB, R - r0);
+        __syncthreads();  // the previous phase's rows are no longer read
+        for (int h = 0; h < nr; h++) {
+            const int r = r0 + h;
+            double *gs = gsb + h * kSig1GS;
+            double *xs = xsb + (size_t)h * nb;
+            int J = ar;
+            bool neg = false;
+            const double *gb = nullptr;
+            if (r > 0) {
+                const int tA = r - 1, ir = tA / ne, iq = tA - ir * ne;
+                uint32_t oo = alpha, ww = ~alpha & full;
+                for (int i = 0; i < ir; i++) oo &= oo - 1;
+                for (int i = 0; i < iq; i++) ww &= ww - 1;
+                const int rA = __ffs(oo) - 1, qA = __ffs(ww) - 1;
+                J = m.rkA[alpha ^ (1u << rA) ^ (1u << qA)];
+                neg = between_parity(alpha, rA, qA);
+                gb = m.g + (size_t)(rA * n + qA) * nn;
+            }
+            for (int e = tid; e < kSig1GS; e += kSig1Threads) {
+                const int ee = e < kSig1NN ? e : e - kSig1NN;
+                double v = 0;
+                if (e < 2 * kSig1NN && ee < nn) {
+                    double cf = 0;
+                    if (r == 0) {
+                        if (ee / n != ee % n)
+                            for (uint32_t o = alpha; o; o &= o - 1) {
+                                const int xo = __ffs(o) - 1;
+                                cf += thr13(m.g[(size_t)(xo * n + xo) * nn + ee]);
+                            }
+                    } else {
+                        cf = neg ? -thr13(gb[ee]) : thr13(gb[ee]);
+                    }
+                    v = e < kSig1NN ? cf : -cf;
+                }
+                gs[e] = v;
+            }
+            const double *src = x + (long long)J * nb;
+            for (int i = tid; i < nb; i += kSig1Threads) xs[i] = src[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSig1BI; i++) {
+            const int beta = tid + kSig1Threads * i;
+            if (beta < nb) {
+                const uint4 *te = (const uint4 *)(a.ts + (size_t)beta * a.ntp);
+                double sr[RB];
+#pragma unroll
+                for (int k = 0; k < RB; k++) sr[k] = 0;
+                for (int t4 = 0; t4 < a.ntp / 4; t4++) {
+                    const uint4 q = te[t4];
+                    const uint32_t ee[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const uint32_t xb = ee[u] & 0xffffu, cb = ee[u] >> 16;
+#pragma unroll
+                        for (int k = 0; k < RB; k++)
+                            if (k < nr)
+                                sr[k] += *(const double *)(gsc + (size_t)k * kSig1GS * 8 + cb) *
+                                         *(const double *)(xsc + (size_t)k * nb * 8 + xb);
+                    }
+                }
+                double t = 0;
+#pragma unroll
+                for (int k = 0; k < RB; k++) t += sr[k];
+                acc[i] += t;
+                if (r0 == 0) acc[i] += m.diag[(long long)ar * nb + beta] * xsb[beta];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSig1BI; i++) {
+        const int beta = tid + kSig1Threads * i;
+        if (beta < nb) y[(long long)ar * nb + beta] = acc[i];
+    }
+}
+
+template __global__ void k_sig1r<2>(Sig1Args);
+template __global__ void k_sig1r<4>(Sig1Args);
+template __global__ void k_sig1r<6>(Sig1Args);
+template __global__ void k_sig1r<8>(Sig1Args);
+
 // part[v * G + block] = block's share of <x_v|y_v>, vectors at stride vstride.
 __global__ void __launch_bounds__(kBlock)
     k_dot_rows(const double *__restrict__ x, const double *__restrict__ y, long long N, long long vstride,

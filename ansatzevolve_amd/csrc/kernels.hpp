@@ -242,6 +242,23 @@ __global__ void k_orb2v(Orb2Args a, const double *v0, double *xout, double *__re
 template <bool BR>
 __global__ void k_orb2s(Orb2Args a, double *v0, double *v1, long long vstride, const double2 *__restrict__ cs, int M,
                         double *__restrict__ partial, int slots);
+// Hessian forward-tangent step over a two-group segment, lane layout over
+// tangents (chunk c = blockIdx.y holds tangents 64c .. 64c+63 of B, cstride
+// apart): psi and lambda at the segment's start (plain vectors, the same for
+// every lane; not written) are rotated operator by operator in registers,
+// and after operator k's rotation (angle table cs[k]) the tangents rotate,
+// the tangent at position inj[k] gains scale[k] G_k psi, and the brackets
+// X[k][tangent] = <lambda|G_k|B_tangent> and D[k] = <lambda|G_k G_k|psi> are
+// accumulated: xpart[((c*M + k)*slots + block)*64 + lane],
+// dpart[k*slots + block] (chunk 0).  grid.x <= slots.
+struct TanArgs {
+    const double *psi, *lam;
+    const int *inj;        // tangent position of operator k's parameter
+    const double *scale;   // scale of operator k
+    double *xpart, *dpart;
+    int slots;
+};
+__global__ void k_orb2t(Orb2Args a, TanArgs t, double *B, long long cstride, const double2 *__restrict__ cs, int M);
 // Three-group orbit segment in lane layout: three consecutive runs of
 // operators on pairwise disjoint orbital groups (three TUPS units on
 // disjoint spatial pairs).  Classes have up to 8 members per spin; ranks are
@@ -474,6 +491,9 @@ struct Sig1Args {
     long long vstride;
 };
 __global__ void k_sig1_sparse(Sig1Args a);  // grid (na, nvec), kSig1Threads, kSig1Lds
+// RB source rows per phase; LDS RB * (kSig1GS + nb) doubles
+template <int RB>
+__global__ void k_sig1r(Sig1Args a);
 __global__ void k_dot_rows(const double *__restrict__ x, const double *__restrict__ y, long long N, long long vstride,
                            double *__restrict__ part);  // grid (G, nvec)
 

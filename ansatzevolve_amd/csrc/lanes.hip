@@ -578,6 +578,156 @@ __global__ void __launch_bounds__(kBlock)
 template __global__ void k_orb2s<false>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
 template __global__ void k_orb2s<true>(Orb2Args, double *, double *, long long, const double2 *, int, double *, int);
 
+// ------------------------------------------- Hessian tangents, two groups
+// One operator of a tangent step on its active pairs in the sub-orbits the
+// orbit type has (cm): psi, lambda and the lane's tangent rotate, the
+// injection lane adds sc G psi, the brackets accumulate.
+template <int G, int H>
+__device__ __forceinline__ void orb2t_apply(unsigned w, bool neg, double C, double S, double (&ps)[16],
+                                            double (&lm)[16], double (&b)[16], bool inj, double sc, unsigned cm,
+                                            double &xacc, double &dacc)
+{
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (!((w >> r) & 1u)) continue;
+        const double s = ((((w >> (6 + r)) & 1u) != 0) != neg) ? -1.0 : 1.0;
+        const double sS = s * S;
+        const int ip = PP[r] >> 1, jp = PP[r] & 1, iq = QQ[r] >> 1, jq = QQ[r] & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (c > 0 && !((cm >> c) & 1u)) continue;
+            const int oa = c & 1, ob = c >> 1;
+            const int sp = ((ip << G) | (oa << (1 - G))) + 4 * ((jp << H) | (ob << (1 - H)));
+            const int sq = ((iq << G) | (oa << (1 - G))) + 4 * ((jq << H) | (ob << (1 - H)));
+            const double pp = ps[sp], pq = ps[sq], lp = lm[sp], lq = lm[sq], bp = b[sp], bq = b[sq];
+            const double pp2 = C * pp + sS * pq, pq2 = C * pq - sS * pp;
+            const double lp2 = C * lp + sS * lq, lq2 = C * lq - sS * lp;
+            double bp2 = C * bp + sS * bq, bq2 = C * bq - sS * bp;
+            if (inj) {
+                bp2 += sc * s * pq2;
+                bq2 -= sc * s * pp2;
+            }
+            ps[sp] = pp2;
+            ps[sq] = pq2;
+            lm[sp] = lp2;
+            lm[sq] = lq2;
+            b[sp] = bp2;
+            b[sq] = bq2;
+            xacc += s * (lp2 * bq2 - lq2 * bp2);
+            dacc -= lp2 * pp2 + lq2 * pq2;
+        }
+    }
+}
+
+template <int G, int H>
+__device__ __forceinline__ void orb2t_op(const Orb2Args &a, const TanArgs &t, const unsigned long long *pw, int kk,
+                                         uint32_t spect, const double2 (*csT)[1], int c, int lane, double (&ps)[16],
+                                         double (&lm)[16], double (&b)[16], unsigned cmask, double *accW, double *dW)
+{
+    const unsigned w = (unsigned)pw[kk] & 0xfffu;
+    if (!w) return;
+    const int k = a.k0 + kk;
+    const bool neg = __popc(spect & a.ops[k].mask) & 1;
+    const double2 cs = csT[kk][0];
+    const bool inj = t.inj[k] == c * kLanes + lane;
+    double xacc = 0, dacc = 0;
+    orb2t_apply<G, H>(w, neg, cs.x, cs.y, ps, lm, b, inj, t.scale[k], (cmask >> (4 * G)) & 0xfu, xacc, dacc);
+    accW[kk * kLanes + lane] += xacc;
+    if (lane == 0) dW[kk] += dacc;
+}
+
+__global__ void __launch_bounds__(kBlock, 4)
+    k_orb2t(Orb2Args a, TanArgs t, double *B, long long cstride, const double2 *__restrict__ cs, int M)
+{
+    constexpr int WPB = kBlock / kLanes;
+    constexpr int K = kOrb2MaxOps;
+    __shared__ double2 csT[K][1];
+    __shared__ double accL[WPB * K * kLanes];
+    __shared__ double dL[WPB * K];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    const int nk = a.k1 - a.k0, m = a.split;
+    if ((int)threadIdx.x < nk) csT[threadIdx.x][0] = cs[a.k0 + threadIdx.x];
+    for (int e = threadIdx.x; e < WPB * K * kLanes; e += kBlock) accL[e] = 0.0;
+    if ((int)threadIdx.x < WPB * K) dL[threadIdx.x] = 0.0;
+    __syncthreads();
+    double *accW = accL + wv * K * kLanes;
+    double *dW = dL + wv * K;
+    B += c * cstride;
+    const int mainA = a.nAm * a.nBc;
+    const int total = mainA + (a.nAc - a.nAm) * a.nBm;
+    const int stride = gridDim.x * WPB;
+    for (int tt = blockIdx.x * WPB + wv; tt < total; tt += stride) {
+        int ia, ib;
+        if (tt < mainA) {
+            ia = tt / a.nBc;
+            ib = tt - ia * a.nBc;
+        } else {
+            const int u = tt - mainA, i = u / a.nBm;
+            ia = a.nAm + i;
+            ib = u - i * a.nBm;
+        }
+        const int4 ra = ld_const(a.aRanks, ia), rb = ld_const(a.bRanks, ib);
+        const uint32_t fa = ld_const(a.aInfo, ia), fb = ld_const(a.bInfo, ib);
+        const int rA[4] = {ra.x, ra.y, ra.z, ra.w}, rB[4] = {rb.x, rb.y, rb.z, rb.w};
+        const int na_ = (ra.x >= 0) + (ra.y >= 0) + (ra.z >= 0) + (ra.w >= 0);
+        const int nb_ = (rb.x >= 0) + (rb.y >= 0) + (rb.z >= 0) + (rb.w >= 0);
+        if (na_ * nb_ < 2) continue;  // a lone determinant: no operator acts
+        const int type = ((fa >> 24) & 3) + 3 * ((fa >> 26) & 3) + 9 * (((fb >> 24) & 3) + 3 * ((fb >> 26) & 3));
+        const uint32_t spect = ((fa & 0xffffffu) << a.n) | (fb & 0xffffffu);
+        unsigned cmask;
+        {
+            const bool a2 = ((fa >> 26) & 3) == 0, b2 = ((fb >> 26) & 3) == 0;
+            const bool a1 = ((fa >> 24) & 3) == 0, b1 = ((fb >> 24) & 3) == 0;
+            cmask = 1u | (a2 ? 2u : 0u) | (b2 ? 4u : 0u) | (a2 && b2 ? 8u : 0u) |
+                    ((1u | (a1 ? 2u : 0u) | (b1 ? 4u : 0u) | (a1 && b1 ? 8u : 0u)) << 4);
+        }
+        double ps[16], lm[16], b[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ma = q & 3, mb = q >> 2;
+            const bool ok = rA[ma] >= 0 && rB[mb] >= 0;
+            const uint32_t d = (uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb];
+            ps[q] = ok ? t.psi[d] : 0.0;
+            lm[q] = ok ? t.lam[d] : 0.0;
+            b[q] = ok ? B[(size_t)d * kLanes + lane] : 0.0;
+        }
+        const unsigned long long *pw = a.prog + type * K;
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            if (kk >= m) break;
+            orb2t_op<0, 0>(a, t, pw, kk, spect, csT, c, lane, ps, lm, b, cmask, accW, dW);
+        }
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            if (kk < m || kk >= nk) continue;
+            orb2t_op<1, 1>(a, t, pw, kk, spect, csT, c, lane, ps, lm, b, cmask, accW, dW);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ma = q & 3, mb = q >> 2;
+            if (rA[ma] >= 0 && rB[mb] >= 0)
+                B[(size_t)((uint32_t)rA[ma] * (uint32_t)a.nb + (uint32_t)rB[mb]) * kLanes + lane] = b[q];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * kLanes; e += kBlock) {
+        const int kk = e / kLanes, l = e - kk * kLanes;
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < WPB; q++) s += accL[(q * K + kk) * kLanes + l];
+        t.xpart[(((long long)c * M + a.k0 + kk) * t.slots + blockIdx.x) * kLanes + l] = s;
+    }
+    if (c == 0 && (int)threadIdx.x < nk) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < WPB; q++) s += dL[q * K + threadIdx.x];
+        t.dpart[(long long)(a.k0 + threadIdx.x) * t.slots + blockIdx.x] = s;
+    }
+}
+
 // ------------------------------------------------ three-group segments
 // f(Idx<I>) for I = B .. E-1, written out at compile time (no loop)
 template <int I>
