@@ -496,7 +496,7 @@ def _main():
     p.add_argument("--only", default="", help="generate / evaluate: these orbital counts only (comma list)")
     p.add_argument("--target-dir", default="", help="fit: where targets_n<n>.npz are (default: out-dir)")
     p.add_argument("parts", nargs="*", help="merge: the per-count eval_n<n>.json files")
-    a = p.parse_args()
+    a = p.parse_intermixed_args()
     only = [int(x) for x in a.only.split(",")] if a.only else None
     if a.stage == "merge":
         s = stage_merge(a.out_dir, a.parts)
