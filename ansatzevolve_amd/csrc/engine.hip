@@ -3042,6 +3042,7 @@ int Engine::option(int key) const
     case kOptSweep: return e.sweepKind;
     case kOptSweepTwoGroupAvailable: return e.sweepS ? 1 : 0;
     case kOptSigma1Rows: return e.sig1Rows;
+    case kOptVectorSegments: return (int)(e.use_s() ? e.laneS.segs.size() : e.segs.size());
     case kOptLaneMin: return e.laneMin;
     case kOptGraph: return e.useGraph ? 1 : 0;
     case kOptFuseDepth: return e.fuseDepth;

@@ -60,6 +60,7 @@ enum EngineOption : int {
     kOptSweepTwoGroupAvailable = 22,  // read only: 1 when kSweepOrbit2 has two-group segments to run
     kOptSigma1Rows = 23,  // vector-major split sigma: source rows per phase (2, 4, 6 or 8 (default); 0: the two-row kernel);
                           // a count whose LDS exceeds the device's falls back to the largest that fits
+    kOptVectorSegments = 24,  // read only: segments per vector-major sweep (launches, the two-group list with kind 3)
 };
 
 class Engine {

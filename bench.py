@@ -301,7 +301,7 @@ def main():
     gemm_ms, gemm_n = a.kernel_timing(3)
     a.set_kernel_timing(False)
     ham_kind = a.get_option(a.OPT_HAM_KIND)
-    nseg = a.get_option(a.OPT_SEGMENTS)
+    nseg = a.get_option(a.OPT_VECTOR_SEGMENTS)  # launches per vector-major sweep
     lanes = opts["lane_min"] > 0 and args.nvec >= opts["lane_min"]
 
     # batched results must equal single-vector results
@@ -349,10 +349,11 @@ def main():
     # in place it reads and writes psi like lambda).  Vector-major: every
     # segment reads and writes each vector once (forward: psi; adjoint: psi
     # and lambda).
+    two_vm = a.get_option(a.OPT_SWEEP) == 3 and a.get_option(a.OPT_SWEEP_TWO_GROUP_AVAILABLE) == 1
     k_fwd = ("k_orb2v<false>" if opts["lane_groups_fwd"] == 2 else "k_orbv<false>") + " (lane layout)" if lanes \
-        else "k_orb<false>"
+        else ("k_orb2s<false>" if two_vm else "k_orb<false>")
     k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
-        else "k_orb<true>"
+        else ("k_orb2s<true>" if two_vm else "k_orb<true>")
     if lanes:
         fwd_launches, rev_launches = max(1, fwd_n // nbd), max(1, rev_n // nbd)
         fwd_pass = 16.0 * amp["forward"] * args.nvec / fwd_launches
@@ -369,7 +370,26 @@ def main():
     rooflines["adjoint_sweep"] = hbm(k_rev, ideal_sweep / rev_launches,
                                      per_launch(rev_ms, rev_n, 1 if lanes else nseg),
                                      rev_n if lanes else rev_n * nseg, rev_ms / nbd, rev_pass)
-    if ham_kind == 3 and gemm_n > 0:
+    if ham_kind == 3 and gemm_n > 0 and not lanes:
+        # vector-major split sigma (k_sig1r): per target and entry one
+        # coefficient and one source amplitude read from LDS, 16 B per FMA
+        per_el = ku * (n_ - ku) * kd * (n_ - kd) + ku * (n_ - ku) * kd + kd * (n_ - kd) + 1
+        na = math.comb(n_, ku)
+        nb = math.comb(n_, kd)
+        avg = ham_ms / max(ham_n, 1)
+        lds_bytes = 16.0 * N * per_el * args.nvec / max(1.0, ham_n / nbd)
+        ach = lds_bytes / (avg * 1e-3) / 1e9
+        rooflines["hamiltonian"] = {
+            "bound": "lds", "achieved": ach, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": ach / LDS_PEAK_GBS,
+            "traffic": None, "kernel": "k_sig1r (vector-major sigma, %d vectors)" % args.nvec,
+            "avg_launch_ms": avg, "launches": ham_n, "algorithmic_lds_bytes_per_launch": lds_bytes,
+            "ms_per_step": ham_ms / nbd, "fp64_tflops": 2.0 * N * per_el * args.nvec / nbd / (ham_ms / nbd * 1e-3) / 1e12,
+            "note": "LDS peak = 256 CUs x 128 B/clock x 2.4 GHz; 16 B of LDS reads per FMA (random gathers)"}
+        gflops = 2.0 * (na * na * nb + nb * nb * na) * args.nvec
+        rooflines["same_spin_dgemm"] = fp64("rocBLAS dgemm_strided_batched x2 (dense same-spin rows)", gflops,
+                                            gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd,
+                                            "dense FLOPs of both GEMMs on FP64 MFMA")
+    elif ham_kind == 3 and gemm_n > 0:
         halves = a.get_option(a.OPT_SIGMA_HALVES) and a.get_option(a.OPT_SIGMA_HALVES_AVAILABLE)
         grouped = a.get_option(a.OPT_SIGMA_GROUPED) and a.get_option(a.OPT_SIGMA_GROUPED_AVAILABLE)
         sig_name = "k_sigg_sparse" if grouped else "k_sigh_sparse" if halves else "k_sigw_sparse"
