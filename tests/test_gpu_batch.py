@@ -542,3 +542,62 @@ def test_two_group_vector_major_sweeps(gpu, n, layers):
         for v in range(len(thetas)):
             E0, g0 = o.energy_gradient(thetas[v])
             assert abs(E[v] - E0) <= 1e-10 and np.max(np.abs(G[v] - g0)) <= 1e-9
+
+
+def _random_sz_operators(n, count, seed):
+    """Random S_z-conserving singles and doubles on 2n spin-orbitals (beta
+    0..n-1, alpha n..2n-1), with repeated-parameter maps and scales."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    while len(ops) < count:
+        if rng.random() < 0.4:
+            spin = n * int(rng.integers(2))
+            p, q = rng.choice(n, 2, replace=False) + spin
+            ops.append([p, q, -1, -1])
+        else:
+            kind = int(rng.integers(3))
+            if kind == 0:  # one alpha and one beta electron move
+                a0, a1 = rng.choice(n, 2, replace=False) + n
+                b0, b1 = rng.choice(n, 2, replace=False)
+                ops.append([a0, b0, a1, b1])
+            else:  # two electrons of one spin
+                spin = n * (kind - 1)
+                c0, c1, d0, d1 = rng.choice(n, 4, replace=False) + spin
+                ops.append([c0, c1, d0, d1])
+    ops = np.array(ops)
+    P = max(2, count * 2 // 3)
+    order = np.concatenate([np.arange(P), rng.integers(0, P, count - P)])
+    rng.shuffle(order)
+    scale = rng.choice([1.0, -1.0, 0.5], count)
+    return ops, order, scale
+
+
+@pytest.mark.parametrize("n,count,seed", [(5, 30, 1), (6, 40, 2), (6, 24, 3)])
+def test_random_operator_lists_every_sweep_kind(gpu, n, count, seed):
+    """Operator lists without TUPS structure (random S_z-conserving singles
+    and doubles, shared parameters, mixed scales): every sweep kind, the
+    lane layout and the vector-major batch, energy, gradient and Hessian
+    against the oracle."""
+    from oracle.oracle import OracleEngine
+    h, g = synthetic_integrals(n, 600 + seed)
+    ops, order, scale = _random_sz_operators(n, count, seed)
+    start = hf_index(n)
+    w = dict(n=n, nq=2 * n, h=h, g=g, ops=ops, order=order, scale=scale, start_idx=start)
+    o = OracleEngine(2 * n, [start], [1.0], ops, order, scale)
+    o.set_integrals(h, g, explicit=True)
+    P = int(order.max()) + 1
+    thetas = seeded_angles(P, 70 + seed, count=4)
+    ref = [o.energy_gradient(t) for t in thetas]
+    with from_workload(w) as a:
+        for sweep in (0, 1, 2, 3):
+            a.set_option(AnsatzSynth.OPT_SWEEP, sweep)
+            for t, (E0, g0) in zip(thetas[:2], ref):
+                E, gr = a.energy_gradient(t)
+                assert abs(E - E0) <= 1e-10 and np.max(np.abs(gr - g0)) <= 1e-9, sweep
+            for lane_min in (0, 1):
+                a.set_option(AnsatzSynth.OPT_LANE_MIN, lane_min)
+                E, G = a.energy_gradients(thetas)
+                for v, (E0, g0) in enumerate(ref):
+                    assert abs(E[v] - E0) <= 1e-10 and np.max(np.abs(G[v] - g0)) <= 1e-9, (sweep, lane_min)
+        H0, _, _ = o.hessian(thetas[0])
+        assert np.max(np.abs(a.hessian(thetas[0]) - H0)) <= 1e-8
