@@ -11,10 +11,11 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libansatzsynth.so")
 ARCH = os.environ.get("AE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "lanes.hip", "sigg.hip", "engine.hip", "mlp.hip", "multi.hip"]
+HIP_SOURCES = ["kernels.hip", "lanes.hip", "window.hip", "sigg.hip", "engine.hip", "mlp.hip", "multi.hip"]
 # per-file device flags: the lane sweeps keep their orbit arrays as separate
 # registers (no promotion of private arrays to one vector register tuple)
-HIP_FILE_FLAGS = {"lanes.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"]}
+HIP_FILE_FLAGS = {"lanes.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"],
+                  "window.hip": ["-mllvm", "-disable-promote-alloca-to-vector=1"]}
 CXX_SOURCES = ["manager.cpp", "newton.cpp", "projected.cpp", "spectral.cpp", "autotune.cpp", "abi.cpp", "runfiles.cpp"]
 # command-line front end over the C ABI (cli.cpp), beside the library
 CLI = os.path.join(HERE, "ansatzsynth")

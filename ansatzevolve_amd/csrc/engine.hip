@@ -315,7 +315,8 @@ struct EngineImpl {
     std::vector<const void *> batch_ptrs() const
     {
         return {b_psi.p,      b_lam.p,   b_partial.p, b_epart.p, b_out.p,  b_cs.p,    b_csneg.p, bl_psi.p,
-                bl_lam.p,     bl_partial.p, bl_epart.p, bl_cs.p,  bl_csneg.p, bl_ck.p, d_suplist.p, d_lrows.p};
+                bl_lam.p,     bl_partial.p, bl_epart.p, bl_cs.p,  bl_csneg.p, bl_ck.p, d_suplist.p, d_lrows.p,
+                d_wtiles.p,   d_wblob.p, d_wrows.p,   d_wkp.p};
     }
 
     bool use_lanes(int nvec) const { return laneMin > 0 && nvec >= laneMin; }
@@ -332,63 +333,69 @@ struct EngineImpl {
             sg.progOff = -1;
             if (sg.orbA < 0 || sg.two) continue;
             sg.progOff = (int)prog.size();
-            uint32_t bitsA[2] = {0, 0}, bitsB[2] = {0, 0};
-            {
-                int i = 0;
-                for (uint32_t u = sg.ua; u && i < 2; u &= u - 1) bitsA[i++] = u & (~u + 1);
-                i = 0;
-                for (uint32_t u = sg.ub; u && i < 2; u &= u - 1) bitsB[i++] = u & (~u + 1);
-            }
-            const bool twoA = __builtin_popcount(sg.ua) == 2, twoB = __builtin_popcount(sg.ub) == 2;
-            for (int ta = 0; ta < 3; ta++)
-                for (int tb = 0; tb < 3; tb++) {
-                    // members in descriptor order: lower string (lower rank) first
-                    uint32_t al[2], bl[2];
-                    int Ra = 1, Rb = 1;
-                    if (ta == 0) {
-                        al[0] = bitsA[0];
-                        al[1] = bitsA[1];
-                        Ra = twoA ? 2 : 1;
-                    } else {
-                        al[0] = al[1] = (ta == 2) ? sg.ua : 0u;
-                    }
-                    if (tb == 0) {
-                        bl[0] = bitsB[0];
-                        bl[1] = bitsB[1];
-                        Rb = twoB ? 2 : 1;
-                    } else {
-                        bl[0] = bl[1] = (tb == 2) ? sg.ub : 0u;
-                    }
-                    uint32_t st[4];
-                    bool ok[4];
-                    for (int p = 0; p < 4; p++) {
-                        const int i = p >> 1, j = p & 1;
-                        ok[p] = i < Ra && j < Rb;
-                        st[p] = (al[i] << n) | bl[j];
-                    }
-                    static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
-                    for (int kk = 0; kk < kOrbMaxOps; kk++) {
-                        uint32_t w = 0;
-                        if (sg.k0 + kk < sg.k1) {
-                            const DevOp &o = hops[sg.k0 + kk];
-                            for (int r = 0; r < 6; r++) {
-                                const int p = PP[r], q = QQ[r];
-                                if (!ok[p] || !ok[q]) continue;
-                                uint32_t b2 = 0;
-                                const int sgn = host_pair_sign(o, st[p], b2);
-                                if (sgn == 0 || b2 != st[q]) continue;
-                                w |= 1u << r;
-                                if (sgn < 0) w |= 1u << (8 + r);
-                            }
-                        }
-                        prog.push_back(w);
-                    }
-                }
+            orbit_program_words(sg, prog);
         }
         if (prog.empty()) prog.push_back(0);
         if (!dprog.ensure(prog.size(), err)) return false;
         HCHK(hipMemcpy(dprog.p, prog.data(), prog.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         return true;
+    }
+
+    // The program of one orbit segment: 9 types x kOrbMaxOps words appended to prog.
+    void orbit_program_words(const Seg &sg, std::vector<uint32_t> &prog) const
+    {
+        uint32_t bitsA[2] = {0, 0}, bitsB[2] = {0, 0};
+        {
+            int i = 0;
+            for (uint32_t u = sg.ua; u && i < 2; u &= u - 1) bitsA[i++] = u & (~u + 1);
+            i = 0;
+            for (uint32_t u = sg.ub; u && i < 2; u &= u - 1) bitsB[i++] = u & (~u + 1);
+        }
+        const bool twoA = __builtin_popcount(sg.ua) == 2, twoB = __builtin_popcount(sg.ub) == 2;
+        for (int ta = 0; ta < 3; ta++)
+            for (int tb = 0; tb < 3; tb++) {
+                // members in descriptor order: lower string (lower rank) first
+                uint32_t al[2], bl[2];
+                int Ra = 1, Rb = 1;
+                if (ta == 0) {
+                    al[0] = bitsA[0];
+                    al[1] = bitsA[1];
+                    Ra = twoA ? 2 : 1;
+                } else {
+                    al[0] = al[1] = (ta == 2) ? sg.ua : 0u;
+                }
+                if (tb == 0) {
+                    bl[0] = bitsB[0];
+                    bl[1] = bitsB[1];
+                    Rb = twoB ? 2 : 1;
+                } else {
+                    bl[0] = bl[1] = (tb == 2) ? sg.ub : 0u;
+                }
+                uint32_t st[4];
+                bool ok[4];
+                for (int p = 0; p < 4; p++) {
+                    const int i = p >> 1, j = p & 1;
+                    ok[p] = i < Ra && j < Rb;
+                    st[p] = (al[i] << n) | bl[j];
+                }
+                static const int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+                for (int kk = 0; kk < kOrbMaxOps; kk++) {
+                    uint32_t w = 0;
+                    if (sg.k0 + kk < sg.k1) {
+                        const DevOp &o = hops[sg.k0 + kk];
+                        for (int r = 0; r < 6; r++) {
+                            const int p = PP[r], q = QQ[r];
+                            if (!ok[p] || !ok[q]) continue;
+                            uint32_t b2 = 0;
+                            const int sgn = host_pair_sign(o, st[p], b2);
+                            if (sgn == 0 || b2 != st[q]) continue;
+                            w |= 1u << r;
+                            if (sgn < 0) w |= 1u << (8 + r);
+                        }
+                    }
+                    prog.push_back(w);
+                }
+            }
     }
 
     int groupset_for(uint32_t g1, uint32_t g2, bool beta, std::string &err)
@@ -575,7 +582,7 @@ struct EngineImpl {
             for (int k = sg.k0; k < sg.k1; k++) rows[k] = sg.grid;
         if (!d_lrows.ensure(std::max(M, 1), err)) return false;
         HCHK(hipMemcpy(d_lrows.p, rows.data(), M * sizeof(int), hipMemcpyHostToDevice));
-        return true;
+        return build_window_plan(err);
     }
 
     // Reachable amplitudes of the lane-layout sweeps.  S_0 = the start
@@ -591,6 +598,7 @@ struct EngineImpl {
     // another kind makes everything reachable from there on.
     bool compute_support(bool ck, std::string &err)
     {
+        if (winOn) return compute_support_win(ck, err);
         supDirty = false;
         supCk = ck;
         supGen++;
@@ -729,6 +737,544 @@ struct EngineImpl {
         if (!lists.empty()) {
             if (!d_suplist.ensure(lists.size(), err)) return false;
             HCHK(hipMemcpy(d_suplist.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        return true;
+    }
+
+    // ---------------------------------------- window passes (kOptWindow)
+    // A pass applies every orbit segment ("phase": consecutive operators on
+    // at most two alpha and two beta spin-orbitals) that lies inside a window
+    // of w consecutive spatial orbitals and commutes with every phase it
+    // overtakes (disjoint spin-orbitals; even fermionic operators on disjoint
+    // modes commute), to LDS-staged tiles (k_win).  The passes are chosen
+    // greedily with one step of look-ahead for the fewest passes.
+    struct WinPass {
+        int w0 = 0;
+        int nph = 0, kp0 = 0, nkp = 0;
+        int nAcls = 0, nBcls = 0;
+        int blobOff = 0, nblob = 0;  // its table blob in d_wblob (WinArgs)
+        int oSubIdx = 0, oSub = 0, oProg = 0, oMask = 0, oPat = 0, oCOff = 0, oCls = 0, oCStart = 0, oProgT = 0;
+        int rowsOff = 0, nrowsA = 0;  // its class member lists in d_wrows
+        std::vector<int> cntA, cntB, startA, startB;  // class sizes and first members (per spin)
+        int maxMembers = 0, maxSide = 0;
+        std::vector<int> allTiles;   // every tile, largest first
+        std::vector<int> tileSize;   // members per tile (indexed by tile id)
+        int tileOff = 0, ntiles = 0; // the tiles a sweep processes (d_wtiles)
+        int gridF = 1, gridA = 1;    // blocks (multiples of 64 / V)
+    };
+    int winW = 0;        // kOptWindow: window width in spatial orbitals (0: off)
+    int winWaves = 4;    // kOptWindowWaves: waves per block of the window kernels (4 or 8)
+    bool winOn = false;  // a plan exists: lane-layout sweeps run as window passes
+    // the plan's windows are four orbitals wide and every phase is a TUPS
+    // unit (the same adjacent pair in both spins): tiles in registers (k_winr)
+    bool winReg = false;
+    // ... and every phase is a TUPS unit of five operators whose programs
+    // rotate exactly that unit's member pairs (straight-line phases, k_winr TU)
+    bool winTu = false;
+    std::vector<WinPass> wpass;
+    DBuf<uint32_t> d_wblob, d_wrows;
+    std::vector<uint32_t> hrows;  // host copy of d_wrows
+    DBuf<int> d_wkp, d_wtiles;
+    int wslots = 1;  // bracket partial rows of the window adjoint passes
+
+    int nfwd() const { return winOn ? (int)wpass.size() : (int)laneF.segs.size(); }
+
+    template <class T>
+    bool upload_vec(DBuf<T> &d, const std::vector<T> &h, std::string &err)
+    {
+        if (!d.ensure(std::max<size_t>(h.size(), 1), err)) return false;
+        if (!h.empty()) HCHK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return true;
+    }
+
+    const void *winr_fn(bool br, bool bc) const
+    {
+        if (winTu)
+            return br ? (const void *)k_winr<true, false, 2, true>
+                      : (bc ? (const void *)k_winr<false, true, 4, true> : (const void *)k_winr<false, false, 4, true>);
+        return br ? (const void *)k_winr<true, false, 2, false>
+                  : (bc ? (const void *)k_winr<false, true, 4, false> : (const void *)k_winr<false, false, 4, false>);
+    }
+    int win_waves(bool br) const { return winReg ? (br ? 2 : 4) : winWaves; }
+    static const void *win_fn(bool br, int nw, bool bc)
+    {
+        if (nw == 8) return br ? (const void *)k_win<true, false, 8>
+                               : (bc ? (const void *)k_win<false, true, 8> : (const void *)k_win<false, false, 8>);
+        return br ? (const void *)k_win<true, false, 4>
+                  : (bc ? (const void *)k_win<false, true, 4> : (const void *)k_win<false, false, 4>);
+    }
+
+    bool build_window_plan(std::string &err)
+    {
+        winOn = false;
+        winReg = winTu = false;
+        wpass.clear();
+        if (winW <= 0 || mode != kSz || cplx || M == 0) return true;
+        const int w = std::min(std::min(winW, kWinMaxW), n);
+        const uint32_t amask = (1u << n) - 1;
+        auto bitsA = [&](int kk) { return ((hops[kk].cre | hops[kk].ann) >> n) & amask; };
+        auto bitsB = [&](int kk) { return (hops[kk].cre | hops[kk].ann) & amask; };
+        struct Ph {
+            int k0, k1;
+            uint32_t ua, ub;
+        };
+        std::vector<Ph> phs;
+        for (int k = 0; k < M;) {
+            uint32_t ua = bitsA(k), ub = bitsB(k);
+            if (hops[k].cre == hops[k].ann || __builtin_popcount(ua) > 2 || __builtin_popcount(ub) > 2) return true;
+            auto span = [](uint32_t sp) { return (31 - __builtin_clz(sp)) - __builtin_ctz(sp) + 1; };
+            if (span(ua | ub) > w) return true;  // an operator wider than the window
+            int k1 = k + 1;
+            while (k1 < M && k1 - k < kOrbMaxOps && hops[k1].cre != hops[k1].ann) {
+                const uint32_t a2 = ua | bitsA(k1), b2 = ub | bitsB(k1);
+                if (__builtin_popcount(a2) > 2 || __builtin_popcount(b2) > 2 || span(a2 | b2) > w) break;
+                ua = a2;
+                ub = b2;
+                k1++;
+            }
+            phs.push_back({k, k1, ua, ub});
+            k = k1;
+        }
+        const int nw = n - w + 1;
+        const size_t P = phs.size();
+        // the phases window w0 can take now: inside it, not overtaking an
+        // unapplied phase on a shared spin-orbital, within the pass limits
+        auto pick = [&](const std::vector<char> &done, int w0, std::vector<int> &list) {
+            const uint32_t W = ((1u << w) - 1) << w0;
+            uint32_t tA = 0, tB = 0;
+            int ops = 0;
+            list.clear();
+            for (size_t i = 0; i < P; i++) {
+                if (done[i]) continue;
+                const Ph &p = phs[i];
+                const int nk = p.k1 - p.k0;
+                if (!((p.ua | p.ub) & ~W) && !(p.ua & tA) && !(p.ub & tB) && (int)list.size() < kWinMaxPhases &&
+                    ops + nk <= kWinMaxOps) {
+                    list.push_back((int)i);
+                    ops += nk;
+                } else {
+                    tA |= p.ua;
+                    tB |= p.ub;
+                }
+            }
+        };
+        auto greedy = [&](std::vector<char> done, size_t left) {
+            int passes = 0;
+            std::vector<int> l, best;
+            while (left) {
+                best.clear();
+                for (int w0 = 0; w0 < nw; w0++) {
+                    pick(done, w0, l);
+                    if (l.size() > best.size()) best = l;
+                }
+                if (best.empty()) return 1 << 20;
+                for (int i : best) done[i] = 1;
+                left -= best.size();
+                passes++;
+            }
+            return passes;
+        };
+        std::vector<char> done(P, 0);
+        size_t left = P;
+        std::vector<std::pair<int, std::vector<int>>> plan;
+        while (left) {
+            int bestCost = 1 << 30, bestW = -1;
+            std::vector<int> bestL, l;
+            for (int w0 = 0; w0 < nw; w0++) {
+                pick(done, w0, l);
+                if (l.empty()) continue;
+                std::vector<char> d2 = done;
+                for (int i : l) d2[i] = 1;
+                const int cost = 1 + greedy(d2, left - l.size());
+                if (cost < bestCost || (cost == bestCost && l.size() > bestL.size())) {
+                    bestCost = cost;
+                    bestW = w0;
+                    bestL = l;
+                }
+            }
+            if (bestW < 0) return true;
+            for (int i : bestL) done[i] = 1;
+            left -= bestL.size();
+            plan.push_back({bestW, bestL});
+        }
+        winReg = w == 4;
+        for (const Ph &p : phs) {
+            const uint32_t lo = p.ua & (~p.ua + 1);
+            winReg = winReg && p.ua == p.ub && p.ua == (lo | (lo << 1));
+        }
+        winTu = winReg;  // narrowed per phase below
+        // tables: one blob of 32-bit words per pass and the class member
+        // lists (rows)
+        const uint32_t full = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1);
+        // x / d = (x * ceil(2^24 / d)) >> 24 exactly for x < 2^24 / d
+        auto magic = [](int d) { return d > 0 ? (uint32_t)((((uint64_t)1 << 24) + d - 1) / d) : 0u; };
+        std::vector<uint32_t> blob, rows;
+        std::vector<int> kpOp;
+        for (const auto &pl : plan) {
+            WinPass wp;
+            wp.w0 = pl.first;
+            const uint32_t W = ((1u << w) - 1) << wp.w0;
+            std::vector<std::vector<uint32_t>> byCount(w + 1);
+            for (uint32_t x = 0; x < (1u << w); x++) byCount[__builtin_popcount(x)].push_back(x);
+            for (int c = 0; c <= w; c++) wp.maxSide = std::max(wp.maxSide, (int)byCount[c].size());
+            std::vector<uint32_t> pat, cls, cstart;
+            std::vector<int> coff;
+            for (int c = 0; c <= w; c++) {
+                coff.push_back((int)pat.size());
+                for (uint32_t x : byCount[c]) pat.push_back(x << wp.w0);
+            }
+            coff.push_back(1 << w);
+            std::map<uint32_t, int> mA, mB;
+            for (uint32_t st : unA) mA[st & ~W] = __builtin_popcount(st & W);
+            for (uint32_t st : unB) mB[st & ~W] = __builtin_popcount(st & W);
+            wp.nAcls = (int)mA.size();
+            wp.nBcls = (int)mB.size();
+            wp.rowsOff = (int)rows.size();
+            std::vector<int> &cntA = wp.cntA, &cntB = wp.cntB;
+            for (int spin = 0; spin < 2; spin++) {
+                const int base = (int)rows.size();
+                for (const auto &kv : spin ? mB : mA) {
+                    cls.push_back(kv.first | ((uint32_t)kv.second << 24));
+                    cstart.push_back((uint32_t)((int)rows.size() - base));
+                    (spin ? wp.startB : wp.startA).push_back((int)rows.size() - base);
+                    (spin ? cntB : cntA).push_back((int)byCount[kv.second].size());
+                    for (uint32_t x : byCount[kv.second]) {
+                        const uint32_t st = kv.first | (x << wp.w0);
+                        rows.push_back(spin ? (uint32_t)rkB[st] : (uint32_t)rkA[st] * (uint32_t)nb);
+                    }
+                }
+                if (spin == 0) wp.nrowsA = (int)rows.size() - base;
+            }
+            wp.tileSize.resize((size_t)wp.nAcls * wp.nBcls);
+            for (int ia = 0; ia < wp.nAcls; ia++)
+                for (int ib = 0; ib < wp.nBcls; ib++) {
+                    wp.tileSize[(size_t)ia * wp.nBcls + ib] = cntA[ia] * cntB[ib];
+                    wp.maxMembers = std::max(wp.maxMembers, cntA[ia] * cntB[ib]);
+                }
+            wp.allTiles.resize(wp.tileSize.size());
+            std::iota(wp.allTiles.begin(), wp.allTiles.end(), 0);
+            std::stable_sort(wp.allTiles.begin(), wp.allTiles.end(),
+                             [&](int x, int y) { return wp.tileSize[x] > wp.tileSize[y]; });
+            // phases, sub-class lists, programs, masks
+            std::vector<WinPhaseDev> ph;
+            std::vector<int4> subIdx;
+            std::vector<uint32_t> sub, mask;
+            std::vector<Seg> psegs;
+            wp.kp0 = (int)kpOp.size();
+            for (int i : pl.second) {
+                const Ph &p = phs[i];
+                WinPhaseDev d;
+                d.k0 = p.k0;
+                d.nk = p.k1 - p.k0;
+                d.progOff = 0;
+                d.kp = (int)kpOp.size() - wp.kp0;
+                d.ua = p.ua;
+                d.ub = p.ub;
+                d.spect = full & ~((p.ua << n) | p.ub);
+                d.pad = 0;
+                for (int k = p.k0; k < p.k1; k++) {
+                    kpOp.push_back(k);
+                    mask.push_back(hops[k].mask);
+                }
+                for (int spin = 0; spin < 2; spin++) {
+                    const uint32_t u = (spin ? p.ub : p.ua) >> wp.w0;
+                    for (int c = 0; c <= w; c++) {
+                        const std::vector<uint32_t> &xs = byCount[c];
+                        std::map<uint32_t, std::vector<int>> grp;
+                        for (int i2 = 0; i2 < (int)xs.size(); i2++) grp[xs[i2] & ~u].push_back(i2);
+                        const int off = (int)sub.size();
+                        int n2 = 0;
+                        for (int pass = 0; pass < 2; pass++)
+                            for (const auto &g : grp) {
+                                const bool two = g.second.size() == 2;
+                                if (two != (pass == 0)) continue;
+                                if (two) {
+                                    sub.push_back((uint32_t)g.second[0] | ((uint32_t)g.second[1] << 8));
+                                    n2++;
+                                } else {
+                                    const uint32_t sh = (xs[g.second[0]] & u) ? 2u : 1u;
+                                    sub.push_back((uint32_t)g.second[0] | (0xffu << 8) | (sh << 16));
+                                }
+                            }
+                        const int ng = (int)grp.size();
+                        subIdx.push_back(make_int4(off, ng | (n2 << 16), (int)magic(ng), (int)magic(n2)));
+                    }
+                }
+                ph.push_back(d);
+                Seg sg{p.k0, p.k1, -1};
+                sg.orbA = 0;
+                sg.ua = p.ua;
+                sg.ub = p.ub;
+                psegs.push_back(sg);
+            }
+            wp.nph = (int)pl.second.size();
+            wp.nkp = (int)kpOp.size() - wp.kp0;
+            // orbit programs of the phases (9 types x kOrbMaxOps words each)
+            std::vector<uint32_t> prog;
+            for (Seg &sg : psegs) {
+                sg.progOff = (int)prog.size();
+                orbit_program_words(sg, prog);
+            }
+            for (size_t i = 0; i < ph.size(); i++) ph[i].progOff = psegs[i].progOff;
+            // TUPS units: beta single, alpha single, paired double, beta
+            // single, alpha single on one pair, each rotating exactly the
+            // member pairs of its kind in every orbit type
+            for (size_t i = 0; winTu && i < ph.size(); i++) {
+                static const int kind[5] = {0, 1, 2, 0, 1};
+                winTu = ph[i].nk == 5;
+                for (int kk = 0; winTu && kk < 5; kk++) {
+                    const int k = ph[i].k0 + kk;
+                    const uint32_t a2 = bitsA(k), b2 = bitsB(k);
+                    winTu = (kind[kk] == 0 ? (a2 == 0 && b2 == ph[i].ub)
+                             : kind[kk] == 1 ? (a2 == ph[i].ua && b2 == 0) : (a2 == ph[i].ua && b2 == ph[i].ub));
+                    for (int ta = 0; winTu && ta < 3; ta++)
+                        for (int tb = 0; winTu && tb < 3; tb++) {
+                            const unsigned want = kind[kk] == 0 ? (tb == 0 ? (1u | (ta == 0 ? 32u : 0u)) : 0u)
+                                                : kind[kk] == 1 ? (ta == 0 ? (2u | (tb == 0 ? 16u : 0u)) : 0u)
+                                                                : (ta == 0 && tb == 0 ? 4u : 0u);
+                            winTu = (prog[ph[i].progOff + (ta * 3 + tb) * kOrbMaxOps + kk] & 0x3fu) == want;
+                        }
+                }
+            }
+            // the blob
+            std::vector<uint32_t> bl;
+            auto put = [&](const void *src, size_t bytes) {
+                const size_t o = bl.size();
+                bl.resize(o + (bytes + 3) / 4);
+                memcpy(bl.data() + o, src, bytes);
+                return (int)o;
+            };
+            put(ph.data(), ph.size() * sizeof(WinPhaseDev));
+            while (bl.size() % 4) bl.push_back(0);  // int4 alignment
+            wp.oSubIdx = put(subIdx.data(), subIdx.size() * sizeof(int4));
+            wp.oSub = put(sub.data(), sub.size() * 4);
+            wp.oProg = put(prog.data(), prog.size() * 4);
+            wp.oMask = put(mask.data(), mask.size() * 4);
+            wp.oPat = put(pat.data(), pat.size() * 4);
+            wp.oCOff = put(coff.data(), coff.size() * 4);
+            wp.oCls = put(cls.data(), cls.size() * 4);
+            wp.oCStart = put(cstart.data(), cstart.size() * 4);
+            // the programs by (phase, operator): the 9 type words of one
+            // operator together (16 words, one wide scalar load)
+            while (bl.size() % 16) bl.push_back(0);
+            wp.oProgT = (int)bl.size();
+            for (size_t q = 0; q < ph.size(); q++)
+                for (int kk = 0; kk < kOrbMaxOps; kk++)
+                    for (int ty = 0; ty < 16; ty++)
+                        bl.push_back(ty < 9 ? prog[ph[q].progOff + ty * kOrbMaxOps + kk] : 0u);
+            while (bl.size() % 4) bl.push_back(0);
+            wp.blobOff = (int)blob.size();
+            wp.nblob = (int)bl.size();
+            blob.insert(blob.end(), bl.begin(), bl.end());
+            wpass.push_back(std::move(wp));
+        }
+        if (!upload_vec(d_wblob, blob, err) || !upload_vec(d_wrows, rows, err) || !upload_vec(d_wkp, kpOp, err))
+            return false;
+        hrows = rows;
+        // every tile listed until the reachable lists are computed
+        std::vector<int> tl;
+        for (WinPass &wp : wpass) {
+            wp.tileOff = (int)tl.size();
+            wp.ntiles = (int)wp.allTiles.size();
+            tl.insert(tl.end(), wp.allTiles.begin(), wp.allTiles.end());
+        }
+        if (!upload_vec(d_wtiles, tl, err)) return false;
+        // grids: as many blocks as the chip holds at once
+        std::vector<int> lrows(M, 0);
+        wslots = 1;
+        for (WinPass &wp : wpass) {
+            for (int dir = 0; dir < 2; dir++) {
+                const bool br = dir == 1;
+                const size_t lds = win_lds(wp, br);
+                if (lds > ldsDevice) {
+                    wpass.clear();
+                    winReg = false;
+                    return true;  // a tile does not fit: no window plan
+                }
+                const int nw = win_waves(br);
+                int nbk = 0;
+                for (int bc = 0; bc < (br ? 1 : 2); bc++) {
+                    const void *fn = winReg ? winr_fn(br, bc == 1) : win_fn(br, nw, bc == 1);
+                    HCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 1)));
+                }
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nbk, winReg ? winr_fn(br, false) : win_fn(br, nw, false), nw * kLanes, lds) != hipSuccess ||
+                    nbk < 1)
+                    nbk = 1;
+                // register tiles: one wave per tile
+                const long long work = winReg ? ((long long)wp.allTiles.size() + nw - 1) / nw : (long long)wp.allTiles.size();
+                const long long g = std::min<long long>((long long)nbk * numCU, work);
+                (br ? wp.gridA : wp.gridF) = (int)std::max<long long>(1, g);
+            }
+            for (int q = 0; q < wp.nkp; q++) lrows[kpOp[wp.kp0 + q]] = wp.gridA;
+            wslots = std::max(wslots, wp.gridA);
+        }
+        for (int k = 0; k < M; k++)
+            if (lrows[k] == 0) {
+                err = "window plan misses an operator";
+                return false;
+            }
+        if (!d_lrows.ensure(std::max(M, 1), err)) return false;
+        HCHK(hipMemcpy(d_lrows.p, lrows.data(), M * sizeof(int), hipMemcpyHostToDevice));
+        lslots = std::max(lslots, wslots);
+        winOn = true;
+        supDirty = true;
+        return true;
+    }
+
+    WinArgs winargs(const WinPass &wp) const
+    {
+        WinArgs a;
+        a.n = n;
+        a.w = std::min(std::min(winW, kWinMaxW), n);
+        a.blob = d_wblob.p + wp.blobOff;
+        a.w0 = wp.w0;
+        a.oSubIdx = wp.oSubIdx;
+        a.oSub = wp.oSub;
+        a.oProg = wp.oProg;
+        a.oMask = wp.oMask;
+        a.oPat = wp.oPat;
+        a.oCOff = wp.oCOff;
+        a.oCls = wp.oCls;
+        a.oCStart = wp.oCStart;
+        a.oProgT = wp.oProgT;
+        a.rows = d_wrows.p + wp.rowsOff;
+        a.nrowsA = wp.nrowsA;
+        a.tiles = d_wtiles.p + wp.tileOff;
+        a.ntiles = wp.ntiles;
+        a.nAcls = wp.nAcls;
+        a.nBcls = wp.nBcls;
+        a.nph = wp.nph;
+        a.nkp = wp.nkp;
+        a.kpOp = d_wkp.p + wp.kp0;
+        a.maxMembers = wp.maxMembers;
+        return a;
+    }
+
+    size_t win_lds(const WinPass &wp, bool br) const
+    {
+        if (winReg) return br ? (size_t)2 * wp.nkp * kLanes * sizeof(double) : (size_t)wp.nkp * kLanes * 16;
+        return win_lds_bytes(br, winWaves, wp.maxMembers, wp.nkp);
+    }
+
+    // Reachable tiles of the window passes: S_0 = the start state's nonzero
+    // determinants; pass p processes the tiles meeting S_p and S_{p+1} = every
+    // member of those tiles.  A tile outside holds exact zeros before and
+    // after the pass.  The adjoint pass p takes the same tiles: S_{p+1} is the
+    // union of them, and the lambda of a determinant outside S_{p+1} never
+    // meets a nonzero state in the passes before it.
+    bool compute_support_win(bool ck, std::string &err)
+    {
+        supDirty = false;
+        supCk = ck;
+        supGen++;
+        const bool lists_on = reachOpt && (long long)hstart.size() == N;
+        std::vector<uint8_t> S(N, (uint8_t)1);
+        if (lists_on)
+            for (long long i = 0; i < N; i++) S[i] = hstart[i] != 0.0;
+        ampF = ampAL = ampAP = 0;
+        std::vector<int> tl;
+        std::vector<long long> mem;
+        for (WinPass &wp : wpass) {
+            wp.tileOff = (int)tl.size();
+            const uint32_t *rA = hrows.data() + wp.rowsOff, *rB = rA + wp.nrowsA;
+            for (int t : wp.allTiles) {
+                const int ia = t / wp.nBcls, ib = t - ia * wp.nBcls;
+                mem.clear();
+                bool any = false;
+                for (int i = 0; i < wp.cntA[ia]; i++)
+                    for (int j = 0; j < wp.cntB[ib]; j++) {
+                        const long long d = (long long)rA[wp.startA[ia] + i] + rB[wp.startB[ib] + j];
+                        mem.push_back(d);
+                        any = any || S[d];
+                    }
+                if (!any) continue;
+                tl.push_back(t);
+                for (long long d : mem) S[d] = 1;
+                ampF += (long long)mem.size();
+            }
+            wp.ntiles = (int)tl.size() - wp.tileOff;
+        }
+        ampAL = ampAP = ampF;
+        if (!upload_vec(d_wtiles, tl, err)) return false;
+        return true;
+    }
+
+    bool sweep_forward_win(double *v, int nch, std::string &err)
+    {
+        const long long cstride = N * kLanes;
+        const dim3 blk(winWaves * kLanes);
+        for (size_t p = 0; p < wpass.size(); p++) {
+            const WinPass &wp = wpass[p];
+            const WinArgs a = winargs(wp);
+            const size_t lds = win_lds(wp, false);
+            const double *in = ckOn ? ck_state((int)p, nch) : v;
+            double *out = ckOn ? ck_state((int)p + 1, nch) : v;
+            const bool bc = p == 0 && ckOn;  // pass 0 reads the start vector
+            if (bc) in = d_start.p;
+            const int mk = mark_begin(1);
+            if (winReg) {
+                const dim3 grid((unsigned)std::max(1, std::min(wp.gridF, (wp.ntiles + 3) / 4)), nch);
+                if (winTu) {
+                    if (bc) hipLaunchKernelGGL((k_winr<false, true, 4, true>), grid, dim3(4 * kLanes), lds, stream, a,
+                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                    else hipLaunchKernelGGL((k_winr<false, false, 4, true>), grid, dim3(4 * kLanes), lds, stream, a,
+                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                } else {
+                    if (bc) hipLaunchKernelGGL((k_winr<false, true, 4, false>), grid, dim3(4 * kLanes), lds, stream, a,
+                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                    else hipLaunchKernelGGL((k_winr<false, false, 4, false>), grid, dim3(4 * kLanes), lds, stream, a,
+                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                }
+                mark_end(mk);
+                LCHK();
+                continue;
+            }
+            const dim3 grid((unsigned)std::max(1, std::min(wp.gridF, wp.ntiles)), nch);
+            if (winWaves == 8) {
+                if (bc) hipLaunchKernelGGL((k_win<false, true, 8>), grid, blk, lds, stream, a, in, out, nullptr,
+                                           cstride, bl_cs.p, M, nullptr, 0);
+                else hipLaunchKernelGGL((k_win<false, false, 8>), grid, blk, lds, stream, a, in, out, nullptr,
+                                        cstride, bl_cs.p, M, nullptr, 0);
+            } else {
+                if (bc) hipLaunchKernelGGL((k_win<false, true, 4>), grid, blk, lds, stream, a, in, out, nullptr,
+                                           cstride, bl_cs.p, M, nullptr, 0);
+                else hipLaunchKernelGGL((k_win<false, false, 4>), grid, blk, lds, stream, a, in, out, nullptr,
+                                        cstride, bl_cs.p, M, nullptr, 0);
+            }
+            mark_end(mk);
+            LCHK();
+        }
+        return true;
+    }
+
+    // every block writes its bracket rows, tiles or not (wp.gridA rows per operator)
+    bool sweep_reverse_win(double *phi, double *lam, int nch, std::string &err)
+    {
+        const long long cstride = N * kLanes;
+        const dim3 blk(winWaves * kLanes);
+        for (int p = (int)wpass.size() - 1; p >= 0; p--) {
+            const WinPass &wp = wpass[p];
+            const WinArgs a = winargs(wp);
+            const size_t lds = win_lds(wp, true);
+            const double *in = ckOn ? ck_state(p + 1, nch) : phi;
+            double *out = ckOn ? nullptr : phi;
+            const dim3 grid((unsigned)wp.gridA, nch);
+            const int mk = mark_begin(2);
+            if (winReg && winTu)
+                hipLaunchKernelGGL((k_winr<true, false, 2, true>), grid, dim3(2 * kLanes), lds, stream, a, in, out,
+                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+            else if (winReg)
+                hipLaunchKernelGGL((k_winr<true, false, 2, false>), grid, dim3(2 * kLanes), lds, stream, a, in, out,
+                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+            else if (winWaves == 8)
+                hipLaunchKernelGGL((k_win<true, false, 8>), grid, blk, lds, stream, a, in, out, lam, cstride,
+                                   bl_csneg.p, M, bl_partial.p, lslots);
+            else
+                hipLaunchKernelGGL((k_win<true, false, 4>), grid, blk, lds, stream, a, in, out, lam, cstride,
+                                   bl_csneg.p, M, bl_partial.p, lslots);
+            mark_end(mk);
+            LCHK();
         }
         return true;
     }
@@ -1553,6 +2099,7 @@ struct EngineImpl {
     // checkpoint mode with a two-group first segment.
     bool first_reads_start() const
     {
+        if (winOn) return ckOn;  // window pass 0 reads the start vector
         return ckOn && !laneF.segs.empty() && laneF.segs[0].two && !laneF.segs[0].three;
     }
 
@@ -1563,6 +2110,7 @@ struct EngineImpl {
     // events.
     bool sweep_forward_lanes(double *v, int nch, std::string &err)
     {
+        if (winOn) return sweep_forward_win(v, nch, err);
         const long long cstride = N * kLanes;
         for (size_t si = 0; si < laneF.segs.size(); si++) {
             const Seg &sg = laneF.segs[si];
@@ -1614,6 +2162,7 @@ struct EngineImpl {
     // not write it back (only lambda is updated in place).
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
     {
+        if (winOn) return sweep_reverse_win(phi, lam, nch, err);
         const long long cstride = N * kLanes;
         for (int si = (int)lsegs.size() - 1; si >= 0; si--) {
             const Seg &sg = lsegs[si];
@@ -2105,6 +2654,8 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         e.laneF.segs.clear();
         e.laneS.segs.clear();
         e.sweepS = e.laneSAllTwo = false;
+        e.winOn = false;
+        e.wpass.clear();
     } else if (!e.build_lane_segments(err)) {
         return false;
     }
@@ -2715,10 +3266,10 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
             return false;
         // stored forward states 0 .. segments, when they fit in half the
         // free memory
-        bool ck = e.ckOpt && e.ckShape && e.hkind == 3 && e.sigv;
+        bool ck = e.ckOpt && (e.ckShape || e.winOn) && e.hkind == 3 && e.sigv;
         const double *ckWas = e.bl_ck.p;
         if (ck) {
-            const size_t need = (e.laneF.segs.size() + 1) * nch * (size_t)e.N * kLanes;
+            const size_t need = (size_t)(e.nfwd() + 1) * nch * (size_t)e.N * kLanes;
             size_t freeB = 0, totalB = 0;
             HCHK(hipMemGetInfo(&freeB, &totalB));
             const size_t have = e.bl_ck.n;
@@ -2783,7 +3334,7 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
         if (!e.sweep_forward_lanes(psi0, nch, err)) return false;
         if (e.hkind == 3 && e.sigv) {
             // matrix-free sigma in lane layout: no transposes
-            const double *psi = e.ckOn ? e.ck_state((int)e.laneF.segs.size(), nch) : e.bl_psi.p;
+            const double *psi = e.ckOn ? e.ck_state(e.nfwd(), nch) : e.bl_psi.p;
             if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
             if (with_gradient) {
                 if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
@@ -3018,6 +3569,22 @@ bool Engine::set_option(int key, int value, std::string &err)
         e.bnvec = 0;  // decided at the next upload
         e.drop_graphs();
         return true;
+    case kOptWindow:
+    case kOptWindowWaves:
+        if (key == kOptWindow ? (value != 0 && (value < 2 || value > kWinMaxW)) : (value != 4 && value != 8)) {
+            err = key == kOptWindow ? "window width must be 0 or 2 to 8" : "window waves per block must be 4 or 8";
+            return false;
+        }
+        if (value == (key == kOptWindow ? e.winW : e.winWaves)) return true;
+        (key == kOptWindow ? e.winW : e.winWaves) = value;
+        if (e.M > 0 && !e.cplx) {
+            e.drop_graphs();
+            if (!e.build_lane_segments(err)) return false;
+        }
+        e.bnvec = 0;  // stored states and lists are renewed at the next upload
+        return true;
+    case kOptWindowPasses:
+    case kOptWindowRegisters:
     case kOptHamKind:
     case kOptSegments:
     case kOptLaneSegments:
@@ -3064,6 +3631,10 @@ int Engine::option(int key) const
     case kOptSweepAmpAdjointLambda: return (int)std::min<long long>(e.ampAL, INT32_MAX);
     case kOptSweepAmpAdjointState: return (int)std::min<long long>(e.ampAP, INT32_MAX);
     case kOptLaneSegments: return (int)e.lsegs.size();
+    case kOptWindow: return e.winW;
+    case kOptWindowWaves: return e.winWaves;
+    case kOptWindowPasses: return e.winOn ? (int)e.wpass.size() : 0;
+    case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }
 }

@@ -497,6 +497,97 @@ __global__ void k_sig1r(Sig1Args a);
 __global__ void k_dot_rows(const double *__restrict__ x, const double *__restrict__ y, long long N, long long vstride,
                            double *__restrict__ part);  // grid (G, nvec)
 
+
+// ------------------------------------------------- window passes (lane layout)
+// A window is a set of w spatial orbitals (the same bits in the alpha and in
+// the beta string).  Determinants with the same bits outside the window in
+// both strings form a tile: (alpha outer class, beta outer class), members =
+// the inner patterns of the class's inner electron counts, C(w, ca) x
+// C(w, cb).  Every orbit segment (phase) whose bits lie inside the window
+// maps each tile into itself, so one pass stages a tile of V vectors in LDS
+// and applies all of the pass's phases there: a sweep reads and writes the
+// state once per pass instead of once per segment.
+constexpr int kWinMaxW = 8;        // window width
+constexpr int kWinMaxOps = 64;     // operators per pass
+constexpr int kWinMaxPhases = 32;  // phases per pass
+struct WinPhaseDev {
+    int k0, nk;       // operators k0 .. k0 + nk - 1 (program order)
+    int progOff;      // orbit program words at prog[progOff + type * kOrbMaxOps + kk] (within the pass)
+    int kp;           // position of operator k0 in the pass's operator list
+    uint32_t ua, ub;  // the phase's alpha / beta bits (string level)
+    uint32_t spect;   // full-string mask of its spectator bits
+    uint32_t pad;
+};
+// Tables of one pass, one blob of 32-bit words in device memory, read by
+// the kernel through the scalar unit (every value is wave-uniform); word
+// offsets:
+//   0       phases (WinPhaseDev, 8 words each)
+//   oSubIdx sub-class index, 4 words per (phase, spin, inner count) at
+//           (phase * 2 + spin) * (w + 1) + count: {offset in sub, n | n2 << 16,
+//           magic(n), magic(n2)}, n sub-classes of which the first n2 have two
+//           members, magic(d) = ceil(2^24 / d) (x / d = (x * magic) >> 24 for
+//           x < 2^24 / d)
+//   oSub    sub-class entries: i0 | i1 << 8 (0xff: none) | shape << 16, the
+//           inner patterns (indices within their count) agreeing outside
+//           the phase's bits; shape as in OrbArgs
+//   oProg   orbit programs of the phases (9 types x kOrbMaxOps words each)
+//   oMask   sign masks of the pass's operators, in pass order
+//   oPat    placed inner patterns by (count, value), 2^w
+//   oCOff   w + 2 offsets of the counts in oPat
+//   oCls    outer classes, alpha then beta: outer bits | inner count << 24
+//   oCStart first member of each class in rows (alpha) / rows + nrowsA (beta)
+//   oProgT  the programs again by (phase, operator): 16 words per operator,
+//           its 9 type words first (16-word aligned)
+struct WinArgs {
+    int n, w, w0;
+    const uint32_t *blob;
+    int oSubIdx, oSub, oProg, oMask, oPat, oCOff, oCls, oCStart, oProgT;
+    // members of the outer classes by class: alpha rank * nb (nrowsA words),
+    // then beta ranks
+    const uint32_t *rows;
+    int nrowsA;
+    const int *tiles;  // tiles to process: alpha class * nBcls + beta class
+    int ntiles, nAcls, nBcls;
+    int nph, nkp;
+    const int *kpOp;   // operator of each pass position
+    int maxMembers;    // largest tile of the pass
+};
+// Dynamic LDS of k_win for a pass (bytes): the tile of 64 vectors (psi, and
+// lambda in the adjoint), the adjoint's per-wave bracket rows of one phase
+// and the pass's bracket rows.
+inline size_t win_lds_bytes(bool br, int waves, int maxMembers, int nkp)
+{
+    size_t b = (size_t)maxMembers * 64 * sizeof(double) * (br ? 2 : 1);
+    if (br) b += ((size_t)waves * kOrbMaxOps + (size_t)nkp) * 64 * sizeof(double);
+    return b;
+}
+// One pass over the listed tiles, grid (blocks, chunks), NW waves per block,
+// lane = vector of the 64-vector chunk; block b takes the tiles b + j * grid.
+// Forward (BR false): in -> out (may be equal); BC: in is one plain vector
+// broadcast to every lane.  Adjoint (BR): phases and operators in reverse
+// with the negated angles, psi from in (written back to out unless null),
+// lambda in place, per-operator brackets into partial rows (slot = b).
+template <bool BR, bool BC, int NW>
+__global__ void k_win(WinArgs a, const double *in, double *out, double *lam, long long cstride,
+                      const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
+
+// Window passes over four consecutive orbitals whose phases are TUPS units
+// (the same adjacent pair in both spins): one wave holds a whole tile (up to
+// 6 x 6 members for 64 vectors) in registers, the member indices of every
+// sub-orbit fixed at compile time per pair of inner counts, so no LDS or
+// barrier is needed between phases.  Grid-stride over the listed tiles by
+// wave (NW waves per block); the forward stages the pass's angles in LDS
+// (nkp x 64 double2); the adjoint keeps its bracket rows per wave in LDS
+// (nkp x 64 doubles each) and writes the block's sum as its partial row
+// (slot = block).  Arguments as k_win, with WinArgs::w0 the window's first
+// orbital.
+// TU: every phase is a TUPS unit of five operators (beta single, alpha
+// single, paired double, beta single, alpha single) whose program words
+// rotate exactly the member pairs of that pattern (checked on the host).
+template <bool BR, bool BC, int NW, bool TU>
+__global__ void k_winr(WinArgs a, const double *in, double *out, double *lam, long long cstride,
+                       const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
+
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>
 __global__ void k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut,

@@ -319,7 +319,21 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     vector and step, summed over the segments: read and written by the
  *     forward sweep (19), of the adjoint vector read and written by the
  *     adjoint sweep (20), of the stored forward states read by it (21).
- * Changing keys 0, 3, 4, 7, 9, 10 or 18 re-segments the operators or changes the
+ *  25 window passes for the lane-layout sweeps: w = 2 to 8 runs each sweep
+ *     as passes over windows of w consecutive spatial orbitals, each pass
+ *     staging the determinants that agree outside its window in LDS and
+ *     applying every operator run inside the window there (one read and
+ *     write of the state per pass instead of per run); 0: orbit segments.
+ *     Used when every operator falls into a run on at most two alpha and
+ *     two beta orbitals spanning at most w orbitals;
+ *  26 waves per block of the window-pass kernels: 4 (default) or 8;
+ *  27 (read only) window passes per sweep, 0 when they are not in use;
+ *  28 (read only) 1 when the window passes keep each tile in registers
+ *     (width 4 and every operator run on one adjacent orbital pair in both
+ *     spins), 2 when in addition every run is a TUPS unit of five operators
+ *     (beta single, alpha single, paired double, beta single, alpha single:
+ *     straight-line phases), 0 when they stage tiles in LDS or are not used.
+ * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */
 int setEngineOption(int key, int value, void *ctx);
