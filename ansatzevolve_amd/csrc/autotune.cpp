@@ -2,8 +2,9 @@
 //
 // Candidates are execution choices that leave the mathematics unchanged:
 // the sweep kind (register orbits, LDS-staged orbits, LDS tiles only), the
-// fusion depth (most operators per pass) and the batch layout (lane layout
-// or one vector per grid slice).  Each candidate runs the caller's angle
+// fusion depth (most operators per pass), the batch layout (lane layout
+// or one vector per grid slice) and, in the lane layout, orbit segments or
+// window passes.  Each candidate runs the caller's angle
 // batch once to warm up (and capture its hipGraph), then `reps` timed
 // replays; the median wall time decides among the candidates whose results
 // agree.  Every candidate's energies and per-operator derivatives are
@@ -57,6 +58,7 @@ bool all_finite(const std::vector<double> &a)
 
 struct Candidate {
     int sweep, depth, laneMin, groups, groupsFwd, ckpt, halves, grouped;
+    int window = 0;            // kOptWindow: 0 orbit segments, -1 automatic window passes
     int gemmB = 0, gemmA = 0;  // rocBLAS solutions of the sigma DGEMMs (0: default; tuned after the choice)
 };
 
@@ -72,8 +74,8 @@ std::string cand_json(const Candidate &c)
     std::ostringstream o;
     o << "{\"sweep\": " << c.sweep << ", \"depth\": " << c.depth << ", \"lane_min\": " << c.laneMin
       << ", \"groups\": " << c.groups << ", \"groups_fwd\": " << c.groupsFwd << ", \"checkpoint\": " << c.ckpt
-      << ", \"sigma_halves\": " << c.halves << ", \"sigma_grouped\": " << c.grouped << ", \"gemm_beta\": " << c.gemmB
-      << ", \"gemm_alpha\": " << c.gemmA << "}";
+      << ", \"sigma_halves\": " << c.halves << ", \"sigma_grouped\": " << c.grouped << ", \"window\": " << c.window
+      << ", \"gemm_beta\": " << c.gemmB << ", \"gemm_alpha\": " << c.gemmA << "}";
     return o.str();
 }
 
@@ -91,9 +93,9 @@ bool lookup(const std::string &file, const std::string &key, Candidate &c)
         if (std::sscanf(line.c_str() + line.find("\"choice\""),
                         "\"choice\": {\"sweep\": %d, \"depth\": %d, \"lane_min\": %d, \"groups\": %d, "
                         "\"groups_fwd\": %d, \"checkpoint\": %d, \"sigma_halves\": %d, \"sigma_grouped\": %d, "
-                        "\"gemm_beta\": %d, \"gemm_alpha\": %d}",
+                        "\"window\": %d, \"gemm_beta\": %d, \"gemm_alpha\": %d}",
                         &t.sweep, &t.depth, &t.laneMin, &t.groups, &t.groupsFwd, &t.ckpt, &t.halves, &t.grouped,
-                        &t.gemmB, &t.gemmA) == 10) {
+                        &t.window, &t.gemmB, &t.gemmA) == 11) {
             c = t;
             found = true;
         }
@@ -107,6 +109,7 @@ bool apply(Engine &eng, const Candidate &c, std::string &err)
            eng.set_option(kOptLaneGroups, c.groups, err) && eng.set_option(kOptLaneGroupsFwd, c.groupsFwd, err) &&
            eng.set_option(kOptLaneMin, c.laneMin, err) && eng.set_option(kOptCheckpoint, c.ckpt, err) &&
            eng.set_option(kOptSigmaHalves, c.halves, err) && eng.set_option(kOptSigmaGrouped, c.grouped, err) &&
+           eng.set_option(kOptWindow, c.window, err) &&
            eng.set_option(kOptGemmSolutionBeta, c.gemmB, err) && eng.set_option(kOptGemmSolutionAlpha, c.gemmA, err) &&
            eng.set_option(kOptGraph, 1, err);
 }
@@ -164,6 +167,11 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
                     cands.push_back({s, d, 1, 3, 3, 1, 0, gr});
                     if (gr) cands.push_back({s, d, 1, 2, 2, 1, 0, 0});
                     if (eng.option(kOptSigmaHalvesAvailable) == 1) cands.push_back({s, d, 1, 2, 2, 1, 1, 0});
+                    // window passes where their tiles fit in registers
+                    // (the engine's automatic choice), with and without
+                    // the stored states
+                    cands.push_back({s, d, 1, 2, 2, 1, 0, gr, -1});
+                    cands.push_back({s, d, 1, 2, 2, 0, 0, gr, -1});
                 }
             }
         }
@@ -171,6 +179,7 @@ bool Manager::autotune(int nvec, int nangles, const double *angles, const char *
     Candidate saved{eng.option(kOptSweep),         eng.option(kOptFuseDepth),     eng.option(kOptLaneMin),
                     eng.option(kOptLaneGroups),    eng.option(kOptLaneGroupsFwd), eng.option(kOptCheckpoint),
                     eng.option(kOptSigmaHalves),   eng.option(kOptSigmaGrouped)};
+    saved.window = eng.option(kOptWindow);
     saved.gemmB = eng.option(kOptGemmSolutionBeta);
     saved.gemmA = eng.option(kOptGemmSolutionAlpha);
     std::vector<double> E0(nvec), D0((size_t)nvec * M), E(nvec), D((size_t)nvec * M);
@@ -277,9 +286,9 @@ bool Manager::getEngineOption(int key, int &value)
 {
     if (!ensureBuilt()) return false;
     value = eng_->option(key);
-    // rocBLAS solution indices may be negative; every other option is >= 0
-    // and -1 marks an unknown key
-    if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha) return true;
+    // rocBLAS solution indices and the window width (-1: automatic) may be
+    // negative; every other option is >= 0 and -1 marks an unknown key
+    if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha || key == kOptWindow) return true;
     return value >= 0;
 }
 

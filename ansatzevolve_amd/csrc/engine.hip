@@ -762,7 +762,9 @@ struct EngineImpl {
         int tileOff = 0, ntiles = 0; // the tiles a sweep processes (d_wtiles)
         int gridF = 1, gridA = 1;    // blocks (multiples of 64 / V)
     };
-    int winW = 0;        // kOptWindow: window width in spatial orbitals (0: off)
+    // kOptWindow: window width in spatial orbitals (0: off; -1, the default:
+    // four-orbital windows when their tiles fit in registers, else off)
+    int winW = -1;
     int winWaves = 4;    // kOptWindowWaves: waves per block of the window kernels (4 or 8)
     bool winOn = false;  // a plan exists: lane-layout sweeps run as window passes
     // the plan's windows are four orbitals wide and every phase is a TUPS
@@ -809,8 +811,8 @@ struct EngineImpl {
         winOn = false;
         winReg = winTu = false;
         wpass.clear();
-        if (winW <= 0 || mode != kSz || cplx || M == 0) return true;
-        const int w = std::min(std::min(winW, kWinMaxW), n);
+        if (winW == 0 || mode != kSz || cplx || M == 0) return true;
+        const int w = std::min(std::min(winW < 0 ? 4 : winW, kWinMaxW), n);
         const uint32_t amask = (1u << n) - 1;
         auto bitsA = [&](int kk) { return ((hops[kk].cre | hops[kk].ann) >> n) & amask; };
         auto bitsB = [&](int kk) { return (hops[kk].cre | hops[kk].ann) & amask; };
@@ -903,6 +905,7 @@ struct EngineImpl {
             winReg = winReg && p.ua == p.ub && p.ua == (lo | (lo << 1));
         }
         winTu = winReg;  // narrowed per phase below
+        if (winW < 0 && !winReg) return true;  // automatic: register tiles only
         // tables: one blob of 32-bit words per pass and the class member
         // lists (rows)
         const uint32_t full = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1);
@@ -1126,7 +1129,7 @@ struct EngineImpl {
     {
         WinArgs a;
         a.n = n;
-        a.w = std::min(std::min(winW, kWinMaxW), n);
+        a.w = std::min(std::min(winW < 0 ? 4 : winW, kWinMaxW), n);
         a.blob = d_wblob.p + wp.blobOff;
         a.w0 = wp.w0;
         a.oSubIdx = wp.oSubIdx;
@@ -3571,8 +3574,9 @@ bool Engine::set_option(int key, int value, std::string &err)
         return true;
     case kOptWindow:
     case kOptWindowWaves:
-        if (key == kOptWindow ? (value != 0 && (value < 2 || value > kWinMaxW)) : (value != 4 && value != 8)) {
-            err = key == kOptWindow ? "window width must be 0 or 2 to 8" : "window waves per block must be 4 or 8";
+        if (key == kOptWindow ? (value != 0 && value != -1 && (value < 2 || value > kWinMaxW))
+                              : (value != 4 && value != 8)) {
+            err = key == kOptWindow ? "window width must be -1, 0 or 2 to 8" : "window waves per block must be 4 or 8";
             return false;
         }
         if (value == (key == kOptWindow ? e.winW : e.winWaves)) return true;

@@ -62,7 +62,8 @@ enum EngineOption : int {
                           // a count whose LDS exceeds the device's falls back to the largest that fits
     kOptVectorSegments = 24,  // read only: segments per vector-major sweep (launches, the two-group list with kind 3)
     kOptWindow = 25,          // lane-layout sweeps as window passes over w consecutive spatial orbitals (2 to 8;
-                              // 0: orbit segments); used when every operator falls into a window phase
+                              // 0: orbit segments; -1, the default: w = 4 when the tiles fit in registers, else
+                              // orbit segments); used when every operator falls into a window phase
     kOptWindowWaves = 26,     // waves per block of the window-pass kernels: 4 (default) or 8
     kOptWindowPasses = 27,    // read only: window passes per sweep (0: window passes not in use)
     kOptWindowRegisters = 28, // read only: 1 when the window passes hold their tiles in registers (four-orbital

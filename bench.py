@@ -255,10 +255,11 @@ def main():
                 "sigma_gemms": rep.get("sigma_gemms"),
                 "candidates": [[c["choice"]["sweep"], c["choice"]["depth"], c["choice"]["lane_min"],
                                 c["choice"]["groups"], c["choice"]["groups_fwd"], c["choice"]["checkpoint"],
-                                c["choice"]["sigma_halves"], c["choice"]["sigma_grouped"], round(c["ms"], 3)]
+                                c["choice"]["sigma_halves"], c["choice"]["sigma_grouped"], c["choice"]["window"],
+                                round(c["ms"], 3)]
                                for c in rep.get("candidates", [])],
                 "candidate_fields": ["sweep", "depth", "lane_min", "groups", "groups_fwd", "checkpoint",
-                                     "sigma_halves", "sigma_grouped", "ms"]}
+                                     "sigma_halves", "sigma_grouped", "window", "ms"]}
     if args.lane_groups:
         a.set_option(a.OPT_LANE_GROUPS, args.lane_groups)
         a.set_option(a.OPT_LANE_GROUPS_FWD, args.lane_groups)
@@ -268,12 +269,15 @@ def main():
             "lane_segments": a.get_option(a.OPT_LANE_SEGMENTS), "checkpoint": a.get_option(a.OPT_CHECKPOINT),
             "sigma_halves": a.get_option(a.OPT_SIGMA_HALVES), "sigma_grouped": a.get_option(a.OPT_SIGMA_GROUPED),
             "gemm_solution_beta": a.get_option(a.OPT_GEMM_SOLUTION_BETA),
-            "gemm_solution_alpha": a.get_option(a.OPT_GEMM_SOLUTION_ALPHA)}
+            "gemm_solution_alpha": a.get_option(a.OPT_GEMM_SOLUTION_ALPHA),
+            "window": a.get_option(a.OPT_WINDOW)}
 
     # timed region: one hipGraph launch per step, inputs resident in HBM
     a.upload_batch(thetas)
     opts["checkpoint_active"] = a.get_option(a.OPT_CHECKPOINT_ACTIVE)
     opts["reachable"] = a.get_option(a.OPT_REACHABLE)
+    opts["window_passes"] = a.get_option(a.OPT_WINDOW_PASSES)
+    opts["window_registers"] = a.get_option(a.OPT_WINDOW_REGISTERS)
     amp = {"forward": a.get_option(a.OPT_SWEEP_AMP_FORWARD),
            "adjoint_lambda": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_LAMBDA),
            "adjoint_state": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_STATE)}
@@ -354,6 +358,11 @@ def main():
         else ("k_orb2s<false>" if two_vm else "k_orb<false>")
     k_rev = ("k_orb2v<true>" if opts["lane_groups"] == 2 else "k_orbv<true>") + " (lane layout)" if lanes \
         else ("k_orb2s<true>" if two_vm else "k_orb<true>")
+    if lanes and opts["window_passes"] > 0:
+        # window passes: register tiles (k_winr; TUPS units straight-line) or LDS tiles (k_win)
+        form = {2: "k_winr<%s, TUPS units>", 1: "k_winr<%s>"}.get(opts["window_registers"], "k_win<%s>")
+        k_fwd = (form % "false") + " (lane layout, %d window passes)" % opts["window_passes"]
+        k_rev = (form % "true") + " (lane layout, %d window passes)" % opts["window_passes"]
     if lanes:
         fwd_launches, rev_launches = max(1, fwd_n // nbd), max(1, rev_n // nbd)
         fwd_pass = 16.0 * amp["forward"] * args.nvec / fwd_launches

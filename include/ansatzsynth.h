@@ -323,9 +323,10 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     as passes over windows of w consecutive spatial orbitals, each pass
  *     staging the determinants that agree outside its window in LDS and
  *     applying every operator run inside the window there (one read and
- *     write of the state per pass instead of per run); 0: orbit segments.
- *     Used when every operator falls into a run on at most two alpha and
- *     two beta orbitals spanning at most w orbitals;
+ *     write of the state per pass instead of per run); 0: orbit segments;
+ *     -1 (default): w = 4 when the tiles fit in registers (key 28), else
+ *     orbit segments.  Used when every operator falls into a run on at most
+ *     two alpha and two beta orbitals spanning at most w orbitals;
  *  26 waves per block of the window-pass kernels: 4 (default) or 8;
  *  27 (read only) window passes per sweep, 0 when they are not in use;
  *  28 (read only) 1 when the window passes keep each tile in registers

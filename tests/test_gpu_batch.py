@@ -85,6 +85,7 @@ def test_two_group_segments_fuse_units(gpu):
     and as the single-vector path."""
     w, thetas = _tups_case(8, 4, 1212, 40)
     with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_WINDOW, 0)  # orbit segments, not window passes
         a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 1)
         E1, G1 = a.energy_gradients(thetas)
@@ -110,6 +111,7 @@ def test_three_group_segments(gpu):
     full and one partial chunk."""
     w, thetas = _tups_case(8, 4, 1313, 70)
     with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_WINDOW, 0)  # orbit segments, not window passes
         a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
         a.set_option(AnsatzSynth.OPT_CHECKPOINT, 0)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 1)
@@ -138,6 +140,7 @@ def test_stored_forward_states(gpu, groups):
     the forward states differ only by rounding of the backward un-rotation."""
     w, thetas = _tups_case(8, 4, 4242, 130)
     with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_WINDOW, 0)  # orbit segments, not window passes
         a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, groups)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, groups)
@@ -175,6 +178,7 @@ def test_reachable_orbits_exact(gpu, ck):
     for idx, cf in (([hf + 1], [1.0]), ([hf + 1, other + 1], [0.8, 0.6])):
         with from_workload(w, storage=2) as a:
             a.set_initial_state(w["nq"], idx, cf)
+            a.set_option(AnsatzSynth.OPT_WINDOW, 0)  # orbit segments, not window passes
             a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
             a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
             a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
@@ -230,6 +234,7 @@ def test_reachable_amplitude_counts(gpu):
     w, thetas = _tups_case(n, layers, 5353, 64)
     nseg, fwd, lam, state, N = _reachable_model(n, layers, sum(1 << i for i in range(0, n, 2)))
     with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_WINDOW, 0)  # orbit segments, not window passes
         a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS, 2)
         a.set_option(AnsatzSynth.OPT_LANE_GROUPS_FWD, 2)
@@ -340,19 +345,23 @@ def test_batch_graph_replays(gpu, lane_min, storage):
 
 
 def test_options_and_fusion_depth(gpu):
-    """Every sweep kind and fusion depth gives the same batch."""
+    """Every sweep kind, fusion depth and lane-layout sweep form (orbit
+    segments with one or two groups, window passes) gives the same batch."""
     w, thetas = _tups_case(6, 2, 31, 20)
     with from_workload(w) as a:
+        assert a.get_option(AnsatzSynth.OPT_WINDOW) == -1  # automatic
         base = a.energy_gradients(thetas)
-        for sweep, depth, lanes, groups in itertools.product((0, 1, 2, 3), (0, 1, 2, 3, 7), (0, 1), (1, 2)):
+        for sweep, depth, lanes, groups, win in itertools.product((0, 1, 2, 3), (0, 1, 2, 3, 7), (0, 1), (1, 2),
+                                                                  (0, -1)):
             a.set_option(AnsatzSynth.OPT_SWEEP, sweep)
             a.set_option(AnsatzSynth.OPT_FUSE_DEPTH, depth)
             a.set_option(AnsatzSynth.OPT_LANE_GROUPS, groups)
             a.set_option(AnsatzSynth.OPT_LANE_MIN, lanes)
+            a.set_option(AnsatzSynth.OPT_WINDOW, win)
             assert a.get_option(AnsatzSynth.OPT_SWEEP) == sweep
             E, G = a.energy_gradients(thetas)
-            assert np.max(np.abs(E - base[0])) <= SAME, (sweep, depth, lanes, groups)
-            assert np.max(np.abs(G - base[1])) <= SAME, (sweep, depth, lanes, groups)
+            assert np.max(np.abs(E - base[0])) <= SAME, (sweep, depth, lanes, groups, win)
+            assert np.max(np.abs(G - base[1])) <= SAME, (sweep, depth, lanes, groups, win)
 
 
 def test_autotune_invariance_and_cache(gpu, tmp_path):

@@ -152,10 +152,11 @@ def test_config3_energy_gradient(gpu, storage):
 def test_config3_benchmarked_batch_path(gpu):
     """The path the throughput benchmark times: 12 orbitals, 853,776
     determinants, TUPS L=6, a 64-vector batch after the autotuner (lane
-    layout, stored forward states, reachable-amplitude lists, grouped-target
-    sigma, tuned DGEMM solutions), replayed from its hipGraph.  Vector 0 is
-    theta_0 and is checked against the reference CPU build's golden vector;
-    other vectors against single-vector calls."""
+    layout, window passes with the tiles in registers, stored forward states,
+    reachable tiles, grouped-target sigma, tuned DGEMM solutions), replayed
+    from its hipGraph.  Vector 0 is theta_0 and is checked against the
+    reference CPU build's golden vector; other vectors against single-vector
+    calls."""
     gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
     if "cfg3_E" not in gz.files:
         pytest.skip("no 12-orbital golden vector")
@@ -167,12 +168,14 @@ def test_config3_benchmarked_batch_path(gpu):
     with from_workload(w) as a:
         rep = a.autotune(thetas, reps=1)
         assert rep["rejected"] == 0
-        # the benchmarked choice whatever the timing said: lane layout, two
-        # orbital groups per pass, stored states (the tuned DGEMMs stay)
-        for key, val in ((AnsatzSynth.OPT_LANE_MIN, 1), (AnsatzSynth.OPT_LANE_GROUPS, 2),
-                         (AnsatzSynth.OPT_LANE_GROUPS_FWD, 2), (AnsatzSynth.OPT_CHECKPOINT, 1)):
+        # the benchmarked choice whatever the timing said: lane layout,
+        # automatic window passes, stored states (the tuned DGEMMs stay)
+        for key, val in ((AnsatzSynth.OPT_LANE_MIN, 1), (AnsatzSynth.OPT_WINDOW, -1),
+                         (AnsatzSynth.OPT_CHECKPOINT, 1)):
             a.set_option(key, val)
         a.upload_batch(thetas)
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 17
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_REGISTERS) == 2
         assert a.get_option(AnsatzSynth.OPT_HAM_KIND) == 3
         assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1
         assert a.get_option(AnsatzSynth.OPT_REACHABLE) == 1
