@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--no-extras", action="store_true",
                    help="skip the Hessian timings and the multi-instance launch comparison reported after the "
                         "timed region")
+    p.add_argument("--eager", action="store_true",
+                   help="launch every kernel directly instead of replaying hipGraphs (for profiler counter passes)")
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain 2 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -162,6 +164,40 @@ def extra_measurements(a, w):
     return out
 
 
+def calib_copy(nbytes):
+    """hipMalloc two buffers, fill one, copy it to the other (device to
+    device), free both; returns 0 on success."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    size = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(a), size) or hip.hipMalloc(ctypes.byref(b), size):
+        return 1
+    rc = hip.hipMemset(a, 0, size) or hip.hipMemcpy(b, a, size, 3) or hip.hipDeviceSynchronize()  # 3: device to device
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return int(rc)
+
+
+def measured_traffic(kernel, config, nvec):
+    """HBM bytes per launch of `kernel` from the counter passes recorded in
+    profiles/ (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known
+    size), for the workload they were taken on; None when not recorded."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.md"))):
+        text = open(path).read()
+        if "```json" not in text:
+            continue
+        rec = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        if rec.get("config") != config or rec.get("nvec") != nvec:
+            continue
+        for name, t in rec.get("kernels", {}).items():
+            if kernel.startswith(name):
+                return {"bytes_per_launch": t["fetch_bytes"] + t["write_bytes"], "fetch_bytes": t["fetch_bytes"],
+                        "write_bytes": t["write_bytes"], "source": os.path.relpath(path, ROOT)}
+    return None
+
+
 def host_cpu_share():
     """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the
     GPU hosts here allot each GPU a fixed CPU share and export it there),
@@ -182,14 +218,15 @@ def main():
 
     load_library()
     if args.calib_copy:
-        import torch
-        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print("[bench] calibration copy: %d bytes read, %d written" % (src.numel() * 8, dst.numel() * 8),
+        # a plain 2 GiB device-to-device copy through the HIP runtime (its
+        # blit kernel), the known size the counter passes are scaled by;
+        # no torch in this process: with torch loaded, rocprofv3's counter
+        # collection did not see the library's kernels and a rocBLAS call
+        # failed
+        nbytes = 1 << 31
+        rc = calib_copy(nbytes)
+        print("[bench] calibration copy: %d bytes read, %d written (status %d)" % (nbytes, nbytes, rc),
               file=sys.stderr, flush=True)
-        del src, dst
     w = make_workload(args.config)
     N, M, P = w["N"], w["M"], w["P"]
     rng = np.random.default_rng(w["seed"] + 3)
@@ -202,6 +239,8 @@ def main():
     a = from_workload(w, storage=args.storage)
     d, M2, P2 = a.dimensions()  # constructs: device sector maps, operators, Hamiltonian rows
     setup_s = time.time() - t0
+    if args.eager:
+        a.set_option(a.OPT_GRAPH, 0)
     assert (d, M2, P2) == (N, M, P)
 
     # parity of the benchmarked path against the reference CPU build's golden
@@ -446,6 +485,12 @@ def main():
         vpl = args.nvec / max(1.0, ham_n / nbd)
         rooflines["hamiltonian"] = hbm("k_ell (Hamiltonian rows x %d vectors)" % vpl, 12.0 * nnz + 16.0 * N * vpl,
                                        ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd)
+    # HBM traffic per launch where the counter passes recorded it (profiles/)
+    for r in rooflines.values():
+        t = measured_traffic(r["kernel"], args.config, args.nvec)
+        if t is not None:
+            r["traffic"] = t["bytes_per_launch"]
+            r["traffic_detail"] = t
     dominant = max(rooflines, key=lambda k: rooflines[k]["ms_per_step"])
     roof = dict(rooflines[dominant])
     roof["class"] = dominant
