@@ -286,9 +286,10 @@ bool Manager::getEngineOption(int key, int &value)
 {
     if (!ensureBuilt()) return false;
     value = eng_->option(key);
-    // rocBLAS solution indices and the window width (-1: automatic) may be
+    // rocBLAS solution indices and the window choices (-1: automatic) may be
     // negative; every other option is >= 0 and -1 marks an unknown key
-    if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha || key == kOptWindow) return true;
+    if (key == kOptGemmSolutionBeta || key == kOptGemmSolutionAlpha || key == kOptWindow || key == kOptWindowVector)
+        return true;
     return value >= 0;
 }
 

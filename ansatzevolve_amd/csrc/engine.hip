@@ -756,11 +756,16 @@ struct EngineImpl {
         int oSubIdx = 0, oSub = 0, oProg = 0, oMask = 0, oPat = 0, oCOff = 0, oCls = 0, oCStart = 0, oProgT = 0;
         int rowsOff = 0, nrowsA = 0;  // its class member lists in d_wrows
         std::vector<int> cntA, cntB, startA, startB;  // class sizes and first members (per spin)
+        std::vector<int> incA, incB;                  // inner electron counts of the classes
         int maxMembers = 0, maxSide = 0;
         std::vector<int> allTiles;   // every tile, largest first
         std::vector<int> tileSize;   // members per tile (indexed by tile id)
         int tileOff = 0, ntiles = 0; // the tiles a sweep processes (d_wtiles)
         int gridF = 1, gridA = 1;    // blocks (multiples of 64 / V)
+        // vector-major register tiles (k_winv): every tile, grouped by class,
+        // in tasks of at most 64 (d_wvtiles, d_wvtasks)
+        int vTileOff = 0, vTaskOff = 0, vTasks = 0;
+        int gridVF = 1, gridVA = 1;
     };
     // kOptWindow: window width in spatial orbitals (0: off; -1, the default:
     // four-orbital windows when their tiles fit in registers, else off)
@@ -776,7 +781,16 @@ struct EngineImpl {
     std::vector<WinPass> wpass;
     DBuf<uint32_t> d_wblob, d_wrows;
     std::vector<uint32_t> hrows;  // host copy of d_wrows
-    DBuf<int> d_wkp, d_wtiles;
+    DBuf<int> d_wkp, d_wtiles, d_wvtiles;
+    DBuf<int4> d_wvtasks;
+    // kOptWindowVector: window passes for the vector-major sweeps (-1, the
+    // default: with register tiles from 2^18 determinants; 0 off; 1 with
+    // register tiles at any size)
+    int winVec = -1;
+    bool use_wv() const
+    {
+        return winOn && winReg && !cplx && (winVec > 0 || (winVec < 0 && N >= (1LL << 18)));
+    }
     int wslots = 1;  // bracket partial rows of the window adjoint passes
 
     int nfwd() const { return winOn ? (int)wpass.size() : (int)laneF.segs.size(); }
@@ -798,6 +812,55 @@ struct EngineImpl {
                   : (bc ? (const void *)k_winr<false, true, 4, false> : (const void *)k_winr<false, false, 4, false>);
     }
     int win_waves(bool br) const { return winReg ? (br ? 2 : 4) : winWaves; }
+    const void *winv_fn(bool br) const
+    {
+        return winTu ? (br ? (const void *)k_winv<true, true> : (const void *)k_winv<false, true>)
+                     : (br ? (const void *)k_winv<true, false> : (const void *)k_winv<false, false>);
+    }
+
+    // Window passes of the vector-major sweeps (k_winv): in place, nvec
+    // vectors N apart, angle tables M apart per vector.
+    bool sweep_forward_wv(double *v, int nvec, const double2 *cs, std::string &err)
+    {
+        const int mk = mark_begin(1);
+        for (const WinPass &wp : wpass) {
+            WinArgs a = winargs(wp);
+            a.tiles = d_wvtiles.p + wp.vTileOff;
+            const dim3 grid((unsigned)wp.gridVF, nvec);
+            if (winTu)
+                hipLaunchKernelGGL((k_winv<false, true>), grid, dim3(4 * kLanes), 0, stream, a,
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, v, nullptr, N, cs, M, nullptr, 0);
+            else
+                hipLaunchKernelGGL((k_winv<false, false>), grid, dim3(4 * kLanes), 0, stream, a,
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, v, nullptr, N, cs, M, nullptr, 0);
+            LCHK();
+        }
+        mark_end(mk);
+        return true;
+    }
+    // partial: nvec x M rows of `slots` (zeroed by the caller); block b of a
+    // pass writes row b of the pass's operators
+    bool sweep_reverse_wv(double *phi, double *lam, int nvec, const double2 *csneg, double *partial,
+                          std::string &err)
+    {
+        const int mk = mark_begin(2);
+        for (int p = (int)wpass.size() - 1; p >= 0; p--) {
+            const WinPass &wp = wpass[p];
+            WinArgs a = winargs(wp);
+            a.tiles = d_wvtiles.p + wp.vTileOff;
+            const size_t lds = (size_t)4 * wp.nkp * kLanes * sizeof(double);
+            const dim3 grid((unsigned)wp.gridVA, nvec);
+            if (winTu)
+                hipLaunchKernelGGL((k_winv<true, true>), grid, dim3(4 * kLanes), lds, stream, a,
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, phi, lam, N, csneg, M, partial, slots);
+            else
+                hipLaunchKernelGGL((k_winv<true, false>), grid, dim3(4 * kLanes), lds, stream, a,
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, phi, lam, N, csneg, M, partial, slots);
+            LCHK();
+        }
+        mark_end(mk);
+        return true;
+    }
     static const void *win_fn(bool br, int nw, bool bc)
     {
         if (nw == 8) return br ? (const void *)k_win<true, false, 8>
@@ -940,6 +1003,7 @@ struct EngineImpl {
                     cls.push_back(kv.first | ((uint32_t)kv.second << 24));
                     cstart.push_back((uint32_t)((int)rows.size() - base));
                     (spin ? wp.startB : wp.startA).push_back((int)rows.size() - base);
+                    (spin ? wp.incB : wp.incA).push_back(kv.second);
                     (spin ? cntB : cntA).push_back((int)byCount[kv.second].size());
                     for (uint32_t x : byCount[kv.second]) {
                         const uint32_t st = kv.first | (x << wp.w0);
@@ -1082,6 +1146,26 @@ struct EngineImpl {
             tl.insert(tl.end(), wp.allTiles.begin(), wp.allTiles.end());
         }
         if (!upload_vec(d_wtiles, tl, err)) return false;
+        // vector-major tasks (register tiles): the tiles by class of inner
+        // counts (5 ca + cb), up to 64 of one class per task
+        if (winReg) {
+            std::vector<int> vt;
+            std::vector<int4> tasks;
+            for (WinPass &wp : wpass) {
+                wp.vTileOff = (int)vt.size();
+                wp.vTaskOff = (int)tasks.size();
+                std::vector<std::vector<int>> byCls(25);
+                for (int t : wp.allTiles) byCls[wp.incA[t / wp.nBcls] * 5 + wp.incB[t % wp.nBcls]].push_back(t);
+                for (int cl = 0; cl < 25; cl++)
+                    for (size_t i = 0; i < byCls[cl].size(); i += 64) {
+                        const int cnt = (int)std::min<size_t>(64, byCls[cl].size() - i);
+                        tasks.push_back(make_int4(cl, (int)vt.size() - wp.vTileOff, cnt, 0));
+                        vt.insert(vt.end(), byCls[cl].begin() + i, byCls[cl].begin() + i + cnt);
+                    }
+                wp.vTasks = (int)tasks.size() - wp.vTaskOff;
+            }
+            if (!upload_vec(d_wvtiles, vt, err) || !upload_vec(d_wvtasks, tasks, err)) return false;
+        }
         // grids: as many blocks as the chip holds at once
         std::vector<int> lrows(M, 0);
         wslots = 1;
@@ -1112,6 +1196,19 @@ struct EngineImpl {
             for (int q = 0; q < wp.nkp; q++) lrows[kpOp[wp.kp0 + q]] = wp.gridA;
             wslots = std::max(wslots, wp.gridA);
         }
+        if (winReg)
+            for (WinPass &wp : wpass)
+                for (int br = 0; br < 2; br++) {
+                    const void *fn = winv_fn(br == 1);
+                    const size_t lds = br ? (size_t)4 * wp.nkp * kLanes * sizeof(double) : 0;
+                    HCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 1)));
+                    int nbk = 0;
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, 4 * kLanes, lds) != hipSuccess || nbk < 1)
+                        nbk = 1;
+                    long long g = std::min<long long>((long long)nbk * numCU, (wp.vTasks + 3) / 4);
+                    if (br) g = std::min<long long>(g, slots);  // one partial row per block
+                    (br ? wp.gridVA : wp.gridVF) = (int)std::max<long long>(1, g);
+                }
         for (int k = 0; k < M; k++)
             if (lrows[k] == 0) {
                 err = "window plan misses an operator";
@@ -1964,6 +2061,7 @@ struct EngineImpl {
 
     bool sweep_forward(double *v, int nvec, const double2 *cs, std::string &err)
     {
+        if (use_wv()) return sweep_forward_wv(v, nvec, cs, err);
         const int mk = mark_begin(1);
         if (use_s()) {
             for (const Seg &sg : laneS.segs) {
@@ -2019,6 +2117,7 @@ struct EngineImpl {
         hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<long long>((total + kBlock - 1) / kBlock, 4096)),
                            dim3(kBlock), 0, stream, partial, total, 0.0);
         LCHK();
+        if (use_wv()) return sweep_reverse_wv(phi, lam, nvec, csneg, partial, err);
         const int mk = mark_begin(2);
         if (use_s()) {
             for (int si = (int)laneS.segs.size() - 1; si >= 0; si--) {
@@ -3587,6 +3686,16 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         e.bnvec = 0;  // stored states and lists are renewed at the next upload
         return true;
+    case kOptWindowVector:
+        if (value < -1 || value > 1) {
+            err = "vector-major window option must be -1, 0 or 1";
+            return false;
+        }
+        if (value != e.winVec) {
+            e.winVec = value;
+            e.drop_graphs();
+        }
+        return true;
     case kOptWindowPasses:
     case kOptWindowRegisters:
     case kOptHamKind:
@@ -3613,7 +3722,8 @@ int Engine::option(int key) const
     case kOptSweep: return e.sweepKind;
     case kOptSweepTwoGroupAvailable: return e.sweepS ? 1 : 0;
     case kOptSigma1Rows: return e.sig1Rows;
-    case kOptVectorSegments: return (int)(e.use_s() ? e.laneS.segs.size() : e.segs.size());
+    case kOptVectorSegments:
+        return (int)(e.use_wv() ? e.wpass.size() : e.use_s() ? e.laneS.segs.size() : e.segs.size());
     case kOptLaneMin: return e.laneMin;
     case kOptGraph: return e.useGraph ? 1 : 0;
     case kOptFuseDepth: return e.fuseDepth;
@@ -3638,6 +3748,7 @@ int Engine::option(int key) const
     case kOptWindow: return e.winW;
     case kOptWindowWaves: return e.winWaves;
     case kOptWindowPasses: return e.winOn ? (int)e.wpass.size() : 0;
+    case kOptWindowVector: return e.winVec;
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }

@@ -69,6 +69,9 @@ enum EngineOption : int {
     kOptWindowRegisters = 28, // read only: 1 when the window passes hold their tiles in registers (four-orbital
                               // windows, every phase on one pair in both spins), 2 when every phase is also a
                               // five-operator TUPS unit (straight-line phases), 0 otherwise
+    kOptWindowVector = 29,    // window passes for the vector-major sweeps (single vectors, small batches): -1, the
+                              // default, from 2^18 determinants on; 0 never; 1 at any size. They apply only when
+                              // the window passes hold register tiles (key 28 is 1 or 2)
 };
 
 class Engine {

@@ -588,6 +588,18 @@ template <bool BR, bool BC, int NW, bool TU>
 __global__ void k_winr(WinArgs a, const double *in, double *out, double *lam, long long cstride,
                        const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots);
 
+// Window passes in the vector-major layout (one vector per grid row,
+// vstride doubles apart), register tiles of four-orbital windows as
+// k_winr, lane = tile: a wave takes a task {class = 5 ca + cb, first listed
+// tile, count <= 64} of tiles of one class of inner counts.  Forward and
+// adjoint in place (psi is written back in the adjoint); the adjoint sums
+// its brackets per block (4 waves, LDS rows of nkp x 64 per wave) into
+// partial[(vector * M + k) * slots + block].
+template <bool BR, bool TU>
+__global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, double *v, double *lam,
+                       long long vstride, const double2 *__restrict__ cs, int M, double *__restrict__ partial,
+                       int slots);
+
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>
 __global__ void k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut,

@@ -409,6 +409,14 @@ __device__ __forceinline__ uint32_t rot4(uint32_t m, int k)
     return ((m >> k) | (m << (4 - k))) & 0xfu;
 }
 
+// one operator's angles, tile sign, mask window bits and program words
+struct wr_op_args {
+    double2 cs;
+    bool negT;
+    uint32_t mA, mB;
+    int progT;
+};
+
 struct TileCtx {
     uint32_t aOut, bOut;
     int sA, sB;  // first members in rows
@@ -556,10 +564,141 @@ __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const d
     });
 }
 
+// ------------------------------------------ vector-major (k_winv)
+// One vector per grid row (vector-major layout, vstride doubles apart);
+// a wave takes a task of up to 64 tiles of one class (inner counts), lane
+// = tile: the member indices are compile-time as above, the addresses and
+// the outer-bit signs per lane, angles and program words uniform.
+template <int CA, int CB, bool BR, bool TU>
+__device__ __forceinline__ void tile_v(const WinArgs &a, int tl, bool valid, double *v, double *lam,
+                                       const double2 *__restrict__ cs, double *brk)
+{
+    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const uint32_t *B = a.blob;
+    const int ja = tl / a.nBcls, jb = tl - ja * a.nBcls;
+    const uint32_t fa = a.blob[a.oCls + ja], fb = a.blob[a.oCls + a.nAcls + jb];
+    const int sA = (int)a.blob[a.oCStart + ja], sB = a.nrowsA + (int)a.blob[a.oCStart + a.nAcls + jb];
+    const uint32_t wm = 0xfu << a.w0;
+    const uint32_t outer = (((fa & 0xffffffu) << a.n) | (fb & 0xffffffu)) & ~((wm << a.n) | wm);
+    uint32_t ra[NA], rb[NB];
+    double x[NM], l[BR ? NM : 1];
+    wfor<0, NA>([&](auto I) { ra[decltype(I)::value] = a.rows[sA + decltype(I)::value]; });
+    wfor<0, NB>([&](auto I) { rb[decltype(I)::value] = a.rows[sB + decltype(I)::value]; });
+    wfor<0, NM>([&](auto I) {
+        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
+        const long long g = (long long)ra[i] + rb[j];
+        x[m] = valid ? v[g] : 0.0;
+        if constexpr (BR) l[m] = valid ? lam[g] : 0.0;
+    });
+    int r = 0;
+    for (int pi = 0; pi < a.nph; pi++) {
+        const int p = BR ? a.nph - 1 - pi : pi;
+        const int k0 = (int)ld(B, p * 8 + 0), nk = (int)ld(B, p * 8 + 1), kp = (int)ld(B, p * 8 + 3);
+        const uint32_t spectM = ld(B, p * 8 + 6);
+        const int lp = __builtin_ctz(ld(B, p * 8 + 4)) - a.w0;
+        to_layout<CA, CB, BR>((lp - r) & 3, x, l);
+        r = lp;
+        auto one = [&](int kk) {
+            const uint32_t m = ld(B, a.oMask + kp + kk) & spectM;
+            wr_op_args o;
+            o.cs = cs[k0 + kk];
+            o.negT = __popc(m & outer) & 1;  // per lane
+            o.mA = rot4((m >> (a.n + a.w0)) & 0xfu, lp);
+            o.mB = rot4((m >> a.w0) & 0xfu, lp);
+            o.progT = a.oProgT + (p * kOrbMaxOps + kk) * 16;
+            return o;
+        };
+        if constexpr (TU) {
+            wfor<0, 5>([&](auto J) {
+                constexpr int kk = BR ? 4 - decltype(J)::value : decltype(J)::value;
+                const wr_op_args o = one(kk);
+                double br = 0;
+                op_tups<CA, CB, kTupsKind[kk], BR>(a, o.progT, o.negT, o.mA, o.mB, o.cs.x, o.cs.y, x, l, br);
+                if constexpr (BR) brk[(kp + kk) * kLanes + lane] += br;
+            });
+        } else {
+            for (int jj = 0; jj < nk; jj++) {
+                const int kk = BR ? nk - 1 - jj : jj;
+                const wr_op_args o = one(kk);
+                double br = 0;
+                op_pair<CA, CB, 0, BR>(a, o.progT, o.negT, o.mA, o.mB, o.cs.x, o.cs.y, x, l, br);
+                if constexpr (BR) brk[(kp + kk) * kLanes + lane] += br;
+            }
+        }
+    }
+    to_layout<CA, CB, BR>((0 - r) & 3, x, l);
+    wfor<0, NM>([&](auto I) {
+        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
+        const long long g = (long long)ra[i] + rb[j];
+        if (valid) {
+            v[g] = x[m];
+            if constexpr (BR) lam[g] = l[m];
+        }
+    });
+}
+
 }  // namespace wr
+
+template <bool BR, bool TU>
+__global__ void __launch_bounds__(4 * kLanes, BR ? 2 : 4)
+    k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, double *v, double *lam, long long vstride,
+           const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+{
+    constexpr int NW = 4;
+    extern __shared__ __align__(16) double lds[];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int vec = blockIdx.y;
+    double *brk = lds + (size_t)wv * a.nkp * kLanes;  // adjoint: this wave's rows, one slot per lane
+    if (BR)
+        for (int e = 0; e < a.nkp; e++) brk[e * kLanes + lane] = 0.0;
+    v += (long long)vec * vstride;
+    if (BR) lam += (long long)vec * vstride;
+    cs += (long long)vec * M;
+    for (int it = blockIdx.x * NW + wv; it < ntasks; it += gridDim.x * NW) {
+        const int4 tk = tasks[it];  // {class, first listed tile, count}
+        const int cls = __builtin_amdgcn_readfirstlane(tk.x), first = __builtin_amdgcn_readfirstlane(tk.y),
+                  cnt = __builtin_amdgcn_readfirstlane(tk.z);
+        const bool valid = lane < cnt;
+        const int tl = a.tiles[first + (valid ? lane : 0)];
+        switch (cls) {
+#define AE_WV_CASE(CA, CB) \
+    case CA * 5 + CB: wr::tile_v<CA, CB, BR, TU>(a, tl, valid, v, lam, cs, brk); break;
+            AE_WV_CASE(0, 0) AE_WV_CASE(0, 1) AE_WV_CASE(0, 2) AE_WV_CASE(0, 3) AE_WV_CASE(0, 4)
+            AE_WV_CASE(1, 0) AE_WV_CASE(1, 1) AE_WV_CASE(1, 2) AE_WV_CASE(1, 3) AE_WV_CASE(1, 4)
+            AE_WV_CASE(2, 0) AE_WV_CASE(2, 1) AE_WV_CASE(2, 2) AE_WV_CASE(2, 3) AE_WV_CASE(2, 4)
+            AE_WV_CASE(3, 0) AE_WV_CASE(3, 1) AE_WV_CASE(3, 2) AE_WV_CASE(3, 3) AE_WV_CASE(3, 4)
+            AE_WV_CASE(4, 0) AE_WV_CASE(4, 1) AE_WV_CASE(4, 2) AE_WV_CASE(4, 3) AE_WV_CASE(4, 4)
+#undef AE_WV_CASE
+        default: break;
+        }
+    }
+    if (BR) {
+        __syncthreads();
+        // fixed-order sum over the waves, then over the lanes (tiles)
+        for (int kp = wv; kp < a.nkp; kp += NW) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) s += lds[((size_t)q * a.nkp + kp) * kLanes + lane];
+            s = wave_sum(s);
+            if (lane == 0) partial[((long long)vec * M + a.kpOp[kp]) * slots + blockIdx.x] = s;
+        }
+    }
+}
+
+template __global__ void k_winv<false, false>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
+                                              const double2 *__restrict__, int, double *__restrict__, int);
+template __global__ void k_winv<true, false>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
+                                             const double2 *__restrict__, int, double *__restrict__, int);
+template __global__ void k_winv<false, true>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
+                                             const double2 *__restrict__, int, double *__restrict__, int);
+template __global__ void k_winv<true, true>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
+                                            const double2 *__restrict__, int, double *__restrict__, int);
 
 // occupancy: forward 4 waves per SIMD (at most 128 VGPRs), adjoint 2 (256)
 template <bool BR, bool BC, int NW, bool TU>
+
 __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
     k_winr(WinArgs a, const double *in, double *out, double *lam, long long cstride, const double2 *__restrict__ cs,
            int M, double *__restrict__ partial, int slots)

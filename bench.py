@@ -198,6 +198,13 @@ def measured_traffic(kernel, config, nvec):
     return None
 
 
+def window_vm(a, opts, N):
+    """True when the vector-major sweeps run as window passes (option 29:
+    -1 from 2^18 determinants on, 1 always; register tiles only)."""
+    wv = a.get_option(a.OPT_WINDOW_VECTOR)
+    return opts["window_registers"] > 0 and (wv == 1 or (wv == -1 and N >= 1 << 18))
+
+
 def host_cpu_share():
     """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the
     GPU hosts here allot each GPU a fixed CPU share and export it there),
@@ -279,7 +286,7 @@ def main():
     single_bound = {
         "bound": "launch latency",
         "ms_per_egrad": single_ms,
-        "sweep_segments_per_sweep": a.get_option(a.OPT_SEGMENTS),
+        "sweep_segments_per_sweep": a.get_option(a.OPT_VECTOR_SEGMENTS),
         "timed_intervals_per_egrad": single_launches / nsingle,
         "ideal_hbm_bytes_per_egrad": 40.0 * N,
         "hbm_frac_of_ideal_bytes": 40.0 * N / (single_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -317,6 +324,7 @@ def main():
     opts["reachable"] = a.get_option(a.OPT_REACHABLE)
     opts["window_passes"] = a.get_option(a.OPT_WINDOW_PASSES)
     opts["window_registers"] = a.get_option(a.OPT_WINDOW_REGISTERS)
+    opts["window_vector"] = a.get_option(a.OPT_WINDOW_VECTOR)
     amp = {"forward": a.get_option(a.OPT_SWEEP_AMP_FORWARD),
            "adjoint_lambda": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_LAMBDA),
            "adjoint_state": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_STATE)}
@@ -402,6 +410,11 @@ def main():
         form = {2: "k_winr<%s, TUPS units>", 1: "k_winr<%s>"}.get(opts["window_registers"], "k_win<%s>")
         k_fwd = (form % "false") + " (lane layout, %d window passes)" % opts["window_passes"]
         k_rev = (form % "true") + " (lane layout, %d window passes)" % opts["window_passes"]
+    elif not lanes and window_vm(a, opts, N):
+        # vector-major window passes: one register tile per lane (k_winv)
+        tu = ", TUPS units" if opts["window_registers"] == 2 else ""
+        k_fwd = "k_winv<false%s> (%d window passes)" % (tu, opts["window_passes"])
+        k_rev = "k_winv<true%s> (%d window passes)" % (tu, opts["window_passes"])
     if lanes:
         fwd_launches, rev_launches = max(1, fwd_n // nbd), max(1, rev_n // nbd)
         fwd_pass = 16.0 * amp["forward"] * args.nvec / fwd_launches

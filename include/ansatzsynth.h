@@ -333,7 +333,11 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     (width 4 and every operator run on one adjacent orbital pair in both
  *     spins), 2 when in addition every run is a TUPS unit of five operators
  *     (beta single, alpha single, paired double, beta single, alpha single:
- *     straight-line phases), 0 when they stage tiles in LDS or are not used.
+ *     straight-line phases), 0 when they stage tiles in LDS or are not used;
+ *  29 window passes for the vector-major sweeps (single vectors and
+ *     batches below key 1): -1 (default) from 2^18 determinants on, 0
+ *     never, 1 at any size.  They use the register tiles of key 28 (1 or 2)
+ *     with one tile per lane, applied in place; key 24 then counts them.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

@@ -219,3 +219,62 @@ def test_window_config3_against_reference(gpu):
             assert abs(E[0] - gz["cfg3_E"][0]) <= E_TOL
             assert np.max(np.abs(G[0] - gz["cfg3_grad"][0])) <= G_TOL
             assert np.max(np.abs(E - E0)) <= SAME and np.max(np.abs(G - G0)) <= 1e-11
+
+
+@pytest.mark.parametrize("n,layers,variant", [(8, 4, "tups"), (10, 3, "tups"), (8, 3, "alpha_first")])
+def test_window_vector_major(gpu, n, layers, variant):
+    """Window passes in the vector-major sweeps (option 29 = 1, one tile per
+    lane, in place): single-vector E+grad and a 5-vector batch below the
+    lane threshold against the orbit-segment sweeps (option 29 = 0) and the
+    oracle.  The straight-line TUPS form and the generic register form
+    (the singles of each group swapped); class task lists with partial
+    waves (the tile counts are not multiples of 64)."""
+    w, thetas = _tups_case(n, layers, 4500 + n, 5)
+    if variant == "alpha_first":
+        ops = w["ops"].copy()
+        for u in range(len(ops) // 5):
+            for g in (5 * u, 5 * u + 3):
+                ops[[g, g + 1]] = ops[[g + 1, g]]
+        w = dict(w, ops=ops)
+    with from_workload(w, storage=2) as a:
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 64)
+        a.set_option(AnsatzSynth.OPT_WINDOW, 4)
+        a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 0)
+        ref = [a.energy_gradient(t) for t in thetas]
+        Eb0, Gb0 = a.energy_gradients(thetas)
+        segs0 = a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS)
+        a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 1)
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_VECTOR) == 1
+        passes = a.get_option(AnsatzSynth.OPT_WINDOW_PASSES)
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_REGISTERS) == (2 if variant == "tups" else 1)
+        assert a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS) == passes
+        print("n=%d L=%d %s: %d window passes vs %d vector segments" % (n, layers, variant, passes, segs0))
+        for rep in range(2):  # the second call replays the captured graph
+            for v, t in enumerate(thetas):
+                E, g = a.energy_gradient(t)
+                assert abs(E - ref[v][0]) <= SAME and np.max(np.abs(g - ref[v][1])) <= SAME, (rep, v)
+        Eb, Gb = a.energy_gradients(thetas)
+        assert np.max(np.abs(Eb - Eb0)) <= SAME and np.max(np.abs(Gb - Gb0)) <= SAME
+        # the energy-only batch path
+        assert np.max(np.abs(a.energies(thetas) - Eb0)) <= SAME
+    if n <= 8:
+        o = OracleEngine(2 * n, [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
+        o.set_integrals(w["h"], w["g"], explicit=True)
+        Eo, go = o.energy_gradient(thetas[2])
+        assert abs(ref[2][0] - Eo) <= E_TOL and np.max(np.abs(ref[2][1] - go)) <= G_TOL
+
+
+def test_window_vector_major_config3(gpu):
+    """12 orbitals, 853,776 determinants: single-vector E+grad with the
+    default vector-major window passes (on from 2^18 determinants) against
+    the reference CPU build's golden vector and the orbit-segment sweeps."""
+    gz = np.load(os.path.join(HERE, "golden", "reference_vectors.npz"), allow_pickle=False)
+    w = make_workload(3)
+    with from_workload(w) as a:
+        E, g = a.energy_gradient(w["theta"])
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_VECTOR) == -1
+        assert a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS) == a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 17
+        assert abs(E - gz["cfg3_E"][0]) <= E_TOL and np.max(np.abs(g - gz["cfg3_grad"][0])) <= G_TOL
+        a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 0)
+        E0, g0 = a.energy_gradient(w["theta"])
+        assert abs(E - E0) <= SAME and np.max(np.abs(g - g0)) <= 1e-11
