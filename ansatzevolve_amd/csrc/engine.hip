@@ -1085,7 +1085,21 @@ struct EngineImpl {
             for (size_t i = 0; i < ph.size(); i++) ph[i].progOff = psegs[i].progOff;
             // TUPS units: beta single, alpha single, paired double, beta
             // single, alpha single on one pair, each rotating exactly the
-            // member pairs of its kind in every orbit type
+            // member pairs of its kind in every orbit type, with an empty
+            // sign mask (no spin-orbital between its indices) and so one
+            // phase for every pair it rotates
+            auto op_phase = [&](const WinPhaseDev &d, int kk) {
+                int sg = -1;  // -1 no pair yet, 0 or 1, 2 mixed
+                for (int ty = 0; ty < 9; ty++) {
+                    const uint32_t w = prog[d.progOff + ty * kOrbMaxOps + kk];
+                    for (int r = 0; r < 6; r++)
+                        if ((w >> r) & 1u) {
+                            const int b = (int)((w >> (8 + r)) & 1u);
+                            sg = sg < 0 ? b : (sg == b ? sg : 2);
+                        }
+                }
+                return sg;
+            };
             for (size_t i = 0; winTu && i < ph.size(); i++) {
                 static const int kind[5] = {0, 1, 2, 0, 1};
                 winTu = ph[i].nk == 5;
@@ -1094,6 +1108,7 @@ struct EngineImpl {
                     const uint32_t a2 = bitsA(k), b2 = bitsB(k);
                     winTu = (kind[kk] == 0 ? (a2 == 0 && b2 == ph[i].ub)
                              : kind[kk] == 1 ? (a2 == ph[i].ua && b2 == 0) : (a2 == ph[i].ua && b2 == ph[i].ub));
+                    winTu = winTu && hops[k].mask == 0 && (op_phase(ph[i], kk) == 0 || op_phase(ph[i], kk) == 1);
                     for (int ta = 0; winTu && ta < 3; ta++)
                         for (int tb = 0; winTu && tb < 3; tb++) {
                             const unsigned want = kind[kk] == 0 ? (tb == 0 ? (1u | (ta == 0 ? 32u : 0u)) : 0u)
@@ -1122,13 +1137,16 @@ struct EngineImpl {
             wp.oCls = put(cls.data(), cls.size() * 4);
             wp.oCStart = put(cstart.data(), cstart.size() * 4);
             // the programs by (phase, operator): the 9 type words of one
-            // operator together (16 words, one wide scalar load)
+            // operator together (16 words, one wide scalar load); word 9: the
+            // operator's phase as a sign bit (0x80000000) when all its pairs
+            // share one (the straight-line TUPS form reads it)
             while (bl.size() % 16) bl.push_back(0);
             wp.oProgT = (int)bl.size();
             for (size_t q = 0; q < ph.size(); q++)
                 for (int kk = 0; kk < kOrbMaxOps; kk++)
                     for (int ty = 0; ty < 16; ty++)
-                        bl.push_back(ty < 9 ? prog[ph[q].progOff + ty * kOrbMaxOps + kk] : 0u);
+                        bl.push_back(ty < 9 ? prog[ph[q].progOff + ty * kOrbMaxOps + kk]
+                                            : ty == 9 && kk < ph[q].nk && op_phase(ph[q], kk) == 1 ? 0x80000000u : 0u);
             while (bl.size() % 4) bl.push_back(0);
             wp.blobOff = (int)blob.size();
             wp.nblob = (int)bl.size();

@@ -537,7 +537,8 @@ struct WinPhaseDev {
 //   oCls    outer classes, alpha then beta: outer bits | inner count << 24
 //   oCStart first member of each class in rows (alpha) / rows + nrowsA (beta)
 //   oProgT  the programs again by (phase, operator): 16 words per operator,
-//           its 9 type words first (16-word aligned)
+//           its 9 type words first (16-word aligned), then its phase as a
+//           sign bit (0x80000000) when every pair it rotates has the same one
 struct WinArgs {
     int n, w, w0;
     const uint32_t *blob;

@@ -356,9 +356,17 @@ constexpr unsigned tups_slots(int kind, int sa, int sb)
                      : (sa == 0 && sb == 0 ? 4u : 0u);
 }
 
+// d with its sign flipped when sg is 0x80000000 (sg 0: unchanged)
+__device__ __forceinline__ double flip(double d, uint32_t sg)
+{
+    return __hiloint2double(__double2hiint(d) ^ (int)sg, __double2loint(d));
+}
+
+// One TUPS operator: its member pairs are compile-time and, its sign mask
+// being empty, one phase serves every pair it rotates; the caller passes S
+// with that phase applied.  The adjoint's bracket sum is returned without it.
 template <int CA, int CB, int KIND, bool BR>
-__device__ __forceinline__ void op_tups(const WinArgs &a, int progT, bool negT, uint32_t mA, uint32_t mB, double C,
-                                        double S, double (&x)[C4[CA] * C4[CB]],
+__device__ __forceinline__ void op_tups(double C, double S, double (&x)[C4[CA] * C4[CB]],
                                         double (&l)[BR ? C4[CA] * C4[CB] : 1], double &br)
 {
     constexpr int NB = C4[CB];
@@ -369,25 +377,18 @@ __device__ __forceinline__ void op_tups(const WinArgs &a, int progT, bool negT, 
         constexpr Sub Bs = sub(CB, 0, decltype(I)::value % NSB);
         constexpr unsigned slots = tups_slots(KIND, A.sh, Bs.sh);
         if constexpr (slots != 0) {
-            const uint32_t w = ld(a.blob, progT + A.sh * 3 + Bs.sh);
-            const bool neg = negT ^ par4<pat(CA, A.i0)>(mA) ^ par4<pat(CB, Bs.i0)>(mB);
+            double brs = 0;
             wfor<0, 6>([&](auto R) {
                 constexpr int r = decltype(R)::value;
                 if constexpr ((slots >> r) & 1u) {
                     constexpr int p = PP[r], q = QQ[r];
                     constexpr int mp = ((p >> 1) ? A.i1 : A.i0) * NB + ((p & 1) ? Bs.i1 : Bs.i0);
                     constexpr int mq = ((q >> 1) ? A.i1 : A.i0) * NB + ((q & 1) ? Bs.i1 : Bs.i0);
-                    const bool minus = (((w >> (8 + r)) & 1u) != 0) != neg;
-                    const double sS = minus ? -S : S;
-                    if constexpr (BR) {
-                        double brr = 0;
-                        rot2(x[mp], x[mq], l[mp], l[mq], brr, C, sS);
-                        br += minus ? -brr : brr;
-                    } else {
-                        rot(x[mp], x[mq], C, sS);
-                    }
+                    if constexpr (BR) rot2(x[mp], x[mq], l[mp], l[mq], brs, C, S);
+                    else rot(x[mp], x[mq], C, S);
                 }
             });
+            if constexpr (BR) br += brs;
         }
     });
 }
@@ -494,13 +495,10 @@ __device__ __forceinline__ void phase_run(const WinArgs &a, const TileCtx &t, in
         wfor<0, 5>([&](auto J) {
             constexpr int kk = BR ? 4 - decltype(J)::value : decltype(J)::value;
             const double2 cst = BR ? cp[kk * kLanes] : csL[(kp + kk) * kLanes + lane];
-            const uint32_t m = ld(B, a.oMask + kp + kk) & spectM;
-            const bool negT = __popc(m & outer) & 1;
-            const uint32_t mA = rot4((m >> (a.n + a.w0)) & 0xfu, lp), mB = rot4((m >> a.w0) & 0xfu, lp);
+            const uint32_t sg = ld(B, a.oProgT + (p * kOrbMaxOps + kk) * 16 + 9);
             double br = 0;
-            op_tups<CA, CB, kTupsKind[kk], BR>(a, a.oProgT + (p * kOrbMaxOps + kk) * 16, negT, mA, mB, cst.x, cst.y,
-                                                x, l, br);
-            if constexpr (BR) brk[(kp + kk) * kLanes + lane] += br;
+            op_tups<CA, CB, kTupsKind[kk], BR>(cst.x, flip(cst.y, sg), x, l, br);
+            if constexpr (BR) brk[(kp + kk) * kLanes + lane] += flip(br, sg);
         });
         return;
     }
@@ -529,20 +527,49 @@ __device__ __forceinline__ void phase_run(const WinArgs &a, const TileCtx &t, in
     }
 }
 
+// Member (i, j) of a tile sits at row rows[sA + i] (alpha rank * nb) plus
+// column rows[sB + j] (beta rank).  Its address is a per-row base in scalar
+// registers (NA of them) plus a per-column byte offset in a vector register
+// (NB of them, lane included), so each load and store is one instruction on
+// a scalar base and a vector offset, with no address arithmetic per member.
+__device__ __forceinline__ double ld_at(const double *row, uint32_t boff)
+{
+    return __builtin_nontemporal_load((const double *)((const char *)row + boff));
+}
+__device__ __forceinline__ void st_at(double v, double *row, uint32_t boff)
+{
+    __builtin_nontemporal_store(v, (double *)((char *)row + boff));
+}
+
 template <int CA, int CB, bool BR, bool BC, int NW, bool TU>
 __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const double *in, double *out, double *lam,
                                      long long base, const double2 *__restrict__ cs, const double2 *csL, int c, int M,
                                      double *brk)
 {
     constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const uint32_t lane8 = (threadIdx.x & (kLanes - 1)) * 8u;
     double x[NM], l[BR ? NM : 1];
-    wfor<0, NM>([&](auto I) {
-        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
-        const long long g = (long long)ld(a.rows, t.sA + i) + ld(a.rows, t.sB + j);
-        if constexpr (BC) x[m] = in[g];
-        else x[m] = __builtin_nontemporal_load(in + base + g * kLanes);
-        if constexpr (BR) l[m] = __builtin_nontemporal_load(lam + base + g * kLanes);
-    });
+    {
+        uint32_t col[NB], colS[NB];
+        wfor<0, NB>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const uint32_t rb = ld(a.rows, t.sB + j);
+            col[j] = rb * (kLanes * 8u) + lane8;
+            colS[j] = rb * 8u;  // the start vector: one amplitude per determinant
+        });
+        wfor<0, NA>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const long long ra = ld(a.rows, t.sA + i);
+            const double *rx = BC ? in + ra : in + base + ra * kLanes;
+            const double *rl = lam + base + ra * kLanes;
+            wfor<0, NB>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if constexpr (BC) x[i * NB + j] = *(const double *)((const char *)rx + colS[j]);
+                else x[i * NB + j] = ld_at(rx, col[j]);
+                if constexpr (BR) l[i * NB + j] = ld_at(rl, col[j]);
+            });
+        });
+    }
     int r = 0;  // the registers' layout
     for (int pi = 0; pi < a.nph; pi++) {
         const int p = BR ? a.nph - 1 - pi : pi;
@@ -552,16 +579,27 @@ __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const d
         phase_run<CA, CB, BR, TU>(a, t, p, lp, cs, csL, c, M, brk, x, l);
     }
     to_layout<CA, CB, BR>((0 - r) & 3, x, l);
-    // the member addresses are loaded again rather than kept live across the
-    // phases (36 of them would take the registers the amplitudes need)
+    // the member addresses are formed again rather than kept live across
+    // the phases
     const uint32_t *rows = a.rows;
     asm volatile("" : "+s"(rows));
-    wfor<0, NM>([&](auto I) {
-        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
-        const long long g = (long long)ld(rows, t.sA + i) + ld(rows, t.sB + j);
-        if (out) __builtin_nontemporal_store(x[m], out + base + g * kLanes);
-        if constexpr (BR) __builtin_nontemporal_store(l[m], lam + base + g * kLanes);
+    uint32_t col[NB];
+    wfor<0, NB>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        col[j] = ld(rows, t.sB + j) * (kLanes * 8u) + lane8;
     });
+    if (out)
+        wfor<0, NA>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            double *ro = out + base + (long long)ld(rows, t.sA + i) * kLanes;
+            wfor<0, NB>([&](auto J) { st_at(x[i * NB + decltype(J)::value], ro, col[decltype(J)::value]); });
+        });
+    if constexpr (BR)
+        wfor<0, NA>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            double *rl = lam + base + (long long)ld(rows, t.sA + i) * kLanes;
+            wfor<0, NB>([&](auto J) { st_at(l[i * NB + decltype(J)::value], rl, col[decltype(J)::value]); });
+        });
 }
 
 // ------------------------------------------ vector-major (k_winv)
@@ -612,10 +650,11 @@ __device__ __forceinline__ void tile_v(const WinArgs &a, int tl, bool valid, dou
         if constexpr (TU) {
             wfor<0, 5>([&](auto J) {
                 constexpr int kk = BR ? 4 - decltype(J)::value : decltype(J)::value;
-                const wr_op_args o = one(kk);
+                const double2 cst = cs[k0 + kk];
+                const uint32_t sg = ld(B, a.oProgT + (p * kOrbMaxOps + kk) * 16 + 9);
                 double br = 0;
-                op_tups<CA, CB, kTupsKind[kk], BR>(a, o.progT, o.negT, o.mA, o.mB, o.cs.x, o.cs.y, x, l, br);
-                if constexpr (BR) brk[(kp + kk) * kLanes + lane] += br;
+                op_tups<CA, CB, kTupsKind[kk], BR>(cst.x, flip(cst.y, sg), x, l, br);
+                if constexpr (BR) brk[(kp + kk) * kLanes + lane] += flip(br, sg);
             });
         } else {
             for (int jj = 0; jj < nk; jj++) {
@@ -715,7 +754,7 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
         for (int e = wv; e < a.nkp; e += NW) csL[e * kLanes + lane] = cs[((long long)c * M + a.kpOp[e]) * kLanes + lane];
         __syncthreads();
     }
-    const long long base = (long long)c * cstride + lane;
+    const long long base = (long long)c * cstride;  // the chunk (lanes: the members' column offsets)
     const uint32_t *B = a.blob;
     for (int it = blockIdx.x * NW + wv; it < a.ntiles; it += gridDim.x * NW) {
         const int tl = __builtin_amdgcn_readfirstlane(a.tiles[it]);
