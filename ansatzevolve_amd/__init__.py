@@ -340,7 +340,8 @@ class AnsatzSynth:
      OPT_SIGMA_HALVES, OPT_SIGMA_HALVES_AVAILABLE, OPT_SIGMA_GROUPED, OPT_SIGMA_GROUPED_AVAILABLE,
      OPT_GEMM_SOLUTION_BETA, OPT_GEMM_SOLUTION_ALPHA, OPT_REACHABLE, OPT_SWEEP_AMP_FORWARD,
      OPT_SWEEP_AMP_ADJOINT_LAMBDA, OPT_SWEEP_AMP_ADJOINT_STATE, OPT_SWEEP_TWO_GROUP_AVAILABLE, OPT_SIGMA1_ROWS, OPT_VECTOR_SEGMENTS,
-     OPT_WINDOW, OPT_WINDOW_WAVES, OPT_WINDOW_PASSES, OPT_WINDOW_REGISTERS, OPT_WINDOW_VECTOR) = range(30)
+     OPT_WINDOW, OPT_WINDOW_WAVES, OPT_WINDOW_PASSES, OPT_WINDOW_REGISTERS, OPT_WINDOW_VECTOR,
+     OPT_WINDOW_ENERGY) = range(31)
 
     def set_option(self, key, value):
         self._chk("setEngineOption", self.lib.setEngineOption(int(key), int(value), self.ctx))

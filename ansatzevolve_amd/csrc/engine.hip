@@ -237,6 +237,7 @@ struct EngineImpl {
     DBuf<int> d_hl;
     rocblas_handle blas = nullptr;
     DBuf<double> bl_epart;
+    DBuf<double> bl_wep;  // energy rows of the first adjoint window pass
     DBuf<int32_t> d_col;
     DBuf<double> d_val;
 
@@ -316,7 +317,7 @@ struct EngineImpl {
     {
         return {b_psi.p,      b_lam.p,   b_partial.p, b_epart.p, b_out.p,  b_cs.p,    b_csneg.p, bl_psi.p,
                 bl_lam.p,     bl_partial.p, bl_epart.p, bl_cs.p,  bl_csneg.p, bl_ck.p, d_suplist.p, d_lrows.p,
-                d_wtiles.p,   d_wblob.p, d_wrows.p,   d_wkp.p};
+                d_wtiles.p,   d_wblob.p, d_wrows.p,   d_wkp.p,     bl_wep.p};
     }
 
     bool use_lanes(int nvec) const { return laneMin > 0 && nvec >= laneMin; }
@@ -787,6 +788,7 @@ struct EngineImpl {
     // default: with register tiles from 2^18 determinants; 0 off; 1 with
     // register tiles at any size)
     int winVec = -1;
+    bool winFuseE = true;  // kOptWindowEnergy
     bool use_wv() const
     {
         return winOn && winReg && !cplx && (winVec > 0 || (winVec < 0 && N >= (1LL << 18)));
@@ -1020,8 +1022,15 @@ struct EngineImpl {
                 }
             wp.allTiles.resize(wp.tileSize.size());
             std::iota(wp.allTiles.begin(), wp.allTiles.end(), 0);
-            std::stable_sort(wp.allTiles.begin(), wp.allTiles.end(),
-                             [&](int x, int y) { return wp.tileSize[x] > wp.tileSize[y]; });
+            // largest first, and one inner-count class after another within a
+            // size, so the waves resident at a time run few of the register
+            // kernels' per-class code paths
+            auto icls = [&](int t) {
+                return wp.incA[t / wp.nBcls] * (kWinMaxW + 1) + wp.incB[t % wp.nBcls];
+            };
+            std::stable_sort(wp.allTiles.begin(), wp.allTiles.end(), [&](int x, int y) {
+                return wp.tileSize[x] != wp.tileSize[y] ? wp.tileSize[x] > wp.tileSize[y] : icls(x) < icls(y);
+            });
             // phases, sub-class lists, programs, masks
             std::vector<WinPhaseDev> ph;
             std::vector<int4> subIdx;
@@ -1335,14 +1344,14 @@ struct EngineImpl {
                 const dim3 grid((unsigned)std::max(1, std::min(wp.gridF, (wp.ntiles + 3) / 4)), nch);
                 if (winTu) {
                     if (bc) hipLaunchKernelGGL((k_winr<false, true, 4, true>), grid, dim3(4 * kLanes), lds, stream, a,
-                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0, nullptr);
                     else hipLaunchKernelGGL((k_winr<false, false, 4, true>), grid, dim3(4 * kLanes), lds, stream, a,
-                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0, nullptr);
                 } else {
                     if (bc) hipLaunchKernelGGL((k_winr<false, true, 4, false>), grid, dim3(4 * kLanes), lds, stream, a,
-                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                                               in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0, nullptr);
                     else hipLaunchKernelGGL((k_winr<false, false, 4, false>), grid, dim3(4 * kLanes), lds, stream, a,
-                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0);
+                                            in, out, nullptr, cstride, bl_cs.p, M, nullptr, 0, nullptr);
                 }
                 mark_end(mk);
                 LCHK();
@@ -1366,8 +1375,10 @@ struct EngineImpl {
         return true;
     }
 
-    // every block writes its bracket rows, tiles or not (wp.gridA rows per operator)
-    bool sweep_reverse_win(double *phi, double *lam, int nch, std::string &err)
+    // every block writes its bracket rows, tiles or not (wp.gridA rows per
+    // operator).  epart (register tiles): the energies' partial rows from the
+    // first adjoint pass (win_erows() per chunk)
+    bool sweep_reverse_win(double *phi, double *lam, int nch, std::string &err, double *epart = nullptr)
     {
         const long long cstride = N * kLanes;
         const dim3 blk(winWaves * kLanes);
@@ -1379,12 +1390,13 @@ struct EngineImpl {
             double *out = ckOn ? nullptr : phi;
             const dim3 grid((unsigned)wp.gridA, nch);
             const int mk = mark_begin(2);
+            double *ep = p == (int)wpass.size() - 1 ? epart : nullptr;
             if (winReg && winTu)
                 hipLaunchKernelGGL((k_winr<true, false, 2, true>), grid, dim3(2 * kLanes), lds, stream, a, in, out,
-                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots, ep);
             else if (winReg)
                 hipLaunchKernelGGL((k_winr<true, false, 2, false>), grid, dim3(2 * kLanes), lds, stream, a, in, out,
-                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots);
+                                   lam, cstride, bl_csneg.p, M, bl_partial.p, lslots, ep);
             else if (winWaves == 8)
                 hipLaunchKernelGGL((k_win<true, false, 8>), grid, blk, lds, stream, a, in, out, lam, cstride,
                                    bl_csneg.p, M, bl_partial.p, lslots);
@@ -2280,9 +2292,12 @@ struct EngineImpl {
     // partial rows are written by exactly one launch (rows 0 .. its grid - 1).
     // Checkpoint mode: segment s reads phi from stored state s + 1 and does
     // not write it back (only lambda is updated in place).
-    bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err)
+    // energy rows of the fused first adjoint pass (register window tiles)
+    bool fused_energy() const { return winOn && winReg && !wpass.empty(); }
+    int win_erows() const { return wpass.empty() ? 0 : wpass.back().gridA * 2; }
+    bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err, double *epart = nullptr)
     {
-        if (winOn) return sweep_reverse_win(phi, lam, nch, err);
+        if (winOn) return sweep_reverse_win(phi, lam, nch, err, epart);
         const long long cstride = N * kLanes;
         for (int si = (int)lsegs.size() - 1; si >= 0; si--) {
             const Seg &sg = lsegs[si];
@@ -2353,8 +2368,10 @@ struct EngineImpl {
     // Lane-layout sigma for nch chunks and the per-vector energies
     // out[v] = <x_v|y_v> (matrix-free storage, n <= 12).
     // epart: 512 x nch x 64 energy partials (the batch's bl_epart when null)
+    // dot false: no energies (the batch takes them from the first adjoint
+    // window pass)
     bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err,
-                         double *epart = nullptr)
+                         double *epart = nullptr, bool dot = true)
     {
         if (!epart) epart = bl_epart.p;
         const int mk = mark_begin(0);
@@ -2401,6 +2418,7 @@ struct EngineImpl {
             }
         }
         mark_end(mk3);
+        if (!dot) return true;
         const int G = 512;
         hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, epart);
         LCHK();
@@ -3382,7 +3400,8 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
     if (e.use_lanes(nvec)) {
         const size_t nch = (size_t)(nvec + kLanes - 1) / kLanes;
         if (!e.bl_psi.ensure(nch * e.N * kLanes, err) || !e.bl_lam.ensure(nch * e.N * kLanes, err) ||
-            !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err))
+            !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err) ||
+            !e.bl_wep.ensure((size_t)nch * std::max(1, e.win_erows()) * kLanes, err))
             return false;
         // stored forward states 0 .. segments, when they fit in half the
         // free memory
@@ -3455,12 +3474,21 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
         if (e.hkind == 3 && e.sigv) {
             // matrix-free sigma in lane layout: no transposes
             const double *psi = e.ckOn ? e.ck_state(e.nfwd(), nch) : e.bl_psi.p;
-            if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err)) return false;
+            // with the gradient, window passes in registers take the energies
+            // in their first adjoint pass (its tiles hold the final state and
+            // H psi) instead of a separate dot over both vectors
+            const bool fe = with_gradient && e.fused_energy() && e.winFuseE;
+            if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err, nullptr, !fe)) return false;
             if (with_gradient) {
-                if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err)) return false;
+                if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err, fe ? e.bl_wep.p : nullptr)) return false;
                 hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
                                    e.M, e.lslots, e.d_lrows.p, nvec, e.b_out.p + nvec, 2.0);
                 LCHK();
+                if (fe) {
+                    hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, e.stream, e.bl_wep.p, 1,
+                                       e.win_erows(), nullptr, nvec, e.b_out.p, 1.0);
+                    LCHK();
+                }
             }
             return true;
         }
@@ -3704,6 +3732,16 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         e.bnvec = 0;  // stored states and lists are renewed at the next upload
         return true;
+    case kOptWindowEnergy:
+        if (value < 0 || value > 1) {
+            err = "window energy option must be 0 or 1";
+            return false;
+        }
+        if ((value == 1) != e.winFuseE) {
+            e.winFuseE = value == 1;
+            e.drop_graphs();
+        }
+        return true;
     case kOptWindowVector:
         if (value < -1 || value > 1) {
             err = "vector-major window option must be -1, 0 or 1";
@@ -3767,6 +3805,7 @@ int Engine::option(int key) const
     case kOptWindowWaves: return e.winWaves;
     case kOptWindowPasses: return e.winOn ? (int)e.wpass.size() : 0;
     case kOptWindowVector: return e.winVec;
+    case kOptWindowEnergy: return e.winFuseE ? 1 : 0;
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }

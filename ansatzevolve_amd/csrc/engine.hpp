@@ -72,6 +72,8 @@ enum EngineOption : int {
     kOptWindowVector = 29,    // window passes for the vector-major sweeps (single vectors, small batches): -1, the
                               // default, from 2^18 determinants on; 0 never; 1 at any size. They apply only when
                               // the window passes hold register tiles (key 28 is 1 or 2)
+    kOptWindowEnergy = 30,    // 1 (default): batches with the gradient and register window tiles take the energies
+                              // in the first adjoint pass instead of a separate dot; 0: the separate dot
 };
 
 class Engine {

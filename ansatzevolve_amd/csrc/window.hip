@@ -541,10 +541,11 @@ __device__ __forceinline__ void st_at(double v, double *row, uint32_t boff)
     __builtin_nontemporal_store(v, (double *)((char *)row + boff));
 }
 
+// en (adjoint): also add the tile's <x|l> to e before the first phase
 template <int CA, int CB, bool BR, bool BC, int NW, bool TU>
 __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const double *in, double *out, double *lam,
                                      long long base, const double2 *__restrict__ cs, const double2 *csL, int c, int M,
-                                     double *brk)
+                                     double *brk, bool en, double &e)
 {
     constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
     const uint32_t lane8 = (threadIdx.x & (kLanes - 1)) * 8u;
@@ -570,6 +571,12 @@ __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const d
             });
         });
     }
+    if constexpr (BR)
+        if (en) {
+            double s = 0;
+            wfor<0, NM>([&](auto I) { s = fma(x[decltype(I)::value], l[decltype(I)::value], s); });
+            e += s;
+        }
     int r = 0;  // the registers' layout
     for (int pi = 0; pi < a.nph; pi++) {
         const int p = BR ? a.nph - 1 - pi : pi;
@@ -740,7 +747,7 @@ template <bool BR, bool BC, int NW, bool TU>
 
 __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
     k_winr(WinArgs a, const double *in, double *out, double *lam, long long cstride, const double2 *__restrict__ cs,
-           int M, double *__restrict__ partial, int slots)
+           int M, double *__restrict__ partial, int slots, double *__restrict__ epart)
 {
     extern __shared__ __align__(16) double lds[];
     const int lane = threadIdx.x & (kLanes - 1);
@@ -756,6 +763,8 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
     }
     const long long base = (long long)c * cstride;  // the chunk (lanes: the members' column offsets)
     const uint32_t *B = a.blob;
+    const bool en = BR && epart != nullptr;
+    double e = 0;  // en: this lane's (vector's) <x|l> over the wave's tiles
     for (int it = blockIdx.x * NW + wv; it < a.ntiles; it += gridDim.x * NW) {
         const int tl = __builtin_amdgcn_readfirstlane(a.tiles[it]);
         const int ja = tl / a.nBcls, jb = tl - ja * a.nBcls;
@@ -768,7 +777,7 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
         const int cls = (int)(fa >> 24) * 5 + (int)(fb >> 24);
         switch (cls) {
 #define AE_WR_CASE(CA, CB) \
-    case CA * 5 + CB: wr::tile<CA, CB, BR, BC, NW, TU>(a, t, in, out, lam, base, cs, csL, c, M, brk); break;
+    case CA * 5 + CB: wr::tile<CA, CB, BR, BC, NW, TU>(a, t, in, out, lam, base, cs, csL, c, M, brk, en, e); break;
             AE_WR_CASE(0, 0) AE_WR_CASE(0, 1) AE_WR_CASE(0, 2) AE_WR_CASE(0, 3) AE_WR_CASE(0, 4)
             AE_WR_CASE(1, 0) AE_WR_CASE(1, 1) AE_WR_CASE(1, 2) AE_WR_CASE(1, 3) AE_WR_CASE(1, 4)
             AE_WR_CASE(2, 0) AE_WR_CASE(2, 1) AE_WR_CASE(2, 2) AE_WR_CASE(2, 3) AE_WR_CASE(2, 4)
@@ -786,12 +795,15 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
             for (int q = 0; q < NW; q++) s += lds[((size_t)q * a.nkp + kp) * kLanes + lane];
             partial[(((long long)c * M + a.kpOp[kp]) * slots + blockIdx.x) * kLanes + lane] = s;
         }
+        // one energy row per wave
+        if (en) epart[(((long long)c * gridDim.x + blockIdx.x) * NW + wv) * kLanes + lane] = e;
     }
 }
 
 #define AE_WINR(BR, BC, NW, TU)                                                                             \
     template __global__ void k_winr<BR, BC, NW, TU>(WinArgs, const double *, double *, double *, long long,\
-                                                    const double2 *__restrict__, int, double *__restrict__, int);
+                                                    const double2 *__restrict__, int, double *__restrict__, int,   \
+                                                    double *__restrict__);
 AE_WINR(false, false, 4, false)
 AE_WINR(false, true, 4, false)
 AE_WINR(true, false, 2, false)

@@ -337,7 +337,11 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *  29 window passes for the vector-major sweeps (single vectors and
  *     batches below key 1): -1 (default) from 2^18 determinants on, 0
  *     never, 1 at any size.  They use the register tiles of key 28 (1 or 2)
- *     with one tile per lane, applied in place; key 24 then counts them.
+ *     with one tile per lane, applied in place; key 24 then counts them;
+ *  30 1 (default): a batch evaluated with its gradient on register window
+ *     tiles takes the energies <psi|H psi> in the first adjoint pass (whose
+ *     tiles hold the final state and H psi) instead of a separate dot over
+ *     both vectors; 0: the separate dot.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

@@ -278,3 +278,25 @@ def test_window_vector_major_config3(gpu):
         a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 0)
         E0, g0 = a.energy_gradient(w["theta"])
         assert abs(E - E0) <= SAME and np.max(np.abs(g - g0)) <= 1e-11
+
+
+def test_window_fused_energy(gpu):
+    """The energies of a gradient batch from the first adjoint window pass
+    (option 30 = 1, the default) against the separate dot (0) and the
+    energy-only batch, 70 vectors (two chunks) at 8 orbitals with stored
+    states and in place; the gradients are unchanged."""
+    w, thetas = _tups_case(8, 4, 4700, 70)
+    with from_workload(w, storage=2) as a:
+        for ck in (1, 0):
+            _window(a, 4, 4, ck)
+            assert a.get_option(AnsatzSynth.OPT_WINDOW_ENERGY) == 1
+            E1, G1 = a.energy_gradients(thetas)
+            Eonly = a.energies(thetas)
+            a.set_option(AnsatzSynth.OPT_WINDOW_ENERGY, 0)
+            E0, G0 = a.energy_gradients(thetas)
+            a.set_option(AnsatzSynth.OPT_WINDOW_ENERGY, 1)
+            assert a.get_option(AnsatzSynth.OPT_WINDOW_REGISTERS) == 2
+            assert np.max(np.abs(E1 - E0)) <= SAME and np.max(np.abs(E1 - Eonly)) <= SAME, ck
+            assert np.array_equal(G1, G0), ck
+            Es, gs = a.energy_gradient(thetas[69])
+            assert abs(E1[69] - Es) <= SAME
