@@ -2294,7 +2294,9 @@ struct EngineImpl {
     // not write it back (only lambda is updated in place).
     // energy rows of the fused first adjoint pass (register window tiles)
     bool fused_energy() const { return winOn && winReg && !wpass.empty(); }
-    int win_erows() const { return wpass.empty() ? 0 : wpass.back().gridA * 2; }
+    int win_erows() const { return wpass.empty() ? 0 : wpass.back().gridA; }
+    // the energy rows' groups (as the kernel forms them)
+    int win_egroups() const { return 1 << std::min(4, __builtin_ctz((unsigned)std::max(1, win_erows()))); }
     bool sweep_reverse_lanes(double *phi, double *lam, int nch, std::string &err, double *epart = nullptr)
     {
         if (winOn) return sweep_reverse_win(phi, lam, nch, err, epart);
@@ -3401,7 +3403,7 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         const size_t nch = (size_t)(nvec + kLanes - 1) / kLanes;
         if (!e.bl_psi.ensure(nch * e.N * kLanes, err) || !e.bl_lam.ensure(nch * e.N * kLanes, err) ||
             !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err) ||
-            !e.bl_wep.ensure((size_t)nch * std::max(1, e.win_erows()) * kLanes, err))
+            !e.bl_wep.ensure((size_t)nch * (std::max(1, e.win_erows()) + 16) * kLanes, err))
             return false;
         // stored forward states 0 .. segments, when they fit in half the
         // free memory
@@ -3485,8 +3487,14 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
                                    e.M, e.lslots, e.d_lrows.p, nvec, e.b_out.p + nvec, 2.0);
                 LCHK();
                 if (fe) {
-                    hipLaunchKernelGGL(k_reduce_lanes, dim3(nch), dim3(kBlock), 0, e.stream, e.bl_wep.p, 1,
-                                       e.win_erows(), nullptr, nvec, e.b_out.p, 1.0);
+                    // per row group, then over the groups
+                    const int P = e.win_egroups();
+                    double *tmp = e.bl_wep.p + (size_t)nch * e.win_erows() * kLanes;
+                    hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * P), dim3(kBlock), 0, e.stream, e.bl_wep.p, P,
+                                       e.win_erows() / P, nullptr, nvec, tmp, 1.0);
+                    LCHK();
+                    hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, tmp, nvec, P, (long long)P,
+                                       1LL, e.b_out.p, 1.0);
                     LCHK();
                 }
             }

@@ -585,11 +585,11 @@ __global__ void k_win(WinArgs a, const double *in, double *out, double *lam, lon
 // TU: every phase is a TUPS unit of five operators (beta single, alpha
 // single, paired double, beta single, alpha single) whose program words
 // rotate exactly the member pairs of that pattern (checked on the host).
-// epart (adjoint, or null): each wave also writes the sum of x * l over its
-// tiles before their first phase, per lane, at row (block * NW + wave) of
-// chunk c (gridDim.x * NW rows per chunk): launched on the sweep's first
-// adjoint pass, where x is the final state and l = H x, the rows sum to the
-// energies.
+// epart (adjoint, or null): each block also writes the sum of x * l over its
+// waves' tiles before their first phase, per lane, as row b / P of row group
+// b % P of chunk c (P = 2^min(4, trailing zeros of gridDim.x), gridDim.x / P
+// rows per group): launched on the sweep's first adjoint pass, where x is
+// the final state and l = H x, the rows sum to the energies.
 template <bool BR, bool BC, int NW, bool TU>
 __global__ void k_winr(WinArgs a, const double *in, double *out, double *lam, long long cstride,
                        const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots,

@@ -795,8 +795,21 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
             for (int q = 0; q < NW; q++) s += lds[((size_t)q * a.nkp + kp) * kLanes + lane];
             partial[(((long long)c * M + a.kpOp[kp]) * slots + blockIdx.x) * kLanes + lane] = s;
         }
-        // one energy row per wave
-        if (en) epart[(((long long)c * gridDim.x + blockIdx.x) * NW + wv) * kLanes + lane] = e;
+        if (en) {
+            // one energy row per block: P = 2^min(4, trailing zeros of the
+            // grid) row groups of gridDim.x / P rows, block b in group b % P
+            __shared__ double esum[NW][kLanes];
+            esum[wv][lane] = e;
+            __syncthreads();
+            if (wv == 0) {
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < NW; q++) s += esum[q][lane];
+                const int P = 1 << min(4, __builtin_ctz(gridDim.x)), R = gridDim.x / P;
+                const int k = blockIdx.x % P, r = blockIdx.x / P;
+                epart[(((long long)c * P + k) * R + r) * kLanes + lane] = s;
+            }
+        }
     }
 }
 
