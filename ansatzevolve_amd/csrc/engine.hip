@@ -758,14 +758,16 @@ struct EngineImpl {
         int rowsOff = 0, nrowsA = 0;  // its class member lists in d_wrows
         std::vector<int> cntA, cntB, startA, startB;  // class sizes and first members (per spin)
         std::vector<int> incA, incB;                  // inner electron counts of the classes
+        std::vector<uint32_t> outA, outB;             // their outer bits
         int maxMembers = 0, maxSide = 0;
         std::vector<int> allTiles;   // every tile, largest first
         std::vector<int> tileSize;   // members per tile (indexed by tile id)
         int tileOff = 0, ntiles = 0; // the tiles a sweep processes (d_wtiles)
         int gridF = 1, gridA = 1;    // blocks (multiples of 64 / V)
         // vector-major register tiles (k_winv): every tile, grouped by class,
-        // in tasks of at most 64 (d_wvtiles, d_wvtasks)
-        int vTileOff = 0, vTaskOff = 0, vTasks = 0;
+        // in tasks of at most 64 (d_wvtasks), each with its record block in
+        // d_wvrec (words from vRecOff)
+        int vRecOff = 0, vTaskOff = 0, vTasks = 0;
         int gridVF = 1, gridVA = 1;
     };
     // kOptWindow: window width in spatial orbitals (0: off; -1, the default:
@@ -782,7 +784,8 @@ struct EngineImpl {
     std::vector<WinPass> wpass;
     DBuf<uint32_t> d_wblob, d_wrows;
     std::vector<uint32_t> hrows;  // host copy of d_wrows
-    DBuf<int> d_wkp, d_wtiles, d_wvtiles;
+    DBuf<int> d_wkp, d_wtiles;
+    DBuf<uint32_t> d_wvrec;
     DBuf<int4> d_wvtasks;
     // kOptWindowVector: window passes for the vector-major sweeps (-1, the
     // default: with register tiles from 2^18 determinants; 0 off; 1 with
@@ -826,15 +829,16 @@ struct EngineImpl {
     {
         const int mk = mark_begin(1);
         for (const WinPass &wp : wpass) {
-            WinArgs a = winargs(wp);
-            a.tiles = d_wvtiles.p + wp.vTileOff;
+            const WinArgs a = winargs(wp);
             const dim3 grid((unsigned)wp.gridVF, nvec);
             if (winTu)
                 hipLaunchKernelGGL((k_winv<false, true>), grid, dim3(4 * kLanes), 0, stream, a,
-                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, v, nullptr, N, cs, M, nullptr, 0);
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, d_wvrec.p + wp.vRecOff, v, nullptr, N, cs,
+                                   M, nullptr, 0);
             else
                 hipLaunchKernelGGL((k_winv<false, false>), grid, dim3(4 * kLanes), 0, stream, a,
-                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, v, nullptr, N, cs, M, nullptr, 0);
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, d_wvrec.p + wp.vRecOff, v, nullptr, N, cs,
+                                   M, nullptr, 0);
             LCHK();
         }
         mark_end(mk);
@@ -848,16 +852,17 @@ struct EngineImpl {
         const int mk = mark_begin(2);
         for (int p = (int)wpass.size() - 1; p >= 0; p--) {
             const WinPass &wp = wpass[p];
-            WinArgs a = winargs(wp);
-            a.tiles = d_wvtiles.p + wp.vTileOff;
+            const WinArgs a = winargs(wp);
             const size_t lds = (size_t)4 * wp.nkp * kLanes * sizeof(double);
             const dim3 grid((unsigned)wp.gridVA, nvec);
             if (winTu)
                 hipLaunchKernelGGL((k_winv<true, true>), grid, dim3(4 * kLanes), lds, stream, a,
-                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, phi, lam, N, csneg, M, partial, slots);
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, d_wvrec.p + wp.vRecOff, phi, lam, N, csneg,
+                                   M, partial, slots);
             else
                 hipLaunchKernelGGL((k_winv<true, false>), grid, dim3(4 * kLanes), lds, stream, a,
-                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, phi, lam, N, csneg, M, partial, slots);
+                                   d_wvtasks.p + wp.vTaskOff, wp.vTasks, d_wvrec.p + wp.vRecOff, phi, lam, N, csneg,
+                                   M, partial, slots);
             LCHK();
         }
         mark_end(mk);
@@ -1006,6 +1011,7 @@ struct EngineImpl {
                     cstart.push_back((uint32_t)((int)rows.size() - base));
                     (spin ? wp.startB : wp.startA).push_back((int)rows.size() - base);
                     (spin ? wp.incB : wp.incA).push_back(kv.second);
+                    (spin ? wp.outB : wp.outA).push_back(kv.first);
                     (spin ? cntB : cntA).push_back((int)byCount[kv.second].size());
                     for (uint32_t x : byCount[kv.second]) {
                         const uint32_t st = kv.first | (x << wp.w0);
@@ -1174,24 +1180,39 @@ struct EngineImpl {
         }
         if (!upload_vec(d_wtiles, tl, err)) return false;
         // vector-major tasks (register tiles): the tiles by class of inner
-        // counts (5 ca + cb), up to 64 of one class per task
+        // counts (5 ca + cb), up to 64 of one class per task.  A task's record
+        // block holds, per lane (tile) and word q, q < NA its alpha rows, then
+        // NB beta rows, then its alpha and beta outer bits: (NA + NB + 2) x 64
+        // words, so a lane finds its members in one coalesced load per word
         if (winReg) {
-            std::vector<int> vt;
+            std::vector<uint32_t> rec;
             std::vector<int4> tasks;
             for (WinPass &wp : wpass) {
-                wp.vTileOff = (int)vt.size();
+                wp.vRecOff = (int)rec.size();
                 wp.vTaskOff = (int)tasks.size();
                 std::vector<std::vector<int>> byCls(25);
                 for (int t : wp.allTiles) byCls[wp.incA[t / wp.nBcls] * 5 + wp.incB[t % wp.nBcls]].push_back(t);
+                const uint32_t *rA = hrows.data() + wp.rowsOff, *rB = rA + wp.nrowsA;
                 for (int cl = 0; cl < 25; cl++)
                     for (size_t i = 0; i < byCls[cl].size(); i += 64) {
                         const int cnt = (int)std::min<size_t>(64, byCls[cl].size() - i);
-                        tasks.push_back(make_int4(cl, (int)vt.size() - wp.vTileOff, cnt, 0));
-                        vt.insert(vt.end(), byCls[cl].begin() + i, byCls[cl].begin() + i + cnt);
+                        const int t0 = byCls[cl][i];
+                        const int NA = wp.cntA[t0 / wp.nBcls], NB = wp.cntB[t0 % wp.nBcls];
+                        const size_t off = rec.size();
+                        tasks.push_back(make_int4(cl, (int)(off - wp.vRecOff), cnt, 0));
+                        rec.resize(off + (size_t)(NA + NB + 2) * kLanes, 0u);
+                        for (int ln = 0; ln < cnt; ln++) {
+                            const int t = byCls[cl][i + ln], ia = t / wp.nBcls, ib = t - ia * wp.nBcls;
+                            for (int q = 0; q < NA; q++) rec[off + (size_t)q * kLanes + ln] = rA[wp.startA[ia] + q];
+                            for (int q = 0; q < NB; q++)
+                                rec[off + (size_t)(NA + q) * kLanes + ln] = rB[wp.startB[ib] + q];
+                            rec[off + (size_t)(NA + NB) * kLanes + ln] = wp.outA[ia];
+                            rec[off + (size_t)(NA + NB + 1) * kLanes + ln] = wp.outB[ib];
+                        }
                     }
                 wp.vTasks = (int)tasks.size() - wp.vTaskOff;
             }
-            if (!upload_vec(d_wvtiles, vt, err) || !upload_vec(d_wvtasks, tasks, err)) return false;
+            if (!upload_vec(d_wvrec, rec, err) || !upload_vec(d_wvtasks, tasks, err)) return false;
         }
         // grids: as many blocks as the chip holds at once
         std::vector<int> lrows(M, 0);

@@ -597,15 +597,18 @@ __global__ void k_winr(WinArgs a, const double *in, double *out, double *lam, lo
 
 // Window passes in the vector-major layout (one vector per grid row,
 // vstride doubles apart), register tiles of four-orbital windows as
-// k_winr, lane = tile: a wave takes a task {class = 5 ca + cb, first listed
-// tile, count <= 64} of tiles of one class of inner counts.  Forward and
-// adjoint in place (psi is written back in the adjoint); the adjoint sums
-// its brackets per block (4 waves, LDS rows of nkp x 64 per wave) into
+// k_winr, lane = tile: a wave takes a task {class = 5 ca + cb, offset of
+// its record block in rec, count <= 64} of tiles of one class of inner
+// counts; the block holds (NA + NB + 2) words per lane, word q at
+// [q * 64 + lane]: the tile's NA alpha rows (alpha rank * nb), its NB beta
+// rows (beta rank), its alpha and beta outer bits.  Forward and adjoint in
+// place (psi is written back in the adjoint); the adjoint sums its brackets
+// per block (4 waves, LDS rows of nkp x 64 per wave) into
 // partial[(vector * M + k) * slots + block].
 template <bool BR, bool TU>
-__global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, double *v, double *lam,
-                       long long vstride, const double2 *__restrict__ cs, int M, double *__restrict__ partial,
-                       int slots);
+__global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, const uint32_t *__restrict__ rec,
+                       double *v, double *lam, long long vstride, const double2 *__restrict__ cs, int M,
+                       double *__restrict__ partial, int slots);
 
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>

@@ -613,23 +613,21 @@ __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const d
 // One vector per grid row (vector-major layout, vstride doubles apart);
 // a wave takes a task of up to 64 tiles of one class (inner counts), lane
 // = tile: the member indices are compile-time as above, the addresses and
-// the outer-bit signs per lane, angles and program words uniform.
+// the outer-bit signs per lane, angles and program words uniform.  rec: the
+// task's record block, word q of lane l at rec[q * 64 + l]: NA alpha rows,
+// NB beta rows, the alpha and the beta outer bits.
 template <int CA, int CB, bool BR, bool TU>
-__device__ __forceinline__ void tile_v(const WinArgs &a, int tl, bool valid, double *v, double *lam,
-                                       const double2 *__restrict__ cs, double *brk)
+__device__ __forceinline__ void tile_v(const WinArgs &a, const uint32_t *__restrict__ rec, bool valid, double *v,
+                                       double *lam, const double2 *__restrict__ cs, double *brk)
 {
     constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
     const int lane = threadIdx.x & (kLanes - 1);
     const uint32_t *B = a.blob;
-    const int ja = tl / a.nBcls, jb = tl - ja * a.nBcls;
-    const uint32_t fa = a.blob[a.oCls + ja], fb = a.blob[a.oCls + a.nAcls + jb];
-    const int sA = (int)a.blob[a.oCStart + ja], sB = a.nrowsA + (int)a.blob[a.oCStart + a.nAcls + jb];
-    const uint32_t wm = 0xfu << a.w0;
-    const uint32_t outer = (((fa & 0xffffffu) << a.n) | (fb & 0xffffffu)) & ~((wm << a.n) | wm);
     uint32_t ra[NA], rb[NB];
     double x[NM], l[BR ? NM : 1];
-    wfor<0, NA>([&](auto I) { ra[decltype(I)::value] = a.rows[sA + decltype(I)::value]; });
-    wfor<0, NB>([&](auto I) { rb[decltype(I)::value] = a.rows[sB + decltype(I)::value]; });
+    wfor<0, NA>([&](auto I) { ra[decltype(I)::value] = rec[decltype(I)::value * kLanes + lane]; });
+    wfor<0, NB>([&](auto I) { rb[decltype(I)::value] = rec[(NA + decltype(I)::value) * kLanes + lane]; });
+    const uint32_t outer = TU ? 0u : (rec[(NA + NB) * kLanes + lane] << a.n) | rec[(NA + NB + 1) * kLanes + lane];
     wfor<0, NM>([&](auto I) {
         constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
         const long long g = (long long)ra[i] + rb[j];
@@ -688,8 +686,9 @@ __device__ __forceinline__ void tile_v(const WinArgs &a, int tl, bool valid, dou
 
 template <bool BR, bool TU>
 __global__ void __launch_bounds__(4 * kLanes, BR ? 2 : 4)
-    k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, double *v, double *lam, long long vstride,
-           const double2 *__restrict__ cs, int M, double *__restrict__ partial, int slots)
+    k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, const uint32_t *__restrict__ rec, double *v,
+           double *lam, long long vstride, const double2 *__restrict__ cs, int M, double *__restrict__ partial,
+           int slots)
 {
     constexpr int NW = 4;
     extern __shared__ __align__(16) double lds[];
@@ -703,14 +702,13 @@ __global__ void __launch_bounds__(4 * kLanes, BR ? 2 : 4)
     if (BR) lam += (long long)vec * vstride;
     cs += (long long)vec * M;
     for (int it = blockIdx.x * NW + wv; it < ntasks; it += gridDim.x * NW) {
-        const int4 tk = tasks[it];  // {class, first listed tile, count}
-        const int cls = __builtin_amdgcn_readfirstlane(tk.x), first = __builtin_amdgcn_readfirstlane(tk.y),
+        const int4 tk = tasks[it];  // {class, record block offset, count}
+        const int cls = __builtin_amdgcn_readfirstlane(tk.x), off = __builtin_amdgcn_readfirstlane(tk.y),
                   cnt = __builtin_amdgcn_readfirstlane(tk.z);
         const bool valid = lane < cnt;
-        const int tl = a.tiles[first + (valid ? lane : 0)];
         switch (cls) {
 #define AE_WV_CASE(CA, CB) \
-    case CA * 5 + CB: wr::tile_v<CA, CB, BR, TU>(a, tl, valid, v, lam, cs, brk); break;
+    case CA * 5 + CB: wr::tile_v<CA, CB, BR, TU>(a, rec + off, valid, v, lam, cs, brk); break;
             AE_WV_CASE(0, 0) AE_WV_CASE(0, 1) AE_WV_CASE(0, 2) AE_WV_CASE(0, 3) AE_WV_CASE(0, 4)
             AE_WV_CASE(1, 0) AE_WV_CASE(1, 1) AE_WV_CASE(1, 2) AE_WV_CASE(1, 3) AE_WV_CASE(1, 4)
             AE_WV_CASE(2, 0) AE_WV_CASE(2, 1) AE_WV_CASE(2, 2) AE_WV_CASE(2, 3) AE_WV_CASE(2, 4)
@@ -733,14 +731,15 @@ __global__ void __launch_bounds__(4 * kLanes, BR ? 2 : 4)
     }
 }
 
-template __global__ void k_winv<false, false>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
-                                              const double2 *__restrict__, int, double *__restrict__, int);
-template __global__ void k_winv<true, false>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
-                                             const double2 *__restrict__, int, double *__restrict__, int);
-template __global__ void k_winv<false, true>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
-                                             const double2 *__restrict__, int, double *__restrict__, int);
-template __global__ void k_winv<true, true>(WinArgs, const int4 *__restrict__, int, double *, double *, long long,
-                                            const double2 *__restrict__, int, double *__restrict__, int);
+#define AE_WINV(BR, TU)                                                                                      \
+    template __global__ void k_winv<BR, TU>(WinArgs, const int4 *__restrict__, int, const uint32_t *__restrict__, \
+                                            double *, double *, long long, const double2 *__restrict__, int,      \
+                                            double *__restrict__, int);
+AE_WINV(false, false)
+AE_WINV(true, false)
+AE_WINV(false, true)
+AE_WINV(true, true)
+#undef AE_WINV
 
 // occupancy: forward 4 waves per SIMD (at most 128 VGPRs), adjoint 2 (256)
 template <bool BR, bool BC, int NW, bool TU>
