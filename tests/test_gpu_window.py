@@ -257,6 +257,17 @@ def test_window_vector_major(gpu, n, layers, variant):
         assert np.max(np.abs(Eb - Eb0)) <= SAME and np.max(np.abs(Gb - Gb0)) <= SAME
         # the energy-only batch path
         assert np.max(np.abs(a.energies(thetas) - Eb0)) <= SAME
+        if n <= 8:
+            # the other vector-major callers: Hessian, projected gradient, subspace products
+            rng = np.random.default_rng(n + 17)
+            proj = rng.standard_normal(1 << (2 * n))
+            proj /= np.linalg.norm(proj)
+            got = (a.hessian(thetas[1]), a.overlap_gradient(thetas[1], proj), a.subspace_diagonalise(thetas[:3]))
+            a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 0)
+            want = (a.hessian(thetas[1]), a.overlap_gradient(thetas[1], proj), a.subspace_diagonalise(thetas[:3]))
+            assert np.max(np.abs(got[0] - want[0])) <= 1e-11
+            assert abs(got[1][0] - want[1][0]) <= SAME and np.max(np.abs(got[1][1] - want[1][1])) <= SAME
+            assert np.max(np.abs(got[2][1] - want[2][1])) <= 1e-11 and np.max(np.abs(got[2][2] - want[2][2])) <= 1e-11
     if n <= 8:
         o = OracleEngine(2 * n, [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
         o.set_integrals(w["h"], w["g"], explicit=True)
