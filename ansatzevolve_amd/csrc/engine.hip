@@ -769,6 +769,8 @@ struct EngineImpl {
         // d_wvrec (words from vRecOff)
         int vRecOff = 0, vTaskOff = 0, vTasks = 0;
         int gridVF = 1, gridVA = 1;
+        int allOff = 0;  // every tile in d_wall (Hessian tangent passes, k_wint)
+        int gridT = 1;
     };
     // kOptWindow: window width in spatial orbitals (0: off; -1, the default:
     // four-orbital windows when their tiles fit in registers, else off)
@@ -784,7 +786,7 @@ struct EngineImpl {
     std::vector<WinPass> wpass;
     DBuf<uint32_t> d_wblob, d_wrows;
     std::vector<uint32_t> hrows;  // host copy of d_wrows
-    DBuf<int> d_wkp, d_wtiles;
+    DBuf<int> d_wkp, d_wtiles, d_wall;
     DBuf<uint32_t> d_wvrec;
     DBuf<int4> d_wvtasks;
     // kOptWindowVector: window passes for the vector-major sweeps (-1, the
@@ -792,6 +794,7 @@ struct EngineImpl {
     // register tiles at any size)
     int winVec = -1;
     bool winFuseE = true;  // kOptWindowEnergy
+    bool winHess = true;   // kOptWindowHessian
     bool use_wv() const
     {
         return winOn && winReg && !cplx && (winVec > 0 || (winVec < 0 && N >= (1LL << 18)));
@@ -1177,8 +1180,9 @@ struct EngineImpl {
             wp.tileOff = (int)tl.size();
             wp.ntiles = (int)wp.allTiles.size();
             tl.insert(tl.end(), wp.allTiles.begin(), wp.allTiles.end());
+            wp.allOff = wp.tileOff;
         }
-        if (!upload_vec(d_wtiles, tl, err)) return false;
+        if (!upload_vec(d_wtiles, tl, err) || !upload_vec(d_wall, tl, err)) return false;
         // vector-major tasks (register tiles): the tiles by class of inner
         // counts (5 ca + cb), up to 64 of one class per task.  A task's record
         // block holds, per lane (tile) and word q, q < NA its alpha rows, then
@@ -1244,6 +1248,20 @@ struct EngineImpl {
             for (int q = 0; q < wp.nkp; q++) lrows[kpOp[wp.kp0 + q]] = wp.gridA;
             wslots = std::max(wslots, wp.gridA);
         }
+        if (winReg && winTu)
+            for (WinPass &wp : wpass) {
+                // Hessian tangent passes: 2 waves per block, at most 4 x CUs blocks (its partial rows)
+                const size_t lds = (size_t)2 * wp.nkp * (kLanes + 1) * sizeof(double);
+                HCHK(hipFuncSetAttribute((const void *)k_wint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                int nbk = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, (const void *)k_wint, 2 * kLanes, lds) !=
+                        hipSuccess ||
+                    nbk < 1)
+                    nbk = 1;
+                const long long g = std::min<long long>(std::min<long long>((long long)nbk * numCU, 4LL * numCU),
+                                                        ((long long)wp.allTiles.size() + 1) / 2);
+                wp.gridT = (int)std::max<long long>(1, g);
+            }
         if (winReg)
             for (WinPass &wp : wpass)
                 for (int br = 0; br < 2; br++) {
@@ -3761,6 +3779,13 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         e.bnvec = 0;  // stored states and lists are renewed at the next upload
         return true;
+    case kOptWindowHessian:
+        if (value < 0 || value > 1) {
+            err = "window Hessian option must be 0 or 1";
+            return false;
+        }
+        e.winHess = value == 1;
+        return true;
     case kOptWindowEnergy:
         if (value < 0 || value > 1) {
             err = "window energy option must be 0 or 1";
@@ -3835,6 +3860,7 @@ int Engine::option(int key) const
     case kOptWindowPasses: return e.winOn ? (int)e.wpass.size() : 0;
     case kOptWindowVector: return e.winVec;
     case kOptWindowEnergy: return e.winFuseE ? 1 : 0;
+    case kOptWindowHessian: return e.winHess ? 1 : 0;
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }
@@ -4087,7 +4113,26 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
         double *psi = e.hz_psi2.p, *lam = e.hz_lam2.p;
         HCHK(hipMemcpyAsync(psi, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
         HCHK(hipMemcpyAsync(lam, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
+        // window passes (TUPS units in four-orbital register tiles): every
+        // tile of a pass, psi and lambda read from one half of their buffers
+        // and written by chunk 0 to the other
+        const bool tanWin = e.winOn && e.winReg && e.winTu && e.winHess;
+        double *psiB = psi + N, *lamB = lam + N;
+        for (size_t p = 0; tanWin && p < e.wpass.size(); p++) {
+            const auto &wp = e.wpass[p];
+            WinArgs a = e.winargs(wp);
+            a.tiles = e.d_wall.p + wp.allOff;
+            a.ntiles = (int)wp.allTiles.size();
+            const size_t lds = (size_t)2 * wp.nkp * (kLanes + 1) * sizeof(double);
+            hipLaunchKernelGGL(k_wint, dim3((unsigned)std::min(wp.gridT, TS), nch), dim3(2 * kLanes), lds, e.stream, a,
+                               psi, lam, psiB, lamB, e.hz_lx.p, (long long)N * kLanes, (const double2 *)e.d_cs.p, M,
+                               e.hz_inj.p, e.hz_scl.p, e.hz_xpart.p, e.hz_dpart.p, TS);
+            LCHK();
+            std::swap(psi, psiB);
+            std::swap(lam, lamB);
+        }
         for (const Seg &sg : e.laneS.segs) {
+            if (tanWin) break;
             Orb2Args oa = e.orb2args(e.laneS, sg, false);
             oa.list = oa.llist = nullptr;
             TanArgs ta;

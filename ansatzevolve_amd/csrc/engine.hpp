@@ -74,6 +74,8 @@ enum EngineOption : int {
                               // the window passes hold register tiles (key 28 is 1 or 2)
     kOptWindowEnergy = 30,    // 1 (default): batches with the gradient and register window tiles take the energies
                               // in the first adjoint pass instead of a separate dot; 0: the separate dot
+    kOptWindowHessian = 31,   // 1 (default): the Hessian's tangent sweep runs as window passes when the window
+                              // passes hold TUPS units in registers (key 28 = 2); 0: two-group segments
 };
 
 class Engine {

@@ -610,6 +610,21 @@ __global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, co
                        double *v, double *lam, long long vstride, const double2 *__restrict__ cs, int M,
                        double *__restrict__ partial, int slots);
 
+// Hessian tangent pass over four-orbital windows whose phases are TUPS
+// units: lane = tangent (64 per chunk, grid.y = chunks, in place in B, lane
+// layout), the tile's psi and lambda (one vector each, psiIn/lamIn) in
+// registers beside it.  Every operator rotates all three, its injection lane
+// (inj[k] = chunk * 64 + lane) adds scale[k] G psi, and the brackets
+// X[lane][k] = <lambda|G_k|B_lane> and D[k] = <lambda|G_k G_k|psi> are summed
+// per block into xpart[((c * M + k) * slots + block) * 64 + lane] and (chunk
+// 0) dpart[k * slots + block].  Chunk 0 writes psi and lambda after the pass
+// to psiOut/lamOut.  Every tile (a.tiles lists them all); 2 waves per block,
+// LDS 2 x nkp x 65 doubles.
+__global__ void k_wint(WinArgs a, const double *psiIn, const double *lamIn, double *psiOut, double *lamOut, double *B,
+                       long long cstride, const double2 *__restrict__ cs, int M, const int *__restrict__ inj,
+                       const double *__restrict__ scale, double *__restrict__ xpart, double *__restrict__ dpart,
+                       int slots);
+
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>
 __global__ void k_tangent(const double *__restrict__ psiIn, double *__restrict__ psiOut,

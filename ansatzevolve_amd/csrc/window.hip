@@ -812,6 +812,178 @@ __global__ void __launch_bounds__(NW * kLanes, BR ? 2 : 4)
     }
 }
 
+// ------------------------------------------- Hessian tangents (k_wint)
+namespace wr {
+// One TUPS operator on the tangent tile: psi, lambda and the lane's tangent
+// rotate (S carries the operator's phase, scs = its scale times the phase);
+// the injection lane adds scs G psi; xacc gets <lambda|G|b> without the
+// phase, dacc <lambda|G G|psi>.
+template <int CA, int CB, int KIND, int NM>
+__device__ __forceinline__ void op_tan(double C, double S, double scs, bool inj, double (&b)[NM], double (&ps)[NM],
+                                       double (&lm)[NM], double &xacc, double &dacc)
+{
+    constexpr int NB = C4[CB];
+    constexpr int NSA = nsub(CA, 0), NSB = nsub(CB, 0);
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+    wfor<0, NSA * NSB>([&](auto I) {
+        constexpr Sub A = sub(CA, 0, decltype(I)::value / NSB);
+        constexpr Sub Bs = sub(CB, 0, decltype(I)::value % NSB);
+        constexpr unsigned slots = tups_slots(KIND, A.sh, Bs.sh);
+        wfor<0, 6>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if constexpr ((slots >> r) & 1u) {
+                constexpr int p = PP[r], q = QQ[r];
+                constexpr int mp = ((p >> 1) ? A.i1 : A.i0) * NB + ((p & 1) ? Bs.i1 : Bs.i0);
+                constexpr int mq = ((q >> 1) ? A.i1 : A.i0) * NB + ((q & 1) ? Bs.i1 : Bs.i0);
+                rot(ps[mp], ps[mq], C, S);
+                rot(lm[mp], lm[mq], C, S);
+                rot(b[mp], b[mq], C, S);
+                if (inj) {
+                    b[mp] = fma(scs, ps[mq], b[mp]);
+                    b[mq] = fma(-scs, ps[mp], b[mq]);
+                }
+                xacc = fma(lm[mp], b[mq], xacc);
+                xacc = fma(-lm[mq], b[mp], xacc);
+                dacc = fma(-lm[mp], ps[mp], dacc);
+                dacc = fma(-lm[mq], ps[mq], dacc);
+            }
+        });
+    });
+}
+
+// A tile of the tangent pass: lane = tangent (chunk c), psi and lambda one
+// vector each (every lane holds the same values).  bx: this wave's bracket
+// rows (nkp x 64), bd: its nkp D sums.
+template <int CA, int CB>
+__device__ __forceinline__ void tile_t(const WinArgs &a, const TileCtx &t, const double *psiIn, const double *lamIn,
+                                       double *psiOut, double *lamOut, double *B, const double2 *__restrict__ cs,
+                                       const int *__restrict__ inj, const double *__restrict__ scale, int c, double *bx,
+                                       double *bd)
+{
+    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const uint32_t lane8 = lane * 8u;
+    const uint32_t *Bl = a.blob;
+    double b[NM], ps[NM], lm[NM];
+    {
+        uint32_t col[NB], colS[NB];
+        wfor<0, NB>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const uint32_t rb = ld(a.rows, t.sB + j);
+            col[j] = rb * (kLanes * 8u) + lane8;
+            colS[j] = rb * 8u;
+        });
+        wfor<0, NA>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const long long ra = ld(a.rows, t.sA + i);
+            const double *rb_ = B + ra * kLanes, *rp = psiIn + ra, *rl = lamIn + ra;
+            wfor<0, NB>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                b[i * NB + j] = ld_at(rb_, col[j]);
+                ps[i * NB + j] = *(const double *)((const char *)rp + colS[j]);
+                lm[i * NB + j] = *(const double *)((const char *)rl + colS[j]);
+            });
+        });
+    }
+    double none[1];
+    int r = 0;
+    for (int p = 0; p < a.nph; p++) {
+        const int lp = __builtin_ctz(ld(Bl, p * 8 + 4)) - a.w0;
+        to_layout<CA, CB, true>((lp - r) & 3, b, ps);
+        to_layout<CA, CB, false>((lp - r) & 3, lm, none);
+        r = lp;
+        const int k0 = (int)ld(Bl, p * 8 + 0), kp = (int)ld(Bl, p * 8 + 3);
+        wfor<0, 5>([&](auto J) {
+            constexpr int kk = decltype(J)::value;
+            const int k = k0 + kk;
+            const double2 cst = cs[k];
+            const uint32_t sg = ld(Bl, a.oProgT + (p * kOrbMaxOps + kk) * 16 + 9);
+            const bool in = __builtin_amdgcn_readfirstlane(inj[k]) == c * kLanes + lane;
+            double xacc = 0, dacc = 0;
+            op_tan<CA, CB, kTupsKind[kk]>(cst.x, flip(cst.y, sg), flip(scale[k], sg), in, b, ps, lm, xacc, dacc);
+            bx[(kp + kk) * kLanes + lane] += flip(xacc, sg);
+            if (lane == 0) bd[kp + kk] += dacc;
+        });
+    }
+    to_layout<CA, CB, true>((0 - r) & 3, b, ps);
+    to_layout<CA, CB, false>((0 - r) & 3, lm, none);
+    const uint32_t *rows = a.rows;
+    asm volatile("" : "+s"(rows));
+    uint32_t col[NB], colS[NB];
+    wfor<0, NB>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const uint32_t rb = ld(rows, t.sB + j);
+        col[j] = rb * (kLanes * 8u) + lane8;
+        colS[j] = rb * 8u;
+    });
+    wfor<0, NA>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const long long ra = ld(rows, t.sA + i);
+        double *rb_ = B + ra * kLanes;
+        wfor<0, NB>([&](auto J) { st_at(b[i * NB + decltype(J)::value], rb_, col[decltype(J)::value]); });
+        if (c == 0 && lane == 0) {
+            double *rp = psiOut + ra, *rl = lamOut + ra;
+            wfor<0, NB>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                *(double *)((char *)rp + colS[j]) = ps[i * NB + j];
+                *(double *)((char *)rl + colS[j]) = lm[i * NB + j];
+            });
+        }
+    });
+}
+}  // namespace wr
+
+__global__ void __launch_bounds__(2 * kLanes, 2)
+    k_wint(WinArgs a, const double *psiIn, const double *lamIn, double *psiOut, double *lamOut, double *B,
+           long long cstride, const double2 *__restrict__ cs, int M, const int *__restrict__ inj,
+           const double *__restrict__ scale, double *__restrict__ xpart, double *__restrict__ dpart, int slots)
+{
+    constexpr int NW = 2;
+    extern __shared__ __align__(16) double lds[];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int c = blockIdx.y;
+    const int rowsW = a.nkp * (kLanes + 1);  // doubles per wave: X rows, then D
+    double *bx = lds + (size_t)wv * rowsW, *bd = bx + (size_t)a.nkp * kLanes;
+    for (int e = lane; e < rowsW; e += kLanes) bx[e] = 0.0;
+    B += (long long)c * cstride;
+    const uint32_t *Bl = a.blob;
+    for (int it = blockIdx.x * NW + wv; it < a.ntiles; it += gridDim.x * NW) {
+        const int tl = __builtin_amdgcn_readfirstlane(a.tiles[it]);
+        const int ja = tl / a.nBcls, jb = tl - ja * a.nBcls;
+        const uint32_t fa = wr::ld(Bl, a.oCls + ja), fb = wr::ld(Bl, a.oCls + a.nAcls + jb);
+        wr::TileCtx t;
+        t.aOut = fa & 0xffffffu;
+        t.bOut = fb & 0xffffffu;
+        t.sA = (int)wr::ld(Bl, a.oCStart + ja);
+        t.sB = a.nrowsA + (int)wr::ld(Bl, a.oCStart + a.nAcls + jb);
+        const int cls = (int)(fa >> 24) * 5 + (int)(fb >> 24);
+        switch (cls) {
+#define AE_WT_CASE(CA, CB) \
+    case CA * 5 + CB: wr::tile_t<CA, CB>(a, t, psiIn, lamIn, psiOut, lamOut, B, cs, inj, scale, c, bx, bd); break;
+            AE_WT_CASE(0, 0) AE_WT_CASE(0, 1) AE_WT_CASE(0, 2) AE_WT_CASE(0, 3) AE_WT_CASE(0, 4)
+            AE_WT_CASE(1, 0) AE_WT_CASE(1, 1) AE_WT_CASE(1, 2) AE_WT_CASE(1, 3) AE_WT_CASE(1, 4)
+            AE_WT_CASE(2, 0) AE_WT_CASE(2, 1) AE_WT_CASE(2, 2) AE_WT_CASE(2, 3) AE_WT_CASE(2, 4)
+            AE_WT_CASE(3, 0) AE_WT_CASE(3, 1) AE_WT_CASE(3, 2) AE_WT_CASE(3, 3) AE_WT_CASE(3, 4)
+            AE_WT_CASE(4, 0) AE_WT_CASE(4, 1) AE_WT_CASE(4, 2) AE_WT_CASE(4, 3) AE_WT_CASE(4, 4)
+#undef AE_WT_CASE
+        default: break;
+        }
+    }
+    __syncthreads();
+    for (int kp = wv; kp < a.nkp; kp += NW) {
+        double sx = 0, sd = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            sx += lds[(size_t)q * rowsW + (size_t)kp * kLanes + lane];
+            sd += lds[(size_t)q * rowsW + (size_t)a.nkp * kLanes + kp];
+        }
+        const int k = a.kpOp[kp];
+        xpart[(((long long)c * M + k) * slots + blockIdx.x) * kLanes + lane] = sx;
+        if (c == 0 && lane == 0) dpart[(long long)k * slots + blockIdx.x] = sd;
+    }
+}
+
 #define AE_WINR(BR, BC, NW, TU)                                                                             \
     template __global__ void k_winr<BR, BC, NW, TU>(WinArgs, const double *, double *, double *, long long,\
                                                     const double2 *__restrict__, int, double *__restrict__, int,   \

@@ -341,7 +341,10 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *  30 1 (default): a batch evaluated with its gradient on register window
  *     tiles takes the energies <psi|H psi> in the first adjoint pass (whose
  *     tiles hold the final state and H psi) instead of a separate dot over
- *     both vectors; 0: the separate dot.
+ *     both vectors; 0: the separate dot;
+ *  31 1 (default): the Hessian's tangent sweep (getHessian_COMP,
+ *     getHessianProj) runs as the window passes when they hold TUPS units
+ *     in registers (key 28 = 2); 0: the two-group segments.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

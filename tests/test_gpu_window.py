@@ -311,3 +311,34 @@ def test_window_fused_energy(gpu):
             assert np.array_equal(G1, G0), ck
             Es, gs = a.energy_gradient(thetas[69])
             assert abs(E1[69] - Es) <= SAME
+
+
+@pytest.mark.parametrize("n,layers", [(8, 4), (10, 3)])
+def test_window_hessian_tangents(gpu, n, layers):
+    """The Hessian's tangent sweep as window passes (option 31 = 1, the
+    default) against the two-group segments (0), for the energy and the
+    projected overlap, and the energy Hessian against the oracle at 8
+    orbitals."""
+    w, thetas = _tups_case(n, layers, 4800 + n, 2)
+    with from_workload(w, storage=2) as a:
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_REGISTERS) == 2
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_HESSIAN) == 1
+        rng = np.random.default_rng(n)
+        proj = rng.standard_normal(1 << (2 * n)) if n <= 8 else None
+        if proj is not None:
+            proj /= np.linalg.norm(proj)
+        H1 = a.hessian(thetas[0])
+        P1 = a.hessian_proj(thetas[1], proj) if proj is not None else None
+        a.set_option(AnsatzSynth.OPT_WINDOW_HESSIAN, 0)
+        H0 = a.hessian(thetas[0])
+        P0 = a.hessian_proj(thetas[1], proj) if proj is not None else None
+        scale = max(1.0, np.max(np.abs(H0)))
+        assert np.max(np.abs(H1 - H0)) <= 1e-12 * scale
+        assert np.max(np.abs(H1 - H1.T)) <= 1e-10 * scale
+        if P1 is not None:
+            assert np.max(np.abs(P1[0] - P0[0])) <= 1e-12 and np.max(np.abs(P1[1] - P0[1])) <= 1e-12
+    if n <= 8:
+        o = OracleEngine(2 * n, [w["start_idx"]], [1.0], w["ops"], w["order"], w["scale"])
+        o.set_integrals(w["h"], w["g"], explicit=True)
+        Ho = o.hessian(thetas[0])[0]
+        assert np.max(np.abs(H1 - Ho)) <= 1e-8
