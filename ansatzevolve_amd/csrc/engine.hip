@@ -4094,7 +4094,9 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
     // per segment for every tangent, psi and lambda advanced by the
     // vector-major two-group kernel), else one launch per operator over the
     // tangents active so far
-    const bool tanLanes = e.mode == kSz && e.laneSAllTwo && P >= kHessLaneMin;
+    // window passes (TUPS units in four-orbital register tiles) or two-group segments
+    const bool tanWin = e.mode == kSz && P >= kHessLaneMin && e.winOn && e.winReg && e.winTu && e.winHess;
+    const bool tanLanes = tanWin || (e.mode == kSz && e.laneSAllTwo && P >= kHessLaneMin);
     const int nch = (P + kLanes - 1) / kLanes;
     const int TS = std::max(1, e.numCU * 4);  // partial rows per operator of the lane tangent steps
     const size_t L = (size_t)nch * N * kLanes;
@@ -4113,10 +4115,8 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
         double *psi = e.hz_psi2.p, *lam = e.hz_lam2.p;
         HCHK(hipMemcpyAsync(psi, e.d_start.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
         HCHK(hipMemcpyAsync(lam, e.d_lam.p, 8 * N, hipMemcpyDeviceToDevice, e.stream));
-        // window passes (TUPS units in four-orbital register tiles): every
-        // tile of a pass, psi and lambda read from one half of their buffers
-        // and written by chunk 0 to the other
-        const bool tanWin = e.winOn && e.winReg && e.winTu && e.winHess;
+        // window passes: every tile of a pass, psi and lambda read from one
+        // half of their buffers and written by chunk 0 to the other
         double *psiB = psi + N, *lamB = lam + N;
         for (size_t p = 0; tanWin && p < e.wpass.size(); p++) {
             const auto &wp = e.wpass[p];
