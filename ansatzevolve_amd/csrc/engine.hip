@@ -275,6 +275,7 @@ struct EngineImpl {
     // reachable part of each
     int reachOpt = 1;
     bool supDirty = true;
+    bool hessDirty = true;  // the Hessian's tile order (hessian_tiles) must be made again
     bool supCk = false;
     long long supGen = 0;
     std::vector<double> hstart;
@@ -528,7 +529,7 @@ struct EngineImpl {
     // operators run one operator per pass (k_opv).
     bool build_lane_segments(std::string &err)
     {
-        supDirty = true;
+        supDirty = hessDirty = true;
         if (!build_lane_list(laneA, maxGroups, err) || !build_lane_list(laneF, maxGroupsF, err)) return false;
         sweepS = laneSAllTwo = false;
         laneS.segs.clear();
@@ -770,6 +771,7 @@ struct EngineImpl {
         int vRecOff = 0, vTaskOff = 0, vTasks = 0;
         int gridVF = 1, gridVA = 1;
         int allOff = 0;  // every tile in d_wall (Hessian tangent passes, k_wint)
+        int nReach = 0;  // ... the first nReach of them reachable from the start state
         int gridT = 1;
     };
     // kOptWindow: window width in spatial orbitals (0: off; -1, the default:
@@ -1284,7 +1286,49 @@ struct EngineImpl {
         HCHK(hipMemcpy(d_lrows.p, lrows.data(), M * sizeof(int), hipMemcpyHostToDevice));
         lslots = std::max(lslots, wslots);
         winOn = true;
-        supDirty = true;
+        supDirty = hessDirty = true;
+        return true;
+    }
+
+    // The Hessian's tangent passes (k_wint): per pass every tile, the ones
+    // the start state and the passes before can reach first (wp.nReach of
+    // them), as compute_support_win finds them; option 18 = 0 counts every
+    // tile as reached
+    bool hessian_tiles(std::string &err)
+    {
+        if (!hessDirty) return true;
+        const bool lists_on = reachOpt && (long long)hstart.size() == N;
+        std::vector<uint8_t> S(N, (uint8_t)1);
+        if (lists_on)
+            for (long long i = 0; i < N; i++) S[i] = hstart[i] != 0.0;
+        std::vector<int> tl, rest;
+        std::vector<long long> mem;
+        for (WinPass &wp : wpass) {
+            wp.allOff = (int)tl.size();
+            rest.clear();
+            const uint32_t *rA = hrows.data() + wp.rowsOff, *rB = rA + wp.nrowsA;
+            for (int t : wp.allTiles) {
+                const int ia = t / wp.nBcls, ib = t - ia * wp.nBcls;
+                mem.clear();
+                bool any = false;
+                for (int i = 0; i < wp.cntA[ia]; i++)
+                    for (int j = 0; j < wp.cntB[ib]; j++) {
+                        const long long d = (long long)rA[wp.startA[ia] + i] + rB[wp.startB[ib] + j];
+                        mem.push_back(d);
+                        any = any || S[d];
+                    }
+                if (!any) {
+                    rest.push_back(t);
+                    continue;
+                }
+                tl.push_back(t);
+                for (long long d : mem) S[d] = 1;
+            }
+            wp.nReach = (int)tl.size() - wp.allOff;
+            tl.insert(tl.end(), rest.begin(), rest.end());
+        }
+        if (!upload_vec(d_wall, tl, err)) return false;
+        hessDirty = false;
         return true;
     }
 
@@ -2858,7 +2902,7 @@ bool Engine::set_start(const std::vector<double> &start, std::string &err)
     if (e.cplx) HCHK(hipMemset(e.d_start.p + e.N, 0, 8 * e.N));
     e.haveStart = true;
     e.hstart = start;
-    e.supDirty = true;
+    e.supDirty = e.hessDirty = true;
     return true;
 }
 
@@ -2900,7 +2944,7 @@ bool Engine::set_start_complex(const std::vector<double> &re, const std::vector<
     HCHK(hipMemcpy(e.d_start.p + e.N, im.data(), 8 * e.N, hipMemcpyHostToDevice));
     e.haveStart = true;
     e.hstart = re;
-    e.supDirty = true;
+    e.supDirty = e.hessDirty = true;
     return true;
 }
 
@@ -3750,7 +3794,7 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         if (value == e.reachOpt) return true;
         e.reachOpt = value;
-        e.supDirty = true;
+        e.supDirty = e.hessDirty = true;
         e.bnvec = 0;  // lists and stored states are renewed at the next upload
         e.drop_graphs();
         return true;
@@ -4118,6 +4162,7 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
         // window passes: every tile of a pass, psi and lambda read from one
         // half of their buffers and written by chunk 0 to the other
         double *psiB = psi + N, *lamB = lam + N;
+        if (tanWin && !e.hessian_tiles(err)) return false;
         for (size_t p = 0; tanWin && p < e.wpass.size(); p++) {
             const auto &wp = e.wpass[p];
             WinArgs a = e.winargs(wp);
@@ -4126,7 +4171,7 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
             const size_t lds = (size_t)2 * wp.nkp * (kLanes + 1) * sizeof(double);
             hipLaunchKernelGGL(k_wint, dim3((unsigned)std::min(wp.gridT, TS), nch), dim3(2 * kLanes), lds, e.stream, a,
                                psi, lam, psiB, lamB, e.hz_lx.p, (long long)N * kLanes, (const double2 *)e.d_cs.p, M,
-                               e.hz_inj.p, e.hz_scl.p, e.hz_xpart.p, e.hz_dpart.p, TS);
+                               e.hz_inj.p, e.hz_scl.p, e.hz_xpart.p, e.hz_dpart.p, TS, wp.nReach);
             LCHK();
             std::swap(psi, psiB);
             std::swap(lam, lamB);

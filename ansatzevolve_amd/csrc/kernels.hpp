@@ -618,12 +618,14 @@ __global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, co
 // X[lane][k] = <lambda|G_k|B_lane> and D[k] = <lambda|G_k G_k|psi> are summed
 // per block into xpart[((c * M + k) * slots + block) * 64 + lane] and (chunk
 // 0) dpart[k * slots + block].  Chunk 0 writes psi and lambda after the pass
-// to psiOut/lamOut.  Every tile (a.tiles lists them all); 2 waves per block,
-// LDS 2 x nkp x 65 doubles.
+// to psiOut/lamOut.  a.tiles lists every tile of the pass, the nreach that
+// psi can have reached first; on the others psi and the tangents are zero
+// and stay zero, so chunk 0 carries lambda there and writes psi's zeros.
+// 2 waves per block, LDS 2 x nkp x 65 doubles.
 __global__ void k_wint(WinArgs a, const double *psiIn, const double *lamIn, double *psiOut, double *lamOut, double *B,
                        long long cstride, const double2 *__restrict__ cs, int M, const int *__restrict__ inj,
                        const double *__restrict__ scale, double *__restrict__ xpart, double *__restrict__ dpart,
-                       int slots);
+                       int slots, int nreach);
 
 // Hessian tangent step for operator k (see Engine::hessian).
 template <int MU>

@@ -851,6 +851,76 @@ __device__ __forceinline__ void op_tan(double C, double S, double scs, bool inj,
     });
 }
 
+// lambda alone (a tile where psi and the tangents are zero)
+template <int CA, int CB, int KIND, int NM>
+__device__ __forceinline__ void op_lam(double C, double S, double (&lm)[NM])
+{
+    constexpr int NB = C4[CB];
+    constexpr int NSA = nsub(CA, 0), NSB = nsub(CB, 0);
+    constexpr int PP[6] = {0, 0, 0, 1, 1, 2}, QQ[6] = {1, 2, 3, 2, 3, 3};
+    wfor<0, NSA * NSB>([&](auto I) {
+        constexpr Sub A = sub(CA, 0, decltype(I)::value / NSB);
+        constexpr Sub Bs = sub(CB, 0, decltype(I)::value % NSB);
+        constexpr unsigned slots = tups_slots(KIND, A.sh, Bs.sh);
+        wfor<0, 6>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if constexpr ((slots >> r) & 1u) {
+                constexpr int p = PP[r], q = QQ[r];
+                constexpr int mp = ((p >> 1) ? A.i1 : A.i0) * NB + ((p & 1) ? Bs.i1 : Bs.i0);
+                constexpr int mq = ((q >> 1) ? A.i1 : A.i0) * NB + ((q & 1) ? Bs.i1 : Bs.i0);
+                rot(lm[mp], lm[mq], C, S);
+            }
+        });
+    });
+}
+
+// A tile of the pass that psi cannot have reached: psi and the tangents are
+// zero on it before and after, so only lambda is carried (chunk 0, written
+// by lane 0) and psi's zeros are written to psiOut.
+template <int CA, int CB>
+__device__ __forceinline__ void tile_l(const WinArgs &a, const TileCtx &t, const double *lamIn, double *psiOut,
+                                       double *lamOut, const double2 *__restrict__ cs)
+{
+    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const uint32_t *Bl = a.blob;
+    double lm[NM], none[1];
+    uint32_t colS[NB];
+    wfor<0, NB>([&](auto J) { colS[decltype(J)::value] = ld(a.rows, t.sB + decltype(J)::value) * 8u; });
+    wfor<0, NA>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const double *rl = lamIn + (long long)ld(a.rows, t.sA + i);
+        wfor<0, NB>([&](auto J) {
+            lm[i * NB + decltype(J)::value] = *(const double *)((const char *)rl + colS[decltype(J)::value]);
+        });
+    });
+    int r = 0;
+    for (int p = 0; p < a.nph; p++) {
+        const int lp = __builtin_ctz(ld(Bl, p * 8 + 4)) - a.w0;
+        to_layout<CA, CB, false>((lp - r) & 3, lm, none);
+        r = lp;
+        const int k0 = (int)ld(Bl, p * 8 + 0);
+        wfor<0, 5>([&](auto J) {
+            constexpr int kk = decltype(J)::value;
+            const double2 cst = cs[k0 + kk];
+            const uint32_t sg = ld(Bl, a.oProgT + (p * kOrbMaxOps + kk) * 16 + 9);
+            op_lam<CA, CB, kTupsKind[kk]>(cst.x, flip(cst.y, sg), lm);
+        });
+    }
+    to_layout<CA, CB, false>((0 - r) & 3, lm, none);
+    if (lane == 0)
+        wfor<0, NA>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            const long long ra = ld(a.rows, t.sA + i);
+            double *rp = psiOut + ra, *rl = lamOut + ra;
+            wfor<0, NB>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                *(double *)((char *)rp + colS[j]) = 0.0;
+                *(double *)((char *)rl + colS[j]) = lm[i * NB + j];
+            });
+        });
+}
+
 // A tile of the tangent pass: lane = tangent (chunk c), psi and lambda one
 // vector each (every lane holds the same values).  bx: this wave's bracket
 // rows (nkp x 64), bd: its nkp D sums.
@@ -936,7 +1006,8 @@ __device__ __forceinline__ void tile_t(const WinArgs &a, const TileCtx &t, const
 __global__ void __launch_bounds__(2 * kLanes, 2)
     k_wint(WinArgs a, const double *psiIn, const double *lamIn, double *psiOut, double *lamOut, double *B,
            long long cstride, const double2 *__restrict__ cs, int M, const int *__restrict__ inj,
-           const double *__restrict__ scale, double *__restrict__ xpart, double *__restrict__ dpart, int slots)
+           const double *__restrict__ scale, double *__restrict__ xpart, double *__restrict__ dpart, int slots,
+           int nreach)
 {
     constexpr int NW = 2;
     extern __shared__ __align__(16) double lds[];
@@ -948,7 +1019,10 @@ __global__ void __launch_bounds__(2 * kLanes, 2)
     for (int e = lane; e < rowsW; e += kLanes) bx[e] = 0.0;
     B += (long long)c * cstride;
     const uint32_t *Bl = a.blob;
-    for (int it = blockIdx.x * NW + wv; it < a.ntiles; it += gridDim.x * NW) {
+    // the first nreach listed tiles are the ones psi can have reached; the
+    // rest carry lambda only (chunk 0)
+    const int nt = c == 0 ? a.ntiles : nreach;
+    for (int it = blockIdx.x * NW + wv; it < nt; it += gridDim.x * NW) {
         const int tl = __builtin_amdgcn_readfirstlane(a.tiles[it]);
         const int ja = tl / a.nBcls, jb = tl - ja * a.nBcls;
         const uint32_t fa = wr::ld(Bl, a.oCls + ja), fb = wr::ld(Bl, a.oCls + a.nAcls + jb);
@@ -957,10 +1031,11 @@ __global__ void __launch_bounds__(2 * kLanes, 2)
         t.bOut = fb & 0xffffffu;
         t.sA = (int)wr::ld(Bl, a.oCStart + ja);
         t.sB = a.nrowsA + (int)wr::ld(Bl, a.oCStart + a.nAcls + jb);
-        const int cls = (int)(fa >> 24) * 5 + (int)(fb >> 24);
+        const int cls = (int)(fa >> 24) * 5 + (int)(fb >> 24) + (it < nreach ? 0 : 25);
         switch (cls) {
 #define AE_WT_CASE(CA, CB) \
-    case CA * 5 + CB: wr::tile_t<CA, CB>(a, t, psiIn, lamIn, psiOut, lamOut, B, cs, inj, scale, c, bx, bd); break;
+    case CA * 5 + CB: wr::tile_t<CA, CB>(a, t, psiIn, lamIn, psiOut, lamOut, B, cs, inj, scale, c, bx, bd); break; \
+    case 25 + CA * 5 + CB: wr::tile_l<CA, CB>(a, t, lamIn, psiOut, lamOut, cs); break;
             AE_WT_CASE(0, 0) AE_WT_CASE(0, 1) AE_WT_CASE(0, 2) AE_WT_CASE(0, 3) AE_WT_CASE(0, 4)
             AE_WT_CASE(1, 0) AE_WT_CASE(1, 1) AE_WT_CASE(1, 2) AE_WT_CASE(1, 3) AE_WT_CASE(1, 4)
             AE_WT_CASE(2, 0) AE_WT_CASE(2, 1) AE_WT_CASE(2, 2) AE_WT_CASE(2, 3) AE_WT_CASE(2, 4)
