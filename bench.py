@@ -281,10 +281,12 @@ def main():
     a.set_kernel_timing(False)
     single_ms = 1000.0 / single_rate
     # one vector of 853,776 amplitudes is 6.8 MB: it stays in the on-chip
-    # caches, and the sequence is a chain of short dependent launches; its
-    # bound is launch latency, reported as the average time per launch
+    # caches, and the sequence is a chain of short dependent launches (17
+    # window passes per sweep, each of about 900 one-wave class tasks, fewer
+    # than the SIMDs); its bound is latency: the heaviest task's chain of
+    # dependent loads, phases and stores in every launch
     single_bound = {
-        "bound": "launch latency",
+        "bound": "latency",
         "ms_per_egrad": single_ms,
         "sweep_segments_per_sweep": a.get_option(a.OPT_VECTOR_SEGMENTS),
         "timed_intervals_per_egrad": single_launches / nsingle,
