@@ -770,6 +770,10 @@ struct EngineImpl {
         // d_wvrec (words from vRecOff)
         int vRecOff = 0, vTaskOff = 0, vTasks = 0;
         int gridVF = 1, gridVA = 1;
+        // staged alpha rows (k_winvs): beta-side tasks and their record
+        // blocks (d_wvsrec, words from vsRecOff), waves per block, grids, LDS
+        int vsRecOff = 0, vsTaskOff = 0, vsTasks = 0, vsWavesF = 1, vsWavesA = 1, gridSF = 1, gridSA = 1;
+        size_t vsLdsF = 0, vsLdsA = 0;
         int allOff = 0;  // every tile in d_wall (Hessian tangent passes, k_wint)
         int nReach = 0;  // ... the first nReach of them reachable from the start state
         int gridT = 1;
@@ -789,7 +793,10 @@ struct EngineImpl {
     DBuf<uint32_t> d_wblob, d_wrows;
     std::vector<uint32_t> hrows;  // host copy of d_wrows
     DBuf<int> d_wkp, d_wtiles, d_wall;
-    DBuf<uint32_t> d_wvrec;
+    DBuf<uint32_t> d_wvrec, d_wvsrec;
+    DBuf<int4> d_wvstasks;
+    bool winVsOK = false;  // every pass fits k_winvs's LDS
+    bool winVs = true;     // kOptWindowStaged
     DBuf<int4> d_wvtasks;
     // kOptWindowVector: window passes for the vector-major sweeps (-1, the
     // default: with register tiles from 2^18 determinants; 0 off; 1 with
@@ -822,6 +829,12 @@ struct EngineImpl {
                   : (bc ? (const void *)k_winr<false, true, 4, false> : (const void *)k_winr<false, false, 4, false>);
     }
     int win_waves(bool br) const { return winReg ? (br ? 2 : 4) : winWaves; }
+    const void *winvs_fn(bool br) const
+    {
+        return winTu ? (br ? (const void *)k_winvs<true, true> : (const void *)k_winvs<false, true>)
+                     : (br ? (const void *)k_winvs<true, false> : (const void *)k_winvs<false, false>);
+    }
+    bool use_vs() const { return winVs && winVsOK; }
     const void *winv_fn(bool br) const
     {
         return winTu ? (br ? (const void *)k_winv<true, true> : (const void *)k_winv<false, true>)
@@ -835,6 +848,19 @@ struct EngineImpl {
         const int mk = mark_begin(1);
         for (const WinPass &wp : wpass) {
             const WinArgs a = winargs(wp);
+            if (use_vs()) {
+                const dim3 grid((unsigned)wp.gridSF, nvec), blk(wp.vsWavesF * kLanes);
+                if (winTu)
+                    hipLaunchKernelGGL((k_winvs<false, true>), grid, blk, wp.vsLdsF, stream, a,
+                                       d_wvstasks.p + wp.vsTaskOff, wp.vsTasks, d_wvsrec.p + wp.vsRecOff, nb, v, nullptr,
+                                       N, cs, M, nullptr, 0);
+                else
+                    hipLaunchKernelGGL((k_winvs<false, false>), grid, blk, wp.vsLdsF, stream, a,
+                                       d_wvstasks.p + wp.vsTaskOff, wp.vsTasks, d_wvsrec.p + wp.vsRecOff, nb, v, nullptr,
+                                       N, cs, M, nullptr, 0);
+                LCHK();
+                continue;
+            }
             const dim3 grid((unsigned)wp.gridVF, nvec);
             if (winTu)
                 hipLaunchKernelGGL((k_winv<false, true>), grid, dim3(4 * kLanes), 0, stream, a,
@@ -858,6 +884,19 @@ struct EngineImpl {
         for (int p = (int)wpass.size() - 1; p >= 0; p--) {
             const WinPass &wp = wpass[p];
             const WinArgs a = winargs(wp);
+            if (use_vs()) {
+                const dim3 grid((unsigned)wp.gridSA, nvec), blk(wp.vsWavesA * kLanes);
+                if (winTu)
+                    hipLaunchKernelGGL((k_winvs<true, true>), grid, blk, wp.vsLdsA, stream, a,
+                                       d_wvstasks.p + wp.vsTaskOff, wp.vsTasks, d_wvsrec.p + wp.vsRecOff, nb, phi, lam,
+                                       N, csneg, M, partial, slots);
+                else
+                    hipLaunchKernelGGL((k_winvs<true, false>), grid, blk, wp.vsLdsA, stream, a,
+                                       d_wvstasks.p + wp.vsTaskOff, wp.vsTasks, d_wvsrec.p + wp.vsRecOff, nb, phi, lam,
+                                       N, csneg, M, partial, slots);
+                LCHK();
+                continue;
+            }
             const size_t lds = (size_t)4 * wp.nkp * kLanes * sizeof(double);
             const dim3 grid((unsigned)wp.gridVA, nvec);
             if (winTu)
@@ -1219,6 +1258,44 @@ struct EngineImpl {
                 wp.vTasks = (int)tasks.size() - wp.vTaskOff;
             }
             if (!upload_vec(d_wvrec, rec, err) || !upload_vec(d_wvtasks, tasks, err)) return false;
+            // staged alpha rows: the beta outer classes by inner count, up to
+            // 64 per task; record block per task: NB beta rows, then the beta
+            // outer bits, (NB + 1) x 64 words
+            std::vector<uint32_t> srec;
+            std::vector<int4> stasks;
+            winVsOK = true;
+            for (WinPass &wp : wpass) {
+                wp.vsRecOff = (int)srec.size();
+                wp.vsTaskOff = (int)stasks.size();
+                const uint32_t *rB = hrows.data() + wp.rowsOff + wp.nrowsA;
+                for (int cb = 0; cb <= 4; cb++) {
+                    std::vector<int> jbs;
+                    for (int jb = 0; jb < wp.nBcls; jb++)
+                        if (wp.incB[jb] == cb) jbs.push_back(jb);
+                    for (size_t i = 0; i < jbs.size(); i += 64) {
+                        const int cnt = (int)std::min<size_t>(64, jbs.size() - i);
+                        const int NB = wp.cntB[jbs[i]];
+                        const size_t off = srec.size();
+                        stasks.push_back(make_int4(cb, (int)(off - wp.vsRecOff), cnt, 0));
+                        srec.resize(off + (size_t)(NB + 1) * kLanes, 0u);
+                        for (int ln = 0; ln < cnt; ln++) {
+                            const int jb = jbs[i + ln];
+                            for (int q = 0; q < NB; q++) srec[off + (size_t)q * kLanes + ln] = rB[wp.startB[jb] + q];
+                            srec[off + (size_t)NB * kLanes + ln] = wp.outB[jb];
+                        }
+                    }
+                }
+                wp.vsTasks = (int)stasks.size() - wp.vsTaskOff;
+                wp.vsWavesF = std::max(1, std::min(16, wp.vsTasks));
+                wp.vsLdsF = (size_t)kWinVsRows * nb * sizeof(double);
+                // the adjoint: as many waves (at most 8) as leave room for their bracket rows
+                const size_t perWave = (size_t)wp.nkp * kLanes * sizeof(double);
+                const long long room = (long long)ldsDevice - 2 * (long long)wp.vsLdsF;
+                wp.vsWavesA = (int)std::min<long long>(std::min(8, wp.vsTasks), room > 0 ? room / (long long)perWave : 0);
+                wp.vsLdsA = 2 * wp.vsLdsF + (size_t)std::max(1, wp.vsWavesA) * perWave;
+                winVsOK = winVsOK && wp.vsWavesA >= 1 && wp.vsLdsA <= ldsDevice;
+            }
+            if (!upload_vec(d_wvsrec, srec, err) || !upload_vec(d_wvstasks, stasks, err)) return false;
         }
         // grids: as many blocks as the chip holds at once
         std::vector<int> lrows(M, 0);
@@ -1276,6 +1353,20 @@ struct EngineImpl {
                     long long g = std::min<long long>((long long)nbk * numCU, (wp.vTasks + 3) / 4);
                     if (br) g = std::min<long long>(g, slots);  // one partial row per block
                     (br ? wp.gridVA : wp.gridVF) = (int)std::max<long long>(1, g);
+                }
+        if (winReg && winVsOK)
+            for (WinPass &wp : wpass)
+                for (int br = 0; br < 2; br++) {
+                    const void *fn = winvs_fn(br == 1);
+                    const size_t lds = br ? wp.vsLdsA : wp.vsLdsF;
+                    const int nw = br ? wp.vsWavesA : wp.vsWavesF;
+                    HCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    int nbk = 0;
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, fn, nw * kLanes, lds) != hipSuccess || nbk < 1)
+                        nbk = 1;
+                    long long g = std::min<long long>((long long)nbk * numCU, wp.nAcls);
+                    if (br) g = std::min<long long>(g, slots);
+                    (br ? wp.gridSA : wp.gridSF) = (int)std::max<long long>(1, g);
                 }
         for (int k = 0; k < M; k++)
             if (lrows[k] == 0) {
@@ -3823,6 +3914,16 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         e.bnvec = 0;  // stored states and lists are renewed at the next upload
         return true;
+    case kOptWindowStaged:
+        if (value < 0 || value > 1) {
+            err = "staged window option must be 0 or 1";
+            return false;
+        }
+        if ((value == 1) != e.winVs) {
+            e.winVs = value == 1;
+            e.drop_graphs();
+        }
+        return true;
     case kOptWindowHessian:
         if (value < 0 || value > 1) {
             err = "window Hessian option must be 0 or 1";
@@ -3905,6 +4006,7 @@ int Engine::option(int key) const
     case kOptWindowVector: return e.winVec;
     case kOptWindowEnergy: return e.winFuseE ? 1 : 0;
     case kOptWindowHessian: return e.winHess ? 1 : 0;
+    case kOptWindowStaged: return e.use_vs() ? 1 : (e.winVs ? 2 : 0);
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }

@@ -76,6 +76,9 @@ enum EngineOption : int {
                               // in the first adjoint pass instead of a separate dot; 0: the separate dot
     kOptWindowHessian = 31,   // 1 (default): the Hessian's tangent sweep runs as window passes when the window
                               // passes hold TUPS units in registers (key 28 = 2); 0: two-group segments
+    kOptWindowStaged = 32,    // 1 (default): the vector-major window passes stage each alpha class's rows in LDS
+                              // (k_winvs) where they fit; 0: one tile per lane straight from memory (k_winv).
+                              // Read back: 1 in use, 2 requested but the rows do not fit, 0 off
 };
 
 class Engine {

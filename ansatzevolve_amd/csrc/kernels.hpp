@@ -610,6 +610,23 @@ __global__ void k_winv(WinArgs a, const int4 *__restrict__ tasks, int ntasks, co
                        double *v, double *lam, long long vstride, const double2 *__restrict__ cs, int M,
                        double *__restrict__ partial, int slots);
 
+// Window passes in the vector-major layout with staged alpha rows: a block
+// takes one alpha outer class at a time (grid-stride), copies its alpha rows
+// of the vector (at most kWinVsRows of nb doubles; the adjoint also
+// lambda's) to LDS with coalesced loads, and each of its waves takes tasks
+// {beta inner count, record block offset, count <= 64} of beta outer
+// classes (lane = tile), whose record block holds per lane word q at
+// [q * 64 + lane]: NB beta rows (beta ranks), then the beta outer bits; the
+// tile's members are read from and written back to the staged rows, which
+// go back to the vector after the class.  Brackets as k_winv (per block:
+// partial[(vector * M + k) * slots + block]).  LDS: kWinVsRows x nb doubles
+// (twice for the adjoint) + the adjoint's waves x nkp x 64 doubles.
+constexpr int kWinVsRows = 6;
+template <bool BR, bool TU>
+__global__ void k_winvs(WinArgs a, const int4 *__restrict__ tasks, int ntasks, const uint32_t *__restrict__ rec,
+                        int nb, double *v, double *lam, long long vstride, const double2 *__restrict__ cs, int M,
+                        double *__restrict__ partial, int slots);
+
 // Hessian tangent pass over four-orbital windows whose phases are TUPS
 // units: lane = tangent (64 per chunk, grid.y = chunks, in place in B, lane
 // layout), the tile's psi and lambda (one vector each, psiIn/lamIn) in

@@ -610,30 +610,16 @@ __device__ __forceinline__ void tile(const WinArgs &a, const TileCtx &t, const d
 }
 
 // ------------------------------------------ vector-major (k_winv)
-// One vector per grid row (vector-major layout, vstride doubles apart);
-// a wave takes a task of up to 64 tiles of one class (inner counts), lane
-// = tile: the member indices are compile-time as above, the addresses and
-// the outer-bit signs per lane, angles and program words uniform.  rec: the
-// task's record block, word q of lane l at rec[q * 64 + l]: NA alpha rows,
-// NB beta rows, the alpha and the beta outer bits.
+// Every phase of the pass on a tile held in registers, lane = tile (its
+// outer bits in outer), angles uniform; the adjoint adds each operator's
+// bracket to this lane's row in brk.
 template <int CA, int CB, bool BR, bool TU>
-__device__ __forceinline__ void tile_v(const WinArgs &a, const uint32_t *__restrict__ rec, bool valid, double *v,
-                                       double *lam, const double2 *__restrict__ cs, double *brk)
+__device__ __forceinline__ void phases_v(const WinArgs &a, uint32_t outer, double (&x)[C4[CA] * C4[CB]],
+                                         double (&l)[BR ? C4[CA] * C4[CB] : 1], const double2 *__restrict__ cs,
+                                         double *brk)
 {
-    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
     const int lane = threadIdx.x & (kLanes - 1);
     const uint32_t *B = a.blob;
-    uint32_t ra[NA], rb[NB];
-    double x[NM], l[BR ? NM : 1];
-    wfor<0, NA>([&](auto I) { ra[decltype(I)::value] = rec[decltype(I)::value * kLanes + lane]; });
-    wfor<0, NB>([&](auto I) { rb[decltype(I)::value] = rec[(NA + decltype(I)::value) * kLanes + lane]; });
-    const uint32_t outer = TU ? 0u : (rec[(NA + NB) * kLanes + lane] << a.n) | rec[(NA + NB + 1) * kLanes + lane];
-    wfor<0, NM>([&](auto I) {
-        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
-        const long long g = (long long)ra[i] + rb[j];
-        x[m] = valid ? v[g] : 0.0;
-        if constexpr (BR) l[m] = valid ? lam[g] : 0.0;
-    });
     int r = 0;
     for (int pi = 0; pi < a.nph; pi++) {
         const int p = BR ? a.nph - 1 - pi : pi;
@@ -672,6 +658,32 @@ __device__ __forceinline__ void tile_v(const WinArgs &a, const uint32_t *__restr
         }
     }
     to_layout<CA, CB, BR>((0 - r) & 3, x, l);
+}
+
+// One vector per grid row (vector-major layout, vstride doubles apart);
+// a wave takes a task of up to 64 tiles of one class (inner counts), lane
+// = tile: the member indices are compile-time as above, the addresses and
+// the outer-bit signs per lane, angles and program words uniform.  rec: the
+// task's record block, word q of lane l at rec[q * 64 + l]: NA alpha rows,
+// NB beta rows, the alpha and the beta outer bits.
+template <int CA, int CB, bool BR, bool TU>
+__device__ __forceinline__ void tile_v(const WinArgs &a, const uint32_t *__restrict__ rec, bool valid, double *v,
+                                       double *lam, const double2 *__restrict__ cs, double *brk)
+{
+    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const int lane = threadIdx.x & (kLanes - 1);
+    uint32_t ra[NA], rb[NB];
+    double x[NM], l[BR ? NM : 1];
+    wfor<0, NA>([&](auto I) { ra[decltype(I)::value] = rec[decltype(I)::value * kLanes + lane]; });
+    wfor<0, NB>([&](auto I) { rb[decltype(I)::value] = rec[(NA + decltype(I)::value) * kLanes + lane]; });
+    const uint32_t outer = TU ? 0u : (rec[(NA + NB) * kLanes + lane] << a.n) | rec[(NA + NB + 1) * kLanes + lane];
+    wfor<0, NM>([&](auto I) {
+        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
+        const long long g = (long long)ra[i] + rb[j];
+        x[m] = valid ? v[g] : 0.0;
+        if constexpr (BR) l[m] = valid ? lam[g] : 0.0;
+    });
+    phases_v<CA, CB, BR, TU>(a, outer, x, l, cs, brk);
     wfor<0, NM>([&](auto I) {
         constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
         const long long g = (long long)ra[i] + rb[j];
@@ -730,6 +742,123 @@ __global__ void __launch_bounds__(4 * kLanes, BR ? 2 : 4)
         }
     }
 }
+
+// ------------------------------- vector-major, staged alpha rows (k_winvs)
+namespace wr {
+// A tile whose members are read from the block's staged alpha rows: member
+// (i, j) at xs[i * nb + beta row j]; lane = tile of the wave's task, whose
+// record block holds per lane NB beta rows (beta ranks) and the beta outer
+// bits.
+template <int CA, int CB, bool BR, bool TU>
+__device__ __forceinline__ void tile_s(const WinArgs &a, const uint32_t *__restrict__ rec, bool valid,
+                                       uint32_t outerA, double *xs, double *ls, int nb,
+                                       const double2 *__restrict__ cs, double *brk)
+{
+    constexpr int NA = C4[CA], NB = C4[CB], NM = NA * NB;
+    const int lane = threadIdx.x & (kLanes - 1);
+    uint32_t rb[NB];
+    double x[NM], l[BR ? NM : 1];
+    wfor<0, NB>([&](auto J) { rb[decltype(J)::value] = rec[decltype(J)::value * kLanes + lane]; });
+    const uint32_t outer = TU ? 0u : (outerA << a.n) | rec[NB * kLanes + lane];
+    wfor<0, NM>([&](auto I) {
+        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
+        const int idx = i * nb + (int)rb[j];
+        x[m] = valid ? xs[idx] : 0.0;
+        if constexpr (BR) l[m] = valid ? ls[idx] : 0.0;
+    });
+    phases_v<CA, CB, BR, TU>(a, outer, x, l, cs, brk);
+    wfor<0, NM>([&](auto I) {
+        constexpr int m = decltype(I)::value, i = m / NB, j = m % NB;
+        const int idx = i * nb + (int)rb[j];
+        if (valid) {
+            xs[idx] = x[m];
+            if constexpr (BR) ls[idx] = l[m];
+        }
+    });
+}
+}  // namespace wr
+
+template <bool BR, bool TU>
+__global__ void __launch_bounds__((BR ? 8 : 16) * kLanes)
+    k_winvs(WinArgs a, const int4 *__restrict__ tasks, int ntasks, const uint32_t *__restrict__ rec, int nb,
+            double *v, double *lam, long long vstride, const double2 *__restrict__ cs, int M,
+            double *__restrict__ partial, int slots)
+{
+    extern __shared__ __align__(16) double lds[];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int nw = blockDim.x / kLanes;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    const int vec = blockIdx.y;
+    double *xs = lds, *ls = xs + (size_t)kWinVsRows * nb;
+    double *brk = (BR ? ls + (size_t)kWinVsRows * nb : ls) + (size_t)wv * a.nkp * kLanes;
+    if (BR)
+        for (int e = 0; e < a.nkp; e++) brk[e * kLanes + lane] = 0.0;
+    v += (long long)vec * vstride;
+    if (BR) lam += (long long)vec * vstride;
+    cs += (long long)vec * M;
+    const uint32_t *Bl = a.blob;
+    for (int ja = blockIdx.x; ja < a.nAcls; ja += gridDim.x) {
+        const uint32_t fa = wr::ld(Bl, a.oCls + ja);
+        const int ca = (int)(fa >> 24), na = ca == 0 || ca == 4 ? 1 : ca == 2 ? 6 : 4;
+        const int sA = (int)wr::ld(Bl, a.oCStart + ja);
+        __syncthreads();  // the previous class's rows are written back
+        for (int i = 0; i < na; i++) {
+            const long long row = wr::ld(a.rows, sA + i);
+            for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+                xs[i * nb + b] = v[row + b];
+                if (BR) ls[i * nb + b] = lam[row + b];
+            }
+        }
+        __syncthreads();
+        for (int it = wv; it < ntasks; it += nw) {
+            const int4 tk = tasks[it];  // {beta inner count, record block offset, count}
+            const int cb = __builtin_amdgcn_readfirstlane(tk.x), off = __builtin_amdgcn_readfirstlane(tk.y),
+                      cnt = __builtin_amdgcn_readfirstlane(tk.z);
+            const bool valid = lane < cnt;
+            switch (ca * 5 + cb) {
+#define AE_WS_CASE(CA, CB)                                                                                          \
+    case CA * 5 + CB:                                                                                               \
+        wr::tile_s<CA, CB, BR, TU>(a, rec + off, valid, fa & 0xffffffu, xs, ls, nb, cs, brk);                       \
+        break;
+                AE_WS_CASE(0, 0) AE_WS_CASE(0, 1) AE_WS_CASE(0, 2) AE_WS_CASE(0, 3) AE_WS_CASE(0, 4)
+                AE_WS_CASE(1, 0) AE_WS_CASE(1, 1) AE_WS_CASE(1, 2) AE_WS_CASE(1, 3) AE_WS_CASE(1, 4)
+                AE_WS_CASE(2, 0) AE_WS_CASE(2, 1) AE_WS_CASE(2, 2) AE_WS_CASE(2, 3) AE_WS_CASE(2, 4)
+                AE_WS_CASE(3, 0) AE_WS_CASE(3, 1) AE_WS_CASE(3, 2) AE_WS_CASE(3, 3) AE_WS_CASE(3, 4)
+                AE_WS_CASE(4, 0) AE_WS_CASE(4, 1) AE_WS_CASE(4, 2) AE_WS_CASE(4, 3) AE_WS_CASE(4, 4)
+#undef AE_WS_CASE
+            default: break;
+            }
+        }
+        __syncthreads();
+        for (int i = 0; i < na; i++) {
+            const long long row = wr::ld(a.rows, sA + i);
+            for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+                v[row + b] = xs[i * nb + b];
+                if (BR) lam[row + b] = ls[i * nb + b];
+            }
+        }
+    }
+    if (BR) {
+        __syncthreads();
+        double *rows0 = ls + (size_t)kWinVsRows * nb;
+        for (int kp = wv; kp < a.nkp; kp += nw) {
+            double s = 0;
+            for (int q = 0; q < nw; q++) s += rows0[((size_t)q * a.nkp + kp) * kLanes + lane];
+            s = wave_sum(s);
+            if (lane == 0) partial[((long long)vec * M + a.kpOp[kp]) * slots + blockIdx.x] = s;
+        }
+    }
+}
+
+#define AE_WINVS(BR, TU)                                                                                       \
+    template __global__ void k_winvs<BR, TU>(WinArgs, const int4 *__restrict__, int, const uint32_t *__restrict__, \
+                                             int, double *, double *, long long, const double2 *__restrict__, int, \
+                                             double *__restrict__, int);
+AE_WINVS(false, false)
+AE_WINVS(true, false)
+AE_WINVS(false, true)
+AE_WINVS(true, true)
+#undef AE_WINVS
 
 #define AE_WINV(BR, TU)                                                                                      \
     template __global__ void k_winv<BR, TU>(WinArgs, const int4 *__restrict__, int, const uint32_t *__restrict__, \

@@ -344,7 +344,12 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     both vectors; 0: the separate dot;
  *  31 1 (default): the Hessian's tangent sweep (getHessian_COMP,
  *     getHessianProj) runs as the window passes when they hold TUPS units
- *     in registers (key 28 = 2); 0: the two-group segments.
+ *     in registers (key 28 = 2); 0: the two-group segments;
+ *  32 1 (default): the vector-major window passes (key 29) copy each alpha
+ *     class's rows of the vector to LDS with coalesced loads and take the
+ *     tiles from there, where six rows (twice for the adjoint) fit in LDS;
+ *     0: each tile straight from memory.  Reads back 1 when in use, 2 when
+ *     requested but the rows do not fit, 0 when off.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

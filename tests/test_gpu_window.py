@@ -257,6 +257,14 @@ def test_window_vector_major(gpu, n, layers, variant):
         assert np.max(np.abs(Eb - Eb0)) <= SAME and np.max(np.abs(Gb - Gb0)) <= SAME
         # the energy-only batch path
         assert np.max(np.abs(a.energies(thetas) - Eb0)) <= SAME
+        # alpha rows staged in LDS (option 32, in use here) against one tile per lane from memory
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_STAGED) == 1
+        a.set_option(AnsatzSynth.OPT_WINDOW_STAGED, 0)
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_STAGED) == 0
+        for v, t in enumerate(thetas[:2]):
+            E, g = a.energy_gradient(t)
+            assert abs(E - ref[v][0]) <= SAME and np.max(np.abs(g - ref[v][1])) <= SAME, v
+        a.set_option(AnsatzSynth.OPT_WINDOW_STAGED, 1)
         if n <= 8:
             # the other vector-major callers: Hessian, projected gradient, subspace products
             rng = np.random.default_rng(n + 17)
@@ -286,6 +294,10 @@ def test_window_vector_major_config3(gpu):
         assert a.get_option(AnsatzSynth.OPT_WINDOW_VECTOR) == -1
         assert a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS) == a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 17
         assert abs(E - gz["cfg3_E"][0]) <= E_TOL and np.max(np.abs(g - gz["cfg3_grad"][0])) <= G_TOL
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_STAGED) == 1
+        a.set_option(AnsatzSynth.OPT_WINDOW_STAGED, 0)
+        E1, g1 = a.energy_gradient(w["theta"])
+        assert abs(E - E1) <= SAME and np.max(np.abs(g - g1)) <= 1e-11
         a.set_option(AnsatzSynth.OPT_WINDOW_VECTOR, 0)
         E0, g0 = a.energy_gradient(w["theta"])
         assert abs(E - E0) <= SAME and np.max(np.abs(g - g0)) <= 1e-11
