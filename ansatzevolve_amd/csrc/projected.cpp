@@ -166,6 +166,7 @@ bool ProjectedDriver::optimise(std::vector<double> &x, double &E)
     std::vector<double> prev, dest, g;
     if (!f_.setProjection(x, prev)) return false;  // p = H psi(x); prev = psi(x)
     std::deque<std::vector<double>> hisTheta, hisErr;
+    std::vector<double> xNewton;  // the last Newton result, whose energy E is
     for (int outer = 0; outer < maxOuter; outer++) {
         if (!newton(x)) return false;
         if (!f_.setProjection(x, dest)) return false;
@@ -173,6 +174,7 @@ bool ProjectedDriver::optimise(std::vector<double> &x, double &E)
         if (!f_.gradProj(x, f, g)) return false;
         energyEvals++;
         E = f;  // <H psi|psi> at p = H psi(x)
+        xNewton = x;
         const double gn = vnorm(g);
         trace.push_back({std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(), E, gn,
                          hessianEvals, energyEvals});
@@ -199,6 +201,11 @@ bool ProjectedDriver::optimise(std::vector<double> &x, double &E)
         }
         prev.swap(dest);
     }
+    // a run that ends on the iteration limit after an extrapolation returns
+    // the last Newton result, the point its energy belongs to (the
+    // reference's loop, whose DIIS branch is switched off, would return the
+    // extrapolated angles with the earlier energy)
+    if (!xNewton.empty()) x = xNewton;
     return true;
 }
 
