@@ -1013,6 +1013,88 @@ struct EngineImpl {
             left -= bestL.size();
             plan.push_back({bestW, bestL});
         }
+        // Fewer passes, if there are: breadth first over the sets of applied
+        // phases, a level's successors being what each window position takes
+        // from each of its sets; each level keeps the sets that no other kept
+        // set contains, at most 256, the fullest first (more applied phases
+        // never leave a window less to take).  Stops at one pass short of the
+        // plan above; its path of window positions is replayed.
+        {
+            const size_t NWd = (P + 63) / 64;
+            using Bits = std::vector<uint64_t>;
+            struct Node {
+                Bits d;
+                int parent, w0;
+            };
+            std::vector<std::vector<Node>> lv(1);
+            lv[0].push_back({Bits(NWd, 0), -1, -1});
+            std::vector<char> dc(P);
+            std::vector<int> l;
+            int found = -1;  // index of the completing node in the last level
+            for (size_t depth = 1; depth < plan.size() && found < 0; depth++) {
+                std::map<Bits, std::pair<int, int>> next;  // set -> (parent, window)
+                const std::vector<Node> &cur = lv.back();
+                for (size_t ni = 0; ni < cur.size() && found < 0; ni++) {
+                    for (size_t i = 0; i < P; i++) dc[i] = (char)((cur[ni].d[i >> 6] >> (i & 63)) & 1);
+                    for (int w0 = 0; w0 < nw; w0++) {
+                        pick(dc, w0, l);
+                        if (l.empty()) continue;
+                        Bits d2 = cur[ni].d;
+                        for (int i : l) d2[(size_t)i >> 6] |= 1ull << (i & 63);
+                        next.emplace(d2, std::make_pair((int)ni, w0));
+                    }
+                }
+                std::vector<Node> kept;
+                std::vector<std::pair<int, const Bits *>> order;
+                for (const auto &kv : next) {
+                    int c = 0;
+                    for (uint64_t x : kv.first) c += __builtin_popcountll(x);
+                    order.push_back({c, &kv.first});
+                }
+                std::stable_sort(order.begin(), order.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+                for (const auto &o : order) {
+                    if ((size_t)o.first == P) {  // every phase applied
+                        const auto &pw = next[*o.second];
+                        kept.clear();
+                        kept.push_back({*o.second, pw.first, pw.second});
+                        found = 0;
+                        break;
+                    }
+                    bool dominated = false;
+                    for (const Node &k : kept) {
+                        bool sub = true;
+                        for (size_t q = 0; q < NWd && sub; q++) sub = (o.second->at(q) & ~k.d[q]) == 0;
+                        if (sub) {
+                            dominated = true;
+                            break;
+                        }
+                    }
+                    if (dominated) continue;
+                    const auto &pw = next[*o.second];
+                    kept.push_back({*o.second, pw.first, pw.second});
+                    if (kept.size() >= 256) break;
+                }
+                if (kept.empty()) break;
+                lv.push_back(std::move(kept));
+            }
+            if (found >= 0) {
+                std::vector<int> ws;
+                for (int li = (int)lv.size() - 1, ni = found; li > 0; li--) {
+                    ws.push_back(lv[li][ni].w0);
+                    ni = lv[li][ni].parent;
+                }
+                std::reverse(ws.begin(), ws.end());
+                std::vector<std::pair<int, std::vector<int>>> plan2;
+                std::fill(dc.begin(), dc.end(), (char)0);
+                for (int w0 : ws) {
+                    pick(dc, w0, l);
+                    for (int i : l) dc[i] = 1;
+                    plan2.push_back({w0, l});
+                }
+                if (std::all_of(dc.begin(), dc.end(), [](char c) { return c != 0; }) && plan2.size() < plan.size())
+                    plan.swap(plan2);
+            }
+        }
         winReg = w == 4;
         for (const Ph &p : phs) {
             const uint32_t lo = p.ua & (~p.ua + 1);

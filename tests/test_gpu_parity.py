@@ -174,7 +174,7 @@ def test_config3_benchmarked_batch_path(gpu):
                          (AnsatzSynth.OPT_CHECKPOINT, 1)):
             a.set_option(key, val)
         a.upload_batch(thetas)
-        assert a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 17
+        assert a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 16
         assert a.get_option(AnsatzSynth.OPT_WINDOW_REGISTERS) == 2
         assert a.get_option(AnsatzSynth.OPT_HAM_KIND) == 3
         assert a.get_option(AnsatzSynth.OPT_CHECKPOINT_ACTIVE) == 1

@@ -207,7 +207,7 @@ def test_window_config3_against_reference(gpu):
         E0, G0 = a.energy_gradients(thetas)
         _window(a, 6)
         assert a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 0
-        for width, waves, passes in ((5, 4, 11), (4, 8, 17)):
+        for width, waves, passes in ((5, 4, 11), (4, 8, 16)):
             _window(a, width, waves, 1)
             a.upload_batch(thetas)
             assert a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == passes
@@ -292,7 +292,7 @@ def test_window_vector_major_config3(gpu):
     with from_workload(w) as a:
         E, g = a.energy_gradient(w["theta"])
         assert a.get_option(AnsatzSynth.OPT_WINDOW_VECTOR) == -1
-        assert a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS) == a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 17
+        assert a.get_option(AnsatzSynth.OPT_VECTOR_SEGMENTS) == a.get_option(AnsatzSynth.OPT_WINDOW_PASSES) == 16
         assert abs(E - gz["cfg3_E"][0]) <= E_TOL and np.max(np.abs(g - gz["cfg3_grad"][0])) <= G_TOL
         assert a.get_option(AnsatzSynth.OPT_WINDOW_STAGED) == 1
         a.set_option(AnsatzSynth.OPT_WINDOW_STAGED, 0)
