@@ -341,7 +341,7 @@ class AnsatzSynth:
      OPT_GEMM_SOLUTION_BETA, OPT_GEMM_SOLUTION_ALPHA, OPT_REACHABLE, OPT_SWEEP_AMP_FORWARD,
      OPT_SWEEP_AMP_ADJOINT_LAMBDA, OPT_SWEEP_AMP_ADJOINT_STATE, OPT_SWEEP_TWO_GROUP_AVAILABLE, OPT_SIGMA1_ROWS, OPT_VECTOR_SEGMENTS,
      OPT_WINDOW, OPT_WINDOW_WAVES, OPT_WINDOW_PASSES, OPT_WINDOW_REGISTERS, OPT_WINDOW_VECTOR,
-     OPT_WINDOW_ENERGY, OPT_WINDOW_HESSIAN, OPT_WINDOW_STAGED) = range(33)
+     OPT_WINDOW_ENERGY, OPT_WINDOW_HESSIAN, OPT_WINDOW_STAGED, OPT_SIGMA_EXCHANGE) = range(34)
 
     def set_option(self, key, value):
         self._chk("setEngineOption", self.lib.setEngineOption(int(key), int(value), self.ctx))

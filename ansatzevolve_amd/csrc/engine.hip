@@ -314,11 +314,67 @@ struct EngineImpl {
     {
         return {d_start.p, d_psi.p, d_phi.p, d_lam.p, d_cs.p, d_csneg.p, d_partial.p, d_epart.p, d_out.p};
     }
+    // The mirror image of every operator under the exchange of the alpha
+    // (upper n) and beta (lower n) bits, with its sign mask mirrored and its
+    // constant sign kept: an operator is its own mirror (the paired double)
+    // or sits next to it, the two acting on opposite spins only (so that
+    // they commute).  opsMirror when every operator has one.
+    void find_mirrors()
+    {
+        const int M = (int)hops.size();
+        const uint32_t lo = (n >= 32 ? 0u : (1u << n)) - 1u;
+        auto mir = [&](uint32_t x) { return ((x & lo) << n) | (x >> n); };
+        auto same = [](const DevOp &p, const DevOp &q) {
+            return p.cre == q.cre && p.ann == q.ann && p.mask == q.mask && p.flip == q.flip;
+        };
+        auto one_spin = [&](const DevOp &p) {
+            const uint32_t u = p.cre | p.ann | p.mask;
+            return (u & lo) == 0 || (u >> n) == 0;
+        };
+        mirror.assign(M, -1);
+        opsMirror = !cplx && n > 0 && 2 * n <= 32 && M > 0;
+        for (int k = 0; k < M && opsMirror; k++) {
+            const DevOp &o = hops[k];
+            const DevOp m = {mir(o.cre), mir(o.ann), mir(o.mask), o.flip};
+            if (same(m, o)) mirror[k] = k;
+            else if (one_spin(o) && k + 1 < M && same(m, hops[k + 1])) mirror[k] = k + 1;
+            else if (one_spin(o) && k > 0 && same(m, hops[k - 1])) mirror[k] = k - 1;
+            else opsMirror = false;
+        }
+        for (int k = 0; k < M && opsMirror; k++)
+            if (mirror[mirror[k]] != k) opsMirror = false;
+    }
+
+    // x[a][b] == x[b][a] for the start vector (index a * nb + b)
+    bool symmetric_start() const
+    {
+        if (na != nb || (long long)hstart.size() != (long long)na * nb) return false;
+        for (long long a = 0; a < na; a++)
+            for (long long b = a + 1; b < nb; b++)
+                if (hstart[a * nb + b] != hstart[b * nb + a]) return false;
+        return true;
+    }
+
+    // whether the batch of nvec operator-angle vectors qualifies for the
+    // exchange-symmetric sigma
+    bool batch_symmetric(const double *angles, int nvec) const
+    {
+        if (exchOpt == 0 || !opsMirror || !startSym || !vSame || !sigg || !siggOpt || hkind != 3 || !sigv ||
+            cplx || na != nb)
+            return false;
+        for (int v = 0; v < nvec; v++) {
+            const double *t = angles + (size_t)v * M;
+            for (int k = 0; k < M; k++)
+                if (t[k] != t[mirror[k]]) return false;
+        }
+        return true;
+    }
+
     std::vector<const void *> batch_ptrs() const
     {
         return {b_psi.p,      b_lam.p,   b_partial.p, b_epart.p, b_out.p,  b_cs.p,    b_csneg.p, bl_psi.p,
                 bl_lam.p,     bl_partial.p, bl_epart.p, bl_cs.p,  bl_csneg.p, bl_ck.p, d_suplist.p, d_lrows.p,
-                d_wtiles.p,   d_wblob.p, d_wrows.p,   d_wkp.p,     bl_wep.p};
+                d_wtiles.p,   d_wblob.p, d_wrows.p,   d_wkp.p,     bl_wep.p,  bl_z.p};
     }
 
     bool use_lanes(int nvec) const { return laneMin > 0 && nvec >= laneMin; }
@@ -797,6 +853,19 @@ struct EngineImpl {
     DBuf<int4> d_wvstasks;
     bool winVsOK = false;  // every pass fits k_winvs's LDS
     bool winVs = true;     // kOptWindowStaged
+    // Exchange symmetry of the alpha and beta strings (kOptSigmaExchange).
+    // With the same strings per spin, x[a][b] = x[b][a] for every state of
+    // the batch when the start vector has it and the unitary commutes with
+    // the exchange; then the beta-beta part of the same-spin sigma,
+    // sum_j V[b][j] x[a][j], equals (V x)[b][a], the alpha-alpha DGEMM's
+    // result read transposed.
+    int exchOpt = -1;            // -1 automatic, 0 off
+    std::vector<int> mirror;     // per operator: itself, or the neighbour that is its mirror image
+    bool opsMirror = false;      // every operator has one (set_operators)
+    bool startSym = false;       // the start vector is exchange-symmetric (set_start)
+    bool vSame = false;          // V_aa == V_bb bit for bit (sigma set-up)
+    bool bSym = false;           // the uploaded batch qualifies
+    DBuf<double> bl_z;           // alpha-alpha DGEMM result of the batch (bSym)
     DBuf<int4> d_wvtasks;
     // kOptWindowVector: window passes for the vector-major sweeps (-1, the
     // default: with register tiles from 2^18 determinants; 0 off; 1 with
@@ -2602,10 +2671,12 @@ struct EngineImpl {
     // (0: the library's default).  which 0, beta part: per alpha row,
     // Y_a (nb x 64) += V_bb X_a (column-major view: Y_a^T += X_a^T V_bb^T);
     // which 1, alpha part: per chunk, Y (na x nb*64) += V_aa X.
+    // overwrite: y = V x (beta 0) instead of y += V x
     rocblas_status sigma_gemm(int which, const double *x, double *y, int nch, int sol, bool list = false,
-                              rocblas_int *sols = nullptr, rocblas_int *nsol = nullptr)
+                              rocblas_int *sols = nullptr, rocblas_int *nsol = nullptr, bool overwrite = false)
     {
-        const double one = 1.0;
+        const double one = 1.0, zero = 0.0;
+        const double *beta = overwrite ? &zero : &one;
         const rocblas_datatype f = rocblas_datatype_f64_r;
         const rocblas_gemm_algo algo = (sol != 0 || list) ? rocblas_gemm_algo_solution_index
                                                           : rocblas_gemm_algo_standard;
@@ -2619,7 +2690,7 @@ struct EngineImpl {
                 blas, rocblas_operation_none, rocblas_operation_none, m, nk, nk, &one, x, f, ld, st, V, f, nk, 0, &one,
                 y, f, ld, st, y, f, ld, st, batch, f, algo, 0, sols, nsol);
         return rocblas_gemm_strided_batched_ex(blas, rocblas_operation_none, rocblas_operation_none, m, nk, nk, &one,
-                                               x, f, ld, st, V, f, nk, 0, &one, y, f, ld, st, y, f, ld, st, batch, f,
+                                               x, f, ld, st, V, f, nk, 0, beta, y, f, ld, st, y, f, ld, st, batch, f,
                                                algo, sol, 0);
     }
 
@@ -2628,10 +2699,27 @@ struct EngineImpl {
     // epart: 512 x nch x 64 energy partials (the batch's bl_epart when null)
     // dot false: no energies (the batch takes them from the first adjoint
     // window pass)
+    // z (exchange-symmetric batches with the grouped sigma): the alpha-alpha
+    // DGEMM writes z = V_aa x first, and the grouped sigma adds z[a][b] +
+    // z[b][a] (both same-spin parts) to its sums; no beta-beta DGEMM
     bool ham_apply_lanes(const double *x, double *y, int nch, int nvec, double *out, std::string &err,
-                         double *epart = nullptr, bool dot = true)
+                         double *epart = nullptr, bool dot = true, double *z = nullptr)
     {
         if (!epart) epart = bl_epart.p;
+        if (z && !(sigg && siggOpt && na == nb)) z = nullptr;
+        if (z) {
+            const int mk3 = mark_begin(3);
+            rocblas_status st = sigma_gemm(1, x, z, nch, gemmSol[1], false, nullptr, nullptr, true);
+            if (st != rocblas_status_success && gemmSol[1] != 0) {
+                gemmSol[1] = 0;
+                st = sigma_gemm(1, x, z, nch, 0, false, nullptr, nullptr, true);
+            }
+            if (st != rocblas_status_success) {
+                err = "rocBLAS DGEMM (alpha part) failed";
+                return false;
+            }
+            mark_end(mk3);
+        }
         const int mk = mark_begin(0);
         SigvArgs s;
         s.m = mfargs();
@@ -2640,6 +2728,7 @@ struct EngineImpl {
         s.ntp = ntp16;
         s.x = x;
         s.y = y;
+        s.z = z;
         s.cstride = N * kLanes;
         const unsigned gx = (unsigned)((na + 7) / 8 * 32);
         if (sigg && siggOpt) {
@@ -2662,20 +2751,22 @@ struct EngineImpl {
         }
         mark_end(mk);
         LCHK();
-        const int mk3 = mark_begin(3);
-        for (int which = 0; which < 2; which++) {
-            rocblas_status st = sigma_gemm(which, x, y, nch, gemmSol[which]);
-            if (st != rocblas_status_success && gemmSol[which] != 0) {
-                // a recorded solution this library does not offer: the default
-                gemmSol[which] = 0;
-                st = sigma_gemm(which, x, y, nch, 0);
+        if (!z) {
+            const int mk3 = mark_begin(3);
+            for (int which = 0; which < 2; which++) {
+                rocblas_status st = sigma_gemm(which, x, y, nch, gemmSol[which]);
+                if (st != rocblas_status_success && gemmSol[which] != 0) {
+                    // a recorded solution this library does not offer: the default
+                    gemmSol[which] = 0;
+                    st = sigma_gemm(which, x, y, nch, 0);
+                }
+                if (st != rocblas_status_success) {
+                    err = which ? "rocBLAS DGEMM (alpha part) failed" : "rocBLAS DGEMM (beta part) failed";
+                    return false;
+                }
             }
-            if (st != rocblas_status_success) {
-                err = which ? "rocBLAS DGEMM (alpha part) failed" : "rocBLAS DGEMM (beta part) failed";
-                return false;
-            }
+            mark_end(mk3);
         }
-        mark_end(mk3);
         if (!dot) return true;
         const int G = 512;
         hipLaunchKernelGGL(k_dot_lanes, dim3(G, nch), dim3(kBlock), 0, stream, x, y, N, epart);
@@ -2925,6 +3016,7 @@ bool Engine::set_operators(const std::vector<int8_t> &ops4, const std::vector<in
         }
     if (!e.d_ops.ensure(M, err)) return false;
     HCHK(hipMemcpy(e.d_ops.p, e.hops.data(), sizeof(DevOp) * M, hipMemcpyHostToDevice));
+    e.find_mirrors();
 
     // Fused segments (S_z sector): maximal runs of consecutive operators whose
     // union U of alpha bits keeps a tile of two vectors within the LDS budget.
@@ -3075,6 +3167,7 @@ bool Engine::set_start(const std::vector<double> &start, std::string &err)
     if (e.cplx) HCHK(hipMemset(e.d_start.p + e.N, 0, 8 * e.N));
     e.haveStart = true;
     e.hstart = start;
+    e.startSym = e.symmetric_start();
     e.supDirty = e.hessDirty = true;
     return true;
 }
@@ -3117,6 +3210,7 @@ bool Engine::set_start_complex(const std::vector<double> &re, const std::vector<
     HCHK(hipMemcpy(e.d_start.p + e.N, im.data(), 8 * e.N, hipMemcpyHostToDevice));
     e.haveStart = true;
     e.hstart = re;
+    e.startSym = false;  // complex amplitudes: no exchange-symmetric sigma
     e.supDirty = e.hessDirty = true;
     return true;
 }
@@ -3236,6 +3330,13 @@ bool Engine::set_ham_integrals(int norb, const std::vector<double> &h, const std
                                a, e.d_ts1.p, e.ntp, 8, kSig1NN, 16, 0);
             LCHK();
             HCHK(hipStreamSynchronize(e.stream));
+            e.vSame = false;
+            if (na == nb) {
+                std::vector<double> va(na * na), vb(nb * nb);
+                HCHK(hipMemcpy(va.data(), e.d_Vaa.p, va.size() * sizeof(double), hipMemcpyDeviceToHost));
+                HCHK(hipMemcpy(vb.data(), e.d_Vbb.p, vb.size() * sizeof(double), hipMemcpyDeviceToHost));
+                e.vSame = va == vb;
+            }
             if (lanesOk)
                 HCHK(hipFuncSetAttribute((const void *)sigw_kernel(e.ntp16 / 4),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSigWLds));
@@ -3661,6 +3762,10 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
             !e.bl_partial.ensure(nch * e.M * e.lslots * kLanes, err) || !e.bl_epart.ensure(nch * 512 * kLanes, err) ||
             !e.bl_wep.ensure((size_t)nch * (std::max(1, e.win_erows()) + 16) * kLanes, err))
             return false;
+        const bool sym = e.batch_symmetric(angles, nvec);
+        if (sym && !e.bl_z.ensure((size_t)nch * e.N * kLanes, err)) return false;
+        if (sym != e.bSym) e.drop_graphs();
+        e.bSym = sym;
         // stored forward states 0 .. segments, when they fit in half the
         // free memory
         bool ck = e.ckOpt && (e.ckShape || e.winOn) && e.hkind == 3 && e.sigv;
@@ -3689,6 +3794,7 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         }
         if (!e.upload_cs_lanes(angles, nvec, err)) return false;
     } else {
+        e.bSym = false;
         if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
         if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
     }
@@ -3736,7 +3842,9 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
             // in their first adjoint pass (its tiles hold the final state and
             // H psi) instead of a separate dot over both vectors
             const bool fe = with_gradient && e.fused_energy() && e.winFuseE;
-            if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err, nullptr, !fe)) return false;
+            if (!e.ham_apply_lanes(psi, e.bl_lam.p, nch, nvec, e.b_out.p, err, nullptr, !fe,
+                                   e.bSym ? e.bl_z.p : nullptr))
+                return false;
             if (with_gradient) {
                 if (!e.sweep_reverse_lanes(e.bl_psi.p, e.bl_lam.p, nch, err, fe ? e.bl_wep.p : nullptr)) return false;
                 hipLaunchKernelGGL(k_reduce_lanes, dim3(nch * e.M), dim3(kBlock), 0, e.stream, e.bl_partial.p,
@@ -3996,6 +4104,20 @@ bool Engine::set_option(int key, int value, std::string &err)
         }
         e.bnvec = 0;  // stored states and lists are renewed at the next upload
         return true;
+    case kOptSigmaExchange:
+        if (value != -1 && value != 0) {
+            err = "exchange-symmetry option must be -1 or 0";
+            return false;
+        }
+        if (value != e.exchOpt) {
+            // off at once; on from the next upload (which checks the batch)
+            e.exchOpt = value;
+            if (value == 0 && e.bSym) {
+                e.bSym = false;
+                e.drop_graphs();
+            }
+        }
+        return true;
     case kOptWindowStaged:
         if (value < 0 || value > 1) {
             err = "staged window option must be 0 or 1";
@@ -4089,6 +4211,7 @@ int Engine::option(int key) const
     case kOptWindowEnergy: return e.winFuseE ? 1 : 0;
     case kOptWindowHessian: return e.winHess ? 1 : 0;
     case kOptWindowStaged: return e.use_vs() ? 1 : (e.winVs ? 2 : 0);
+    case kOptSigmaExchange: return e.bSym ? 1 : (e.exchOpt != 0 ? 2 : 0);
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }

@@ -79,6 +79,13 @@ enum EngineOption : int {
     kOptWindowStaged = 32,    // 1 (default): the vector-major window passes stage each alpha class's rows in LDS
                               // (k_winvs) where they fit; 0: one tile per lane straight from memory (k_winv).
                               // Read back: 1 in use, 2 requested but the rows do not fit, 0 off
+    kOptSigmaExchange = 33,   // -1 (default): a lane-layout batch whose states are symmetric under the exchange
+                              // of the alpha and beta strings (equal strings per spin, the same same-spin matrix,
+                              // an exchange-symmetric start vector, every operator its own mirror or next to its
+                              // mirror with the same angle in every vector) takes the beta-beta part of the
+                              // grouped sigma from the alpha-alpha DGEMM's result read transposed, instead of a
+                              // second DGEMM; 0: never.  Read back: 1 in use for the uploaded batch, 2 allowed
+                              // but not applicable, 0 off
 };
 
 class Engine {

@@ -419,6 +419,9 @@ struct SigvArgs {
     const double *x;
     double *y;
     long long cstride;   // N * 64
+    // k_sigg_sparse, exchange-symmetric batches: z = V_aa x (same layout as
+    // x); y[a][b] = the sums + z[a][b] + z[b][a].  nullptr: y = the sums
+    const double *z;
 };
 __global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
 // entries: source byte offset J * xunit | coefficient byte offset (slot

@@ -104,12 +104,25 @@ __device__ __forceinline__ void sigg_task(int h, uint32_t lb, uint32_t gsb, cons
     if constexpr (K == 6) sigg_prog6<M, AO>(acc, ad, cv);
 }
 
+// targets b0 .. b0 + NT - 1 of alpha row ar; with z (exchange-symmetric
+// batches) both same-spin parts are added: z[ar][b] and, transposed,
+// z[b][ar]
 template <int AO, int NT>
-__device__ __forceinline__ void sigg_store(const double (&acc)[kSiggAcc], double *__restrict__ y, long long row0,
-                                           int v)
+__device__ __forceinline__ void sigg_store(const double (&acc)[kSiggAcc], double *__restrict__ y,
+                                           const double *__restrict__ z, int ar, int nb, int b0, int v)
 {
+    const long long row0 = (long long)ar * nb + b0;
+    if (z) {
+        double zs[NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) y[(row0 + t) * kLanes + v] = acc[AO + t];
+        for (int t = 0; t < NT; t++)
+            zs[t] = z[(row0 + t) * kLanes + v] + z[((long long)(b0 + t) * nb + ar) * kLanes + v];
+#pragma unroll
+        for (int t = 0; t < NT; t++) y[(row0 + t) * kLanes + v] = acc[AO + t] + zs[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < NT; t++) y[(row0 + t) * kLanes + v] = acc[AO + t];
+    }
 }
 
 // The whole source-row loop for a wave whose tasks are programs MA (and MB,
@@ -118,7 +131,8 @@ __device__ __forceinline__ void sigg_store(const double (&acc)[kSiggAcc], double
 // the same barriers the same number of times.
 template <int K, int MA, int MB>
 __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int hB, double *gs, double *dt,
-                                          double *xs, const double *__restrict__ x, double *__restrict__ y)
+                                          double *xs, const double *__restrict__ x, double *__restrict__ y,
+                                          const double *__restrict__ z)
 {
     const MfArgs &m = a.m;
     const int n = m.n, nb = m.nb, nn = n * n;
@@ -196,9 +210,9 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
         if constexpr (MB >= 0) sigg_task<K, MB, 10>(hB, lb, gsb, dt, diag, ident, acc);
     }
     const uint32_t *bt = SiggTables<K>::base();
-    if (hA >= 0) sigg_store<0, sg_comb6(MA)>(acc, y, (long long)ar * nb + bt[hA * 16] / 128, v);
+    if (hA >= 0) sigg_store<0, sg_comb6(MA)>(acc, y, z, ar, nb, (int)(bt[hA * 16] / 128), v);
     if constexpr (MB >= 0)
-        if (hB >= 0) sigg_store<10, sg_comb6(MB)>(acc, y, (long long)ar * nb + bt[hB * 16] / 128, v);
+        if (hB >= 0) sigg_store<10, sg_comb6(MB)>(acc, y, z, ar, nb, (int)(bt[hB * 16] / 128), v);
 }
 
 }  // namespace
@@ -217,6 +231,7 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigg_sparse(SigvArgs a)
     const long long coff = (long long)blockIdx.y * a.cstride + sub * kSigWVC;
     const double *__restrict__ x = a.x + coff;
     double *__restrict__ y = a.y + coff;
+    const double *__restrict__ z = a.z ? a.z + coff : nullptr;
     double *gs = sm, *dt = sm + kSigGS, *xs = dt + 128;
     const int rs = (threadIdx.x & (kLanes - 1)) >> 4;
     const int wu = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
@@ -224,12 +239,12 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigg_sparse(SigvArgs a)
     const int *pl = SiggTables<K>::plan() + wu * 12;
     const int mA = pl[0], hA = pl[2 + rs], mB = pl[6], hB = pl[8 + rs];
     static_assert(K == 6, "programs are generated for 6 beta electrons");
-    if (mA == 3) sigg_loop<K, 3, -1>(a, ar, hA, hB, gs, dt, xs, x, y);
-    else if (mA == 2) sigg_loop<K, 2, -1>(a, ar, hA, hB, gs, dt, xs, x, y);
-    else if (mA == 4) sigg_loop<K, 4, -1>(a, ar, hA, hB, gs, dt, xs, x, y);
-    else if (mA == 5 && mB == 5) sigg_loop<K, 5, 5>(a, ar, hA, hB, gs, dt, xs, x, y);
-    else if (mA == 1 && mB == 1) sigg_loop<K, 1, 1>(a, ar, hA, hB, gs, dt, xs, x, y);
-    else sigg_loop<K, 6, 0>(a, ar, hA, hB, gs, dt, xs, x, y);
+    if (mA == 3) sigg_loop<K, 3, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    else if (mA == 2) sigg_loop<K, 2, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    else if (mA == 4) sigg_loop<K, 4, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    else if (mA == 5 && mB == 5) sigg_loop<K, 5, 5>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    else if (mA == 1 && mB == 1) sigg_loop<K, 1, 1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    else sigg_loop<K, 6, 0>(a, ar, hA, hB, gs, dt, xs, x, y, z);
 }
 
 template __global__ void k_sigg_sparse<6>(SigvArgs);

@@ -61,6 +61,9 @@ def parse():
                         "timed region")
     p.add_argument("--eager", action="store_true",
                    help="launch every kernel directly instead of replaying hipGraphs (for profiler counter passes)")
+    p.add_argument("--no-exchange", action="store_true",
+                   help="both same-spin DGEMMs even where the batch is symmetric under the exchange of the alpha "
+                        "and beta strings (engine option 33 = 0)")
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain 2 GiB device copy first (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -248,6 +251,8 @@ def main():
     setup_s = time.time() - t0
     if args.eager:
         a.set_option(a.OPT_GRAPH, 0)
+    if args.no_exchange:
+        a.set_option(a.OPT_SIGMA_EXCHANGE, 0)
     assert (d, M2, P2) == (N, M, P)
 
     # parity of the benchmarked path against the reference CPU build's golden
@@ -327,6 +332,7 @@ def main():
     opts["window_passes"] = a.get_option(a.OPT_WINDOW_PASSES)
     opts["window_registers"] = a.get_option(a.OPT_WINDOW_REGISTERS)
     opts["window_vector"] = a.get_option(a.OPT_WINDOW_VECTOR)
+    opts["sigma_exchange"] = a.get_option(a.OPT_SIGMA_EXCHANGE)
     amp = {"forward": a.get_option(a.OPT_SWEEP_AMP_FORWARD),
            "adjoint_lambda": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_LAMBDA),
            "adjoint_state": a.get_option(a.OPT_SWEEP_AMP_ADJOINT_STATE)}
@@ -485,10 +491,20 @@ def main():
             "lds_wave_reads_per_staged_row": reads, "ms_per_step": ham_ms / nbd,
             "fp64_tflops": flops / (avg * 1e-3) / 1e12,
             "note": "LDS peak = 256 CUs x 128 B/clock x 2.4 GHz; FP64 work (no MFMA) for reference"}
-        gflops = 2.0 * na * na * nb * 64 * math.ceil(args.nvec / 64) + 2.0 * nb * nb * 64 * na * math.ceil(args.nvec / 64)
-        rooflines["same_spin_dgemm"] = fp64("rocBLAS dgemm_strided_batched x2 (dense same-spin rows)", gflops,
-                                            gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd,
-                                            "dense FLOPs of both GEMMs on FP64 MFMA")
+        if opts.get("sigma_exchange") == 1:
+            # exchange-symmetric batch: the alpha-alpha DGEMM alone; the
+            # grouped sigma adds its result and its transpose
+            rooflines["hamiltonian"]["kernel"] += ", beta-beta part from the alpha-alpha DGEMM read transposed"
+            gflops = 2.0 * na * na * nb * 64 * math.ceil(args.nvec / 64)
+            rooflines["same_spin_dgemm"] = fp64(
+                "rocBLAS dgemm_strided_batched x1 (dense alpha-alpha rows; exchange symmetry gives the beta-beta part)",
+                gflops, gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd, "dense FLOPs of the GEMM on FP64 MFMA")
+        else:
+            gflops = (2.0 * na * na * nb * 64 * math.ceil(args.nvec / 64) +
+                      2.0 * nb * nb * 64 * na * math.ceil(args.nvec / 64))
+            rooflines["same_spin_dgemm"] = fp64("rocBLAS dgemm_strided_batched x2 (dense same-spin rows)", gflops,
+                                                gemm_ms / max(gemm_n, 1), gemm_n, gemm_ms / nbd,
+                                                "dense FLOPs of both GEMMs on FP64 MFMA")
     elif ham_kind == 3:
         r = fp64("k_mf_sigma (matrix-free sigma)", 2.0 * nnz * args.nvec / max(1.0, ham_n / nbd),
                  ham_ms / max(ham_n, 1), ham_n, ham_ms / nbd, "FP64 vector-ALU work priced against the FP64 peak")

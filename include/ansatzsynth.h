@@ -349,7 +349,18 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     class's rows of the vector to LDS with coalesced loads and take the
  *     tiles from there, where six rows (twice for the adjoint) fit in LDS;
  *     0: each tile straight from memory.  Reads back 1 when in use, 2 when
- *     requested but the rows do not fit, 0 when off.
+ *     requested but the rows do not fit, 0 when off;
+ *  33 -1 (default): a lane-layout batch whose states are symmetric under the
+ *     exchange of the alpha and beta strings takes the beta-beta part of the
+ *     grouped sigma (12 orbitals) from the alpha-alpha DGEMM's result read
+ *     transposed instead of a second DGEMM.  It applies when both spins have
+ *     the same strings and the same same-spin matrix (bit for bit), the
+ *     start vector satisfies x[a][b] == x[b][a], and every operator is its
+ *     own mirror image under the exchange or sits next to its mirror (acting
+ *     on the other spin only) with the same angle in every vector of the
+ *     batch, as in TUPS units; checked at each upload.  0: never (takes
+ *     effect at once; -1 from the next upload).  Reads back 1 when in use for
+ *     the uploaded batch, 2 when allowed but not applicable, 0 when off.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
  * buffers; an uploaded angle batch must then be uploaded again.  1 ctx NULL, 2 pointer NULL,
  * 5 rejected value or failure. */

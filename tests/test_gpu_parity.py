@@ -181,12 +181,61 @@ def test_config3_benchmarked_batch_path(gpu):
         assert a.get_option(AnsatzSynth.OPT_REACHABLE) == 1
         assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED_AVAILABLE) == 1
         assert a.get_option(AnsatzSynth.OPT_SIGMA_GROUPED) == 1
+        # TUPS units from the Hartree-Fock determinant: exchange-symmetric
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_EXCHANGE) == 1
         for _ in range(3):  # capture, then replays
             a.run_batch(True)
         E, G = a.download_batch(True)
         assert abs(E[0] - gz["cfg3_E"][0]) <= E_TOL
         assert np.max(np.abs(G[0] - gz["cfg3_grad"][0])) <= G_TOL
         for v in (1, 17, 40, 63):
+            Es, gs = a.energy_gradient(thetas[v])
+            assert abs(E[v] - Es) <= 1e-12, v
+            assert np.max(np.abs(G[v] - gs)) <= 1e-11, v
+        # the same batch with both same-spin DGEMMs
+        a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, 0)
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_EXCHANGE) == 0
+        for _ in range(2):
+            a.run_batch(True)
+        E0, G0 = a.download_batch(True)
+        assert np.max(np.abs(E - E0)) <= 1e-12
+        assert np.max(np.abs(G - G0)) <= 1e-11
+        a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, -1)
+        a.upload_batch(thetas)
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_EXCHANGE) == 1
+
+
+@pytest.mark.parametrize("sym", [True, False], ids=["symmetric", "asymmetric"])
+def test_config3_exchange_start_states(gpu, sym):
+    """The exchange-symmetric sigma applies only to batches whose states are
+    symmetric under the exchange of the alpha and beta strings: a start
+    vector with x[a][b] == x[b][a] keeps it, one without falls back to both
+    DGEMMs (option 33 reads 2); either way the 64-vector batch equals
+    single-vector calls (vector-major sigma with both DGEMMs)."""
+    w = make_workload(3)
+    n = w["n"]
+    hf = w["start_idx"]
+    # move one electron p -> q in the lower-bit string, and its mirror in
+    # the upper-bit string
+    p, q = 0, 1
+    lo_move = hf ^ (1 << p) ^ (1 << q)
+    hi_move = hf ^ (1 << (p + n)) ^ (1 << (q + n))
+    assert bin(lo_move).count("1") == bin(hf).count("1")
+    idx = [hf, lo_move, hi_move]
+    coef = [0.9, 0.3, 0.3 if sym else -0.1]
+    nrm = float(np.sqrt(np.sum(np.square(coef))))
+    coef = [c / nrm for c in coef]
+    rng = np.random.default_rng(w["seed"] + 5)
+    thetas = w["theta"][None, :] + 1e-2 * rng.uniform(-np.pi, np.pi, size=(64, w["P"]))
+    with from_workload(w) as a:
+        a.set_initial_state(w["nq"], [i + 1 for i in idx], coef)
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1)
+        a.upload_batch(thetas)
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_EXCHANGE) == (1 if sym else 2)
+        for _ in range(2):
+            a.run_batch(True)
+        E, G = a.download_batch(True)
+        for v in (0, 31, 63):
             Es, gs = a.energy_gradient(thetas[v])
             assert abs(E[v] - Es) <= 1e-12, v
             assert np.max(np.abs(G[v] - gs)) <= 1e-11, v
