@@ -2733,6 +2733,12 @@ struct EngineImpl {
         const unsigned gx = (unsigned)((na + 7) / 8 * 32);
         if (sigg && siggOpt) {
             hipLaunchKernelGGL(k_sigg_sparse<6>, dim3(gx, nch), dim3(kSigWThreads), kSigWLds, stream, s);
+            if (z) {
+                // the targets below the diagonal from their mirror images
+                LCHK();
+                hipLaunchKernelGGL(k_mirror_lanes, dim3((unsigned)na, nch), dim3(kBlock), 0, stream, y, nb,
+                                   s.cstride);
+            }
         } else if (sigh && sighOpt) {
             SighArgs h;
             h.m = s.m;

@@ -1696,6 +1696,28 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
 
+// One wave per (a, b) pair at a time, four pairs in flight per wave; each
+// pair is one 512-byte line of 64 vectors read and one written.
+__global__ void __launch_bounds__(kBlock) k_mirror_lanes(double *__restrict__ y, int nb, long long cstride)
+{
+    constexpr int W = kBlock / kLanes, U = 4;
+    const int a = blockIdx.x, lane = threadIdx.x & (kLanes - 1), w = threadIdx.x / kLanes;
+    double *yc = y + (long long)blockIdx.y * cstride;
+    for (int b0 = w; b0 < a; b0 += W * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int b = b0 + u * W;
+            v[u] = b < a ? yc[((long long)b * nb + a) * kLanes + lane] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int b = b0 + u * W;
+            if (b < a) yc[((long long)a * nb + b) * kLanes + lane] = v[u];
+        }
+    }
+}
+
 // Per-lane dot products of lane-layout chunks: part[(c * G + block) * 64 + l].
 __global__ void __launch_bounds__(kBlock)
     k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N, double *__restrict__ part)

@@ -420,7 +420,9 @@ struct SigvArgs {
     double *y;
     long long cstride;   // N * 64
     // k_sigg_sparse, exchange-symmetric batches: z = V_aa x (same layout as
-    // x); y[a][b] = the sums + z[a][b] + z[b][a].  nullptr: y = the sums
+    // x); y[a][b] = the sums + z[a][b] + z[b][a], stored for the targets
+    // b >= a at least (a wave skips a task whose rows all lie below a);
+    // k_mirror_lanes fills b < a.  nullptr: every target, y = the sums
     const double *z;
 };
 __global__ void k_dense_spin(MfArgs a, int spin, double *__restrict__ V);
@@ -474,6 +476,9 @@ __global__ void k_sigw_sparse(SigvArgs a);  // grid (ceil(na / 8) * 32, chunks),
 // six orbitals, one group per 16-lane row (sigg.hip); same grid, block and LDS
 template <int K>
 __global__ void k_sigg_sparse(SigvArgs a);
+// y[a][b] = y[b][a] for b < a (lane layout, nb strings per spin, cstride
+// doubles per chunk); grid (nb, chunks), kBlock
+__global__ void k_mirror_lanes(double *__restrict__ y, int nb, long long cstride);
 __global__ void k_dot_lanes(const double *__restrict__ x, const double *__restrict__ y, long long N,
                             double *__restrict__ part);  // grid (G, chunks)
 

@@ -132,7 +132,7 @@ __device__ __forceinline__ void sigg_store(const double (&acc)[kSiggAcc], double
 template <int K, int MA, int MB>
 __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int hB, double *gs, double *dt,
                                           double *xs, const double *__restrict__ x, double *__restrict__ y,
-                                          const double *__restrict__ z)
+                                          const double *__restrict__ z, bool runA, bool runB)
 {
     const MfArgs &m = a.m;
     const int n = m.n, nb = m.nb, nn = n * n;
@@ -206,13 +206,14 @@ __device__ __forceinline__ void sigg_loop(const SigvArgs &a, int ar, int hA, int
         }
         __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's row copies have landed
         __syncthreads();
-        sigg_task<K, MA, 0>(hA, lb, gsb, dt, diag, ident, acc);
-        if constexpr (MB >= 0) sigg_task<K, MB, 10>(hB, lb, gsb, dt, diag, ident, acc);
+        if (runA) sigg_task<K, MA, 0>(hA, lb, gsb, dt, diag, ident, acc);
+        if constexpr (MB >= 0)
+            if (runB) sigg_task<K, MB, 10>(hB, lb, gsb, dt, diag, ident, acc);
     }
     const uint32_t *bt = SiggTables<K>::base();
-    if (hA >= 0) sigg_store<0, sg_comb6(MA)>(acc, y, z, ar, nb, (int)(bt[hA * 16] / 128), v);
+    if (runA && hA >= 0) sigg_store<0, sg_comb6(MA)>(acc, y, z, ar, nb, (int)(bt[hA * 16] / 128), v);
     if constexpr (MB >= 0)
-        if (hB >= 0) sigg_store<10, sg_comb6(MB)>(acc, y, z, ar, nb, (int)(bt[hB * 16] / 128), v);
+        if (runB && hB >= 0) sigg_store<10, sg_comb6(MB)>(acc, y, z, ar, nb, (int)(bt[hB * 16] / 128), v);
 }
 
 }  // namespace
@@ -239,12 +240,22 @@ __global__ void __launch_bounds__(kSigWThreads) k_sigg_sparse(SigvArgs a)
     const int *pl = SiggTables<K>::plan() + wu * 12;
     const int mA = pl[0], hA = pl[2 + rs], mB = pl[6], hB = pl[8 + rs];
     static_assert(K == 6, "programs are generated for 6 beta electrons");
-    if (mA == 3) sigg_loop<K, 3, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
-    else if (mA == 2) sigg_loop<K, 2, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
-    else if (mA == 4) sigg_loop<K, 4, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
-    else if (mA == 5 && mB == 5) sigg_loop<K, 5, 5>(a, ar, hA, hB, gs, dt, xs, x, y, z);
-    else if (mA == 1 && mB == 1) sigg_loop<K, 1, 1>(a, ar, hA, hB, gs, dt, xs, x, y, z);
-    else sigg_loop<K, 6, 0>(a, ar, hA, hB, gs, dt, xs, x, y, z);
+    // exchange-symmetric batches need the targets b >= ar only: a task runs
+    // when one of its rows reaches that far (wave-uniform)
+    bool runA = true, runB = true;
+    if (z) {
+        const uint32_t *bt = SiggTables<K>::base();
+        const bool topA = hA >= 0 && (int)(bt[hA * 16] / 128) + sg_comb6(mA) - 1 >= ar;
+        const bool topB = mB >= 0 && hB >= 0 && (int)(bt[hB * 16] / 128) + sg_comb6(mB) - 1 >= ar;
+        runA = __ballot(topA) != 0;
+        runB = __ballot(topB) != 0;
+    }
+    if (mA == 3) sigg_loop<K, 3, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
+    else if (mA == 2) sigg_loop<K, 2, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
+    else if (mA == 4) sigg_loop<K, 4, -1>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
+    else if (mA == 5 && mB == 5) sigg_loop<K, 5, 5>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
+    else if (mA == 1 && mB == 1) sigg_loop<K, 1, 1>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
+    else sigg_loop<K, 6, 0>(a, ar, hA, hB, gs, dt, xs, x, y, z, runA, runB);
 }
 
 template __global__ void k_sigg_sparse<6>(SigvArgs);
