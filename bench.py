@@ -475,7 +475,19 @@ def main():
         nsrc = 1 + ku * (n_ - ku)
         if grouped:
             from ansatzevolve_amd import sigg
-            reads = sum(len(sigg.program(kd, m)) for wv in sigg.wave_plan(kd) for (m, _, _) in wv)
+            plan = sigg.wave_plan(kd)
+            plen = {m: len(sigg.program(kd, m)) for wv in plan for (m, _, _) in wv}
+            reads = sum(plen[m] for wv in plan for (m, _, _) in wv)
+            if opts.get("sigma_exchange") == 1:
+                # one triangle of targets: in alpha row ar a wave skips a task
+                # whose rows' beta ranks all lie below ar (mean over the rows)
+                def top(h):
+                    m = kd - bin(h).count("1")
+                    return sigg.rh(h, m) + math.comb(6, m) - 1
+                tops = {(m, tuple(rows)): max(top(h) for h in rows if h >= 0)
+                        for wv in plan for (m, _, rows) in wv}
+                reads = sum(plen[m] for ar in range(na) for wv in plan for (m, _, rows) in wv
+                            if tops[(m, tuple(rows))] >= ar) / na
         else:
             # one read per (entry, four targets): the beta singles and the diagonal of every target
             reads = (nb + 3) // 4 * (kd * (n_ - kd) + 1)
