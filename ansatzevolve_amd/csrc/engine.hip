@@ -299,6 +299,7 @@ struct EngineImpl {
 
     // hipGraphs: the fused single-vector E+grad sequence, the batch sequence
     hipGraphExec_t egGraph = nullptr;
+    bool egSym = false;  // egGraph computes half the sigma targets (exchange-symmetric calls)
     hipGraphExec_t bGraph = nullptr;
     int bGraphNvec = 0;
     bool bGraphGrad = false;
@@ -367,6 +368,15 @@ struct EngineImpl {
             for (int k = 0; k < M; k++)
                 if (t[k] != t[mirror[k]]) return false;
         }
+        return true;
+    }
+
+    // the same for single-vector calls on the vector-major split sigma
+    bool vec_symmetric(const double *angles) const
+    {
+        if (exchOpt == 0 || !opsMirror || !startSym || hkind != 3 || !sig1 || cplx || na != nb) return false;
+        for (int k = 0; k < M; k++)
+            if (angles[k] != angles[mirror[k]]) return false;
         return true;
     }
 
@@ -2783,7 +2793,10 @@ struct EngineImpl {
     }
 
     // y = M^T x for nvec vectors (stride N); energy partials epart[v][eslots].
-    bool ham_apply(const double *x, double *y, int nvec, double *epart, std::string &err)
+    // half: the vectors are exchange-symmetric (x[a][b] == x[b][a]); the
+    // vector-major split sigma then computes the targets b >= a and copies
+    // the rest from their mirror images
+    bool ham_apply(const double *x, double *y, int nvec, double *epart, std::string &err, bool half = false)
     {
         const int es = eslots();
         if (hkind == 2) {
@@ -2828,6 +2841,7 @@ struct EngineImpl {
             s.y = y;
             s.vstride = N;
             const int rb = sig1_rows();
+            s.half = half && rb > 0 && na == nb ? 1 : 0;
             const size_t lds = (size_t)rb * (kSig1GS + nb) * sizeof(double);
             if (rb == 8)
                 hipLaunchKernelGGL(k_sig1r<8>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
@@ -2839,6 +2853,11 @@ struct EngineImpl {
                 hipLaunchKernelGGL(k_sig1r<2>, dim3(na, nvec), dim3(kSig1Threads), lds, stream, s);
             else
                 hipLaunchKernelGGL(k_sig1_sparse, dim3(na, nvec), dim3(kSig1Threads), kSig1Lds, stream, s);
+            if (s.half) {
+                LCHK();
+                const unsigned tb = (unsigned)((nb + 31) / 32);
+                hipLaunchKernelGGL(k_mirror_rows, dim3(tb, tb, nvec), dim3(kBlock), 0, stream, y, nb, (long long)N);
+            }
             mark_end(mk);
             LCHK();
             mk = mark_begin(3);
@@ -3469,6 +3488,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
         return false;
     }
     if (!e.upload_cs_single(angles, err)) return false;
+    const bool sym = e.vec_symmetric(angles);
     // complex amplitudes: both planes through every step; out holds the
     // per-operator brackets of each plane, then each plane's energy
     const int pl = e.planes(), es = e.eslots();
@@ -3480,7 +3500,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
         if (!e.sweep_forward(e.d_psi.p, pl, e.d_cs.p, err)) return false;
         hipLaunchKernelGGL(k_bcast, dim3(cgrid), dim3(kBlock), 0, e.stream, e.d_psi.p, e.d_phi.p, NP, 1);
         LCHK();
-        if (!e.ham_apply(e.d_phi.p, e.d_lam.p, pl, e.d_epart.p, err)) return false;
+        if (!e.ham_apply(e.d_phi.p, e.d_lam.p, pl, e.d_epart.p, err, sym)) return false;
         hipLaunchKernelGGL(k_reduce, dim3(pl), dim3(kBlock), 0, e.stream, e.d_epart.p, pl, es, (long long)es, 1LL,
                            e.d_out.p + (size_t)pl * e.M, 1.0);
         LCHK();
@@ -3491,7 +3511,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
         return true;
     };
     if (e.useGraph && !e.timingOn) {
-        if (e.egGraph && e.egGraphPtrs != e.eg_ptrs()) {
+        if (e.egGraph && (e.egGraphPtrs != e.eg_ptrs() || e.egSym != sym)) {
             (void)hipGraphExecDestroy(e.egGraph);
             e.egGraph = nullptr;
         }
@@ -3515,6 +3535,7 @@ bool Engine::energy_gradient(const double *angles, double *E, double *deriv, std
             (void)hipGraphDestroy(g);
             HCHK(ei);
             e.egGraphPtrs = e.eg_ptrs();
+            e.egSym = sym;
         }
         HCHK(hipGraphLaunch(e.egGraph, e.stream));
     } else if (!enqueue()) {

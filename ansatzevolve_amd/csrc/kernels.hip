@@ -1599,6 +1599,7 @@ __global__ void __launch_bounds__(kSig1Threads) k_sig1r(Sig1Args a)
     double acc[kSig1BI];
 #pragma unroll
     for (int i = 0; i < kSig1BI; i++) acc[i] = 0;
+    const int blo = a.half ? ar : 0;  // the first target this block stores
     for (int r0 = 0; r0 < R; r0 += RB) {
         const int nr = min(RB, R - r0);
         __syncthreads();  // the previous phase's rows are no longer read
@@ -1644,7 +1645,7 @@ __global__ void __launch_bounds__(kSig1Threads) k_sig1r(Sig1Args a)
 #pragma unroll
         for (int i = 0; i < kSig1BI; i++) {
             const int beta = tid + kSig1Threads * i;
-            if (beta < nb) {
+            if (beta < nb && beta >= blo) {
                 const uint4 *te = (const uint4 *)(a.ts + (size_t)beta * a.ntp);
                 double sr[RB];
 #pragma unroll
@@ -1673,7 +1674,7 @@ __global__ void __launch_bounds__(kSig1Threads) k_sig1r(Sig1Args a)
 #pragma unroll
     for (int i = 0; i < kSig1BI; i++) {
         const int beta = tid + kSig1Threads * i;
-        if (beta < nb) y[(long long)ar * nb + beta] = acc[i];
+        if (beta < nb && beta >= blo) y[(long long)ar * nb + beta] = acc[i];
     }
 }
 
@@ -1694,6 +1695,28 @@ __global__ void __launch_bounds__(kBlock)
         s += x[o + i] * y[o + i];
     const double t = block_sum(s, red);
     if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// Tile (a block bx, b block by) with by <= bx: the source rows b of the
+// tile's columns a are read coalesced into LDS and written transposed.  A
+// block reads only entries b < a of other tiles (and its own diagonal tile
+// before the barrier), and writes only entries b < a of its own.
+__global__ void __launch_bounds__(kBlock) k_mirror_rows(double *__restrict__ y, int nb, long long vstride)
+{
+    __shared__ double t[32][33];
+    const int bx = blockIdx.x, by = blockIdx.y;
+    if (by > bx) return;
+    double *yv = y + (long long)blockIdx.z * vstride;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += kBlock / 32) {
+        const int b = by * 32 + r, a = bx * 32 + tx;
+        if (b < nb && a < nb) t[r][tx] = yv[(long long)b * nb + a];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += kBlock / 32) {
+        const int a = bx * 32 + r, b = by * 32 + tx;
+        if (a < nb && b < a) yv[(long long)a * nb + b] = t[tx][r];
+    }
 }
 
 // One wave per (a, b) pair at a time, four pairs in flight per wave; each

@@ -497,8 +497,15 @@ struct Sig1Args {
     const double *x;
     double *y;
     long long vstride;
+    // k_sig1r: 1 for an exchange-symmetric vector, whose targets b < a are
+    // left to k_mirror_rows; 0 every target
+    int half;
 };
-__global__ void k_sig1_sparse(Sig1Args a);  // grid (na, nvec), kSig1Threads, kSig1Lds
+__global__ void k_sig1_sparse(Sig1Args a);
+// y[a][b] = y[b][a] for b < a, vector-major (nb strings per spin, vstride
+// doubles per vector), through 32 x 32 LDS tiles; grid (ceil(nb / 32),
+// ceil(nb / 32), nvec), kBlock
+__global__ void k_mirror_rows(double *__restrict__ y, int nb, long long vstride);  // grid (na, nvec), kSig1Threads, kSig1Lds
 // RB source rows per phase; LDS RB * (kSig1GS + nb) doubles
 template <int RB>
 __global__ void k_sig1r(Sig1Args a);

@@ -239,6 +239,16 @@ def test_config3_exchange_start_states(gpu, sym):
             Es, gs = a.energy_gradient(thetas[v])
             assert abs(E[v] - Es) <= 1e-12, v
             assert np.max(np.abs(G[v] - gs)) <= 1e-11, v
+        # single-vector calls: half the sigma targets (symmetric start) or all
+        # of them, against option 33 off (every target); twice each, so the
+        # captured call sequence is replayed and recaptured when it changes
+        for _ in range(2):
+            E1, g1 = a.energy_gradient(thetas[5])
+            a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, 0)
+            E0, g0 = a.energy_gradient(thetas[5])
+            a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, -1)
+            assert abs(E1 - E0) <= 1e-12
+            assert np.max(np.abs(g1 - g0)) <= 1e-11
 
 
 def test_config4_energy_gradient(gpu):
