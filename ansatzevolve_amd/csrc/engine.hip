@@ -371,6 +371,16 @@ struct EngineImpl {
         return true;
     }
 
+    // the Hessian's tangents d psi / d theta_p are exchange-symmetric when
+    // the state is and each mirror pair of operators shares its parameter
+    bool tangent_symmetric(const double *angles) const
+    {
+        if (!batch_symmetric(angles, 1)) return false;
+        for (int k = 0; k < M; k++)
+            if (order[k] != order[mirror[k]] || scale[k] != scale[mirror[k]]) return false;
+        return true;
+    }
+
     // the same for single-vector calls on the vector-major split sigma
     bool vec_symmetric(const double *angles) const
     {
@@ -876,6 +886,7 @@ struct EngineImpl {
     bool vSame = false;          // V_aa == V_bb bit for bit (sigma set-up)
     bool bSym = false;           // the uploaded batch qualifies
     DBuf<double> bl_z;           // alpha-alpha DGEMM result of the batch (bSym)
+    DBuf<double> hz_z;           // the same for the Hessian's tangents
     DBuf<int4> d_wvtasks;
     // kOptWindowVector: window passes for the vector-major sweeps (-1, the
     // default: with register tiles from 2^18 determinants; 0 off; 1 with
@@ -4574,7 +4585,12 @@ bool Engine::second_derivatives(const double *angles, const double *proj, double
             hipLaunchKernelGGL(k_to_lanes, tgrid, dim3(kBlock), 0, e.stream, e.hz_B.p, e.hz_lx.p, N, P);
             LCHK();
         }
-        if (!e.ham_apply_lanes(e.hz_lx.p, e.hz_ly.p, nch, P, e.hz_epart.p, err, e.hz_lep.p)) return false;
+        double *zs = nullptr;
+        if (e.tangent_symmetric(angles)) {
+            if (!e.hz_z.ensure(L, err)) return false;
+            zs = e.hz_z.p;
+        }
+        if (!e.ham_apply_lanes(e.hz_lx.p, e.hz_ly.p, nch, P, e.hz_epart.p, err, e.hz_lep.p, true, zs)) return false;
         hipLaunchKernelGGL(k_from_lanes, tgrid, dim3(kBlock), 0, e.stream, e.hz_ly.p, e.hz_HB.p, N, P);
         LCHK();
     }

@@ -408,3 +408,20 @@ def test_sector_modes_and_start_states(gpu, case):
         Eb = a.energies(seeded_angles(P, 71, count=20))
         for v, th in enumerate(seeded_angles(P, 71, count=20)):
             assert abs(Eb[v] - o.energy(th)) <= E_TOL
+
+
+def test_config3_hessian_exchange_symmetry(gpu):
+    """The Hessian's H B at 853,776 determinants: TUPS tangents of a state
+    from the Hartree-Fock determinant are exchange-symmetric (mirror
+    operators share their parameter), so H B takes one DGEMM and one
+    triangle of sigma targets; against option 33 off (both DGEMMs, every
+    target)."""
+    w = make_workload(3)
+    with from_workload(w) as a:
+        H1 = a.hessian(w["theta"])
+        a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, 0)
+        H0 = a.hessian(w["theta"])
+        a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, -1)
+    scale = max(1.0, float(np.max(np.abs(H0))))
+    assert np.max(np.abs(H1 - H0)) <= 1e-12 * scale
+    assert np.max(np.abs(H1 - H1.T)) <= 1e-10 * scale
