@@ -885,6 +885,7 @@ struct EngineImpl {
     bool startSym = false;       // the start vector is exchange-symmetric (set_start)
     bool vSame = false;          // V_aa == V_bb bit for bit (sigma set-up)
     bool bSym = false;           // the uploaded batch qualifies
+    bool bSymV = false;          // the uploaded vector-major batch qualifies (vec_symmetric per vector)
     DBuf<double> bl_z;           // alpha-alpha DGEMM result of the batch (bSym)
     DBuf<double> hz_z;           // the same for the Hessian's tangents
     DBuf<int4> d_wvtasks;
@@ -3833,6 +3834,10 @@ bool Engine::batch_upload(int nvec, const double *angles, std::string &err)
         if (!e.upload_cs_lanes(angles, nvec, err)) return false;
     } else {
         e.bSym = false;
+        bool symv = true;
+        for (int v = 0; v < nvec && symv; v++) symv = e.vec_symmetric(angles + (size_t)v * e.M);
+        if (symv != e.bSymV) e.drop_graphs();
+        e.bSymV = symv;
         if (!e.b_partial.ensure((size_t)nvec * e.M * e.slots, err)) return false;
         if (!e.upload_cs(angles, nvec, e.b_cs, e.b_csneg, err)) return false;
     }
@@ -3923,7 +3928,7 @@ bool Engine::batch_enqueue(bool with_gradient, std::string &err)
                        dim3(kBlock), 0, e.stream, e.d_start.p, e.b_psi.p, e.N, nvec);
     LCHK();
     if (!e.sweep_forward(e.b_psi.p, nvec, e.b_cs.p, err)) return false;
-    if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err)) return false;
+    if (!e.ham_apply(e.b_psi.p, e.b_lam.p, nvec, e.b_epart.p, err, e.bSymV)) return false;
     hipLaunchKernelGGL(k_reduce, dim3(nvec), dim3(kBlock), 0, e.stream, e.b_epart.p, nvec, es, (long long)es, 1LL,
                        e.b_out.p, 1.0);
     LCHK();
@@ -4150,8 +4155,8 @@ bool Engine::set_option(int key, int value, std::string &err)
         if (value != e.exchOpt) {
             // off at once; on from the next upload (which checks the batch)
             e.exchOpt = value;
-            if (value == 0 && e.bSym) {
-                e.bSym = false;
+            if (value == 0 && (e.bSym || e.bSymV)) {
+                e.bSym = e.bSymV = false;
                 e.drop_graphs();
             }
         }
@@ -4249,7 +4254,7 @@ int Engine::option(int key) const
     case kOptWindowEnergy: return e.winFuseE ? 1 : 0;
     case kOptWindowHessian: return e.winHess ? 1 : 0;
     case kOptWindowStaged: return e.use_vs() ? 1 : (e.winVs ? 2 : 0);
-    case kOptSigmaExchange: return e.bSym ? 1 : (e.exchOpt != 0 ? 2 : 0);
+    case kOptSigmaExchange: return e.bSym || e.bSymV ? 1 : (e.exchOpt != 0 ? 2 : 0);
     case kOptWindowRegisters: return e.winOn && e.winReg ? (e.winTu ? 2 : 1) : 0;
     default: return -1;
     }

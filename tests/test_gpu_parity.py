@@ -249,6 +249,17 @@ def test_config3_exchange_start_states(gpu, sym):
             a.set_option(AnsatzSynth.OPT_SIGMA_EXCHANGE, -1)
             assert abs(E1 - E0) <= 1e-12
             assert np.max(np.abs(g1 - g0)) <= 1e-11
+        # a vector-major batch (below the lane threshold) takes the same
+        # choice per upload
+        a.set_option(AnsatzSynth.OPT_LANE_MIN, 1000)
+        a.upload_batch(thetas[:4])
+        assert a.get_option(AnsatzSynth.OPT_SIGMA_EXCHANGE) == (1 if sym else 2)
+        for _ in range(2):
+            a.run_batch(True)
+        Ev, Gv = a.download_batch(True)
+        for v in range(4):
+            assert abs(Ev[v] - E[v]) <= 1e-12, v
+            assert np.max(np.abs(Gv[v] - G[v])) <= 1e-11, v
 
 
 def test_config4_energy_gradient(gpu):
