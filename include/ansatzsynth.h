@@ -359,8 +359,10 @@ int getDimensions(long long *sectordim, int *noperators, int *nparams, void *ctx
  *     own mirror image under the exchange or sits next to its mirror (acting
  *     on the other spin only) with the same angle in every vector of the
  *     batch, as in TUPS units; checked at each upload.  Single-vector calls
- *     (getEnergyGradient) with such angles compute one triangle of the
- *     vector-major sigma's targets and copy the rest.  0: never (takes
+ *     (getEnergyGradient) and vector-major batches with such angles compute
+ *     one triangle of the vector-major sigma's targets and copy the rest;
+ *     the Hessian's H B does so too when mirror operators also share their
+ *     parameter.  0: never (takes
  *     effect at once; -1 from the next upload).  Reads back 1 when in use for
  *     the uploaded batch, 2 when allowed but not applicable, 0 when off.
  * Changing keys 0, 3, 4, 7, 9, 10, 18, 25 or 26 re-segments the operators or changes the
